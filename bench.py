@@ -1,0 +1,201 @@
+"""VIA-C server answer benchmark on one MI355X.
+
+A step is one Server::answer of the paper VIA-C stack (n1=2048, two-prime
+q1, q2~2^34, q3~2^23) over a synthetic database of uniform random 256-byte
+records, with the query and the answer buffers resident in HBM. Prints one
+JSON line: answers/s, p50/p99 answer latency, the FirstDim roofline (HBM
+bytes per launch from live HIP events) and the CPU restatement timed on one
+answer of the same workload, whose answer must equal the GPU's bit for bit.
+
+    python bench.py [--steps K] [--warmup W] [--rows I] [--cols J] [--no-cpu]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def synthetic_inputs(pp, seed=2048):
+    """Uniform residues in the layouts of include/via_gpu.h. The answer path
+    is exact integer arithmetic whose work does not depend on the values, so
+    uniform keys and LWEs exercise it exactly as real ones do."""
+    rng = np.random.default_rng(seed)
+    q0, q1 = pp.q1[0], pp.q1[1]
+    n1, L = pp.n1, pp.l_ck
+
+    def limbs(words, period):
+        out = np.empty(words, dtype=np.uint64)
+        idx = (np.arange(words) // period) % 2
+        out[idx == 0] = rng.integers(0, q0, size=int((idx == 0).sum()), dtype=np.uint64)
+        out[idx == 1] = rng.integers(0, q1, size=int((idx == 1).sum()), dtype=np.uint64)
+        return out
+
+    steps = []
+    nout = 2
+    while nout <= n1:
+        steps.append(limbs((2 * n1 // nout) * L * 2 * 2 * nout, nout))
+        nout *= 2
+    cascade = np.concatenate(steps)
+    conv = limbs(L * 2 * 2 * n1, n1)
+    d = n1 // pp.n2
+    rsk = rng.integers(0, pp.q3, size=d * pp.l_rsk * 2 * pp.n2, dtype=np.uint64)
+    lwes = limbs(pp.query_lwes * 2 * (n1 + 1), n1 + 1)
+    n_rec = 4 * pp.num_rows * pp.num_cols
+    record_bytes = rng.integers(0, 256, size=n_rec * pp.n_rec // 2, dtype=np.uint8)
+    return cascade, conv, rsk, lwes, record_bytes, n_rec
+
+
+def cpu_baseline(pp, cascade, conv, rsk, lwes, record_bytes, n_rec):
+    """The C restatement (oracle/) on one answer of the same workload, with
+    FirstDim fanned out over the host's cores as the reference does."""
+    from oracle.vo import Oracle, make_params, u64p
+
+    o = Oracle()
+    op = make_params(pp.n1, pp.n2, (pp.q1[0], pp.q1[1]), pp.q2, pp.q3, pp.q4, pp.p, pp.b1, pp.b2, pp.b_rsk,
+                     pp.ck_base, pp.l_query, pp.l_ck, pp.l_rsk, pp.num_rows, pp.num_cols)
+    b = record_bytes
+    recs = np.empty(b.size * 2, dtype=np.uint64)
+    recs[0::2] = b & 0xF
+    recs[1::2] = b >> 4
+    P = lambda a: a.ctypes.data_as(u64p)
+    h = C.c_void_p()
+    t0 = time.perf_counter()
+    assert o.server_new(op, P(recs), n_rec, P(cascade), P(conv), P(rsk), C.byref(h)) == 0
+    t_setup = time.perf_counter() - t0
+    del recs
+    am, ab = np.zeros(pp.n2, np.uint64), np.zeros(pp.n2, np.uint64)
+    t0 = time.perf_counter()
+    assert o.server_answer(h, P(lwes), pp.query_lwes, P(am), P(ab), None, None) == 0
+    t_ans = time.perf_counter() - t0
+    steps = (C.c_double * 7)()
+    o.server_step_times(h, steps)
+    o.server_free(h)
+    return am, ab, t_ans, t_setup, list(steps)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rows", type=int, default=256)
+    ap.add_argument("--cols", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: the answer path is single-GPU in this build "
+              "(multi-GPU sharding is designed, not built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__
+
+    via_gpu = __graft_entry__.load_package()
+    pp = via_gpu.paper_params(args.rows, args.cols)
+    cascade, conv, rsk, lwes, record_bytes, n_rec = synthetic_inputs(pp)
+
+    ctx = via_gpu.Context(0)
+    t0 = time.perf_counter()
+    db = via_gpu.PreparedDb(ctx, pp, record_bytes=record_bytes.tobytes(), n_records=n_rec)
+    keys = via_gpu.PreparedKeys(ctx, pp, cascade, conv, rsk)
+    t_setup = time.perf_counter() - t0
+
+    L = via_gpu.lib()
+    d_q = ctx.alloc(lwes.nbytes)
+    ctx.to_device(d_q, lwes)
+    d_m, d_b = ctx.alloc(8 * pp.n2), ctx.alloc(8 * pp.n2)
+    err = via_gpu.ErrDetail()
+
+    def answer():
+        st = L.via_answer_device(ctx.h, db.h, keys.h, d_q, pp.query_lwes, d_m, d_b, C.byref(err))
+        if st:
+            raise via_gpu.ViaError(st, msg=L.via_status_str(st).decode())
+
+    for _ in range(args.warmup):
+        answer()
+    ctx.synchronize()
+    lat = []
+    fd_ms = []
+    steps_acc = np.zeros(7)
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        answer()
+        ctx.synchronize()
+        lat.append(time.perf_counter() - t0)
+        st = ctx.step_times()
+        steps_acc += np.array(st)
+        fd_ms.append(ctx.first_dim_time() * 1e3)
+    elapsed = time.perf_counter() - t_start
+    gpu_mask = ctx.to_host(np.zeros(pp.n2, np.uint64), d_m)
+    gpu_body = ctx.to_host(np.zeros(pp.n2, np.uint64), d_b)
+
+    I, J, n = pp.num_rows, pp.num_cols, pp.n1
+    fd_bytes = I * J * n * 8 + 2 * I * n * 8 + 2 * J * n * 8
+    fd_s = float(np.mean(fd_ms)) * 1e-3
+    fd_gbs = fd_bytes / fd_s / 1e9
+    lat_ms = np.array(lat) * 1e3
+    value = args.steps / elapsed
+
+    out = {
+        "metric": "server answers/sec + p50/p99 answer latency, VIA-C n=2048, bit-exact vs CPU",
+        "value": round(value, 3),
+        "unit": "answers/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int (exact modular, residues < 2^38 in fp64 lanes)",
+        "data": "synthetic: uniform random 256-byte records, uniform key and query residues",
+        "config": {
+            "workload": f"VIA-C paper stack single query, I={I} J={J}, {n_rec} records x 256 B, "
+                        f"prepared DB {I * J * n * 8 / 2**30:.2f} GiB eval u64-equivalent",
+            "num_rows": I, "num_cols": J, "n1": n, "n2": pp.n2, "query_lwes": pp.query_lwes,
+        },
+        "step_ms": {k: round(v / args.steps * 1e3, 3) for k, v in zip(
+            ["query_decomp", "dmux", "mod_switch", "first_dim", "cmux", "crot", "resp_comp"], steps_acc)},
+        "setup_s": round(t_setup, 3),
+        "roofline": {
+            "kernel": "k_first_dim (database stream multiply-accumulate)",
+            "bound": "hbm",
+            "achieved": round(fd_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(fd_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu:
+        am, ab, t_cpu, t_cpu_setup, cpu_steps = cpu_baseline(pp, cascade, conv, rsk, lwes, record_bytes, n_rec)
+        out["cpu_baseline"] = {
+            "value": round(1.0 / t_cpu, 5),
+            "unit": "answers/s",
+            "cores": os.cpu_count(),
+            "kind": "port",
+            "sample": f"one answer of the same workload (C restatement, FirstDim over {os.cpu_count()} threads, "
+                      f"{t_cpu:.2f} s; setup {t_cpu_setup:.2f} s)",
+            "step_s": [round(x, 4) for x in cpu_steps],
+        }
+        out["bit_exact_vs_cpu"] = bool(np.array_equal(am, gpu_mask) and np.array_equal(ab, gpu_body))
+    print(json.dumps(out))
+    ctx.free(d_q)
+    ctx.free(d_m)
+    ctx.free(d_b)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

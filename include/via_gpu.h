@@ -1,0 +1,135 @@
+/*
+ * via_gpu.h — C ABI of the MI355X (gfx950) VIA-C server answer path.
+ *
+ * This is the drop-in boundary for the via-rs server crate: each entry point
+ * replaces one reference interface (cited per declaration, paths relative to
+ * the via-rs checkout). All buffers are dense little-endian uint64 canonical
+ * residues with the layouts below; no Rust or torch types cross the boundary.
+ *
+ * Layouts
+ *   poly   : nl x n residues, limb-major (coefficient i of limb l at l*n + i);
+ *            q1 is the two-prime RNS (nl = 2), every other ring has nl = 1.
+ *   RLWE   : mask poly then body poly.
+ *   RLev L : L RLWE samples; sample k encrypts m * round(Q / B^(k+1)).
+ *   LWE    : per limb, n masks then the body: lwe[l*(n+1) + i].
+ *   record : n_rec coefficients in [0, p) (setup_db.rs:39-75).
+ *   cascade key : steps in ascending output degree n_out = 2, 4, ..., n1; step
+ *            s holds rank_in = 2*n1/n_out RLev(L_CK) at (n_out, q1)
+ *            (cascade.rs:90-110, conv.rs:161-190).
+ *   conv key : RLev(L_CK) of S1^2 at (n1, q1) (convert.rs:46-58).
+ *   rsk    : d = n1/n2 RLev(L_RSK) at (n2, q3) (ring_switch.rs:224-246).
+ *   answer : mask[n2] mod q3, body[n2] mod q4 (ModSwitchedCiphertext).
+ */
+#ifndef VIA_GPU_H
+#define VIA_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ViaError (via-protocol/src/error.rs:20-64) plus the device-side failures. */
+typedef enum {
+    VIA_OK = 0,
+    VIA_E_DIM = 1,         /* ViaError::DimMismatch */
+    VIA_E_QUERY_LEN = 2,   /* ViaError::QueryLengthMismatch { expected, got } */
+    VIA_E_INDEX = 3,       /* ViaError::IndexOutOfRange */
+    VIA_E_NOMEM = 4,       /* HBM admission refused / allocation failed */
+    VIA_E_INVALID = 5,     /* parameters outside what the device path supports */
+    VIA_E_DEVICE = 6,      /* HIP runtime error */
+    VIA_E_BATCH_LEN = 7    /* answer_batch: wrong number of queries (batch.rs:124-128) */
+} via_status;
+
+/* PIRParams (via-protocol/src/params.rs) restricted to the server path, plus
+ * the conversion-key base that PublicParams carries (wire/keys.rs:136-141). */
+typedef struct {
+    uint32_t n1, n2, n_rec;
+    uint32_t q1_nl;
+    uint64_t q1[2];
+    uint64_t q2, q3, q4, p;
+    uint64_t b1, b2, b_rsk, ck_base;
+    uint32_t l_query, l_ck, l_rsk;
+    uint32_t num_rows, num_cols;
+} via_params;
+
+typedef struct {
+    uint64_t expected, got;
+    char msg[128];
+} via_err_detail;
+
+typedef struct via_ctx via_ctx;
+typedef struct via_db via_db;
+typedef struct via_keys via_keys;
+
+/* Device context: one HIP stream and the NTT tables of every ring in use. */
+via_status via_ctx_create(int device, via_ctx **out);
+void via_ctx_destroy(via_ctx *ctx);
+/* Seconds spent in each of the seven answer steps of the last via_answer on
+ * this context (answer.rs:166-343 span names step1..step7), by HIP events. */
+via_status via_ctx_step_times(via_ctx *ctx, double *t7);
+/* Seconds of the last FirstDim database-stream kernel alone (HIP events on
+ * the context stream, around the one launch that reads the I*J cells). */
+via_status via_ctx_first_dim_time(via_ctx *ctx, double *seconds);
+
+/* Server::setup, database half: setup_db + PreparedDb::from_encoded
+ * (answer.rs:486-528, setup_db.rs:39-75, prepared_db.rs:57-86). Records are
+ * [n_records][n_rec] coefficients; more records than I*J*(n1/n_rec) is
+ * VIA_E_DIM. The prepared database stays resident in HBM. */
+via_status via_db_create(via_ctx *ctx, const via_params *pp, const uint64_t *records, size_t n_records,
+                         via_db **out);
+/* Same, with records as packed nibbles: coefficient 2k = byte k & 0xF,
+ * 2k+1 = byte k >> 4 (p = 16, n_rec/2 bytes per record). */
+via_status via_db_create_bytes(via_ctx *ctx, const via_params *pp, const uint8_t *bytes, size_t n_records,
+                               via_db **out);
+void via_db_destroy(via_db *db);
+/* The prepared cells, eval form at q2, [I][J][n1] (PreparedDb::cell). */
+via_status via_db_read_cells(const via_db *db, uint64_t *out, size_t words);
+
+/* PreparedKeys::from_public_params (prepared_keys.rs:69-77): cascade key,
+ * conversion key and ring-switch key to eval form, resident in HBM. */
+via_status via_keys_create(via_ctx *ctx, const via_params *pp, const uint64_t *cascade_key,
+                           const uint64_t *conv_key, const uint64_t *rsk, via_keys **out);
+/* Zeroizes the device copies before release (prepared_keys.rs:89-125). */
+void via_keys_destroy(via_keys *keys);
+
+/* Server::answer (answer.rs:534-554) -> answer_one_query (answer.rs:289-354):
+ * query_decomp, dmux_tree, mod switch, first_dim, cmux_tree, crot, resp_comp.
+ * lwes = n_lwe LWEs at (n1, q1). Host buffers. */
+via_status via_answer(via_ctx *ctx, const via_db *db, const via_keys *keys, const uint64_t *lwes, size_t n_lwe,
+                      uint64_t *ans_mask, uint64_t *ans_body, via_err_detail *err);
+/* Same with device-resident query and answer buffers; enqueued on the
+ * context stream, returns without waiting. */
+via_status via_answer_device(via_ctx *ctx, const via_db *db, const via_keys *keys, const uint64_t *d_lwes,
+                             size_t n_lwe, uint64_t *d_ans_mask, uint64_t *d_ans_body, via_err_detail *err);
+via_status via_ctx_synchronize(via_ctx *ctx);
+
+/* Intermediate values of the last answer for parity tests, coefficient form:
+ * stage 1 RGSW bits [bit][half][i][part][limb][n1] at q1, 2 DMux outputs
+ * [I][part][limb][n1] at q1, 3 mod-switched rows [I][part][n1] at q2,
+ * 4 FirstDim outputs [J][part][n1] at q2, 5 CMux output, 6 CRot output. */
+via_status via_debug_stage(via_ctx *ctx, int stage, uint64_t *out, size_t words);
+
+/* RingPolyEval batch (ntt_abstraction.rs:61-78): c = a * b in
+ * Z_q[X]/(X^n+1) for count polys; poly p uses modulus moduli[(p / group) % nmod].
+ * Device pointers, enqueued on the context stream. */
+via_status via_poly_mul_device(via_ctx *ctx, uint32_t n, const uint64_t *moduli, uint32_t nmod, size_t group,
+                               const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_c, size_t count);
+/* Forward / inverse negacyclic NTT of count polys in place (bit-reversed
+ * eval order, ring/ntt.rs:305-399), host buffers. */
+via_status via_ntt(via_ctx *ctx, uint32_t n, const uint64_t *moduli, uint32_t nmod, size_t group, uint64_t *polys,
+                   size_t count, int inverse);
+
+/* Device memory helpers for callers without a runtime of their own. */
+via_status via_device_alloc(via_ctx *ctx, size_t bytes, void **d_ptr);
+via_status via_device_free(via_ctx *ctx, void *d_ptr);
+via_status via_copy_to_device(via_ctx *ctx, void *d_dst, const void *src, size_t bytes);
+via_status via_copy_to_host(via_ctx *ctx, void *dst, const void *d_src, size_t bytes);
+
+const char *via_status_str(via_status s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,127 @@
+// Host check of via-rs_amd/csrc/modarith.h against 128-bit integer arithmetic.
+// Built and run by tests/test_host_arith.py; prints "OK <count>" on success.
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+
+#include "../../via-rs_amd/csrc/modarith.h"
+
+static int fails = 0;
+#define CHECK(c, ...)                                   \
+    do {                                                \
+        if (!(c)) {                                     \
+            if (fails++ < 10) fprintf(stderr, __VA_ARGS__); \
+        }                                               \
+    } while (0)
+
+static via_u128 exact_rmd(via_u128 a, via_u128 s, via_u128 d) {
+    // floor((a*s + d/2)/d) with 256-bit long division in 32-bit digits.
+    uint64_t a0 = (uint64_t)a, a1 = (uint64_t)(a >> 64), s0 = (uint64_t)s, s1 = (uint64_t)(s >> 64);
+    via_u128 p00 = (via_u128)a0 * s0, p01 = (via_u128)a0 * s1, p10 = (via_u128)a1 * s0, p11 = (via_u128)a1 * s1;
+    via_u128 mid = (p00 >> 64) + (uint64_t)p01 + (uint64_t)p10;
+    via_u128 lo = ((via_u128)(uint64_t)mid << 64) | (uint64_t)p00;
+    via_u128 hi = p11 + (p01 >> 64) + (p10 >> 64) + (mid >> 64);
+    via_u128 h = d / 2, nlo = lo + h;
+    if (nlo < lo) hi++;
+    uint32_t w[8];
+    for (int i = 0; i < 4; i++) {
+        w[i] = (uint32_t)(nlo >> (32 * i));
+        w[4 + i] = (uint32_t)(hi >> (32 * i));
+    }
+    via_u128 rem = 0, quo = 0;
+    for (int i = 7; i >= 0; i--) {
+        via_u128 cur = (rem << 32) | w[i];
+        via_u128 qd = cur / d;
+        rem = cur - qd * d;
+        if (i < 4) quo |= qd << (32 * i);
+    }
+    return quo;
+}
+
+int main() {
+    std::mt19937_64 rng(12345);
+    long count = 0;
+    const uint64_t primes[] = {137438822401ULL, 274810798081ULL, 17175674881ULL, 8380417ULL,
+                               268369921ULL,    536608769ULL,    65537ULL,       17ULL};
+    for (uint64_t q : primes) {
+        FMod m = fmod_make(q);
+        for (int t = 0; t < 200000; t++) {
+            uint64_t a = rng() % q, b = rng() % q;
+            if (t < 4) { a = q - 1 - t; b = q - 1; }
+            double r = mod_mul((double)a, (double)b, m);
+            uint64_t want = (uint64_t)(((via_u128)a * b) % q);
+            CHECK((uint64_t)r == want && r >= 0 && r < (double)q, "mul q=%llu a=%llu b=%llu\n",
+                  (unsigned long long)q, (unsigned long long)a, (unsigned long long)b);
+            double s = mod_add((double)a, (double)b, m), d = mod_sub((double)a, (double)b, m);
+            CHECK((uint64_t)s == (a + b) % q && (uint64_t)d == (a + q - b) % q, "add/sub\n");
+            count++;
+        }
+    }
+    // Wide scaling: gadget scale at q1 RNS (B=18, L=18 and B=55879, L=2) and mod switch q1 -> q2.
+    const via_u128 Q1 = (via_u128)137438822401ULL * 274810798081ULL;
+    via_u128 bl18 = 1, bl55 = (via_u128)55879 * 55879;
+    for (int i = 0; i < 18; i++) bl18 *= 18;
+    WideDiv wq1 = widediv_make(Q1);
+    for (int t = 0; t < 300000; t++) {
+        via_u128 c = (((via_u128)rng() << 64) | rng()) % (Q1 / 2 + 1);
+        if (t < 8) c = (t < 4) ? (via_u128)t : Q1 / 2 - (t - 4);
+        via_u128 s = (t % 3 == 0) ? bl18 : (t % 3 == 1 ? bl55 : (via_u128)17175674881ULL);
+        via_u128 got = round_mul_div(c, s, wq1), want = exact_rmd(c, s, Q1);
+        CHECK(got == want, "rmd t=%d\n", t);
+        count++;
+    }
+    // Digits against the reference rule (rem_euclid, rebalance above B/2).
+    const uint32_t bases[] = {18, 55879, 81, 8, 64, 4, 2};
+    for (uint32_t B : bases) {
+        for (int t = 0; t < 100000; t++) {
+            via_i128 val = (via_i128)((((via_u128)rng() << 64) | rng()) >> (48 + t % 40));
+            if (rng() & 1) val = -val;
+            if (t < 3) val = t - 1;
+            SMag v;
+            v.neg = val < 0;
+            v.mag = (via_u128)(val < 0 ? -val : val);
+            via_i128 ref = val;
+            for (int k = 0; k < 4; k++) {
+                via_i128 d = ref % (via_i128)B;
+                if (d < 0) d += B;
+                if (d > (via_i128)(B / 2)) d -= B;
+                ref = (ref - d) / (via_i128)B;
+                int32_t got = extract_digit(v, B);
+                via_i128 cur = v.neg ? -(via_i128)v.mag : (via_i128)v.mag;
+                CHECK(got == (int32_t)d && cur == ref, "digit B=%u t=%d k=%d\n", B, t, k);
+            }
+            count++;
+        }
+    }
+    // Garner reconstruction.
+    Rns2 b;
+    b.q0 = 137438822401ULL;
+    b.q1 = 274810798081ULL;
+    b.m0 = fmod_make(b.q0);
+    b.m1 = fmod_make(b.q1);
+    {
+        // q0^-1 mod q1 by extended Euclid
+        long long t0 = 0, t1 = 1;
+        long long r0 = (long long)b.q1, r1 = (long long)(b.q0 % b.q1);
+        while (r1) {
+            long long qq = r0 / r1, tmp = t0 - qq * t1;
+            t0 = t1; t1 = tmp;
+            tmp = r0 - qq * r1; r0 = r1; r1 = tmp;
+        }
+        if (t0 < 0) t0 += (long long)b.q1;
+        b.q0_inv_mod_q1 = (uint64_t)t0;
+    }
+    b.big_q = Q1;
+    for (int t = 0; t < 200000; t++) {
+        via_u128 x = (((via_u128)rng() << 64) | rng()) % Q1;
+        double a0 = (double)(uint64_t)(x % b.q0), a1 = (double)(uint64_t)(x % b.q1);
+        CHECK(rns_reconstruct(a0, a1, b) == x, "garner\n");
+        count++;
+    }
+    if (fails) {
+        printf("FAIL %d\n", fails);
+        return 1;
+    }
+    printf("OK %ld\n", count);
+    return 0;
+}

@@ -1,0 +1,128 @@
+"""Parity of the HIP path with the CPU restatement (bit-exact, integer work).
+
+Run on an MI355X: pytest -m gpu. Every comparison is exact equality of u64
+residues; the paper-stack cases also check client recovery of the record.
+"""
+import numpy as np
+import pytest
+
+from tests.paper_setup import paper_case
+
+pytestmark = pytest.mark.gpu
+
+PRIMES = [137438822401, 274810798081, 17175674881, 8380417, 268369921, 536608769]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import via_gpu
+
+    c = via_gpu.Context(0)
+    yield c
+    c.close()
+
+
+def P(a):
+    from oracle.vo import u64p
+
+    return a.ctypes.data_as(u64p)
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048])
+def test_ntt_matches_oracle(ctx, oracle, n):
+    from oracle.vo import Ring
+
+    rng = np.random.default_rng(n)
+    for q in PRIMES:
+        if (q - 1) % (2 * n):
+            continue
+        count = max(1, 4096 // n)
+        a = rng.integers(0, q, size=count * n, dtype=np.uint64)
+        got = ctx.ntt(n, [q], a)
+        want = a.copy()
+        r = Ring.make(n, q)
+        for p in range(count):
+            seg = np.ascontiguousarray(want[p * n:(p + 1) * n])
+            oracle.ntt_fwd(r, P(seg))
+            want[p * n:(p + 1) * n] = seg
+        assert np.array_equal(got, want), (n, q)
+        back = ctx.ntt(n, [q], got, inverse=True)
+        assert np.array_equal(back, a), (n, q)
+
+
+def test_poly_mul_microbench_shape(ctx, oracle):
+    # negacyclic NTT -> pointwise -> iNTT for the two VIA q1 primes at n=2048
+    from oracle.vo import Ring
+
+    n, count, qs = 2048, 16, [268369921, 536608769]
+    rng = np.random.default_rng(5)
+    a = np.concatenate([rng.integers(0, qs[p // 8], size=n, dtype=np.uint64) for p in range(count)])
+    b = np.concatenate([rng.integers(0, qs[p // 8], size=n, dtype=np.uint64) for p in range(count)])
+    da, db, dc = ctx.alloc(a.nbytes), ctx.alloc(b.nbytes), ctx.alloc(a.nbytes)
+    ctx.to_device(da, a)
+    ctx.to_device(db, b)
+    ctx.poly_mul_device(n, qs, 8, da, db, dc, count)
+    ctx.synchronize()
+    c = ctx.to_host(np.zeros_like(a), dc)
+    for p in range(count):
+        r = Ring.make(n, qs[p // 8])
+        want = np.zeros(n, np.uint64)
+        oracle.poly_mul(r, P(want), P(np.ascontiguousarray(a[p * n:(p + 1) * n])),
+                        P(np.ascontiguousarray(b[p * n:(p + 1) * n])))
+        assert np.array_equal(c[p * n:(p + 1) * n], want), p
+    for d in (da, db, dc):
+        ctx.free(d)
+
+
+def run_case(ctx, case):
+    import via_gpu
+
+    server = via_gpu.Server.setup(
+        via_gpu.ServerConfig(case.pp_gpu, case.cascade_key, case.conv_key, case.rsk), case.records, ctx=ctx)
+    assert np.array_equal(server.db.cells(), case.oracle_db), "prepared database differs"
+    mask, body = server.answer(case.lwes)
+    return server, mask, body
+
+
+def test_answer_paper_2x2_index15(ctx):
+    # client_server_e2e_paper.rs:316-349, answered on the GPU.
+    case = paper_case(2, 2, seed=b"via-c-paper-scale-e2e", index=15, stages=True)
+    server, mask, body = run_case(ctx, case)
+    n1 = 2048
+    for k, words in [(1, 4 * 2 * 2 * 2 * 2 * n1), (2, 2 * 2 * 2 * n1), (3, 2 * 2 * n1), (4, 2 * 2 * n1),
+                     (5, 2 * n1), (6, 2 * n1)]:
+        got = ctx.stage(k, words)
+        assert np.array_equal(got, case.st[k]), f"stage {k} differs"
+    assert np.array_equal(mask, case.ans_mask) and np.array_equal(body, case.ans_body)
+    assert np.array_equal(case.recover(mask, body), case.record)
+
+
+@pytest.mark.parametrize("rows,cols,index", [(4, 8, 77), (8, 4, 3), (16, 16, 1000)])
+def test_answer_paper_random_db(ctx, rows, cols, index):
+    case = paper_case(rows, cols, seed=b"via-gpu/parity/%d" % index, index=index, records="random",
+                      rec_seed=b"via-gpu/db/%d" % index)
+    server, mask, body = run_case(ctx, case)
+    assert np.array_equal(mask, case.ans_mask) and np.array_equal(body, case.ans_body)
+    assert np.array_equal(case.recover(mask, body), case.record)
+
+
+def test_answer_ragged_db(ctx):
+    # fewer records than cells: setup_db zero-pads (setup_db.rs:39-75); query
+    # a present record and a padded slot.
+    for index in (5, 30):
+        case = paper_case(4, 4, seed=b"via-gpu/ragged", index=index, records="random", n_records=21,
+                          rec_seed=b"via-gpu/ragged-db")
+        server, mask, body = run_case(ctx, case)
+        assert np.array_equal(mask, case.ans_mask) and np.array_equal(body, case.ans_body)
+        assert np.array_equal(case.recover(mask, body), case.record)
+
+
+def test_answer_query_length_error(ctx):
+    import via_gpu
+
+    case = paper_case(2, 2, seed=b"via-gpu/err", index=0)
+    server, _, _ = run_case(ctx, case)
+    with pytest.raises(via_gpu.ViaError) as e:
+        server.answer(case.lwes[: -case.pp_gpu.lwe_words])
+    assert e.value.status == via_gpu.VIA_E_QUERY_LEN
+    assert (e.value.expected, e.value.got) == (8, 7)

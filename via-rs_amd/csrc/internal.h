@@ -1,0 +1,125 @@
+// internal.h — device data structures and launchers shared by the HIP kernels
+// and the host library. Not part of the C ABI (see include/via_gpu.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "modarith.h"
+
+namespace via {
+
+constexpr int kSlots = 2048;    // coefficients handled by one workgroup
+constexpr int kThreads = 256;   // workgroup size of the polynomial kernels
+constexpr int kPer = kSlots / kThreads;
+
+// Twiddles of one (q, n): bit-reversed psi^k and psi^-k as exact doubles,
+// the layout of via-rs ring/ntt.rs:206-221.
+struct NttTab {
+    const double *fwd;
+    const double *inv;
+    double n_inv;
+    FMod m;
+};
+
+// Modulus context of a ring with NL limbs (1 or 2) at degree n.
+struct RingDev {
+    int nl;
+    int n;
+    int logn;
+    NttTab t[2];
+    Rns2 rns;          // valid when nl == 2
+    WideDiv qdiv;      // Q as a wide divisor (nl == 2)
+    uint64_t q_single; // Q when nl == 1
+};
+
+// Gadget parameters: base, depth, B^L.
+struct Gadget {
+    uint32_t base;
+    int L;
+    via_u128 bl;
+};
+
+// ---- launchers (ntt.hip / kernels.hip) ----------------------------------------
+
+// In-place batched NTT over `npolys` polys of degree r.n stored contiguously;
+// poly p uses limb (p / limb_group) % r.nl.
+void launch_ntt(const RingDev &r, double *data, size_t npolys, size_t limb_group, bool inverse,
+                hipStream_t s);
+void launch_u64_to_f64(const uint64_t *src, double *dst, size_t n, hipStream_t s);
+void launch_f64_to_u64(const double *src, uint64_t *dst, size_t n, hipStream_t s);
+// Negacyclic products c = iNTT(NTT(a) * NTT(b)) of `count` u64 polys per limb
+// group; poly p uses limb (p / limb_group) % r.nl.
+void launch_polymul_u64(const RingDev &r, const uint64_t *a, const uint64_t *b, uint64_t *c, size_t count,
+                        size_t limb_group, hipStream_t s);
+
+// Gadget-product multiply-accumulate.
+struct MacArgs {
+    const double *in;        // coefficient-form inputs
+    size_t in_item_stride;   // words between items
+    size_t in_limb_stride;   // words between limbs of one item
+    int embed;               // 1: input degree n/2 embedded at slot 0 (x -> x^2)
+    const double *key;       // eval-form RLev, layout [k][part][limb][key_slots]
+    size_t key_item_stride;  // 0 for a key shared by all items
+    size_t key_slots;        // slots per (k, part, limb) plane
+    int tiles;               // workgroups per item (tile t covers slots [t*S, t*S+S))
+    double *out_mask;        // [item][tile][limb][S] eval
+    double *out_body;        // [item][tile][limb][S] or [item][tile][limb][n] when reduce_body
+    int reduce_body;
+    int items;
+};
+void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t s);
+
+// One cascade combine: masks_next = -iNTT(pm[t0] + pm[t1]), body_next =
+// embed(body_prev) - iNTT(pb[t0] + pb[t1]).
+struct CombineArgs {
+    const double *pm, *pb;    // MAC outputs with tiles == 2
+    const double *body_prev;  // [item] stride body_item_stride, limb stride body_limb_stride
+    size_t body_item_stride, body_limb_stride;
+    double *masks_next;       // [item][limb][S]
+    double *body_next;        // [item][limb][n]
+    int items;
+};
+void launch_cascade_combine(const RingDev &r, const CombineArgs &a, hipStream_t s);
+
+// rlwe_to_rgsw combine: neg_s_m[i] = (B_i + iNTT(pm), iNTT(pb)); m[i] = level i.
+void launch_r2g_combine(const RingDev &r, const double *pm, const double *pb, const double *rlwe_masks,
+                        const double *rlwe_bodies, double *rgsw, int bits, int lq, hipStream_t s);
+
+// Gate kernels over an eval-form RGSW (layout [half][i][part][limb][n]).
+enum GateMode { GATE_DMUX = 0, GATE_CMUX = 1, GATE_CROT = 2 };
+void launch_gate(const RingDev &r, const Gadget &g, int mode, const double *rgsw_eval, const double *in,
+                 double *out, int items, int rot_shift, hipStream_t s);
+
+// Mod switch of a coefficient-form array at (n, Q1 two limbs) to q2: rows of
+// 2 polys [row][part][limb][n] -> [row][part][n].
+void launch_modswitch_q1_q2(const RingDev &r1, uint64_t q2, const double *in, double *out, size_t polys,
+                            hipStream_t s);
+
+// FirstDim multiply-accumulate: out[j][part][k] = sum_i db[i][j][k] * sel[i][part][k].
+void launch_first_dim(const double *db, const double *sel_eval, double *out_eval, int I, int J, int n, FMod m,
+                      hipStream_t s);
+
+// RespComp for one ciphertext at (n1, q2): mod switch to q3, ring switch to n2
+// with the eval RSK ([j][k][part][n2]), then mask at q3 / body at q4.
+struct RespArgs {
+    const double *ct;  // [part][n1] at q2
+    const double *rsk_eval;
+    int n1, n2, d, l_rsk;
+    uint32_t b_rsk;
+    uint64_t q2, q3, q4;
+    NttTab t3;        // q3 at n2
+    uint64_t *out_mask, *out_body;
+};
+void launch_resp_comp(const RespArgs &a, hipStream_t s);
+
+// Relayout of a cascade step key [idx][k][part][limb][n] (u64) to the
+// eval-ready layout [k][part][limb][idx*n + c] (doubles).
+void launch_key_relayout(const uint64_t *src, double *dst, int rank_in, int L, int nl, int n, hipStream_t s);
+// DB packing: records [rec][n_rec] (u64 < p) -> cells [i][j][n1] doubles (setup_db.rs:39-75).
+void launch_setup_db(const uint64_t *records, size_t n_records, double *cells, int I, int J, int n1, int n_rec,
+                     uint64_t p, hipStream_t s);
+void launch_setup_db_bytes(const uint8_t *bytes, size_t n_records, double *cells, int I, int J, int n1,
+                           int n_rec, hipStream_t s);
+
+}  // namespace via

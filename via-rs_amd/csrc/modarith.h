@@ -1,0 +1,238 @@
+// modarith.h — exact modular arithmetic for moduli below 2^50, host and device.
+//
+// Residues live in fp64 registers as exact integers in [0, q). A product is
+// reduced with one fp64 Barrett estimate and an FMA-exact remainder:
+//   h = a*b (rounded), l = fma(a, b, -h) (the exact rounding error),
+//   t = floor(h * (1/q)), r = fma(-t, q, h) + l,
+// and r lands in [-q, 2q) because t is within one of floor(a*b/q) for
+// a, b < q < 2^50; two conditional corrections make it canonical. Every step
+// is exact: fma(-t, q, h) and r + l are integers below 2^53 in magnitude.
+// Compile with -ffp-contract=off so that h stays a rounded product.
+//
+// The wide helpers implement the 128/256-bit integer steps of the gadget
+// scale (via-rs encryption/gadget.rs:115-151, algebra/wide.rs:145-153) and
+// the mod switch (switching/kernels/mod_switch.rs:59-61) without 128-bit
+// division: the quotient is estimated in fp64 and corrected exactly.
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#define VIA_HD __host__ __device__ __forceinline__
+#else
+#define VIA_HD static inline
+#endif
+
+typedef unsigned __int128 via_u128;
+typedef __int128 via_i128;
+
+struct FMod {
+    double q;     // modulus
+    double qinv;  // 1/q rounded
+};
+
+VIA_HD FMod fmod_make(uint64_t q) {
+    FMod m;
+    m.q = (double)q;
+    m.qinv = 1.0 / (double)q;
+    return m;
+}
+
+VIA_HD double mod_add(double a, double b, const FMod &m) {
+    double s = a + b;
+    return s >= m.q ? s - m.q : s;
+}
+
+VIA_HD double mod_sub(double a, double b, const FMod &m) {
+    double d = a - b;
+    return d < 0.0 ? d + m.q : d;
+}
+
+VIA_HD double mod_neg(double a, const FMod &m) { return a == 0.0 ? 0.0 : m.q - a; }
+
+VIA_HD double mod_mul(double a, double b, const FMod &m) {
+    double h = a * b;
+    double l = fma(a, b, -h);
+    double t = floor(h * m.qinv);
+    double r = fma(-t, m.q, h) + l;
+    r = r < 0.0 ? r + m.q : r;
+    r = r >= m.q ? r - m.q : r;
+    return r;
+}
+
+// Reduce an exact integer value v (|v| < 2^52) into [0, q).
+VIA_HD double mod_reduce_small(double v, const FMod &m) {
+    double t = floor(v * m.qinv);
+    double r = fma(-t, m.q, v);
+    r = r < 0.0 ? r + m.q : r;
+    r = r >= m.q ? r - m.q : r;
+    return r;
+}
+
+// ---- 128-bit helpers ---------------------------------------------------------
+
+VIA_HD double u128_to_f64(via_u128 x) {
+    return (double)(uint64_t)(x >> 64) * 18446744073709551616.0 + (double)(uint64_t)x;
+}
+
+// Exact conversion of an integral double in [0, 2^106) to u128.
+VIA_HD via_u128 f64_to_u128(double v) {
+    double hi = floor(v * (1.0 / 18446744073709551616.0));
+    double lo = fma(-hi, 18446744073709551616.0, v);  // exact: v - hi*2^64 in [0, 2^64)
+    if (lo < 0.0) {
+        hi -= 1.0;
+        lo += 18446744073709551616.0;
+    }
+    return ((via_u128)(uint64_t)hi << 64) + (via_u128)(uint64_t)lo;
+}
+
+// 3-word little-endian 192-bit value.
+struct U192 {
+    uint64_t w0, w1, w2;
+};
+
+VIA_HD U192 mul_u128_u128_192(via_u128 a, via_u128 b) {
+    // a, b < 2^96 on every call site; the product fits in 192 bits.
+    uint64_t a0 = (uint64_t)a, a1 = (uint64_t)(a >> 64);
+    uint64_t b0 = (uint64_t)b, b1 = (uint64_t)(b >> 64);
+    via_u128 p00 = (via_u128)a0 * b0;
+    via_u128 p01 = (via_u128)a0 * b1;
+    via_u128 p10 = (via_u128)a1 * b0;
+    via_u128 p11 = (via_u128)a1 * b1;
+    via_u128 mid = (p00 >> 64) + (uint64_t)p01 + (uint64_t)p10;
+    U192 r;
+    r.w0 = (uint64_t)p00;
+    r.w1 = (uint64_t)mid;
+    r.w2 = (uint64_t)((mid >> 64) + (p01 >> 64) + (p10 >> 64) + p11);
+    return r;
+}
+
+VIA_HD U192 add_u192_u128(U192 x, via_u128 y) {
+    via_u128 lo = ((via_u128)x.w1 << 64) | x.w0;
+    via_u128 s = lo + y;
+    U192 r;
+    r.w0 = (uint64_t)s;
+    r.w1 = (uint64_t)(s >> 64);
+    r.w2 = x.w2 + (s < lo ? 1 : 0);
+    return r;
+}
+
+// Divisor description for floor(x / D) with D < 2^96.
+struct WideDiv {
+    via_u128 d;
+    double df;    // (double)d
+    double dinv;  // 1/(double)d
+};
+
+VIA_HD WideDiv widediv_make(via_u128 d) {
+    WideDiv w;
+    w.d = d;
+    w.df = u128_to_f64(d);
+    w.dinv = 1.0 / w.df;
+    return w;
+}
+
+// floor(x / D) for x < 2^160 whose quotient is below 2^100.
+// Two fp64 estimate/remainder rounds, then an exact +-1 correction. The first
+// estimate is within q*2^-51 of the quotient, so the remainder x - q*D stays
+// below 2^126 in magnitude and is exact when taken modulo 2^128 as a signed
+// value; the second estimate is then within one.
+VIA_HD via_u128 div_u192(U192 x, const WideDiv &w) {
+    double xf = (double)x.w2 * 340282366920938463463374607431768211456.0 +
+                (double)x.w1 * 18446744073709551616.0 + (double)x.w0;
+    double qf = floor(xf * w.dinv);
+    if (qf < 0.0) qf = 0.0;
+    via_u128 q = f64_to_u128(qf);
+    via_u128 xlo = ((via_u128)x.w1 << 64) | x.w0;
+    // r = x - q*d, |r| < 2^115 after the first round
+    via_i128 r = (via_i128)(xlo - q * w.d);
+    double rf = (double)(int64_t)(r >> 64) * 18446744073709551616.0 + (double)(uint64_t)r;
+    double cf = floor(rf * w.dinv);
+    via_i128 c = (via_i128)(int64_t)cf;  // |c| < 2^63 while the quotient is below 2^110
+    q = (via_u128)((via_i128)q + c);
+    r -= c * (via_i128)w.d;
+    for (int i = 0; i < 2 && r < 0; i++) {
+        q -= 1;
+        r += (via_i128)w.d;
+    }
+    for (int i = 0; i < 2 && r >= (via_i128)w.d; i++) {
+        q += 1;
+        r -= (via_i128)w.d;
+    }
+    return q;
+}
+
+// Round-half-up scaling floor((a*s + floor(D/2)) / D) (wide.rs:145-153 and
+// RescaleConsts::scale), a*s < 2^160.
+VIA_HD via_u128 round_mul_div(via_u128 a, via_u128 s, const WideDiv &w) {
+    U192 x = mul_u128_u128_192(a, s);
+    x = add_u192_u128(x, w.d >> 1);
+    return div_u192(x, w);
+}
+
+// ---- RNS (2 limbs) --------------------------------------------------------------
+
+struct Rns2 {
+    uint64_t q0, q1;
+    uint64_t q0_inv_mod_q1;
+    FMod m0, m1;
+    via_u128 big_q;
+};
+
+// Garner reconstruction (via-rs rns/basis.rs:101-109), limbs given as doubles.
+VIA_HD via_u128 rns_reconstruct(double a0, double a1, const Rns2 &b) {
+    double a0m = mod_reduce_small(a0, b.m1);  // a0 < q0 < 2^38 < 2^52
+    double diff = mod_sub(a1, a0m, b.m1);
+    double t = mod_mul(diff, (double)b.q0_inv_mod_q1, b.m1);
+    return (via_u128)(uint64_t)a0 + (via_u128)b.q0 * (uint64_t)t;
+}
+
+// ---- signed-balanced gadget digits (gadget.rs:168-190) ------------------------
+
+// Signed magnitude value: val = neg ? -mag : mag.
+struct SMag {
+    via_u128 mag;
+    int neg;
+};
+
+// One LSB digit of base B (B < 2^20), returning the digit and advancing val
+// to (val - digit) / B, with the Euclidean remainder and the "d > B/2 => d -= B"
+// rebalance of the reference.
+VIA_HD int32_t extract_digit(SMag &v, uint32_t B) {
+    uint64_t hi = (uint64_t)(v.mag >> 64), lo = (uint64_t)v.mag;
+    uint64_t qh = hi / B;
+    uint64_t rh = hi % B;
+    uint64_t t1 = (rh << 32) | (lo >> 32);
+    uint64_t q1 = t1 / B, r1 = t1 % B;
+    uint64_t t0 = (r1 << 32) | (lo & 0xffffffffu);
+    uint64_t q0 = t0 / B, rm = t0 % B;
+    via_u128 fl = ((via_u128)qh << 64) | ((via_u128)q1 << 32) | q0;  // floor(mag / B)
+    int32_t half = (int32_t)(B / 2);
+    int32_t d;
+    if (!v.neg) {
+        d = (int32_t)rm;
+        if (d > half) {
+            d -= (int32_t)B;
+            v.mag = fl + 1;
+        } else {
+            v.mag = fl;
+        }
+    } else {
+        if (rm == 0) {
+            d = 0;
+            v.mag = fl;
+        } else {
+            int32_t re = (int32_t)(B - rm);
+            if (re > half) {
+                d = -(int32_t)rm;
+                v.mag = fl;
+            } else {
+                d = re;
+                v.mag = fl + 1;
+            }
+        }
+    }
+    if (v.mag == 0) v.neg = 0;
+    return d;
+}

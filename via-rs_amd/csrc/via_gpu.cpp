@@ -1,0 +1,860 @@
+// via_gpu.cpp — host side of the MI355X VIA-C answer path behind the C ABI of
+// include/via_gpu.h. One HIP stream per context; every step of an answer is
+// enqueued on it without host synchronisation.
+#include "../../include/via_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "internal.h"
+
+using namespace via;
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            fprintf(stderr, "via_gpu: %s failed: %s\n", #expr, hipGetErrorString(e_)); \
+            return VIA_E_DEVICE;                                                       \
+        }                                                                              \
+    } while (0)
+
+namespace {
+
+typedef unsigned __int128 u128;
+
+uint64_t pow_mod(uint64_t b, uint64_t e, uint64_t q) {
+    u128 r = 1 % q, x = b % q;
+    while (e) {
+        if (e & 1) r = r * x % q;
+        x = x * x % q;
+        e >>= 1;
+    }
+    return (uint64_t)r;
+}
+
+uint64_t inv_mod(uint64_t a, uint64_t q) {
+    __int128 t = 0, nt = 1, r = q, nr = a % q;
+    while (nr) {
+        __int128 qq = r / nr, tmp = t - qq * nt;
+        t = nt;
+        nt = tmp;
+        tmp = r - qq * nr;
+        r = nr;
+        nr = tmp;
+    }
+    if (r != 1) return 0;
+    if (t < 0) t += q;
+    return (uint64_t)t;
+}
+
+// Primitive 2n-th root: the smallest base g = 2, 3, ... whose g^((q-1)/2n) has
+// order exactly 2n, at most 1024 bases (ring/ntt.rs:155-198).
+uint64_t find_psi(uint64_t q, uint32_t n) {
+    uint64_t two_n = 2ULL * n;
+    if (q < 3 || (q - 1) % two_n) return 0;
+    uint64_t quot = (q - 1) / two_n;
+    int log2n = __builtin_ctzll(two_n);
+    uint64_t cap = q < 1026 ? q : 1026;
+    for (uint64_t base = 2; base < cap; base++) {
+        uint64_t c = pow_mod(base, quot, q), cur = c;
+        bool ok = true;
+        for (int i = 0; i < log2n; i++) {
+            if (cur == 1) {
+                ok = false;
+                break;
+            }
+            cur = (uint64_t)((u128)cur * cur % q);
+        }
+        if (ok && cur == 1) return c;
+    }
+    return 0;
+}
+
+uint32_t bitrev(uint32_t x, int logn) {
+    uint32_t r = 0;
+    for (int i = 0; i < logn; i++)
+        if ((x >> i) & 1) r |= 1u << (logn - 1 - i);
+    return r;
+}
+
+int ilog2(uint64_t x) { return x <= 1 ? 0 : 64 - __builtin_clzll(x - 1); }
+bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
+
+struct DevTab {
+    double *fwd = nullptr, *inv = nullptr;
+    double n_inv = 0;
+    uint64_t q = 0;
+};
+
+}  // namespace
+
+struct Workspace {
+    size_t B = 0, I = 0, J = 0, bits = 0;
+    double *lwe_f = nullptr, *masks[2] = {nullptr, nullptr}, *body[2] = {nullptr, nullptr};
+    double *pm = nullptr, *pb = nullptr, *rgsw = nullptr, *rgsw_eval = nullptr;
+    double *dm[2] = {nullptr, nullptr}, *sw = nullptr, *sel = nullptr, *fd = nullptr;
+    double *cm[2] = {nullptr, nullptr}, *rgsw2 = nullptr, *res[2] = {nullptr, nullptr};
+    uint64_t *lwe_u = nullptr, *ans_u = nullptr;
+    std::vector<void *> all;
+};
+
+struct via_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::map<std::pair<uint64_t, uint32_t>, DevTab> tabs;
+    std::mutex mu;
+    Workspace ws;
+    hipEvent_t ev[8] = {};
+    hipEvent_t ev_fd[2] = {};
+    bool have_times = false;
+    // pointers to the last answer's stage buffers
+    const double *stage_ptr[7] = {};
+    size_t stage_words[7] = {};
+    int stage_nl[7] = {};
+};
+
+struct via_db {
+    via_ctx *ctx = nullptr;
+    via_params pp;
+    double *cells = nullptr;  // eval form at q2, [I][J][n1]
+};
+
+struct via_keys {
+    via_ctx *ctx = nullptr;
+    via_params pp;
+    std::vector<double *> cascade;  // per step, [k][part][limb][rank_in*n_out]
+    double *conv = nullptr;         // [k][part][limb][n1]
+    double *rsk = nullptr;          // [j][k][part][n2]
+    std::vector<size_t> bytes;
+};
+
+namespace {
+
+via_status get_tab(via_ctx *c, uint64_t q, uint32_t n, DevTab *out) {
+    std::lock_guard<std::mutex> g(c->mu);
+    auto key = std::make_pair(q, n);
+    auto it = c->tabs.find(key);
+    if (it != c->tabs.end()) {
+        *out = it->second;
+        return VIA_OK;
+    }
+    uint64_t psi = find_psi(q, n);
+    if (!psi) return VIA_E_INVALID;
+    uint64_t psi_inv = inv_mod(psi, q);
+    int logn = __builtin_ctz(n);
+    std::vector<double> f(n), iv(n);
+    uint64_t cf = 1, ci = 1;
+    for (uint32_t k = 0; k < n; k++) {
+        uint32_t br = bitrev(k, logn);
+        f[br] = (double)cf;
+        iv[br] = (double)ci;
+        cf = (uint64_t)((u128)cf * psi % q);
+        ci = (uint64_t)((u128)ci * psi_inv % q);
+    }
+    DevTab t;
+    t.q = q;
+    t.n_inv = (double)inv_mod(n % q, q);
+    HIP_TRY(hipMalloc(&t.fwd, sizeof(double) * n));
+    HIP_TRY(hipMalloc(&t.inv, sizeof(double) * n));
+    HIP_TRY(hipMemcpy(t.fwd, f.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.inv, iv.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    c->tabs[key] = t;
+    *out = t;
+    return VIA_OK;
+}
+
+via_status make_ring(via_ctx *c, uint32_t n, const uint64_t *q, int nl, RingDev *r) {
+    memset(r, 0, sizeof *r);
+    r->nl = nl;
+    r->n = (int)n;
+    r->logn = __builtin_ctz(n);
+    for (int l = 0; l < nl; l++) {
+        DevTab t;
+        via_status s = get_tab(c, q[l], n, &t);
+        if (s) return s;
+        r->t[l].fwd = t.fwd;
+        r->t[l].inv = t.inv;
+        r->t[l].n_inv = t.n_inv;
+        r->t[l].m = fmod_make(q[l]);
+    }
+    if (nl == 1) {
+        r->t[1] = r->t[0];
+        r->q_single = q[0];
+    } else {
+        r->rns.q0 = q[0];
+        r->rns.q1 = q[1];
+        r->rns.q0_inv_mod_q1 = inv_mod(q[0] % q[1], q[1]);
+        r->rns.m0 = fmod_make(q[0]);
+        r->rns.m1 = fmod_make(q[1]);
+        r->rns.big_q = (u128)q[0] * q[1];
+        r->qdiv = widediv_make(r->rns.big_q);
+    }
+    return VIA_OK;
+}
+
+Gadget make_gadget(uint64_t base, uint32_t L) {
+    Gadget g;
+    g.base = (uint32_t)base;
+    g.L = (int)L;
+    g.bl = 1;
+    for (uint32_t k = 0; k < L; k++) g.bl *= base;
+    return g;
+}
+
+bool fits_single(uint64_t q, uint64_t base, uint32_t L) {
+    u128 bl = 1;
+    for (uint32_t k = 0; k < L; k++) bl *= base;
+    return (u128)(q / 2 + 1) * bl + q < ((u128)1 << 63);
+}
+
+// Parameters the device path implements: the paper n1 = 2048 stack with a
+// two-prime q1, NTT-friendly q1 limbs, q2 and q3 (q4 and p powers of two).
+via_status check_params(const via_params *pp) {
+    if (!pp) return VIA_E_INVALID;
+    if (pp->n1 != (uint32_t)kSlots || pp->q1_nl != 2) return VIA_E_INVALID;
+    if (!is_pow2(pp->n2) || pp->n2 > 1024 || pp->n1 % pp->n2 || !is_pow2(pp->n_rec) || pp->n1 % pp->n_rec)
+        return VIA_E_INVALID;
+    const uint64_t qs[4] = {pp->q1[0], pp->q1[1], pp->q2, pp->q3};
+    for (uint64_t q : qs)
+        if (q < 3 || q >= (1ULL << 50) || (q - 1) % (2ULL * pp->n1)) return VIA_E_INVALID;
+    if (!is_pow2(pp->q4) || !is_pow2(pp->p)) return VIA_E_INVALID;
+    u128 Q1 = (u128)pp->q1[0] * pp->q1[1];
+    if (Q1 >> 96) return VIA_E_INVALID;
+    u128 bl = 1;
+    for (uint32_t k = 0; k < pp->l_ck; k++) {
+        bl *= pp->ck_base;
+        if (bl >> 96) return VIA_E_INVALID;
+    }
+    if (!fits_single(pp->q2, pp->b2, pp->l_query) || !fits_single(pp->q3, pp->b_rsk, pp->l_rsk))
+        return VIA_E_INVALID;
+    if (pp->b1 < 2 || pp->b1 >= (1u << 20) || pp->b2 < 2 || pp->b2 >= (1u << 20) || pp->ck_base < 2 ||
+        pp->ck_base >= (1u << 20) || pp->b_rsk < 2 || pp->b_rsk >= (1u << 20))
+        return VIA_E_INVALID;
+    if (pp->l_query < 1 || pp->l_ck < 1 || pp->l_rsk < 1) return VIA_E_INVALID;
+    if ((u128)pp->q2 * pp->q3 >> 63) return VIA_E_INVALID;  // q2 -> q3 switch in 64 bits
+    if ((u128)pp->q3 * pp->q4 >> 63) return VIA_E_INVALID;
+    return VIA_OK;
+}
+
+template <typename T>
+via_status dalloc(Workspace &w, T **p, size_t count) {
+    HIP_TRY(hipMalloc((void **)p, sizeof(T) * (count ? count : 1)));
+    w.all.push_back((void *)*p);
+    return VIA_OK;
+}
+
+void free_ws(Workspace &w) {
+    for (void *p : w.all) (void)hipFree(p);
+    w = Workspace();
+}
+
+via_status ensure_ws(via_ctx *c, const via_params *pp, size_t B, size_t bits) {
+    Workspace &w = c->ws;
+    size_t I = pp->num_rows, J = pp->num_cols;
+    if (w.B >= B && w.I >= I && w.J >= J && w.bits >= bits && w.lwe_f) return VIA_OK;
+    free_ws(w);
+    const size_t n = pp->n1, nl = 2, S = kSlots, lq = pp->l_query;
+    w.B = B;
+    w.I = I;
+    w.J = J;
+    w.bits = bits;
+    via_status s;
+#define A(p, cnt) \
+    if ((s = dalloc(w, &(p), (cnt)))) return s
+    A(w.lwe_u, B * nl * (n + 1));
+    A(w.lwe_f, B * nl * (n + 1));
+    A(w.masks[0], B * nl * S);
+    A(w.masks[1], B * nl * S);
+    A(w.body[0], B * nl * S);
+    A(w.body[1], B * nl * S);
+    A(w.pm, B * 2 * nl * S);
+    A(w.pb, B * 2 * nl * S);
+    A(w.rgsw, bits * 4 * lq * nl * n);
+    A(w.rgsw_eval, bits * 4 * lq * nl * n);
+    A(w.dm[0], I * 2 * nl * n);
+    A(w.dm[1], I * 2 * nl * n);
+    A(w.sw, I * 2 * n);
+    A(w.sel, I * 2 * n);
+    A(w.fd, J * 2 * n);
+    A(w.cm[0], J * 2 * n);
+    A(w.cm[1], J * 2 * n);
+    A(w.rgsw2, bits * 4 * lq * n);
+    A(w.res[0], 2 * n);
+    A(w.res[1], 2 * n);
+    A(w.ans_u, 2 * 1024);
+#undef A
+    return VIA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *via_status_str(via_status s) {
+    switch (s) {
+        case VIA_OK: return "ok";
+        case VIA_E_DIM: return "dimension mismatch";
+        case VIA_E_QUERY_LEN: return "query length mismatch";
+        case VIA_E_INDEX: return "index out of range";
+        case VIA_E_NOMEM: return "out of device memory";
+        case VIA_E_INVALID: return "parameters not supported by the device path";
+        case VIA_E_DEVICE: return "HIP runtime error";
+        case VIA_E_BATCH_LEN: return "batch length mismatch";
+    }
+    return "unknown";
+}
+
+via_status via_ctx_create(int device, via_ctx **out) {
+    if (!out) return VIA_E_INVALID;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return VIA_E_DEVICE;
+    HIP_TRY(hipSetDevice(device));
+    via_ctx *c = new via_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return VIA_E_DEVICE;
+    }
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete c;
+            return VIA_E_DEVICE;
+        }
+    for (auto &e : c->ev_fd)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete c;
+            return VIA_E_DEVICE;
+        }
+    *out = c;
+    return VIA_OK;
+}
+
+void via_ctx_destroy(via_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_ws(c->ws);
+    for (auto &kv : c->tabs) {
+        (void)hipFree(kv.second.fwd);
+        (void)hipFree(kv.second.inv);
+    }
+    for (auto &e : c->ev) (void)hipEventDestroy(e);
+    for (auto &e : c->ev_fd) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+via_status via_ctx_synchronize(via_ctx *c) {
+    if (!c) return VIA_E_INVALID;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return VIA_OK;
+}
+
+via_status via_ctx_step_times(via_ctx *c, double *t7) {
+    if (!c || !t7 || !c->have_times) return VIA_E_INVALID;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < 7; i++) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+        t7[i] = ms * 1e-3;
+    }
+    return VIA_OK;
+}
+
+via_status via_ctx_first_dim_time(via_ctx *c, double *seconds) {
+    if (!c || !seconds || !c->have_times) return VIA_E_INVALID;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev_fd[0], c->ev_fd[1]));
+    *seconds = ms * 1e-3;
+    return VIA_OK;
+}
+
+// ---- database ---------------------------------------------------------------------
+
+static via_status db_common(via_ctx *c, const via_params *pp, const uint64_t *records, const uint8_t *bytes,
+                            size_t n_records, via_db **out) {
+    if (!c || !out) return VIA_E_INVALID;
+    via_status s = check_params(pp);
+    if (s) return s;
+    if (!is_pow2(pp->num_rows) || !is_pow2(pp->num_cols)) return VIA_E_DIM;
+    const size_t I = pp->num_rows, J = pp->num_cols, n1 = pp->n1, nr = pp->n_rec;
+    if (n_records > I * J * (n1 / nr)) return VIA_E_DIM;  // answer.rs:503-508
+    if (bytes && (pp->p != 16 || nr % 2)) return VIA_E_INVALID;
+    HIP_TRY(hipSetDevice(c->device));
+    via_db *db = new via_db();
+    db->ctx = c;
+    db->pp = *pp;
+    if (hipMalloc(&db->cells, sizeof(double) * I * J * n1) != hipSuccess) {
+        delete db;
+        return VIA_E_NOMEM;
+    }
+    // Stream the records through a bounded staging buffer.
+    const size_t rec_words = bytes ? nr / 2 : nr * 8;  // bytes per record
+    size_t chunk = n_records ? n_records : 1;
+    void *stage = nullptr;
+    if (hipMalloc(&stage, rec_words * chunk) != hipSuccess) {
+        (void)hipFree(db->cells);
+        delete db;
+        return VIA_E_NOMEM;
+    }
+    if (n_records) HIP_TRY(hipMemcpyAsync(stage, bytes ? (const void *)bytes : (const void *)records, rec_words * n_records,
+                                          hipMemcpyHostToDevice, c->stream));
+    if (bytes)
+        launch_setup_db_bytes((const uint8_t *)stage, n_records, db->cells, (int)I, (int)J, (int)n1, (int)nr, c->stream);
+    else
+        launch_setup_db((const uint64_t *)stage, n_records, db->cells, (int)I, (int)J, (int)n1, (int)nr, pp->p,
+                        c->stream);
+    RingDev r2;
+    uint64_t q2 = pp->q2;
+    if ((s = make_ring(c, (uint32_t)n1, &q2, 1, &r2))) return s;
+    launch_ntt(r2, db->cells, I * J, 1, false, c->stream);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipFree(stage));
+    HIP_TRY(hipGetLastError());
+    *out = db;
+    return VIA_OK;
+}
+
+via_status via_db_create(via_ctx *c, const via_params *pp, const uint64_t *records, size_t n_records, via_db **out) {
+    if (n_records && !records) return VIA_E_INVALID;
+    return db_common(c, pp, records, nullptr, n_records, out);
+}
+
+via_status via_db_create_bytes(via_ctx *c, const via_params *pp, const uint8_t *bytes, size_t n_records,
+                               via_db **out) {
+    if (n_records && !bytes) return VIA_E_INVALID;
+    return db_common(c, pp, nullptr, bytes, n_records, out);
+}
+
+void via_db_destroy(via_db *db) {
+    if (!db) return;
+    (void)hipSetDevice(db->ctx->device);
+    (void)hipStreamSynchronize(db->ctx->stream);
+    (void)hipFree(db->cells);
+    delete db;
+}
+
+via_status via_db_read_cells(const via_db *db, uint64_t *out, size_t words) {
+    if (!db || !out) return VIA_E_INVALID;
+    const size_t total = (size_t)db->pp.num_rows * db->pp.num_cols * db->pp.n1;
+    if (words < total) return VIA_E_INVALID;
+    via_ctx *c = db->ctx;
+    uint64_t *tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, sizeof(uint64_t) * total));
+    launch_f64_to_u64(db->cells, tmp, total, c->stream);
+    HIP_TRY(hipMemcpyAsync(out, tmp, sizeof(uint64_t) * total, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipFree(tmp));
+    return VIA_OK;
+}
+
+// ---- keys --------------------------------------------------------------------------------
+
+via_status via_keys_create(via_ctx *c, const via_params *pp, const uint64_t *cascade_key, const uint64_t *conv_key,
+                           const uint64_t *rsk, via_keys **out) {
+    if (!c || !out || !cascade_key || !conv_key || !rsk) return VIA_E_INVALID;
+    via_status s = check_params(pp);
+    if (s) return s;
+    HIP_TRY(hipSetDevice(c->device));
+    via_keys *k = new via_keys();
+    k->ctx = c;
+    k->pp = *pp;
+    const uint32_t n1 = pp->n1, nl = 2, L = pp->l_ck;
+    uint64_t *stage = nullptr;
+    const size_t max_step = (size_t)2 * n1 * L * 2 * nl;
+    HIP_TRY(hipMalloc(&stage, sizeof(uint64_t) * max_step));
+    size_t off = 0;
+    for (uint32_t nout = 2; nout <= n1; nout <<= 1) {
+        const uint32_t rank_in = 2 * n1 / nout;
+        const size_t words = (size_t)rank_in * L * 2 * nl * nout;
+        double *d = nullptr;
+        HIP_TRY(hipMalloc(&d, sizeof(double) * words));
+        k->cascade.push_back(d);
+        k->bytes.push_back(sizeof(double) * words);
+        HIP_TRY(hipMemcpyAsync(stage, cascade_key + off, sizeof(uint64_t) * words, hipMemcpyHostToDevice, c->stream));
+        launch_key_relayout(stage, d, (int)rank_in, (int)L, (int)nl, (int)nout, c->stream);
+        RingDev r;
+        if ((s = make_ring(c, nout, pp->q1, 2, &r))) return s;
+        launch_ntt(r, d, (size_t)L * 2 * nl * rank_in, rank_in, false, c->stream);
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        off += words;
+    }
+    HIP_TRY(hipFree(stage));
+    // conversion key RLev(L_CK) at (n1, q1): layout already [k][part][limb][n]
+    {
+        const size_t words = (size_t)L * 2 * nl * n1;
+        uint64_t *u = nullptr;
+        HIP_TRY(hipMalloc(&u, sizeof(uint64_t) * words));
+        HIP_TRY(hipMalloc(&k->conv, sizeof(double) * words));
+        HIP_TRY(hipMemcpyAsync(u, conv_key, sizeof(uint64_t) * words, hipMemcpyHostToDevice, c->stream));
+        launch_u64_to_f64(u, k->conv, words, c->stream);
+        RingDev r;
+        if ((s = make_ring(c, n1, pp->q1, 2, &r))) return s;
+        launch_ntt(r, k->conv, (size_t)L * 2 * nl, 1, false, c->stream);
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipFree(u));
+        k->bytes.push_back(sizeof(double) * words);
+    }
+    // ring-switch key: d RLev(L_RSK) at (n2, q3)
+    {
+        const uint32_t d = n1 / pp->n2;
+        const size_t words = (size_t)d * pp->l_rsk * 2 * pp->n2;
+        uint64_t *u = nullptr;
+        HIP_TRY(hipMalloc(&u, sizeof(uint64_t) * words));
+        HIP_TRY(hipMalloc(&k->rsk, sizeof(double) * words));
+        HIP_TRY(hipMemcpyAsync(u, rsk, sizeof(uint64_t) * words, hipMemcpyHostToDevice, c->stream));
+        launch_u64_to_f64(u, k->rsk, words, c->stream);
+        RingDev r;
+        uint64_t q3 = pp->q3;
+        if ((s = make_ring(c, pp->n2, &q3, 1, &r))) return s;
+        launch_ntt(r, k->rsk, (size_t)d * pp->l_rsk * 2, 1, false, c->stream);
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipFree(u));
+        k->bytes.push_back(sizeof(double) * words);
+    }
+    HIP_TRY(hipGetLastError());
+    *out = k;
+    return VIA_OK;
+}
+
+void via_keys_destroy(via_keys *k) {
+    if (!k) return;
+    via_ctx *c = k->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    size_t i = 0;
+    for (double *p : k->cascade) {
+        (void)hipMemsetAsync(p, 0, k->bytes[i++], c->stream);
+    }
+    (void)hipMemsetAsync(k->conv, 0, k->bytes[i++], c->stream);
+    (void)hipMemsetAsync(k->rsk, 0, k->bytes[i++], c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    for (double *p : k->cascade) (void)hipFree(p);
+    (void)hipFree(k->conv);
+    (void)hipFree(k->rsk);
+    delete k;
+}
+
+// ---- answer -------------------------------------------------------------------------------
+
+static via_status answer_impl(via_ctx *c, const via_db *db, const via_keys *keys, const uint64_t *d_lwes,
+                              size_t n_lwe, uint64_t *d_mask, uint64_t *d_body, via_err_detail *err) {
+    const via_params *pp = &keys->pp;
+    const size_t I = pp->num_rows, J = pp->num_cols, n = pp->n1, nl = 2, S = kSlots;
+    // guards of answer_through_crot (answer.rs:131-157)
+    if (!is_pow2(I) || !is_pow2(J)) return VIA_E_DIM;
+    if (db->pp.num_rows != pp->num_rows || db->pp.num_cols != pp->num_cols || db->pp.n1 != pp->n1 ||
+        db->pp.q2 != pp->q2)
+        return VIA_E_DIM;
+    const int nd = ilog2(I), nc = ilog2(J), nr = ilog2(pp->n1 / pp->n_rec);
+    const size_t bits = (size_t)(nd + nc + nr), lq = pp->l_query;
+    const size_t expected = bits * lq;
+    if (n_lwe != expected) {
+        if (err) {
+            err->expected = expected;
+            err->got = n_lwe;
+            snprintf(err->msg, sizeof err->msg, "query length mismatch: expected %zu LWEs, got %zu", expected, n_lwe);
+        }
+        return VIA_E_QUERY_LEN;
+    }
+    via_status s = ensure_ws(c, pp, n_lwe, bits);
+    if (s) return s;
+    Workspace &w = c->ws;
+    hipStream_t st = c->stream;
+    const size_t B = n_lwe;
+
+    RingDev r1n;  // q1 at n1
+    if ((s = make_ring(c, (uint32_t)n, pp->q1, 2, &r1n))) return s;
+    RingDev r2;
+    uint64_t q2 = pp->q2;
+    if ((s = make_ring(c, (uint32_t)n, &q2, 1, &r2))) return s;
+    DevTab t3;
+    if ((s = get_tab(c, pp->q3, pp->n2, &t3))) return s;
+
+    HIP_TRY(hipEventRecord(c->ev[0], st));
+    // ---- step 1: query_decomp (query_decomp.rs:39-95) ----
+    launch_u64_to_f64(d_lwes, w.lwe_f, B * nl * (n + 1), st);
+    const Gadget gck = make_gadget(pp->ck_base, pp->l_ck);
+    int cur = 0;
+    for (uint32_t nout = 2, step = 0; nout <= n; nout <<= 1, step++) {
+        RingDev r;
+        if ((s = make_ring(c, nout, pp->q1, 2, &r))) return s;
+        const uint32_t rank_in = (uint32_t)(2 * n / nout);
+        const bool first = (nout == 2);
+        MacArgs a;
+        a.in = first ? w.lwe_f : w.masks[cur];
+        a.in_item_stride = first ? nl * (n + 1) : nl * S;
+        a.in_limb_stride = first ? (n + 1) : S;
+        a.embed = 1;
+        a.key = keys->cascade[step];
+        a.key_item_stride = 0;
+        a.key_slots = (size_t)rank_in * nout;
+        a.tiles = 2;
+        a.out_mask = w.pm;
+        a.out_body = w.pb;
+        a.reduce_body = 1;
+        a.items = (int)B;
+        launch_gadget_mac(r, gck, a, st);
+        CombineArgs cb;
+        cb.pm = w.pm;
+        cb.pb = w.pb;
+        const uint32_t nin = nout / 2;
+        cb.body_prev = first ? w.lwe_f + n : w.body[cur];
+        cb.body_item_stride = first ? nl * (n + 1) : nl * nin;
+        cb.body_limb_stride = first ? (n + 1) : nin;
+        cb.masks_next = w.masks[cur ^ 1];
+        cb.body_next = w.body[cur ^ 1];
+        cb.items = (int)B;
+        launch_cascade_combine(r, cb, st);
+        cur ^= 1;
+    }
+    // rlwe_to_rgsw_eval (convert.rs:162-180) with the conversion key
+    {
+        MacArgs a;
+        a.in = w.masks[cur];
+        a.in_item_stride = nl * n;
+        a.in_limb_stride = n;
+        a.embed = 0;
+        a.key = keys->conv;
+        a.key_item_stride = 0;
+        a.key_slots = n;
+        a.tiles = 1;
+        a.out_mask = w.pm;
+        a.out_body = w.pb;
+        a.reduce_body = 0;
+        a.items = (int)B;
+        launch_gadget_mac(r1n, gck, a, st);
+        launch_r2g_combine(r1n, w.pm, w.pb, w.masks[cur], w.body[cur], w.rgsw, (int)bits, (int)lq, st);
+    }
+    c->stage_ptr[0] = w.rgsw;
+    c->stage_words[0] = bits * 4 * lq * nl * n;
+    c->stage_nl[0] = 2;
+    HIP_TRY(hipEventRecord(c->ev[1], st));
+
+    // ---- step 2: dmux_tree from trivial(Delta) (answer.rs:181-191, mux.rs:222-251) ----
+    const Gadget g1 = make_gadget(pp->b1, pp->l_query);
+    const size_t rgw = 4 * lq * nl * n;  // words per RGSW at q1
+    HIP_TRY(hipMemcpyAsync(w.rgsw_eval, w.rgsw, sizeof(double) * nd * rgw, hipMemcpyDeviceToDevice, st));
+    launch_ntt(r1n, w.rgsw_eval, (size_t)nd * 4 * lq * nl, 1, false, st);
+    {
+        u128 Q1 = (u128)pp->q1[0] * pp->q1[1];
+        u128 delta = (Q1 + pp->p - 1) / pp->p;
+        std::vector<double> triv(2 * nl * n, 0.0);
+        triv[nl * n + 0] = (double)(uint64_t)(delta % pp->q1[0]);
+        triv[nl * n + n] = (double)(uint64_t)(delta % pp->q1[1]);
+        HIP_TRY(hipMemcpyAsync(w.dm[0], triv.data(), sizeof(double) * triv.size(), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));  // triv is a host temporary
+    }
+    int dc = 0;
+    for (int i = 0; i < nd; i++) {
+        launch_gate(r1n, g1, GATE_DMUX, w.rgsw_eval + (size_t)i * rgw, w.dm[dc], w.dm[dc ^ 1], 1 << i, 0, st);
+        dc ^= 1;
+    }
+    c->stage_ptr[1] = w.dm[dc];
+    c->stage_words[1] = I * 2 * nl * n;
+    c->stage_nl[1] = 2;
+    HIP_TRY(hipEventRecord(c->ev[2], st));
+
+    // ---- step 3: mod_switch_sym q1 -> q2 per row ----
+    launch_modswitch_q1_q2(r1n, pp->q2, w.dm[dc], w.sw, I * 2, st);
+    c->stage_ptr[2] = w.sw;
+    c->stage_words[2] = I * 2 * n;
+    c->stage_nl[2] = 1;
+    HIP_TRY(hipEventRecord(c->ev[3], st));
+
+    // ---- step 4: first_dim (first_dim.rs:94-184) ----
+    HIP_TRY(hipMemcpyAsync(w.sel, w.sw, sizeof(double) * I * 2 * n, hipMemcpyDeviceToDevice, st));
+    launch_ntt(r2, w.sel, I * 2, 1, false, st);
+    HIP_TRY(hipEventRecord(c->ev_fd[0], st));
+    launch_first_dim(db->cells, w.sel, w.fd, (int)I, (int)J, (int)n, fmod_make(pp->q2), st);
+    HIP_TRY(hipEventRecord(c->ev_fd[1], st));
+    launch_ntt(r2, w.fd, J * 2, 1, true, st);
+    c->stage_ptr[3] = w.fd;
+    c->stage_words[3] = J * 2 * n;
+    c->stage_nl[3] = 1;
+    HIP_TRY(hipEventRecord(c->ev[4], st));
+
+    // ---- step 5: cmux_tree over mod-switched RGSWs (answer.rs:194-203) ----
+    const Gadget g2 = make_gadget(pp->b2, pp->l_query);
+    const size_t rgw2 = 4 * lq * n;
+    if (nc + nr > 0) {
+        launch_modswitch_q1_q2(r1n, pp->q2, w.rgsw + (size_t)nd * rgw, w.rgsw2, (size_t)(nc + nr) * 4 * lq, st);
+        launch_ntt(r2, w.rgsw2, (size_t)(nc + nr) * 4 * lq, 1, false, st);
+    }
+    const double *cm_in = w.fd;
+    int cc = 0;
+    for (int b = 0; b < nc; b++) {
+        launch_gate(r2, g2, GATE_CMUX, w.rgsw2 + (size_t)b * rgw2, cm_in, w.cm[cc], (int)(J >> (b + 1)), 0, st);
+        cm_in = w.cm[cc];
+        cc ^= 1;
+    }
+    c->stage_ptr[4] = cm_in;
+    c->stage_words[4] = 2 * n;
+    c->stage_nl[4] = 1;
+    HIP_TRY(hipEventRecord(c->ev[5], st));
+
+    // ---- step 6: crot SlotExtract (rotate.rs:128-149) ----
+    const double *res = cm_in;
+    int rc = 0;
+    for (int b = 0; b < nr; b++) {
+        launch_gate(r2, g2, GATE_CROT, w.rgsw2 + (size_t)(nc + b) * rgw2, res, w.res[rc], 1, (int)(n - (1u << b)), st);
+        res = w.res[rc];
+        rc ^= 1;
+    }
+    c->stage_ptr[5] = res;
+    c->stage_words[5] = 2 * n;
+    c->stage_nl[5] = 1;
+    HIP_TRY(hipEventRecord(c->ev[6], st));
+
+    // ---- step 7: resp_comp (resp_comp.rs:46-76) ----
+    RespArgs ra;
+    ra.ct = res;
+    ra.rsk_eval = keys->rsk;
+    ra.n1 = (int)n;
+    ra.n2 = (int)pp->n2;
+    ra.d = (int)(n / pp->n2);
+    ra.l_rsk = (int)pp->l_rsk;
+    ra.b_rsk = (uint32_t)pp->b_rsk;
+    ra.q2 = pp->q2;
+    ra.q3 = pp->q3;
+    ra.q4 = pp->q4;
+    ra.t3.fwd = t3.fwd;
+    ra.t3.inv = t3.inv;
+    ra.t3.n_inv = t3.n_inv;
+    ra.t3.m = fmod_make(pp->q3);
+    ra.out_mask = d_mask;
+    ra.out_body = d_body;
+    launch_resp_comp(ra, st);
+    HIP_TRY(hipEventRecord(c->ev[7], st));
+    HIP_TRY(hipGetLastError());
+    c->have_times = true;
+    return VIA_OK;
+}
+
+via_status via_answer_device(via_ctx *c, const via_db *db, const via_keys *keys, const uint64_t *d_lwes,
+                             size_t n_lwe, uint64_t *d_mask, uint64_t *d_body, via_err_detail *err) {
+    if (!c || !db || !keys || !d_mask || !d_body || (n_lwe && !d_lwes)) return VIA_E_INVALID;
+    HIP_TRY(hipSetDevice(c->device));
+    return answer_impl(c, db, keys, d_lwes, n_lwe, d_mask, d_body, err);
+}
+
+via_status via_answer(via_ctx *c, const via_db *db, const via_keys *keys, const uint64_t *lwes, size_t n_lwe,
+                      uint64_t *ans_mask, uint64_t *ans_body, via_err_detail *err) {
+    if (!c || !db || !keys || !ans_mask || !ans_body || (n_lwe && !lwes)) return VIA_E_INVALID;
+    HIP_TRY(hipSetDevice(c->device));
+    const via_params *pp = &keys->pp;
+    const int nd = ilog2(pp->num_rows), nc = ilog2(pp->num_cols), nr = ilog2(pp->n1 / pp->n_rec);
+    const size_t expected = (size_t)(nd + nc + nr) * pp->l_query;
+    if (!is_pow2(pp->num_rows) || !is_pow2(pp->num_cols)) return VIA_E_DIM;
+    if (n_lwe != expected) return answer_impl(c, db, keys, nullptr, n_lwe, nullptr, nullptr, err);
+    via_status s = ensure_ws(c, pp, n_lwe, (size_t)(nd + nc + nr));
+    if (s) return s;
+    Workspace &w = c->ws;
+    const size_t lw = (size_t)2 * (pp->n1 + 1);
+    HIP_TRY(hipMemcpyAsync(w.lwe_u, lwes, sizeof(uint64_t) * lw * n_lwe, hipMemcpyHostToDevice, c->stream));
+    s = answer_impl(c, db, keys, w.lwe_u, n_lwe, w.ans_u, w.ans_u + 1024, err);
+    if (s) return s;
+    HIP_TRY(hipMemcpyAsync(ans_mask, w.ans_u, sizeof(uint64_t) * pp->n2, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(ans_body, w.ans_u + 1024, sizeof(uint64_t) * pp->n2, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return VIA_OK;
+}
+
+via_status via_debug_stage(via_ctx *c, int stage, uint64_t *out, size_t words) {
+    if (!c || stage < 1 || stage > 6 || !out) return VIA_E_INVALID;
+    const double *p = c->stage_ptr[stage - 1];
+    const size_t total = c->stage_words[stage - 1];
+    if (!p || words < total) return VIA_E_INVALID;
+    uint64_t *tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, sizeof(uint64_t) * total));
+    launch_f64_to_u64(p, tmp, total, c->stream);
+    HIP_TRY(hipMemcpyAsync(out, tmp, sizeof(uint64_t) * total, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipFree(tmp));
+    return VIA_OK;
+}
+
+// ---- ring-level entry points -----------------------------------------------------------
+
+static via_status ring_for(via_ctx *c, uint32_t n, const uint64_t *moduli, uint32_t nmod, RingDev *r) {
+    if (!is_pow2(n) || n < 2 || n > (uint32_t)kSlots || nmod < 1 || nmod > 2) return VIA_E_INVALID;
+    for (uint32_t i = 0; i < nmod; i++)
+        if (moduli[i] >= (1ULL << 50) || (moduli[i] - 1) % (2ULL * n)) return VIA_E_INVALID;
+    return make_ring(c, n, moduli, (int)nmod, r);
+}
+
+via_status via_poly_mul_device(via_ctx *c, uint32_t n, const uint64_t *moduli, uint32_t nmod, size_t group,
+                               const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_c, size_t count) {
+    if (!c || !moduli || !group) return VIA_E_INVALID;
+    HIP_TRY(hipSetDevice(c->device));
+    RingDev r;
+    via_status s = ring_for(c, n, moduli, nmod, &r);
+    if (s) return s;
+    if (n > (uint32_t)kSlots) return VIA_E_INVALID;
+    launch_polymul_u64(r, d_a, d_b, d_c, count, group, c->stream);
+    HIP_TRY(hipGetLastError());
+    return VIA_OK;
+}
+
+via_status via_ntt(via_ctx *c, uint32_t n, const uint64_t *moduli, uint32_t nmod, size_t group, uint64_t *polys,
+                   size_t count, int inverse) {
+    if (!c || !moduli || !polys || !group) return VIA_E_INVALID;
+    HIP_TRY(hipSetDevice(c->device));
+    RingDev r;
+    via_status s = ring_for(c, n, moduli, nmod, &r);
+    if (s) return s;
+    const size_t P = kSlots / n;
+    if (nmod == 2 && group % P) return VIA_E_INVALID;
+    const size_t words = count * n;
+    uint64_t *u = nullptr;
+    double *d = nullptr;
+    HIP_TRY(hipMalloc(&u, sizeof(uint64_t) * words));
+    HIP_TRY(hipMalloc(&d, sizeof(double) * words));
+    HIP_TRY(hipMemcpyAsync(u, polys, sizeof(uint64_t) * words, hipMemcpyHostToDevice, c->stream));
+    launch_u64_to_f64(u, d, words, c->stream);
+    launch_ntt(r, d, count, group, inverse != 0, c->stream);
+    launch_f64_to_u64(d, u, words, c->stream);
+    HIP_TRY(hipMemcpyAsync(polys, u, sizeof(uint64_t) * words, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipFree(u));
+    HIP_TRY(hipFree(d));
+    return VIA_OK;
+}
+
+via_status via_device_alloc(via_ctx *c, size_t bytes, void **d) {
+    if (!c || !d) return VIA_E_INVALID;
+    HIP_TRY(hipSetDevice(c->device));
+    if (hipMalloc(d, bytes ? bytes : 1) != hipSuccess) return VIA_E_NOMEM;
+    return VIA_OK;
+}
+
+via_status via_device_free(via_ctx *c, void *d) {
+    if (!c) return VIA_E_INVALID;
+    HIP_TRY(hipFree(d));
+    return VIA_OK;
+}
+
+via_status via_copy_to_device(via_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (!c) return VIA_E_INVALID;
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return VIA_OK;
+}
+
+via_status via_copy_to_host(via_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (!c) return VIA_E_INVALID;
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return VIA_OK;
+}
+
+}  // extern "C"

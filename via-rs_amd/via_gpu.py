@@ -1,0 +1,297 @@
+"""Host-side mirror of the via-rs server interface over the C ABI (include/via_gpu.h).
+
+`Server.setup(config, records)` / `Server.answer(query)` follow
+`via_server::Server::{setup, answer}` (crates/via-server/src/answer.rs:486-554);
+errors are raised as `ViaError` with the reference's variants
+(crates/via-protocol/src/error.rs:20-64). Every call goes to the HIP library
+libvia_gpu.so built for gfx950; there is no other backend, and loading fails
+loudly when the library is missing.
+"""
+import ctypes as C
+import os
+import weakref
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvia_gpu.so")
+
+u64p = C.POINTER(C.c_uint64)
+
+VIA_OK, VIA_E_DIM, VIA_E_QUERY_LEN, VIA_E_INDEX, VIA_E_NOMEM, VIA_E_INVALID, VIA_E_DEVICE, VIA_E_BATCH_LEN = range(8)
+
+
+class Params(C.Structure):
+    """via_params: PIRParams restricted to the server path (params.rs)."""
+
+    _fields_ = [
+        ("n1", C.c_uint32), ("n2", C.c_uint32), ("n_rec", C.c_uint32),
+        ("q1_nl", C.c_uint32), ("q1", C.c_uint64 * 2),
+        ("q2", C.c_uint64), ("q3", C.c_uint64), ("q4", C.c_uint64), ("p", C.c_uint64),
+        ("b1", C.c_uint64), ("b2", C.c_uint64), ("b_rsk", C.c_uint64), ("ck_base", C.c_uint64),
+        ("l_query", C.c_uint32), ("l_ck", C.c_uint32), ("l_rsk", C.c_uint32),
+        ("num_rows", C.c_uint32), ("num_cols", C.c_uint32),
+    ]
+
+    @property
+    def total_bits(self):
+        lg = lambda x: (x - 1).bit_length() if x > 1 else 0
+        return lg(self.num_rows) + lg(self.num_cols) + lg(self.n1 // self.n_rec)
+
+    @property
+    def query_lwes(self):
+        return self.total_bits * self.l_query
+
+    @property
+    def lwe_words(self):
+        return self.q1_nl * (self.n1 + 1)
+
+
+def paper_params(num_rows, num_cols):
+    """The paper VIA-C stack as the reference's paper e2e runs it
+    (client_server_e2e_paper.rs:205-277): n1=2048, n2=512, q1 two-prime RNS,
+    q2=17175674881, q3=8380417, q4=2^12, p=16, L_Q=2 (b1=55879, b2=81),
+    L_CK=18 (base 18), L_RSK=8 (base 8)."""
+    pp = Params()
+    pp.n1, pp.n2, pp.n_rec, pp.q1_nl = 2048, 512, 512, 2
+    pp.q1[0], pp.q1[1] = 137438822401, 274810798081
+    pp.q2, pp.q3, pp.q4, pp.p = 17175674881, 8380417, 4096, 16
+    pp.b1, pp.b2, pp.b_rsk, pp.ck_base = 55879, 81, 8, 18
+    pp.l_query, pp.l_ck, pp.l_rsk = 2, 18, 8
+    pp.num_rows, pp.num_cols = num_rows, num_cols
+    return pp
+
+
+class ErrDetail(C.Structure):
+    _fields_ = [("expected", C.c_uint64), ("got", C.c_uint64), ("msg", C.c_char * 128)]
+
+
+class ViaError(Exception):
+    """ViaError::{DimMismatch, QueryLengthMismatch, IndexOutOfRange} and the
+    device-side failures of the C ABI."""
+
+    def __init__(self, status, expected=None, got=None, msg=""):
+        self.status = status
+        self.expected = expected
+        self.got = got
+        super().__init__(f"via status {status}: {msg}")
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run python -c 'import __graft_entry__ as g; g.build()'")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER(Params)
+        vp = C.c_void_p
+        sig = {
+            "via_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+            "via_ctx_destroy": (None, [vp]),
+            "via_ctx_step_times": (C.c_int, [vp, C.POINTER(C.c_double)]),
+            "via_ctx_synchronize": (C.c_int, [vp]),
+            "via_ctx_first_dim_time": (C.c_int, [vp, C.POINTER(C.c_double)]),
+            "via_db_create": (C.c_int, [vp, P, u64p, C.c_size_t, C.POINTER(vp)]),
+            "via_db_create_bytes": (C.c_int, [vp, P, C.c_char_p, C.c_size_t, C.POINTER(vp)]),
+            "via_db_destroy": (None, [vp]),
+            "via_db_read_cells": (C.c_int, [vp, u64p, C.c_size_t]),
+            "via_keys_create": (C.c_int, [vp, P, u64p, u64p, u64p, C.POINTER(vp)]),
+            "via_keys_destroy": (None, [vp]),
+            "via_answer": (C.c_int, [vp, vp, vp, u64p, C.c_size_t, u64p, u64p, C.POINTER(ErrDetail)]),
+            "via_answer_device": (C.c_int, [vp, vp, vp, vp, C.c_size_t, vp, vp, C.POINTER(ErrDetail)]),
+            "via_debug_stage": (C.c_int, [vp, C.c_int, u64p, C.c_size_t]),
+            "via_poly_mul_device": (C.c_int, [vp, C.c_uint32, u64p, C.c_uint32, C.c_size_t, vp, vp, vp, C.c_size_t]),
+            "via_ntt": (C.c_int, [vp, C.c_uint32, u64p, C.c_uint32, C.c_size_t, u64p, C.c_size_t, C.c_int]),
+            "via_device_alloc": (C.c_int, [vp, C.c_size_t, C.POINTER(vp)]),
+            "via_device_free": (C.c_int, [vp, vp]),
+            "via_copy_to_device": (C.c_int, [vp, vp, vp, C.c_size_t]),
+            "via_copy_to_host": (C.c_int, [vp, vp, vp, C.c_size_t]),
+            "via_status_str": (C.c_char_p, [C.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status, err=None):
+    if status != VIA_OK:
+        msg = lib().via_status_str(status).decode()
+        if err is not None and status == VIA_E_QUERY_LEN:
+            raise ViaError(status, err.expected, err.got, msg)
+        raise ViaError(status, msg=msg)
+
+
+def _u64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint64))
+
+
+def _p(a):
+    return a.ctypes.data_as(u64p)
+
+
+class Context:
+    """A device context (one HIP stream on one GPU)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().via_ctx_create(device, C.byref(h)))
+        self.h = h
+        self._children = weakref.WeakSet()
+
+    def adopt(self, child):
+        """Databases and keys live on this context and are released before it."""
+        self._children.add(child)
+
+    def close(self):
+        if self.h:
+            for ch in list(self._children):
+                ch.close()
+            lib().via_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        _check(lib().via_ctx_synchronize(self.h))
+
+    def step_times(self):
+        t = (C.c_double * 7)()
+        _check(lib().via_ctx_step_times(self.h, t))
+        return list(t)
+
+    def first_dim_time(self):
+        t = C.c_double()
+        _check(lib().via_ctx_first_dim_time(self.h, C.byref(t)))
+        return t.value
+
+    def stage(self, k, words):
+        out = np.zeros(words, dtype=np.uint64)
+        _check(lib().via_debug_stage(self.h, k, _p(out), words))
+        return out
+
+    def ntt(self, n, moduli, polys, group=1, inverse=False):
+        m = _u64(moduli)
+        a = _u64(polys).copy()
+        _check(lib().via_ntt(self.h, n, _p(m), len(m), group, _p(a), a.size // n, int(inverse)))
+        return a
+
+    # device buffers for callers that keep inputs resident in HBM
+    def alloc(self, nbytes):
+        d = C.c_void_p()
+        _check(lib().via_device_alloc(self.h, nbytes, C.byref(d)))
+        return d
+
+    def free(self, d):
+        _check(lib().via_device_free(self.h, d))
+
+    def to_device(self, d, arr):
+        arr = np.ascontiguousarray(arr)
+        _check(lib().via_copy_to_device(self.h, d, arr.ctypes.data_as(C.c_void_p), arr.nbytes))
+
+    def to_host(self, arr, d):
+        _check(lib().via_copy_to_host(self.h, arr.ctypes.data_as(C.c_void_p), d, arr.nbytes))
+        return arr
+
+    def poly_mul_device(self, n, moduli, group, d_a, d_b, d_c, count):
+        m = _u64(moduli)
+        _check(lib().via_poly_mul_device(self.h, n, _p(m), len(m), group, d_a, d_b, d_c, count))
+
+
+class PreparedDb:
+    """PreparedDb (prepared_db.rs): the HBM-resident eval-form database."""
+
+    def __init__(self, ctx, pp, records=None, record_bytes=None, n_records=None):
+        h = C.c_void_p()
+        if record_bytes is not None:
+            b = bytes(record_bytes)
+            _check(lib().via_db_create_bytes(ctx.h, C.byref(pp), b, n_records, C.byref(h)))
+        else:
+            r = _u64(records)
+            _check(lib().via_db_create(ctx.h, C.byref(pp), _p(r), r.size // pp.n_rec, C.byref(h)))
+        self.h, self.ctx, self.pp = h, ctx, pp
+        ctx.adopt(self)
+
+    def cells(self):
+        words = self.pp.num_rows * self.pp.num_cols * self.pp.n1
+        out = np.zeros(words, dtype=np.uint64)
+        _check(lib().via_db_read_cells(self.h, _p(out), words))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().via_db_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PreparedKeys:
+    """PreparedKeys (prepared_keys.rs:69-77): eval-form cascade, conversion
+    and ring-switch keys, resident in HBM."""
+
+    def __init__(self, ctx, pp, cascade_key, conv_key, rsk):
+        h = C.c_void_p()
+        ck, cv, rs = _u64(cascade_key), _u64(conv_key), _u64(rsk)
+        _check(lib().via_keys_create(ctx.h, C.byref(pp), _p(ck), _p(cv), _p(rs), C.byref(h)))
+        self.h, self.ctx = h, ctx
+        ctx.adopt(self)
+
+    def close(self):
+        if self.h:
+            lib().via_keys_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ServerConfig:
+    """ServerConfig (answer.rs:361-400): public parameters and keys."""
+
+    def __init__(self, pp, cascade_key, conv_key, rsk):
+        self.pp, self.cascade_key, self.conv_key, self.rsk = pp, cascade_key, conv_key, rsk
+
+
+class Server:
+    """via_server::Server on one MI355X."""
+
+    def __init__(self, ctx, pp, db, keys):
+        self.ctx, self.pp, self.db, self.keys = ctx, pp, db, keys
+
+    @classmethod
+    def setup(cls, config, records, ctx=None):
+        """Server::setup (answer.rs:486-528): records [n][n_rec] in [0, p)."""
+        ctx = ctx or Context(0)
+        pp = config.pp
+        db = PreparedDb(ctx, pp, records=records)
+        keys = PreparedKeys(ctx, pp, config.cascade_key, config.conv_key, config.rsk)
+        return cls(ctx, pp, db, keys)
+
+    def answer(self, query):
+        """Server::answer (answer.rs:534-554). `query` is the flat LWE array
+        (n_lwe x q1_nl x (n1+1) words); returns (mask mod q3, body mod q4)."""
+        q = _u64(query)
+        n_lwe = q.size // self.pp.lwe_words
+        am = np.zeros(self.pp.n2, dtype=np.uint64)
+        ab = np.zeros(self.pp.n2, dtype=np.uint64)
+        err = ErrDetail()
+        st = lib().via_answer(self.ctx.h, self.db.h, self.keys.h, _p(q), n_lwe, _p(am), _p(ab), C.byref(err))
+        _check(st, err)
+        return am, ab
