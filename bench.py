@@ -24,6 +24,38 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def synthetic_keys(pp, rng):
+    """One client's public keys (cascade, conversion, ring switch) as uniform
+    residues in the layouts of include/via_gpu.h."""
+    q0, q1 = pp.q1[0], pp.q1[1]
+    n1, L = pp.n1, pp.l_ck
+
+    def limbs(words, period):
+        idx = (np.arange(words) // period) % 2
+        out = rng.integers(0, q0, size=words, dtype=np.uint64)
+        out[idx == 1] = rng.integers(0, q1, size=int((idx == 1).sum()), dtype=np.uint64)
+        return out
+
+    steps = []
+    nout = 2
+    while nout <= n1:
+        steps.append(limbs((2 * n1 // nout) * L * 2 * 2 * nout, nout))
+        nout *= 2
+    cascade = np.concatenate(steps)
+    conv = limbs(L * 2 * 2 * n1, n1)
+    rsk = rng.integers(0, pp.q3, size=(n1 // pp.n2) * pp.l_rsk * 2 * pp.n2, dtype=np.uint64)
+    return cascade, conv, rsk
+
+
+def synthetic_query(pp, rng):
+    q0, q1 = pp.q1[0], pp.q1[1]
+    words = pp.query_lwes * 2 * (pp.n1 + 1)
+    idx = (np.arange(words) // (pp.n1 + 1)) % 2
+    out = rng.integers(0, q0, size=words, dtype=np.uint64)
+    out[idx == 1] = rng.integers(0, q1, size=int((idx == 1).sum()), dtype=np.uint64)
+    return out
+
+
 def synthetic_inputs(pp, seed=2048):
     """Uniform residues in the layouts of include/via_gpu.h. The answer path
     is exact integer arithmetic whose work does not depend on the values, so
@@ -90,6 +122,7 @@ def main():
     ap.add_argument("--rows", type=int, default=256)
     ap.add_argument("--cols", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=1, help="queries per batched pass (distinct clients)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the answer path is single-GPU in this build "
@@ -109,13 +142,24 @@ def main():
     t_setup = time.perf_counter() - t0
 
     L = via_gpu.lib()
-    d_q = ctx.alloc(lwes.nbytes)
-    ctx.to_device(d_q, lwes)
-    d_m, d_b = ctx.alloc(8 * pp.n2), ctx.alloc(8 * pp.n2)
+    Q = args.batch
+    rng = np.random.default_rng(99)
+    client_keys = [keys] + [via_gpu.PreparedKeys(ctx, pp, *synthetic_keys(pp, rng)) for _ in range(Q - 1)]
+    queries = [lwes] + [synthetic_query(pp, rng) for _ in range(Q - 1)]
+    d_q = [ctx.alloc(q.nbytes) for q in queries]
+    for d, q in zip(d_q, queries):
+        ctx.to_device(d, q)
+    d_m = [ctx.alloc(8 * pp.n2) for _ in range(Q)]
+    d_b = [ctx.alloc(8 * pp.n2) for _ in range(Q)]
     err = via_gpu.ErrDetail()
+    vp = C.c_void_p
+    Karr = (vp * Q)(*[k.h.value for k in client_keys])
+    Qarr = (vp * Q)(*[d.value for d in d_q])
+    Marr = (vp * Q)(*[d.value for d in d_m])
+    Barr = (vp * Q)(*[d.value for d in d_b])
 
     def answer():
-        st = L.via_answer_device(ctx.h, db.h, keys.h, d_q, pp.query_lwes, d_m, d_b, C.byref(err))
+        st = L.via_answer_batch_device(ctx.h, db.h, Q, Karr, Qarr, pp.query_lwes, Marr, Barr, C.byref(err))
         if st:
             raise via_gpu.ViaError(st, msg=L.via_status_str(st).decode())
 
@@ -135,15 +179,15 @@ def main():
         steps_acc += np.array(st)
         fd_ms.append(ctx.first_dim_time() * 1e3)
     elapsed = time.perf_counter() - t_start
-    gpu_mask = ctx.to_host(np.zeros(pp.n2, np.uint64), d_m)
-    gpu_body = ctx.to_host(np.zeros(pp.n2, np.uint64), d_b)
+    gpu_mask = ctx.to_host(np.zeros(pp.n2, np.uint64), d_m[0])
+    gpu_body = ctx.to_host(np.zeros(pp.n2, np.uint64), d_b[0])
 
     I, J, n = pp.num_rows, pp.num_cols, pp.n1
-    fd_bytes = I * J * n * 8 + 2 * I * n * 8 + 2 * J * n * 8
+    fd_bytes = I * J * n * 8 + Q * (2 * I * n * 8 + 2 * J * n * 8)
     fd_s = float(np.mean(fd_ms)) * 1e-3
     fd_gbs = fd_bytes / fd_s / 1e9
     lat_ms = np.array(lat) * 1e3
-    value = args.steps / elapsed
+    value = args.steps * Q / elapsed
 
     out = {
         "metric": "server answers/sec + p50/p99 answer latency, VIA-C n=2048, bit-exact vs CPU",
@@ -153,6 +197,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "batch": Q,
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
         "higher_is_better": True,
@@ -161,7 +206,8 @@ def main():
         "dtype": "int (exact modular, residues < 2^38 in fp64 lanes)",
         "data": "synthetic: uniform random 256-byte records, uniform key and query residues",
         "config": {
-            "workload": f"VIA-C paper stack single query, I={I} J={J}, {n_rec} records x 256 B, "
+            "workload": f"VIA-C paper stack, {Q} quer{'y' if Q == 1 else 'ies from distinct clients'} per pass, "
+                        f"I={I} J={J}, {n_rec} records x 256 B, "
                         f"prepared DB {I * J * n * 8 / 2**30:.2f} GiB eval u64-equivalent",
             "num_rows": I, "num_cols": J, "n1": n, "n2": pp.n2, "query_lwes": pp.query_lwes,
         },
@@ -191,9 +237,8 @@ def main():
         }
         out["bit_exact_vs_cpu"] = bool(np.array_equal(am, gpu_mask) and np.array_equal(ab, gpu_body))
     print(json.dumps(out))
-    ctx.free(d_q)
-    ctx.free(d_m)
-    ctx.free(d_b)
+    for d in d_q + d_m + d_b:
+        ctx.free(d)
     ctx.close()
 
 

@@ -105,6 +105,20 @@ via_status via_answer_device(via_ctx *ctx, const via_db *db, const via_keys *key
                              size_t n_lwe, uint64_t *d_ans_mask, uint64_t *d_ans_body, via_err_detail *err);
 via_status via_ctx_synchronize(via_ctx *ctx);
 
+/* Server::answer for nq queries against one database, each with the keys of
+ * its own client (keys[q]); every step runs batched across the queries, the
+ * database is streamed once per group of up to 8 queries. Each answer is
+ * bit-identical to answering the query alone. Host buffers: lwes[q] holds
+ * n_lwe LWEs, masks[q] / bodies[q] receive n2 words. No counterpart in the
+ * reference, which answers one query per call (answer.rs:534-554). */
+via_status via_answer_batch(via_ctx *ctx, const via_db *db, uint32_t nq, const via_keys *const *keys,
+                            const uint64_t *const *lwes, size_t n_lwe, uint64_t *const *masks, uint64_t *const *bodies,
+                            via_err_detail *err);
+/* Same with device-resident buffers, enqueued on the context stream. */
+via_status via_answer_batch_device(via_ctx *ctx, const via_db *db, uint32_t nq, const via_keys *const *keys,
+                                   const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_masks,
+                                   uint64_t *const *d_bodies, via_err_detail *err);
+
 /* Intermediate values of the last answer for parity tests, coefficient form:
  * stage 1 RGSW bits [bit][half][i][part][limb][n1] at q1, 2 DMux outputs
  * [I][part][limb][n1] at q1, 3 mod-switched rows [I][part][n1] at q2,

@@ -93,6 +93,40 @@ int main() {
             count++;
         }
     }
+    // int64 split digits against the u128 reference digits.
+    {
+        const uint32_t bs[] = {18, 55879, 81, 8, 64};
+        const int Ls[] = {18, 2, 2, 8, 7};
+        for (int bi = 0; bi < 5; bi++) {
+            uint32_t B = bs[bi];
+            int L = Ls[bi];
+            DigitPlan pl = digit_plan(B, L);
+            via_u128 bl = 1;
+            for (int k = 0; k < L; k++) bl *= B;
+            for (int t = 0; t < 200000; t++) {
+                via_u128 mag = ((((via_u128)rng() << 64) | rng()) % (bl / 2 + 2));
+                if (t < 4) mag = (t < 2) ? (via_u128)t : bl / 2 + (t - 2);
+                SMag v;
+                v.mag = mag;
+                v.neg = (rng() & 1) && mag != 0;
+                SMag ref = v;
+                int64_t lo, hi;
+                digit_split(v, pl, lo, hi);
+                for (int k = 0; k < L; k++) {
+                    int32_t want = extract_digit(ref, B);
+                    int32_t got;
+                    if (k < pl.h) {
+                        got = digit_i64(lo, B, pl.inv_base);
+                        if (k == pl.h - 1) hi += lo;
+                    } else {
+                        got = digit_i64(hi, B, pl.inv_base);
+                    }
+                    CHECK(got == want, "split digit B=%u t=%d k=%d\n", B, t, k);
+                }
+                count++;
+            }
+        }
+    }
     // Garner reconstruction.
     Rns2 b;
     b.q0 = 137438822401ULL;

@@ -126,3 +126,23 @@ def test_answer_query_length_error(ctx):
         server.answer(case.lwes[: -case.pp_gpu.lwe_words])
     assert e.value.status == via_gpu.VIA_E_QUERY_LEN
     assert (e.value.expected, e.value.got) == (8, 7)
+
+
+def test_answer_batch_mixed_clients(ctx):
+    # Q queries from different clients answered in one batched pass: each
+    # answer equals the oracle's answer for that query alone.
+    import via_gpu
+
+    cases = [paper_case(8, 8, seed=b"via-gpu/batch/client%d" % q, index=(37 * q + 5) % 256, records="random",
+                        rec_seed=b"via-gpu/batch-db") for q in range(5)]
+    pp = cases[0].pp_gpu
+    db = via_gpu.PreparedDb(ctx, pp, records=cases[0].records)
+    keys = [via_gpu.PreparedKeys(ctx, pp, c.cascade_key, c.conv_key, c.rsk) for c in cases]
+    masks, bodies = via_gpu.answer_many(ctx, db, keys, [c.lwes for c in cases], pp)
+    for c, m, b in zip(cases, masks, bodies):
+        assert np.array_equal(m, c.ans_mask) and np.array_equal(b, c.ans_body)
+        assert np.array_equal(c.recover(m, b), c.record)
+    # the same query alone gives the same bytes
+    server = via_gpu.Server(ctx, pp, db, keys[2])
+    m2, b2 = server.answer(cases[2].lwes)
+    assert np.array_equal(m2, masks[2]) and np.array_equal(b2, bodies[2])

@@ -12,6 +12,7 @@ namespace via {
 constexpr int kSlots = 2048;    // coefficients handled by one workgroup
 constexpr int kThreads = 256;   // workgroup size of the polynomial kernels
 constexpr int kPer = kSlots / kThreads;
+constexpr int kMaxQ = 32;       // queries answered together in one batched launch
 
 // Twiddles of one (q, n): bit-reversed psi^k and psi^-k as exact doubles,
 // the layout of via-rs ring/ntt.rs:206-221.
@@ -53,63 +54,94 @@ void launch_f64_to_u64(const double *src, uint64_t *dst, size_t n, hipStream_t s
 void launch_polymul_u64(const RingDev &r, const uint64_t *a, const uint64_t *b, uint64_t *c, size_t count,
                         size_t limb_group, hipStream_t s);
 
-// Gadget-product multiply-accumulate.
+// Digits of the LWE masks of `items` LWEs (int16 planes [item][k][n]).
+void launch_lwe_digits(const RingDev &r1, const Gadget &g, const DigitPlan &dp, const double *lwe, int items,
+                       int16_t *dig, hipStream_t s);
+
+// Gadget-product multiply-accumulate over digit planes.
 struct MacArgs {
-    const double *in;        // coefficient-form inputs
-    size_t in_item_stride;   // words between items
-    size_t in_limb_stride;   // words between limbs of one item
-    int embed;               // 1: input degree n/2 embedded at slot 0 (x -> x^2)
-    const double *key;       // eval-form RLev, layout [k][part][limb][key_slots]
-    size_t key_item_stride;  // 0 for a key shared by all items
+    const int16_t *dig;      // [item][k][dig_coeffs] digits of the inputs
+    size_t dig_coeffs;       // coefficients per digit plane (tiles * S or tiles * S/2)
+    int embed;               // 1: inputs of degree n/2 embedded at slot 0 (x -> x^2)
+    const double *key[kMaxQ];  // eval-form RLev per query (client), [k][part][limb][key_slots]
+    int items_per_key;       // consecutive items that share one key
     size_t key_slots;        // slots per (k, part, limb) plane
-    int tiles;               // workgroups per item (tile t covers slots [t*S, t*S+S))
-    double *out_mask;        // [item][tile][limb][S] eval
-    double *out_body;        // [item][tile][limb][S] or [item][tile][limb][n] when reduce_body
+    int tiles;               // workgroups per item along the slots (tile t covers [t*S, t*S+S))
+    int groups;              // workgroups per item along the digits
+    double *out_mask;        // [group][item][tile][limb][S] eval
+    double *out_body;        // same, or [group][item][tile][limb][n] when reduce_body
     int reduce_body;
     int items;
 };
 void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t s);
 
-// One cascade combine: masks_next = -iNTT(pm[t0] + pm[t1]), body_next =
-// embed(body_prev) - iNTT(pb[t0] + pb[t1]).
+// One cascade combine: masks_next = -iNTT(sum pm), body_next =
+// embed(body_prev) - iNTT(sum pb), digits of masks_next into dig_next.
 struct CombineArgs {
     const double *pm, *pb;    // MAC outputs with tiles == 2
+    int groups, items;
     const double *body_prev;  // [item] stride body_item_stride, limb stride body_limb_stride
     size_t body_item_stride, body_limb_stride;
     double *masks_next;       // [item][limb][S]
     double *body_next;        // [item][limb][n]
-    int items;
+    int16_t *dig_next;        // [item][k][S] or null
 };
-void launch_cascade_combine(const RingDev &r, const CombineArgs &a, hipStream_t s);
+void launch_cascade_combine(const RingDev &r, const Gadget &g, const DigitPlan &dp, const RingDev &rq,
+                            const CombineArgs &a, hipStream_t s);
 
 // rlwe_to_rgsw combine: neg_s_m[i] = (B_i + iNTT(pm), iNTT(pb)); m[i] = level i.
-void launch_r2g_combine(const RingDev &r, const double *pm, const double *pb, const double *rlwe_masks,
+void launch_r2g_combine(const RingDev &r, const double *pm, const double *pb, int groups, const double *rlwe_masks,
                         const double *rlwe_bodies, double *rgsw, int bits, int lq, hipStream_t s);
 
-// Gate kernels over an eval-form RGSW (layout [half][i][part][limb][n]).
+// Gate kernels over an eval-form RGSW (layout [half][i][part][limb][n]). Item
+// b of the launch is output j = b % items_per_query of query b / items_per_query.
 enum GateMode { GATE_DMUX = 0, GATE_CMUX = 1, GATE_CROT = 2 };
-void launch_gate(const RingDev &r, const Gadget &g, int mode, const double *rgsw_eval, const double *in,
-                 double *out, int items, int rot_shift, hipStream_t s);
+struct GateArgs {
+    int mode;
+    const double *rgsw;  // query q's RGSW at rgsw + q * rgsw_qstride
+    size_t rgsw_qstride;
+    const double *in;
+    size_t in_qstride;
+    double *out;
+    size_t out_qstride;
+    int items_per_query;
+    int queries;
+    int shift;           // CRot rotation N - 2^i
+};
+void launch_gate(const RingDev &r, const Gadget &g, const GateArgs &a, hipStream_t s);
+
+// Gadget digits of Delta (the trivial DMux root) and Delta per limb.
+struct DeltaDigits {
+    int32_t d[8];
+    double delta[2];
+};
+void launch_dmux_trivial(const RingDev &r, const double *rgsw_coeff, size_t rgsw_qstride, int L, const DeltaDigits &dd,
+                         double *out, size_t out_qstride, int queries, hipStream_t s);
 
 // Mod switch of a coefficient-form array at (n, Q1 two limbs) to q2: rows of
 // 2 polys [row][part][limb][n] -> [row][part][n].
 void launch_modswitch_q1_q2(const RingDev &r1, uint64_t q2, const double *in, double *out, size_t polys,
                             hipStream_t s);
 
-// FirstDim multiply-accumulate: out[j][part][k] = sum_i db[i][j][k] * sel[i][part][k].
-void launch_first_dim(const double *db, const double *sel_eval, double *out_eval, int I, int J, int n, FMod m,
-                      hipStream_t s);
+// FirstDim multiply-accumulate for Q queries in one database pass:
+// out[q][j][part][k] = sum_i db[i][j][k] * sel[q][i][part][k]. out_eval holds
+// `chunks` partial planes of Q*J*2*n words; the result lands in the first.
+int first_dim_chunks(int I, int J, int n, int Q);
+void launch_first_dim(const double *db, const double *sel_eval, double *out_eval, int I, int J, int n, int Q,
+                      int chunks, FMod m, hipStream_t s);
 
-// RespComp for one ciphertext at (n1, q2): mod switch to q3, ring switch to n2
-// with the eval RSK ([j][k][part][n2]), then mask at q3 / body at q4.
+// RespComp for Q ciphertexts at (n1, q2): mod switch to q3, ring switch to n2
+// with each query's eval RSK ([j][k][part][n2]), then mask at q3 / body at q4.
 struct RespArgs {
-    const double *ct;  // [part][n1] at q2
-    const double *rsk_eval;
+    const double *ct;  // query q at ct + q * 2 * n1, [part][n1] at q2
+    const double *rsk_eval[kMaxQ];
+    uint64_t *out_mask[kMaxQ];
+    uint64_t *out_body[kMaxQ];
     int n1, n2, d, l_rsk;
     uint32_t b_rsk;
     uint64_t q2, q3, q4;
     NttTab t3;        // q3 at n2
-    uint64_t *out_mask, *out_body;
+    int queries;
 };
 void launch_resp_comp(const RespArgs &a, hipStream_t s);
 

@@ -35,54 +35,80 @@ __device__ __forceinline__ SMag gadget_scale(double a0, double a1, const RingDev
 
 __device__ __forceinline__ double digit_residue(int32_t d, const FMod &m) { return d >= 0 ? (double)d : (double)d + m.q; }
 
-// ---- gadget multiply-accumulate ------------------------------------------------------
+// Digits of one coefficient into int16 planes dig[k * stride] (LSB first,
+// gadget.rs:168-190), from the gadget scale of gadget.rs:115-151.
+template <int NL>
+__device__ __forceinline__ void write_digits(double a0, double a1, const RingDev &r, const Gadget &g,
+                                             const DigitPlan &dp, int16_t *dig, size_t stride) {
+    const SMag v = gadget_scale<NL>(a0, a1, r, g);
+    int64_t lo, hi;
+    digit_split(v, dp, lo, hi);
+    for (int k = 0; k < g.L; k++) {
+        int32_t d;
+        if (k < dp.h) {
+            d = digit_i64(lo, dp.base, dp.inv_base);
+            if (k == dp.h - 1) hi += lo;
+        } else {
+            d = digit_i64(hi, dp.base, dp.inv_base);
+        }
+        dig[(size_t)k * stride] = (int16_t)d;
+    }
+}
 
-// RLevEval::gadget_product (rlev.rs:259-283) without the final inverse:
-// acc = sum_k NTT(digit_k) (.) key[L-1-k] for mask and body. One workgroup
-// handles kSlots coefficient slots of one item (a tile); the digit of every
-// slot is extracted in registers, LSB first, one level per NTT round.
+// ---- LWE -> RLWE cascade (conv.rs:62-95, cascade.rs:388-405) -----------------------
+
+// Digits of the LWE masks, the rank-n degree-1 input of the first conv step.
+template <int NL>
+__global__ __launch_bounds__(256) void k_lwe_digits(RingDev r1, Gadget g, DigitPlan dp, const double *lwe, int n,
+                                                    int16_t *dig) {
+    const int item = blockIdx.y;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double *x = lwe + (size_t)item * NL * (n + 1);
+    const double a0 = x[c], a1 = NL == 2 ? x[(n + 1) + c] : 0.0;
+    write_digits<NL>(a0, a1, r1, g, dp, dig + (size_t)item * g.L * n + c, (size_t)n);
+}
+
+void launch_lwe_digits(const RingDev &r1, const Gadget &g, const DigitPlan &dp, const double *lwe, int items,
+                       int16_t *dig, hipStream_t st) {
+    dim3 grid((r1.n + 255) / 256, items);
+    if (r1.nl == 2)
+        hipLaunchKernelGGL(k_lwe_digits<2>, grid, dim3(256), 0, st, r1, g, dp, lwe, r1.n, dig);
+    else
+        hipLaunchKernelGGL(k_lwe_digits<1>, grid, dim3(256), 0, st, r1, g, dp, lwe, r1.n, dig);
+}
+
+// Gadget-product multiply-accumulate over a digit range (RLevEval::
+// gadget_product, rlev.rs:259-283, without the inverse): workgroup (item,
+// group, tile) adds NTT(digit_k) (.) key[L-1-k] for k in its group. Inputs
+// that are embedded (x -> x^2) transform n/2 coefficients.
 template <int NL>
 __global__ __launch_bounds__(kThreads) void k_gadget_mac(RingDev r, Gadget g, MacArgs a) {
     __shared__ double s[NL * kSlots];
-    const int tile = blockIdx.x, item = blockIdx.y;
+    const int item = blockIdx.x, grp = blockIdx.y, tile = blockIdx.z;
     const int S = kSlots;
-    SMag v[kPer];
-    const double *in = a.in + (size_t)item * a.in_item_stride;
-#pragma unroll
-    for (int e = 0; e < kPer; e++) {
-        const int sg = threadIdx.x + e * kThreads;
-        double a0 = 0.0, a1 = 0.0;
-        bool zero = false;
-        size_t idx;
-        if (a.embed) {
-            zero = (sg & 1) != 0;
-            idx = (size_t)tile * (S / 2) + (sg >> 1);
-        } else {
-            idx = (size_t)tile * S + sg;
-        }
-        if (!zero) {
-            a0 = in[idx];
-            if (NL == 2) a1 = in[a.in_limb_stride + idx];
-        }
-        v[e] = gadget_scale<NL>(a0, a1, r, g);
-    }
+    const int cin = a.embed ? S / 2 : S;
+    const int nt = a.embed ? r.n / 2 : r.n;
+    const int lognt = a.embed ? r.logn - 1 : r.logn;
+    const NttLimbs T = limbs_fwd(r);
+    const FMod m0 = r.t[0].m, m1 = r.t[1].m;
+    const int k0 = grp * g.L / a.groups, k1 = (grp + 1) * g.L / a.groups;
     double am[NL][kPer], ab[NL][kPer];
 #pragma unroll
     for (int l = 0; l < NL; l++)
 #pragma unroll
         for (int e = 0; e < kPer; e++) am[l][e] = ab[l][e] = 0.0;
-    const double *key = a.key + (size_t)item * a.key_item_stride + (size_t)tile * S;
-    const FMod m0 = r.t[0].m, m1 = r.t[1].m;
-    for (int k = 0; k < g.L; k++) {
-#pragma unroll
-        for (int e = 0; e < kPer; e++) {
-            const int sg = threadIdx.x + e * kThreads;
-            const int32_t d = extract_digit(v[e], g.base);
-            s[sg] = digit_residue(d, m0);
-            if (NL == 2) s[S + sg] = digit_residue(d, m1);
+    const int16_t *dig = a.dig + (size_t)item * g.L * a.dig_coeffs + (size_t)tile * cin;
+    const double *key = a.key[item / a.items_per_key] + (size_t)tile * S;
+    for (int k = k0; k < k1; k++) {
+        const int16_t *dk = dig + (size_t)k * a.dig_coeffs;
+        for (int c = threadIdx.x; c < cin; c += blockDim.x) {
+            const int32_t d = dk[c];
+            s[c] = digit_residue(d, m0);
+            if (NL == 2) s[cin + c] = digit_residue(d, m1);
         }
         __syncthreads();
-        lds_ntt_fwd<NL>(s, S, r);
+        ntt_fwd<NL>(s, cin, nt, lognt, T);
         const int ks = g.L - 1 - k;
 #pragma unroll
         for (int l = 0; l < NL; l++) {
@@ -92,19 +118,22 @@ __global__ __launch_bounds__(kThreads) void k_gadget_mac(RingDev r, Gadget g, Ma
 #pragma unroll
             for (int e = 0; e < kPer; e++) {
                 const int sg = threadIdx.x + e * kThreads;
-                const double x = s[l * S + sg];
-                am[l][e] = mod_add(am[l][e], mod_mul(x, km[sg], md), md);
-                ab[l][e] = mod_add(ab[l][e], mod_mul(x, kb[sg], md), md);
+                const double x = s[l * cin + (a.embed ? (sg >> 1) : sg)];
+                am[l][e] += mod_mul_lazy(x, km[sg], md);
+                ab[l][e] += mod_mul_lazy(x, kb[sg], md);
             }
         }
         __syncthreads();
     }
-    const size_t ot = (size_t)item * a.tiles + tile;
+    const size_t ot = ((size_t)grp * a.items + item) * a.tiles + tile;
 #pragma unroll
     for (int l = 0; l < NL; l++)
 #pragma unroll
         for (int e = 0; e < kPer; e++) {
             const int sg = threadIdx.x + e * kThreads;
+            const FMod md = l ? m1 : m0;
+            am[l][e] = mod_reduce_small(am[l][e], md);
+            ab[l][e] = mod_reduce_small(ab[l][e], md);
             a.out_mask[(ot * NL + l) * S + sg] = am[l][e];
             if (!a.reduce_body) a.out_body[(ot * NL + l) * S + sg] = ab[l][e];
             else s[l * S + sg] = ab[l][e];
@@ -129,82 +158,108 @@ __global__ __launch_bounds__(kThreads) void k_gadget_mac(RingDev r, Gadget g, Ma
 }
 
 void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t st) {
-    dim3 grid(a.tiles, a.items);
+    dim3 grid(a.items, a.groups, a.tiles);
     if (r.nl == 2)
         hipLaunchKernelGGL(k_gadget_mac<2>, grid, dim3(kThreads), 0, st, r, g, a);
     else
         hipLaunchKernelGGL(k_gadget_mac<1>, grid, dim3(kThreads), 0, st, r, g, a);
 }
 
-// ---- cascade combine (conv.rs:62-95) --------------------------------------------
-
+// Cascade combine: masks_next = -iNTT(sum of mask partials), body_next =
+// embed(body_prev) - iNTT(sum of body partials); then the digits of
+// masks_next for the next gadget product. Workgroup (item, chunk) handles the
+// chunk's mask polynomials (chunks of at least one degree-n polynomial);
+// chunk 0 also handles the body.
 template <int NL>
-__global__ __launch_bounds__(kThreads) void k_cascade_combine(RingDev r, CombineArgs a) {
+__global__ __launch_bounds__(kThreads) void k_cascade_combine(RingDev r, Gadget g, DigitPlan dp, CombineArgs a) {
     __shared__ double s[NL * kSlots];
-    const int item = blockIdx.x;
+    const int item = blockIdx.x, chunk = blockIdx.y, chunks = gridDim.y;
     const int S = kSlots, n = r.n;
+    const int CS = S / chunks;  // slots of this chunk (a multiple of n)
+    const int c0 = chunk * CS;
     const FMod m0 = r.t[0].m, m1 = r.t[1].m;
-    // masks: -iNTT(pm[tile 0] + pm[tile 1])
-    const double *pm0 = a.pm + (size_t)item * 2 * NL * S;
-    const double *pm1 = pm0 + (size_t)NL * S;
-    for (int t = threadIdx.x; t < NL * S; t += blockDim.x) {
-        const FMod md = (t >= S) ? m1 : m0;
-        s[t] = mod_add(pm0[t], pm1[t], md);
-    }
-    __syncthreads();
-    lds_ntt_inv_noscale<NL>(s, S, r);
-    for (int t = threadIdx.x; t < NL * S; t += blockDim.x) {
-        const int l = t >= S ? 1 : 0;
+    const NttLimbs T = limbs_inv(r);
+    const size_t gstride = (size_t)a.items * 2 * NL * S;
+    for (int t = threadIdx.x; t < NL * CS; t += blockDim.x) {
+        const int l = (NL == 2 && t >= CS) ? 1 : 0;
+        const int c = t - l * CS;
         const FMod md = l ? m1 : m0;
-        a.masks_next[(size_t)item * NL * S + t] = mod_neg(mod_mul(s[t], l ? r.t[1].n_inv : r.t[0].n_inv, md), md);
+        const double *p = a.pm + (size_t)item * 2 * NL * S + (size_t)l * S + c0 + c;
+        double acc = mod_add(p[0], p[(size_t)NL * S], md);
+        for (int gi = 1; gi < a.groups; gi++)
+            acc = mod_add(acc, mod_add(p[gi * gstride], p[gi * gstride + (size_t)NL * S], md), md);
+        s[t] = acc;
     }
     __syncthreads();
-    // body: embed(body_prev) - iNTT(pb[tile 0] + pb[tile 1]); the limbs are
-    // packed at stride n so the NL-limb routine runs on one poly per limb.
-    const double *pb0 = a.pb + (size_t)item * 2 * NL * n;
-    const double *pb1 = pb0 + (size_t)NL * n;
+    ntt_inv_noscale<NL>(s, CS, n, r.logn, T);
+    const double ni0 = r.t[0].n_inv, ni1 = r.t[1].n_inv;
+    for (int c = threadIdx.x; c < CS; c += blockDim.x) {
+        const double x0 = mod_neg(mod_mul(s[c], ni0, m0), m0);
+        const double x1 = NL == 2 ? mod_neg(mod_mul(s[CS + c], ni1, m1), m1) : 0.0;
+        double *mo = a.masks_next + (size_t)item * NL * S + c0 + c;
+        mo[0] = x0;
+        if (NL == 2) mo[S] = x1;
+        if (a.dig_next) write_digits<NL>(x0, x1, r, g, dp, a.dig_next + (size_t)item * g.L * S + c0 + c, (size_t)S);
+    }
+    if (chunk != 0) return;
+    __syncthreads();
+    const size_t gstride_b = (size_t)a.items * 2 * NL * n;
     for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
         const FMod md = (t >= n) ? m1 : m0;
-        s[t] = mod_add(pb0[t], pb1[t], md);
+        const double *p = a.pb + (size_t)item * 2 * NL * n + t;
+        double acc = mod_add(p[0], p[(size_t)NL * n], md);
+        for (int gi = 1; gi < a.groups; gi++)
+            acc = mod_add(acc, mod_add(p[gi * gstride_b], p[gi * gstride_b + (size_t)NL * n], md), md);
+        s[t] = acc;
     }
     __syncthreads();
-    lds_ntt_inv_noscale<NL>(s, n, r);
+    ntt_inv_noscale<NL>(s, n, n, r.logn, T);
     for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
         const int l = t >= n ? 1 : 0, c = t - l * n;
         const FMod md = l ? m1 : m0;
-        const double x = mod_mul(s[t], l ? r.t[1].n_inv : r.t[0].n_inv, md);
+        const double x = mod_mul(s[t], l ? ni1 : ni0, md);
         double e = 0.0;
         if ((c & 1) == 0) e = a.body_prev[(size_t)item * a.body_item_stride + (size_t)l * a.body_limb_stride + (c >> 1)];
         a.body_next[((size_t)item * NL + l) * n + c] = mod_sub(e, x, md);
     }
 }
 
-void launch_cascade_combine(const RingDev &r, const CombineArgs &a, hipStream_t st) {
+void launch_cascade_combine(const RingDev &r, const Gadget &g, const DigitPlan &dp, const RingDev &rq,
+                            const CombineArgs &a, hipStream_t st) {
+    (void)rq;
+    int chunks = kSlots / r.n;
+    if (chunks > 8) chunks = 8;
+    dim3 grid(a.items, chunks);
     if (r.nl == 2)
-        hipLaunchKernelGGL(k_cascade_combine<2>, dim3(a.items), dim3(kThreads), 0, st, r, a);
+        hipLaunchKernelGGL(k_cascade_combine<2>, grid, dim3(kThreads), 0, st, r, g, dp, a);
     else
-        hipLaunchKernelGGL(k_cascade_combine<1>, dim3(a.items), dim3(kThreads), 0, st, r, a);
+        hipLaunchKernelGGL(k_cascade_combine<1>, grid, dim3(kThreads), 0, st, r, g, dp, a);
 }
 
 // ---- rlwe_to_rgsw combine (convert.rs:162-180) -----------------------------------
 
 template <int NL>
-__global__ __launch_bounds__(kThreads) void k_r2g_combine(RingDev r, const double *pm, const double *pb,
-                                                          const double *rlwe_masks, const double *rlwe_bodies,
-                                                          double *rgsw, int lq) {
+__global__ __launch_bounds__(kThreads) void k_r2g_combine(RingDev r, const double *pm, const double *pb, int groups,
+                                                          int items, const double *rlwe_masks,
+                                                          const double *rlwe_bodies, double *rgsw, int lq) {
     __shared__ double s[NL * kSlots];
     const int item = blockIdx.x;  // bit * lq + i
     const int bit = item / lq, i = item - bit * lq;
     const int n = r.n;  // == kSlots
     const FMod m0 = r.t[0].m, m1 = r.t[1].m;
     const size_t pw = (size_t)NL * n;
-    double *g = rgsw + (size_t)bit * 4 * lq * pw;
-    double *nsm = g + (size_t)i * 2 * pw;            // neg_s_m[i]
-    double *mm = g + (size_t)(lq + i) * 2 * pw;      // m[i]
+    double *gp = rgsw + (size_t)bit * 4 * lq * pw;
+    double *nsm = gp + (size_t)i * 2 * pw;        // neg_s_m[i]
+    double *mm = gp + (size_t)(lq + i) * 2 * pw;  // m[i]
     const double *rm = rlwe_masks + (size_t)item * pw, *rb = rlwe_bodies + (size_t)item * pw;
     for (int part = 0; part < 2; part++) {
         const double *src = (part ? pb : pm) + (size_t)item * pw;
-        for (int t = threadIdx.x; t < (int)pw; t += blockDim.x) s[t] = src[t];
+        for (int t = threadIdx.x; t < (int)pw; t += blockDim.x) {
+            const FMod md = t >= n ? m1 : m0;
+            double acc = src[t];
+            for (int gi = 1; gi < groups; gi++) acc = mod_add(acc, src[(size_t)gi * items * pw + t], md);
+            s[t] = acc;
+        }
         __syncthreads();
         lds_ntt_inv_noscale<NL>(s, n, r);
         for (int t = threadIdx.x; t < (int)pw; t += blockDim.x) {
@@ -222,13 +277,14 @@ __global__ __launch_bounds__(kThreads) void k_r2g_combine(RingDev r, const doubl
     }
 }
 
-void launch_r2g_combine(const RingDev &r, const double *pm, const double *pb, const double *rlwe_masks,
+void launch_r2g_combine(const RingDev &r, const double *pm, const double *pb, int groups, const double *rlwe_masks,
                         const double *rlwe_bodies, double *rgsw, int bits, int lq, hipStream_t st) {
+    const int items = bits * lq;
     if (r.nl == 2)
-        hipLaunchKernelGGL(k_r2g_combine<2>, dim3(bits * lq), dim3(kThreads), 0, st, r, pm, pb, rlwe_masks,
+        hipLaunchKernelGGL(k_r2g_combine<2>, dim3(items), dim3(kThreads), 0, st, r, pm, pb, groups, items, rlwe_masks,
                            rlwe_bodies, rgsw, lq);
     else
-        hipLaunchKernelGGL(k_r2g_combine<1>, dim3(bits * lq), dim3(kThreads), 0, st, r, pm, pb, rlwe_masks,
+        hipLaunchKernelGGL(k_r2g_combine<1>, dim3(items), dim3(kThreads), 0, st, r, pm, pb, groups, items, rlwe_masks,
                            rlwe_bodies, rgsw, lq);
 }
 
@@ -264,25 +320,31 @@ __device__ __forceinline__ void gate_input(int mode, const double *in, int j, in
 
 // RGSWCiphertext::external_product (rgsw.rs:108-120) over an eval-form RGSW
 // [half][i][part][limb][n], then dmux (mux.rs:129-137), cmux (mux.rs:75-84)
-// or the SlotExtract step of crot. One workgroup per output item, n == kSlots.
-template <int NL>
-__global__ __launch_bounds__(kThreads) void k_gate(RingDev r, Gadget g, int mode, const double *rgsw,
-                                                   const double *in, double *out, int shift) {
+// or the SlotExtract step of crot. One workgroup per output item, n == kSlots;
+// TPB threads with kSlots/TPB slots each (more threads for narrow tree levels).
+template <int NL, int TPB>
+__global__ __launch_bounds__(TPB) void k_gate(RingDev r, Gadget g, GateArgs ga) {
+    constexpr int PER = kSlots / TPB;
     __shared__ double s[NL * kSlots];
-    const int j = blockIdx.x;
+    const int q = blockIdx.x / ga.items_per_query;
+    const int j = blockIdx.x - q * ga.items_per_query;
+    const int mode = ga.mode, shift = ga.shift;
+    const double *rgsw = ga.rgsw + (size_t)q * ga.rgsw_qstride;
+    const double *in = ga.in + (size_t)q * ga.in_qstride;
+    double *out = ga.out + (size_t)q * ga.out_qstride;
     const int n = r.n, S = kSlots;
     const size_t pw = (size_t)NL * n;
     const FMod m0 = r.t[0].m, m1 = r.t[1].m;
-    double am[NL][kPer], ab[NL][kPer];
+    double am[NL][PER], ab[NL][PER];
 #pragma unroll
     for (int l = 0; l < NL; l++)
 #pragma unroll
-        for (int e = 0; e < kPer; e++) am[l][e] = ab[l][e] = 0.0;
+        for (int e = 0; e < PER; e++) am[l][e] = ab[l][e] = 0.0;
     for (int half = 0; half < 2; half++) {  // 0: ct.mask with -s*m, 1: ct.body with m
-        SMag v[kPer];
+        SMag v[PER];
 #pragma unroll
-        for (int e = 0; e < kPer; e++) {
-            const int sg = threadIdx.x + e * kThreads;
+        for (int e = 0; e < PER; e++) {
+            const int sg = threadIdx.x + e * TPB;
             double b0, c0, b1 = 0.0, c1 = 0.0;
             gate_input<NL>(mode, in, j, half, 0, sg, n, shift, m0, b0, c0);
             if (NL == 2) gate_input<NL>(mode, in, j, half, 1, sg, n, shift, m1, b1, c1);
@@ -291,8 +353,8 @@ __global__ __launch_bounds__(kThreads) void k_gate(RingDev r, Gadget g, int mode
         const double *key = rgsw + (size_t)half * g.L * 2 * pw;
         for (int k = 0; k < g.L; k++) {
 #pragma unroll
-            for (int e = 0; e < kPer; e++) {
-                const int sg = threadIdx.x + e * kThreads;
+            for (int e = 0; e < PER; e++) {
+                const int sg = threadIdx.x + e * TPB;
                 const int32_t d = extract_digit(v[e], g.base);
                 s[sg] = digit_residue(d, m0);
                 if (NL == 2) s[S + sg] = digit_residue(d, m1);
@@ -306,11 +368,11 @@ __global__ __launch_bounds__(kThreads) void k_gate(RingDev r, Gadget g, int mode
                 const double *km = key + (size_t)ks * 2 * pw + (size_t)l * n;
                 const double *kb = km + pw;
 #pragma unroll
-                for (int e = 0; e < kPer; e++) {
-                    const int sg = threadIdx.x + e * kThreads;
+                for (int e = 0; e < PER; e++) {
+                    const int sg = threadIdx.x + e * TPB;
                     const double x = s[l * S + sg];
-                    am[l][e] = mod_add(am[l][e], mod_mul(x, km[sg], md), md);
-                    ab[l][e] = mod_add(ab[l][e], mod_mul(x, kb[sg], md), md);
+                    am[l][e] += mod_mul_lazy(x, km[sg], md);
+                    ab[l][e] += mod_mul_lazy(x, kb[sg], md);
                 }
             }
             __syncthreads();
@@ -319,9 +381,12 @@ __global__ __launch_bounds__(kThreads) void k_gate(RingDev r, Gadget g, int mode
     // inverse transforms of the product, then the gate combine
     for (int part = 0; part < 2; part++) {
 #pragma unroll
-        for (int l = 0; l < NL; l++)
+        for (int l = 0; l < NL; l++) {
+            const FMod md = l ? m1 : m0;
 #pragma unroll
-            for (int e = 0; e < kPer; e++) s[l * S + threadIdx.x + e * kThreads] = part ? ab[l][e] : am[l][e];
+            for (int e = 0; e < PER; e++)
+                s[l * S + threadIdx.x + e * TPB] = mod_reduce_small(part ? ab[l][e] : am[l][e], md);
+        }
         __syncthreads();
         lds_ntt_inv_noscale<NL>(s, S, r);
 #pragma unroll
@@ -329,8 +394,8 @@ __global__ __launch_bounds__(kThreads) void k_gate(RingDev r, Gadget g, int mode
             const FMod md = l ? m1 : m0;
             const double ni = l ? r.t[1].n_inv : r.t[0].n_inv;
 #pragma unroll
-            for (int e = 0; e < kPer; e++) {
-                const int sg = threadIdx.x + e * kThreads;
+            for (int e = 0; e < PER; e++) {
+                const int sg = threadIdx.x + e * TPB;
                 const double p = mod_mul(s[l * S + sg], ni, md);
                 double bv, cv;
                 gate_input<NL>(mode, in, j, part, l, sg, n, shift, md, bv, cv);
@@ -347,12 +412,55 @@ __global__ __launch_bounds__(kThreads) void k_gate(RingDev r, Gadget g, int mode
     }
 }
 
-void launch_gate(const RingDev &r, const Gadget &g, int mode, const double *rgsw_eval, const double *in,
-                 double *out, int items, int rot_shift, hipStream_t st) {
-    if (r.nl == 2)
-        hipLaunchKernelGGL(k_gate<2>, dim3(items), dim3(kThreads), 0, st, r, g, mode, rgsw_eval, in, out, rot_shift);
-    else
-        hipLaunchKernelGGL(k_gate<1>, dim3(items), dim3(kThreads), 0, st, r, g, mode, rgsw_eval, in, out, rot_shift);
+void launch_gate(const RingDev &r, const Gadget &g, const GateArgs &a, hipStream_t st) {
+    const int blocks = a.items_per_query * a.queries;
+    const bool wide = blocks < 64;  // narrow levels: shorter per-workgroup latency
+    if (r.nl == 2) {
+        if (wide)
+            hipLaunchKernelGGL((k_gate<2, 1024>), dim3(blocks), dim3(1024), 0, st, r, g, a);
+        else
+            hipLaunchKernelGGL((k_gate<2, 256>), dim3(blocks), dim3(256), 0, st, r, g, a);
+    } else {
+        if (wide)
+            hipLaunchKernelGGL((k_gate<1, 1024>), dim3(blocks), dim3(1024), 0, st, r, g, a);
+        else
+            hipLaunchKernelGGL((k_gate<1, 256>), dim3(blocks), dim3(256), 0, st, r, g, a);
+    }
+}
+
+// First DMux level on trivial(Delta) = (0, Delta): the mask's digits are all
+// zero and the body's digits are the constants d_k of Delta at coefficient 0,
+// so the external product is sum_k d_k * m[L-1-k] in coefficient form.
+__global__ void k_dmux_trivial(RingDev r, const double *rgsw_all, size_t rgsw_qstride, int L, DeltaDigits dd,
+                               double *out_all, size_t out_qstride) {
+    const int n = r.n;
+    const size_t pw = (size_t)2 * n;  // NL = 2
+    const size_t total = 2 * pw;      // mask and body
+    const int q = blockIdx.y;
+    const double *rgsw_coeff = rgsw_all + (size_t)q * rgsw_qstride;
+    double *out = out_all + (size_t)q * out_qstride;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int part = (int)(t / pw);
+        const size_t rem = t - part * pw;
+        const int l = (int)(rem / n), c = (int)(rem - (size_t)l * n);
+        const FMod md = l ? r.t[1].m : r.t[0].m;
+        double acc = 0.0;
+        for (int k = 0; k < L; k++) {
+            const double *m_k = rgsw_coeff + (size_t)(L + (L - 1 - k)) * 2 * pw;  // m half, sample L-1-k
+            acc += mod_mul_lazy(digit_residue(dd.d[k], md), m_k[part * pw + rem], md);
+        }
+        const double p = mod_reduce_small(acc, md);
+        double base = 0.0;
+        if (part == 1 && c == 0) base = l ? dd.delta[1] : dd.delta[0];
+        out[t] = mod_sub(base, p, md);  // out[0] = trivial - p
+        out[total + t] = p;             // out[1] = p
+    }
+}
+
+void launch_dmux_trivial(const RingDev &r, const double *rgsw_coeff, size_t rgsw_qstride, int L, const DeltaDigits &dd,
+                         double *out, size_t out_qstride, int queries, hipStream_t st) {
+    hipLaunchKernelGGL(k_dmux_trivial, dim3(32, queries), dim3(256), 0, st, r, rgsw_coeff, rgsw_qstride, L, dd, out,
+                       out_qstride);
 }
 
 // ---- mod switch q1 -> q2 (mod_switch.rs:57-86) ----------------------------------
@@ -382,60 +490,134 @@ void launch_modswitch_q1_q2(const RingDev &r1, uint64_t q2, const double *in, do
 
 // ---- FirstDim (first_dim.rs:94-184) -------------------------------------------------
 
-constexpr int kFdCols = 4;
-
+// out[chunk][q][j][part][k] = sum over the chunk's rows i of db[i][j][k] *
+// sel[q][i][part][k] for QB queries sharing each database load; each thread
+// owns two adjacent slots k (16-byte loads) of TJ columns. Products are summed
+// lazily (exact below 2^53) and reduced once; chunks of rows give the launch
+// enough workgroups to keep HBM busy.
+template <int QB, int TJ>
 __global__ __launch_bounds__(256) void k_first_dim(const double *db, const double *sel, double *out, int I, int J,
-                                                   int n, FMod m) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j0 = blockIdx.y * kFdCols;
+                                                   int n, int Q, int q0, int rows_per_chunk, FMod m) {
+    const int k = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+    const int j0 = blockIdx.y * TJ;
+    const int chunk = blockIdx.z;
     if (k >= n) return;
-    double am[kFdCols], ab[kFdCols];
+    const int i0 = chunk * rows_per_chunk;
+    const int i1 = min(I, i0 + rows_per_chunk);
+    double am[QB][TJ][2], ab[QB][TJ][2];
 #pragma unroll
-    for (int t = 0; t < kFdCols; t++) am[t] = ab[t] = 0.0;
-    for (int i = 0; i < I; i++) {
-        const double sm = sel[(size_t)(2 * i) * n + k];
-        const double sb = sel[(size_t)(2 * i + 1) * n + k];
+    for (int qb = 0; qb < QB; qb++)
+#pragma unroll
+        for (int t = 0; t < TJ; t++) am[qb][t][0] = am[qb][t][1] = ab[qb][t][0] = ab[qb][t][1] = 0.0;
+    const int ncols = min(TJ, J - j0);
+    const size_t qs = (size_t)I * 2 * n;
+#pragma unroll 2
+    for (int i = i0; i < i1; i++) {
+        double2 sm[QB], sb[QB];
+#pragma unroll
+        for (int qb = 0; qb < QB; qb++) {
+            sm[qb] = *(const double2 *)(sel + (size_t)(q0 + qb) * qs + (size_t)(2 * i) * n + k);
+            sb[qb] = *(const double2 *)(sel + (size_t)(q0 + qb) * qs + (size_t)(2 * i + 1) * n + k);
+        }
         const double *row = db + ((size_t)i * J + j0) * n + k;
 #pragma unroll
-        for (int t = 0; t < kFdCols; t++) {
-            if (j0 + t < J) {
-                const double d = row[(size_t)t * n];
-                am[t] = mod_add(am[t], mod_mul(d, sm, m), m);
-                ab[t] = mod_add(ab[t], mod_mul(d, sb, m), m);
+        for (int t = 0; t < TJ; t++) {
+            if (t < ncols) {
+                const double2 d = *(const double2 *)(row + (size_t)t * n);
+#pragma unroll
+                for (int qb = 0; qb < QB; qb++) {
+                    am[qb][t][0] += mod_mul_lazy(d.x, sm[qb].x, m);
+                    am[qb][t][1] += mod_mul_lazy(d.y, sm[qb].y, m);
+                    ab[qb][t][0] += mod_mul_lazy(d.x, sb[qb].x, m);
+                    ab[qb][t][1] += mod_mul_lazy(d.y, sb[qb].y, m);
+                }
             }
         }
     }
+    double *o = out + (size_t)chunk * Q * J * 2 * n;
 #pragma unroll
-    for (int t = 0; t < kFdCols; t++) {
-        if (j0 + t < J) {
-            out[((size_t)(j0 + t) * 2 + 0) * n + k] = am[t];
-            out[((size_t)(j0 + t) * 2 + 1) * n + k] = ab[t];
+    for (int qb = 0; qb < QB; qb++)
+#pragma unroll
+        for (int t = 0; t < TJ; t++) {
+            if (t < ncols) {
+                double2 vm, vb;
+                vm.x = mod_reduce_small(am[qb][t][0], m);
+                vm.y = mod_reduce_small(am[qb][t][1], m);
+                vb.x = mod_reduce_small(ab[qb][t][0], m);
+                vb.y = mod_reduce_small(ab[qb][t][1], m);
+                double *oq = o + (size_t)(q0 + qb) * J * 2 * n;
+                *(double2 *)(oq + ((size_t)(j0 + t) * 2 + 0) * n + k) = vm;
+                *(double2 *)(oq + ((size_t)(j0 + t) * 2 + 1) * n + k) = vb;
+            }
         }
+}
+
+// Sum of the row-chunk partials into chunk 0.
+__global__ void k_sum_chunks(double *part, size_t words, int chunks, FMod m) {
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < words; t += (size_t)gridDim.x * blockDim.x) {
+        double acc = part[t];
+        for (int c = 1; c < chunks; c++) acc += part[(size_t)c * words + t];
+        part[t] = mod_reduce_small(acc, m);
     }
 }
 
-void launch_first_dim(const double *db, const double *sel_eval, double *out_eval, int I, int J, int n, FMod m,
-                      hipStream_t st) {
-    dim3 grid((n + 255) / 256, (J + kFdCols - 1) / kFdCols);
-    hipLaunchKernelGGL(k_first_dim, grid, dim3(256), 0, st, db, sel_eval, out_eval, I, J, n, m);
+static int fd_cols(int QB) { return QB <= 2 ? 4 : (QB == 4 ? 2 : 1); }
+static int fd_qb(int Q) { return Q >= 8 ? 8 : (Q >= 4 ? 4 : (Q >= 2 ? 2 : 1)); }
+
+int first_dim_chunks(int I, int J, int n, int Q) {
+    // aim for about 4096 wavefronts of 64 threads per database pass
+    const int tj = fd_cols(fd_qb(Q));
+    const long threads = (long)(n / 2) * ((J + tj - 1) / tj);
+    int chunks = 1;
+    while (chunks < I && threads * chunks < 4096L * 64) chunks *= 2;
+    return chunks;
+}
+
+template <int QB>
+static void fd_pass(const double *db, const double *sel, double *out, int I, int J, int n, int Q, int q0, int chunks,
+                    FMod m, hipStream_t st) {
+    constexpr int TJ = QB <= 2 ? 4 : (QB == 4 ? 2 : 1);
+    const int rows = (I + chunks - 1) / chunks;
+    dim3 grid((n / 2 + 255) / 256, (J + TJ - 1) / TJ, chunks);
+    hipLaunchKernelGGL((k_first_dim<QB, TJ>), grid, dim3(256), 0, st, db, sel, out, I, J, n, Q, q0, rows, m);
+}
+
+void launch_first_dim(const double *db, const double *sel_eval, double *out_eval, int I, int J, int n, int Q,
+                      int chunks, FMod m, hipStream_t st) {
+    for (int q0 = 0; q0 < Q;) {
+        const int qb = fd_qb(Q - q0);
+        if (qb == 8) fd_pass<8>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
+        else if (qb == 4) fd_pass<4>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
+        else if (qb == 2) fd_pass<2>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
+        else fd_pass<1>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
+        q0 += qb;
+    }
+    if (chunks > 1) {
+        const size_t words = (size_t)Q * J * 2 * n;
+        unsigned blocks = (unsigned)((words + 255) / 256);
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(k_sum_chunks, dim3(blocks), dim3(256), 0, st, out_eval, words, chunks, m);
+    }
 }
 
 // ---- RespComp (resp_comp.rs:46-76) -----------------------------------------------------
 
 __global__ __launch_bounds__(1024) void k_resp_comp(RespArgs a) {
     extern __shared__ double sm[];
+    const int qy = blockIdx.x;
     double *c3m = sm;           // [n1] mask at q3
     double *buf = sm + a.n1;    // [n2] NTT buffer
     const int n1 = a.n1, n2 = a.n2, d = a.d;
     const int i = threadIdx.x;  // one slot of the n2 ring per thread
     const uint64_t q2 = a.q2, q3 = a.q3, q4 = a.q4;
     const FMod m3 = a.t3.m;
+    const double *ct = a.ct + (size_t)qy * 2 * n1;
     // mod_switch_sym q2 -> q3 at n1
     for (int c = threadIdx.x; c < n1; c += blockDim.x) {
-        const uint64_t x = (uint64_t)a.ct[c];
+        const uint64_t x = (uint64_t)ct[c];
         c3m[c] = (double)(((x * q3 + (q2 >> 1)) / q2) % q3);
     }
-    const uint64_t xb = (uint64_t)a.ct[n1 + d * i];
+    const uint64_t xb = (uint64_t)ct[n1 + d * i];
     const double body_proj = (double)(((xb * q3 + (q2 >> 1)) / q2) % q3);
     __syncthreads();
     RingDev r;
@@ -458,7 +640,7 @@ __global__ __launch_bounds__(1024) void k_resp_comp(RespArgs a) {
         SMag v;
         v.mag = (via_u128)((mag * bl + (q3 >> 1)) / q3);
         v.neg = neg && v.mag != 0;
-        const double *key = a.rsk_eval + (size_t)j * kw;
+        const double *key = a.rsk_eval[qy] + (size_t)j * kw;
         for (int k = 0; k < a.l_rsk; k++) {
             const int32_t dg = extract_digit(v, a.b_rsk);
             buf[i] = digit_residue(dg, m3);
@@ -482,13 +664,13 @@ __global__ __launch_bounds__(1024) void k_resp_comp(RespArgs a) {
     const uint64_t mask = (uint64_t)mod_neg(res[0], m3);
     const uint64_t body = (uint64_t)mod_sub(body_proj, res[1], m3);
     // mod_switch_asym (mod_switch.rs:124-154): mask q3 -> q3, body q3 -> q4
-    a.out_mask[i] = ((mask * q3 + (q3 >> 1)) / q3) % q3;
-    a.out_body[i] = ((body * q4 + (q3 >> 1)) / q3) % q4;
+    a.out_mask[qy][i] = ((mask * q3 + (q3 >> 1)) / q3) % q3;
+    a.out_body[qy][i] = ((body * q4 + (q3 >> 1)) / q3) % q4;
 }
 
 void launch_resp_comp(const RespArgs &a, hipStream_t st) {
     const size_t shmem = (size_t)(a.n1 + a.n2) * sizeof(double);
-    hipLaunchKernelGGL(k_resp_comp, dim3(1), dim3(a.n2), shmem, st, a);
+    hipLaunchKernelGGL(k_resp_comp, dim3(a.queries), dim3(a.n2), shmem, st, a);
 }
 
 // ---- setup: key relayout and DB packing ----------------------------------------------

@@ -61,6 +61,17 @@ VIA_HD double mod_mul(double a, double b, const FMod &m) {
     return r;
 }
 
+// Product a*b less a multiple of q, left in (-q, 2q) without the final
+// corrections: an exact integer. Sums of up to 2^16 such terms stay exact in
+// fp64 (|sum| < 2^53 for q < 2^36, or 2^14 terms for q < 2^38) and are
+// brought into [0, q) once with mod_reduce_small.
+VIA_HD double mod_mul_lazy(double a, double b, const FMod &m) {
+    double h = a * b;
+    double l = fma(a, b, -h);
+    double t = floor(h * m.qinv);
+    return fma(-t, m.q, h) + l;
+}
+
 // Reduce an exact integer value v (|v| < 2^52) into [0, q).
 VIA_HD double mod_reduce_small(double v, const FMod &m) {
     double t = floor(v * m.qinv);
@@ -235,4 +246,83 @@ VIA_HD int32_t extract_digit(SMag &v, uint32_t B) {
     }
     if (v.mag == 0) v.neg = 0;
     return d;
+}
+
+// ---- digits in int64 lanes -------------------------------------------------------
+//
+// The scaled value |val| <= B^L/2 + 1 is split as val = hi*B^h + lo with
+// B^h < 2^50, so both halves fit exact fp64/int64 arithmetic. Extracting the
+// first h digits from lo leaves a carry in {-1, 0, 1} that joins hi, and the
+// remaining L-h digits come from hi + carry: the same sequence as running
+// the reference's rem_euclid / rebalance on the full value.
+
+struct DigitPlan {
+    uint32_t base;
+    int L;
+    int h;            // digits taken from the low part
+    uint64_t bh;      // B^h
+    double inv_base;  // 1/B
+    double inv_bh;    // 1/B^h
+};
+
+VIA_HD DigitPlan digit_plan(uint32_t base, int L) {
+    DigitPlan p;
+    p.base = base;
+    p.L = L;
+    p.h = 0;
+    uint64_t bh = 1;
+    while (p.h < L && (double)bh * base < 1125899906842624.0) {  // 2^50
+        bh *= base;
+        p.h++;
+    }
+    p.bh = bh;
+    p.inv_base = 1.0 / (double)base;
+    p.inv_bh = 1.0 / (double)bh;
+    return p;
+}
+
+// One digit of an int64 value |x| < 2^52: Euclidean remainder, rebalance
+// above B/2, then x <- (x - d) / B.
+VIA_HD int32_t digit_i64(int64_t &x, uint32_t base, double inv_base) {
+    const int64_t B = (int64_t)base;
+    int64_t q = (int64_t)floor((double)x * inv_base);
+    int64_t r = x - q * B;
+    if (r < 0) {
+        q -= 1;
+        r += B;
+    } else if (r >= B) {
+        q += 1;
+        r -= B;
+    }
+    if (r > (int64_t)(base >> 1)) {
+        r -= B;
+        q += 1;
+    }
+    x = q;
+    return (int32_t)r;
+}
+
+// Splits the scaled magnitude into signed low/high int64 parts.
+VIA_HD void digit_split(const SMag &v, const DigitPlan &p, int64_t &lo, int64_t &hi) {
+    via_u128 m = v.mag;
+    uint64_t qh;
+    if ((m >> 64) == 0 && (uint64_t)m < p.bh) {
+        qh = 0;
+    } else {
+        double qf = floor(u128_to_f64(m) * p.inv_bh);
+        via_i128 q = (via_i128)(int64_t)qf;
+        via_i128 r = (via_i128)m - q * (via_i128)p.bh;
+        while (r < 0) {
+            q -= 1;
+            r += (via_i128)p.bh;
+        }
+        while (r >= (via_i128)p.bh) {
+            q += 1;
+            r -= (via_i128)p.bh;
+        }
+        qh = (uint64_t)q;
+    }
+    const int64_t l = (int64_t)((uint64_t)m - qh * p.bh);
+    lo = v.neg ? -l : l;
+    hi = v.neg ? -(int64_t)qh : (int64_t)qh;
 }

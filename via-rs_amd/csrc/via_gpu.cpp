@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -97,12 +98,15 @@ struct DevTab {
 }  // namespace
 
 struct Workspace {
-    size_t B = 0, I = 0, J = 0, bits = 0;
+    size_t Q = 0, B = 0, I = 0, J = 0, bits = 0;
     double *lwe_f = nullptr, *masks[2] = {nullptr, nullptr}, *body[2] = {nullptr, nullptr};
     double *pm = nullptr, *pb = nullptr, *rgsw = nullptr, *rgsw_eval = nullptr;
     double *dm[2] = {nullptr, nullptr}, *sw = nullptr, *sel = nullptr, *fd = nullptr;
     double *cm[2] = {nullptr, nullptr}, *rgsw2 = nullptr, *res[2] = {nullptr, nullptr};
     uint64_t *lwe_u = nullptr, *ans_u = nullptr;
+    int16_t *dig = nullptr;
+    int groups = 1;
+    int fd_chunks = 1;
     std::vector<void *> all;
 };
 
@@ -118,7 +122,6 @@ struct via_ctx {
     // pointers to the last answer's stage buffers
     const double *stage_ptr[7] = {};
     size_t stage_words[7] = {};
-    int stage_nl[7] = {};
 };
 
 struct via_db {
@@ -239,6 +242,7 @@ via_status check_params(const via_params *pp) {
         pp->ck_base >= (1u << 20) || pp->b_rsk < 2 || pp->b_rsk >= (1u << 20))
         return VIA_E_INVALID;
     if (pp->l_query < 1 || pp->l_ck < 1 || pp->l_rsk < 1) return VIA_E_INVALID;
+    if (pp->ck_base >= (1u << 15)) return VIA_E_INVALID;  // int16 digit planes
     if ((u128)pp->q2 * pp->q3 >> 63) return VIA_E_INVALID;  // q2 -> q3 switch in 64 bits
     if ((u128)pp->q3 * pp->q4 >> 63) return VIA_E_INVALID;
     return VIA_OK;
@@ -256,40 +260,49 @@ void free_ws(Workspace &w) {
     w = Workspace();
 }
 
-via_status ensure_ws(via_ctx *c, const via_params *pp, size_t B, size_t bits) {
+via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_t bits) {
     Workspace &w = c->ws;
     size_t I = pp->num_rows, J = pp->num_cols;
-    if (w.B >= B && w.I >= I && w.J >= J && w.bits >= bits && w.lwe_f) return VIA_OK;
+    if (w.Q >= Q && w.B >= B && w.I >= I && w.J >= J && w.bits >= bits && w.lwe_f) return VIA_OK;
+    Q = std::max(Q, w.Q);
     free_ws(w);
     const size_t n = pp->n1, nl = 2, S = kSlots, lq = pp->l_query;
+    w.Q = Q;
     w.B = B;
     w.I = I;
     w.J = J;
     w.bits = bits;
+    const size_t items = Q * B;
+    // digit groups per gadget product: enough workgroups to cover the chip
+    int groups = 1;
+    while (groups < (int)pp->l_ck && items * 2 * groups < 512) groups++;
+    w.groups = groups;
+    w.fd_chunks = first_dim_chunks((int)I, (int)J, (int)n, (int)Q);
     via_status s;
 #define A(p, cnt) \
     if ((s = dalloc(w, &(p), (cnt)))) return s
-    A(w.lwe_u, B * nl * (n + 1));
-    A(w.lwe_f, B * nl * (n + 1));
-    A(w.masks[0], B * nl * S);
-    A(w.masks[1], B * nl * S);
-    A(w.body[0], B * nl * S);
-    A(w.body[1], B * nl * S);
-    A(w.pm, B * 2 * nl * S);
-    A(w.pb, B * 2 * nl * S);
-    A(w.rgsw, bits * 4 * lq * nl * n);
-    A(w.rgsw_eval, bits * 4 * lq * nl * n);
-    A(w.dm[0], I * 2 * nl * n);
-    A(w.dm[1], I * 2 * nl * n);
-    A(w.sw, I * 2 * n);
-    A(w.sel, I * 2 * n);
-    A(w.fd, J * 2 * n);
-    A(w.cm[0], J * 2 * n);
-    A(w.cm[1], J * 2 * n);
-    A(w.rgsw2, bits * 4 * lq * n);
-    A(w.res[0], 2 * n);
-    A(w.res[1], 2 * n);
-    A(w.ans_u, 2 * 1024);
+    A(w.lwe_u, items * nl * (n + 1));
+    A(w.lwe_f, items * nl * (n + 1));
+    A(w.masks[0], items * nl * S);
+    A(w.masks[1], items * nl * S);
+    A(w.body[0], items * nl * S);
+    A(w.body[1], items * nl * S);
+    A(w.dig, items * pp->l_ck * S);
+    A(w.pm, (size_t)groups * items * 2 * nl * S);
+    A(w.pb, (size_t)groups * items * 2 * nl * S);
+    A(w.rgsw, Q * bits * 4 * lq * nl * n);
+    A(w.rgsw_eval, Q * bits * 4 * lq * nl * n);
+    A(w.dm[0], Q * I * 2 * nl * n);
+    A(w.dm[1], Q * I * 2 * nl * n);
+    A(w.sw, Q * I * 2 * n);
+    A(w.sel, Q * I * 2 * n);
+    A(w.fd, (size_t)w.fd_chunks * Q * J * 2 * n);
+    A(w.cm[0], Q * J * 2 * n);
+    A(w.cm[1], Q * J * 2 * n);
+    A(w.rgsw2, Q * bits * 4 * lq * n);
+    A(w.res[0], Q * 2 * n);
+    A(w.res[1], Q * 2 * n);
+    A(w.ans_u, Q * 2 * 1024);
 #undef A
     return VIA_OK;
 }
@@ -546,14 +559,24 @@ void via_keys_destroy(via_keys *k) {
 
 // ---- answer -------------------------------------------------------------------------------
 
-static via_status answer_impl(via_ctx *c, const via_db *db, const via_keys *keys, const uint64_t *d_lwes,
-                              size_t n_lwe, uint64_t *d_mask, uint64_t *d_body, via_err_detail *err) {
-    const via_params *pp = &keys->pp;
+// Answers Q queries against one database in one pass over the pipeline:
+// answer_through_crot (answer.rs:83-228) then resp_comp (resp_comp.rs:46-76)
+// for each, with every step batched across the queries; query q uses the
+// keys of its own client.
+static via_status answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via_keys *const *keys,
+                                    const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_mask,
+                                    uint64_t *const *d_body, via_err_detail *err) {
+    if (Q < 1 || Q > kMaxQ) return VIA_E_INVALID;
+    const via_params *pp = &keys[0]->pp;
     const size_t I = pp->num_rows, J = pp->num_cols, n = pp->n1, nl = 2, S = kSlots;
     // guards of answer_through_crot (answer.rs:131-157)
     if (!is_pow2(I) || !is_pow2(J)) return VIA_E_DIM;
+    for (int q = 0; q < Q; q++) {
+        const via_params *kp = &keys[q]->pp;
+        if (memcmp(kp, pp, sizeof *pp) != 0) return VIA_E_DIM;
+    }
     if (db->pp.num_rows != pp->num_rows || db->pp.num_cols != pp->num_cols || db->pp.n1 != pp->n1 ||
-        db->pp.q2 != pp->q2)
+        db->pp.q2 != pp->q2 || db->pp.n_rec != pp->n_rec)
         return VIA_E_DIM;
     const int nd = ilog2(I), nc = ilog2(J), nr = ilog2(pp->n1 / pp->n_rec);
     const size_t bits = (size_t)(nd + nc + nr), lq = pp->l_query;
@@ -566,11 +589,12 @@ static via_status answer_impl(via_ctx *c, const via_db *db, const via_keys *keys
         }
         return VIA_E_QUERY_LEN;
     }
-    via_status s = ensure_ws(c, pp, n_lwe, bits);
+    via_status s = ensure_ws(c, pp, (size_t)Q, n_lwe, bits);
     if (s) return s;
     Workspace &w = c->ws;
     hipStream_t st = c->stream;
-    const size_t B = n_lwe;
+    const size_t B = n_lwe, items = (size_t)Q * B;
+    const size_t lw = nl * (n + 1);
 
     RingDev r1n;  // q1 at n1
     if ((s = make_ring(c, (uint32_t)n, pp->q1, 2, &r1n))) return s;
@@ -582,8 +606,10 @@ static via_status answer_impl(via_ctx *c, const via_db *db, const via_keys *keys
 
     HIP_TRY(hipEventRecord(c->ev[0], st));
     // ---- step 1: query_decomp (query_decomp.rs:39-95) ----
-    launch_u64_to_f64(d_lwes, w.lwe_f, B * nl * (n + 1), st);
+    for (int q = 0; q < Q; q++) launch_u64_to_f64(d_lwes[q], w.lwe_f + (size_t)q * B * lw, B * lw, st);
     const Gadget gck = make_gadget(pp->ck_base, pp->l_ck);
+    const DigitPlan dpk = digit_plan((uint32_t)pp->ck_base, (int)pp->l_ck);
+    launch_lwe_digits(r1n, gck, dpk, w.lwe_f, (int)items, w.dig, st);
     int cur = 0;
     for (uint32_t nout = 2, step = 0; nout <= n; nout <<= 1, step++) {
         RingDev r;
@@ -591,134 +617,209 @@ static via_status answer_impl(via_ctx *c, const via_db *db, const via_keys *keys
         const uint32_t rank_in = (uint32_t)(2 * n / nout);
         const bool first = (nout == 2);
         MacArgs a;
-        a.in = first ? w.lwe_f : w.masks[cur];
-        a.in_item_stride = first ? nl * (n + 1) : nl * S;
-        a.in_limb_stride = first ? (n + 1) : S;
+        memset(&a, 0, sizeof a);
+        a.dig = w.dig;
+        a.dig_coeffs = S;
         a.embed = 1;
-        a.key = keys->cascade[step];
-        a.key_item_stride = 0;
+        for (int q = 0; q < Q; q++) a.key[q] = keys[q]->cascade[step];
+        a.items_per_key = (int)B;
         a.key_slots = (size_t)rank_in * nout;
         a.tiles = 2;
+        a.groups = w.groups;
         a.out_mask = w.pm;
         a.out_body = w.pb;
         a.reduce_body = 1;
-        a.items = (int)B;
+        a.items = (int)items;
         launch_gadget_mac(r, gck, a, st);
         CombineArgs cb;
         cb.pm = w.pm;
         cb.pb = w.pb;
+        cb.groups = w.groups;
+        cb.items = (int)items;
         const uint32_t nin = nout / 2;
         cb.body_prev = first ? w.lwe_f + n : w.body[cur];
-        cb.body_item_stride = first ? nl * (n + 1) : nl * nin;
+        cb.body_item_stride = first ? lw : nl * nin;
         cb.body_limb_stride = first ? (n + 1) : nin;
         cb.masks_next = w.masks[cur ^ 1];
         cb.body_next = w.body[cur ^ 1];
-        cb.items = (int)B;
-        launch_cascade_combine(r, cb, st);
+        cb.dig_next = w.dig;
+        launch_cascade_combine(r, gck, dpk, r, cb, st);
         cur ^= 1;
     }
-    // rlwe_to_rgsw_eval (convert.rs:162-180) with the conversion key
+    // rlwe_to_rgsw_eval (convert.rs:162-180) with each client's conversion
+    // key; the last combine left the digits of the RLWE masks in w.dig
     {
         MacArgs a;
-        a.in = w.masks[cur];
-        a.in_item_stride = nl * n;
-        a.in_limb_stride = n;
+        memset(&a, 0, sizeof a);
+        a.dig = w.dig;
+        a.dig_coeffs = S;
         a.embed = 0;
-        a.key = keys->conv;
-        a.key_item_stride = 0;
+        for (int q = 0; q < Q; q++) a.key[q] = keys[q]->conv;
+        a.items_per_key = (int)B;
         a.key_slots = n;
         a.tiles = 1;
+        a.groups = w.groups;
         a.out_mask = w.pm;
         a.out_body = w.pb;
         a.reduce_body = 0;
-        a.items = (int)B;
+        a.items = (int)items;
         launch_gadget_mac(r1n, gck, a, st);
-        launch_r2g_combine(r1n, w.pm, w.pb, w.masks[cur], w.body[cur], w.rgsw, (int)bits, (int)lq, st);
+        launch_r2g_combine(r1n, w.pm, w.pb, w.groups, w.masks[cur], w.body[cur], w.rgsw, (int)(Q * bits), (int)lq,
+                           st);
     }
+    const size_t rgw = 4 * lq * nl * n;  // words per RGSW at q1
     c->stage_ptr[0] = w.rgsw;
-    c->stage_words[0] = bits * 4 * lq * nl * n;
-    c->stage_nl[0] = 2;
+    c->stage_words[0] = bits * rgw;
     HIP_TRY(hipEventRecord(c->ev[1], st));
 
     // ---- step 2: dmux_tree from trivial(Delta) (answer.rs:181-191, mux.rs:222-251) ----
     const Gadget g1 = make_gadget(pp->b1, pp->l_query);
-    const size_t rgw = 4 * lq * nl * n;  // words per RGSW at q1
-    HIP_TRY(hipMemcpyAsync(w.rgsw_eval, w.rgsw, sizeof(double) * nd * rgw, hipMemcpyDeviceToDevice, st));
-    launch_ntt(r1n, w.rgsw_eval, (size_t)nd * 4 * lq * nl, 1, false, st);
-    {
-        u128 Q1 = (u128)pp->q1[0] * pp->q1[1];
-        u128 delta = (Q1 + pp->p - 1) / pp->p;
-        std::vector<double> triv(2 * nl * n, 0.0);
-        triv[nl * n + 0] = (double)(uint64_t)(delta % pp->q1[0]);
-        triv[nl * n + n] = (double)(uint64_t)(delta % pp->q1[1]);
-        HIP_TRY(hipMemcpyAsync(w.dm[0], triv.data(), sizeof(double) * triv.size(), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipStreamSynchronize(st));  // triv is a host temporary
+    const size_t ctw1 = 2 * nl * n;  // words per ciphertext at q1
+    if (nd > 1) {
+        // eval form of the DMux bits 1..nd-1 of every query
+        for (int q = 0; q < Q; q++)
+            HIP_TRY(hipMemcpyAsync(w.rgsw_eval + ((size_t)q * bits + 1) * rgw, w.rgsw + ((size_t)q * bits + 1) * rgw,
+                                   sizeof(double) * (nd - 1) * rgw, hipMemcpyDeviceToDevice, st));
+        for (int q = 0; q < Q; q++)
+            launch_ntt(r1n, w.rgsw_eval + ((size_t)q * bits + 1) * rgw, (size_t)(nd - 1) * 4 * lq * nl, 1, false, st);
     }
     int dc = 0;
-    for (int i = 0; i < nd; i++) {
-        launch_gate(r1n, g1, GATE_DMUX, w.rgsw_eval + (size_t)i * rgw, w.dm[dc], w.dm[dc ^ 1], 1 << i, 0, st);
+    const u128 Q1 = (u128)pp->q1[0] * pp->q1[1];
+    const u128 delta = (Q1 + pp->p - 1) / pp->p;
+    if (nd > 0) {
+        // level 0 acts on trivial(Delta): digits of Delta are constants
+        DeltaDigits dd;
+        memset(&dd, 0, sizeof dd);
+        if (pp->l_query > 8) return VIA_E_INVALID;
+        SMag v;
+        v.neg = delta > (Q1 >> 1);
+        const u128 mag = v.neg ? Q1 - delta : delta;
+        v.mag = round_mul_div(mag, g1.bl, widediv_make(Q1));
+        if (v.mag == 0) v.neg = 0;
+        for (uint32_t k = 0; k < pp->l_query; k++) dd.d[k] = extract_digit(v, g1.base);
+        dd.delta[0] = (double)(uint64_t)(delta % pp->q1[0]);
+        dd.delta[1] = (double)(uint64_t)(delta % pp->q1[1]);
+        launch_dmux_trivial(r1n, w.rgsw, bits * rgw, (int)pp->l_query, dd, w.dm[1], I * ctw1, Q, st);
+        dc = 1;
+    } else {
+        std::vector<double> triv(ctw1, 0.0);
+        triv[nl * n + 0] = (double)(uint64_t)(delta % pp->q1[0]);
+        triv[nl * n + n] = (double)(uint64_t)(delta % pp->q1[1]);
+        for (int q = 0; q < Q; q++)
+            HIP_TRY(hipMemcpyAsync(w.dm[0] + (size_t)q * I * ctw1, triv.data(), sizeof(double) * ctw1,
+                                   hipMemcpyHostToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));  // triv is a host temporary
+    }
+    for (int i = 1; i < nd; i++) {
+        GateArgs ga;
+        ga.mode = GATE_DMUX;
+        ga.rgsw = w.rgsw_eval + (size_t)i * rgw;
+        ga.rgsw_qstride = bits * rgw;
+        ga.in = w.dm[dc];
+        ga.in_qstride = I * ctw1;
+        ga.out = w.dm[dc ^ 1];
+        ga.out_qstride = I * ctw1;
+        ga.items_per_query = 1 << i;
+        ga.queries = Q;
+        ga.shift = 0;
+        launch_gate(r1n, g1, ga, st);
         dc ^= 1;
     }
     c->stage_ptr[1] = w.dm[dc];
-    c->stage_words[1] = I * 2 * nl * n;
-    c->stage_nl[1] = 2;
+    c->stage_words[1] = I * ctw1;
     HIP_TRY(hipEventRecord(c->ev[2], st));
 
     // ---- step 3: mod_switch_sym q1 -> q2 per row ----
-    launch_modswitch_q1_q2(r1n, pp->q2, w.dm[dc], w.sw, I * 2, st);
+    launch_modswitch_q1_q2(r1n, pp->q2, w.dm[dc], w.sw, (size_t)Q * I * 2, st);
     c->stage_ptr[2] = w.sw;
     c->stage_words[2] = I * 2 * n;
-    c->stage_nl[2] = 1;
     HIP_TRY(hipEventRecord(c->ev[3], st));
 
-    // ---- step 4: first_dim (first_dim.rs:94-184) ----
-    HIP_TRY(hipMemcpyAsync(w.sel, w.sw, sizeof(double) * I * 2 * n, hipMemcpyDeviceToDevice, st));
-    launch_ntt(r2, w.sel, I * 2, 1, false, st);
+    // ---- step 4: first_dim (first_dim.rs:94-184), one database pass for all queries ----
+    HIP_TRY(hipMemcpyAsync(w.sel, w.sw, sizeof(double) * Q * I * 2 * n, hipMemcpyDeviceToDevice, st));
+    launch_ntt(r2, w.sel, (size_t)Q * I * 2, 1, false, st);
+    const int chunks = first_dim_chunks((int)I, (int)J, (int)n, Q);
     HIP_TRY(hipEventRecord(c->ev_fd[0], st));
-    launch_first_dim(db->cells, w.sel, w.fd, (int)I, (int)J, (int)n, fmod_make(pp->q2), st);
+    launch_first_dim(db->cells, w.sel, w.fd, (int)I, (int)J, (int)n, Q, chunks, fmod_make(pp->q2), st);
     HIP_TRY(hipEventRecord(c->ev_fd[1], st));
-    launch_ntt(r2, w.fd, J * 2, 1, true, st);
+    launch_ntt(r2, w.fd, (size_t)Q * J * 2, 1, true, st);
     c->stage_ptr[3] = w.fd;
     c->stage_words[3] = J * 2 * n;
-    c->stage_nl[3] = 1;
     HIP_TRY(hipEventRecord(c->ev[4], st));
 
     // ---- step 5: cmux_tree over mod-switched RGSWs (answer.rs:194-203) ----
     const Gadget g2 = make_gadget(pp->b2, pp->l_query);
     const size_t rgw2 = 4 * lq * n;
+    const size_t ctw2 = 2 * n;
     if (nc + nr > 0) {
-        launch_modswitch_q1_q2(r1n, pp->q2, w.rgsw + (size_t)nd * rgw, w.rgsw2, (size_t)(nc + nr) * 4 * lq, st);
-        launch_ntt(r2, w.rgsw2, (size_t)(nc + nr) * 4 * lq, 1, false, st);
+        for (int q = 0; q < Q; q++)
+            launch_modswitch_q1_q2(r1n, pp->q2, w.rgsw + ((size_t)q * bits + nd) * rgw,
+                                   w.rgsw2 + (size_t)q * bits * rgw2, (size_t)(nc + nr) * 4 * lq, st);
+        launch_ntt(r2, w.rgsw2, (size_t)Q * bits * 4 * lq, 1, false, st);
     }
     const double *cm_in = w.fd;
     int cc = 0;
     for (int b = 0; b < nc; b++) {
-        launch_gate(r2, g2, GATE_CMUX, w.rgsw2 + (size_t)b * rgw2, cm_in, w.cm[cc], (int)(J >> (b + 1)), 0, st);
+        GateArgs ga;
+        ga.mode = GATE_CMUX;
+        ga.rgsw = w.rgsw2 + (size_t)b * rgw2;
+        ga.rgsw_qstride = bits * rgw2;
+        ga.in = cm_in;
+        ga.in_qstride = J * ctw2;
+        ga.out = w.cm[cc];
+        ga.out_qstride = J * ctw2;
+        ga.items_per_query = (int)(J >> (b + 1));
+        ga.queries = Q;
+        ga.shift = 0;
+        launch_gate(r2, g2, ga, st);
         cm_in = w.cm[cc];
         cc ^= 1;
     }
+    // gather each query's tree root into res[0] (stride 2n)
+    const double *res = cm_in;
+    size_t res_qstride = J * ctw2;
     c->stage_ptr[4] = cm_in;
-    c->stage_words[4] = 2 * n;
-    c->stage_nl[4] = 1;
+    c->stage_words[4] = ctw2;
     HIP_TRY(hipEventRecord(c->ev[5], st));
 
     // ---- step 6: crot SlotExtract (rotate.rs:128-149) ----
-    const double *res = cm_in;
     int rc = 0;
     for (int b = 0; b < nr; b++) {
-        launch_gate(r2, g2, GATE_CROT, w.rgsw2 + (size_t)(nc + b) * rgw2, res, w.res[rc], 1, (int)(n - (1u << b)), st);
+        GateArgs ga;
+        ga.mode = GATE_CROT;
+        ga.rgsw = w.rgsw2 + (size_t)(nc + b) * rgw2;
+        ga.rgsw_qstride = bits * rgw2;
+        ga.in = res;
+        ga.in_qstride = res_qstride;
+        ga.out = w.res[rc];
+        ga.out_qstride = ctw2;
+        ga.items_per_query = 1;
+        ga.queries = Q;
+        ga.shift = (int)(n - (1u << b));
+        launch_gate(r2, g2, ga, st);
         res = w.res[rc];
+        res_qstride = ctw2;
         rc ^= 1;
     }
+    if (res_qstride != ctw2) {
+        for (int q = 0; q < Q; q++)
+            HIP_TRY(hipMemcpyAsync(w.res[rc] + (size_t)q * ctw2, res + (size_t)q * res_qstride, sizeof(double) * ctw2,
+                                   hipMemcpyDeviceToDevice, st));
+        res = w.res[rc];
+    }
     c->stage_ptr[5] = res;
-    c->stage_words[5] = 2 * n;
-    c->stage_nl[5] = 1;
+    c->stage_words[5] = ctw2;
     HIP_TRY(hipEventRecord(c->ev[6], st));
 
     // ---- step 7: resp_comp (resp_comp.rs:46-76) ----
     RespArgs ra;
+    memset(&ra, 0, sizeof ra);
     ra.ct = res;
-    ra.rsk_eval = keys->rsk;
+    for (int q = 0; q < Q; q++) {
+        ra.rsk_eval[q] = keys[q]->rsk;
+        ra.out_mask[q] = d_mask[q];
+        ra.out_body[q] = d_body[q];
+    }
     ra.n1 = (int)n;
     ra.n2 = (int)pp->n2;
     ra.d = (int)(n / pp->n2);
@@ -731,8 +832,7 @@ static via_status answer_impl(via_ctx *c, const via_db *db, const via_keys *keys
     ra.t3.inv = t3.inv;
     ra.t3.n_inv = t3.n_inv;
     ra.t3.m = fmod_make(pp->q3);
-    ra.out_mask = d_mask;
-    ra.out_body = d_body;
+    ra.queries = Q;
     launch_resp_comp(ra, st);
     HIP_TRY(hipEventRecord(c->ev[7], st));
     HIP_TRY(hipGetLastError());
@@ -744,29 +844,67 @@ via_status via_answer_device(via_ctx *c, const via_db *db, const via_keys *keys,
                              size_t n_lwe, uint64_t *d_mask, uint64_t *d_body, via_err_detail *err) {
     if (!c || !db || !keys || !d_mask || !d_body || (n_lwe && !d_lwes)) return VIA_E_INVALID;
     HIP_TRY(hipSetDevice(c->device));
-    return answer_impl(c, db, keys, d_lwes, n_lwe, d_mask, d_body, err);
+    return answer_batch_impl(c, db, 1, &keys, &d_lwes, n_lwe, &d_mask, &d_body, err);
+}
+
+via_status via_answer_batch_device(via_ctx *c, const via_db *db, uint32_t nq, const via_keys *const *keys,
+                                   const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_masks,
+                                   uint64_t *const *d_bodies, via_err_detail *err) {
+    if (!c || !db || !keys || !d_lwes || !d_masks || !d_bodies || nq < 1) return VIA_E_INVALID;
+    HIP_TRY(hipSetDevice(c->device));
+    for (uint32_t q0 = 0; q0 < nq; q0 += kMaxQ) {
+        const int Q = (int)std::min<uint32_t>(kMaxQ, nq - q0);
+        via_status s = answer_batch_impl(c, db, Q, keys + q0, d_lwes + q0, n_lwe, d_masks + q0, d_bodies + q0, err);
+        if (s) return s;
+    }
+    return VIA_OK;
+}
+
+via_status via_answer_batch(via_ctx *c, const via_db *db, uint32_t nq, const via_keys *const *keys,
+                            const uint64_t *const *lwes, size_t n_lwe, uint64_t *const *masks, uint64_t *const *bodies,
+                            via_err_detail *err) {
+    if (!c || !db || !keys || !lwes || !masks || !bodies || nq < 1) return VIA_E_INVALID;
+    HIP_TRY(hipSetDevice(c->device));
+    const via_params *pp = &keys[0]->pp;
+    const int nd = ilog2(pp->num_rows), nc = ilog2(pp->num_cols), nr = ilog2(pp->n1 / pp->n_rec);
+    if (!is_pow2(pp->num_rows) || !is_pow2(pp->num_cols)) return VIA_E_DIM;
+    const size_t bits = (size_t)(nd + nc + nr);
+    const size_t expected = bits * pp->l_query;
+    if (n_lwe != expected) {
+        const uint64_t *dummy[1] = {nullptr};
+        uint64_t *od[1] = {nullptr};
+        return answer_batch_impl(c, db, 1, keys, dummy, n_lwe, od, od, err);
+    }
+    const size_t lw = (size_t)2 * (pp->n1 + 1);
+    for (uint32_t q0 = 0; q0 < nq; q0 += kMaxQ) {
+        const int Q = (int)std::min<uint32_t>(kMaxQ, nq - q0);
+        via_status s = ensure_ws(c, pp, (size_t)Q, n_lwe, bits);
+        if (s) return s;
+        Workspace &w = c->ws;
+        std::vector<const uint64_t *> dl(Q);
+        std::vector<uint64_t *> dm(Q), dbp(Q);
+        for (int q = 0; q < Q; q++) {
+            HIP_TRY(hipMemcpyAsync(w.lwe_u + (size_t)q * lw * n_lwe, lwes[q0 + q], sizeof(uint64_t) * lw * n_lwe,
+                                   hipMemcpyHostToDevice, c->stream));
+            dl[q] = w.lwe_u + (size_t)q * lw * n_lwe;
+            dm[q] = w.ans_u + (size_t)q * 2048;
+            dbp[q] = w.ans_u + (size_t)q * 2048 + 1024;
+        }
+        s = answer_batch_impl(c, db, Q, keys + q0, dl.data(), n_lwe, dm.data(), dbp.data(), err);
+        if (s) return s;
+        for (int q = 0; q < Q; q++) {
+            HIP_TRY(hipMemcpyAsync(masks[q0 + q], dm[q], sizeof(uint64_t) * pp->n2, hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipMemcpyAsync(bodies[q0 + q], dbp[q], sizeof(uint64_t) * pp->n2, hipMemcpyDeviceToHost, c->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return VIA_OK;
 }
 
 via_status via_answer(via_ctx *c, const via_db *db, const via_keys *keys, const uint64_t *lwes, size_t n_lwe,
                       uint64_t *ans_mask, uint64_t *ans_body, via_err_detail *err) {
     if (!c || !db || !keys || !ans_mask || !ans_body || (n_lwe && !lwes)) return VIA_E_INVALID;
-    HIP_TRY(hipSetDevice(c->device));
-    const via_params *pp = &keys->pp;
-    const int nd = ilog2(pp->num_rows), nc = ilog2(pp->num_cols), nr = ilog2(pp->n1 / pp->n_rec);
-    const size_t expected = (size_t)(nd + nc + nr) * pp->l_query;
-    if (!is_pow2(pp->num_rows) || !is_pow2(pp->num_cols)) return VIA_E_DIM;
-    if (n_lwe != expected) return answer_impl(c, db, keys, nullptr, n_lwe, nullptr, nullptr, err);
-    via_status s = ensure_ws(c, pp, n_lwe, (size_t)(nd + nc + nr));
-    if (s) return s;
-    Workspace &w = c->ws;
-    const size_t lw = (size_t)2 * (pp->n1 + 1);
-    HIP_TRY(hipMemcpyAsync(w.lwe_u, lwes, sizeof(uint64_t) * lw * n_lwe, hipMemcpyHostToDevice, c->stream));
-    s = answer_impl(c, db, keys, w.lwe_u, n_lwe, w.ans_u, w.ans_u + 1024, err);
-    if (s) return s;
-    HIP_TRY(hipMemcpyAsync(ans_mask, w.ans_u, sizeof(uint64_t) * pp->n2, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(ans_body, w.ans_u + 1024, sizeof(uint64_t) * pp->n2, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return VIA_OK;
+    return via_answer_batch(c, db, 1, &keys, &lwes, n_lwe, &ans_mask, &ans_body, err);
 }
 
 via_status via_debug_stage(via_ctx *c, int stage, uint64_t *out, size_t words) {

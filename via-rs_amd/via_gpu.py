@@ -102,6 +102,10 @@ def lib():
             "via_keys_destroy": (None, [vp]),
             "via_answer": (C.c_int, [vp, vp, vp, u64p, C.c_size_t, u64p, u64p, C.POINTER(ErrDetail)]),
             "via_answer_device": (C.c_int, [vp, vp, vp, vp, C.c_size_t, vp, vp, C.POINTER(ErrDetail)]),
+            "via_answer_batch": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.c_size_t,
+                                           C.POINTER(vp), C.POINTER(vp), C.POINTER(ErrDetail)]),
+            "via_answer_batch_device": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.c_size_t,
+                                                  C.POINTER(vp), C.POINTER(vp), C.POINTER(ErrDetail)]),
             "via_debug_stage": (C.c_int, [vp, C.c_int, u64p, C.c_size_t]),
             "via_poly_mul_device": (C.c_int, [vp, C.c_uint32, u64p, C.c_uint32, C.c_size_t, vp, vp, vp, C.c_size_t]),
             "via_ntt": (C.c_int, [vp, C.c_uint32, u64p, C.c_uint32, C.c_size_t, u64p, C.c_size_t, C.c_int]),
@@ -295,3 +299,22 @@ class Server:
         st = lib().via_answer(self.ctx.h, self.db.h, self.keys.h, _p(q), n_lwe, _p(am), _p(ab), C.byref(err))
         _check(st, err)
         return am, ab
+
+
+def answer_many(ctx, db, keys_list, queries, pp):
+    """Answers several queries (each with its own client's PreparedKeys) in
+    one batched pass over the pipeline (via_answer_batch); each answer equals
+    the one Server::answer gives for that query alone."""
+    nq = len(queries)
+    qs = [_u64(q) for q in queries]
+    n_lwe = qs[0].size // pp.lwe_words
+    masks = [np.zeros(pp.n2, dtype=np.uint64) for _ in range(nq)]
+    bodies = [np.zeros(pp.n2, dtype=np.uint64) for _ in range(nq)]
+    vp = C.c_void_p
+    K = (vp * nq)(*[k.h.value for k in keys_list])
+    Qp = (vp * nq)(*[q.ctypes.data for q in qs])
+    M = (vp * nq)(*[m.ctypes.data for m in masks])
+    B = (vp * nq)(*[b.ctypes.data for b in bodies])
+    err = ErrDetail()
+    _check(lib().via_answer_batch(ctx.h, db.h, nq, K, Qp, n_lwe, M, B, C.byref(err)), err)
+    return masks, bodies
