@@ -135,6 +135,35 @@ via_status via_poly_mul_device(via_ctx *ctx, uint32_t n, const uint64_t *moduli,
 via_status via_ntt(via_ctx *ctx, uint32_t n, const uint64_t *moduli, uint32_t nmod, size_t group, uint64_t *polys,
                    size_t count, int inverse);
 
+/* ---- serving loop (no counterpart in the reference, which has no server
+ * process; the answers it produces are those of Server::answer) ----------- */
+typedef struct via_server via_server;
+typedef struct {
+    uint64_t answered, batches, max_batch_seen, clients, key_bytes, pending;
+    double p50_s, p99_s;  /* submit-to-result latency percentiles */
+} via_server_stats;
+
+/* Starts a request loop over `db` on `device` with `slots` device contexts
+ * (batches in flight at once) and batches of at most `max_batch` queries.
+ * Client keys are admitted while their HBM footprint fits key_budget_bytes. */
+via_status via_server_create(int device, const via_params *pp, const via_db *db, size_t key_budget_bytes,
+                             uint32_t max_batch, uint32_t slots, via_server **out);
+void via_server_destroy(via_server *srv);
+/* PreparedKeys of one client, resident until evicted; VIA_E_NOMEM when the
+ * budget is exhausted. */
+via_status via_server_admit(via_server *srv, uint64_t client_id, const uint64_t *cascade_key,
+                            const uint64_t *conv_key, const uint64_t *rsk);
+/* Waits for the client's outstanding queries, then frees (and zeroizes) its keys. */
+via_status via_server_evict(via_server *srv, uint64_t client_id);
+/* Queues one query of an admitted client; returns at once with a ticket. */
+via_status via_server_submit(via_server *srv, uint64_t client_id, const uint64_t *lwes, size_t n_lwe,
+                             uint64_t *ticket);
+via_status via_server_poll(via_server *srv, uint64_t ticket, int *done);
+/* Blocks until the ticket's answer is ready and copies it out. */
+via_status via_server_wait(via_server *srv, uint64_t ticket, uint64_t *mask, uint64_t *body, double *latency_s);
+via_status via_server_get_stats(via_server *srv, via_server_stats *out);
+via_status via_server_reset_stats(via_server *srv);
+
 /* Device memory helpers for callers without a runtime of their own. */
 via_status via_device_alloc(via_ctx *ctx, size_t bytes, void **d_ptr);
 via_status via_device_free(via_ctx *ctx, void *d_ptr);

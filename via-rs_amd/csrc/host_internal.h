@@ -1,0 +1,75 @@
+// host_internal.h — host-side objects shared by the answer path (via_gpu.cpp)
+// and the serving loop (serve.cpp). Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "../../include/via_gpu.h"
+#include "internal.h"
+
+struct DevTab {
+    double *fwd = nullptr, *inv = nullptr;
+    double n_inv = 0;
+    uint64_t q = 0;
+};
+
+struct Workspace {
+    size_t Q = 0, B = 0, I = 0, J = 0, bits = 0;
+    double *lwe_f = nullptr, *masks[2] = {nullptr, nullptr}, *body[2] = {nullptr, nullptr};
+    double *pm = nullptr, *pb = nullptr, *rgsw = nullptr, *rgsw_eval = nullptr;
+    double *dm[2] = {nullptr, nullptr}, *sw = nullptr, *sel = nullptr, *fd = nullptr;
+    double *cm[2] = {nullptr, nullptr}, *rgsw2 = nullptr, *res[2] = {nullptr, nullptr};
+    uint64_t *lwe_u = nullptr, *ans_u = nullptr;
+    int16_t *dig = nullptr;
+    int groups = 1;
+    int fd_chunks = 1;
+    std::vector<void *> all;
+};
+
+struct via_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::map<std::pair<uint64_t, uint32_t>, DevTab> tabs;
+    std::mutex mu;
+    Workspace ws;
+    hipEvent_t ev[8] = {};
+    hipEvent_t ev_fd[2] = {};
+    bool have_times = false;
+    // pointers to the last answer's stage buffers
+    const double *stage_ptr[7] = {};
+    size_t stage_words[7] = {};
+};
+
+struct via_db {
+    via_ctx *ctx = nullptr;
+    via_params pp;
+    double *cells = nullptr;  // eval form at q2, [I][J][n1]
+};
+
+struct via_keys {
+    via_ctx *ctx = nullptr;
+    via_params pp;
+    std::vector<double *> cascade;  // per step, [k][part][limb][rank_in*n_out]
+    double *conv = nullptr;         // [k][part][limb][n1]
+    double *rsk = nullptr;          // [j][k][part][n2]
+    std::vector<size_t> bytes;
+};
+
+
+namespace via {
+// Answers Q <= kMaxQ queries (device buffers) against one database on the
+// context's stream, asynchronously.
+via_status answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via_keys *const *keys,
+                             const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_mask,
+                             uint64_t *const *d_body, via_err_detail *err);
+// Grows the context workspace for Q queries of n_lwe LWEs.
+via_status ensure_workspace(via_ctx *c, const via_params *pp, size_t Q, size_t n_lwe);
+size_t keys_device_bytes(const via_keys *k);
+// Builds the twiddle tables of every ring the answer path uses on this context.
+via_status prepare_tables(via_ctx *c, const via_params *pp);
+}  // namespace via

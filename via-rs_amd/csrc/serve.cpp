@@ -1,0 +1,375 @@
+// serve.cpp — continuous-batching request loop for the VIA-C answer path.
+//
+// The reference has no server process (it is a library answering one query
+// per call); this loop is the serving layer around it. Clients are admitted
+// against an HBM budget for their evaluation keys and stay resident. Queries
+// arrive at any time; a worker thread gathers whatever is pending into a batch
+// of up to max_batch queries as soon as one of `slots` device contexts is free,
+// launches it asynchronously on that context's stream and retires it when its
+// completion event fires, so batches launched while others run overlap on the
+// GPU. Every answer is bit-identical to answering its query alone.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "host_internal.h"
+
+using namespace via;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    uint64_t ticket;
+    uint64_t client;
+    std::vector<uint64_t> lwes;
+    Clock::time_point t_submit;
+};
+
+struct Result {
+    bool done = false;
+    via_status status = VIA_OK;
+    std::vector<uint64_t> mask, body;
+    double latency_s = 0;
+};
+
+struct Slot {
+    via_ctx *ctx = nullptr;
+    uint64_t *h_q = nullptr;   // pinned query staging, max_batch x lwe words
+    uint64_t *h_a = nullptr;   // pinned answers, max_batch x 2 x n2
+    uint64_t *d_q = nullptr;   // device queries
+    uint64_t *d_a = nullptr;   // device answers
+    hipEvent_t done = nullptr;
+    std::vector<Request> batch;
+    bool busy = false;
+};
+
+}  // namespace
+
+struct via_server {
+    via_params pp;
+    const via_db *db = nullptr;
+    int device = 0;
+    size_t key_budget = 0, key_bytes = 0;
+    uint32_t max_batch = 1;
+    size_t n_lwe = 0, lwe_words = 0;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request> pending;
+    std::unordered_map<uint64_t, Result> results;
+    std::unordered_map<uint64_t, via_keys *> clients;
+    std::unordered_map<uint64_t, int> client_inflight;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    std::thread worker;
+    // statistics
+    uint64_t answered = 0, batches = 0, max_seen_batch = 0;
+    std::vector<double> latencies;
+};
+
+namespace {
+
+// Launch one batch on a free slot (caller holds no lock; the slot is owned by
+// the worker thread).
+via_status launch_batch(via_server *s, Slot &sl) {
+    const int Q = (int)sl.batch.size();
+    via_status st = ensure_workspace(sl.ctx, &s->pp, (size_t)Q, s->n_lwe);
+    if (st) return st;
+    std::vector<const via_keys *> keys(Q);
+    std::vector<const uint64_t *> dq(Q);
+    std::vector<uint64_t *> dm(Q), dbd(Q);
+    for (int q = 0; q < Q; q++) {
+        const Request &r = sl.batch[q];
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            keys[q] = s->clients.at(r.client);
+        }
+        memcpy(sl.h_q + (size_t)q * s->lwe_words, r.lwes.data(), sizeof(uint64_t) * s->lwe_words);
+        dq[q] = sl.d_q + (size_t)q * s->lwe_words;
+        dm[q] = sl.d_a + (size_t)q * 2 * s->pp.n2;
+        dbd[q] = dm[q] + s->pp.n2;
+    }
+    hipStream_t stream = sl.ctx->stream;
+    if (hipMemcpyAsync(sl.d_q, sl.h_q, sizeof(uint64_t) * s->lwe_words * Q, hipMemcpyHostToDevice, stream) !=
+        hipSuccess)
+        return VIA_E_DEVICE;
+    st = answer_batch_impl(sl.ctx, s->db, Q, keys.data(), dq.data(), s->n_lwe, dm.data(), dbd.data(), nullptr);
+    if (st) return st;
+    if (hipMemcpyAsync(sl.h_a, sl.d_a, sizeof(uint64_t) * 2 * s->pp.n2 * Q, hipMemcpyDeviceToHost, stream) !=
+        hipSuccess)
+        return VIA_E_DEVICE;
+    if (hipEventRecord(sl.done, stream) != hipSuccess) return VIA_E_DEVICE;
+    return VIA_OK;
+}
+
+void finish_batch(via_server *s, Slot &sl, via_status st) {
+    const auto now = Clock::now();
+    std::lock_guard<std::mutex> g(s->mu);
+    const int Q = (int)sl.batch.size();
+    for (int q = 0; q < Q; q++) {
+        const Request &r = sl.batch[q];
+        Result &res = s->results[r.ticket];
+        res.status = st;
+        if (st == VIA_OK) {
+            const uint64_t *a = sl.h_a + (size_t)q * 2 * s->pp.n2;
+            res.mask.assign(a, a + s->pp.n2);
+            res.body.assign(a + s->pp.n2, a + 2 * s->pp.n2);
+        }
+        res.latency_s = std::chrono::duration<double>(now - r.t_submit).count();
+        res.done = true;
+        s->latencies.push_back(res.latency_s);
+        s->client_inflight[r.client]--;
+    }
+    s->answered += Q;
+    s->batches++;
+    s->max_seen_batch = std::max<uint64_t>(s->max_seen_batch, (uint64_t)Q);
+    sl.batch.clear();
+    sl.busy = false;
+    s->cv_done.notify_all();
+}
+
+void worker_loop(via_server *s) {
+    (void)hipSetDevice(s->device);
+    for (;;) {
+        // retire finished batches
+        for (Slot &sl : s->slots)
+            if (sl.busy && hipEventQuery(sl.done) == hipSuccess) finish_batch(s, sl, VIA_OK);
+        // launch pending work on free slots
+        bool launched = false;
+        for (Slot &sl : s->slots) {
+            if (sl.busy) continue;
+            {
+                std::unique_lock<std::mutex> lk(s->mu);
+                if (s->pending.empty()) break;
+                const size_t take = std::min<size_t>(s->max_batch, s->pending.size());
+                for (size_t k = 0; k < take; k++) {
+                    sl.batch.push_back(std::move(s->pending.front()));
+                    s->pending.pop_front();
+                }
+            }
+            sl.busy = true;
+            via_status st = launch_batch(s, sl);
+            if (st != VIA_OK) {
+                (void)hipStreamSynchronize(sl.ctx->stream);
+                finish_batch(s, sl, st);
+            }
+            launched = true;
+        }
+        bool any_busy = false;
+        for (Slot &sl : s->slots) any_busy |= sl.busy;
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (s->stop && !any_busy && s->pending.empty()) return;
+        if (launched) continue;
+        if (any_busy) {
+            // poll the in-flight batches; new arrivals wake us early
+            s->cv_work.wait_for(lk, std::chrono::microseconds(50));
+        } else {
+            s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+via_status via_server_create(int device, const via_params *pp, const via_db *db, size_t key_budget_bytes,
+                             uint32_t max_batch, uint32_t slots, via_server **out) {
+    if (!pp || !db || !out || max_batch < 1 || slots < 1) return VIA_E_INVALID;
+    if (max_batch > (uint32_t)kMaxQ) max_batch = kMaxQ;
+    if (memcmp(&db->pp, pp, sizeof *pp) != 0) return VIA_E_DIM;
+    if (hipSetDevice(device) != hipSuccess) return VIA_E_DEVICE;
+    auto s = std::make_unique<via_server>();
+    s->pp = *pp;
+    s->db = db;
+    s->device = device;
+    s->key_budget = key_budget_bytes;
+    s->max_batch = max_batch;
+    auto lg = [](uint32_t x) { return x <= 1 ? 0u : 32u - (uint32_t)__builtin_clz(x - 1); };
+    const uint32_t bits = lg(pp->num_rows) + lg(pp->num_cols) + lg(pp->n1 / pp->n_rec);
+    s->n_lwe = (size_t)bits * pp->l_query;
+    s->lwe_words = s->n_lwe * pp->q1_nl * (pp->n1 + 1);
+    s->slots.resize(slots);
+    for (Slot &sl : s->slots) {
+        via_status st = via_ctx_create(device, &sl.ctx);
+        if (st) return st;
+        if (hipHostMalloc((void **)&sl.h_q, sizeof(uint64_t) * s->lwe_words * max_batch) != hipSuccess ||
+            hipHostMalloc((void **)&sl.h_a, sizeof(uint64_t) * 2 * pp->n2 * max_batch) != hipSuccess ||
+            hipMalloc((void **)&sl.d_q, sizeof(uint64_t) * s->lwe_words * max_batch) != hipSuccess ||
+            hipMalloc((void **)&sl.d_a, sizeof(uint64_t) * 2 * pp->n2 * max_batch) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
+            return VIA_E_NOMEM;
+        st = ensure_workspace(sl.ctx, pp, max_batch, s->n_lwe);
+        if (st) return st;
+        st = prepare_tables(sl.ctx, pp);
+        if (st) return st;
+    }
+    via_server *raw = s.release();
+    raw->worker = std::thread(worker_loop, raw);
+    *out = raw;
+    return VIA_OK;
+}
+
+void via_server_destroy(via_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->device);
+    for (auto &kv : s->clients) via_keys_destroy(kv.second);
+    for (Slot &sl : s->slots) {
+        if (sl.ctx) (void)hipStreamSynchronize(sl.ctx->stream);
+        (void)hipHostFree(sl.h_q);
+        (void)hipHostFree(sl.h_a);
+        (void)hipFree(sl.d_q);
+        (void)hipFree(sl.d_a);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        via_ctx_destroy(sl.ctx);
+    }
+    delete s;
+}
+
+via_status via_server_admit(via_server *s, uint64_t client_id, const uint64_t *cascade_key, const uint64_t *conv_key,
+                            const uint64_t *rsk) {
+    if (!s) return VIA_E_INVALID;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->clients.count(client_id)) return VIA_OK;
+    }
+    // device footprint of one client's eval keys (fp64 words of the three keys)
+    const size_t n1 = s->pp.n1;
+    uint32_t logn = 0;
+    while ((1u << logn) < n1) logn++;
+    const size_t need = sizeof(double) * ((size_t)logn * 2 * n1 * s->pp.l_ck * 2 * 2 +
+                                          (size_t)s->pp.l_ck * 2 * 2 * n1 +
+                                          (size_t)(n1 / s->pp.n2) * s->pp.l_rsk * 2 * s->pp.n2);
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->key_bytes + need > s->key_budget) return VIA_E_NOMEM;
+        s->key_bytes += need;  // reserve
+    }
+    via_keys *k = nullptr;
+    via_status st = via_keys_create(s->slots[0].ctx, &s->pp, cascade_key, conv_key, rsk, &k);
+    std::lock_guard<std::mutex> g(s->mu);
+    if (st) {
+        s->key_bytes -= need;
+        return st;
+    }
+    s->key_bytes += keys_device_bytes(k) - need;
+    s->clients[client_id] = k;
+    s->client_inflight[client_id] = 0;
+    return VIA_OK;
+}
+
+via_status via_server_evict(via_server *s, uint64_t client_id) {
+    if (!s) return VIA_E_INVALID;
+    via_keys *k = nullptr;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->clients.find(client_id);
+        if (it == s->clients.end()) return VIA_E_INVALID;
+        // wait until none of its queries is pending or in flight
+        s->cv_done.wait(lk, [&] {
+            if (s->client_inflight[client_id] > 0) return false;
+            for (const Request &r : s->pending)
+                if (r.client == client_id) return false;
+            return true;
+        });
+        k = it->second;
+        s->key_bytes -= keys_device_bytes(k);
+        s->clients.erase(it);
+        s->client_inflight.erase(client_id);
+    }
+    via_keys_destroy(k);
+    return VIA_OK;
+}
+
+via_status via_server_submit(via_server *s, uint64_t client_id, const uint64_t *lwes, size_t n_lwe,
+                             uint64_t *ticket) {
+    if (!s || !lwes || !ticket) return VIA_E_INVALID;
+    if (n_lwe != s->n_lwe) return VIA_E_QUERY_LEN;
+    Request r;
+    r.client = client_id;
+    r.lwes.assign(lwes, lwes + s->lwe_words);
+    r.t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (!s->clients.count(client_id)) return VIA_E_INVALID;
+        r.ticket = s->next_ticket++;
+        *ticket = r.ticket;
+        s->results[r.ticket] = Result();
+        s->client_inflight[client_id]++;
+        s->pending.push_back(std::move(r));
+    }
+    s->cv_work.notify_one();
+    return VIA_OK;
+}
+
+via_status via_server_poll(via_server *s, uint64_t ticket, int *done) {
+    if (!s || !done) return VIA_E_INVALID;
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->results.find(ticket);
+    if (it == s->results.end()) return VIA_E_INVALID;
+    *done = it->second.done ? 1 : 0;
+    return VIA_OK;
+}
+
+via_status via_server_wait(via_server *s, uint64_t ticket, uint64_t *mask, uint64_t *body, double *latency_s) {
+    if (!s || !mask || !body) return VIA_E_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->results.find(ticket);
+    if (it == s->results.end()) return VIA_E_INVALID;
+    s->cv_done.wait(lk, [&] { return s->results[ticket].done; });
+    Result res = std::move(s->results[ticket]);
+    s->results.erase(ticket);
+    lk.unlock();
+    if (res.status) return res.status;
+    memcpy(mask, res.mask.data(), sizeof(uint64_t) * s->pp.n2);
+    memcpy(body, res.body.data(), sizeof(uint64_t) * s->pp.n2);
+    if (latency_s) *latency_s = res.latency_s;
+    return VIA_OK;
+}
+
+via_status via_server_get_stats(via_server *s, via_server_stats *out) {
+    if (!s || !out) return VIA_E_INVALID;
+    std::lock_guard<std::mutex> g(s->mu);
+    memset(out, 0, sizeof *out);
+    out->answered = s->answered;
+    out->batches = s->batches;
+    out->max_batch_seen = s->max_seen_batch;
+    out->clients = s->clients.size();
+    out->key_bytes = s->key_bytes;
+    out->pending = s->pending.size();
+    std::vector<double> l = s->latencies;
+    if (!l.empty()) {
+        std::sort(l.begin(), l.end());
+        auto pct = [&](double p) { return l[std::min(l.size() - 1, (size_t)(p * (l.size() - 1) + 0.5))]; };
+        out->p50_s = pct(0.50);
+        out->p99_s = pct(0.99);
+    }
+    return VIA_OK;
+}
+
+via_status via_server_reset_stats(via_server *s) {
+    if (!s) return VIA_E_INVALID;
+    std::lock_guard<std::mutex> g(s->mu);
+    s->answered = s->batches = s->max_seen_batch = 0;
+    s->latencies.clear();
+    return VIA_OK;
+}
+
+}  // extern "C"

@@ -14,7 +14,7 @@
 #include <utility>
 #include <vector>
 
-#include "internal.h"
+#include "host_internal.h"
 
 using namespace via;
 
@@ -89,55 +89,7 @@ uint32_t bitrev(uint32_t x, int logn) {
 int ilog2(uint64_t x) { return x <= 1 ? 0 : 64 - __builtin_clzll(x - 1); }
 bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
 
-struct DevTab {
-    double *fwd = nullptr, *inv = nullptr;
-    double n_inv = 0;
-    uint64_t q = 0;
-};
-
 }  // namespace
-
-struct Workspace {
-    size_t Q = 0, B = 0, I = 0, J = 0, bits = 0;
-    double *lwe_f = nullptr, *masks[2] = {nullptr, nullptr}, *body[2] = {nullptr, nullptr};
-    double *pm = nullptr, *pb = nullptr, *rgsw = nullptr, *rgsw_eval = nullptr;
-    double *dm[2] = {nullptr, nullptr}, *sw = nullptr, *sel = nullptr, *fd = nullptr;
-    double *cm[2] = {nullptr, nullptr}, *rgsw2 = nullptr, *res[2] = {nullptr, nullptr};
-    uint64_t *lwe_u = nullptr, *ans_u = nullptr;
-    int16_t *dig = nullptr;
-    int groups = 1;
-    int fd_chunks = 1;
-    std::vector<void *> all;
-};
-
-struct via_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::map<std::pair<uint64_t, uint32_t>, DevTab> tabs;
-    std::mutex mu;
-    Workspace ws;
-    hipEvent_t ev[8] = {};
-    hipEvent_t ev_fd[2] = {};
-    bool have_times = false;
-    // pointers to the last answer's stage buffers
-    const double *stage_ptr[7] = {};
-    size_t stage_words[7] = {};
-};
-
-struct via_db {
-    via_ctx *ctx = nullptr;
-    via_params pp;
-    double *cells = nullptr;  // eval form at q2, [I][J][n1]
-};
-
-struct via_keys {
-    via_ctx *ctx = nullptr;
-    via_params pp;
-    std::vector<double *> cascade;  // per step, [k][part][limb][rank_in*n_out]
-    double *conv = nullptr;         // [k][part][limb][n1]
-    double *rsk = nullptr;          // [j][k][part][n2]
-    std::vector<size_t> bytes;
-};
 
 namespace {
 
@@ -308,6 +260,28 @@ via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_
 }
 
 }  // namespace
+
+via_status via::ensure_workspace(via_ctx *c, const via_params *pp, size_t Q, size_t n_lwe) {
+    const size_t bits = n_lwe / pp->l_query;
+    return ensure_ws(c, pp, Q, n_lwe, bits);
+}
+
+via_status via::prepare_tables(via_ctx *c, const via_params *pp) {
+    RingDev r;
+    via_status s;
+    for (uint32_t nout = 2; nout <= pp->n1; nout <<= 1)
+        if ((s = make_ring(c, nout, pp->q1, 2, &r))) return s;
+    uint64_t q2 = pp->q2;
+    if ((s = make_ring(c, pp->n1, &q2, 1, &r))) return s;
+    DevTab t;
+    return get_tab(c, pp->q3, pp->n2, &t);
+}
+
+size_t via::keys_device_bytes(const via_keys *k) {
+    size_t b = 0;
+    for (size_t x : k->bytes) b += x;
+    return b;
+}
 
 extern "C" {
 
@@ -559,13 +533,15 @@ void via_keys_destroy(via_keys *k) {
 
 // ---- answer -------------------------------------------------------------------------------
 
+}  // extern "C"
+
 // Answers Q queries against one database in one pass over the pipeline:
 // answer_through_crot (answer.rs:83-228) then resp_comp (resp_comp.rs:46-76)
 // for each, with every step batched across the queries; query q uses the
 // keys of its own client.
-static via_status answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via_keys *const *keys,
-                                    const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_mask,
-                                    uint64_t *const *d_body, via_err_detail *err) {
+via_status via::answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via_keys *const *keys,
+                                  const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_mask,
+                                  uint64_t *const *d_body, via_err_detail *err) {
     if (Q < 1 || Q > kMaxQ) return VIA_E_INVALID;
     const via_params *pp = &keys[0]->pp;
     const size_t I = pp->num_rows, J = pp->num_cols, n = pp->n1, nl = 2, S = kSlots;
@@ -839,6 +815,8 @@ static via_status answer_batch_impl(via_ctx *c, const via_db *db, int Q, const v
     c->have_times = true;
     return VIA_OK;
 }
+
+extern "C" {
 
 via_status via_answer_device(via_ctx *c, const via_db *db, const via_keys *keys, const uint64_t *d_lwes,
                              size_t n_lwe, uint64_t *d_mask, uint64_t *d_body, via_err_detail *err) {

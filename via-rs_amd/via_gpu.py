@@ -66,6 +66,12 @@ class ErrDetail(C.Structure):
     _fields_ = [("expected", C.c_uint64), ("got", C.c_uint64), ("msg", C.c_char * 128)]
 
 
+class ServerStats(C.Structure):
+    _fields_ = [("answered", C.c_uint64), ("batches", C.c_uint64), ("max_batch_seen", C.c_uint64),
+                ("clients", C.c_uint64), ("key_bytes", C.c_uint64), ("pending", C.c_uint64),
+                ("p50_s", C.c_double), ("p99_s", C.c_double)]
+
+
 class ViaError(Exception):
     """ViaError::{DimMismatch, QueryLengthMismatch, IndexOutOfRange} and the
     device-side failures of the C ABI."""
@@ -114,6 +120,15 @@ def lib():
             "via_copy_to_device": (C.c_int, [vp, vp, vp, C.c_size_t]),
             "via_copy_to_host": (C.c_int, [vp, vp, vp, C.c_size_t]),
             "via_status_str": (C.c_char_p, [C.c_int]),
+            "via_server_create": (C.c_int, [C.c_int, P, vp, C.c_size_t, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
+            "via_server_destroy": (None, [vp]),
+            "via_server_admit": (C.c_int, [vp, C.c_uint64, u64p, u64p, u64p]),
+            "via_server_evict": (C.c_int, [vp, C.c_uint64]),
+            "via_server_submit": (C.c_int, [vp, C.c_uint64, u64p, C.c_size_t, C.POINTER(C.c_uint64)]),
+            "via_server_poll": (C.c_int, [vp, C.c_uint64, C.POINTER(C.c_int)]),
+            "via_server_wait": (C.c_int, [vp, C.c_uint64, u64p, u64p, C.POINTER(C.c_double)]),
+            "via_server_get_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
+            "via_server_reset_stats": (C.c_int, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -318,3 +333,58 @@ def answer_many(ctx, db, keys_list, queries, pp):
     err = ErrDetail()
     _check(lib().via_answer_batch(ctx.h, db.h, nq, K, Qp, n_lwe, M, B, C.byref(err)), err)
     return masks, bodies
+
+
+class ServeLoop:
+    """Continuous-batching request loop (via_server_*): clients admitted under
+    an HBM key budget, queries submitted at any time, batches of up to
+    max_batch queries launched on `slots` device contexts as they free up."""
+
+    def __init__(self, pp, db, key_budget_bytes, max_batch=16, slots=2, device=0):
+        h = C.c_void_p()
+        _check(lib().via_server_create(device, C.byref(pp), db.h, key_budget_bytes, max_batch, slots, C.byref(h)))
+        self.h, self.pp, self.db = h, pp, db
+
+    def close(self):
+        if self.h:
+            lib().via_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, client_id, cascade_key, conv_key, rsk):
+        ck, cv, rs = _u64(cascade_key), _u64(conv_key), _u64(rsk)
+        _check(lib().via_server_admit(self.h, client_id, _p(ck), _p(cv), _p(rs)))
+
+    def evict(self, client_id):
+        _check(lib().via_server_evict(self.h, client_id))
+
+    def submit(self, client_id, query):
+        q = _u64(query)
+        t = C.c_uint64()
+        _check(lib().via_server_submit(self.h, client_id, _p(q), q.size // self.pp.lwe_words, C.byref(t)))
+        return t.value
+
+    def done(self, ticket):
+        d = C.c_int()
+        _check(lib().via_server_poll(self.h, ticket, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket):
+        m = np.zeros(self.pp.n2, dtype=np.uint64)
+        b = np.zeros(self.pp.n2, dtype=np.uint64)
+        lat = C.c_double()
+        _check(lib().via_server_wait(self.h, ticket, _p(m), _p(b), C.byref(lat)))
+        return m, b, lat.value
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().via_server_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        _check(lib().via_server_reset_stats(self.h))
