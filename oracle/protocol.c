@@ -213,6 +213,7 @@ struct vo_server {
     uint64_t *cascade_eval; /* per-step eval cascade key */
     uint64_t *conv_eval;    /* RLev(L_CK) eval at q1 */
     uint64_t *rsk_eval;     /* D x RLev(L_RSK) eval at (n2, q3) */
+    uint64_t *cascade_coeff; /* the client's cascade key as shipped (VIA-B repack keys) */
     double t[7];
 };
 
@@ -245,6 +246,8 @@ int vo_server_new(const vo_params *pp, const uint64_t *records, size_t n_records
     /* PreparedKeys::from_public_params (prepared_keys.rs:69-77). */
     s->cascade_eval = vo_alloc(vo_words_cascade_key(pp));
     cascade_key_to_eval(&r1, cascade_key, pp->l_ck, s->cascade_eval);
+    s->cascade_coeff = vo_alloc(vo_words_cascade_key(pp));
+    memcpy(s->cascade_coeff, cascade_key, 8 * vo_words_cascade_key(pp));
     s->conv_eval = vo_alloc(vo_words_conv_key(pp));
     vo_rlev_to_eval(&r1, conv_key, pp->l_ck, s->conv_eval);
     s->rsk_eval = vo_alloc(vo_words_rsk(pp));
@@ -257,6 +260,7 @@ void vo_server_free(vo_server *s) {
     if (!s) return;
     free(s->db_eval);
     free(s->cascade_eval);
+    free(s->cascade_coeff);
     free(s->conv_eval);
     free(s->rsk_eval);
     free(s);
@@ -366,15 +370,43 @@ void vo_first_dim(const vo_params *pp, const uint64_t *switched, const uint64_t 
     free(sel);
 }
 
+/* resp_comp (resp_comp.rs:46-76): q2 -> q3 at n1, ring switch to n2 with the
+ * eval RSK, then the mask stays at q3 and the body goes to q4. */
+static void resp_comp(const vo_server *s, const uint64_t *cr, uint64_t *ans_mask, uint64_t *ans_body) {
+    const vo_params *pp = &s->pp;
+    size_t n1 = pp->n1;
+    vo_ring r2 = ring_q2(pp), r3n1 = ring_q3n1(pp), r3n2 = ring_q3n2(pp);
+    uint64_t *c3 = vo_alloc(2 * n1);
+    vo_mod_switch_sym(&r2, cr, &r3n1, c3);
+    size_t n2 = pp->n2, D = n1 / n2;
+    size_t kw = (size_t)pp->l_rsk * 2 * n2;
+    uint64_t *out = vo_alloc(2 * n2), *rot = vo_alloc(n1), *proj = vo_alloc(n2), *acc = vo_alloc(2 * n2);
+    vo_project_at(1, (uint32_t)n1, (uint32_t)n2, c3 + n1, 0, out + n2);
+    for (size_t j = 0; j < D; j++) {
+        vo_mul_x_pow(&r3n1, c3, j, rot);
+        vo_project_at(1, (uint32_t)n1, (uint32_t)n2, rot, 0, proj);
+        vo_gadget_product_eval(&r3n2, s->rsk_eval + j * kw, pp->l_rsk, pp->b_rsk, proj, acc, acc + n2, 0);
+        vo_from_eval(&r3n2, acc);
+        vo_from_eval(&r3n2, acc + n2);
+        vo_poly_sub(&r3n2, out, out, acc);
+        vo_poly_sub(&r3n2, out + n2, out + n2, acc + n2);
+    }
+    vo_mod_switch_asym(&r3n2, out, pp->q3, pp->q4, ans_mask, ans_body);
+    free(c3);
+    free(out);
+    free(rot);
+    free(proj);
+    free(acc);
+}
+
 /* answer_through_crot (answer.rs:83-228) + answer_one_query (answer.rs:289-354). */
-int vo_server_answer_stages(const vo_server *cs, const uint64_t *lwes, size_t n_lwe, uint64_t *rgsw_bits,
-                            uint64_t *dmux_out, uint64_t *switched, uint64_t *fd_out, uint64_t *cmux_out,
-                            uint64_t *crot_out, uint64_t *ans_mask, uint64_t *ans_body) {
-    vo_server *s = (vo_server *)cs;
+static int answer_stages(const vo_server *s, const uint64_t *lwes, size_t n_lwe, uint64_t *rgsw_bits,
+                         uint64_t *dmux_out, uint64_t *switched, uint64_t *fd_out, uint64_t *cmux_out,
+                         uint64_t *crot_out, uint64_t *ans_mask, uint64_t *ans_body, double *times) {
     const vo_params *pp = &s->pp;
     size_t I = pp->num_rows, J = pp->num_cols, n1 = pp->n1;
     uint32_t LQ = pp->l_query;
-    vo_ring r1 = ring_q1(pp), r2 = ring_q2(pp), r3n1 = ring_q3n1(pp), r3n2 = ring_q3n2(pp);
+    vo_ring r1 = ring_q1(pp), r2 = ring_q2(pp);
     size_t w1 = vo_pw(&r1), w2 = n1, cw1 = 2 * w1, cw2 = 2 * w2, gw1 = 4 * (size_t)LQ * w1, gw2 = 4 * (size_t)LQ * w2;
     uint32_t nd = ilog2((uint32_t)I), nc = ilog2((uint32_t)J), nr = ilog2(pp->n1 / pp->n_rec);
     uint32_t bits = nd + nc + nr;
@@ -491,39 +523,17 @@ int vo_server_answer_stages(const vo_server *cs, const uint64_t *lwes, size_t n_
     free(e2);
     double t6 = now_s();
 
-    /* step 7: resp_comp (resp_comp.rs:46-76): q2 -> q3 at n1, ring switch to
-     * n2 with the eval RSK, then mask stays at q3 and body goes to q4. */
-    {
-        uint64_t *c3 = vo_alloc(2 * n1);
-        vo_mod_switch_sym(&r2, cr, &r3n1, c3);
-        size_t n2 = pp->n2, D = n1 / n2;
-        size_t kw = (size_t)pp->l_rsk * 2 * n2;
-        uint64_t *out = vo_alloc(2 * n2), *rot = vo_alloc(n1), *proj = vo_alloc(n2), *acc = vo_alloc(2 * n2);
-        vo_project_at(1, (uint32_t)n1, (uint32_t)n2, c3 + n1, 0, out + n2);
-        for (size_t j = 0; j < D; j++) {
-            vo_mul_x_pow(&r3n1, c3, j, rot);
-            vo_project_at(1, (uint32_t)n1, (uint32_t)n2, rot, 0, proj);
-            vo_gadget_product_eval(&r3n2, s->rsk_eval + j * kw, pp->l_rsk, pp->b_rsk, proj, acc, acc + n2, 0);
-            vo_from_eval(&r3n2, acc);
-            vo_from_eval(&r3n2, acc + n2);
-            vo_poly_sub(&r3n2, out, out, acc);
-            vo_poly_sub(&r3n2, out + n2, out + n2, acc + n2);
-        }
-        vo_mod_switch_asym(&r3n2, out, pp->q3, pp->q4, ans_mask, ans_body);
-        free(c3);
-        free(out);
-        free(rot);
-        free(proj);
-        free(acc);
-    }
+    /* step 7: resp_comp (resp_comp.rs:46-76), skipped when only the CRot
+     * output is wanted (the VIA-B prefix). */
+    if (ans_mask) resp_comp(s, cr, ans_mask, ans_body);
     double t7 = now_s();
-    s->t[0] = t1 - t0;
-    s->t[1] = t2 - t1;
-    s->t[2] = t3 - t2;
-    s->t[3] = t4 - t3;
-    s->t[4] = t5 - t4;
-    s->t[5] = t6 - t5;
-    s->t[6] = t7 - t6;
+    times[0] = t1 - t0;
+    times[1] = t2 - t1;
+    times[2] = t3 - t2;
+    times[3] = t4 - t3;
+    times[4] = t5 - t4;
+    times[5] = t6 - t5;
+    times[6] = t7 - t6;
     if (!rgsw_bits) free(rgsw);
     if (!dmux_out) free(dm);
     if (!switched) free(sw);
@@ -532,6 +542,14 @@ int vo_server_answer_stages(const vo_server *cs, const uint64_t *lwes, size_t n_
     if (!crot_out) free(cr);
     (void)n_lwe;
     return 0;
+}
+
+int vo_server_answer_stages(const vo_server *cs, const uint64_t *lwes, size_t n_lwe, uint64_t *rgsw_bits,
+                            uint64_t *dmux_out, uint64_t *switched, uint64_t *fd_out, uint64_t *cmux_out,
+                            uint64_t *crot_out, uint64_t *ans_mask, uint64_t *ans_body) {
+    vo_server *s = (vo_server *)cs;
+    return answer_stages(s, lwes, n_lwe, rgsw_bits, dmux_out, switched, fd_out, cmux_out, crot_out, ans_mask,
+                         ans_body, s->t);
 }
 
 int vo_server_answer(const vo_server *s, const uint64_t *lwes, size_t n_lwe, uint64_t *ans_mask, uint64_t *ans_body,
@@ -548,4 +566,70 @@ int vo_server_answer(const vo_server *s, const uint64_t *lwes, size_t n_lwe, uin
         return 2;
     }
     return vo_server_answer_stages(s, lwes, n_lwe, NULL, NULL, NULL, NULL, NULL, NULL, ans_mask, ans_body);
+}
+
+typedef struct {
+    const vo_server *s;
+    const uint64_t *lwes;
+    size_t n_lwe, lw;
+    uint64_t *rot;
+    uint32_t T;
+    size_t k, stride;
+} prefix_job;
+
+static void *prefix_worker(void *arg) {
+    prefix_job *j = (prefix_job *)arg;
+    size_t n1 = j->s->pp.n1;
+    double times[7];
+    for (size_t t = j->k; t < j->T; t += j->stride)
+        answer_stages(j->s, j->lwes + t * j->n_lwe * j->lw, j->n_lwe, NULL, NULL, NULL, NULL, NULL,
+                      j->rot + t * 2 * n1, NULL, NULL, times);
+    return NULL;
+}
+
+/* Server::answer_batch::<T> (answer.rs:561-602) -> answer_batch (batch.rs:72-163):
+ * answer_through_crot for each of the T queries (record degree n_rec), the
+ * repack with the cascade-key suffix mod-switched to q2 (scheme.rs:244-266;
+ * the extraction degree is G = n2/T, the presets' K/T), then one resp_comp.
+ * lwes holds the T queries back to back, n_lwe LWEs each. Returns 0,
+ * 1 = DimMismatch (T unsupported or mismatched) or 2 = QueryLengthMismatch. */
+int vo_server_answer_batch(const vo_server *s, uint32_t T, const uint64_t *lwes, size_t n_lwe, uint64_t *ans_mask,
+                           uint64_t *ans_body, size_t *expected, size_t *got) {
+    const vo_params *pp = &s->pp;
+    if (!pp->num_rows || !pp->num_cols || (pp->num_rows & (pp->num_rows - 1)) ||
+        (pp->num_cols & (pp->num_cols - 1)))
+        return 1;
+    if (!T || (T & (T - 1)) || pp->n2 % T || (size_t)T * pp->n_rec > pp->n2) return 1;
+    size_t exp = vo_query_lwes(pp);
+    if (n_lwe != exp) {
+        if (expected) *expected = exp;
+        if (got) *got = n_lwe;
+        return 2;
+    }
+    size_t n1 = pp->n1, lw = vo_words_lwe(pp);
+    uint32_t G = pp->n2 / T;
+    uint64_t *rot = vo_alloc((size_t)T * 2 * n1);
+    /* the T prefixes are independent: one thread each, at most one per core */
+    long ncpu = sysconf(_SC_NPROCESSORS_ONLN);
+    size_t nth = ncpu > 0 ? (size_t)ncpu : 1;
+    if (nth > T) nth = T;
+    prefix_job *jobs = (prefix_job *)calloc(nth, sizeof *jobs);
+    pthread_t *th = (pthread_t *)calloc(nth, sizeof *th);
+    for (size_t k = 0; k < nth; k++) {
+        jobs[k] = (prefix_job){s, lwes, n_lwe, lw, rot, T, k, nth};
+        pthread_create(&th[k], NULL, prefix_worker, &jobs[k]);
+    }
+    for (size_t k = 0; k < nth; k++) pthread_join(th[k], NULL);
+    free(jobs);
+    free(th);
+    vo_ring r1 = ring_q1(pp), r2 = ring_q2(pp);
+    uint64_t *key = vo_alloc(vo_repack_key_words((uint32_t)n1, G, pp->l_ck));
+    vo_repack_key_from_cascade(&r1, s->cascade_coeff, pp->q2, G, pp->l_ck, key);
+    uint64_t *packed = vo_alloc(2 * n1);
+    int rc = vo_repack(&r2, T, G, rot, key, pp->ck_base, pp->l_ck, packed);
+    if (!rc) resp_comp(s, packed, ans_mask, ans_body);
+    free(rot);
+    free(key);
+    free(packed);
+    return rc ? 1 : 0;
 }

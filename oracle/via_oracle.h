@@ -1,5 +1,5 @@
 /*
- * via_oracle.h — CPU restatement of the via-rs VIA-C server answer path.
+ * via_oracle.h — CPU restatement of the via-rs VIA-C and VIA-B server answer path.
  *
  * TEST INFRASTRUCTURE ONLY. This library is the bit-exactness checker for the
  * HIP implementation in via-rs_amd/ and the timed CPU baseline in bench.py.
@@ -180,6 +180,22 @@ void vo_setup_db(const vo_params *pp, const uint64_t *records, size_t n_records,
                  uint64_t *cells /* [I][J][n1] mod p */);
 void vo_first_dim(const vo_params *pp, const uint64_t *switched /* I cts at q2 */,
                   const uint64_t *db_eval, int threads, uint64_t *out /* J cts */);
+
+/* VIA-B: Server::answer_batch::<T> over T queries stored back to back (n_lwe
+ * LWEs each); extraction degree n2/T. Returns 0, 1 = DimMismatch,
+ * 2 = QueryLengthMismatch. */
+int vo_server_answer_batch(const vo_server *s, uint32_t T, const uint64_t *lwes, size_t n_lwe, uint64_t *ans_mask,
+                           uint64_t *ans_body, size_t *expected, size_t *got);
+/* Repacking pieces (conversion/extract.rs, conversion/repack.rs). r is the
+ * single-limb ring at the repack modulus. */
+void vo_extr(const vo_ring *r, const uint64_t *ct, uint32_t G, uint64_t *masks, uint64_t *body);
+size_t vo_repack_key_words(uint32_t n, uint32_t G, uint32_t L);
+void vo_repack_key_from_cascade(const vo_ring *r1, const uint64_t *cascade_key, uint64_t q2, uint32_t G, uint32_t L,
+                                uint64_t *out);
+int vo_repack(const vo_ring *r2, uint32_t T, uint32_t G, const uint64_t *rotateds, const uint64_t *key,
+              uint64_t base, uint32_t L, uint64_t *out);
+void vo_embed_at(uint32_t nl, uint32_t ns, uint32_t nd, const uint64_t *src, size_t slot, uint64_t *dst);
+void vo_project_at(uint32_t nl, uint32_t ns, uint32_t nd, const uint64_t *src, size_t slot, uint64_t *dst);
 
 /* Wide helpers exposed for tests of the 256-bit path. */
 void vo_round_mul_div(uint64_t a_lo, uint64_t a_hi, uint64_t s_lo, uint64_t s_hi,

@@ -87,6 +87,19 @@ def toy_params(num_rows=2, num_cols=2):
                        num_rows, num_cols)
 
 
+def paper_b_params(num_rows, num_cols, T=8):
+    """Paper VIA-B stack (batch_e2e_paper.rs:38-100): the VIA-C paper moduli with
+    record degree n3 = n2 / T (n3 = 64 at T = 8)."""
+    return make_params(2048, 512, PAPER_Q1, PAPER_Q2, PAPER_Q3, 4096, 16, 55879, 81, 8, 18,
+                       2, 18, 8, num_rows, num_cols, n_rec=512 // T)
+
+
+def toy_b_params(num_rows=2, num_cols=2):
+    """VIA-B toy n64 stack of batch_e2e_toy.rs:40-68 (n3 = 2, T = 8)."""
+    return make_params(64, 16, 1 << 36, 1 << 28, 1 << 20, 1 << 12, 16, 64, 64, 8, 8, 7, 12, 8,
+                       num_rows, num_cols, n_rec=2)
+
+
 def ptr(a):
     assert a.flags["C_CONTIGUOUS"]
     if a.dtype == np.uint64:
@@ -170,6 +183,14 @@ class Oracle:
             "vo_setup_db": (None, [P, u64p, C.c_size_t, u64p]),
             "vo_first_dim": (None, [P, u64p, u64p, C.c_int, u64p]),
             "vo_round_mul_div": (None, [C.c_uint64] * 6 + [u64p, u64p]),
+            "vo_server_answer_batch": (C.c_int, [C.c_void_p, C.c_uint32, u64p, C.c_size_t, u64p, u64p,
+                                                 C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+            "vo_extr": (None, [R, u64p, C.c_uint32, u64p, u64p]),
+            "vo_repack_key_words": (C.c_size_t, [C.c_uint32, C.c_uint32, C.c_uint32]),
+            "vo_repack_key_from_cascade": (None, [R, u64p, C.c_uint64, C.c_uint32, C.c_uint32, u64p]),
+            "vo_repack": (C.c_int, [R, C.c_uint32, C.c_uint32, u64p, u64p, C.c_uint64, C.c_uint32, u64p]),
+            "vo_embed_at": (None, [C.c_uint32, C.c_uint32, C.c_uint32, u64p, C.c_size_t, u64p]),
+            "vo_project_at": (None, [C.c_uint32, C.c_uint32, C.c_uint32, u64p, C.c_size_t, u64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
