@@ -223,7 +223,7 @@ def main():
     Barr = (vp * Q)(*[d.value for d in d_b])
 
     def answer():
-        st = L.via_answer_batch_device(ctx.h, db.h, Q, Karr, Qarr, pp.query_lwes, Marr, Barr, C.byref(err))
+        st = L.via_answer_many_device(ctx.h, db.h, Q, Karr, Qarr, pp.query_lwes, Marr, Barr, C.byref(err))
         if st:
             raise via_gpu.ViaError(st, msg=L.via_status_str(st).decode())
 

@@ -88,7 +88,10 @@ void via_db_destroy(via_db *db);
 via_status via_db_read_cells(const via_db *db, uint64_t *out, size_t words);
 
 /* PreparedKeys::from_public_params (prepared_keys.rs:69-77): cascade key,
- * conversion key and ring-switch key to eval form, resident in HBM. */
+ * conversion key and ring-switch key to eval form, resident in HBM. With
+ * pp->n_rec < n2 the cascade steps of output degree >= 2 n_rec are also kept
+ * mod-switched to q2 for VIA-B batches (the keys of repack.rs:684-805). The
+ * keys serve any database whose record degree is >= pp->n_rec. */
 via_status via_keys_create(via_ctx *ctx, const via_params *pp, const uint64_t *cascade_key,
                            const uint64_t *conv_key, const uint64_t *rsk, via_keys **out);
 /* Zeroizes the device copies before release (prepared_keys.rs:89-125). */
@@ -111,18 +114,39 @@ via_status via_ctx_synchronize(via_ctx *ctx);
  * bit-identical to answering the query alone. Host buffers: lwes[q] holds
  * n_lwe LWEs, masks[q] / bodies[q] receive n2 words. No counterpart in the
  * reference, which answers one query per call (answer.rs:534-554). */
-via_status via_answer_batch(via_ctx *ctx, const via_db *db, uint32_t nq, const via_keys *const *keys,
+via_status via_answer_many(via_ctx *ctx, const via_db *db, uint32_t nq, const via_keys *const *keys,
                             const uint64_t *const *lwes, size_t n_lwe, uint64_t *const *masks, uint64_t *const *bodies,
                             via_err_detail *err);
 /* Same with device-resident buffers, enqueued on the context stream. */
-via_status via_answer_batch_device(via_ctx *ctx, const via_db *db, uint32_t nq, const via_keys *const *keys,
+via_status via_answer_many_device(via_ctx *ctx, const via_db *db, uint32_t nq, const via_keys *const *keys,
                                    const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_masks,
                                    uint64_t *const *d_bodies, via_err_detail *err);
+
+/* Server::answer_batch::<T> (answer.rs:561-602) -> answer_batch
+ * (batch.rs:72-163), VIA-B: answer_through_crot for each of the T queries of
+ * one client at the database's record degree n_rec, Repack_{n2} of the T
+ * outputs (repack.rs:331-369, extraction degree n2/T) with the client's
+ * cascade keys mod-switched to q2, then one resp_comp. lwes[t] holds
+ * n_lwe[t] LWEs. Errors: VIA_E_DIM if n_queries != T (batch.rs:124-128),
+ * VIA_E_QUERY_LEN for the first query of the wrong length, VIA_E_INVALID
+ * for a T the record degree does not fit (T a power of two, T * n_rec <= n2,
+ * n2/T >= 2, T <= 256; the reference fixes T at compile time) or keys
+ * created with a record degree above the database's. Host buffers. */
+via_status via_answer_batch(via_ctx *ctx, const via_db *db, const via_keys *keys, uint32_t T,
+                            const uint64_t *const *lwes, const size_t *n_lwe, size_t n_queries, uint64_t *ans_mask,
+                            uint64_t *ans_body, via_err_detail *err);
+/* Same with the T queries back to back in one device buffer (n_lwe LWEs
+ * each) and device answer buffers, enqueued on the context stream. */
+via_status via_answer_batch_device(via_ctx *ctx, const via_db *db, const via_keys *keys, uint32_t T,
+                                   const uint64_t *d_lwes, size_t n_lwe, uint64_t *d_ans_mask, uint64_t *d_ans_body,
+                                   via_err_detail *err);
 
 /* Intermediate values of the last answer for parity tests, coefficient form:
  * stage 1 RGSW bits [bit][half][i][part][limb][n1] at q1, 2 DMux outputs
  * [I][part][limb][n1] at q1, 3 mod-switched rows [I][part][n1] at q2,
- * 4 FirstDim outputs [J][part][n1] at q2, 5 CMux output, 6 CRot output. */
+ * 4 FirstDim outputs [J][part][n1] at q2, 5 CMux output, 6 CRot output,
+ * 7 the repacked RLWE [part][n1] at q2 of the last VIA-B answer. Stages 1-6
+ * of a VIA-B answer are those of its last group of queries. */
 via_status via_debug_stage(via_ctx *ctx, int stage, uint64_t *out, size_t words);
 
 /* RingPolyEval batch (ntt_abstraction.rs:61-78): c = a * b in

@@ -1,0 +1,63 @@
+// digits.h — gadget scaling and signed-balanced digit extraction on the
+// device, shared by the answer kernels and the VIA-B repack.
+#pragma once
+
+#include "internal.h"
+
+namespace via {
+
+// ---- gadget digits ---------------------------------------------------------------
+
+// Centred lift and gadget scale of one coefficient (gadget.rs:115-151):
+// val = round-half-away(c_centred * B^L / Q), returned as sign and magnitude.
+template <int NL>
+__device__ __forceinline__ SMag gadget_scale(double a0, double a1, const RingDev &r, const Gadget &g) {
+    SMag v;
+    if (NL == 2) {
+        via_u128 x = rns_reconstruct(a0, a1, r.rns);
+        via_u128 Q = r.rns.big_q;
+        int neg = x > (Q >> 1);
+        via_u128 mag = neg ? Q - x : x;
+        v.mag = round_mul_div(mag, g.bl, r.qdiv);
+        v.neg = neg && v.mag != 0;
+    } else {
+        uint64_t q = r.q_single;
+        uint64_t x = (uint64_t)a0;
+        int neg = x > (q >> 1);
+        uint64_t mag = neg ? q - x : x;
+        if (g.wide) {
+            // B^L beyond 64-bit products (the base-18 depth-18 gadget at q2)
+            v.mag = round_mul_div((via_u128)mag, g.bl, r.qdiv);
+        } else {
+            // (mag * B^L + floor(q/2)) / q fits in 64 bits (checked by the host).
+            uint64_t bl = (uint64_t)g.bl;
+            v.mag = (via_u128)((mag * bl + (q >> 1)) / q);
+        }
+        v.neg = neg && v.mag != 0;
+    }
+    return v;
+}
+
+__device__ __forceinline__ double digit_residue(int32_t d, const FMod &m) { return d >= 0 ? (double)d : (double)d + m.q; }
+
+// Digits of one coefficient into int16 planes dig[k * stride] (LSB first,
+// gadget.rs:168-190), from the gadget scale of gadget.rs:115-151.
+template <int NL>
+__device__ __forceinline__ void write_digits(double a0, double a1, const RingDev &r, const Gadget &g,
+                                             const DigitPlan &dp, int16_t *dig, size_t stride) {
+    const SMag v = gadget_scale<NL>(a0, a1, r, g);
+    int64_t lo, hi;
+    digit_split(v, dp, lo, hi);
+    for (int k = 0; k < g.L; k++) {
+        int32_t d;
+        if (k < dp.h) {
+            d = digit_i64(lo, dp.base, dp.inv_base);
+            if (k == dp.h - 1) hi += lo;
+        } else {
+            d = digit_i64(hi, dp.base, dp.inv_base);
+        }
+        dig[(size_t)k * stride] = (int16_t)d;
+    }
+}
+
+}  // namespace via

@@ -31,12 +31,27 @@ struct Workspace {
     std::vector<void *> all;
 };
 
+// Buffers of the VIA-B repack, sized for batches of up to T queries.
+struct RepackWs {
+    int T = 0;
+    size_t n_lwe = 0;
+    double *rot = nullptr;      // [T][part][n1] post-CRot ciphertexts at q2
+    double *packed = nullptr;   // [part][n1] repacked RLWE
+    double *body[2] = {nullptr, nullptr};  // [pair][n1] interleaved bodies
+    double *pm = nullptr, *pb = nullptr;   // gadget-product partials
+    int16_t *dig = nullptr;     // [pair][k][2*n1]
+    uint64_t *lwe_u = nullptr;  // staging for host queries [T][n_lwe][limb][n1+1]
+    uint64_t *ans_u = nullptr;  // [part][n2] staging for the host answer
+    std::vector<void *> all;
+};
+
 struct via_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::map<std::pair<uint64_t, uint32_t>, DevTab> tabs;
     std::mutex mu;
     Workspace ws;
+    RepackWs rw;
     hipEvent_t ev[8] = {};
     hipEvent_t ev_fd[2] = {};
     bool have_times = false;
@@ -57,6 +72,10 @@ struct via_keys {
     std::vector<double *> cascade;  // per step, [k][part][limb][rank_in*n_out]
     double *conv = nullptr;         // [k][part][limb][n1]
     double *rsk = nullptr;          // [j][k][part][n2]
+    // VIA-B repack keys: the cascade steps of output degree repack_nout0..n1
+    // mod-switched to q2, eval form, [k][part][rank_in*n_out] per step
+    std::vector<double *> repack;
+    uint32_t repack_nout0 = 0;
     std::vector<size_t> bytes;
 };
 
@@ -67,6 +86,12 @@ namespace via {
 via_status answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via_keys *const *keys,
                              const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_mask,
                              uint64_t *const *d_body, via_err_detail *err);
+// VIA-B: T queries of one client (d_lwes holds them back to back), repacked
+// into one answer, on the context's stream, asynchronously.
+via_status answer_vb_impl(via_ctx *c, const via_db *db, const via_keys *keys, int T, const uint64_t *d_lwes,
+                          size_t n_lwe, uint64_t *d_mask, uint64_t *d_body, via_err_detail *err);
+// Checks a VIA-B batch size against the database and keys (VIA_E_INVALID).
+via_status check_vb(const via_db *db, const via_keys *keys, uint32_t T);
 // Grows the context workspace for Q queries of n_lwe LWEs.
 via_status ensure_workspace(via_ctx *c, const via_params *pp, size_t Q, size_t n_lwe);
 size_t keys_device_bytes(const via_keys *k);

@@ -30,15 +30,17 @@ struct RingDev {
     int logn;
     NttTab t[2];
     Rns2 rns;          // valid when nl == 2
-    WideDiv qdiv;      // Q as a wide divisor (nl == 2)
+    WideDiv qdiv;      // Q as a wide divisor
     uint64_t q_single; // Q when nl == 1
 };
 
-// Gadget parameters: base, depth, B^L.
+// Gadget parameters: base, depth, B^L. wide: single-modulus scaling needs
+// the 192-bit path ((q/2) * B^L beyond 63 bits).
 struct Gadget {
     uint32_t base;
     int L;
     via_u128 bl;
+    int wide;
 };
 
 // ---- launchers (ntt.hip / kernels.hip) ----------------------------------------
@@ -144,6 +146,26 @@ struct RespArgs {
     int queries;
 };
 void launch_resp_comp(const RespArgs &a, hipStream_t s);
+
+// ---- VIA-B repack (repack.hip) ----
+// Level-0 digits [pair][k][2*n1] and interleaved bodies [pair][body_stride]
+// from the T post-CRot ciphertexts rot [T][part][n1] (ring r: q2 at n1).
+void launch_repack_extract(const RingDev &r, const Gadget &g, const DigitPlan &dp, const double *rot, int T, int G,
+                           int pairs, double *body0, size_t body_stride, int16_t *dig, hipStream_t s);
+struct RepackCombineArgs {
+    const double *pm, *pb;  // MAC outputs (tiles == 2, reduced bodies)
+    int groups, items;
+    const double *body_cur;  // [item][body_stride], degree n
+    size_t body_stride;
+    int final_level;
+    double *out;            // final level: RLWE [part][n1]
+    int16_t *dig_next;      // next level: [item/2][k][2*n1]
+    double *body_next;      // next level: [item/2][body_stride]
+    int zero_partner;       // one live pair: the next pair's odd entries are zero
+};
+void launch_repack_combine(const RingDev &r, const Gadget &g, const DigitPlan &dp, const RepackCombineArgs &a,
+                           hipStream_t s);
+void launch_key_relayout_q2(const double *src, double *dst, int rank_in, int L, int n, hipStream_t s);
 
 // Relayout of a cascade step key [idx][k][part][limb][n] (u64) to the
 // eval-ready layout [k][part][limb][idx*n + c] (doubles).

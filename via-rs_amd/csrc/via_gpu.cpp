@@ -143,6 +143,7 @@ via_status make_ring(via_ctx *c, uint32_t n, const uint64_t *q, int nl, RingDev 
     if (nl == 1) {
         r->t[1] = r->t[0];
         r->q_single = q[0];
+        r->qdiv = widediv_make(q[0]);
     } else {
         r->rns.q0 = q[0];
         r->rns.q1 = q[1];
@@ -161,6 +162,7 @@ Gadget make_gadget(uint64_t base, uint32_t L) {
     g.L = (int)L;
     g.bl = 1;
     for (uint32_t k = 0; k < L; k++) g.bl *= base;
+    g.wide = 0;
     return g;
 }
 
@@ -168,6 +170,13 @@ bool fits_single(uint64_t q, uint64_t base, uint32_t L) {
     u128 bl = 1;
     for (uint32_t k = 0; k < L; k++) bl *= base;
     return (u128)(q / 2 + 1) * bl + q < ((u128)1 << 63);
+}
+
+// Gadget at a single modulus q, taking the 192-bit scaling when B^L is wide.
+Gadget make_gadget_q(uint64_t base, uint32_t L, uint64_t q) {
+    Gadget g = make_gadget(base, L);
+    g.wide = fits_single(q, base, L) ? 0 : 1;
+    return g;
 }
 
 // Parameters the device path implements: the paper n1 = 2048 stack with a
@@ -259,6 +268,44 @@ via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_
     return VIA_OK;
 }
 
+// Workspace of the VIA-B repack for up to T queries of n_lwe LWEs.
+via_status ensure_rw(via_ctx *c, const via_params *pp, int T, size_t n_lwe) {
+    RepackWs &w = c->rw;
+    if (w.T >= T && w.n_lwe >= n_lwe && w.rot) return VIA_OK;
+    T = std::max(T, w.T);
+    n_lwe = std::max(n_lwe, w.n_lwe);
+    for (void *p : w.all) (void)hipFree(p);
+    w = RepackWs();
+    const size_t n = pp->n1, S = kSlots, L = pp->l_ck;
+    const size_t pairs = (size_t)std::max(1, T / 2);
+    // largest items * groups over the levels
+    size_t ig = 0;
+    for (size_t items = pairs;; items = std::max<size_t>(1, items / 2)) {
+        size_t g = 1;
+        while (g < L && items * 2 * g < 512) g++;
+        ig = std::max(ig, items * g);
+        if (items == 1) break;
+    }
+    auto al = [&](void **p, size_t bytes) -> via_status {
+        if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) return VIA_E_NOMEM;
+        w.all.push_back(*p);
+        return VIA_OK;
+    };
+    via_status s;
+    if ((s = al((void **)&w.rot, sizeof(double) * T * 2 * n))) return s;
+    if ((s = al((void **)&w.packed, sizeof(double) * 2 * n))) return s;
+    if ((s = al((void **)&w.body[0], sizeof(double) * pairs * n))) return s;
+    if ((s = al((void **)&w.body[1], sizeof(double) * pairs * n))) return s;
+    if ((s = al((void **)&w.pm, sizeof(double) * ig * 2 * S))) return s;
+    if ((s = al((void **)&w.pb, sizeof(double) * ig * 2 * S))) return s;
+    if ((s = al((void **)&w.dig, sizeof(int16_t) * pairs * L * 2 * S))) return s;
+    if ((s = al((void **)&w.lwe_u, sizeof(uint64_t) * T * n_lwe * 2 * (n + 1)))) return s;
+    if ((s = al((void **)&w.ans_u, sizeof(uint64_t) * 2 * pp->n2))) return s;
+    w.T = T;
+    w.n_lwe = n_lwe;
+    return VIA_OK;
+}
+
 }  // namespace
 
 via_status via::ensure_workspace(via_ctx *c, const via_params *pp, size_t Q, size_t n_lwe) {
@@ -272,7 +319,9 @@ via_status via::prepare_tables(via_ctx *c, const via_params *pp) {
     for (uint32_t nout = 2; nout <= pp->n1; nout <<= 1)
         if ((s = make_ring(c, nout, pp->q1, 2, &r))) return s;
     uint64_t q2 = pp->q2;
-    if ((s = make_ring(c, pp->n1, &q2, 1, &r))) return s;
+    // q2 at n1 (FirstDim, gates) and at the VIA-B repack degrees
+    for (uint32_t nout = 4; nout <= pp->n1; nout <<= 1)
+        if ((s = make_ring(c, nout, &q2, 1, &r))) return s;
     DevTab t;
     return get_tab(c, pp->q3, pp->n2, &t);
 }
@@ -329,6 +378,7 @@ void via_ctx_destroy(via_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     free_ws(c->ws);
+    for (void *p : c->rw.all) (void)hipFree(p);
     for (auto &kv : c->tabs) {
         (void)hipFree(kv.second.fwd);
         (void)hipFree(kv.second.inv);
@@ -459,6 +509,16 @@ via_status via_keys_create(via_ctx *c, const via_params *pp, const uint64_t *cas
     uint64_t *stage = nullptr;
     const size_t max_step = (size_t)2 * n1 * L * 2 * nl;
     HIP_TRY(hipMalloc(&stage, sizeof(uint64_t) * max_step));
+    // VIA-B: the steps of output degree >= 2 n_rec also serve the repack at
+    // q2 (a batch of T has extraction degree n2/T >= n_rec), mod-switched
+    // once here (repack.rs:684-805 does it per answer; the values are equal).
+    const bool vb = pp->n_rec < pp->n2;
+    k->repack_nout0 = vb ? 2 * pp->n_rec : 0;
+    double *tmp_f = nullptr, *tmp_q2 = nullptr;
+    if (vb) {
+        HIP_TRY(hipMalloc(&tmp_f, sizeof(double) * max_step));
+        HIP_TRY(hipMalloc(&tmp_q2, sizeof(double) * max_step / 2));
+    }
     size_t off = 0;
     for (uint32_t nout = 2; nout <= n1; nout <<= 1) {
         const uint32_t rank_in = 2 * n1 / nout;
@@ -472,10 +532,28 @@ via_status via_keys_create(via_ctx *c, const via_params *pp, const uint64_t *cas
         RingDev r;
         if ((s = make_ring(c, nout, pp->q1, 2, &r))) return s;
         launch_ntt(r, d, (size_t)L * 2 * nl * rank_in, rank_in, false, c->stream);
+        if (vb && nout >= k->repack_nout0) {
+            // mod_switch_rlev (repack.rs:43-56) of every sample, q1 -> q2
+            const size_t w2 = words / nl;
+            double *rk = nullptr;
+            HIP_TRY(hipMalloc(&rk, sizeof(double) * w2));
+            k->repack.push_back(rk);
+            launch_u64_to_f64(stage, tmp_f, words, c->stream);
+            launch_modswitch_q1_q2(r, pp->q2, tmp_f, tmp_q2, (size_t)rank_in * L * 2, c->stream);
+            launch_key_relayout_q2(tmp_q2, rk, (int)rank_in, (int)L, (int)nout, c->stream);
+            RingDev r2;
+            uint64_t q2 = pp->q2;
+            if ((s = make_ring(c, nout, &q2, 1, &r2))) return s;
+            launch_ntt(r2, rk, (size_t)L * 2 * rank_in, rank_in, false, c->stream);
+        }
         HIP_TRY(hipStreamSynchronize(c->stream));
         off += words;
     }
     HIP_TRY(hipFree(stage));
+    if (vb) {
+        HIP_TRY(hipFree(tmp_f));
+        HIP_TRY(hipFree(tmp_q2));
+    }
     // conversion key RLev(L_CK) at (n1, q1): layout already [k][part][limb][n]
     {
         const size_t words = (size_t)L * 2 * nl * n1;
@@ -508,6 +586,10 @@ via_status via_keys_create(via_ctx *c, const via_params *pp, const uint64_t *cas
         HIP_TRY(hipFree(u));
         k->bytes.push_back(sizeof(double) * words);
     }
+    for (size_t i = 0; i < k->repack.size(); i++) {
+        const uint32_t nout = k->repack_nout0 << i;
+        k->bytes.push_back(sizeof(double) * (size_t)(2 * n1 / nout) * L * 2 * nout);
+    }
     HIP_TRY(hipGetLastError());
     *out = k;
     return VIA_OK;
@@ -524,7 +606,9 @@ void via_keys_destroy(via_keys *k) {
     }
     (void)hipMemsetAsync(k->conv, 0, k->bytes[i++], c->stream);
     (void)hipMemsetAsync(k->rsk, 0, k->bytes[i++], c->stream);
+    for (double *p : k->repack) (void)hipMemsetAsync(p, 0, k->bytes[i++], c->stream);
     (void)hipStreamSynchronize(c->stream);
+    for (double *p : k->repack) (void)hipFree(p);
     for (double *p : k->cascade) (void)hipFree(p);
     (void)hipFree(k->conv);
     (void)hipFree(k->rsk);
@@ -535,24 +619,33 @@ void via_keys_destroy(via_keys *k) {
 
 }  // extern "C"
 
-// Answers Q queries against one database in one pass over the pipeline:
-// answer_through_crot (answer.rs:83-228) then resp_comp (resp_comp.rs:46-76)
-// for each, with every step batched across the queries; query q uses the
-// keys of its own client.
-via_status via::answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via_keys *const *keys,
-                                  const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_mask,
-                                  uint64_t *const *d_body, via_err_detail *err) {
+namespace {
+
+// Client parameters agree apart from the record degree, which belongs to the
+// database (the reference's N_REC is a server type parameter).
+bool same_client_params(const via_params &a, const via_params &b) {
+    return a.n1 == b.n1 && a.n2 == b.n2 && a.q1_nl == b.q1_nl && a.q1[0] == b.q1[0] && a.q1[1] == b.q1[1] &&
+           a.q2 == b.q2 && a.q3 == b.q3 && a.q4 == b.q4 && a.p == b.p && a.b1 == b.b1 && a.b2 == b.b2 &&
+           a.b_rsk == b.b_rsk && a.ck_base == b.ck_base && a.l_query == b.l_query && a.l_ck == b.l_ck &&
+           a.l_rsk == b.l_rsk && a.num_rows == b.num_rows && a.num_cols == b.num_cols;
+}
+
+// answer_through_crot (answer.rs:83-228) for Q queries against one database,
+// every step batched across the queries; query q uses the keys of its own
+// client. *res_out receives the CRot outputs [q][part][n1] at q2.
+via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *const *keys,
+                       const uint64_t *const *d_lwes, size_t n_lwe, via_err_detail *err, const double **res_out) {
     if (Q < 1 || Q > kMaxQ) return VIA_E_INVALID;
-    const via_params *pp = &keys[0]->pp;
+    via_params ep = keys[0]->pp;
+    ep.n_rec = db->pp.n_rec;
+    const via_params *pp = &ep;
     const size_t I = pp->num_rows, J = pp->num_cols, n = pp->n1, nl = 2, S = kSlots;
     // guards of answer_through_crot (answer.rs:131-157)
     if (!is_pow2(I) || !is_pow2(J)) return VIA_E_DIM;
-    for (int q = 0; q < Q; q++) {
-        const via_params *kp = &keys[q]->pp;
-        if (memcmp(kp, pp, sizeof *pp) != 0) return VIA_E_DIM;
-    }
+    for (int q = 0; q < Q; q++)
+        if (!same_client_params(keys[q]->pp, keys[0]->pp)) return VIA_E_DIM;
     if (db->pp.num_rows != pp->num_rows || db->pp.num_cols != pp->num_cols || db->pp.n1 != pp->n1 ||
-        db->pp.q2 != pp->q2 || db->pp.n_rec != pp->n_rec)
+        db->pp.q2 != pp->q2)
         return VIA_E_DIM;
     const int nd = ilog2(I), nc = ilog2(J), nr = ilog2(pp->n1 / pp->n_rec);
     const size_t bits = (size_t)(nd + nc + nr), lq = pp->l_query;
@@ -577,8 +670,6 @@ via_status via::answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via
     RingDev r2;
     uint64_t q2 = pp->q2;
     if ((s = make_ring(c, (uint32_t)n, &q2, 1, &r2))) return s;
-    DevTab t3;
-    if ((s = get_tab(c, pp->q3, pp->n2, &t3))) return s;
 
     HIP_TRY(hipEventRecord(c->ev[0], st));
     // ---- step 1: query_decomp (query_decomp.rs:39-95) ----
@@ -787,18 +878,28 @@ via_status via::answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via
     c->stage_words[5] = ctw2;
     HIP_TRY(hipEventRecord(c->ev[6], st));
 
-    // ---- step 7: resp_comp (resp_comp.rs:46-76) ----
+    *res_out = res;
+    return VIA_OK;
+}
+
+// resp_comp (resp_comp.rs:46-76) of Q ciphertexts ct [q][part][n1] at q2,
+// query q with the ring-switch key of keys[q].
+via_status resp_comp_impl(via_ctx *c, const via_params *pp, int Q, const via_keys *const *keys, const double *ct,
+                          uint64_t *const *d_mask, uint64_t *const *d_body) {
+    DevTab t3;
+    via_status s;
+    if ((s = get_tab(c, pp->q3, pp->n2, &t3))) return s;
     RespArgs ra;
     memset(&ra, 0, sizeof ra);
-    ra.ct = res;
+    ra.ct = ct;
     for (int q = 0; q < Q; q++) {
         ra.rsk_eval[q] = keys[q]->rsk;
         ra.out_mask[q] = d_mask[q];
         ra.out_body[q] = d_body[q];
     }
-    ra.n1 = (int)n;
+    ra.n1 = (int)pp->n1;
     ra.n2 = (int)pp->n2;
-    ra.d = (int)(n / pp->n2);
+    ra.d = (int)(pp->n1 / pp->n2);
     ra.l_rsk = (int)pp->l_rsk;
     ra.b_rsk = (uint32_t)pp->b_rsk;
     ra.q2 = pp->q2;
@@ -809,10 +910,119 @@ via_status via::answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via
     ra.t3.n_inv = t3.n_inv;
     ra.t3.m = fmod_make(pp->q3);
     ra.queries = Q;
-    launch_resp_comp(ra, st);
-    HIP_TRY(hipEventRecord(c->ev[7], st));
+    launch_resp_comp(ra, c->stream);
+    return VIA_OK;
+}
+
+}  // namespace
+
+// Answers Q queries against one database in one pass over the pipeline:
+// answer_through_crot then resp_comp for each (answer_one_query,
+// answer.rs:289-354), every step batched across the queries.
+via_status via::answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via_keys *const *keys,
+                                  const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_mask,
+                                  uint64_t *const *d_body, via_err_detail *err) {
+    const double *res = nullptr;
+    via_status s = prefix_impl(c, db, Q, keys, d_lwes, n_lwe, err, &res);
+    if (s) return s;
+    if ((s = resp_comp_impl(c, &keys[0]->pp, Q, keys, res, d_mask, d_body))) return s;
+    HIP_TRY(hipEventRecord(c->ev[7], c->stream));
     HIP_TRY(hipGetLastError());
     c->have_times = true;
+    return VIA_OK;
+}
+
+via_status via::check_vb(const via_db *db, const via_keys *keys, uint32_t T) {
+    const via_params *pp = &db->pp;
+    if (!is_pow2(T) || pp->n2 % T || (uint64_t)T * pp->n_rec > pp->n2) return VIA_E_INVALID;
+    const uint32_t G = pp->n2 / T;
+    if (G < 2 || !keys->repack_nout0 || keys->repack_nout0 > 2 * G) return VIA_E_INVALID;
+    if (T > 256) return VIA_E_INVALID;
+    return VIA_OK;
+}
+
+// answer_batch (batch.rs:72-163): answer_through_crot for each of the T
+// queries (record degree n_rec of the database), Repack_{n2} of the T
+// outputs with the client's q2 repack keys (repack.rs:331-369), resp_comp
+// once.
+via_status via::answer_vb_impl(via_ctx *c, const via_db *db, const via_keys *keys, int T, const uint64_t *d_lwes,
+                               size_t n_lwe, uint64_t *d_mask, uint64_t *d_body, via_err_detail *err) {
+    via_status s = check_vb(db, keys, (uint32_t)T);
+    if (s) return s;
+    via_params ep = keys->pp;
+    ep.n_rec = db->pp.n_rec;
+    const via_params *pp = &ep;
+    const size_t n = pp->n1, S = kSlots, lw = 2 * (n + 1);
+    if ((s = ensure_rw(c, pp, T, n_lwe))) return s;
+    RepackWs &w = c->rw;
+    hipStream_t st = c->stream;
+    // steps 1-6 for every query, up to kMaxQ per batched pass
+    for (int q0 = 0; q0 < T; q0 += kMaxQ) {
+        const int Q = std::min(kMaxQ, T - q0);
+        std::vector<const via_keys *> ks(Q, keys);
+        std::vector<const uint64_t *> dl(Q);
+        for (int q = 0; q < Q; q++) dl[q] = d_lwes + (size_t)(q0 + q) * n_lwe * lw;
+        const double *res = nullptr;
+        if ((s = prefix_impl(c, db, Q, ks.data(), dl.data(), n_lwe, err, &res))) return s;
+        HIP_TRY(hipMemcpyAsync(w.rot + (size_t)q0 * 2 * n, res, sizeof(double) * Q * 2 * n, hipMemcpyDeviceToDevice,
+                               st));
+    }
+    HIP_TRY(hipEventRecord(c->ev[7], st));
+    // repack at q2 with the base-ck_base gadget of the cascade keys
+    const uint32_t G = pp->n2 / (uint32_t)T;
+    const Gadget g = make_gadget_q(pp->ck_base, pp->l_ck, pp->q2);
+    const DigitPlan dp = digit_plan((uint32_t)pp->ck_base, (int)pp->l_ck);
+    uint64_t q2 = pp->q2;
+    RingDev r2n;
+    if ((s = make_ring(c, (uint32_t)n, &q2, 1, &r2n))) return s;
+    const int pairs0 = std::max(1, T / 2);
+    launch_repack_extract(r2n, g, dp, w.rot, T, (int)G, pairs0, w.body[0], n, w.dig, st);
+    const int levels = ilog2(n / G);
+    int cur = 0;
+    for (int lev = 0; lev < levels; lev++) {
+        const uint32_t nout = (2 * G) << lev;
+        const int items = std::max(1, T >> (lev + 1));
+        RingDev r;
+        if ((s = make_ring(c, nout, &q2, 1, &r))) return s;
+        int groups = 1;
+        while (groups < (int)pp->l_ck && items * 2 * groups < 512) groups++;
+        MacArgs a;
+        memset(&a, 0, sizeof a);
+        a.dig = w.dig;
+        a.dig_coeffs = 2 * S;
+        a.embed = 0;
+        a.key[0] = keys->repack[ilog2(nout) - ilog2(keys->repack_nout0)];
+        a.items_per_key = items;
+        a.key_slots = 2 * S;
+        a.tiles = 2;
+        a.groups = groups;
+        a.out_mask = w.pm;
+        a.out_body = w.pb;
+        a.reduce_body = 1;
+        a.items = items;
+        launch_gadget_mac(r, g, a, st);
+        RepackCombineArgs cb;
+        memset(&cb, 0, sizeof cb);
+        cb.pm = w.pm;
+        cb.pb = w.pb;
+        cb.groups = groups;
+        cb.items = items;
+        cb.body_cur = w.body[cur];
+        cb.body_stride = n;
+        cb.final_level = lev == levels - 1;
+        cb.out = w.packed;
+        cb.dig_next = w.dig;
+        cb.body_next = w.body[cur ^ 1];
+        cb.zero_partner = items == 1;
+        launch_repack_combine(r, g, dp, cb, st);
+        cur ^= 1;
+    }
+    c->stage_ptr[6] = w.packed;
+    c->stage_words[6] = 2 * n;
+    const via_keys *k1[1] = {keys};
+    if ((s = resp_comp_impl(c, pp, 1, k1, w.packed, &d_mask, &d_body))) return s;
+    HIP_TRY(hipGetLastError());
+    c->have_times = false;
     return VIA_OK;
 }
 
@@ -825,7 +1035,7 @@ via_status via_answer_device(via_ctx *c, const via_db *db, const via_keys *keys,
     return answer_batch_impl(c, db, 1, &keys, &d_lwes, n_lwe, &d_mask, &d_body, err);
 }
 
-via_status via_answer_batch_device(via_ctx *c, const via_db *db, uint32_t nq, const via_keys *const *keys,
+via_status via_answer_many_device(via_ctx *c, const via_db *db, uint32_t nq, const via_keys *const *keys,
                                    const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_masks,
                                    uint64_t *const *d_bodies, via_err_detail *err) {
     if (!c || !db || !keys || !d_lwes || !d_masks || !d_bodies || nq < 1) return VIA_E_INVALID;
@@ -838,7 +1048,7 @@ via_status via_answer_batch_device(via_ctx *c, const via_db *db, uint32_t nq, co
     return VIA_OK;
 }
 
-via_status via_answer_batch(via_ctx *c, const via_db *db, uint32_t nq, const via_keys *const *keys,
+via_status via_answer_many(via_ctx *c, const via_db *db, uint32_t nq, const via_keys *const *keys,
                             const uint64_t *const *lwes, size_t n_lwe, uint64_t *const *masks, uint64_t *const *bodies,
                             via_err_detail *err) {
     if (!c || !db || !keys || !lwes || !masks || !bodies || nq < 1) return VIA_E_INVALID;
@@ -882,11 +1092,61 @@ via_status via_answer_batch(via_ctx *c, const via_db *db, uint32_t nq, const via
 via_status via_answer(via_ctx *c, const via_db *db, const via_keys *keys, const uint64_t *lwes, size_t n_lwe,
                       uint64_t *ans_mask, uint64_t *ans_body, via_err_detail *err) {
     if (!c || !db || !keys || !ans_mask || !ans_body || (n_lwe && !lwes)) return VIA_E_INVALID;
-    return via_answer_batch(c, db, 1, &keys, &lwes, n_lwe, &ans_mask, &ans_body, err);
+    return via_answer_many(c, db, 1, &keys, &lwes, n_lwe, &ans_mask, &ans_body, err);
+}
+
+via_status via_answer_batch_device(via_ctx *c, const via_db *db, const via_keys *keys, uint32_t T,
+                                   const uint64_t *d_lwes, size_t n_lwe, uint64_t *d_ans_mask, uint64_t *d_ans_body,
+                                   via_err_detail *err) {
+    if (!c || !db || !keys || !d_lwes || !d_ans_mask || !d_ans_body) return VIA_E_INVALID;
+    HIP_TRY(hipSetDevice(c->device));
+    return answer_vb_impl(c, db, keys, (int)T, d_lwes, n_lwe, d_ans_mask, d_ans_body, err);
+}
+
+via_status via_answer_batch(via_ctx *c, const via_db *db, const via_keys *keys, uint32_t T,
+                            const uint64_t *const *lwes, const size_t *n_lwe, size_t n_queries, uint64_t *ans_mask,
+                            uint64_t *ans_body, via_err_detail *err) {
+    if (!c || !db || !keys || !ans_mask || !ans_body || (n_queries && (!lwes || !n_lwe))) return VIA_E_INVALID;
+    via_status s = check_vb(db, keys, T);
+    if (s) return s;
+    if (n_queries != T) {  // batch.rs:124-128
+        if (err) {
+            err->expected = T;
+            err->got = n_queries;
+            snprintf(err->msg, sizeof err->msg, "answer_batch: batch.len() must equal T");
+        }
+        return VIA_E_DIM;
+    }
+    via_params ep = keys->pp;
+    ep.n_rec = db->pp.n_rec;
+    if (!is_pow2(ep.num_rows) || !is_pow2(ep.num_cols)) return VIA_E_DIM;
+    const size_t bits = (size_t)(ilog2(ep.num_rows) + ilog2(ep.num_cols) + ilog2(ep.n1 / ep.n_rec));
+    const size_t expected = bits * ep.l_query;
+    for (uint32_t t = 0; t < T; t++)
+        if (n_lwe[t] != expected) {  // answer.rs:132-157, per inner query
+            if (err) {
+                err->expected = expected;
+                err->got = n_lwe[t];
+                snprintf(err->msg, sizeof err->msg, "query %u: expected %zu LWEs, got %zu", t, expected, n_lwe[t]);
+            }
+            return VIA_E_QUERY_LEN;
+        }
+    HIP_TRY(hipSetDevice(c->device));
+    if ((s = ensure_rw(c, &ep, (int)T, expected))) return s;
+    RepackWs &w = c->rw;
+    const size_t lw = (size_t)2 * (ep.n1 + 1);
+    for (uint32_t t = 0; t < T; t++)
+        HIP_TRY(hipMemcpyAsync(w.lwe_u + (size_t)t * expected * lw, lwes[t], sizeof(uint64_t) * expected * lw,
+                               hipMemcpyHostToDevice, c->stream));
+    if ((s = answer_vb_impl(c, db, keys, (int)T, w.lwe_u, expected, w.ans_u, w.ans_u + ep.n2, err))) return s;
+    HIP_TRY(hipMemcpyAsync(ans_mask, w.ans_u, sizeof(uint64_t) * ep.n2, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(ans_body, w.ans_u + ep.n2, sizeof(uint64_t) * ep.n2, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return VIA_OK;
 }
 
 via_status via_debug_stage(via_ctx *c, int stage, uint64_t *out, size_t words) {
-    if (!c || stage < 1 || stage > 6 || !out) return VIA_E_INVALID;
+    if (!c || stage < 1 || stage > 7 || !out) return VIA_E_INVALID;
     const double *p = c->stage_ptr[stage - 1];
     const size_t total = c->stage_words[stage - 1];
     if (!p || words < total) return VIA_E_INVALID;

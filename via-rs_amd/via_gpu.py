@@ -62,6 +62,14 @@ def paper_params(num_rows, num_cols):
     return pp
 
 
+def paper_b_params(num_rows, num_cols, T=8):
+    """The paper VIA-B stack of batch_e2e_paper.rs:38-100: the paper VIA-C
+    moduli with record degree n3 = n2 / T (n3 = 64 at T = 8)."""
+    pp = paper_params(num_rows, num_cols)
+    pp.n_rec = 512 // T
+    return pp
+
+
 class ErrDetail(C.Structure):
     _fields_ = [("expected", C.c_uint64), ("got", C.c_uint64), ("msg", C.c_char * 128)]
 
@@ -108,10 +116,14 @@ def lib():
             "via_keys_destroy": (None, [vp]),
             "via_answer": (C.c_int, [vp, vp, vp, u64p, C.c_size_t, u64p, u64p, C.POINTER(ErrDetail)]),
             "via_answer_device": (C.c_int, [vp, vp, vp, vp, C.c_size_t, vp, vp, C.POINTER(ErrDetail)]),
-            "via_answer_batch": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.c_size_t,
+            "via_answer_many": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.c_size_t,
                                            C.POINTER(vp), C.POINTER(vp), C.POINTER(ErrDetail)]),
-            "via_answer_batch_device": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.c_size_t,
+            "via_answer_many_device": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.c_size_t,
                                                   C.POINTER(vp), C.POINTER(vp), C.POINTER(ErrDetail)]),
+            "via_answer_batch": (C.c_int, [vp, vp, vp, C.c_uint32, C.POINTER(vp), C.POINTER(C.c_size_t),
+                                           C.c_size_t, u64p, u64p, C.POINTER(ErrDetail)]),
+            "via_answer_batch_device": (C.c_int, [vp, vp, vp, C.c_uint32, vp, C.c_size_t, vp, vp,
+                                                  C.POINTER(ErrDetail)]),
             "via_debug_stage": (C.c_int, [vp, C.c_int, u64p, C.c_size_t]),
             "via_poly_mul_device": (C.c_int, [vp, C.c_uint32, u64p, C.c_uint32, C.c_size_t, vp, vp, vp, C.c_size_t]),
             "via_ntt": (C.c_int, [vp, C.c_uint32, u64p, C.c_uint32, C.c_size_t, u64p, C.c_size_t, C.c_int]),
@@ -315,10 +327,28 @@ class Server:
         _check(st, err)
         return am, ab
 
+    def answer_batch(self, T, queries):
+        """Server::answer_batch::<T> (answer.rs:561-602), VIA-B: `queries` is
+        the BatchedQuery's list of T flat LWE arrays; returns one answer
+        (mask mod q3, body mod q4) whose degree-n2 plaintext holds record t at
+        slots t + (n2/n_rec) k (deinterleave_batch, via-client/src/batch.rs)."""
+        qs = [_u64(q) for q in queries]
+        nq = len(qs)
+        vp = C.c_void_p
+        Qp = (vp * max(nq, 1))(*[q.ctypes.data for q in qs])
+        NL = (C.c_size_t * max(nq, 1))(*[q.size // self.pp.lwe_words for q in qs])
+        am = np.zeros(self.pp.n2, dtype=np.uint64)
+        ab = np.zeros(self.pp.n2, dtype=np.uint64)
+        err = ErrDetail()
+        st = lib().via_answer_batch(self.ctx.h, self.db.h, self.keys.h, T, Qp, NL, nq, _p(am), _p(ab),
+                                    C.byref(err))
+        _check(st, err)
+        return am, ab
+
 
 def answer_many(ctx, db, keys_list, queries, pp):
     """Answers several queries (each with its own client's PreparedKeys) in
-    one batched pass over the pipeline (via_answer_batch); each answer equals
+    one batched pass over the pipeline (via_answer_many); each answer equals
     the one Server::answer gives for that query alone."""
     nq = len(queries)
     qs = [_u64(q) for q in queries]
@@ -331,7 +361,7 @@ def answer_many(ctx, db, keys_list, queries, pp):
     M = (vp * nq)(*[m.ctypes.data for m in masks])
     B = (vp * nq)(*[b.ctypes.data for b in bodies])
     err = ErrDetail()
-    _check(lib().via_answer_batch(ctx.h, db.h, nq, K, Qp, n_lwe, M, B, C.byref(err)), err)
+    _check(lib().via_answer_many(ctx.h, db.h, nq, K, Qp, n_lwe, M, B, C.byref(err)), err)
     return masks, bodies
 
 
