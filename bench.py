@@ -151,6 +151,74 @@ def serve_run(via_gpu, pp, db, n_clients, qps, n_req, max_batch, slots, seed=7):
             "requests": n_req}
 
 
+def vb_run(via_gpu, args):
+    """VIA-B: `indices` seeded distinct indices answered as batched queries of
+    T (answer_batch::<T>) against a database at record degree n2/T; one step
+    answers all of them. Reports records/s and per-batch latency."""
+    T = args.vb
+    pp = via_gpu.paper_b_params(args.rows, args.cols, T=T)
+    I, J, n = pp.num_rows, pp.num_cols, pp.n1
+    rng = np.random.default_rng(4096)
+    ctx = via_gpu.Context(0)
+    n_rec = (n // pp.n_rec) * I * J
+    record_bytes = rng.integers(0, 256, size=n_rec * pp.n_rec // 2, dtype=np.uint8)
+    t0 = time.perf_counter()
+    db = via_gpu.PreparedDb(ctx, pp, record_bytes=record_bytes.tobytes(), n_records=n_rec)
+    del record_bytes
+    keys = via_gpu.PreparedKeys(ctx, pp, *synthetic_keys(pp, rng))
+    t_setup = time.perf_counter() - t0
+    nb = max(1, args.indices // T)
+    lw = pp.query_lwes * 2 * (n + 1)
+    d_q, d_m, d_b = [], [], []
+    for _ in range(nb):
+        qs = np.concatenate([synthetic_query(pp, rng) for _ in range(T)])
+        d = ctx.alloc(qs.nbytes)
+        ctx.to_device(d, qs)
+        d_q.append(d)
+        d_m.append(ctx.alloc(8 * pp.n2))
+        d_b.append(ctx.alloc(8 * pp.n2))
+    L = via_gpu.lib()
+    err = via_gpu.ErrDetail()
+
+    def one(b):
+        st = L.via_answer_batch_device(ctx.h, db.h, keys.h, T, d_q[b], pp.query_lwes, d_m[b], d_b[b], C.byref(err))
+        if st:
+            raise via_gpu.ViaError(st, msg=L.via_status_str(st).decode())
+
+    for b in range(min(nb, max(1, args.warmup))):
+        one(b)
+    ctx.synchronize()
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        for b in range(nb):
+            t0 = time.perf_counter()
+            one(b)
+            ctx.synchronize()
+            lat.append(time.perf_counter() - t0)
+    elapsed = time.perf_counter() - t_start
+    lat_ms = np.array(lat) * 1e3
+    recs = args.steps * nb * T
+    out = {
+        "metric": "VIA-B records/sec (answer_batch of T seeded indices) + p50/p99 batch answer latency",
+        "value": round(recs / elapsed, 2), "unit": "records/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "batch_answers_per_s": round(args.steps * nb / elapsed, 3), "T": T,
+        "p50_ms": round(float(np.percentile(lat_ms, 50)), 3), "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int (exact modular, residues < 2^38 in fp64 lanes)",
+        "data": "synthetic: uniform random records, uniform key and query residues",
+        "config": {"workload": f"VIA-B, {nb * T} indices per step as {nb} batch(es) of T={T}, record degree "
+                               f"{pp.n_rec} ({pp.n_rec // 2} B records), I={I} J={J}, prepared DB "
+                               f"{I * J * n * 8 / 2**30:.2f} GiB", "num_rows": I, "num_cols": J, "T": T},
+        "setup_s": round(t_setup, 3),
+    }
+    print(json.dumps(out))
+    for d in d_q + d_m + d_b:
+        ctx.free(d)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +232,8 @@ def main():
     ap.add_argument("--qps", type=float, default=0.0, help="serving arrival rate (0: sweep)")
     ap.add_argument("--clients", type=int, default=64)
     ap.add_argument("--requests", type=int, default=600)
+    ap.add_argument("--vb", type=int, default=0, help="VIA-B batch size T (answer_batch), 0: VIA-C")
+    ap.add_argument("--indices", type=int, default=64, help="VIA-B indices per step")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the answer path is single-GPU in this build "
@@ -173,6 +243,9 @@ def main():
     import __graft_entry__
 
     via_gpu = __graft_entry__.load_package()
+    if args.vb:
+        vb_run(via_gpu, args)
+        return
     pp = via_gpu.paper_params(args.rows, args.cols)
     cascade, conv, rsk, lwes, record_bytes, n_rec = synthetic_inputs(pp)
 
@@ -290,12 +363,15 @@ def main():
     }
     if not args.no_cpu:
         am, ab, t_cpu, t_cpu_setup, cpu_steps = cpu_baseline(pp, cascade, conv, rsk, lwes, record_bytes, n_rec)
+        from oracle.vo import Oracle
+
+        cores = Oracle().host_threads()
         out["cpu_baseline"] = {
             "value": round(1.0 / t_cpu, 5),
             "unit": "answers/s",
-            "cores": os.cpu_count(),
+            "cores": cores,
             "kind": "port",
-            "sample": f"one answer of the same workload (C restatement, FirstDim over {os.cpu_count()} threads, "
+            "sample": f"one answer of the same workload (C restatement, FirstDim over {cores} threads, "
                       f"{t_cpu:.2f} s; setup {t_cpu_setup:.2f} s)",
             "step_s": [round(x, 4) for x in cpu_steps],
         }

@@ -2,7 +2,9 @@
  * protocol.c — LWE->RLWE cascade, VIA-C client and server.
  * Test infrastructure (see via_oracle.h).
  */
+#define _GNU_SOURCE
 #include <pthread.h>
+#include <sched.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -14,6 +16,17 @@ static double now_s(void) {
     struct timespec ts;
     clock_gettime(CLOCK_MONOTONIC, &ts);
     return ts.tv_sec + ts.tv_nsec * 1e-9;
+}
+
+/* Worker threads for the parallel parts: the CPUs this process may run on. */
+int vo_host_threads(void) {
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof set, &set) == 0) {
+        int c = CPU_COUNT(&set);
+        if (c > 0) return c;
+    }
+    long n = sysconf(_SC_NPROCESSORS_ONLN);
+    return n > 0 ? (int)n : 1;
 }
 
 /* ---- cascade -------------------------------------------------------------- */
@@ -345,7 +358,7 @@ void vo_first_dim(const vo_params *pp, const uint64_t *switched, const uint64_t 
     for (size_t i = 0; i < 2 * I; i++) vo_to_eval(&r2, sel + i * n);
     size_t workers = 1;
     if (I * J * n >= (1u << 16)) {
-        long hc = threads > 0 ? threads : sysconf(_SC_NPROCESSORS_ONLN);
+        long hc = threads > 0 ? threads : vo_host_threads();
         workers = hc < 1 ? 1 : (size_t)hc;
         if (workers > J) workers = J;
     }
@@ -610,7 +623,7 @@ int vo_server_answer_batch(const vo_server *s, uint32_t T, const uint64_t *lwes,
     uint32_t G = pp->n2 / T;
     uint64_t *rot = vo_alloc((size_t)T * 2 * n1);
     /* the T prefixes are independent: one thread each, at most one per core */
-    long ncpu = sysconf(_SC_NPROCESSORS_ONLN);
+    long ncpu = vo_host_threads();
     size_t nth = ncpu > 0 ? (size_t)ncpu : 1;
     if (nth > T) nth = T;
     prefix_job *jobs = (prefix_job *)calloc(nth, sizeof *jobs);

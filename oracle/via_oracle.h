@@ -197,6 +197,9 @@ int vo_repack(const vo_ring *r2, uint32_t T, uint32_t G, const uint64_t *rotated
 void vo_embed_at(uint32_t nl, uint32_t ns, uint32_t nd, const uint64_t *src, size_t slot, uint64_t *dst);
 void vo_project_at(uint32_t nl, uint32_t ns, uint32_t nd, const uint64_t *src, size_t slot, uint64_t *dst);
 
+/* Threads the parallel parts use (the CPUs of this process's affinity mask). */
+int vo_host_threads(void);
+
 /* Wide helpers exposed for tests of the 256-bit path. */
 void vo_round_mul_div(uint64_t a_lo, uint64_t a_hi, uint64_t s_lo, uint64_t s_hi,
                       uint64_t d_lo, uint64_t d_hi, uint64_t *q_lo, uint64_t *q_hi);

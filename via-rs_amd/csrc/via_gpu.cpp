@@ -234,9 +234,12 @@ via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_
     w.J = J;
     w.bits = bits;
     const size_t items = Q * B;
-    // digit groups per gadget product: enough workgroups to cover the chip
+    // digit groups per gadget product: about eight workgroups per item,
+    // between 512 and 4096 in all (measured best at 36, 288 and 1152 items;
+    // more groups add partial-sum traffic that small batches do not repay)
+    const long wg_target = std::min<long>(4096, std::max<long>(512, (long)items * 8));
     int groups = 1;
-    while (groups < (int)pp->l_ck && items * 2 * groups < 512) groups++;
+    while (groups < (int)pp->l_ck && (long)items * 2 * groups < wg_target) groups++;
     w.groups = groups;
     w.fd_chunks = first_dim_chunks((int)I, (int)J, (int)n, (int)Q);
     via_status s;
