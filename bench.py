@@ -114,25 +114,39 @@ def cpu_baseline(pp, cascade, conv, rsk, lwes, record_bytes, n_rec):
     return am, ab, t_ans, t_setup, list(steps)
 
 
-def serve_run(via_gpu, pp, db, n_clients, qps, n_req, max_batch, slots, seed=7):
+def serve_run(via_gpu, pp, db, n_clients, qps, n_req, max_batch, slots, seed=7, db_b=None, frac_b=0.0,
+              b_sizes=(4, 64)):
     """Poisson arrivals at `qps` from `n_clients` admitted clients through the
-    continuous-batching loop; returns answers/s and the latency percentiles."""
+    continuous-batching loop. With db_b, a fraction frac_b of the requests are
+    VIA-B requests for a number of records uniform in b_sizes, which the client
+    splits into batches of T = n2 / n_rec(db_b) queries, padding the last one
+    with index-0 queries. Returns answers/s, records/s and the latency
+    percentiles of the requests (a VIA-B request completes with its last batch)."""
     rng = np.random.default_rng(seed)
-    loop = via_gpu.ServeLoop(pp, db, key_budget_bytes=64 << 30, max_batch=max_batch, slots=slots)
-    queries = []
+    kpp = db_b.pp if db_b is not None else pp
+    T = pp.n2 // kpp.n_rec if db_b is not None else 0
+    loop = via_gpu.ServeLoop(pp, db, key_budget_bytes=64 << 30, max_batch=max_batch, slots=slots, db_b=db_b)
+    queries, bqueries = [], []
     for c in range(n_clients):
-        loop.admit(c, *synthetic_keys(pp, rng))
+        loop.admit(c, *synthetic_keys(kpp, rng))
         queries.append(synthetic_query(pp, rng))
+        if db_b is not None:
+            bqueries.append([synthetic_query(kpp, rng) for _ in range(T)])
     # warm every slot once
-    for t in [loop.submit(c % n_clients, queries[c % n_clients]) for c in range(2 * max_batch * slots)]:
+    warm = [loop.submit(c % n_clients, queries[c % n_clients]) for c in range(2 * max_batch * slots)]
+    if db_b is not None:
+        warm += [loop.submit_batch(c % n_clients, T, bqueries[c % n_clients]) for c in range(2 * slots)]
+    for t in warm:
         loop.wait(t)
     loop.reset_stats()
     gaps = rng.exponential(1.0 / qps, size=n_req)
     who = rng.integers(0, n_clients, size=n_req)
-    tickets = []
+    is_b = (rng.random(n_req) < frac_b) if db_b is not None else np.zeros(n_req, bool)
+    sizes = rng.integers(b_sizes[0], b_sizes[1] + 1, size=n_req)
+    reqs = []  # (kind, records, [tickets])
     t0 = time.perf_counter()
     t_next = t0
-    for g, c in zip(gaps, who):
+    for g, c, b, sz in zip(gaps, who, is_b, sizes):
         t_next += g
         while True:
             now = time.perf_counter()
@@ -140,15 +154,30 @@ def serve_run(via_gpu, pp, db, n_clients, qps, n_req, max_batch, slots, seed=7):
                 break
             if t_next - now > 0.002:
                 time.sleep(0.001)
-        tickets.append(loop.submit(int(c), queries[c]))
-    for t in tickets:
-        loop.wait(t)
+        c = int(c)
+        if b:
+            nb = -(-int(sz) // T)
+            reqs.append(("b", int(sz), [loop.submit_batch(c, T, bqueries[c]) for _ in range(nb)]))
+        else:
+            reqs.append(("c", 1, [loop.submit(c, queries[c])]))
+    lat_c, lat_b = [], []
+    for kind, _, ts in reqs:
+        worst = max(loop.wait(t)[2] for t in ts)
+        (lat_b if kind == "b" else lat_c).append(worst)
     elapsed = time.perf_counter() - t0
     st = loop.stats()
     loop.close()
-    return {"offered_qps": qps, "answers_per_s": round(n_req / elapsed, 2), "p50_ms": round(st["p50_s"] * 1e3, 3),
-            "p99_ms": round(st["p99_s"] * 1e3, 3), "batches": st["batches"], "max_batch_seen": st["max_batch_seen"],
-            "requests": n_req}
+    pct = lambda v, q: round(float(np.percentile(np.array(v) * 1e3, q)), 3) if v else None
+    allv = lat_c + lat_b
+    out = {"offered_qps": qps, "answers_per_s": round(st["answered"] / elapsed, 2),
+           "requests_per_s": round(n_req / elapsed, 2),
+           "records_per_s": round(sum(r for _, r, _ in reqs) / elapsed, 2),
+           "p50_ms": pct(allv, 50), "p99_ms": pct(allv, 99), "batches": st["batches"],
+           "max_batch_seen": st["max_batch_seen"], "requests": n_req}
+    if db_b is not None:
+        out.update({"via_b_requests": len(lat_b), "via_b_T": T, "p50_c_ms": pct(lat_c, 50), "p99_c_ms": pct(lat_c, 99),
+                    "p50_b_ms": pct(lat_b, 50), "p99_b_ms": pct(lat_b, 99)})
+    return out
 
 
 def vb_run(via_gpu, args):
@@ -232,6 +261,7 @@ def main():
     ap.add_argument("--qps", type=float, default=0.0, help="serving arrival rate (0: sweep)")
     ap.add_argument("--clients", type=int, default=64)
     ap.add_argument("--requests", type=int, default=600)
+    ap.add_argument("--mix", action="store_true", help="serving: 30%% VIA-B requests against a second database")
     ap.add_argument("--vb", type=int, default=0, help="VIA-B batch size T (answer_batch), 0: VIA-C")
     ap.add_argument("--indices", type=int, default=64, help="VIA-B indices per step")
     args = ap.parse_args()
@@ -257,10 +287,17 @@ def main():
 
     if args.serve:
         rates = [args.qps] if args.qps > 0 else [200.0, 400.0, 800.0, 1200.0, 1600.0, 2400.0]
+        db_b = None
+        if args.mix:
+            pb = via_gpu.paper_b_params(args.rows, args.cols, T=8)
+            nb_rec = (pb.n1 // pb.n_rec) * pb.num_rows * pb.num_cols
+            rb = np.random.default_rng(4097).integers(0, 256, size=nb_rec * pb.n_rec // 2, dtype=np.uint8)
+            db_b = via_gpu.PreparedDb(ctx, pb, record_bytes=rb.tobytes(), n_records=nb_rec)
+            del rb
         runs = []
         for r in rates:
             runs.append(serve_run(via_gpu, pp, db, args.clients, r, args.requests, max_batch=args.batch,
-                                  slots=2))
+                                  slots=2, db_b=db_b, frac_b=0.3 if args.mix else 0.0))
             print(json.dumps({"serving": runs[-1]}), file=sys.stderr)
         best = max(runs, key=lambda x: x["answers_per_s"])
         out = {
@@ -270,8 +307,10 @@ def main():
             "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "int (exact modular, residues < 2^38 in fp64 lanes)",
             "data": "synthetic: uniform random records, uniform key and query residues",
-            "config": {"workload": f"VIA-C serving loop, Poisson arrivals from {args.clients} clients, "
-                                   f"batches <= {args.batch}, 2 slots, I={pp.num_rows} J={pp.num_cols}"},
+            "config": {"workload": (f"serving loop, Poisson arrivals from {args.clients} clients, "
+                                    + ("70% VIA-C queries / 30% VIA-B requests of 4..64 records in T=8 batches, "
+                                       if args.mix else "VIA-C queries, ")
+                                    + f"VIA-C batches <= {args.batch}, 2 slots, I={pp.num_rows} J={pp.num_cols}")},
             "sweep": runs,
         }
         print(json.dumps(out))

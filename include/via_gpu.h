@@ -163,15 +163,23 @@ via_status via_ntt(via_ctx *ctx, uint32_t n, const uint64_t *moduli, uint32_t nm
  * process; the answers it produces are those of Server::answer) ----------- */
 typedef struct via_server via_server;
 typedef struct {
-    uint64_t answered, batches, max_batch_seen, clients, key_bytes, pending;
-    double p50_s, p99_s;  /* submit-to-result latency percentiles */
+    uint64_t answered;       /* requests answered (a VIA-B batch counts once) */
+    uint64_t batches;        /* launches */
+    uint64_t max_batch_seen; /* largest fused VIA-C batch */
+    uint64_t clients, key_bytes, pending;
+    uint64_t records;        /* records delivered: 1 per VIA-C answer, T per VIA-B answer */
+    uint64_t answered_b;     /* VIA-B requests answered */
+    double p50_s, p99_s;     /* submit-to-result latency percentiles, all requests */
+    double p50_b_s, p99_b_s; /* the same over VIA-B requests */
 } via_server_stats;
 
-/* Starts a request loop over `db` on `device` with `slots` device contexts
- * (batches in flight at once) and batches of at most `max_batch` queries.
- * Client keys are admitted while their HBM footprint fits key_budget_bytes. */
-via_status via_server_create(int device, const via_params *pp, const via_db *db, size_t key_budget_bytes,
-                             uint32_t max_batch, uint32_t slots, via_server **out);
+/* Starts a request loop over `db` (VIA-C, record degree pp->n_rec) and, when
+ * db_b is not NULL, `db_b` (VIA-B: the same grid and client parameters at a
+ * finer record degree) on `device` with `slots` device contexts (launches in
+ * flight at once) and VIA-C batches of at most `max_batch` queries. Client
+ * keys are admitted while their HBM footprint fits key_budget_bytes. */
+via_status via_server_create(int device, const via_params *pp, const via_db *db, const via_db *db_b,
+                             size_t key_budget_bytes, uint32_t max_batch, uint32_t slots, via_server **out);
 void via_server_destroy(via_server *srv);
 /* PreparedKeys of one client, resident until evicted; VIA_E_NOMEM when the
  * budget is exhausted. */
@@ -182,6 +190,11 @@ via_status via_server_evict(via_server *srv, uint64_t client_id);
 /* Queues one query of an admitted client; returns at once with a ticket. */
 via_status via_server_submit(via_server *srv, uint64_t client_id, const uint64_t *lwes, size_t n_lwe,
                              uint64_t *ticket);
+/* Queues one VIA-B batch (answer_batch::<T> against db_b): T queries of the
+ * client back to back, n_lwe LWEs each; the ticket's answer is the one
+ * repacked ciphertext. T at most 64 and accepted by via_answer_batch. */
+via_status via_server_submit_batch(via_server *srv, uint64_t client_id, uint32_t T, const uint64_t *lwes,
+                                   size_t n_lwe, uint64_t *ticket);
 via_status via_server_poll(via_server *srv, uint64_t ticket, int *done);
 /* Blocks until the ticket's answer is ready and copies it out. */
 via_status via_server_wait(via_server *srv, uint64_t ticket, uint64_t *mask, uint64_t *body, double *latency_s);

@@ -91,3 +91,40 @@ def paper_case(num_rows, num_cols, seed, index, records="paper", n_records=None,
     g = via_gpu.paper_params(num_rows, num_cols)
     c.pp_gpu = g
     return c
+
+
+def arr(x):
+    return np.ascontiguousarray(np.array(x, dtype=np.uint64))
+
+
+class BatchClient:
+    """Client keys and batch queries from the CPU restatement (the reference's
+    Client::setup / batch_query order)."""
+
+    def __init__(self, oracle, pp, seed):
+        from oracle.vo import TERNARY
+
+        self.o, self.pp, self.T_ = oracle, pp, TERNARY
+        self.prg = oracle.prg(seed)
+        n1, n2 = pp.n1, pp.n2
+        self.sk1, self.sk2 = arr([0] * (2 * n1)), arr([0] * n2)
+        self.ck = arr([0] * oracle.words_cascade_key(pp))
+        self.cv = arr([0] * oracle.words_conv_key(pp))
+        self.rsk = arr([0] * oracle.words_rsk(pp))
+        assert oracle.client_setup(pp, TERNARY, TERNARY, TERNARY, self.prg.h, P(self.sk1), P(self.sk2),
+                                   P(self.ck), P(self.cv), P(self.rsk)) == 0
+
+    def queries(self, idxs):
+        o, pp = self.o, self.pp
+        out = []
+        for idx in idxs:
+            q = arr([0] * (o.query_lwes(pp) * o.words_lwe(pp)))
+            assert o.client_query(pp, P(self.sk1), idx, self.T_, self.prg.h, P(q)) == 0
+            out.append(q)
+        return out
+
+    def recover(self, T, am, ab):
+        o, pp = self.o, self.pp
+        pt = arr([0] * pp.n2)
+        o.client_recover(pp, P(self.sk2), P(arr(am)), P(arr(ab)), P(pt))
+        return [pt[t::pp.n2 // pp.n_rec][:pp.n_rec].tolist() for t in range(T)]

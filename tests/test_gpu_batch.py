@@ -14,6 +14,8 @@ import ctypes
 import numpy as np
 import pytest
 
+from tests.paper_setup import BatchClient as Client
+
 pytestmark = pytest.mark.gpu
 
 
@@ -25,39 +27,6 @@ def P(a):
 
 def arr(x):
     return np.ascontiguousarray(np.array(x, dtype=np.uint64))
-
-
-class Client:
-    """Client keys and batch queries from the CPU restatement (the reference's
-    Client::setup / batch_query order)."""
-
-    def __init__(self, oracle, pp, seed):
-        from oracle.vo import TERNARY
-
-        self.o, self.pp, self.T_ = oracle, pp, TERNARY
-        self.prg = oracle.prg(seed)
-        n1, n2 = pp.n1, pp.n2
-        self.sk1, self.sk2 = arr([0] * (2 * n1)), arr([0] * n2)
-        self.ck = arr([0] * oracle.words_cascade_key(pp))
-        self.cv = arr([0] * oracle.words_conv_key(pp))
-        self.rsk = arr([0] * oracle.words_rsk(pp))
-        assert oracle.client_setup(pp, TERNARY, TERNARY, TERNARY, self.prg.h, P(self.sk1), P(self.sk2),
-                                   P(self.ck), P(self.cv), P(self.rsk)) == 0
-
-    def queries(self, idxs):
-        o, pp = self.o, self.pp
-        out = []
-        for idx in idxs:
-            q = arr([0] * (o.query_lwes(pp) * o.words_lwe(pp)))
-            assert o.client_query(pp, P(self.sk1), idx, self.T_, self.prg.h, P(q)) == 0
-            out.append(q)
-        return out
-
-    def recover(self, T, am, ab):
-        o, pp = self.o, self.pp
-        pt = arr([0] * pp.n2)
-        o.client_recover(pp, P(self.sk2), P(arr(am)), P(arr(ab)), P(pt))
-        return [pt[t::pp.n2 // pp.n_rec][:pp.n_rec].tolist() for t in range(T)]
 
 
 def oracle_batch(oracle, pp, client, records, T, queries):
@@ -193,3 +162,21 @@ def test_t256_batch_recovers_all(oracle):
         assert client.recover(T, am, ab) == [records[i] for i in idxs]
     finally:
         server.ctx.close()
+
+
+def test_query_length_follows_database_record_degree(oracle, paper_b):
+    """Keys made at the VIA-B record degree serve a VIA-C database: the
+    expected query length is the database's (8 LWEs at I = J = 2, n_rec =
+    512), and a query of the VIA-B length is refused before any launch."""
+    import via_gpu
+
+    pp, client, records, server = paper_b
+    g_c = via_gpu.paper_params(2, 2)
+    db_c = via_gpu.PreparedDb(server.ctx, g_c, records=np.zeros((16, 512), dtype=np.uint64))
+    s = via_gpu.Server(server.ctx, g_c, db_c, server.keys)
+    q = client.queries([5])[0]  # 14 LWEs: the VIA-B length
+    with pytest.raises(via_gpu.ViaError) as e:
+        s.answer(q)
+    assert e.value.status == via_gpu.VIA_E_QUERY_LEN
+    assert (e.value.expected, e.value.got) == (8, 14)
+    db_c.close()

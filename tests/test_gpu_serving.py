@@ -67,3 +67,65 @@ def test_admission_by_key_budget(ctx):
     assert np.array_equal(m3, case.ans_mask) and np.array_equal(case.recover(m3, b3), case.record)
     loop.close()
     db.close()
+
+
+def test_serving_mixes_via_c_and_via_b(ctx, oracle):
+    """VIA-C queries and VIA-B batches (T = 8) of three clients through one
+    loop with two slots: every answer equals the one the same request gets
+    alone (via_answer / via_answer_batch on a fresh context), and recovers."""
+    import via_gpu
+    from oracle.vo import paper_b_params, paper_params
+    from tests.paper_setup import BatchClient as Client
+
+    pp_c, pp_b = paper_params(2, 2), paper_b_params(2, 2, T=8)
+    g_c, g_b = via_gpu.paper_params(2, 2), via_gpu.paper_b_params(2, 2, T=8)
+    rng = np.random.default_rng(77)
+    rec_c = rng.integers(0, 16, size=(16, 512), dtype=np.uint64)
+    rec_b = rng.integers(0, 16, size=(128, 64), dtype=np.uint64)
+    db_c = via_gpu.PreparedDb(ctx, g_c, records=rec_c)
+    db_b = via_gpu.PreparedDb(ctx, g_b, records=rec_b)
+    clients = [Client(oracle, pp_b, b"via-gpu/serve-mix/client%d" % i) for i in range(3)]
+    reqs = []  # (client, kind, payload, expected records)
+    for i, cl in enumerate(clients):
+        for rep in range(2):
+            idx = int(rng.integers(0, 16))
+            cl.pp = pp_c
+            q = cl.queries([idx])[0]
+            reqs.append((i, "c", q, rec_c[idx].tolist()))
+        idxs = [int(x) for x in rng.choice(128, size=8, replace=False)]
+        cl.pp = pp_b
+        reqs.append((i, "b", cl.queries(idxs), [rec_b[j].tolist() for j in idxs]))
+    # solo answers, one request per call
+    solo = []
+    sctx = via_gpu.Context(0)
+    keys = [via_gpu.PreparedKeys(sctx, g_b, cl.ck, cl.cv, cl.rsk) for cl in clients]
+    for i, kind, payload, _ in reqs:
+        if kind == "c":
+            solo.append(via_gpu.Server(sctx, g_c, db_c, keys[i]).answer(payload))
+        else:
+            solo.append(via_gpu.Server(sctx, g_b, db_b, keys[i]).answer_batch(8, payload))
+    sctx.close()
+    loop = via_gpu.ServeLoop(g_c, db_c, key_budget_bytes=1 << 34, max_batch=4, slots=2, db_b=db_b)
+    for i, cl in enumerate(clients):
+        loop.admit(i, cl.ck, cl.cv, cl.rsk)
+    tickets = []
+    for rep in range(2):
+        for i, kind, payload, _ in reqs:
+            tickets.append(loop.submit(i, payload) if kind == "c" else loop.submit_batch(i, 8, payload))
+    for k, t in enumerate(tickets):
+        i, kind, payload, want = reqs[k % len(reqs)]
+        m, b, _ = loop.wait(t)
+        sm, sb = solo[k % len(reqs)]
+        assert np.array_equal(m, sm) and np.array_equal(b, sb), (k, kind)
+        cl = clients[i]
+        if kind == "c":
+            cl.pp = pp_c
+            assert cl.recover(1, m, b)[0] == want
+        else:
+            cl.pp = pp_b
+            assert cl.recover(8, m, b) == want
+    st = loop.stats()
+    assert st["answered"] == len(tickets) and st["answered_b"] == 6 and st["records"] == 12 + 6 * 8
+    loop.close()
+    db_c.close()
+    db_b.close()

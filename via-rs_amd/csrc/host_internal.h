@@ -94,7 +94,15 @@ via_status answer_vb_impl(via_ctx *c, const via_db *db, const via_keys *keys, in
 via_status check_vb(const via_db *db, const via_keys *keys, uint32_t T);
 // Grows the context workspace for Q queries of n_lwe LWEs.
 via_status ensure_workspace(via_ctx *c, const via_params *pp, size_t Q, size_t n_lwe);
+// Grows the context's VIA-B repack workspace for batches of T queries.
+via_status ensure_repack_workspace(via_ctx *c, const via_params *pp, int T, size_t n_lwe);
 size_t keys_device_bytes(const via_keys *k);
+// Client parameters agree apart from the record degree.
+bool same_client_params(const via_params &a, const via_params &b);
+// Device bytes via_keys_create allocates for keys of these parameters.
+size_t keys_footprint(const via_params *pp);
+// Largest VIA-B batch the serving loop stages.
+constexpr uint32_t kServeMaxT = 64;
 // Builds the twiddle tables of every ring the answer path uses on this context.
 via_status prepare_tables(via_ctx *c, const via_params *pp);
 }  // namespace via

@@ -1,13 +1,17 @@
-// serve.cpp — continuous-batching request loop for the VIA-C answer path.
+// serve.cpp — continuous-batching request loop for the VIA-C and VIA-B
+// answer paths.
 //
 // The reference has no server process (it is a library answering one query
 // per call); this loop is the serving layer around it. Clients are admitted
-// against an HBM budget for their evaluation keys and stay resident. Queries
-// arrive at any time; a worker thread gathers whatever is pending into a batch
-// of up to max_batch queries as soon as one of `slots` device contexts is free,
-// launches it asynchronously on that context's stream and retires it when its
-// completion event fires, so batches launched while others run overlap on the
-// GPU. Every answer is bit-identical to answering its query alone.
+// against an HBM budget for their evaluation keys and stay resident. Requests
+// arrive at any time: VIA-C queries (Server::answer) and VIA-B batches of T
+// queries (Server::answer_batch::<T>, against a second database packed at the
+// VIA-B record degree). A worker thread takes work as soon as one of `slots`
+// device contexts is free: a VIA-B request runs as one launch sequence of its
+// own, VIA-C queries pending at that moment are gathered into one batch of up
+// to max_batch. Work is launched asynchronously on the slot's stream and
+// retired when its completion event fires, so launches overlap on the GPU.
+// Every answer is bit-identical to answering its request alone.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -32,6 +36,7 @@ using Clock = std::chrono::steady_clock;
 struct Request {
     uint64_t ticket;
     uint64_t client;
+    uint32_t T = 0;  // 0: VIA-C query; else a VIA-B batch of T queries
     std::vector<uint64_t> lwes;
     Clock::time_point t_submit;
 };
@@ -45,7 +50,7 @@ struct Result {
 
 struct Slot {
     via_ctx *ctx = nullptr;
-    uint64_t *h_q = nullptr;   // pinned query staging, max_batch x lwe words
+    uint64_t *h_q = nullptr;   // pinned query staging (q_cap words)
     uint64_t *h_a = nullptr;   // pinned answers, max_batch x 2 x n2
     uint64_t *d_q = nullptr;   // device queries
     uint64_t *d_a = nullptr;   // device answers
@@ -57,12 +62,16 @@ struct Slot {
 }  // namespace
 
 struct via_server {
-    via_params pp;
+    via_params pp;       // VIA-C database parameters
+    via_params kp;       // client key parameters (record degree of the VIA-B database when present)
     const via_db *db = nullptr;
+    const via_db *db_b = nullptr;
     int device = 0;
     size_t key_budget = 0, key_bytes = 0;
     uint32_t max_batch = 1;
-    size_t n_lwe = 0, lwe_words = 0;
+    size_t n_lwe = 0, lwe_words = 0;      // one VIA-C query
+    size_t n_lwe_b = 0, lwe_words_b = 0;  // one query of a VIA-B batch
+    size_t q_cap = 0;                     // staging words per slot
     std::vector<Slot> slots;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -74,15 +83,37 @@ struct via_server {
     bool stop = false;
     std::thread worker;
     // statistics
-    uint64_t answered = 0, batches = 0, max_seen_batch = 0;
-    std::vector<double> latencies;
+    uint64_t answered = 0, batches = 0, max_seen_batch = 0, records = 0, answered_b = 0;
+    std::vector<double> latencies, latencies_b;
 };
 
 namespace {
 
 // Launch one batch on a free slot (caller holds no lock; the slot is owned by
 // the worker thread).
+via_status launch_vb(via_server *s, Slot &sl) {
+    const Request &r = sl.batch[0];
+    const via_keys *k;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        k = s->clients.at(r.client);
+    }
+    const size_t words = (size_t)r.T * s->lwe_words_b;
+    memcpy(sl.h_q, r.lwes.data(), sizeof(uint64_t) * words);
+    hipStream_t stream = sl.ctx->stream;
+    if (hipMemcpyAsync(sl.d_q, sl.h_q, sizeof(uint64_t) * words, hipMemcpyHostToDevice, stream) != hipSuccess)
+        return VIA_E_DEVICE;
+    via_status st = answer_vb_impl(sl.ctx, s->db_b, k, (int)r.T, sl.d_q, s->n_lwe_b, sl.d_a, sl.d_a + s->pp.n2,
+                                   nullptr);
+    if (st) return st;
+    if (hipMemcpyAsync(sl.h_a, sl.d_a, sizeof(uint64_t) * 2 * s->pp.n2, hipMemcpyDeviceToHost, stream) != hipSuccess)
+        return VIA_E_DEVICE;
+    if (hipEventRecord(sl.done, stream) != hipSuccess) return VIA_E_DEVICE;
+    return VIA_OK;
+}
+
 via_status launch_batch(via_server *s, Slot &sl) {
+    if (sl.batch.size() == 1 && sl.batch[0].T > 0) return launch_vb(s, sl);
     const int Q = (int)sl.batch.size();
     via_status st = ensure_workspace(sl.ctx, &s->pp, (size_t)Q, s->n_lwe);
     if (st) return st;
@@ -129,11 +160,16 @@ void finish_batch(via_server *s, Slot &sl, via_status st) {
         res.latency_s = std::chrono::duration<double>(now - r.t_submit).count();
         res.done = true;
         s->latencies.push_back(res.latency_s);
+        if (r.T) {
+            s->latencies_b.push_back(res.latency_s);
+            s->answered_b++;
+        }
+        s->records += r.T ? r.T : 1;
         s->client_inflight[r.client]--;
     }
     s->answered += Q;
     s->batches++;
-    s->max_seen_batch = std::max<uint64_t>(s->max_seen_batch, (uint64_t)Q);
+    if (!(Q == 1 && sl.batch[0].T)) s->max_seen_batch = std::max<uint64_t>(s->max_seen_batch, (uint64_t)Q);
     sl.batch.clear();
     sl.busy = false;
     s->cv_done.notify_all();
@@ -152,10 +188,20 @@ void worker_loop(via_server *s) {
             {
                 std::unique_lock<std::mutex> lk(s->mu);
                 if (s->pending.empty()) break;
-                const size_t take = std::min<size_t>(s->max_batch, s->pending.size());
-                for (size_t k = 0; k < take; k++) {
+                if (s->pending.front().T) {
+                    // a VIA-B request runs on its own
                     sl.batch.push_back(std::move(s->pending.front()));
                     s->pending.pop_front();
+                } else {
+                    // gather the pending VIA-C queries, oldest first
+                    for (auto it = s->pending.begin(); it != s->pending.end() && sl.batch.size() < s->max_batch;) {
+                        if (it->T) {
+                            ++it;
+                            continue;
+                        }
+                        sl.batch.push_back(std::move(*it));
+                        it = s->pending.erase(it);
+                    }
                 }
             }
             sl.busy = true;
@@ -184,15 +230,19 @@ void worker_loop(via_server *s) {
 
 extern "C" {
 
-via_status via_server_create(int device, const via_params *pp, const via_db *db, size_t key_budget_bytes,
-                             uint32_t max_batch, uint32_t slots, via_server **out) {
+via_status via_server_create(int device, const via_params *pp, const via_db *db, const via_db *db_b,
+                             size_t key_budget_bytes, uint32_t max_batch, uint32_t slots, via_server **out) {
     if (!pp || !db || !out || max_batch < 1 || slots < 1) return VIA_E_INVALID;
     if (max_batch > (uint32_t)kMaxQ) max_batch = kMaxQ;
-    if (memcmp(&db->pp, pp, sizeof *pp) != 0) return VIA_E_DIM;
+    if (!same_client_params(db->pp, *pp) || db->pp.n_rec != pp->n_rec) return VIA_E_DIM;
+    // db_b: the same client parameters and grid at a finer record degree
+    if (db_b && (!same_client_params(db_b->pp, *pp) || db_b->pp.n_rec >= pp->n_rec)) return VIA_E_DIM;
     if (hipSetDevice(device) != hipSuccess) return VIA_E_DEVICE;
     auto s = std::make_unique<via_server>();
     s->pp = *pp;
+    s->kp = db_b ? db_b->pp : *pp;
     s->db = db;
+    s->db_b = db_b;
     s->device = device;
     s->key_budget = key_budget_bytes;
     s->max_batch = max_batch;
@@ -200,18 +250,33 @@ via_status via_server_create(int device, const via_params *pp, const via_db *db,
     const uint32_t bits = lg(pp->num_rows) + lg(pp->num_cols) + lg(pp->n1 / pp->n_rec);
     s->n_lwe = (size_t)bits * pp->l_query;
     s->lwe_words = s->n_lwe * pp->q1_nl * (pp->n1 + 1);
+    s->q_cap = s->lwe_words * max_batch;
+    const uint32_t t_b = db_b ? pp->n2 / db_b->pp.n_rec : 0;  // the VIA-B batch size of the record degree
+    if (db_b) {
+        const uint32_t bits_b = lg(pp->num_rows) + lg(pp->num_cols) + lg(pp->n1 / db_b->pp.n_rec);
+        s->n_lwe_b = (size_t)bits_b * pp->l_query;
+        s->lwe_words_b = s->n_lwe_b * pp->q1_nl * (pp->n1 + 1);
+        s->q_cap = std::max(s->q_cap, s->lwe_words_b * std::min<uint32_t>(t_b, kServeMaxT));
+    }
     s->slots.resize(slots);
     for (Slot &sl : s->slots) {
         via_status st = via_ctx_create(device, &sl.ctx);
         if (st) return st;
-        if (hipHostMalloc((void **)&sl.h_q, sizeof(uint64_t) * s->lwe_words * max_batch) != hipSuccess ||
+        if (hipHostMalloc((void **)&sl.h_q, sizeof(uint64_t) * s->q_cap) != hipSuccess ||
             hipHostMalloc((void **)&sl.h_a, sizeof(uint64_t) * 2 * pp->n2 * max_batch) != hipSuccess ||
-            hipMalloc((void **)&sl.d_q, sizeof(uint64_t) * s->lwe_words * max_batch) != hipSuccess ||
+            hipMalloc((void **)&sl.d_q, sizeof(uint64_t) * s->q_cap) != hipSuccess ||
             hipMalloc((void **)&sl.d_a, sizeof(uint64_t) * 2 * pp->n2 * max_batch) != hipSuccess ||
             hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
             return VIA_E_NOMEM;
         st = ensure_workspace(sl.ctx, pp, max_batch, s->n_lwe);
         if (st) return st;
+        if (db_b) {
+            const uint32_t tmax = std::min<uint32_t>(t_b, kServeMaxT);
+            st = ensure_workspace(sl.ctx, &db_b->pp, std::min<uint32_t>(tmax, kMaxQ), s->n_lwe_b);
+            if (st) return st;
+            st = ensure_repack_workspace(sl.ctx, &db_b->pp, (int)tmax, s->n_lwe_b);
+            if (st) return st;
+        }
         st = prepare_tables(sl.ctx, pp);
         if (st) return st;
     }
@@ -250,20 +315,14 @@ via_status via_server_admit(via_server *s, uint64_t client_id, const uint64_t *c
         std::lock_guard<std::mutex> g(s->mu);
         if (s->clients.count(client_id)) return VIA_OK;
     }
-    // device footprint of one client's eval keys (fp64 words of the three keys)
-    const size_t n1 = s->pp.n1;
-    uint32_t logn = 0;
-    while ((1u << logn) < n1) logn++;
-    const size_t need = sizeof(double) * ((size_t)logn * 2 * n1 * s->pp.l_ck * 2 * 2 +
-                                          (size_t)s->pp.l_ck * 2 * 2 * n1 +
-                                          (size_t)(n1 / s->pp.n2) * s->pp.l_rsk * 2 * s->pp.n2);
+    const size_t need = keys_footprint(&s->kp);
     {
         std::lock_guard<std::mutex> g(s->mu);
         if (s->key_bytes + need > s->key_budget) return VIA_E_NOMEM;
         s->key_bytes += need;  // reserve
     }
     via_keys *k = nullptr;
-    via_status st = via_keys_create(s->slots[0].ctx, &s->pp, cascade_key, conv_key, rsk, &k);
+    via_status st = via_keys_create(s->slots[0].ctx, &s->kp, cascade_key, conv_key, rsk, &k);
     std::lock_guard<std::mutex> g(s->mu);
     if (st) {
         s->key_bytes -= need;
@@ -298,14 +357,7 @@ via_status via_server_evict(via_server *s, uint64_t client_id) {
     return VIA_OK;
 }
 
-via_status via_server_submit(via_server *s, uint64_t client_id, const uint64_t *lwes, size_t n_lwe,
-                             uint64_t *ticket) {
-    if (!s || !lwes || !ticket) return VIA_E_INVALID;
-    if (n_lwe != s->n_lwe) return VIA_E_QUERY_LEN;
-    Request r;
-    r.client = client_id;
-    r.lwes.assign(lwes, lwes + s->lwe_words);
-    r.t_submit = Clock::now();
+static via_status enqueue(via_server *s, Request &&r, uint64_t client_id, uint64_t *ticket) {
     {
         std::lock_guard<std::mutex> g(s->mu);
         if (!s->clients.count(client_id)) return VIA_E_INVALID;
@@ -317,6 +369,37 @@ via_status via_server_submit(via_server *s, uint64_t client_id, const uint64_t *
     }
     s->cv_work.notify_one();
     return VIA_OK;
+}
+
+via_status via_server_submit(via_server *s, uint64_t client_id, const uint64_t *lwes, size_t n_lwe,
+                             uint64_t *ticket) {
+    if (!s || !lwes || !ticket) return VIA_E_INVALID;
+    if (n_lwe != s->n_lwe) return VIA_E_QUERY_LEN;
+    Request r;
+    r.client = client_id;
+    r.lwes.assign(lwes, lwes + s->lwe_words);
+    r.t_submit = Clock::now();
+    return enqueue(s, std::move(r), client_id, ticket);
+}
+
+via_status via_server_submit_batch(via_server *s, uint64_t client_id, uint32_t T, const uint64_t *lwes,
+                                   size_t n_lwe, uint64_t *ticket) {
+    if (!s || !lwes || !ticket || !s->db_b) return VIA_E_INVALID;
+    if ((size_t)T * s->lwe_words_b > s->q_cap || T == 0) return VIA_E_INVALID;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->clients.find(client_id);
+        if (it == s->clients.end()) return VIA_E_INVALID;
+        via_status st = check_vb(s->db_b, it->second, T);
+        if (st) return st;
+    }
+    if (n_lwe != s->n_lwe_b) return VIA_E_QUERY_LEN;
+    Request r;
+    r.client = client_id;
+    r.T = T;
+    r.lwes.assign(lwes, lwes + (size_t)T * s->lwe_words_b);
+    r.t_submit = Clock::now();
+    return enqueue(s, std::move(r), client_id, ticket);
 }
 
 via_status via_server_poll(via_server *s, uint64_t ticket, int *done) {
@@ -354,21 +437,26 @@ via_status via_server_get_stats(via_server *s, via_server_stats *out) {
     out->clients = s->clients.size();
     out->key_bytes = s->key_bytes;
     out->pending = s->pending.size();
-    std::vector<double> l = s->latencies;
-    if (!l.empty()) {
+    out->records = s->records;
+    out->answered_b = s->answered_b;
+    auto pcts = [](std::vector<double> l, double *p50, double *p99) {
+        if (l.empty()) return;
         std::sort(l.begin(), l.end());
         auto pct = [&](double p) { return l[std::min(l.size() - 1, (size_t)(p * (l.size() - 1) + 0.5))]; };
-        out->p50_s = pct(0.50);
-        out->p99_s = pct(0.99);
-    }
+        *p50 = pct(0.50);
+        *p99 = pct(0.99);
+    };
+    pcts(s->latencies, &out->p50_s, &out->p99_s);
+    pcts(s->latencies_b, &out->p50_b_s, &out->p99_b_s);
     return VIA_OK;
 }
 
 via_status via_server_reset_stats(via_server *s) {
     if (!s) return VIA_E_INVALID;
     std::lock_guard<std::mutex> g(s->mu);
-    s->answered = s->batches = s->max_seen_batch = 0;
+    s->answered = s->batches = s->max_seen_batch = s->records = s->answered_b = 0;
     s->latencies.clear();
+    s->latencies_b.clear();
     return VIA_OK;
 }
 

@@ -225,7 +225,12 @@ via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_
     Workspace &w = c->ws;
     size_t I = pp->num_rows, J = pp->num_cols;
     if (w.Q >= Q && w.B >= B && w.I >= I && w.J >= J && w.bits >= bits && w.lwe_f) return VIA_OK;
+    // grow every dimension, so that alternating shapes settle on one workspace
     Q = std::max(Q, w.Q);
+    B = std::max(B, w.B);
+    I = std::max(I, w.I);
+    J = std::max(J, w.J);
+    bits = std::max(bits, w.bits);
     free_ws(w);
     const size_t n = pp->n1, nl = 2, S = kSlots, lq = pp->l_query;
     w.Q = Q;
@@ -314,6 +319,23 @@ via_status ensure_rw(via_ctx *c, const via_params *pp, int T, size_t n_lwe) {
 via_status via::ensure_workspace(via_ctx *c, const via_params *pp, size_t Q, size_t n_lwe) {
     const size_t bits = n_lwe / pp->l_query;
     return ensure_ws(c, pp, Q, n_lwe, bits);
+}
+
+via_status via::ensure_repack_workspace(via_ctx *c, const via_params *pp, int T, size_t n_lwe) {
+    return ensure_rw(c, pp, T, n_lwe);
+}
+
+size_t via::keys_footprint(const via_params *pp) {
+    const size_t n1 = pp->n1, L = pp->l_ck;
+    size_t words = 0;
+    for (size_t nout = 2; nout <= n1; nout <<= 1) {
+        const size_t step = (2 * n1 / nout) * L * 2 * nout;  // one limb
+        words += 2 * step;
+        if (pp->n_rec < pp->n2 && nout >= 2 * (size_t)pp->n_rec) words += step;  // q2 repack copy
+    }
+    words += L * 2 * 2 * n1;                               // conversion key
+    words += (n1 / pp->n2) * pp->l_rsk * 2 * pp->n2;       // ring-switch key
+    return sizeof(double) * words;
 }
 
 via_status via::prepare_tables(via_ctx *c, const via_params *pp) {
@@ -624,14 +646,18 @@ void via_keys_destroy(via_keys *k) {
 
 namespace {
 
+}  // namespace
+
 // Client parameters agree apart from the record degree, which belongs to the
 // database (the reference's N_REC is a server type parameter).
-bool same_client_params(const via_params &a, const via_params &b) {
+bool via::same_client_params(const via_params &a, const via_params &b) {
     return a.n1 == b.n1 && a.n2 == b.n2 && a.q1_nl == b.q1_nl && a.q1[0] == b.q1[0] && a.q1[1] == b.q1[1] &&
            a.q2 == b.q2 && a.q3 == b.q3 && a.q4 == b.q4 && a.p == b.p && a.b1 == b.b1 && a.b2 == b.b2 &&
            a.b_rsk == b.b_rsk && a.ck_base == b.ck_base && a.l_query == b.l_query && a.l_ck == b.l_ck &&
            a.l_rsk == b.l_rsk && a.num_rows == b.num_rows && a.num_cols == b.num_cols;
 }
+
+namespace {
 
 // answer_through_crot (answer.rs:83-228) for Q queries against one database,
 // every step batched across the queries; query q uses the keys of its own
@@ -647,6 +673,8 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     if (!is_pow2(I) || !is_pow2(J)) return VIA_E_DIM;
     for (int q = 0; q < Q; q++)
         if (!same_client_params(keys[q]->pp, keys[0]->pp)) return VIA_E_DIM;
+    for (int q = 0; q < Q; q++)
+        if (!d_lwes[q] || !keys[q]) return VIA_E_INVALID;
     if (db->pp.num_rows != pp->num_rows || db->pp.num_cols != pp->num_cols || db->pp.n1 != pp->n1 ||
         db->pp.q2 != pp->q2)
         return VIA_E_DIM;
@@ -1056,15 +1084,21 @@ via_status via_answer_many(via_ctx *c, const via_db *db, uint32_t nq, const via_
                             via_err_detail *err) {
     if (!c || !db || !keys || !lwes || !masks || !bodies || nq < 1) return VIA_E_INVALID;
     HIP_TRY(hipSetDevice(c->device));
-    const via_params *pp = &keys[0]->pp;
-    const int nd = ilog2(pp->num_rows), nc = ilog2(pp->num_cols), nr = ilog2(pp->n1 / pp->n_rec);
+    // the query length follows the database's record degree
+    via_params ep = keys[0]->pp;
+    ep.n_rec = db->pp.n_rec;
+    const via_params *pp = &ep;
     if (!is_pow2(pp->num_rows) || !is_pow2(pp->num_cols)) return VIA_E_DIM;
+    const int nd = ilog2(pp->num_rows), nc = ilog2(pp->num_cols), nr = ilog2(pp->n1 / pp->n_rec);
     const size_t bits = (size_t)(nd + nc + nr);
     const size_t expected = bits * pp->l_query;
-    if (n_lwe != expected) {
-        const uint64_t *dummy[1] = {nullptr};
-        uint64_t *od[1] = {nullptr};
-        return answer_batch_impl(c, db, 1, keys, dummy, n_lwe, od, od, err);
+    if (n_lwe != expected) {  // answer.rs:132-157
+        if (err) {
+            err->expected = expected;
+            err->got = n_lwe;
+            snprintf(err->msg, sizeof err->msg, "query length mismatch: expected %zu LWEs, got %zu", expected, n_lwe);
+        }
+        return VIA_E_QUERY_LEN;
     }
     const size_t lw = (size_t)2 * (pp->n1 + 1);
     for (uint32_t q0 = 0; q0 < nq; q0 += kMaxQ) {

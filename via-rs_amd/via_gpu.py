@@ -77,7 +77,8 @@ class ErrDetail(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("answered", C.c_uint64), ("batches", C.c_uint64), ("max_batch_seen", C.c_uint64),
                 ("clients", C.c_uint64), ("key_bytes", C.c_uint64), ("pending", C.c_uint64),
-                ("p50_s", C.c_double), ("p99_s", C.c_double)]
+                ("records", C.c_uint64), ("answered_b", C.c_uint64),
+                ("p50_s", C.c_double), ("p99_s", C.c_double), ("p50_b_s", C.c_double), ("p99_b_s", C.c_double)]
 
 
 class ViaError(Exception):
@@ -132,7 +133,9 @@ def lib():
             "via_copy_to_device": (C.c_int, [vp, vp, vp, C.c_size_t]),
             "via_copy_to_host": (C.c_int, [vp, vp, vp, C.c_size_t]),
             "via_status_str": (C.c_char_p, [C.c_int]),
-            "via_server_create": (C.c_int, [C.c_int, P, vp, C.c_size_t, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
+            "via_server_create": (C.c_int, [C.c_int, P, vp, vp, C.c_size_t, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
+            "via_server_submit_batch": (C.c_int, [vp, C.c_uint64, C.c_uint32, u64p, C.c_size_t,
+                                                  C.POINTER(C.c_uint64)]),
             "via_server_destroy": (None, [vp]),
             "via_server_admit": (C.c_int, [vp, C.c_uint64, u64p, u64p, u64p]),
             "via_server_evict": (C.c_int, [vp, C.c_uint64]),
@@ -368,12 +371,15 @@ def answer_many(ctx, db, keys_list, queries, pp):
 class ServeLoop:
     """Continuous-batching request loop (via_server_*): clients admitted under
     an HBM key budget, queries submitted at any time, batches of up to
-    max_batch queries launched on `slots` device contexts as they free up."""
+    max_batch queries launched on `slots` device contexts as they free up.
+    With `db_b` (a database at a VIA-B record degree) the loop also takes
+    VIA-B batches (submit_batch), each answered as answer_batch::<T>."""
 
-    def __init__(self, pp, db, key_budget_bytes, max_batch=16, slots=2, device=0):
+    def __init__(self, pp, db, key_budget_bytes, max_batch=16, slots=2, device=0, db_b=None):
         h = C.c_void_p()
-        _check(lib().via_server_create(device, C.byref(pp), db.h, key_budget_bytes, max_batch, slots, C.byref(h)))
-        self.h, self.pp, self.db = h, pp, db
+        _check(lib().via_server_create(device, C.byref(pp), db.h, db_b.h if db_b is not None else None,
+                                       key_budget_bytes, max_batch, slots, C.byref(h)))
+        self.h, self.pp, self.db, self.db_b = h, pp, db, db_b
 
     def close(self):
         if self.h:
@@ -397,6 +403,14 @@ class ServeLoop:
         q = _u64(query)
         t = C.c_uint64()
         _check(lib().via_server_submit(self.h, client_id, _p(q), q.size // self.pp.lwe_words, C.byref(t)))
+        return t.value
+
+    def submit_batch(self, client_id, T, queries):
+        """One VIA-B request: the T queries (flat LWE arrays) of one client."""
+        q = np.ascontiguousarray(np.concatenate([_u64(x) for x in queries]))
+        t = C.c_uint64()
+        n_lwe = q.size // (T * self.pp.lwe_words)
+        _check(lib().via_server_submit_batch(self.h, client_id, T, _p(q), n_lwe, C.byref(t)))
         return t.value
 
     def done(self, ticket):
