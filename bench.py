@@ -248,6 +248,88 @@ def vb_run(via_gpu, args):
     ctx.close()
 
 
+NTT_PRIMES = (268369921, 536608769)
+
+
+def ntt_run(via_gpu, args):
+    """Batched negacyclic NTT -> pointwise -> iNTT: `--polys` RNS polynomial
+    pairs at n=2048 over the two primes 268369921 and 536608769 (both = 1 mod
+    4096), residues uniform below each prime from a seed, u64 storage. One step
+    is one launch computing every product c = iNTT(NTT(a) * NTT(b)); its time
+    comes from HIP events on the library stream around each launch."""
+    n, P = 2048, args.polys
+    rng = np.random.default_rng(2048)
+    a = np.empty((P, 2, n), np.uint64)
+    b = np.empty((P, 2, n), np.uint64)
+    for l, q in enumerate(NTT_PRIMES):
+        a[:, l, :] = rng.integers(0, q, size=(P, n), dtype=np.uint64)
+        b[:, l, :] = rng.integers(0, q, size=(P, n), dtype=np.uint64)
+    ctx = via_gpu.Context(0)
+    nb = a.nbytes
+    da, db, dc = ctx.alloc(nb), ctx.alloc(nb), ctx.alloc(nb)
+    ctx.to_device(da, a)
+    ctx.to_device(db, b)
+    count = 2 * P  # limb polys; poly p is limb p % 2 (one RNS poly = [limb0][limb1])
+    for _ in range(args.warmup):
+        ctx.poly_mul_device(n, NTT_PRIMES, 1, da, db, dc, count)
+    ctx.synchronize()
+    kt = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.mark(0)
+        ctx.poly_mul_device(n, NTT_PRIMES, 1, da, db, dc, count)
+        ctx.mark(1)
+        kt.append(ctx.marked_time())
+    elapsed = time.perf_counter() - t_start
+    c = ctx.to_host(np.zeros_like(a), dc)
+    if args.calib_copy:
+        ctx.copy_device(dc, da, nb)
+        ctx.synchronize()
+    k_s = float(np.mean(kt))
+    bytes_ = 3 * nb
+    logn = 11
+    bfly = 3 * count * (n // 2) * logn
+    modmul = bfly + 2 * count * n  # butterflies + pointwise + N^-1 scaling
+    out = {
+        "metric": "negacyclic NTT->pointwise->iNTT products/sec (n=2048, two-prime RNS), bit-exact vs CPU",
+        "value": round(P / k_s, 1), "unit": "RNS products/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "kernel_ms": round(k_s * 1e3, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int (exact modular, residues < 2^30 in fp64 lanes)",
+        "data": f"synthetic: residues uniform below each prime, seed 2048",
+        "config": {"workload": f"{P} polynomial pairs x n={n} x primes {NTT_PRIMES[0]}, {NTT_PRIMES[1]}; "
+                               f"c = iNTT(NTT(a) . NTT(b)) per prime", "polys": P, "n": n,
+                   "primes": list(NTT_PRIMES)},
+        "roofline": {"kernel": "k_polymul (fused forward NTTs, pointwise product, inverse NTT)", "bound": "hbm",
+                     "achieved": round(bytes_ / k_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bytes_ / k_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": bytes_},
+        "modmul": {"per_launch": modmul, "butterflies": bfly, "achieved_G_per_s": round(modmul / k_s / 1e9, 1)},
+    }
+    if not args.no_cpu:
+        from oracle.vo import Oracle, Ring, u64p
+
+        o = Oracle()
+        r = Ring.make(n, *NTT_PRIMES)
+        S = min(P, args.ntt_sample)
+        pick = np.random.default_rng(7).choice(P, size=S, replace=False)
+        want = np.zeros((S, 2, n), np.uint64)
+        asub = np.ascontiguousarray(a[pick])
+        bsub = np.ascontiguousarray(b[pick])
+        t0 = time.perf_counter()
+        for i in range(S):
+            o.poly_mul(r, want[i].ctypes.data_as(u64p), asub[i].ctypes.data_as(u64p), bsub[i].ctypes.data_as(u64p))
+        t_cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(S / t_cpu, 2), "unit": "RNS products/s", "cores": 1, "kind": "port",
+                               "sample": f"{S} of the {P} products (C restatement of ring/ntt.rs, one thread, "
+                                         f"{t_cpu:.2f} s)"}
+        out["bit_exact_vs_cpu"] = bool(np.array_equal(want, c[pick]))
+    print(json.dumps(out))
+    for d in (da, db, dc):
+        ctx.free(d)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,6 +346,12 @@ def main():
     ap.add_argument("--mix", action="store_true", help="serving: 30%% VIA-B requests against a second database")
     ap.add_argument("--vb", type=int, default=0, help="VIA-B batch size T (answer_batch), 0: VIA-C")
     ap.add_argument("--indices", type=int, default=64, help="VIA-B indices per step")
+    ap.add_argument("--ntt", action="store_true", help="NTT->pointwise->iNTT microbench")
+    ap.add_argument("--polys", type=int, default=8192, help="NTT microbench: RNS polynomial pairs")
+    ap.add_argument("--ntt-sample", type=int, default=8192, help="NTT microbench: products checked on the CPU")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after the timed region, one device copy of known size (256 MiB; one NTT operand "
+                         "in --ntt mode): a plain copy kernel for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the answer path is single-GPU in this build "
@@ -275,6 +363,9 @@ def main():
     via_gpu = __graft_entry__.load_package()
     if args.vb:
         vb_run(via_gpu, args)
+        return
+    if args.ntt:
+        ntt_run(via_gpu, args)
         return
     pp = via_gpu.paper_params(args.rows, args.cols)
     cascade, conv, rsk, lwes, record_bytes, n_rec = synthetic_inputs(pp)
@@ -355,6 +446,13 @@ def main():
         steps_acc += np.array(st)
         fd_ms.append(ctx.first_dim_time() * 1e3)
     elapsed = time.perf_counter() - t_start
+    if args.calib_copy:
+        cb = 256 << 20
+        c_src, c_dst = ctx.alloc(cb), ctx.alloc(cb)
+        ctx.copy_device(c_dst, c_src, cb)
+        ctx.synchronize()
+        ctx.free(c_src)
+        ctx.free(c_dst)
     gpu_mask = ctx.to_host(np.zeros(pp.n2, np.uint64), d_m[0])
     gpu_body = ctx.to_host(np.zeros(pp.n2, np.uint64), d_b[0])
 

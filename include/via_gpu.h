@@ -72,6 +72,12 @@ via_status via_ctx_step_times(via_ctx *ctx, double *t7);
 /* Seconds of the last FirstDim database-stream kernel alone (HIP events on
  * the context stream, around the one launch that reads the I*J cells). */
 via_status via_ctx_first_dim_time(via_ctx *ctx, double *seconds);
+/* Stream timer: which = 0 records the start event, 1 the stop event, on the
+ * context stream; via_ctx_marked_time waits for the stop event and returns
+ * the seconds between them. No counterpart in the reference (its benches
+ * time with criterion on the host). */
+via_status via_ctx_mark(via_ctx *ctx, int which);
+via_status via_ctx_marked_time(via_ctx *ctx, double *seconds);
 
 /* Server::setup, database half: setup_db + PreparedDb::from_encoded
  * (answer.rs:486-528, setup_db.rs:39-75, prepared_db.rs:57-86). Records are
@@ -206,6 +212,8 @@ via_status via_device_alloc(via_ctx *ctx, size_t bytes, void **d_ptr);
 via_status via_device_free(via_ctx *ctx, void *d_ptr);
 via_status via_copy_to_device(via_ctx *ctx, void *d_dst, const void *src, size_t bytes);
 via_status via_copy_to_host(via_ctx *ctx, void *dst, const void *d_src, size_t bytes);
+/* Device-to-device copy enqueued on the context stream (asynchronous). */
+via_status via_copy_device(via_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
 
 const char *via_status_str(via_status s);
 

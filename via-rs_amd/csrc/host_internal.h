@@ -54,6 +54,7 @@ struct via_ctx {
     RepackWs rw;
     hipEvent_t ev[8] = {};
     hipEvent_t ev_fd[2] = {};
+    hipEvent_t ev_mark[2] = {};  // via_ctx_mark start / stop
     bool have_times = false;
     // pointers to the last answer's stage buffers
     const double *stage_ptr[7] = {};

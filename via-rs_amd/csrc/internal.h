@@ -13,6 +13,9 @@ constexpr int kSlots = 2048;    // coefficients handled by one workgroup
 constexpr int kThreads = 256;   // workgroup size of the polynomial kernels
 constexpr int kPer = kSlots / kThreads;
 constexpr int kMaxQ = 32;       // queries answered together in one batched launch
+// NTT moduli stay below 2^38: the lazy transforms keep residues below 64q,
+// exact in fp64 lanes (ntt_lazy.h).
+constexpr uint64_t kMaxModulus = 1ULL << 38;
 
 // Twiddles of one (q, n): bit-reversed psi^k and psi^-k as exact doubles,
 // the layout of via-rs ring/ntt.rs:206-221.
@@ -59,6 +62,10 @@ void launch_polymul_u64(const RingDev &r, const uint64_t *a, const uint64_t *b, 
 // Digits of the LWE masks of `items` LWEs (int16 planes [item][k][n]).
 void launch_lwe_digits(const RingDev &r1, const Gadget &g, const DigitPlan &dp, const double *lwe, int items,
                        int16_t *dig, hipStream_t s);
+// Digits of `items` vectors of n coefficients (limb l of item i at
+// src[i*item_stride + l*limb_stride]) into int16 planes [item][k][n].
+void launch_poly_digits(const RingDev &r1, const Gadget &g, const DigitPlan &dp, const double *src, size_t item_stride,
+                        size_t limb_stride, int n, int items, int16_t *dig, hipStream_t s);
 
 // Gadget-product multiply-accumulate over digit planes.
 struct MacArgs {
@@ -78,7 +85,8 @@ struct MacArgs {
 void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t s);
 
 // One cascade combine: masks_next = -iNTT(sum pm), body_next =
-// embed(body_prev) - iNTT(sum pb), digits of masks_next into dig_next.
+// embed(body_prev) - iNTT(sum pb), then the digits of masks_next into
+// dig_next (rq: the q1 ring whose RNS and gadget constants the scale uses).
 struct CombineArgs {
     const double *pm, *pb;    // MAC outputs with tiles == 2
     int groups, items;

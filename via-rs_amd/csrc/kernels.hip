@@ -9,25 +9,35 @@ namespace via {
 
 // ---- LWE -> RLWE cascade (conv.rs:62-95, cascade.rs:388-405) -----------------------
 
-// Digits of the LWE masks, the rank-n degree-1 input of the first conv step.
+// Gadget digits of `items` coefficient vectors of n coefficients: item i,
+// limb l, coefficient c at src[i*item_stride + l*limb_stride + c], digits to
+// int16 planes dig[i][k][n]. One thread per coefficient over the whole grid
+// (the wide gadget scale dominates; the combine kernels that produce the
+// inputs run one workgroup per polynomial). Used for the LWE masks (the
+// rank-n degree-1 input of the first conv step) and every cascade output.
 template <int NL>
-__global__ __launch_bounds__(256) void k_lwe_digits(RingDev r1, Gadget g, DigitPlan dp, const double *lwe, int n,
-                                                    int16_t *dig) {
+__global__ __launch_bounds__(256) void k_poly_digits(RingDev r1, Gadget g, DigitPlan dp, const double *src,
+                                                     size_t item_stride, size_t limb_stride, int n, int16_t *dig) {
     const int item = blockIdx.y;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    const double *x = lwe + (size_t)item * NL * (n + 1);
-    const double a0 = x[c], a1 = NL == 2 ? x[(n + 1) + c] : 0.0;
+    const double *x = src + (size_t)item * item_stride;
+    const double a0 = x[c], a1 = NL == 2 ? x[limb_stride + c] : 0.0;
     write_digits<NL>(a0, a1, r1, g, dp, dig + (size_t)item * g.L * n + c, (size_t)n);
+}
+
+void launch_poly_digits(const RingDev &r1, const Gadget &g, const DigitPlan &dp, const double *src, size_t item_stride,
+                        size_t limb_stride, int n, int items, int16_t *dig, hipStream_t st) {
+    dim3 grid((n + 255) / 256, items);
+    if (r1.nl == 2)
+        hipLaunchKernelGGL(k_poly_digits<2>, grid, dim3(256), 0, st, r1, g, dp, src, item_stride, limb_stride, n, dig);
+    else
+        hipLaunchKernelGGL(k_poly_digits<1>, grid, dim3(256), 0, st, r1, g, dp, src, item_stride, limb_stride, n, dig);
 }
 
 void launch_lwe_digits(const RingDev &r1, const Gadget &g, const DigitPlan &dp, const double *lwe, int items,
                        int16_t *dig, hipStream_t st) {
-    dim3 grid((r1.n + 255) / 256, items);
-    if (r1.nl == 2)
-        hipLaunchKernelGGL(k_lwe_digits<2>, grid, dim3(256), 0, st, r1, g, dp, lwe, r1.n, dig);
-    else
-        hipLaunchKernelGGL(k_lwe_digits<1>, grid, dim3(256), 0, st, r1, g, dp, lwe, r1.n, dig);
+    launch_poly_digits(r1, g, dp, lwe, (size_t)r1.nl * (r1.n + 1), (size_t)r1.n + 1, r1.n, items, dig, st);
 }
 
 // Gadget-product multiply-accumulate over a digit range (RLevEval::
@@ -55,9 +65,10 @@ __global__ __launch_bounds__(kThreads) void k_gadget_mac(RingDev r, Gadget g, Ma
     for (int k = k0; k < k1; k++) {
         const int16_t *dk = dig + (size_t)k * a.dig_coeffs;
         for (int c = threadIdx.x; c < cin; c += blockDim.x) {
-            const int32_t d = dk[c];
-            s[c] = digit_residue(d, m0);
-            if (NL == 2) s[cin + c] = digit_residue(d, m1);
+            // signed digits are valid NTT inputs (the lazy transform takes (-2q, 2q))
+            const double d = (double)dk[c];
+            s[c] = d;
+            if (NL == 2) s[cin + c] = d;
         }
         __syncthreads();
         ntt_fwd<NL>(s, cin, nt, lognt, T);
@@ -118,10 +129,10 @@ void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipS
 }
 
 // Cascade combine: masks_next = -iNTT(sum of mask partials), body_next =
-// embed(body_prev) - iNTT(sum of body partials); then the digits of
-// masks_next for the next gadget product. Workgroup (item, chunk) handles the
-// chunk's mask polynomials (chunks of at least one degree-n polynomial);
-// chunk 0 also handles the body.
+// embed(body_prev) - iNTT(sum of body partials); the digits of masks_next for
+// the next gadget product follow in k_poly_digits. Workgroup (item, chunk)
+// handles the chunk's mask polynomials (chunks of at least one degree-n
+// polynomial); chunk 0 also handles the body.
 template <int NL>
 __global__ __launch_bounds__(kThreads) void k_cascade_combine(RingDev r, Gadget g, DigitPlan dp, CombineArgs a) {
     __shared__ double s[NL * kSlots];
@@ -151,7 +162,6 @@ __global__ __launch_bounds__(kThreads) void k_cascade_combine(RingDev r, Gadget 
         double *mo = a.masks_next + (size_t)item * NL * S + c0 + c;
         mo[0] = x0;
         if (NL == 2) mo[S] = x1;
-        if (a.dig_next) write_digits<NL>(x0, x1, r, g, dp, a.dig_next + (size_t)item * g.L * S + c0 + c, (size_t)S);
     }
     if (chunk != 0) return;
     __syncthreads();
@@ -178,7 +188,6 @@ __global__ __launch_bounds__(kThreads) void k_cascade_combine(RingDev r, Gadget 
 
 void launch_cascade_combine(const RingDev &r, const Gadget &g, const DigitPlan &dp, const RingDev &rq,
                             const CombineArgs &a, hipStream_t st) {
-    (void)rq;
     int chunks = kSlots / r.n;
     if (chunks > 8) chunks = 8;
     dim3 grid(a.items, chunks);
@@ -186,6 +195,10 @@ void launch_cascade_combine(const RingDev &r, const Gadget &g, const DigitPlan &
         hipLaunchKernelGGL(k_cascade_combine<2>, grid, dim3(kThreads), 0, st, r, g, dp, a);
     else
         hipLaunchKernelGGL(k_cascade_combine<1>, grid, dim3(kThreads), 0, st, r, g, dp, a);
+    // digits of every output coefficient (rq: the q1 ring at n1, whose RNS
+    // and gadget constants the scale uses)
+    if (a.dig_next)
+        launch_poly_digits(rq, g, dp, a.masks_next, (size_t)r.nl * kSlots, kSlots, kSlots, a.items, a.dig_next, st);
 }
 
 // ---- rlwe_to_rgsw combine (convert.rs:162-180) -----------------------------------
@@ -196,6 +209,7 @@ __global__ __launch_bounds__(kThreads) void k_r2g_combine(RingDev r, const doubl
                                                           const double *rlwe_bodies, double *rgsw, int lq) {
     __shared__ double s[NL * kSlots];
     const int item = blockIdx.x;  // bit * lq + i
+    const int part = blockIdx.y;  // 0: mask of neg_s_m[i] (and the copy of m[i]), 1: its body
     const int bit = item / lq, i = item - bit * lq;
     const int n = r.n;  // == kSlots
     const FMod m0 = r.t[0].m, m1 = r.t[1].m;
@@ -204,28 +218,26 @@ __global__ __launch_bounds__(kThreads) void k_r2g_combine(RingDev r, const doubl
     double *nsm = gp + (size_t)i * 2 * pw;        // neg_s_m[i]
     double *mm = gp + (size_t)(lq + i) * 2 * pw;  // m[i]
     const double *rm = rlwe_masks + (size_t)item * pw, *rb = rlwe_bodies + (size_t)item * pw;
-    for (int part = 0; part < 2; part++) {
-        const double *src = (part ? pb : pm) + (size_t)item * pw;
-        for (int t = threadIdx.x; t < (int)pw; t += blockDim.x) {
-            const FMod md = t >= n ? m1 : m0;
-            double acc = src[t];
-            for (int gi = 1; gi < groups; gi++) acc = mod_add(acc, src[(size_t)gi * items * pw + t], md);
-            s[t] = acc;
-        }
-        __syncthreads();
-        lds_ntt_inv_noscale<NL>(s, n, r);
-        for (int t = threadIdx.x; t < (int)pw; t += blockDim.x) {
-            const int l = t >= n ? 1 : 0;
-            const FMod md = l ? m1 : m0;
-            const double x = mod_mul(s[t], l ? r.t[1].n_inv : r.t[0].n_inv, md);
-            if (part == 0) nsm[t] = mod_add(rb[t], x, md);
-            else nsm[pw + t] = x;
-        }
-        __syncthreads();
-    }
+    const double *src = (part ? pb : pm) + (size_t)item * pw;
     for (int t = threadIdx.x; t < (int)pw; t += blockDim.x) {
-        mm[t] = rm[t];
-        mm[pw + t] = rb[t];
+        const FMod md = t >= n ? m1 : m0;
+        double acc = src[t];
+        for (int gi = 1; gi < groups; gi++) acc = mod_add(acc, src[(size_t)gi * items * pw + t], md);
+        s[t] = acc;
+    }
+    __syncthreads();
+    lds_ntt_inv_noscale<NL>(s, n, r);
+    for (int t = threadIdx.x; t < (int)pw; t += blockDim.x) {
+        const int l = t >= n ? 1 : 0;
+        const FMod md = l ? m1 : m0;
+        const double x = mod_mul(s[t], l ? r.t[1].n_inv : r.t[0].n_inv, md);
+        if (part == 0) {
+            nsm[t] = mod_add(rb[t], x, md);
+            mm[t] = rm[t];
+            mm[pw + t] = rb[t];
+        } else {
+            nsm[pw + t] = x;
+        }
     }
 }
 
@@ -233,11 +245,11 @@ void launch_r2g_combine(const RingDev &r, const double *pm, const double *pb, in
                         const double *rlwe_bodies, double *rgsw, int bits, int lq, hipStream_t st) {
     const int items = bits * lq;
     if (r.nl == 2)
-        hipLaunchKernelGGL(k_r2g_combine<2>, dim3(items), dim3(kThreads), 0, st, r, pm, pb, groups, items, rlwe_masks,
-                           rlwe_bodies, rgsw, lq);
+        hipLaunchKernelGGL(k_r2g_combine<2>, dim3(items, 2), dim3(kThreads), 0, st, r, pm, pb, groups, items,
+                           rlwe_masks, rlwe_bodies, rgsw, lq);
     else
-        hipLaunchKernelGGL(k_r2g_combine<1>, dim3(items), dim3(kThreads), 0, st, r, pm, pb, groups, items, rlwe_masks,
-                           rlwe_bodies, rgsw, lq);
+        hipLaunchKernelGGL(k_r2g_combine<1>, dim3(items, 2), dim3(kThreads), 0, st, r, pm, pb, groups, items,
+                           rlwe_masks, rlwe_bodies, rgsw, lq);
 }
 
 // ---- gates: external product with DMux / CMux / CRot combine -------------------
@@ -307,9 +319,9 @@ __global__ __launch_bounds__(TPB) void k_gate(RingDev r, Gadget g, GateArgs ga) 
 #pragma unroll
             for (int e = 0; e < PER; e++) {
                 const int sg = threadIdx.x + e * TPB;
-                const int32_t d = extract_digit(v[e], g.base);
-                s[sg] = digit_residue(d, m0);
-                if (NL == 2) s[S + sg] = digit_residue(d, m1);
+                const double d = (double)extract_digit(v[e], g.base);
+                s[sg] = d;
+                if (NL == 2) s[S + sg] = d;
             }
             __syncthreads();
             lds_ntt_fwd<NL>(s, S, r);
@@ -364,8 +376,120 @@ __global__ __launch_bounds__(TPB) void k_gate(RingDev r, Gadget g, GateArgs ga) 
     }
 }
 
+// The same gate with the 2L <= 4 digit transforms of one item running at
+// once: 1024 threads, one LDS buffer per (half, digit) polynomial. Phase 1
+// scales each input coefficient once and writes its L digits; phase 2
+// transforms the 2L digit polynomials concurrently (1024 / 2L threads each);
+// phase 3 multiplies and sums against the RGSW rows; phase 4 runs the two
+// inverse transforms (mask and body parts) concurrently; phase 5 combines.
+// The critical path is one forward and one inverse transform instead of
+// 2L + 2 in sequence, which is what the narrow tree levels wait on.
+template <int NL>
+__global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateArgs ga) {
+    constexpr int S = kSlots;
+    constexpr int TPB = 1024;
+    __shared__ double s[4 * NL * S];
+    const int q = blockIdx.x / ga.items_per_query;
+    const int j = blockIdx.x - q * ga.items_per_query;
+    const int mode = ga.mode, shift = ga.shift;
+    const double *rgsw = ga.rgsw + (size_t)q * ga.rgsw_qstride;
+    const double *in = ga.in + (size_t)q * ga.in_qstride;
+    double *out = ga.out + (size_t)q * ga.out_qstride;
+    const int n = r.n;
+    const size_t pw = (size_t)NL * n;
+    const FMod m0 = r.t[0].m, m1 = r.t[1].m;
+    const int L = g.L;
+    const int tid = threadIdx.x;
+    // phase 1: digits of ct's two halves (0: mask, with -s*m; 1: body, with m)
+    {
+        const int half = tid >> 9;
+        for (int e = 0; e < S / 512; e++) {
+            const int c = (tid & 511) + 512 * e;
+            double b0, c0, b1 = 0.0, c1 = 0.0;
+            gate_input<NL>(mode, in, j, half, 0, c, n, shift, m0, b0, c0);
+            if (NL == 2) gate_input<NL>(mode, in, j, half, 1, c, n, shift, m1, b1, c1);
+            SMag v = gadget_scale<NL>(c0, c1, r, g);
+            for (int k = 0; k < L; k++) {
+                const double d = (double)extract_digit(v, g.base);
+                double *buf = s + (size_t)(half * L + k) * NL * S;
+                buf[c] = d;
+                if (NL == 2) buf[S + c] = d;
+            }
+        }
+    }
+    __syncthreads();
+    // phase 2: the 2L forward transforms side by side
+    {
+        const int np = 2 * L;
+        const int per = TPB / np;
+        const int gi = tid / per;
+        ntt_fwd<NL>(s + (size_t)gi * NL * S, S, n, r.logn, limbs_fwd(r), tid - gi * per, per);
+    }
+    // phase 3: external product sums (both parts) for NL * S slots per part
+    constexpr int PER = NL * S / TPB;
+    double am[PER], ab[PER];
+#pragma unroll
+    for (int e = 0; e < PER; e++) {
+        const int t = tid + e * TPB;  // limb-major slot
+        const int l = NL == 2 && t >= S ? 1 : 0;
+        const int c = t - l * S;
+        const FMod md = l ? m1 : m0;
+        double sm = 0.0, sb = 0.0;
+        for (int half = 0; half < 2; half++)
+            for (int k = 0; k < L; k++) {
+                const double x = s[(size_t)(half * L + k) * NL * S + t];
+                const double *key = rgsw + ((size_t)half * L + (L - 1 - k)) * 2 * pw + (size_t)l * n + c;
+                sm += mod_mul_lazy(x, key[0], md);
+                sb += mod_mul_lazy(x, key[pw], md);
+            }
+        am[e] = mod_reduce_small(sm, md);
+        ab[e] = mod_reduce_small(sb, md);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER; e++) {
+        const int t = tid + e * TPB;
+        s[t] = am[e];
+        s[NL * S + t] = ab[e];
+    }
+    __syncthreads();
+    // phase 4: inverse transforms of the two parts side by side
+    {
+        const int part = tid >> 9;
+        ntt_inv_noscale<NL>(s + (size_t)part * NL * S, S, n, r.logn, limbs_inv(r), tid & 511, 512);
+    }
+    // phase 5: gate combine
+#pragma unroll
+    for (int part = 0; part < 2; part++)
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            const int t = tid + e * TPB;
+            const int l = NL == 2 && t >= S ? 1 : 0;
+            const int c = t - l * S;
+            const FMod md = l ? m1 : m0;
+            const double ni = l ? r.t[1].n_inv : r.t[0].n_inv;
+            const double p = mod_mul(s[(size_t)part * NL * S + t], ni, md);
+            double bv, cv;
+            gate_input<NL>(mode, in, j, part, l, c, n, shift, md, bv, cv);
+            const size_t o = (size_t)part * pw + (size_t)l * n + c;
+            if (mode == GATE_DMUX) {
+                out[(size_t)(2 * j) * 2 * pw + o] = mod_sub(cv, p, md);
+                out[(size_t)(2 * j + 1) * 2 * pw + o] = p;
+            } else {
+                out[(size_t)j * 2 * pw + o] = mod_add(bv, p, md);
+            }
+        }
+}
+
 void launch_gate(const RingDev &r, const Gadget &g, const GateArgs &a, hipStream_t st) {
     const int blocks = a.items_per_query * a.queries;
+    if (g.L <= 2 && r.n == kSlots) {
+        if (r.nl == 2)
+            hipLaunchKernelGGL(k_gate_fused<2>, dim3(blocks), dim3(1024), 0, st, r, g, a);
+        else
+            hipLaunchKernelGGL(k_gate_fused<1>, dim3(blocks), dim3(1024), 0, st, r, g, a);
+        return;
+    }
     const bool wide = blocks < 64;  // narrow levels: shorter per-workgroup latency
     if (r.nl == 2) {
         if (wide)
@@ -443,67 +567,72 @@ void launch_modswitch_q1_q2(const RingDev &r1, uint64_t q2, const double *in, do
 // ---- FirstDim (first_dim.rs:94-184) -------------------------------------------------
 
 // out[chunk][q][j][part][k] = sum over the chunk's rows i of db[i][j][k] *
-// sel[q][i][part][k] for QB queries sharing each database load; each thread
-// owns two adjacent slots k (16-byte loads) of TJ columns. Products are summed
-// lazily (exact below 2^53) and reduced once; chunks of rows give the launch
-// enough workgroups to keep HBM busy.
+// sel[q][i][part][k] for QB queries sharing each database load. A workgroup
+// is four wavefronts over the same 64 slots k (one per lane) and 4*TJ
+// columns (TJ per wavefront); each lane keeps 2*QB*TJ accumulators. The
+// database row of the next i is loaded while the current one is multiplied;
+// the selection vectors (shared by the four wavefronts) come through the L1
+// cache. Products are summed lazily
+// (exact below 2^53) and reduced once; chunks of rows give the launch enough
+// workgroups to keep HBM busy.
 template <int QB, int TJ>
 __global__ __launch_bounds__(256) void k_first_dim(const double *db, const double *sel, double *out, int I, int J,
                                                    int n, int Q, int q0, int rows_per_chunk, FMod m) {
-    const int k = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
-    const int j0 = blockIdx.y * TJ;
-    const int chunk = blockIdx.z;
-    if (k >= n) return;
-    const int i0 = chunk * rows_per_chunk;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane;
+    const int j0 = (blockIdx.y * 4 + wv) * TJ;
+    if (j0 >= J) return;  // narrow databases: J < 4 * TJ (TJ divides J)
+    const int i0 = blockIdx.z * rows_per_chunk;
     const int i1 = min(I, i0 + rows_per_chunk);
-    double am[QB][TJ][2], ab[QB][TJ][2];
+    double am[QB][TJ], ab[QB][TJ];
 #pragma unroll
     for (int qb = 0; qb < QB; qb++)
 #pragma unroll
-        for (int t = 0; t < TJ; t++) am[qb][t][0] = am[qb][t][1] = ab[qb][t][0] = ab[qb][t][1] = 0.0;
-    const int ncols = min(TJ, J - j0);
+        for (int t = 0; t < TJ; t++) am[qb][t] = ab[qb][t] = 0.0;
     const size_t qs = (size_t)I * 2 * n;
-#pragma unroll 2
-    for (int i = i0; i < i1; i++) {
-        double2 sm[QB], sb[QB];
+    const size_t rstride = (size_t)J * n;
+    const double *col = db + (size_t)j0 * n + k;
+    const double *selk = sel + (size_t)q0 * qs + k;
+    // Row i's loads: selection values first, then the database (streamed past
+    // the caches: it is read once per pass), so that waiting for a row's
+    // selection values never waits for a later row's database loads.
+    auto load_row = [&](int i, double (&d)[TJ], double (&sm)[QB], double (&sb)[QB]) {
 #pragma unroll
         for (int qb = 0; qb < QB; qb++) {
-            sm[qb] = *(const double2 *)(sel + (size_t)(q0 + qb) * qs + (size_t)(2 * i) * n + k);
-            sb[qb] = *(const double2 *)(sel + (size_t)(q0 + qb) * qs + (size_t)(2 * i + 1) * n + k);
+            sm[qb] = selk[(size_t)qb * qs + (size_t)(2 * i) * n];
+            sb[qb] = selk[(size_t)qb * qs + (size_t)(2 * i + 1) * n];
         }
-        const double *row = db + ((size_t)i * J + j0) * n + k;
 #pragma unroll
-        for (int t = 0; t < TJ; t++) {
-            if (t < ncols) {
-                // the database is read once per pass: stream it past the caches
-                double2 d;
-                d.x = __builtin_nontemporal_load(row + (size_t)t * n);
-                d.y = __builtin_nontemporal_load(row + (size_t)t * n + 1);
+        for (int t = 0; t < TJ; t++) d[t] = __builtin_nontemporal_load(col + (size_t)i * rstride + (size_t)t * n);
+    };
+    auto mac_row = [&](const double (&d)[TJ], const double (&sm)[QB], const double (&sb)[QB]) {
 #pragma unroll
-                for (int qb = 0; qb < QB; qb++) {
-                    am[qb][t][0] += mod_mul_lazy(d.x, sm[qb].x, m);
-                    am[qb][t][1] += mod_mul_lazy(d.y, sm[qb].y, m);
-                    ab[qb][t][0] += mod_mul_lazy(d.x, sb[qb].x, m);
-                    ab[qb][t][1] += mod_mul_lazy(d.y, sb[qb].y, m);
-                }
+        for (int qb = 0; qb < QB; qb++)
+#pragma unroll
+            for (int t = 0; t < TJ; t++) {
+                am[qb][t] += mod_mul_lazy(d[t], sm[qb], m);
+                ab[qb][t] += mod_mul_lazy(d[t], sb[qb], m);
             }
-        }
+    };
+    // two rows in registers: row i+1 loads while row i multiplies
+    double dA[TJ], smA[QB], sbA[QB], dB[TJ], smB[QB], sbB[QB];
+    int i = i0;
+    if (i < i1) load_row(i, dA, smA, sbA);
+    for (; i + 1 < i1; i += 2) {
+        load_row(i + 1, dB, smB, sbB);
+        mac_row(dA, smA, sbA);
+        load_row(min(i + 2, i1 - 1), dA, smA, sbA);  // clamped: the last reload is unused
+        mac_row(dB, smB, sbB);
     }
-    double *o = out + (size_t)chunk * Q * J * 2 * n;
+    if (i < i1) mac_row(dA, smA, sbA);
+    double *o = out + (size_t)blockIdx.z * Q * J * 2 * n;
 #pragma unroll
     for (int qb = 0; qb < QB; qb++)
 #pragma unroll
         for (int t = 0; t < TJ; t++) {
-            if (t < ncols) {
-                double2 vm, vb;
-                vm.x = mod_reduce_small(am[qb][t][0], m);
-                vm.y = mod_reduce_small(am[qb][t][1], m);
-                vb.x = mod_reduce_small(ab[qb][t][0], m);
-                vb.y = mod_reduce_small(ab[qb][t][1], m);
-                double *oq = o + (size_t)(q0 + qb) * J * 2 * n;
-                *(double2 *)(oq + ((size_t)(j0 + t) * 2 + 0) * n + k) = vm;
-                *(double2 *)(oq + ((size_t)(j0 + t) * 2 + 1) * n + k) = vb;
-            }
+            double *oq = o + (size_t)(q0 + qb) * J * 2 * n + ((size_t)(j0 + t) * 2) * n + k;
+            oq[0] = mod_reduce_small(am[qb][t], m);
+            oq[n] = mod_reduce_small(ab[qb][t], m);
         }
 }
 
@@ -516,36 +645,48 @@ __global__ void k_sum_chunks(double *part, size_t words, int chunks, FMod m) {
     }
 }
 
-static int fd_cols(int QB) { return QB <= 2 ? 4 : (QB == 4 ? 2 : 1); }
 static int fd_qb(int Q) { return Q >= 8 ? 8 : (Q >= 4 ? 4 : (Q >= 2 ? 2 : 1)); }
+// columns per wavefront: 4 while J allows (J is a power of two), 2 for eight
+// queries (their 2*8*TJ accumulators then fit 128 registers)
+static int fd_tj(int J, int QB) {
+    const int t = J >= 16 ? 4 : (J >= 8 ? 2 : 1);
+    return QB >= 8 && t > 2 ? 2 : t;
+}
 
 int first_dim_chunks(int I, int J, int n, int Q) {
-    // aim for about 8192 wavefronts of 64 threads per database pass (eight
-    // per SIMD), the occupancy at which the stream measured fastest
-    const int tj = fd_cols(fd_qb(Q));
-    const long threads = (long)(n / 2) * ((J + tj - 1) / tj);
+    // about 2048 workgroups (eight per CU) per database pass
+    const int tj = fd_tj(J, fd_qb(Q));
+    const long wgs = (long)(n / 64) * ((J + 4 * tj - 1) / (4 * tj));
     int chunks = 1;
-    while (chunks < I && threads * chunks < 8192L * 64) chunks *= 2;
+    while (chunks < I && wgs * chunks < 2048) chunks *= 2;
     return chunks;
 }
 
-template <int QB>
+template <int QB, int TJ>
 static void fd_pass(const double *db, const double *sel, double *out, int I, int J, int n, int Q, int q0, int chunks,
                     FMod m, hipStream_t st) {
-    constexpr int TJ = QB <= 2 ? 4 : (QB == 4 ? 2 : 1);
     const int rows = (I + chunks - 1) / chunks;
-    dim3 grid((n / 2 + 255) / 256, (J + TJ - 1) / TJ, chunks);
+    dim3 grid(n / 64, (J + 4 * TJ - 1) / (4 * TJ), chunks);
     hipLaunchKernelGGL((k_first_dim<QB, TJ>), grid, dim3(256), 0, st, db, sel, out, I, J, n, Q, q0, rows, m);
+}
+
+template <int QB>
+static void fd_pass_q(const double *db, const double *sel, double *out, int I, int J, int n, int Q, int q0, int chunks,
+                      FMod m, hipStream_t st) {
+    const int tj = fd_tj(J, QB);
+    if (QB < 8 && tj == 4) fd_pass<QB, (QB < 8 ? 4 : 2)>(db, sel, out, I, J, n, Q, q0, chunks, m, st);
+    else if (tj == 2) fd_pass<QB, 2>(db, sel, out, I, J, n, Q, q0, chunks, m, st);
+    else fd_pass<QB, 1>(db, sel, out, I, J, n, Q, q0, chunks, m, st);
 }
 
 void launch_first_dim(const double *db, const double *sel_eval, double *out_eval, int I, int J, int n, int Q,
                       int chunks, FMod m, hipStream_t st) {
     for (int q0 = 0; q0 < Q;) {
         const int qb = fd_qb(Q - q0);
-        if (qb == 8) fd_pass<8>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
-        else if (qb == 4) fd_pass<4>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
-        else if (qb == 2) fd_pass<2>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
-        else fd_pass<1>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
+        if (qb == 8) fd_pass_q<8>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
+        else if (qb == 4) fd_pass_q<4>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
+        else if (qb == 2) fd_pass_q<2>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
+        else fd_pass_q<1>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
         q0 += qb;
     }
     if (chunks > 1) {
@@ -599,7 +740,7 @@ __global__ __launch_bounds__(1024) void k_resp_comp(RespArgs a) {
         const double *key = a.rsk_eval[qy] + (size_t)j * kw;
         for (int k = 0; k < a.l_rsk; k++) {
             const int32_t dg = extract_digit(v, a.b_rsk);
-            buf[i] = digit_residue(dg, m3);
+            buf[i] = (double)dg;
             __syncthreads();
             lds_ntt_fwd<1>(buf, n2, r);
             const int ks = a.l_rsk - 1 - k;

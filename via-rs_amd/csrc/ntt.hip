@@ -22,7 +22,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_batch(RingDev r, double *data,
     __syncthreads();
     if (!INV) {
         lds_ntt_fwd<1>(s, S, r1);
-        for (int i = threadIdx.x; i < S; i += blockDim.x) g[i] = s[i];
+        for (int i = threadIdx.x; i < S; i += blockDim.x) g[i] = mod_reduce_small(s[i], r1.t[0].m);
     } else {
         lds_ntt_inv_noscale<1>(s, S, r1);
         const double ni = r1.t[0].n_inv;
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kThreads) void k_polymul(RingDev r, const uint64_t 
     __syncthreads();
     lds_ntt_fwd<2>(s, n, r2);
     const FMod md = r2.t[0].m;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s[i] = mod_mul(s[i], s[n + i], md);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s[i] = mod_mul(mod_reduce_small(s[i], md), s[n + i], md);
     __syncthreads();
     lds_ntt_inv_noscale<1>(s, n, r2);
     const double ni = r2.t[0].n_inv;

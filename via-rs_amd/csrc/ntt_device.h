@@ -5,8 +5,8 @@
 // (via-rs ring/ntt.rs:305-343). Inverse: decimation in frequency from
 // bit-reversed order, then the N^-1 scaling (ring/ntt.rs:356-399). Both are
 // exact, so the result is bit-identical to the reference; the stages run in
-// the reference order, two at a time in registers (radix-4 passes, see
-// ntt_index.h) with one barrier per pass.
+// the reference order, up to four at a time in registers (radix-16 passes
+// over lazy residues, ntt_lazy.h) with one barrier per pass.
 //
 // The LDS buffer holds NL limbs of S slots; each limb is S/n independent
 // polynomials of degree n (n a power of two, 2 <= n <= S). The twiddle
@@ -19,7 +19,7 @@
 #pragma once
 
 #include "internal.h"
-#include "ntt_index.h"
+#include "ntt_lazy.h"
 
 namespace via {
 
@@ -46,105 +46,95 @@ __device__ __forceinline__ NttLimbs limbs_inv(const RingDev &r) {
     return t;
 }
 
-// Forward transform of every degree-n segment of the NL x S buffer.
-template <int NL>
-__device__ void ntt_fwd(double *s, int S, int n, int logn, const NttLimbs &T) {
-    const int quarterS = S >> 2;
-    int lm = 0;
-    if (logn >= 2) {
-        const int un = n >> 2;  // units per segment
-        const int lun = logn - 2;
-        for (; lm + 1 < logn; lm += 2) {
-            for (int t = threadIdx.x; t < NL * quarterS; t += blockDim.x) {
-                const int l = (NL == 1) ? 0 : (t >= quarterS ? 1 : 0);
-                const int ua = t - l * quarterS;
-                const int seg = ua >> lun;
-                const int u = ua & (un - 1);
-                const R4Idx r = r4_fwd(u, lm, logn);
-                double *p = s + l * S + seg * n;
-                const double *tw = l ? T.tw[1] : T.tw[0];
-                const FMod md = l ? T.m[1] : T.m[0];
-                const double w1 = tw[r.w1];
-                const double a = p[r.a], b = p[r.b];
-                const double t1 = mod_mul(p[r.c], w1, md), t2 = mod_mul(p[r.d], w1, md);
-                const double a1 = mod_add(a, t1, md), c1 = mod_sub(a, t1, md);
-                const double b1 = mod_add(b, t2, md), d1 = mod_sub(b, t2, md);
-                const double t3 = mod_mul(b1, tw[r.w2], md), t4 = mod_mul(d1, tw[r.w3], md);
-                p[r.a] = mod_add(a1, t3, md);
-                p[r.b] = mod_sub(a1, t3, md);
-                p[r.c] = mod_add(c1, t4, md);
-                p[r.d] = mod_sub(c1, t4, md);
-            }
-            __syncthreads();
-        }
-    }
-    if (lm < logn) {  // last radix-2 stage (h = 1): pairs (2i, 2i+1)
-        const int m = 1 << lm;
-        const int halfS = S >> 1;
-        for (int t = threadIdx.x; t < NL * halfS; t += blockDim.x) {
-            const int l = (NL == 1) ? 0 : (t >= halfS ? 1 : 0);
-            const int u = t - l * halfS;
-            const int seg = u >> (logn - 1);
-            const int blk = u & ((n >> 1) - 1);
-            double *p = s + l * S + seg * n + 2 * blk;
+// LDS view for the pass kernels of ntt_lazy.h.
+struct LdsMem {
+    double *p;
+    __device__ __forceinline__ double ld(int i) const { return p[i]; }
+    __device__ __forceinline__ void st(int i, double v) const { p[i] = v; }
+};
+
+constexpr int kNttMaxR = 3;  // default stages per register pass (radix 8)
+
+// Forward transform of every degree-n segment of the NL x S buffer by threads
+// tid = 0 .. nth-1 (a subset of the workgroup may transform its own buffer;
+// every thread of the workgroup must reach the pass barriers). Inputs
+// are exact integers in (-2q, 2q); outputs are lazy, in (-24q, 24q) for
+// n <= 2048 (ntt_lazy.h): reduce before storing canonical residues.
+template <int NL, int MAXR = kNttMaxR>
+__device__ void ntt_fwd(double *s, int S, int n, int logn, const NttLimbs &T, int tid, int nth) {
+    int rs[12];
+    const int np = ntt_schedule(logn, MAXR, rs);
+    int L0 = 0;
+    for (int pi = 0; pi < np; pi++) {
+        const int R = rs[pi];
+        const int upl = S >> R;  // units per limb
+        const int useg = logn - R;
+        for (int t = tid; t < NL * upl; t += nth) {
+            const int l = (NL == 1) ? 0 : (t >= upl ? 1 : 0);
+            const int ua = t - l * upl;
+            const int seg = ua >> useg;
+            const int u = ua & ((1 << useg) - 1);
+            const LdsMem p{s + l * S + seg * n};
             const double *tw = l ? T.tw[1] : T.tw[0];
             const FMod md = l ? T.m[1] : T.m[0];
-            const double x = p[0];
-            const double y = mod_mul(p[1], tw[m + blk], md);
-            p[0] = mod_add(x, y, md);
-            p[1] = mod_sub(x, y, md);
+            if (MAXR >= 4 && R == 4)
+                ntt_fwd_unit<(MAXR >= 4 ? 4 : 1)>(p, logn, L0, u, tw, md);
+            else if (MAXR >= 3 && R == 3)
+                ntt_fwd_unit<(MAXR >= 3 ? 3 : 1)>(p, logn, L0, u, tw, md);
+            else if (R == 2)
+                ntt_fwd_unit<2>(p, logn, L0, u, tw, md);
+            else
+                ntt_fwd_unit<1>(p, logn, L0, u, tw, md);
         }
         __syncthreads();
+        L0 += R;
     }
 }
 
-// Inverse transform without the N^-1 scaling (callers fold it into their store).
-template <int NL>
-__device__ void ntt_inv_noscale(double *s, int S, int n, int logn, const NttLimbs &T) {
-    const int quarterS = S >> 2;
-    int lh = 0;
-    if (logn >= 2) {
-        const int un = n >> 2;
-        const int lun = logn - 2;
-        for (; lh + 1 < logn; lh += 2) {
-            for (int t = threadIdx.x; t < NL * quarterS; t += blockDim.x) {
-                const int l = (NL == 1) ? 0 : (t >= quarterS ? 1 : 0);
-                const int ua = t - l * quarterS;
-                const int seg = ua >> lun;
-                const int u = ua & (un - 1);
-                const R4Idx r = r4_inv(u, lh, logn);
-                double *p = s + l * S + seg * n;
-                const double *tw = l ? T.tw[1] : T.tw[0];
-                const FMod md = l ? T.m[1] : T.m[0];
-                const double a = p[r.a], b = p[r.b], c = p[r.c], d = p[r.d];
-                const double a1 = mod_add(a, b, md), b1 = mod_mul(mod_sub(a, b, md), tw[r.w1], md);
-                const double c1 = mod_add(c, d, md), d1 = mod_mul(mod_sub(c, d, md), tw[r.w2], md);
-                const double w3 = tw[r.w3];
-                p[r.a] = mod_add(a1, c1, md);
-                p[r.c] = mod_mul(mod_sub(a1, c1, md), w3, md);
-                p[r.b] = mod_add(b1, d1, md);
-                p[r.d] = mod_mul(mod_sub(b1, d1, md), w3, md);
-            }
-            __syncthreads();
-        }
-    }
-    if (lh < logn) {  // last radix-2 stage: half n/2, one block per segment
-        const int h = 1 << lh;
-        const int halfS = S >> 1;
-        for (int t = threadIdx.x; t < NL * halfS; t += blockDim.x) {
-            const int l = (NL == 1) ? 0 : (t >= halfS ? 1 : 0);
-            const int u = t - l * halfS;
-            const int seg = u >> (logn - 1);
-            const int j = u & (h - 1);
-            double *p = s + l * S + seg * n + j;
+// Inverse transform without the N^-1 scaling (callers fold it into their
+// store with mod_mul, which also makes the lazy outputs canonical), by
+// threads tid = 0 .. nth-1 as for ntt_fwd. Inputs are exact integers in
+// (-4q, 4q).
+template <int NL, int MAXR = kNttMaxR>
+__device__ void ntt_inv_noscale(double *s, int S, int n, int logn, const NttLimbs &T, int tid, int nth) {
+    int rs[12];
+    const int np = ntt_schedule(logn, MAXR, rs);
+    int lh0 = 0;
+    for (int pi = 0; pi < np; pi++) {
+        const int R = rs[pi];
+        const int upl = S >> R;
+        const int useg = logn - R;
+        for (int t = tid; t < NL * upl; t += nth) {
+            const int l = (NL == 1) ? 0 : (t >= upl ? 1 : 0);
+            const int ua = t - l * upl;
+            const int seg = ua >> useg;
+            const int u = ua & ((1 << useg) - 1);
+            const LdsMem p{s + l * S + seg * n};
             const double *tw = l ? T.tw[1] : T.tw[0];
             const FMod md = l ? T.m[1] : T.m[0];
-            const double x = p[0], y = p[h];
-            p[0] = mod_add(x, y, md);
-            p[h] = mod_mul(mod_sub(x, y, md), tw[1], md);
+            if (MAXR >= 4 && R == 4)
+                ntt_inv_unit<(MAXR >= 4 ? 4 : 1)>(p, logn, lh0, u, tw, md);
+            else if (MAXR >= 3 && R == 3)
+                ntt_inv_unit<(MAXR >= 3 ? 3 : 1)>(p, logn, lh0, u, tw, md);
+            else if (R == 2)
+                ntt_inv_unit<2>(p, logn, lh0, u, tw, md);
+            else
+                ntt_inv_unit<1>(p, logn, lh0, u, tw, md);
         }
         __syncthreads();
+        lh0 += R;
     }
+}
+
+// Every thread of the workgroup takes part.
+template <int NL, int MAXR = kNttMaxR>
+__device__ __forceinline__ void ntt_fwd(double *s, int S, int n, int logn, const NttLimbs &T) {
+    ntt_fwd<NL, MAXR>(s, S, n, logn, T, threadIdx.x, blockDim.x);
+}
+
+template <int NL, int MAXR = kNttMaxR>
+__device__ __forceinline__ void ntt_inv_noscale(double *s, int S, int n, int logn, const NttLimbs &T) {
+    ntt_inv_noscale<NL, MAXR>(s, S, n, logn, T, threadIdx.x, blockDim.x);
 }
 
 // Wrappers with the ring's own degree.

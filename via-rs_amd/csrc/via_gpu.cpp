@@ -188,7 +188,7 @@ via_status check_params(const via_params *pp) {
         return VIA_E_INVALID;
     const uint64_t qs[4] = {pp->q1[0], pp->q1[1], pp->q2, pp->q3};
     for (uint64_t q : qs)
-        if (q < 3 || q >= (1ULL << 50) || (q - 1) % (2ULL * pp->n1)) return VIA_E_INVALID;
+        if (q < 3 || q >= kMaxModulus || (q - 1) % (2ULL * pp->n1)) return VIA_E_INVALID;
     if (!is_pow2(pp->q4) || !is_pow2(pp->p)) return VIA_E_INVALID;
     u128 Q1 = (u128)pp->q1[0] * pp->q1[1];
     if (Q1 >> 96) return VIA_E_INVALID;
@@ -394,6 +394,11 @@ via_status via_ctx_create(int device, via_ctx **out) {
             delete c;
             return VIA_E_DEVICE;
         }
+    for (auto &e : c->ev_mark)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete c;
+            return VIA_E_DEVICE;
+        }
     *out = c;
     return VIA_OK;
 }
@@ -410,6 +415,7 @@ void via_ctx_destroy(via_ctx *c) {
     }
     for (auto &e : c->ev) (void)hipEventDestroy(e);
     for (auto &e : c->ev_fd) (void)hipEventDestroy(e);
+    for (auto &e : c->ev_mark) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -436,6 +442,21 @@ via_status via_ctx_first_dim_time(via_ctx *c, double *seconds) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev_fd[0], c->ev_fd[1]));
+    *seconds = ms * 1e-3;
+    return VIA_OK;
+}
+
+via_status via_ctx_mark(via_ctx *c, int which) {
+    if (!c || which < 0 || which > 1) return VIA_E_INVALID;
+    HIP_TRY(hipEventRecord(c->ev_mark[which], c->stream));
+    return VIA_OK;
+}
+
+via_status via_ctx_marked_time(via_ctx *c, double *seconds) {
+    if (!c || !seconds) return VIA_E_INVALID;
+    HIP_TRY(hipEventSynchronize(c->ev_mark[1]));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev_mark[0], c->ev_mark[1]));
     *seconds = ms * 1e-3;
     return VIA_OK;
 }
@@ -741,7 +762,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         cb.masks_next = w.masks[cur ^ 1];
         cb.body_next = w.body[cur ^ 1];
         cb.dig_next = w.dig;
-        launch_cascade_combine(r, gck, dpk, r, cb, st);
+        launch_cascade_combine(r, gck, dpk, r1n, cb, st);
         cur ^= 1;
     }
     // rlwe_to_rgsw_eval (convert.rs:162-180) with each client's conversion
@@ -1201,7 +1222,7 @@ via_status via_debug_stage(via_ctx *c, int stage, uint64_t *out, size_t words) {
 static via_status ring_for(via_ctx *c, uint32_t n, const uint64_t *moduli, uint32_t nmod, RingDev *r) {
     if (!is_pow2(n) || n < 2 || n > (uint32_t)kSlots || nmod < 1 || nmod > 2) return VIA_E_INVALID;
     for (uint32_t i = 0; i < nmod; i++)
-        if (moduli[i] >= (1ULL << 50) || (moduli[i] - 1) % (2ULL * n)) return VIA_E_INVALID;
+        if (moduli[i] >= kMaxModulus || (moduli[i] - 1) % (2ULL * n)) return VIA_E_INVALID;
     return make_ring(c, n, moduli, (int)nmod, r);
 }
 
@@ -1260,6 +1281,12 @@ via_status via_copy_to_device(via_ctx *c, void *dst, const void *src, size_t byt
     if (!c) return VIA_E_INVALID;
     HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return VIA_OK;
+}
+
+via_status via_copy_device(via_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (!c) return VIA_E_INVALID;
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
     return VIA_OK;
 }
 

@@ -109,6 +109,8 @@ def lib():
             "via_ctx_step_times": (C.c_int, [vp, C.POINTER(C.c_double)]),
             "via_ctx_synchronize": (C.c_int, [vp]),
             "via_ctx_first_dim_time": (C.c_int, [vp, C.POINTER(C.c_double)]),
+            "via_ctx_mark": (C.c_int, [vp, C.c_int]),
+            "via_ctx_marked_time": (C.c_int, [vp, C.POINTER(C.c_double)]),
             "via_db_create": (C.c_int, [vp, P, u64p, C.c_size_t, C.POINTER(vp)]),
             "via_db_create_bytes": (C.c_int, [vp, P, C.c_char_p, C.c_size_t, C.POINTER(vp)]),
             "via_db_destroy": (None, [vp]),
@@ -132,6 +134,7 @@ def lib():
             "via_device_free": (C.c_int, [vp, vp]),
             "via_copy_to_device": (C.c_int, [vp, vp, vp, C.c_size_t]),
             "via_copy_to_host": (C.c_int, [vp, vp, vp, C.c_size_t]),
+            "via_copy_device": (C.c_int, [vp, vp, vp, C.c_size_t]),
             "via_status_str": (C.c_char_p, [C.c_int]),
             "via_server_create": (C.c_int, [C.c_int, P, vp, vp, C.c_size_t, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
             "via_server_submit_batch": (C.c_int, [vp, C.c_uint64, C.c_uint32, u64p, C.c_size_t,
@@ -208,6 +211,16 @@ class Context:
         _check(lib().via_ctx_first_dim_time(self.h, C.byref(t)))
         return t.value
 
+    def mark(self, which):
+        """Record the start (0) or stop (1) event of the stream timer."""
+        _check(lib().via_ctx_mark(self.h, which))
+
+    def marked_time(self):
+        """Seconds between the last start and stop marks (waits for the stop)."""
+        t = C.c_double()
+        _check(lib().via_ctx_marked_time(self.h, C.byref(t)))
+        return t.value
+
     def stage(self, k, words):
         out = np.zeros(words, dtype=np.uint64)
         _check(lib().via_debug_stage(self.h, k, _p(out), words))
@@ -235,6 +248,9 @@ class Context:
     def to_host(self, arr, d):
         _check(lib().via_copy_to_host(self.h, arr.ctypes.data_as(C.c_void_p), d, arr.nbytes))
         return arr
+
+    def copy_device(self, dst, src, nbytes):
+        _check(lib().via_copy_device(self.h, dst, src, nbytes))
 
     def poly_mul_device(self, n, moduli, group, d_a, d_b, d_c, count):
         m = _u64(moduli)
