@@ -449,6 +449,7 @@ def main():
     if args.calib_copy:
         cb = 256 << 20
         c_src, c_dst = ctx.alloc(cb), ctx.alloc(cb)
+        ctx.to_device(c_src, np.arange(cb // 8, dtype=np.uint64))  # a written source, as a real copy reads
         ctx.copy_device(c_dst, c_src, cb)
         ctx.synchronize()
         ctx.free(c_src)

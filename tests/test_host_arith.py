@@ -30,6 +30,12 @@ def test_modarith_against_u128(tmp_path):
     build_and_run("modarith_check.cpp", tmp_path)
 
 
+def test_xcd_task_map(tmp_path):
+    # workgroup -> (item, digit group, tile) map of the gadget MAC: a bijection
+    # that keeps each client's tasks on the XCDs congruent to it
+    build_and_run("xcd_map_check.cpp", tmp_path)
+
+
 def test_lazy_ntt_passes(tmp_path):
     # radix-2^R passes over lazy residues: bit-identical canonical results
     # and the stated intermediate bounds, for every paper prime and schedule

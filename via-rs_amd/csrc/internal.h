@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "modarith.h"
+#include "xcd_map.h"
 
 namespace via {
 

@@ -46,8 +46,9 @@ void launch_lwe_digits(const RingDev &r1, const Gadget &g, const DigitPlan &dp, 
 // that are embedded (x -> x^2) transform n/2 coefficients.
 template <int NL>
 __global__ __launch_bounds__(kThreads) void k_gadget_mac(RingDev r, Gadget g, MacArgs a) {
-    __shared__ double s[NL * kSlots];
-    const int item = blockIdx.x, grp = blockIdx.y, tile = blockIdx.z;
+    __shared__ double s[lds_words(NL * kSlots)];
+    const MacTask task = mac_task(blockIdx.x, a.items, a.items_per_key, a.groups, a.tiles);
+    const int item = task.item, grp = task.grp, tile = task.tile;
     const int S = kSlots;
     const int cin = a.embed ? S / 2 : S;
     const int nt = a.embed ? r.n / 2 : r.n;
@@ -67,8 +68,8 @@ __global__ __launch_bounds__(kThreads) void k_gadget_mac(RingDev r, Gadget g, Ma
         for (int c = threadIdx.x; c < cin; c += blockDim.x) {
             // signed digits are valid NTT inputs (the lazy transform takes (-2q, 2q))
             const double d = (double)dk[c];
-            s[c] = d;
-            if (NL == 2) s[cin + c] = d;
+            s[lds_pad(c)] = d;
+            if (NL == 2) s[lds_pad(cin + c)] = d;
         }
         __syncthreads();
         ntt_fwd<NL>(s, cin, nt, lognt, T);
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(kThreads) void k_gadget_mac(RingDev r, Gadget g, Ma
 #pragma unroll
             for (int e = 0; e < kPer; e++) {
                 const int sg = threadIdx.x + e * kThreads;
-                const double x = s[l * cin + (a.embed ? (sg >> 1) : sg)];
+                const double x = s[lds_pad(l * cin + (a.embed ? (sg >> 1) : sg))];
                 am[l][e] += mod_mul_lazy(x, km[sg], md);
                 ab[l][e] += mod_mul_lazy(x, kb[sg], md);
             }
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(kThreads) void k_gadget_mac(RingDev r, Gadget g, Ma
             ab[l][e] = mod_reduce_small(ab[l][e], md);
             a.out_mask[(ot * NL + l) * S + sg] = am[l][e];
             if (!a.reduce_body) a.out_body[(ot * NL + l) * S + sg] = ab[l][e];
-            else s[l * S + sg] = ab[l][e];
+            else s[lds_pad(l * S + sg)] = ab[l][e];
         }
     if (!a.reduce_body) return;
     __syncthreads();
@@ -110,18 +111,18 @@ __global__ __launch_bounds__(kThreads) void k_gadget_mac(RingDev r, Gadget g, Ma
             const int l = t / (half * n);
             const int rem = t - l * half * n;
             const FMod md = l ? m1 : m0;
-            s[l * S + rem] = mod_add(s[l * S + rem], s[l * S + half * n + rem], md);
+            s[lds_pad(l * S + rem)] = mod_add(s[lds_pad(l * S + rem)], s[lds_pad(l * S + half * n + rem)], md);
         }
         __syncthreads();
     }
     for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
         const int l = t / n, c = t - l * n;
-        a.out_body[(ot * NL + l) * n + c] = s[l * S + c];
+        a.out_body[(ot * NL + l) * n + c] = s[lds_pad(l * S + c)];
     }
 }
 
 void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t st) {
-    dim3 grid(a.items, a.groups, a.tiles);
+    dim3 grid(a.items * a.groups * a.tiles);
     if (r.nl == 2)
         hipLaunchKernelGGL(k_gadget_mac<2>, grid, dim3(kThreads), 0, st, r, g, a);
     else
@@ -132,54 +133,56 @@ void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipS
 // embed(body_prev) - iNTT(sum of body partials); the digits of masks_next for
 // the next gadget product follow in k_poly_digits. Workgroup (item, chunk)
 // handles the chunk's mask polynomials (chunks of at least one degree-n
-// polynomial); chunk 0 also handles the body.
+// polynomial). The partial sums of all digit groups and both tiles are added lazily
+// (canonical terms, at most 2 * 18 of them: below 2^44) and brought into
+// the inverse transform's input range with one lazy reduction. Workgroups
+// y < chunks handle the masks, workgroup y == chunks the body, each with
+// 1024 threads: at the large output degrees there is one polynomial per
+// item and the workgroups, not the work per workgroup, are scarce.
 template <int NL>
-__global__ __launch_bounds__(kThreads) void k_cascade_combine(RingDev r, Gadget g, DigitPlan dp, CombineArgs a) {
-    __shared__ double s[NL * kSlots];
-    const int item = blockIdx.x, chunk = blockIdx.y, chunks = gridDim.y;
+__global__ __launch_bounds__(1024) void k_cascade_combine(RingDev r, Gadget g, DigitPlan dp, CombineArgs a) {
+    __shared__ double s[lds_words(NL * kSlots)];
+    const int item = blockIdx.x, chunk = blockIdx.y, chunks = gridDim.y - 1;
     const int S = kSlots, n = r.n;
-    const int CS = S / chunks;  // slots of this chunk (a multiple of n)
-    const int c0 = chunk * CS;
     const FMod m0 = r.t[0].m, m1 = r.t[1].m;
     const NttLimbs T = limbs_inv(r);
-    const size_t gstride = (size_t)a.items * 2 * NL * S;
-    for (int t = threadIdx.x; t < NL * CS; t += blockDim.x) {
-        const int l = (NL == 2 && t >= CS) ? 1 : 0;
-        const int c = t - l * CS;
-        const FMod md = l ? m1 : m0;
-        const double *p = a.pm + (size_t)item * 2 * NL * S + (size_t)l * S + c0 + c;
-        double acc = mod_add(p[0], p[(size_t)NL * S], md);
-        for (int gi = 1; gi < a.groups; gi++)
-            acc = mod_add(acc, mod_add(p[gi * gstride], p[gi * gstride + (size_t)NL * S], md), md);
-        s[t] = acc;
-    }
-    __syncthreads();
-    ntt_inv_noscale<NL>(s, CS, n, r.logn, T);
     const double ni0 = r.t[0].n_inv, ni1 = r.t[1].n_inv;
-    for (int c = threadIdx.x; c < CS; c += blockDim.x) {
-        const double x0 = mod_neg(mod_mul(s[c], ni0, m0), m0);
-        const double x1 = NL == 2 ? mod_neg(mod_mul(s[CS + c], ni1, m1), m1) : 0.0;
-        double *mo = a.masks_next + (size_t)item * NL * S + c0 + c;
-        mo[0] = x0;
-        if (NL == 2) mo[S] = x1;
+    if (chunk < chunks) {
+        const int CS = S / chunks;  // slots of this chunk (a multiple of n)
+        const int c0 = chunk * CS;
+        const size_t gstride = (size_t)a.items * 2 * NL * S;
+        for (int t = threadIdx.x; t < NL * CS; t += blockDim.x) {
+            const int l = (NL == 2 && t >= CS) ? 1 : 0;
+            const int c = t - l * CS;
+            const double *p = a.pm + (size_t)item * 2 * NL * S + (size_t)l * S + c0 + c;
+            double acc = p[0] + p[(size_t)NL * S];
+            for (int gi = 1; gi < a.groups; gi++) acc += p[gi * gstride] + p[gi * gstride + (size_t)NL * S];
+            s[lds_pad(t)] = lazy_reduce(acc, l ? m1 : m0);
+        }
+        __syncthreads();
+        ntt_inv_noscale<NL>(s, CS, n, r.logn, T);
+        for (int c = threadIdx.x; c < CS; c += blockDim.x) {
+            const double x0 = mod_neg(mod_mul(s[lds_pad(c)], ni0, m0), m0);
+            const double x1 = NL == 2 ? mod_neg(mod_mul(s[lds_pad(CS + c)], ni1, m1), m1) : 0.0;
+            double *mo = a.masks_next + (size_t)item * NL * S + c0 + c;
+            mo[0] = x0;
+            if (NL == 2) mo[S] = x1;
+        }
+        return;
     }
-    if (chunk != 0) return;
-    __syncthreads();
     const size_t gstride_b = (size_t)a.items * 2 * NL * n;
     for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
-        const FMod md = (t >= n) ? m1 : m0;
         const double *p = a.pb + (size_t)item * 2 * NL * n + t;
-        double acc = mod_add(p[0], p[(size_t)NL * n], md);
-        for (int gi = 1; gi < a.groups; gi++)
-            acc = mod_add(acc, mod_add(p[gi * gstride_b], p[gi * gstride_b + (size_t)NL * n], md), md);
-        s[t] = acc;
+        double acc = p[0] + p[(size_t)NL * n];
+        for (int gi = 1; gi < a.groups; gi++) acc += p[gi * gstride_b] + p[gi * gstride_b + (size_t)NL * n];
+        s[lds_pad(t)] = lazy_reduce(acc, t >= n ? m1 : m0);
     }
     __syncthreads();
     ntt_inv_noscale<NL>(s, n, n, r.logn, T);
     for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
         const int l = t >= n ? 1 : 0, c = t - l * n;
         const FMod md = l ? m1 : m0;
-        const double x = mod_mul(s[t], l ? ni1 : ni0, md);
+        const double x = mod_mul(s[lds_pad(t)], l ? ni1 : ni0, md);
         double e = 0.0;
         if ((c & 1) == 0) e = a.body_prev[(size_t)item * a.body_item_stride + (size_t)l * a.body_limb_stride + (c >> 1)];
         a.body_next[((size_t)item * NL + l) * n + c] = mod_sub(e, x, md);
@@ -190,11 +193,11 @@ void launch_cascade_combine(const RingDev &r, const Gadget &g, const DigitPlan &
                             const CombineArgs &a, hipStream_t st) {
     int chunks = kSlots / r.n;
     if (chunks > 8) chunks = 8;
-    dim3 grid(a.items, chunks);
+    dim3 grid(a.items, chunks + 1);
     if (r.nl == 2)
-        hipLaunchKernelGGL(k_cascade_combine<2>, grid, dim3(kThreads), 0, st, r, g, dp, a);
+        hipLaunchKernelGGL(k_cascade_combine<2>, grid, dim3(1024), 0, st, r, g, dp, a);
     else
-        hipLaunchKernelGGL(k_cascade_combine<1>, grid, dim3(kThreads), 0, st, r, g, dp, a);
+        hipLaunchKernelGGL(k_cascade_combine<1>, grid, dim3(1024), 0, st, r, g, dp, a);
     // digits of every output coefficient (rq: the q1 ring at n1, whose RNS
     // and gadget constants the scale uses)
     if (a.dig_next)
@@ -204,10 +207,10 @@ void launch_cascade_combine(const RingDev &r, const Gadget &g, const DigitPlan &
 // ---- rlwe_to_rgsw combine (convert.rs:162-180) -----------------------------------
 
 template <int NL>
-__global__ __launch_bounds__(kThreads) void k_r2g_combine(RingDev r, const double *pm, const double *pb, int groups,
+__global__ __launch_bounds__(1024) void k_r2g_combine(RingDev r, const double *pm, const double *pb, int groups,
                                                           int items, const double *rlwe_masks,
                                                           const double *rlwe_bodies, double *rgsw, int lq) {
-    __shared__ double s[NL * kSlots];
+    __shared__ double s[lds_words(NL * kSlots)];
     const int item = blockIdx.x;  // bit * lq + i
     const int part = blockIdx.y;  // 0: mask of neg_s_m[i] (and the copy of m[i]), 1: its body
     const int bit = item / lq, i = item - bit * lq;
@@ -220,17 +223,16 @@ __global__ __launch_bounds__(kThreads) void k_r2g_combine(RingDev r, const doubl
     const double *rm = rlwe_masks + (size_t)item * pw, *rb = rlwe_bodies + (size_t)item * pw;
     const double *src = (part ? pb : pm) + (size_t)item * pw;
     for (int t = threadIdx.x; t < (int)pw; t += blockDim.x) {
-        const FMod md = t >= n ? m1 : m0;
-        double acc = src[t];
-        for (int gi = 1; gi < groups; gi++) acc = mod_add(acc, src[(size_t)gi * items * pw + t], md);
-        s[t] = acc;
+        double acc = src[t];  // canonical partials, at most 18: a lazy sum below 2^43
+        for (int gi = 1; gi < groups; gi++) acc += src[(size_t)gi * items * pw + t];
+        s[lds_pad(t)] = lazy_reduce(acc, t >= n ? m1 : m0);
     }
     __syncthreads();
     lds_ntt_inv_noscale<NL>(s, n, r);
     for (int t = threadIdx.x; t < (int)pw; t += blockDim.x) {
         const int l = t >= n ? 1 : 0;
         const FMod md = l ? m1 : m0;
-        const double x = mod_mul(s[t], l ? r.t[1].n_inv : r.t[0].n_inv, md);
+        const double x = mod_mul(s[lds_pad(t)], l ? r.t[1].n_inv : r.t[0].n_inv, md);
         if (part == 0) {
             nsm[t] = mod_add(rb[t], x, md);
             mm[t] = rm[t];
@@ -245,10 +247,10 @@ void launch_r2g_combine(const RingDev &r, const double *pm, const double *pb, in
                         const double *rlwe_bodies, double *rgsw, int bits, int lq, hipStream_t st) {
     const int items = bits * lq;
     if (r.nl == 2)
-        hipLaunchKernelGGL(k_r2g_combine<2>, dim3(items, 2), dim3(kThreads), 0, st, r, pm, pb, groups, items,
+        hipLaunchKernelGGL(k_r2g_combine<2>, dim3(items, 2), dim3(1024), 0, st, r, pm, pb, groups, items,
                            rlwe_masks, rlwe_bodies, rgsw, lq);
     else
-        hipLaunchKernelGGL(k_r2g_combine<1>, dim3(items, 2), dim3(kThreads), 0, st, r, pm, pb, groups, items,
+        hipLaunchKernelGGL(k_r2g_combine<1>, dim3(items, 2), dim3(1024), 0, st, r, pm, pb, groups, items,
                            rlwe_masks, rlwe_bodies, rgsw, lq);
 }
 
@@ -289,7 +291,7 @@ __device__ __forceinline__ void gate_input(int mode, const double *in, int j, in
 template <int NL, int TPB>
 __global__ __launch_bounds__(TPB) void k_gate(RingDev r, Gadget g, GateArgs ga) {
     constexpr int PER = kSlots / TPB;
-    __shared__ double s[NL * kSlots];
+    __shared__ double s[lds_words(NL * kSlots)];
     const int q = blockIdx.x / ga.items_per_query;
     const int j = blockIdx.x - q * ga.items_per_query;
     const int mode = ga.mode, shift = ga.shift;
@@ -320,8 +322,8 @@ __global__ __launch_bounds__(TPB) void k_gate(RingDev r, Gadget g, GateArgs ga) 
             for (int e = 0; e < PER; e++) {
                 const int sg = threadIdx.x + e * TPB;
                 const double d = (double)extract_digit(v[e], g.base);
-                s[sg] = d;
-                if (NL == 2) s[S + sg] = d;
+                s[lds_pad(sg)] = d;
+                if (NL == 2) s[lds_pad(S + sg)] = d;
             }
             __syncthreads();
             lds_ntt_fwd<NL>(s, S, r);
@@ -334,7 +336,7 @@ __global__ __launch_bounds__(TPB) void k_gate(RingDev r, Gadget g, GateArgs ga) 
 #pragma unroll
                 for (int e = 0; e < PER; e++) {
                     const int sg = threadIdx.x + e * TPB;
-                    const double x = s[l * S + sg];
+                    const double x = s[lds_pad(l * S + sg)];
                     am[l][e] += mod_mul_lazy(x, km[sg], md);
                     ab[l][e] += mod_mul_lazy(x, kb[sg], md);
                 }
@@ -349,7 +351,7 @@ __global__ __launch_bounds__(TPB) void k_gate(RingDev r, Gadget g, GateArgs ga) 
             const FMod md = l ? m1 : m0;
 #pragma unroll
             for (int e = 0; e < PER; e++)
-                s[l * S + threadIdx.x + e * TPB] = mod_reduce_small(part ? ab[l][e] : am[l][e], md);
+                s[lds_pad(l * S + threadIdx.x + e * TPB)] = mod_reduce_small(part ? ab[l][e] : am[l][e], md);
         }
         __syncthreads();
         lds_ntt_inv_noscale<NL>(s, S, r);
@@ -360,7 +362,7 @@ __global__ __launch_bounds__(TPB) void k_gate(RingDev r, Gadget g, GateArgs ga) 
 #pragma unroll
             for (int e = 0; e < PER; e++) {
                 const int sg = threadIdx.x + e * TPB;
-                const double p = mod_mul(s[l * S + sg], ni, md);
+                const double p = mod_mul(s[lds_pad(l * S + sg)], ni, md);
                 double bv, cv;
                 gate_input<NL>(mode, in, j, part, l, sg, n, shift, md, bv, cv);
                 const size_t o = (size_t)part * pw + (size_t)l * n + sg;
@@ -388,7 +390,7 @@ template <int NL>
 __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateArgs ga) {
     constexpr int S = kSlots;
     constexpr int TPB = 1024;
-    __shared__ double s[4 * NL * S];
+    __shared__ double s[lds_words(4 * NL * S)];
     const int q = blockIdx.x / ga.items_per_query;
     const int j = blockIdx.x - q * ga.items_per_query;
     const int mode = ga.mode, shift = ga.shift;
@@ -411,9 +413,9 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
             SMag v = gadget_scale<NL>(c0, c1, r, g);
             for (int k = 0; k < L; k++) {
                 const double d = (double)extract_digit(v, g.base);
-                double *buf = s + (size_t)(half * L + k) * NL * S;
-                buf[c] = d;
-                if (NL == 2) buf[S + c] = d;
+                const int b0 = (half * L + k) * NL * S;
+                s[lds_pad(b0 + c)] = d;
+                if (NL == 2) s[lds_pad(b0 + S + c)] = d;
             }
         }
     }
@@ -423,7 +425,7 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
         const int np = 2 * L;
         const int per = TPB / np;
         const int gi = tid / per;
-        ntt_fwd<NL>(s + (size_t)gi * NL * S, S, n, r.logn, limbs_fwd(r), tid - gi * per, per);
+        ntt_fwd<NL>(s + lds_words(gi * NL * S), S, n, r.logn, limbs_fwd(r), tid - gi * per, per);
     }
     // phase 3: external product sums (both parts) for NL * S slots per part
     constexpr int PER = NL * S / TPB;
@@ -437,7 +439,7 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
         double sm = 0.0, sb = 0.0;
         for (int half = 0; half < 2; half++)
             for (int k = 0; k < L; k++) {
-                const double x = s[(size_t)(half * L + k) * NL * S + t];
+                const double x = s[lds_pad((half * L + k) * NL * S + t)];
                 const double *key = rgsw + ((size_t)half * L + (L - 1 - k)) * 2 * pw + (size_t)l * n + c;
                 sm += mod_mul_lazy(x, key[0], md);
                 sb += mod_mul_lazy(x, key[pw], md);
@@ -449,14 +451,14 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
 #pragma unroll
     for (int e = 0; e < PER; e++) {
         const int t = tid + e * TPB;
-        s[t] = am[e];
-        s[NL * S + t] = ab[e];
+        s[lds_pad(t)] = am[e];
+        s[lds_pad(NL * S + t)] = ab[e];
     }
     __syncthreads();
     // phase 4: inverse transforms of the two parts side by side
     {
         const int part = tid >> 9;
-        ntt_inv_noscale<NL>(s + (size_t)part * NL * S, S, n, r.logn, limbs_inv(r), tid & 511, 512);
+        ntt_inv_noscale<NL>(s + lds_words(part * NL * S), S, n, r.logn, limbs_inv(r), tid & 511, 512);
     }
     // phase 5: gate combine
 #pragma unroll
@@ -468,7 +470,7 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
             const int c = t - l * S;
             const FMod md = l ? m1 : m0;
             const double ni = l ? r.t[1].n_inv : r.t[0].n_inv;
-            const double p = mod_mul(s[(size_t)part * NL * S + t], ni, md);
+            const double p = mod_mul(s[lds_pad(part * NL * S + t)], ni, md);
             double bv, cv;
             gate_input<NL>(mode, in, j, part, l, c, n, shift, md, bv, cv);
             const size_t o = (size_t)part * pw + (size_t)l * n + c;
@@ -567,55 +569,54 @@ void launch_modswitch_q1_q2(const RingDev &r1, uint64_t q2, const double *in, do
 // ---- FirstDim (first_dim.rs:94-184) -------------------------------------------------
 
 // out[chunk][q][j][part][k] = sum over the chunk's rows i of db[i][j][k] *
-// sel[q][i][part][k] for QB queries sharing each database load. A workgroup
-// is four wavefronts over the same 64 slots k (one per lane) and 4*TJ
-// columns (TJ per wavefront); each lane keeps 2*QB*TJ accumulators. The
-// database row of the next i is loaded while the current one is multiplied;
-// the selection vectors (shared by the four wavefronts) come through the L1
-// cache. Products are summed lazily
-// (exact below 2^53) and reduced once; chunks of rows give the launch enough
-// workgroups to keep HBM busy.
+// sel[q][i][part][k] for QB queries sharing each database load. Each lane
+// owns two adjacent slots k (16-byte loads) of TJ columns and keeps 4*QB*TJ
+// accumulators; a workgroup covers 512 slots. Row i+1 is loaded while row i
+// multiplies (two rows in registers), with each row's selection values
+// requested before its database values so that waiting for them never waits
+// for a later row. Products are summed lazily (exact below 2^53) and reduced
+// once.
 template <int QB, int TJ>
 __global__ __launch_bounds__(256) void k_first_dim(const double *db, const double *sel, double *out, int I, int J,
                                                    int n, int Q, int q0, int rows_per_chunk, FMod m) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int k = blockIdx.x * 64 + lane;
-    const int j0 = (blockIdx.y * 4 + wv) * TJ;
-    if (j0 >= J) return;  // narrow databases: J < 4 * TJ (TJ divides J)
+    typedef double dbl2 __attribute__((ext_vector_type(2)));
+    const int k = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= n) return;
+    const int j0 = blockIdx.y * TJ;
     const int i0 = blockIdx.z * rows_per_chunk;
     const int i1 = min(I, i0 + rows_per_chunk);
-    double am[QB][TJ], ab[QB][TJ];
+    double am[QB][TJ][2], ab[QB][TJ][2];
 #pragma unroll
     for (int qb = 0; qb < QB; qb++)
 #pragma unroll
-        for (int t = 0; t < TJ; t++) am[qb][t] = ab[qb][t] = 0.0;
+        for (int t = 0; t < TJ; t++) am[qb][t][0] = am[qb][t][1] = ab[qb][t][0] = ab[qb][t][1] = 0.0;
     const size_t qs = (size_t)I * 2 * n;
     const size_t rstride = (size_t)J * n;
     const double *col = db + (size_t)j0 * n + k;
     const double *selk = sel + (size_t)q0 * qs + k;
-    // Row i's loads: selection values first, then the database (streamed past
-    // the caches: it is read once per pass), so that waiting for a row's
-    // selection values never waits for a later row's database loads.
-    auto load_row = [&](int i, double (&d)[TJ], double (&sm)[QB], double (&sb)[QB]) {
+    auto load_row = [&](int i, dbl2 (&d)[TJ], dbl2 (&sm)[QB], dbl2 (&sb)[QB]) {
 #pragma unroll
         for (int qb = 0; qb < QB; qb++) {
-            sm[qb] = selk[(size_t)qb * qs + (size_t)(2 * i) * n];
-            sb[qb] = selk[(size_t)qb * qs + (size_t)(2 * i + 1) * n];
+            sm[qb] = *(const dbl2 *)(selk + (size_t)qb * qs + (size_t)(2 * i) * n);
+            sb[qb] = *(const dbl2 *)(selk + (size_t)qb * qs + (size_t)(2 * i + 1) * n);
         }
+        // the database is read once per pass: stream it past the caches
 #pragma unroll
-        for (int t = 0; t < TJ; t++) d[t] = __builtin_nontemporal_load(col + (size_t)i * rstride + (size_t)t * n);
+        for (int t = 0; t < TJ; t++)
+            d[t] = __builtin_nontemporal_load((const dbl2 *)(col + (size_t)i * rstride + (size_t)t * n));
     };
-    auto mac_row = [&](const double (&d)[TJ], const double (&sm)[QB], const double (&sb)[QB]) {
+    auto mac_row = [&](const dbl2 (&d)[TJ], const dbl2 (&sm)[QB], const dbl2 (&sb)[QB]) {
 #pragma unroll
         for (int qb = 0; qb < QB; qb++)
 #pragma unroll
             for (int t = 0; t < TJ; t++) {
-                am[qb][t] += mod_mul_lazy(d[t], sm[qb], m);
-                ab[qb][t] += mod_mul_lazy(d[t], sb[qb], m);
+                am[qb][t][0] += mod_mul_lazy(d[t].x, sm[qb].x, m);
+                am[qb][t][1] += mod_mul_lazy(d[t].y, sm[qb].y, m);
+                ab[qb][t][0] += mod_mul_lazy(d[t].x, sb[qb].x, m);
+                ab[qb][t][1] += mod_mul_lazy(d[t].y, sb[qb].y, m);
             }
     };
-    // two rows in registers: row i+1 loads while row i multiplies
-    double dA[TJ], smA[QB], sbA[QB], dB[TJ], smB[QB], sbB[QB];
+    dbl2 dA[TJ], smA[QB], sbA[QB], dB[TJ], smB[QB], sbB[QB];
     int i = i0;
     if (i < i1) load_row(i, dA, smA, sbA);
     for (; i + 1 < i1; i += 2) {
@@ -630,9 +631,14 @@ __global__ __launch_bounds__(256) void k_first_dim(const double *db, const doubl
     for (int qb = 0; qb < QB; qb++)
 #pragma unroll
         for (int t = 0; t < TJ; t++) {
+            dbl2 vm, vb;
+            vm.x = mod_reduce_small(am[qb][t][0], m);
+            vm.y = mod_reduce_small(am[qb][t][1], m);
+            vb.x = mod_reduce_small(ab[qb][t][0], m);
+            vb.y = mod_reduce_small(ab[qb][t][1], m);
             double *oq = o + (size_t)(q0 + qb) * J * 2 * n + ((size_t)(j0 + t) * 2) * n + k;
-            oq[0] = mod_reduce_small(am[qb][t], m);
-            oq[n] = mod_reduce_small(ab[qb][t], m);
+            *(dbl2 *)oq = vm;
+            *(dbl2 *)(oq + n) = vb;
         }
 }
 
@@ -646,27 +652,29 @@ __global__ void k_sum_chunks(double *part, size_t words, int chunks, FMod m) {
 }
 
 static int fd_qb(int Q) { return Q >= 8 ? 8 : (Q >= 4 ? 4 : (Q >= 2 ? 2 : 1)); }
-// columns per wavefront: 4 while J allows (J is a power of two), 2 for eight
-// queries (their 2*8*TJ accumulators then fit 128 registers)
+// columns per workgroup: 4 while J allows (J is a power of two), 2 from four
+// queries on (their 4*QB*TJ accumulators stay within the register file)
 static int fd_tj(int J, int QB) {
-    const int t = J >= 16 ? 4 : (J >= 8 ? 2 : 1);
-    return QB >= 8 && t > 2 ? 2 : t;
+    const int t = J >= 4 ? 4 : J;
+    return QB >= 4 && t > 2 ? 2 : t;
 }
 
+// One chunk of rows: the whole column per workgroup measured fastest for one
+// and for eight queries per pass at 256 x 256 cells (the partial planes and
+// their sum cost more than the extra workgroups gain).
 int first_dim_chunks(int I, int J, int n, int Q) {
-    // about 2048 workgroups (eight per CU) per database pass
-    const int tj = fd_tj(J, fd_qb(Q));
-    const long wgs = (long)(n / 64) * ((J + 4 * tj - 1) / (4 * tj));
-    int chunks = 1;
-    while (chunks < I && wgs * chunks < 2048) chunks *= 2;
-    return chunks;
+    (void)I;
+    (void)J;
+    (void)n;
+    (void)Q;
+    return 1;
 }
 
 template <int QB, int TJ>
 static void fd_pass(const double *db, const double *sel, double *out, int I, int J, int n, int Q, int q0, int chunks,
                     FMod m, hipStream_t st) {
     const int rows = (I + chunks - 1) / chunks;
-    dim3 grid(n / 64, (J + 4 * TJ - 1) / (4 * TJ), chunks);
+    dim3 grid((n / 2 + 255) / 256, J / TJ, chunks);
     hipLaunchKernelGGL((k_first_dim<QB, TJ>), grid, dim3(256), 0, st, db, sel, out, I, J, n, Q, q0, rows, m);
 }
 
@@ -674,7 +682,7 @@ template <int QB>
 static void fd_pass_q(const double *db, const double *sel, double *out, int I, int J, int n, int Q, int q0, int chunks,
                       FMod m, hipStream_t st) {
     const int tj = fd_tj(J, QB);
-    if (QB < 8 && tj == 4) fd_pass<QB, (QB < 8 ? 4 : 2)>(db, sel, out, I, J, n, Q, q0, chunks, m, st);
+    if (QB < 4 && tj == 4) fd_pass<QB, (QB < 4 ? 4 : 2)>(db, sel, out, I, J, n, Q, q0, chunks, m, st);
     else if (tj == 2) fd_pass<QB, 2>(db, sel, out, I, J, n, Q, q0, chunks, m, st);
     else fd_pass<QB, 1>(db, sel, out, I, J, n, Q, q0, chunks, m, st);
 }
@@ -699,75 +707,100 @@ void launch_first_dim(const double *db, const double *sel_eval, double *out_eval
 
 // ---- RespComp (resp_comp.rs:46-76) -----------------------------------------------------
 
+// One workgroup of 1024 threads per query. mod_switch_sym q2 -> q3 of the
+// mask (n1 coefficients, kept in LDS) and of the projected body; then the
+// ring switch (ring_switch.rs:358-385): for every j < d the projection
+// pi_0(mask * X^j) is gadget-decomposed (base b_rsk, l_rsk digits) and its
+// digit polynomials of degree n2 are multiplied with rsk_j. All d * l_rsk
+// digit transforms of a round (as many as the LDS holds, 32 at the paper
+// sizes) run side by side, then one multiply-accumulate pass and the two
+// inverse transforms (mask and body parts) side by side; finally
+// mod_switch_asym (mod_switch.rs:124-154) to mask mod q3, body mod q4.
+constexpr int kRespBufWords = 16384;  // digit polynomials of one round
+
 __global__ __launch_bounds__(1024) void k_resp_comp(RespArgs a) {
-    extern __shared__ double sm[];
+    __shared__ double c3m[kSlots];           // mask at q3, degree n1
+    __shared__ double buf[lds_words(kRespBufWords)];  // digit polynomials, then the two parts
     const int qy = blockIdx.x;
-    double *c3m = sm;           // [n1] mask at q3
-    double *buf = sm + a.n1;    // [n2] NTT buffer
-    const int n1 = a.n1, n2 = a.n2, d = a.d;
-    const int i = threadIdx.x;  // one slot of the n2 ring per thread
+    const int n1 = a.n1, n2 = a.n2, d = a.d, L = a.l_rsk;
+    const int logn2 = 31 - __clz(n2);
     const uint64_t q2 = a.q2, q3 = a.q3, q4 = a.q4;
     const FMod m3 = a.t3.m;
     const double *ct = a.ct + (size_t)qy * 2 * n1;
-    // mod_switch_sym q2 -> q3 at n1
     for (int c = threadIdx.x; c < n1; c += blockDim.x) {
         const uint64_t x = (uint64_t)ct[c];
         c3m[c] = (double)(((x * q3 + (q2 >> 1)) / q2) % q3);
     }
-    const uint64_t xb = (uint64_t)ct[n1 + d * i];
-    const double body_proj = (double)(((xb * q3 + (q2 >> 1)) / q2) % q3);
     __syncthreads();
-    RingDev r;
-    r.nl = 1;
-    r.n = n2;
-    r.logn = 31 - __clz(n2);
-    r.t[0] = a.t3;
-    r.t[1] = a.t3;
+    NttLimbs Tf, Ti;
+    Tf.tw[0] = Tf.tw[1] = a.t3.fwd;
+    Ti.tw[0] = Ti.tw[1] = a.t3.inv;
+    Tf.m[0] = Tf.m[1] = Ti.m[0] = Ti.m[1] = m3;
     uint64_t bl = 1;
-    for (int k = 0; k < a.l_rsk; k++) bl *= a.b_rsk;
-    double am = 0.0, ab = 0.0;
-    const size_t kw = (size_t)a.l_rsk * 2 * n2;
-    for (int j = 0; j < d; j++) {
-        // (mask * X^j).project_at(0) (ring_switch.rs:358-385)
-        const int idx = d * i - j;
-        const double pv = idx >= 0 ? c3m[idx] : mod_neg(c3m[idx + n1], m3);
-        const uint64_t x = (uint64_t)pv;
-        const int neg = x > (q3 >> 1);
-        const uint64_t mag = neg ? q3 - x : x;
-        SMag v;
-        v.mag = (via_u128)((mag * bl + (q3 >> 1)) / q3);
-        v.neg = neg && v.mag != 0;
-        const double *key = a.rsk_eval[qy] + (size_t)j * kw;
-        for (int k = 0; k < a.l_rsk; k++) {
-            const int32_t dg = extract_digit(v, a.b_rsk);
-            buf[i] = (double)dg;
-            __syncthreads();
-            lds_ntt_fwd<1>(buf, n2, r);
-            const int ks = a.l_rsk - 1 - k;
-            const double y = buf[i];
-            am = mod_add(am, mod_mul(y, key[(size_t)ks * 2 * n2 + i], m3), m3);
-            ab = mod_add(ab, mod_mul(y, key[(size_t)ks * 2 * n2 + n2 + i], m3), m3);
-            __syncthreads();
+    for (int k = 0; k < L; k++) bl *= a.b_rsk;
+    const int npoly = d * L;
+    const int per_round = kRespBufWords / n2;
+    const size_t kw = (size_t)L * 2 * n2;  // key words per j
+    const double *rsk = a.rsk_eval[qy];
+    // accumulators of the outputs (part, i) this thread owns: t = part * n2 + i
+    double acc[2] = {0.0, 0.0};
+    for (int p0 = 0; p0 < npoly; p0 += per_round) {
+        const int p1 = min(npoly, p0 + per_round);
+        // digits of pi_0(mask * X^j) for the (j, k) of this round
+        for (int t = threadIdx.x; t < d * n2; t += blockDim.x) {
+            const int j = t / n2, i = t - j * n2;
+            if ((j + 1) * L <= p0 || j * L >= p1) continue;
+            const int idx = d * i - j;
+            const double pv = idx >= 0 ? c3m[idx] : mod_neg(c3m[idx + n1], m3);
+            const uint64_t x = (uint64_t)pv;
+            const int neg = x > (q3 >> 1);
+            const uint64_t mag = neg ? q3 - x : x;
+            SMag v;
+            v.mag = (via_u128)((mag * bl + (q3 >> 1)) / q3);
+            v.neg = neg && v.mag != 0;
+            for (int k = 0; k < L; k++) {
+                const int dg = extract_digit(v, a.b_rsk);
+                const int pidx = j * L + k;
+                if (pidx >= p0 && pidx < p1) buf[lds_pad((pidx - p0) * n2 + i)] = (double)dg;
+            }
         }
-    }
-    double res[2];
-    for (int part = 0; part < 2; part++) {
-        buf[i] = part ? ab : am;
         __syncthreads();
-        lds_ntt_inv_noscale<1>(buf, n2, r);
-        res[part] = mod_mul(buf[i], a.t3.n_inv, m3);
+        ntt_fwd<1>(buf, (p1 - p0) * n2, n2, logn2, Tf);
+        for (int e = 0; e < 2; e++) {
+            const int t = threadIdx.x + e * blockDim.x;
+            if (t >= 2 * n2) break;
+            const int part = t >= n2 ? 1 : 0, i = t - part * n2;
+            double sacc = 0.0;
+            for (int pidx = p0; pidx < p1; pidx++) {
+                const int j = pidx / L, k = pidx - j * L;
+                const int ks = L - 1 - k;
+                sacc += mod_mul_lazy(buf[lds_pad((pidx - p0) * n2 + i)],
+                                     rsk[(size_t)j * kw + (size_t)ks * 2 * n2 + part * n2 + i], m3);
+            }
+            acc[e] = mod_add(acc[e], mod_reduce_small(sacc, m3), m3);
+        }
         __syncthreads();
     }
-    const uint64_t mask = (uint64_t)mod_neg(res[0], m3);
-    const uint64_t body = (uint64_t)mod_sub(body_proj, res[1], m3);
-    // mod_switch_asym (mod_switch.rs:124-154): mask q3 -> q3, body q3 -> q4
-    a.out_mask[qy][i] = ((mask * q3 + (q3 >> 1)) / q3) % q3;
-    a.out_body[qy][i] = ((body * q4 + (q3 >> 1)) / q3) % q4;
+    for (int e = 0; e < 2; e++) {
+        const int t = threadIdx.x + e * blockDim.x;
+        if (t < 2 * n2) buf[lds_pad(t)] = acc[e];
+    }
+    __syncthreads();
+    ntt_inv_noscale<1>(buf, 2 * n2, n2, logn2, Ti);
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+        const double rm = mod_mul(buf[lds_pad(i)], a.t3.n_inv, m3);
+        const double rb = mod_mul(buf[lds_pad(n2 + i)], a.t3.n_inv, m3);
+        const uint64_t xb = (uint64_t)ct[n1 + d * i];
+        const double body_proj = (double)(((xb * q3 + (q2 >> 1)) / q2) % q3);
+        const uint64_t mask = (uint64_t)mod_neg(rm, m3);
+        const uint64_t body = (uint64_t)mod_sub(body_proj, rb, m3);
+        a.out_mask[qy][i] = ((mask * q3 + (q3 >> 1)) / q3) % q3;
+        a.out_body[qy][i] = ((body * q4 + (q3 >> 1)) / q3) % q4;
+    }
 }
 
 void launch_resp_comp(const RespArgs &a, hipStream_t st) {
-    const size_t shmem = (size_t)(a.n1 + a.n2) * sizeof(double);
-    hipLaunchKernelGGL(k_resp_comp, dim3(a.queries), dim3(a.n2), shmem, st, a);
+    hipLaunchKernelGGL(k_resp_comp, dim3(a.queries), dim3(1024), 0, st, a);
 }
 
 // ---- setup: key relayout and DB packing ----------------------------------------------

@@ -8,7 +8,7 @@ namespace via {
 // n == kSlots). The polys of a workgroup share a limb (checked by the host).
 template <bool INV>
 __global__ __launch_bounds__(kThreads) void k_ntt_batch(RingDev r, double *data, size_t npolys, size_t limb_group) {
-    __shared__ double s[kSlots];
+    __shared__ double s[lds_words(kSlots)];
     const int n = r.n;
     const int P = kSlots / n;
     const size_t p0 = (size_t)blockIdx.x * P;
@@ -18,15 +18,15 @@ __global__ __launch_bounds__(kThreads) void k_ntt_batch(RingDev r, double *data,
     if (l == 1) r1.t[0] = r.t[1];
     double *g = data + p0 * n;
     const int S = cnt * n;
-    for (int i = threadIdx.x; i < S; i += blockDim.x) s[i] = g[i];
+    for (int i = threadIdx.x; i < S; i += blockDim.x) s[lds_pad(i)] = g[i];
     __syncthreads();
     if (!INV) {
         lds_ntt_fwd<1>(s, S, r1);
-        for (int i = threadIdx.x; i < S; i += blockDim.x) g[i] = mod_reduce_small(s[i], r1.t[0].m);
+        for (int i = threadIdx.x; i < S; i += blockDim.x) g[i] = mod_reduce_small(s[lds_pad(i)], r1.t[0].m);
     } else {
         lds_ntt_inv_noscale<1>(s, S, r1);
         const double ni = r1.t[0].n_inv;
-        for (int i = threadIdx.x; i < S; i += blockDim.x) g[i] = mod_mul(s[i], ni, r1.t[0].m);
+        for (int i = threadIdx.x; i < S; i += blockDim.x) g[i] = mod_mul(s[lds_pad(i)], ni, r1.t[0].m);
     }
 }
 
@@ -68,7 +68,7 @@ void launch_f64_to_u64(const double *src, uint64_t *dst, size_t n, hipStream_t s
 // modulus).
 __global__ __launch_bounds__(kThreads) void k_polymul(RingDev r, const uint64_t *a, const uint64_t *b, uint64_t *c,
                                                       size_t limb_group) {
-    __shared__ double s[2 * kSlots];
+    __shared__ double s[lds_words(2 * kSlots)];
     const int n = r.n;
     const size_t p = blockIdx.x;
     const int l = (int)((p / limb_group) % (size_t)r.nl);
@@ -77,18 +77,19 @@ __global__ __launch_bounds__(kThreads) void k_polymul(RingDev r, const uint64_t 
     r2.t[1] = r2.t[0];
     const uint64_t *ga = a + p * n, *gb = b + p * n;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        s[i] = (double)ga[i];
-        s[n + i] = (double)gb[i];
+        s[lds_pad(i)] = (double)ga[i];
+        s[lds_pad(n + i)] = (double)gb[i];
     }
     __syncthreads();
     lds_ntt_fwd<2>(s, n, r2);
     const FMod md = r2.t[0].m;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s[i] = mod_mul(mod_reduce_small(s[i], md), s[n + i], md);
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        s[lds_pad(i)] = mod_mul(mod_reduce_small(s[lds_pad(i)], md), s[lds_pad(n + i)], md);
     __syncthreads();
     lds_ntt_inv_noscale<1>(s, n, r2);
     const double ni = r2.t[0].n_inv;
     uint64_t *gc = c + p * n;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) gc[i] = (uint64_t)mod_mul(s[i], ni, md);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) gc[i] = (uint64_t)mod_mul(s[lds_pad(i)], ni, md);
 }
 
 void launch_polymul_u64(const RingDev &r, const uint64_t *a, const uint64_t *b, uint64_t *c, size_t count,

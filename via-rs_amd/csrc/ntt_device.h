@@ -46,16 +46,27 @@ __device__ __forceinline__ NttLimbs limbs_inv(const RingDev &r) {
     return t;
 }
 
-// LDS view for the pass kernels of ntt_lazy.h.
+// LDS layout of every transform buffer: one pad word after every 16, so
+// that the strided element groups of the radix-8 passes (strides of 4 and
+// 32 words, 16 lanes per LDS cycle) fall into distinct banks. Logical word i
+// of a buffer lives at lds_pad(i); a buffer of w logical words takes
+// lds_words(w).
+__host__ __device__ constexpr int lds_words(int w) { return w + w / 16; }
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+
+// LDS view for the pass kernels of ntt_lazy.h: a segment starting at logical
+// word `base` of a padded buffer.
 struct LdsMem {
-    double *p;
-    __device__ __forceinline__ double ld(int i) const { return p[i]; }
-    __device__ __forceinline__ void st(int i, double v) const { p[i] = v; }
+    double *s;
+    int base;
+    __device__ __forceinline__ double ld(int i) const { return s[lds_pad(base + i)]; }
+    __device__ __forceinline__ void st(int i, double v) const { s[lds_pad(base + i)] = v; }
 };
 
 constexpr int kNttMaxR = 3;  // default stages per register pass (radix 8)
 
-// Forward transform of every degree-n segment of the NL x S buffer by threads
+// Forward transform of every degree-n segment of the NL x S buffer (padded
+// layout, lds_words(NL * S) words) by threads
 // tid = 0 .. nth-1 (a subset of the workgroup may transform its own buffer;
 // every thread of the workgroup must reach the pass barriers). Inputs
 // are exact integers in (-2q, 2q); outputs are lazy, in (-24q, 24q) for
@@ -74,7 +85,7 @@ __device__ void ntt_fwd(double *s, int S, int n, int logn, const NttLimbs &T, in
             const int ua = t - l * upl;
             const int seg = ua >> useg;
             const int u = ua & ((1 << useg) - 1);
-            const LdsMem p{s + l * S + seg * n};
+            const LdsMem p{s, l * S + seg * n};
             const double *tw = l ? T.tw[1] : T.tw[0];
             const FMod md = l ? T.m[1] : T.m[0];
             if (MAXR >= 4 && R == 4)
@@ -109,7 +120,7 @@ __device__ void ntt_inv_noscale(double *s, int S, int n, int logn, const NttLimb
             const int ua = t - l * upl;
             const int seg = ua >> useg;
             const int u = ua & ((1 << useg) - 1);
-            const LdsMem p{s + l * S + seg * n};
+            const LdsMem p{s, l * S + seg * n};
             const double *tw = l ? T.tw[1] : T.tw[0];
             const FMod md = l ? T.m[1] : T.m[0];
             if (MAXR >= 4 && R == 4)

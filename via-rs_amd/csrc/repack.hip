@@ -81,7 +81,7 @@ void launch_repack_extract(const RingDev &r, const Gadget &g, const DigitPlan &d
 // writes the RLWE (mask, body). Workgroup (p, chunk) handles a chunk of the
 // n1 mask slots; chunk 0 also the body.
 __global__ __launch_bounds__(kThreads) void k_repack_combine(RingDev r, Gadget g, DigitPlan dp, RepackCombineArgs a) {
-    __shared__ double s[kSlots];
+    __shared__ double s[lds_words(kSlots)];
     const int item = blockIdx.x, chunk = blockIdx.y, chunks = gridDim.y;
     const int S = kSlots, n = r.n;
     const int CS = S / chunks;  // a multiple of n
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kThreads) void k_repack_combine(RingDev r, Gadget g
         const double *p = a.pm + (size_t)item * 2 * S + c0 + t;
         double acc = mod_add(p[0], p[S], m);
         for (int gi = 1; gi < a.groups; gi++) acc = mod_add(acc, mod_add(p[gi * gstride], p[gi * gstride + S], m), m);
-        s[t] = acc;
+        s[lds_pad(t)] = acc;
     }
     __syncthreads();
     ntt_inv_noscale<1>(s, CS, n, r.logn, T);
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kThreads) void k_repack_combine(RingDev r, Gadget g
     const size_t next_slots = 2 * (size_t)S;
     int16_t *dn = a.dig_next ? a.dig_next + (size_t)(item >> 1) * g.L * next_slots : nullptr;
     for (int t = threadIdx.x; t < CS; t += blockDim.x) {
-        const double x = mod_neg(mod_mul(s[t], ninv, m), m);
+        const double x = mod_neg(mod_mul(s[lds_pad(t)], ninv, m), m);
         const int u = c0 + t;
         if (a.final_level) {
             a.out[u] = x;
@@ -122,12 +122,12 @@ __global__ __launch_bounds__(kThreads) void k_repack_combine(RingDev r, Gadget g
         double acc = mod_add(p[0], p[n], m);
         for (int gi = 1; gi < a.groups; gi++)
             acc = mod_add(acc, mod_add(p[gi * gstride_b], p[gi * gstride_b + n], m), m);
-        s[t] = acc;
+        s[lds_pad(t)] = acc;
     }
     __syncthreads();
     ntt_inv_noscale<1>(s, n, n, r.logn, T);
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        const double x = mod_mul(s[t], ninv, m);
+        const double x = mod_mul(s[lds_pad(t)], ninv, m);
         const double v = mod_sub(a.body_cur[(size_t)item * a.body_stride + t], x, m);
         if (a.final_level) {
             a.out[S + t] = v;
