@@ -346,6 +346,7 @@ def main():
     ap.add_argument("--mix", action="store_true", help="serving: 30%% VIA-B requests against a second database")
     ap.add_argument("--vb", type=int, default=0, help="VIA-B batch size T (answer_batch), 0: VIA-C")
     ap.add_argument("--indices", type=int, default=64, help="VIA-B indices per step")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (heuristic digit groups)")
     ap.add_argument("--ntt", action="store_true", help="NTT->pointwise->iNTT microbench")
     ap.add_argument("--polys", type=int, default=8192, help="NTT microbench: RNS polynomial pairs")
     ap.add_argument("--ntt-sample", type=int, default=8192, help="NTT microbench: products checked on the CPU")
@@ -430,6 +431,9 @@ def main():
         if st:
             raise via_gpu.ViaError(st, msg=L.via_status_str(st).decode())
 
+    tuned = None
+    if not args.no_tune:
+        tuned = ctx.autotune(db, client_keys, d_q, pp.query_lwes, d_m, d_b, reps=2)
     for _ in range(args.warmup):
         answer()
     ctx.synchronize()
@@ -489,6 +493,7 @@ def main():
         "step_ms": {k: round(v / args.steps * 1e3, 3) for k, v in zip(
             ["query_decomp", "dmux", "mod_switch", "first_dim", "cmux", "crot", "resp_comp"], steps_acc)},
         "setup_s": round(t_setup, 3),
+        "tuned_digit_groups": tuned,
         "roofline": {
             "kernel": "k_first_dim (database stream multiply-accumulate)",
             "bound": "hbm",

@@ -128,6 +128,19 @@ via_status via_answer_many_device(via_ctx *ctx, const via_db *db, uint32_t nq, c
                                    const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_masks,
                                    uint64_t *const *d_bodies, via_err_detail *err);
 
+/* Autotuner (no counterpart in the reference): answers the nq device-resident
+ * queries with each candidate launch shape of every query-decompression step
+ * (the number of digit groups, i.e. workgroups splitting the 18 digits of one
+ * gadget product, and the kernel variant: 256 threads with one digit
+ * transform at a time, or 1024 threads with several side by side), reps timed
+ * runs per candidate, and keeps the fastest per step for batches of nq queries
+ * on this context. groups_out and wide_out (12 entries each: the eleven
+ * cascade steps and the RLWE -> RGSW conversion) receive the choice when not
+ * null. The answers written meanwhile are those of via_answer_many_device. */
+via_status via_ctx_autotune(via_ctx *ctx, const via_db *db, uint32_t nq, const via_keys *const *keys,
+                            const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_masks,
+                            uint64_t *const *d_bodies, int reps, int *groups_out, int *wide_out);
+
 /* Server::answer_batch::<T> (answer.rs:561-602) -> answer_batch
  * (batch.rs:72-163), VIA-B: answer_through_crot for each of the T queries of
  * one client at the database's record degree n_rec, Repack_{n2} of the T

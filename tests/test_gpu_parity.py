@@ -146,3 +146,33 @@ def test_answer_batch_mixed_clients(ctx):
     server = via_gpu.Server(ctx, pp, db, keys[2])
     m2, b2 = server.answer(cases[2].lwes)
     assert np.array_equal(m2, masks[2]) and np.array_equal(b2, bodies[2])
+
+
+def test_autotuned_answers_match(ctx):
+    # the autotuner changes only how the digits of each gadget product are
+    # split over workgroups: after tuning for a batch size, the batched
+    # answers still equal the oracle's and the per-step choices are counts
+    # the tuner offers
+    import via_gpu
+
+    cases = [paper_case(4, 4, seed=b"via-gpu/tune/client%d" % q, index=(11 * q + 3) % 64, records="random",
+                        rec_seed=b"via-gpu/tune-db") for q in range(3)]
+    pp = cases[0].pp_gpu
+    tctx = via_gpu.Context(0)
+    db = via_gpu.PreparedDb(tctx, pp, records=cases[0].records)
+    keys = [via_gpu.PreparedKeys(tctx, pp, c.cascade_key, c.conv_key, c.rsk) for c in cases]
+    n_lwe = pp.query_lwes
+    d_q = [tctx.alloc(c.lwes.nbytes) for c in cases]
+    for d, c in zip(d_q, cases):
+        tctx.to_device(d, c.lwes)
+    d_m = [tctx.alloc(8 * pp.n2) for _ in cases]
+    d_b = [tctx.alloc(8 * pp.n2) for _ in cases]
+    shapes = tctx.autotune(db, keys, d_q, n_lwe, d_m, d_b, reps=1)
+    assert len(shapes) == 12
+    assert all((w == 0 and g in (1, 2, 3, 4, 6, 9, 18)) or (w == 1 and g in (1, 2, 3, 5)) for g, w in shapes)
+    masks, bodies = via_gpu.answer_many(tctx, db, keys, [c.lwes for c in cases], pp)
+    for c, m, b in zip(cases, masks, bodies):
+        assert np.array_equal(m, c.ans_mask) and np.array_equal(b, c.ans_body)
+    for d in d_q + d_m + d_b:
+        tctx.free(d)
+    tctx.close()

@@ -26,9 +26,22 @@ struct Workspace {
     double *cm[2] = {nullptr, nullptr}, *rgsw2 = nullptr, *res[2] = {nullptr, nullptr};
     uint64_t *lwe_u = nullptr, *ans_u = nullptr;
     int16_t *dig = nullptr;
-    int groups = 1;
+    int groups = 1;      // heuristic digit groups per gadget product
+    int groups_max = 1;  // the partial-sum planes pm / pb hold this many groups
     int fd_chunks = 1;
     std::vector<void *> all;
+};
+
+// Gadget-product steps of query decompression: the eleven cascade steps and
+// the RLWE -> RGSW conversion.
+constexpr int kDecompSteps = 12;
+
+// Launch shape of one decompression step: digit groups (workgroups per item
+// and tile along the digits) and the kernel variant (wide: the digit
+// transforms of a group side by side).
+struct StepTune {
+    int groups = 1;
+    int wide = 0;
 };
 
 // Buffers of the VIA-B repack, sized for batches of up to T queries.
@@ -55,6 +68,12 @@ struct via_ctx {
     hipEvent_t ev[8] = {};
     hipEvent_t ev_fd[2] = {};
     hipEvent_t ev_mark[2] = {};  // via_ctx_mark start / stop
+    hipEvent_t ev_dec[kDecompSteps + 1] = {};  // around each decompression step (autotuner)
+    bool time_decomp = false;
+    // launch shape per decompression step, chosen by via_ctx_autotune for a
+    // batch size (number of queries per pass); absent: the heuristic
+    std::map<int, std::vector<StepTune>> tuned;
+    std::vector<StepTune> tune_override;  // set while the autotuner measures
     bool have_times = false;
     // pointers to the last answer's stage buffers
     const double *stage_ptr[7] = {};

@@ -82,6 +82,7 @@ struct MacArgs {
     double *out_body;        // same, or [group][item][tile][limb][n] when reduce_body
     int reduce_body;
     int items;
+    int wide;                // 1: digit transforms of a group side by side (k_gadget_mac_wide)
 };
 void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t s);
 

@@ -121,8 +121,103 @@ __global__ __launch_bounds__(kThreads) void k_gadget_mac(RingDev r, Gadget g, Ma
     }
 }
 
+// The same multiply-accumulate with G digit transforms side by side: 1024
+// threads, one LDS buffer per digit of a round (G digits per round, rounds
+// until the group's digits are done), each transform by 1024/G threads,
+// then one multiply-accumulate pass over the G transforms. Fewer
+// workgroups carry more digits each, with one transform's latency per round
+// instead of one per digit.
+template <int NL, int G>
+__global__ __launch_bounds__(1024) void k_gadget_mac_wide(RingDev r, Gadget g, MacArgs a) {
+    constexpr int S = kSlots, TPB = 1024, BW = lds_words(NL * kSlots), PER = NL * kSlots / TPB;
+    __shared__ double s[G * BW];
+    const MacTask task = mac_task(blockIdx.x, a.items, a.items_per_key, a.groups, a.tiles);
+    const int item = task.item, grp = task.grp, tile = task.tile;
+    const int cin = a.embed ? S / 2 : S;
+    const int nt = a.embed ? r.n / 2 : r.n;
+    const int lognt = a.embed ? r.logn - 1 : r.logn;
+    const NttLimbs T = limbs_fwd(r);
+    const FMod m0 = r.t[0].m, m1 = r.t[1].m;
+    const int k0 = grp * g.L / a.groups, k1 = (grp + 1) * g.L / a.groups;
+    const int tid = threadIdx.x;
+    double am[PER], ab[PER];
+#pragma unroll
+    for (int e = 0; e < PER; e++) am[e] = ab[e] = 0.0;
+    const int16_t *dig = a.dig + (size_t)item * g.L * a.dig_coeffs + (size_t)tile * cin;
+    const double *key = a.key[item / a.items_per_key] + (size_t)tile * S;
+    for (int kb = k0; kb < k1; kb += G) {
+        const int nk = min(G, k1 - kb);
+        for (int t = tid; t < nk * cin; t += TPB) {
+            const int kk = t / cin, c = t - kk * cin;
+            const double d = (double)dig[(size_t)(kb + kk) * a.dig_coeffs + c];
+            s[kk * BW + lds_pad(c)] = d;
+            if (NL == 2) s[kk * BW + lds_pad(cin + c)] = d;
+        }
+        __syncthreads();
+        {
+            constexpr int per = TPB / G;
+            const int gi = tid / per;
+            // buffers past the round's digits only take part in the barriers
+            ntt_fwd<NL>(s + gi * BW, cin, nt, lognt, T, gi < nk ? tid - gi * per : (1 << 30), per);
+        }
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            const int t = tid + e * TPB;  // limb-major slot
+            const int l = NL == 2 && t >= S ? 1 : 0;
+            const int sg = t - l * S;
+            const FMod md = l ? m1 : m0;
+            const int xi = l * cin + (a.embed ? (sg >> 1) : sg);
+            for (int kk = 0; kk < nk; kk++) {
+                const int ks = g.L - 1 - (kb + kk);
+                const double x = s[kk * BW + lds_pad(xi)];
+                const double *km = key + ((size_t)(ks * 2 + 0) * NL + l) * a.key_slots;
+                const double *kbp = key + ((size_t)(ks * 2 + 1) * NL + l) * a.key_slots;
+                am[e] += mod_mul_lazy(x, km[sg], md);
+                ab[e] += mod_mul_lazy(x, kbp[sg], md);
+            }
+        }
+        __syncthreads();
+    }
+    const size_t ot = ((size_t)grp * a.items + item) * a.tiles + tile;
+#pragma unroll
+    for (int e = 0; e < PER; e++) {
+        const int t = tid + e * TPB;
+        const int l = NL == 2 && t >= S ? 1 : 0;
+        const int sg = t - l * S;
+        const FMod md = l ? m1 : m0;
+        a.out_mask[(ot * NL + l) * S + sg] = mod_reduce_small(am[e], md);
+        const double b = mod_reduce_small(ab[e], md);
+        if (!a.reduce_body) a.out_body[(ot * NL + l) * S + sg] = b;
+        else s[lds_pad(l * S + sg)] = b;
+    }
+    if (!a.reduce_body) return;
+    __syncthreads();
+    // Sum the S/n segment bodies of the tile (exact modular additions).
+    const int n = r.n;
+    for (int half = (S / n) >> 1; half >= 1; half >>= 1) {
+        for (int t = tid; t < NL * half * n; t += TPB) {
+            const int l = t / (half * n);
+            const int rem = t - l * half * n;
+            const FMod md = l ? m1 : m0;
+            s[lds_pad(l * S + rem)] = mod_add(s[lds_pad(l * S + rem)], s[lds_pad(l * S + half * n + rem)], md);
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < NL * n; t += TPB) {
+        const int l = t / n, c = t - l * n;
+        a.out_body[(ot * NL + l) * n + c] = s[lds_pad(l * S + c)];
+    }
+}
+
 void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t st) {
     dim3 grid(a.items * a.groups * a.tiles);
+    if (a.wide) {
+        if (r.nl == 2)
+            hipLaunchKernelGGL((k_gadget_mac_wide<2, 4>), grid, dim3(1024), 0, st, r, g, a);
+        else
+            hipLaunchKernelGGL((k_gadget_mac_wide<1, 8>), grid, dim3(1024), 0, st, r, g, a);
+        return;
+    }
     if (r.nl == 2)
         hipLaunchKernelGGL(k_gadget_mac<2>, grid, dim3(kThreads), 0, st, r, g, a);
     else
@@ -136,11 +231,11 @@ void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipS
 // polynomial). The partial sums of all digit groups and both tiles are added lazily
 // (canonical terms, at most 2 * 18 of them: below 2^44) and brought into
 // the inverse transform's input range with one lazy reduction. Workgroups
-// y < chunks handle the masks, workgroup y == chunks the body, each with
-// 1024 threads: at the large output degrees there is one polynomial per
-// item and the workgroups, not the work per workgroup, are scarce.
-template <int NL>
-__global__ __launch_bounds__(1024) void k_cascade_combine(RingDev r, Gadget g, DigitPlan dp, CombineArgs a) {
+// y < chunks handle the masks, workgroup y == chunks the body. TPB = 1024
+// at the large output degrees (one or two polynomials per item, where the
+// workgroups rather than the work per workgroup are scarce), 256 otherwise.
+template <int NL, int TPB>
+__global__ __launch_bounds__(TPB) void k_cascade_combine(RingDev r, Gadget g, DigitPlan dp, CombineArgs a) {
     __shared__ double s[lds_words(NL * kSlots)];
     const int item = blockIdx.x, chunk = blockIdx.y, chunks = gridDim.y - 1;
     const int S = kSlots, n = r.n;
@@ -194,10 +289,18 @@ void launch_cascade_combine(const RingDev &r, const Gadget &g, const DigitPlan &
     int chunks = kSlots / r.n;
     if (chunks > 8) chunks = 8;
     dim3 grid(a.items, chunks + 1);
-    if (r.nl == 2)
-        hipLaunchKernelGGL(k_cascade_combine<2>, grid, dim3(1024), 0, st, r, g, dp, a);
-    else
-        hipLaunchKernelGGL(k_cascade_combine<1>, grid, dim3(1024), 0, st, r, g, dp, a);
+    const bool wide = chunks <= 2;
+    if (r.nl == 2) {
+        if (wide)
+            hipLaunchKernelGGL((k_cascade_combine<2, 1024>), grid, dim3(1024), 0, st, r, g, dp, a);
+        else
+            hipLaunchKernelGGL((k_cascade_combine<2, 256>), grid, dim3(256), 0, st, r, g, dp, a);
+    } else {
+        if (wide)
+            hipLaunchKernelGGL((k_cascade_combine<1, 1024>), grid, dim3(1024), 0, st, r, g, dp, a);
+        else
+            hipLaunchKernelGGL((k_cascade_combine<1, 256>), grid, dim3(256), 0, st, r, g, dp, a);
+    }
     // digits of every output coefficient (rq: the q1 ring at n1, whose RNS
     // and gadget constants the scale uses)
     if (a.dig_next)

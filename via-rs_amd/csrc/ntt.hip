@@ -6,6 +6,7 @@ namespace via {
 
 // One workgroup transforms kSlots/n polys of one limb (or one poly when
 // n == kSlots). The polys of a workgroup share a limb (checked by the host).
+// Radix-16 passes: these kernels hold little else in registers.
 template <bool INV>
 __global__ __launch_bounds__(kThreads) void k_ntt_batch(RingDev r, double *data, size_t npolys, size_t limb_group) {
     __shared__ double s[lds_words(kSlots)];
@@ -21,10 +22,10 @@ __global__ __launch_bounds__(kThreads) void k_ntt_batch(RingDev r, double *data,
     for (int i = threadIdx.x; i < S; i += blockDim.x) s[lds_pad(i)] = g[i];
     __syncthreads();
     if (!INV) {
-        lds_ntt_fwd<1>(s, S, r1);
+        lds_ntt_fwd<1, 4>(s, S, r1);
         for (int i = threadIdx.x; i < S; i += blockDim.x) g[i] = mod_reduce_small(s[lds_pad(i)], r1.t[0].m);
     } else {
-        lds_ntt_inv_noscale<1>(s, S, r1);
+        lds_ntt_inv_noscale<1, 4>(s, S, r1);
         const double ni = r1.t[0].n_inv;
         for (int i = threadIdx.x; i < S; i += blockDim.x) g[i] = mod_mul(s[lds_pad(i)], ni, r1.t[0].m);
     }
@@ -81,12 +82,12 @@ __global__ __launch_bounds__(kThreads) void k_polymul(RingDev r, const uint64_t 
         s[lds_pad(n + i)] = (double)gb[i];
     }
     __syncthreads();
-    lds_ntt_fwd<2>(s, n, r2);
+    lds_ntt_fwd<2, 4>(s, n, r2);
     const FMod md = r2.t[0].m;
     for (int i = threadIdx.x; i < n; i += blockDim.x)
         s[lds_pad(i)] = mod_mul(mod_reduce_small(s[lds_pad(i)], md), s[lds_pad(n + i)], md);
     __syncthreads();
-    lds_ntt_inv_noscale<1>(s, n, r2);
+    lds_ntt_inv_noscale<1, 4>(s, n, r2);
     const double ni = r2.t[0].n_inv;
     uint64_t *gc = c + p * n;
     for (int i = threadIdx.x; i < n; i += blockDim.x) gc[i] = (uint64_t)mod_mul(s[lds_pad(i)], ni, md);

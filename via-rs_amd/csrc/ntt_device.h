@@ -149,14 +149,14 @@ __device__ __forceinline__ void ntt_inv_noscale(double *s, int S, int n, int log
 }
 
 // Wrappers with the ring's own degree.
-template <int NL>
+template <int NL, int MAXR = kNttMaxR>
 __device__ __forceinline__ void lds_ntt_fwd(double *s, int S, const RingDev &r) {
-    ntt_fwd<NL>(s, S, r.n, r.logn, limbs_fwd(r));
+    ntt_fwd<NL, MAXR>(s, S, r.n, r.logn, limbs_fwd(r));
 }
 
-template <int NL>
+template <int NL, int MAXR = kNttMaxR>
 __device__ __forceinline__ void lds_ntt_inv_noscale(double *s, int S, const RingDev &r) {
-    ntt_inv_noscale<NL>(s, S, r.n, r.logn, limbs_inv(r));
+    ntt_inv_noscale<NL, MAXR>(s, S, r.n, r.logn, limbs_inv(r));
 }
 
 }  // namespace via

@@ -221,6 +221,30 @@ void free_ws(Workspace &w) {
     w = Workspace();
 }
 
+// Launch shapes the autotuner tries for `items` gadget products of depth L:
+// the 256-thread kernel with 1, 2, 3, 4, 6, 9 or 18 digit groups and the wide
+// kernel with 1, 2, 3 or 5 (group counts up to L), as long as a launch stays
+// within 8192 workgroups (two tiles per item).
+std::vector<StepTune> tune_candidates(int items, int L) {
+    std::vector<StepTune> out;
+    for (int wide = 0; wide < 2; wide++)
+        for (int g : wide ? std::initializer_list<int>{1, 2, 3, 5} : std::initializer_list<int>{1, 2, 3, 4, 6, 9, 18})
+            if (g <= L && (long)items * 2 * g <= 8192) {
+                StepTune t;
+                t.groups = g;
+                t.wide = wide;
+                out.push_back(t);
+            }
+    if (out.empty()) out.push_back(StepTune());
+    return out;
+}
+
+int max_tune_groups(int items, int L) {
+    int m = 1;
+    for (const StepTune &t : tune_candidates(items, L)) m = std::max(m, t.groups);
+    return m;
+}
+
 via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_t bits) {
     Workspace &w = c->ws;
     size_t I = pp->num_rows, J = pp->num_cols;
@@ -246,6 +270,8 @@ via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_
     int groups = 1;
     while (groups < (int)pp->l_ck && (long)items * 2 * groups < wg_target) groups++;
     w.groups = groups;
+    // room for the autotuner's candidates (tune_candidates)
+    w.groups_max = std::max(groups, max_tune_groups((int)items, (int)pp->l_ck));
     w.fd_chunks = first_dim_chunks((int)I, (int)J, (int)n, (int)Q);
     via_status s;
 #define A(p, cnt) \
@@ -257,8 +283,8 @@ via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_
     A(w.body[0], items * nl * S);
     A(w.body[1], items * nl * S);
     A(w.dig, items * pp->l_ck * S);
-    A(w.pm, (size_t)groups * items * 2 * nl * S);
-    A(w.pb, (size_t)groups * items * 2 * nl * S);
+    A(w.pm, (size_t)w.groups_max * items * 2 * nl * S);
+    A(w.pb, (size_t)w.groups_max * items * 2 * nl * S);
     A(w.rgsw, Q * bits * 4 * lq * nl * n);
     A(w.rgsw_eval, Q * bits * 4 * lq * nl * n);
     A(w.dm[0], Q * I * 2 * nl * n);
@@ -399,6 +425,11 @@ via_status via_ctx_create(int device, via_ctx **out) {
             delete c;
             return VIA_E_DEVICE;
         }
+    for (auto &e : c->ev_dec)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete c;
+            return VIA_E_DEVICE;
+        }
     *out = c;
     return VIA_OK;
 }
@@ -416,6 +447,7 @@ void via_ctx_destroy(via_ctx *c) {
     for (auto &e : c->ev) (void)hipEventDestroy(e);
     for (auto &e : c->ev_fd) (void)hipEventDestroy(e);
     for (auto &e : c->ev_mark) (void)hipEventDestroy(e);
+    for (auto &e : c->ev_dec) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -729,6 +761,23 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     const Gadget gck = make_gadget(pp->ck_base, pp->l_ck);
     const DigitPlan dpk = digit_plan((uint32_t)pp->ck_base, (int)pp->l_ck);
     launch_lwe_digits(r1n, gck, dpk, w.lwe_f, (int)items, w.dig, st);
+    // digit groups of each decompression step: the autotuner's measurement,
+    // its stored choice for this batch size, or the heuristic
+    StepTune step_tune[kDecompSteps];
+    {
+        const std::vector<StepTune> *tg = nullptr;
+        if (!c->tune_override.empty()) tg = &c->tune_override;
+        else if (c->tuned.count(Q)) tg = &c->tuned[Q];
+        for (int i = 0; i < kDecompSteps; i++) {
+            if (tg && (int)tg->size() == kDecompSteps) {
+                step_tune[i] = (*tg)[i];
+                step_tune[i].groups = std::min(step_tune[i].groups, w.groups_max);
+            } else {
+                step_tune[i].groups = w.groups;
+            }
+        }
+    }
+    if (c->time_decomp) HIP_TRY(hipEventRecord(c->ev_dec[0], st));
     int cur = 0;
     for (uint32_t nout = 2, step = 0; nout <= n; nout <<= 1, step++) {
         RingDev r;
@@ -744,7 +793,8 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         a.items_per_key = (int)B;
         a.key_slots = (size_t)rank_in * nout;
         a.tiles = 2;
-        a.groups = w.groups;
+        a.groups = step_tune[step].groups;
+        a.wide = step_tune[step].wide;
         a.out_mask = w.pm;
         a.out_body = w.pb;
         a.reduce_body = 1;
@@ -753,7 +803,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         CombineArgs cb;
         cb.pm = w.pm;
         cb.pb = w.pb;
-        cb.groups = w.groups;
+        cb.groups = step_tune[step].groups;
         cb.items = (int)items;
         const uint32_t nin = nout / 2;
         cb.body_prev = first ? w.lwe_f + n : w.body[cur];
@@ -763,6 +813,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         cb.body_next = w.body[cur ^ 1];
         cb.dig_next = w.dig;
         launch_cascade_combine(r, gck, dpk, r1n, cb, st);
+        if (c->time_decomp) HIP_TRY(hipEventRecord(c->ev_dec[step + 1], st));
         cur ^= 1;
     }
     // rlwe_to_rgsw_eval (convert.rs:162-180) with each client's conversion
@@ -777,14 +828,16 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         a.items_per_key = (int)B;
         a.key_slots = n;
         a.tiles = 1;
-        a.groups = w.groups;
+        a.groups = step_tune[kDecompSteps - 1].groups;
+        a.wide = step_tune[kDecompSteps - 1].wide;
         a.out_mask = w.pm;
         a.out_body = w.pb;
         a.reduce_body = 0;
         a.items = (int)items;
         launch_gadget_mac(r1n, gck, a, st);
-        launch_r2g_combine(r1n, w.pm, w.pb, w.groups, w.masks[cur], w.body[cur], w.rgsw, (int)(Q * bits), (int)lq,
+        launch_r2g_combine(r1n, w.pm, w.pb, a.groups, w.masks[cur], w.body[cur], w.rgsw, (int)(Q * bits), (int)lq,
                            st);
+        if (c->time_decomp) HIP_TRY(hipEventRecord(c->ev_dec[kDecompSteps], st));
     }
     const size_t rgw = 4 * lq * nl * n;  // words per RGSW at q1
     c->stage_ptr[0] = w.rgsw;
@@ -1097,6 +1150,61 @@ via_status via_answer_many_device(via_ctx *c, const via_db *db, uint32_t nq, con
         via_status s = answer_batch_impl(c, db, Q, keys + q0, d_lwes + q0, n_lwe, d_masks + q0, d_bodies + q0, err);
         if (s) return s;
     }
+    return VIA_OK;
+}
+
+// Autotuner: runs the batched answer with each candidate launch shape
+// (tune_candidates) for every decompression step, `reps` timed runs each
+// after one untimed run, timing each step with HIP events on the context
+// stream, and keeps the fastest shape per step for this batch size.
+via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via_keys *const *keys,
+                            const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_mask,
+                            uint64_t *const *d_body, int reps, int *groups_out, int *wide_out) {
+    if (!c || !db || !keys || !d_lwes || !d_mask || !d_body || nq < 1 || nq > (uint32_t)kMaxQ || reps < 1)
+        return VIA_E_INVALID;
+    for (uint32_t q = 0; q < nq; q++)
+        if (!keys[q] || !d_lwes[q] || !d_mask[q] || !d_body[q]) return VIA_E_INVALID;
+    HIP_TRY(hipSetDevice(c->device));
+    struct Guard {
+        via_ctx *c;
+        ~Guard() {
+            c->time_decomp = false;
+            c->tune_override.clear();
+        }
+    } guard{c};
+    const int Q = (int)nq;
+    via_status s = answer_batch_impl(c, db, Q, keys, d_lwes, n_lwe, d_mask, d_body, nullptr);
+    if (s) return s;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int L = (int)keys[0]->pp.l_ck;
+    std::vector<StepTune> cands;
+    for (const StepTune &t : tune_candidates((int)(Q * n_lwe), L))
+        if (t.groups <= c->ws.groups_max) cands.push_back(t);
+    std::vector<std::vector<double>> t(cands.size(), std::vector<double>(kDecompSteps, 0.0));
+    c->time_decomp = true;
+    for (size_t ci = 0; ci < cands.size(); ci++) {
+        c->tune_override.assign(kDecompSteps, cands[ci]);
+        for (int r = 0; r <= reps; r++) {
+            if ((s = answer_batch_impl(c, db, Q, keys, d_lwes, n_lwe, d_mask, d_body, nullptr))) return s;
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            if (r == 0) continue;
+            for (int i = 0; i < kDecompSteps; i++) {
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, c->ev_dec[i], c->ev_dec[i + 1]));
+                t[ci][i] += ms;
+            }
+        }
+    }
+    std::vector<StepTune> best(kDecompSteps);
+    for (int i = 0; i < kDecompSteps; i++) {
+        size_t bi = 0;
+        for (size_t ci = 1; ci < cands.size(); ci++)
+            if (t[ci][i] < t[bi][i]) bi = ci;
+        best[i] = cands[bi];
+        if (groups_out) groups_out[i] = best[i].groups;
+        if (wide_out) wide_out[i] = best[i].wide;
+    }
+    c->tuned[Q] = best;
     return VIA_OK;
 }
 

@@ -110,6 +110,9 @@ def lib():
             "via_ctx_synchronize": (C.c_int, [vp]),
             "via_ctx_first_dim_time": (C.c_int, [vp, C.POINTER(C.c_double)]),
             "via_ctx_mark": (C.c_int, [vp, C.c_int]),
+            "via_ctx_autotune": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.c_size_t,
+                                           C.POINTER(vp), C.POINTER(vp), C.c_int, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int)]),
             "via_ctx_marked_time": (C.c_int, [vp, C.POINTER(C.c_double)]),
             "via_db_create": (C.c_int, [vp, P, u64p, C.c_size_t, C.POINTER(vp)]),
             "via_db_create_bytes": (C.c_int, [vp, P, C.c_char_p, C.c_size_t, C.POINTER(vp)]),
@@ -210,6 +213,20 @@ class Context:
         t = C.c_double()
         _check(lib().via_ctx_first_dim_time(self.h, C.byref(t)))
         return t.value
+
+    def autotune(self, db, keys, d_lwes, n_lwe, d_masks, d_bodies, reps=2):
+        """via_ctx_autotune: per-step launch shapes for batches of len(keys)
+        queries (device buffers); returns the twelve (groups, wide) choices."""
+        nq = len(keys)
+        vp = C.c_void_p
+        K = (vp * nq)(*[k.h.value for k in keys])
+        Qp = (vp * nq)(*[d.value for d in d_lwes])
+        M = (vp * nq)(*[d.value for d in d_masks])
+        B = (vp * nq)(*[d.value for d in d_bodies])
+        g = (C.c_int * 12)()
+        w = (C.c_int * 12)()
+        _check(lib().via_ctx_autotune(self.h, db.h, nq, K, Qp, n_lwe, M, B, reps, g, w))
+        return list(zip(list(g), list(w)))
 
     def mark(self, which):
         """Record the start (0) or stop (1) event of the stream timer."""
