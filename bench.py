@@ -23,6 +23,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM traffic of one FirstDim launch at I = J = 256 with one query per pass,
+# measured with rocprofv3 --pmc on this command: FETCH_SIZE 532530 KiB and
+# WRITE_SIZE 8192 KiB per dispatch; a 256 MiB device copy in the same runs
+# reads FETCH_SIZE 131127 KiB and WRITE_SIZE 262144 KiB, so fetched bytes are
+# 2.000 x FETCH_SIZE and written bytes 1.000 x WRITE_SIZE.
+FD_TRAFFIC = {(256, 256, 1): (2.0 * 532530.0 + 8192.0) * 1024}
+
 
 def synthetic_keys(pp, rng):
     """One client's public keys (cascade, conversion, ring switch) as uniform
@@ -114,21 +121,40 @@ def cpu_baseline(pp, cascade, conv, rsk, lwes, record_bytes, n_rec):
     return am, ab, t_ans, t_setup, list(steps)
 
 
-def serve_run(via_gpu, pp, db, n_clients, qps, n_req, max_batch, slots, seed=7, db_b=None, frac_b=0.0,
-              b_sizes=(4, 64)):
-    """Poisson arrivals at `qps` from `n_clients` admitted clients through the
-    continuous-batching loop. With db_b, a fraction frac_b of the requests are
-    VIA-B requests for a number of records uniform in b_sizes, which the client
-    splits into batches of T = n2 / n_rec(db_b) queries, padding the last one
-    with index-0 queries. Returns answers/s, records/s and the latency
-    percentiles of the requests (a VIA-B request completes with its last batch)."""
+def tune_batches(via_gpu, ctx, pp, db, keys, sizes, seed=11):
+    """Autotune the query-decompression launch shapes for each batch size in
+    `sizes` (the choice is shared by every context on the device, the serving
+    loop's included). Returns {size: [(groups, wide) x 12]}."""
+    rng = np.random.default_rng(seed)
+    out = {}
+    for Q in sizes:
+        qs = [synthetic_query(pp, rng) for _ in range(Q)]
+        d_q = [ctx.alloc(q.nbytes) for q in qs]
+        for d, q in zip(d_q, qs):
+            ctx.to_device(d, q)
+        d_m = [ctx.alloc(8 * pp.n2) for _ in range(Q)]
+        d_b = [ctx.alloc(8 * pp.n2) for _ in range(Q)]
+        out[Q] = ctx.autotune(db, [keys] * Q, d_q, pp.query_lwes, d_m, d_b, reps=2)
+        for d in d_q + d_m + d_b:
+            ctx.free(d)
+    return out
+
+
+def serve_setup(via_gpu, pp, db, n_clients, max_batch, slots, seed=7, db_b=None, key_sets=8):
+    """A serving loop with n_clients admitted clients (their eval keys
+    resident in HBM under the loop's key budget) and one VIA-C query (and,
+    with db_b, one VIA-B batch of T = n2 / n_rec(db_b) queries) per client.
+    Key material cycles through `key_sets` distinct synthetic sets: every
+    client still gets its own device copy, and the work of an answer does not
+    depend on the residues."""
     rng = np.random.default_rng(seed)
     kpp = db_b.pp if db_b is not None else pp
     T = pp.n2 // kpp.n_rec if db_b is not None else 0
     loop = via_gpu.ServeLoop(pp, db, key_budget_bytes=64 << 30, max_batch=max_batch, slots=slots, db_b=db_b)
+    sets = [synthetic_keys(kpp, rng) for _ in range(min(key_sets, n_clients))]
     queries, bqueries = [], []
     for c in range(n_clients):
-        loop.admit(c, *synthetic_keys(kpp, rng))
+        loop.admit(c, *sets[c % len(sets)])
         queries.append(synthetic_query(pp, rng))
         if db_b is not None:
             bqueries.append([synthetic_query(kpp, rng) for _ in range(T)])
@@ -138,10 +164,22 @@ def serve_run(via_gpu, pp, db, n_clients, qps, n_req, max_batch, slots, seed=7, 
         warm += [loop.submit_batch(c % n_clients, T, bqueries[c % n_clients]) for c in range(2 * slots)]
     for t in warm:
         loop.wait(t)
+    return loop, queries, bqueries, T
+
+
+def serve_run(loop, queries, bqueries, T, qps, n_req, seed=7, frac_b=0.0, b_sizes=(4, 64)):
+    """Poisson arrivals at `qps` through the continuous-batching loop. A
+    fraction frac_b of the requests are VIA-B requests for a number of
+    records uniform in b_sizes, which the client splits into batches of T
+    queries, padding the last one with index-0 queries. Returns answers/s,
+    records/s and the latency percentiles of the requests (a VIA-B request
+    completes with its last batch)."""
+    rng = np.random.default_rng(seed)
+    n_clients = len(queries)
     loop.reset_stats()
     gaps = rng.exponential(1.0 / qps, size=n_req)
     who = rng.integers(0, n_clients, size=n_req)
-    is_b = (rng.random(n_req) < frac_b) if db_b is not None else np.zeros(n_req, bool)
+    is_b = (rng.random(n_req) < frac_b) if bqueries else np.zeros(n_req, bool)
     sizes = rng.integers(b_sizes[0], b_sizes[1] + 1, size=n_req)
     reqs = []  # (kind, records, [tickets])
     t0 = time.perf_counter()
@@ -166,7 +204,6 @@ def serve_run(via_gpu, pp, db, n_clients, qps, n_req, max_batch, slots, seed=7, 
         (lat_b if kind == "b" else lat_c).append(worst)
     elapsed = time.perf_counter() - t0
     st = loop.stats()
-    loop.close()
     pct = lambda v, q: round(float(np.percentile(np.array(v) * 1e3, q)), 3) if v else None
     allv = lat_c + lat_b
     out = {"offered_qps": qps, "answers_per_s": round(st["answered"] / elapsed, 2),
@@ -174,7 +211,7 @@ def serve_run(via_gpu, pp, db, n_clients, qps, n_req, max_batch, slots, seed=7, 
            "records_per_s": round(sum(r for _, r, _ in reqs) / elapsed, 2),
            "p50_ms": pct(allv, 50), "p99_ms": pct(allv, 99), "batches": st["batches"],
            "max_batch_seen": st["max_batch_seen"], "requests": n_req}
-    if db_b is not None:
+    if bqueries:
         out.update({"via_b_requests": len(lat_b), "via_b_T": T, "p50_c_ms": pct(lat_c, 50), "p99_c_ms": pct(lat_c, 99),
                     "p50_b_ms": pct(lat_b, 50), "p99_b_ms": pct(lat_b, 99)})
     return out
@@ -330,6 +367,56 @@ def ntt_run(via_gpu, args):
     ctx.close()
 
 
+def batched_run(via_gpu, ctx, pp, db, Q, args, seed=31):
+    """Q queries from Q distinct clients answered per pass (via_answer_many):
+    each client has its own device copy of its keys (key material cycles
+    through eight synthetic sets; the work does not depend on the residues).
+    Autotuned for Q, warmed, then `steps` timed passes."""
+    rng = np.random.default_rng(seed)
+    sets = [synthetic_keys(pp, rng) for _ in range(min(8, Q))]
+    keys = [via_gpu.PreparedKeys(ctx, pp, *sets[q % len(sets)]) for q in range(Q)]
+    queries = [synthetic_query(pp, rng) for _ in range(Q)]
+    d_q = [ctx.alloc(q.nbytes) for q in queries]
+    for d, q in zip(d_q, queries):
+        ctx.to_device(d, q)
+    d_m = [ctx.alloc(8 * pp.n2) for _ in range(Q)]
+    d_b = [ctx.alloc(8 * pp.n2) for _ in range(Q)]
+    L = via_gpu.lib()
+    err = via_gpu.ErrDetail()
+    vp = C.c_void_p
+    Karr = (vp * Q)(*[k.h.value for k in keys])
+    Qarr = (vp * Q)(*[d.value for d in d_q])
+    Marr = (vp * Q)(*[d.value for d in d_m])
+    Barr = (vp * Q)(*[d.value for d in d_b])
+
+    def answer():
+        st = L.via_answer_many_device(ctx.h, db.h, Q, Karr, Qarr, pp.query_lwes, Marr, Barr, C.byref(err))
+        if st:
+            raise via_gpu.ViaError(st, msg=L.via_status_str(st).decode())
+
+    if not args.no_tune:
+        ctx.autotune(db, keys, d_q, pp.query_lwes, d_m, d_b, reps=2)
+    for _ in range(max(1, args.warmup)):
+        answer()
+    ctx.synchronize()
+    lat = []
+    steps = max(3, args.steps // 2)
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        answer()
+        ctx.synchronize()
+        lat.append(time.perf_counter() - t0)
+    elapsed = time.perf_counter() - t_start
+    lat_ms = np.array(lat) * 1e3
+    for d in d_q + d_m + d_b:
+        ctx.free(d)
+    for k in keys:
+        k.close()
+    return {"queries_per_pass": Q, "answers_per_s": round(steps * Q / elapsed, 2), "passes": steps,
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 3), "p99_ms": round(float(np.percentile(lat_ms, 99)), 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -339,8 +426,11 @@ def main():
     ap.add_argument("--cols", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=1, help="queries per batched pass (distinct clients)")
+    ap.add_argument("--batched", type=int, default=32,
+                    help="with one query per pass: also measure this many queries per pass (0: skip)")
     ap.add_argument("--serve", action="store_true", help="open-loop Poisson arrivals through the serving loop")
     ap.add_argument("--qps", type=float, default=0.0, help="serving arrival rate (0: sweep)")
+    ap.add_argument("--rates", type=str, default="", help="serving: comma-separated arrival rates to sweep")
     ap.add_argument("--clients", type=int, default=64)
     ap.add_argument("--requests", type=int, default=600)
     ap.add_argument("--mix", action="store_true", help="serving: 30%% VIA-B requests against a second database")
@@ -378,7 +468,10 @@ def main():
     t_setup = time.perf_counter() - t0
 
     if args.serve:
-        rates = [args.qps] if args.qps > 0 else [200.0, 400.0, 800.0, 1200.0, 1600.0, 2400.0]
+        if args.rates:
+            rates = [float(x) for x in args.rates.split(",")]
+        else:
+            rates = [args.qps] if args.qps > 0 else [200.0, 400.0, 800.0, 1200.0, 1600.0, 2400.0]
         db_b = None
         if args.mix:
             pb = via_gpu.paper_b_params(args.rows, args.cols, T=8)
@@ -386,11 +479,16 @@ def main():
             rb = np.random.default_rng(4097).integers(0, 256, size=nb_rec * pb.n_rec // 2, dtype=np.uint8)
             db_b = via_gpu.PreparedDb(ctx, pb, record_bytes=rb.tobytes(), n_records=nb_rec)
             del rb
+        tuned = None
+        if not args.no_tune:
+            tuned = tune_batches(via_gpu, ctx, pp, db, keys, [q for q in (1, 2, 4, 8, 16, 32) if q <= args.batch])
+        loop, queries, bqueries, T = serve_setup(via_gpu, pp, db, args.clients, max_batch=args.batch, slots=2,
+                                                 db_b=db_b)
         runs = []
         for r in rates:
-            runs.append(serve_run(via_gpu, pp, db, args.clients, r, args.requests, max_batch=args.batch,
-                                  slots=2, db_b=db_b, frac_b=0.3 if args.mix else 0.0))
+            runs.append(serve_run(loop, queries, bqueries, T, r, args.requests, frac_b=0.3 if args.mix else 0.0))
             print(json.dumps({"serving": runs[-1]}), file=sys.stderr)
+        loop.close()
         best = max(runs, key=lambda x: x["answers_per_s"])
         out = {
             "metric": "server answers/sec + p50/p99 answer latency, VIA-C n=2048, bit-exact vs CPU",
@@ -404,6 +502,7 @@ def main():
                                        if args.mix else "VIA-C queries, ")
                                     + f"VIA-C batches <= {args.batch}, 2 slots, I={pp.num_rows} J={pp.num_cols}")},
             "sweep": runs,
+            "tuned_digit_groups": tuned,
         }
         print(json.dumps(out))
         ctx.close()
@@ -501,9 +600,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(fd_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": FD_TRAFFIC.get((I, J, Q)),
+            "algorithmic_bytes": fd_bytes,
         },
     }
+    if args.batched > 1 and Q == 1:
+        out["batched"] = batched_run(via_gpu, ctx, pp, db, args.batched, args)
+        out["batched"]["vs_one_query"] = round(out["batched"]["answers_per_s"] / out["value"], 2)
     if not args.no_cpu:
         am, ab, t_cpu, t_cpu_setup, cpu_steps = cpu_baseline(pp, cascade, conv, rsk, lwes, record_bytes, n_rec)
         from oracle.vo import Oracle

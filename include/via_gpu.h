@@ -133,10 +133,13 @@ via_status via_answer_many_device(via_ctx *ctx, const via_db *db, uint32_t nq, c
  * (the number of digit groups, i.e. workgroups splitting the 18 digits of one
  * gadget product, and the kernel variant: 256 threads with one digit
  * transform at a time, or 1024 threads with several side by side), reps timed
- * runs per candidate, and keeps the fastest per step for batches of nq queries
- * on this context. groups_out and wide_out (12 entries each: the eleven
+ * runs per candidate, and keeps the fastest per step for batches of nq
+ * queries. groups_out and wide_out (12 entries each: the eleven
  * cascade steps and the RLWE -> RGSW conversion) receive the choice when not
- * null. The answers written meanwhile are those of via_answer_many_device. */
+ * null. The choice serves every context of the process on the same device
+ * (the serving loop's included); an untuned batch size takes the shapes of
+ * the largest tuned size below it. The answers written meanwhile are those
+ * of via_answer_many_device. */
 via_status via_ctx_autotune(via_ctx *ctx, const via_db *db, uint32_t nq, const via_keys *const *keys,
                             const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_masks,
                             uint64_t *const *d_bodies, int reps, int *groups_out, int *wide_out);

@@ -1,7 +1,10 @@
 // Host check of via-rs_amd/csrc/ntt_lazy.h: the lazy radix-2^R passes, under
 // every pass schedule the kernels use, give the radix-2 reference schedule's
 // canonical result exactly, and every intermediate stays inside the bounds the
-// header states (|v| < 24q forward, < 64q inverse; lazy_mul operands < 2^45).
+// header states (|v| < 24q forward, < 64q inverse; lazy_mul operands < 2^45);
+// the fixed-shape units over the padded LDS layout (the form the kernels
+// run) give the same results for every degree and schedule, two limbs and
+// many segments per buffer, without touching the pad words.
 // Built and run by tests/test_host_arith.py; prints "OK <count>".
 #include <cmath>
 #include <cstdint>
@@ -69,6 +72,115 @@ static void lazy_inv(double *x, int logn, int maxr, const double *tw, const FMod
         }
         lh0 += rs[i];
     }
+}
+
+// Host mirror of the device pass drivers over the padded layout: every unit
+// of every segment of an NL x S buffer, one pass after the other.
+template <int LOGN, int MAXR, int P>
+static void fwd_passes_host(double *s, int S, int NL, const double *const *tw, const FMod *m) {
+    if constexpr (P < sched_np(LOGN, MAXR)) {
+        constexpr int R = sched_r(LOGN, MAXR, P), L0 = sched_l0(LOGN, MAXR, P), USEG = LOGN - R;
+        const int upl = S >> R;
+        for (int t = 0; t < NL * upl; t++) {
+            const int l = t / upl, ua = t % upl, seg = ua >> USEG, u = ua & ((1 << USEG) - 1);
+            ntt_fwd_unit_p<R, LOGN, L0>(s, l * S + (seg << LOGN), u, tw[l], m[l]);
+        }
+        fwd_passes_host<LOGN, MAXR, P + 1>(s, S, NL, tw, m);
+    }
+}
+
+template <int LOGN, int MAXR, int P>
+static void inv_passes_host(double *s, int S, int NL, const double *const *tw, const FMod *m) {
+    if constexpr (P < sched_np(LOGN, MAXR)) {
+        constexpr int R = sched_r(LOGN, MAXR, P), LH0 = sched_l0(LOGN, MAXR, P), USEG = LOGN - R;
+        const int upl = S >> R;
+        for (int t = 0; t < NL * upl; t++) {
+            const int l = t / upl, ua = t % upl, seg = ua >> USEG, u = ua & ((1 << USEG) - 1);
+            ntt_inv_unit_p<R, LOGN, LH0>(s, l * S + (seg << LOGN), u, tw[l], m[l]);
+        }
+        inv_passes_host<LOGN, MAXR, P + 1>(s, S, NL, tw, m);
+    }
+}
+
+// The padded fixed-shape passes against the reference schedule: two limbs
+// (primes q0, q1), S = 2048 logical words per limb holding 2048 / n segments,
+// every pass schedule of at most MAXR stages.
+template <int LOGN, int MAXR>
+static int check_padded(std::mt19937_64 &rng, uint64_t q0, uint64_t q1, long &count) {
+    const int n = 1 << LOGN, S = 2048, NL = 2;
+    const uint64_t qs[2] = {q0, q1};
+    const FMod m[2] = {fmod_make(q0), fmod_make(q1)};
+    std::vector<double> tw[2], itw[2];
+    std::vector<uint64_t> twu[2], itwu[2];
+    for (int l = 0; l < 2; l++) {
+        twu[l].resize(n);
+        itwu[l].resize(n);
+        for (int i = 0; i < n; i++) {
+            twu[l][i] = rng() % qs[l];
+            itwu[l][i] = rng() % qs[l];
+        }
+        tw[l].assign(twu[l].begin(), twu[l].end());
+        itw[l].assign(itwu[l].begin(), itwu[l].end());
+    }
+    const double *twp[2] = {tw[0].data(), tw[1].data()}, *itwp[2] = {itw[0].data(), itw[1].data()};
+    std::vector<uint64_t> x(NL * S);
+    std::vector<double> buf(lds_words(NL * S) + 16, 0.0);
+    for (int l = 0; l < NL; l++)
+        for (int i = 0; i < S; i++) {
+            x[l * S + i] = rng() % qs[l];
+            buf[lds_pad(l * S + i)] = (double)x[l * S + i];
+        }
+    std::vector<double> ibuf = buf;
+    fwd_passes_host<LOGN, MAXR, 0>(buf.data(), S, NL, twp, m);
+    inv_passes_host<LOGN, MAXR, 0>(ibuf.data(), S, NL, itwp, m);
+    int fails = 0;
+    for (int l = 0; l < NL; l++)
+        for (int seg = 0; seg < S / n; seg++) {
+            std::vector<uint64_t> yf(x.begin() + l * S + seg * n, x.begin() + l * S + seg * n + n), yi = yf;
+            const uint64_t q = qs[l];
+            for (int lm = 0; lm < LOGN; lm++) {
+                const int mb = 1 << lm, h = n >> (lm + 1);
+                for (int b = 0; b < mb; b++)
+                    for (int j = 0; j < h; j++) {
+                        const int i0 = b * 2 * h + j, i1 = i0 + h;
+                        const uint64_t t = mulm(yf[i1], twu[l][mb + b], q), a = yf[i0];
+                        yf[i0] = addm(a, t, q);
+                        yf[i1] = subm(a, t, q);
+                    }
+            }
+            for (int lh = 0; lh < LOGN; lh++) {
+                const int h = 1 << lh, mb = n >> (lh + 1);
+                for (int b = 0; b < mb; b++)
+                    for (int j = 0; j < h; j++) {
+                        const int i0 = b * 2 * h + j, i1 = i0 + h;
+                        const uint64_t a = yi[i0], c = yi[i1];
+                        yi[i0] = addm(a, c, q);
+                        yi[i1] = mulm(subm(a, c, q), itwu[l][mb + b], q);
+                    }
+            }
+            for (int i = 0; i < n; i++) {
+                const int at = lds_pad(l * S + seg * n + i);
+                if (canon(buf[at], q) != yf[i] || canon(ibuf[at], q) != yi[i]) {
+                    if (fails++ < 5) printf("padded logn=%d maxr=%d limb=%d seg=%d i=%d\n", LOGN, MAXR, l, seg, i);
+                    break;
+                }
+            }
+            count++;
+        }
+    // the pad words are never written
+    for (int i = 0; i < lds_words(NL * S); i++)
+        if ((i % 17) == 16 && (buf[i] != 0.0 || ibuf[i] != 0.0)) fails++;
+    return fails;
+}
+
+template <int LOGN>
+static int check_padded_all(std::mt19937_64 &rng, long &count) {
+    int f = 0;
+    f += check_padded<LOGN, 2>(rng, 137438822401ULL, 274810798081ULL, count);
+    f += check_padded<LOGN, 3>(rng, 137438822401ULL, 274810798081ULL, count);
+    f += check_padded<LOGN, 4>(rng, 268369921ULL, 536608769ULL, count);
+    f += check_padded<LOGN, 3>(rng, 17175674881ULL, 8380417ULL, count);
+    return f;
 }
 
 int main() {
@@ -213,6 +325,10 @@ int main() {
             count++;
         }
     }
+    fails += check_padded_all<1>(rng, count) + check_padded_all<2>(rng, count) + check_padded_all<3>(rng, count) +
+             check_padded_all<4>(rng, count) + check_padded_all<5>(rng, count) + check_padded_all<6>(rng, count) +
+             check_padded_all<7>(rng, count) + check_padded_all<8>(rng, count) + check_padded_all<9>(rng, count) +
+             check_padded_all<10>(rng, count) + check_padded_all<11>(rng, count);
     if (fails) {
         printf("FAIL %d\n", fails);
         return 1;

@@ -70,9 +70,6 @@ struct via_ctx {
     hipEvent_t ev_mark[2] = {};  // via_ctx_mark start / stop
     hipEvent_t ev_dec[kDecompSteps + 1] = {};  // around each decompression step (autotuner)
     bool time_decomp = false;
-    // launch shape per decompression step, chosen by via_ctx_autotune for a
-    // batch size (number of queries per pass); absent: the heuristic
-    std::map<int, std::vector<StepTune>> tuned;
     std::vector<StepTune> tune_override;  // set while the autotuner measures
     bool have_times = false;
     // pointers to the last answer's stage buffers
@@ -101,6 +98,12 @@ struct via_keys {
 
 
 namespace via {
+// Launch shapes chosen by via_ctx_autotune, shared by every context of the
+// process on the same device (the serving loop's slots included): per
+// device and batch size, the twelve decompression steps. A batch size that
+// was not tuned takes the largest tuned size below it.
+void tuned_store(int device, int Q, const std::vector<StepTune> &steps);
+bool tuned_lookup(int device, int Q, std::vector<StepTune> *steps);
 // Answers Q <= kMaxQ queries (device buffers) against one database on the
 // context's stream, asynchronously.
 via_status answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via_keys *const *keys,

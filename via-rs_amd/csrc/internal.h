@@ -119,6 +119,7 @@ struct GateArgs {
     int items_per_query;
     int queries;
     int shift;           // CRot rotation N - 2^i
+    DigitPlan dp;        // digit plan of the gadget (set by launch_gate)
 };
 void launch_gate(const RingDev &r, const Gadget &g, const GateArgs &a, hipStream_t s);
 
@@ -154,6 +155,7 @@ struct RespArgs {
     uint64_t q2, q3, q4;
     NttTab t3;        // q3 at n2
     int queries;
+    DigitPlan dp;     // digit plan of (b_rsk, l_rsk) (set by launch_resp_comp)
 };
 void launch_resp_comp(const RespArgs &a, hipStream_t s);
 

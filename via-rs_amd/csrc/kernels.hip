@@ -410,21 +410,21 @@ __global__ __launch_bounds__(TPB) void k_gate(RingDev r, Gadget g, GateArgs ga) 
 #pragma unroll
         for (int e = 0; e < PER; e++) am[l][e] = ab[l][e] = 0.0;
     for (int half = 0; half < 2; half++) {  // 0: ct.mask with -s*m, 1: ct.body with m
-        SMag v[PER];
+        int64_t vlo[PER], vhi[PER];
 #pragma unroll
         for (int e = 0; e < PER; e++) {
             const int sg = threadIdx.x + e * TPB;
             double b0, c0, b1 = 0.0, c1 = 0.0;
             gate_input<NL>(mode, in, j, half, 0, sg, n, shift, m0, b0, c0);
             if (NL == 2) gate_input<NL>(mode, in, j, half, 1, sg, n, shift, m1, b1, c1);
-            v[e] = gadget_scale<NL>(c0, c1, r, g);
+            digit_split(gadget_scale<NL>(c0, c1, r, g), ga.dp, vlo[e], vhi[e]);
         }
         const double *key = rgsw + (size_t)half * g.L * 2 * pw;
         for (int k = 0; k < g.L; k++) {
 #pragma unroll
             for (int e = 0; e < PER; e++) {
                 const int sg = threadIdx.x + e * TPB;
-                const double d = (double)extract_digit(v[e], g.base);
+                const double d = (double)next_digit(vlo[e], vhi[e], k, ga.dp);
                 s[lds_pad(sg)] = d;
                 if (NL == 2) s[lds_pad(S + sg)] = d;
             }
@@ -513,9 +513,10 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
             double b0, c0, b1 = 0.0, c1 = 0.0;
             gate_input<NL>(mode, in, j, half, 0, c, n, shift, m0, b0, c0);
             if (NL == 2) gate_input<NL>(mode, in, j, half, 1, c, n, shift, m1, b1, c1);
-            SMag v = gadget_scale<NL>(c0, c1, r, g);
+            int64_t lo, hi;
+            digit_split(gadget_scale<NL>(c0, c1, r, g), ga.dp, lo, hi);
             for (int k = 0; k < L; k++) {
-                const double d = (double)extract_digit(v, g.base);
+                const double d = (double)next_digit(lo, hi, k, ga.dp);
                 const int b0 = (half * L + k) * NL * S;
                 s[lds_pad(b0 + c)] = d;
                 if (NL == 2) s[lds_pad(b0 + S + c)] = d;
@@ -586,7 +587,9 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
         }
 }
 
-void launch_gate(const RingDev &r, const Gadget &g, const GateArgs &a, hipStream_t st) {
+void launch_gate(const RingDev &r, const Gadget &g, const GateArgs &a_in, hipStream_t st) {
+    GateArgs a = a_in;
+    a.dp = digit_plan(g.base, g.L);
     const int blocks = a.items_per_query * a.queries;
     if (g.L <= 2 && r.n == kSlots) {
         if (r.nl == 2)
@@ -861,8 +864,10 @@ __global__ __launch_bounds__(1024) void k_resp_comp(RespArgs a) {
             SMag v;
             v.mag = (via_u128)((mag * bl + (q3 >> 1)) / q3);
             v.neg = neg && v.mag != 0;
+            int64_t lo, hi;
+            digit_split(v, a.dp, lo, hi);
             for (int k = 0; k < L; k++) {
-                const int dg = extract_digit(v, a.b_rsk);
+                const int dg = next_digit(lo, hi, k, a.dp);
                 const int pidx = j * L + k;
                 if (pidx >= p0 && pidx < p1) buf[lds_pad((pidx - p0) * n2 + i)] = (double)dg;
             }
@@ -902,7 +907,9 @@ __global__ __launch_bounds__(1024) void k_resp_comp(RespArgs a) {
     }
 }
 
-void launch_resp_comp(const RespArgs &a, hipStream_t st) {
+void launch_resp_comp(const RespArgs &a_in, hipStream_t st) {
+    RespArgs a = a_in;
+    a.dp = digit_plan(a.b_rsk, a.l_rsk);
     hipLaunchKernelGGL(k_resp_comp, dim3(a.queries), dim3(1024), 0, st, a);
 }
 

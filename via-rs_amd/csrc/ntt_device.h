@@ -51,54 +51,68 @@ __device__ __forceinline__ NttLimbs limbs_inv(const RingDev &r) {
 // 32 words, 16 lanes per LDS cycle) fall into distinct banks. Logical word i
 // of a buffer lives at lds_pad(i); a buffer of w logical words takes
 // lds_words(w).
-__host__ __device__ constexpr int lds_words(int w) { return w + w / 16; }
-__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
-
-// LDS view for the pass kernels of ntt_lazy.h: a segment starting at logical
-// word `base` of a padded buffer.
-struct LdsMem {
-    double *s;
-    int base;
-    __device__ __forceinline__ double ld(int i) const { return s[lds_pad(base + i)]; }
-    __device__ __forceinline__ void st(int i, double v) const { s[lds_pad(base + i)] = v; }
-};
+// (lds_words and lds_pad are defined in ntt_lazy.h.)
 
 constexpr int kNttMaxR = 3;  // default stages per register pass (radix 8)
 
-// Forward transform of every degree-n segment of the NL x S buffer (padded
-// layout, lds_words(NL * S) words) by threads
-// tid = 0 .. nth-1 (a subset of the workgroup may transform its own buffer;
-// every thread of the workgroup must reach the pass barriers). Inputs
-// are exact integers in (-2q, 2q); outputs are lazy, in (-24q, 24q) for
-// n <= 2048 (ntt_lazy.h): reduce before storing canonical residues.
-template <int NL, int MAXR = kNttMaxR>
-__device__ void ntt_fwd(double *s, int S, int n, int logn, const NttLimbs &T, int tid, int nth) {
-    int rs[12];
-    const int np = ntt_schedule(logn, MAXR, rs);
-    int L0 = 0;
-    for (int pi = 0; pi < np; pi++) {
-        const int R = rs[pi];
+// Passes P, P+1, ... of a degree-2^LOGN transform over every segment of the
+// NL x S buffer, with the degree and schedule fixed at compile time.
+template <int NL, int LOGN, int MAXR, int P>
+__device__ __forceinline__ void fwd_passes(double *s, int S, const NttLimbs &T, int tid, int nth) {
+    if constexpr (P < sched_np(LOGN, MAXR)) {
+        constexpr int R = sched_r(LOGN, MAXR, P), L0 = sched_l0(LOGN, MAXR, P), USEG = LOGN - R;
         const int upl = S >> R;  // units per limb
-        const int useg = logn - R;
         for (int t = tid; t < NL * upl; t += nth) {
             const int l = (NL == 1) ? 0 : (t >= upl ? 1 : 0);
             const int ua = t - l * upl;
-            const int seg = ua >> useg;
-            const int u = ua & ((1 << useg) - 1);
-            const LdsMem p{s, l * S + seg * n};
-            const double *tw = l ? T.tw[1] : T.tw[0];
-            const FMod md = l ? T.m[1] : T.m[0];
-            if (MAXR >= 4 && R == 4)
-                ntt_fwd_unit<(MAXR >= 4 ? 4 : 1)>(p, logn, L0, u, tw, md);
-            else if (MAXR >= 3 && R == 3)
-                ntt_fwd_unit<(MAXR >= 3 ? 3 : 1)>(p, logn, L0, u, tw, md);
-            else if (R == 2)
-                ntt_fwd_unit<2>(p, logn, L0, u, tw, md);
-            else
-                ntt_fwd_unit<1>(p, logn, L0, u, tw, md);
+            const int seg = ua >> USEG;
+            const int u = ua & ((1 << USEG) - 1);
+            ntt_fwd_unit_p<R, LOGN, L0>(s, l * S + (seg << LOGN), u, l ? T.tw[1] : T.tw[0], l ? T.m[1] : T.m[0]);
         }
         __syncthreads();
-        L0 += R;
+        fwd_passes<NL, LOGN, MAXR, P + 1>(s, S, T, tid, nth);
+    }
+}
+
+template <int NL, int LOGN, int MAXR, int P>
+__device__ __forceinline__ void inv_passes(double *s, int S, const NttLimbs &T, int tid, int nth) {
+    if constexpr (P < sched_np(LOGN, MAXR)) {
+        constexpr int R = sched_r(LOGN, MAXR, P), LH0 = sched_l0(LOGN, MAXR, P), USEG = LOGN - R;
+        const int upl = S >> R;
+        for (int t = tid; t < NL * upl; t += nth) {
+            const int l = (NL == 1) ? 0 : (t >= upl ? 1 : 0);
+            const int ua = t - l * upl;
+            const int seg = ua >> USEG;
+            const int u = ua & ((1 << USEG) - 1);
+            ntt_inv_unit_p<R, LOGN, LH0>(s, l * S + (seg << LOGN), u, l ? T.tw[1] : T.tw[0], l ? T.m[1] : T.m[0]);
+        }
+        __syncthreads();
+        inv_passes<NL, LOGN, MAXR, P + 1>(s, S, T, tid, nth);
+    }
+}
+
+// Forward transform of every degree-n segment (n = 2^logn <= 2048) of the
+// NL x S buffer (padded layout, lds_words(NL * S) words; S a multiple of n)
+// by threads tid = 0 .. nth-1 (a subset of the workgroup may transform its
+// own buffer; every thread of the workgroup must reach the pass barriers).
+// Inputs are exact integers in (-2q, 2q); outputs are lazy, in (-24q, 24q)
+// (ntt_lazy.h): reduce before storing canonical residues.
+template <int NL, int MAXR = kNttMaxR>
+__device__ void ntt_fwd(double *s, int S, int n, int logn, const NttLimbs &T, int tid, int nth) {
+    (void)n;
+    switch (logn) {
+        case 1: fwd_passes<NL, 1, MAXR, 0>(s, S, T, tid, nth); break;
+        case 2: fwd_passes<NL, 2, MAXR, 0>(s, S, T, tid, nth); break;
+        case 3: fwd_passes<NL, 3, MAXR, 0>(s, S, T, tid, nth); break;
+        case 4: fwd_passes<NL, 4, MAXR, 0>(s, S, T, tid, nth); break;
+        case 5: fwd_passes<NL, 5, MAXR, 0>(s, S, T, tid, nth); break;
+        case 6: fwd_passes<NL, 6, MAXR, 0>(s, S, T, tid, nth); break;
+        case 7: fwd_passes<NL, 7, MAXR, 0>(s, S, T, tid, nth); break;
+        case 8: fwd_passes<NL, 8, MAXR, 0>(s, S, T, tid, nth); break;
+        case 9: fwd_passes<NL, 9, MAXR, 0>(s, S, T, tid, nth); break;
+        case 10: fwd_passes<NL, 10, MAXR, 0>(s, S, T, tid, nth); break;
+        case 11: fwd_passes<NL, 11, MAXR, 0>(s, S, T, tid, nth); break;
+        default: break;  // degree 1: the identity
     }
 }
 
@@ -108,32 +122,20 @@ __device__ void ntt_fwd(double *s, int S, int n, int logn, const NttLimbs &T, in
 // (-4q, 4q).
 template <int NL, int MAXR = kNttMaxR>
 __device__ void ntt_inv_noscale(double *s, int S, int n, int logn, const NttLimbs &T, int tid, int nth) {
-    int rs[12];
-    const int np = ntt_schedule(logn, MAXR, rs);
-    int lh0 = 0;
-    for (int pi = 0; pi < np; pi++) {
-        const int R = rs[pi];
-        const int upl = S >> R;
-        const int useg = logn - R;
-        for (int t = tid; t < NL * upl; t += nth) {
-            const int l = (NL == 1) ? 0 : (t >= upl ? 1 : 0);
-            const int ua = t - l * upl;
-            const int seg = ua >> useg;
-            const int u = ua & ((1 << useg) - 1);
-            const LdsMem p{s, l * S + seg * n};
-            const double *tw = l ? T.tw[1] : T.tw[0];
-            const FMod md = l ? T.m[1] : T.m[0];
-            if (MAXR >= 4 && R == 4)
-                ntt_inv_unit<(MAXR >= 4 ? 4 : 1)>(p, logn, lh0, u, tw, md);
-            else if (MAXR >= 3 && R == 3)
-                ntt_inv_unit<(MAXR >= 3 ? 3 : 1)>(p, logn, lh0, u, tw, md);
-            else if (R == 2)
-                ntt_inv_unit<2>(p, logn, lh0, u, tw, md);
-            else
-                ntt_inv_unit<1>(p, logn, lh0, u, tw, md);
-        }
-        __syncthreads();
-        lh0 += R;
+    (void)n;
+    switch (logn) {
+        case 1: inv_passes<NL, 1, MAXR, 0>(s, S, T, tid, nth); break;
+        case 2: inv_passes<NL, 2, MAXR, 0>(s, S, T, tid, nth); break;
+        case 3: inv_passes<NL, 3, MAXR, 0>(s, S, T, tid, nth); break;
+        case 4: inv_passes<NL, 4, MAXR, 0>(s, S, T, tid, nth); break;
+        case 5: inv_passes<NL, 5, MAXR, 0>(s, S, T, tid, nth); break;
+        case 6: inv_passes<NL, 6, MAXR, 0>(s, S, T, tid, nth); break;
+        case 7: inv_passes<NL, 7, MAXR, 0>(s, S, T, tid, nth); break;
+        case 8: inv_passes<NL, 8, MAXR, 0>(s, S, T, tid, nth); break;
+        case 9: inv_passes<NL, 9, MAXR, 0>(s, S, T, tid, nth); break;
+        case 10: inv_passes<NL, 10, MAXR, 0>(s, S, T, tid, nth); break;
+        case 11: inv_passes<NL, 11, MAXR, 0>(s, S, T, tid, nth); break;
+        default: break;
     }
 }
 

@@ -103,6 +103,93 @@ NTT_HD void ntt_inv_unit(Mem p, int logn, int lh0, int u, const double *tw, cons
     for (int t = 0; t < E; t++) p.st(base + t * stride, x[t]);
 }
 
+// ---- fixed-shape units over the padded LDS layout -------------------------------
+//
+// Transform buffers in LDS carry one pad word per 16 (logical word i at
+// lds_pad(i)). For a unit of 2^R elements at logical base + t*stride, with
+// the segment start a multiple of the degree, the elements never straddle a
+// pad differently from the unit's first one, so element t sits at
+// lds_pad(base) + t*stride + ((t*stride) >> 4): with the degree and the pass
+// known at compile time these are immediate offsets.
+
+NTT_HD constexpr int lds_words(int w) { return w + w / 16; }
+NTT_HD int lds_pad(int i) { return i + (i >> 4); }
+
+template <int STRIDE>
+NTT_HD constexpr int unit_off(int t) {
+    return t * STRIDE + ((t * STRIDE) >> 4);
+}
+
+// Forward pass from stage L0 over R stages of a degree-2^LOGN segment whose
+// logical start is seg_base (a multiple of 2^LOGN), unit u: the same
+// butterflies and twiddles as ntt_fwd_unit.
+template <int R, int LOGN, int L0>
+NTT_HD void ntt_fwd_unit_p(double *s, int seg_base, int u, const double *tw, const FMod &m) {
+    constexpr int E = 1 << R;
+    constexpr int JB = LOGN - L0 - R;
+    constexpr int STRIDE = 1 << JB;
+    const int SB = u >> JB;
+    const int j = u & (STRIDE - 1);
+    double *p = s + lds_pad(seg_base + (SB << (LOGN - L0)) + j);
+    double x[E];
+#pragma unroll
+    for (int t = 0; t < E; t++) x[t] = p[unit_off<STRIDE>(t)];
+#pragma unroll
+    for (int st = 0; st < R; st++) {
+        const int half = E >> (st + 1);
+        const double *w = tw + (1 << (L0 + st)) + (SB << st);
+#pragma unroll
+        for (int t = 0; t < E; t++) {
+            if (t & half) continue;
+            const double y = mod_mul_lazy(x[t + half], w[t >> (R - st)], m);
+            const double a = x[t];
+            x[t] = a + y;
+            x[t + half] = a - y;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < E; t++) p[unit_off<STRIDE>(t)] = x[t];
+}
+
+// Inverse pass over stages LH0 .. LH0+R-1, unit u: the same butterflies,
+// twiddles and reductions as ntt_inv_unit.
+template <int R, int LOGN, int LH0>
+NTT_HD void ntt_inv_unit_p(double *s, int seg_base, int u, const double *tw, const FMod &m) {
+    constexpr int E = 1 << R;
+    constexpr int NRED = E >= 4 ? E / 4 : 1;
+    constexpr int STRIDE = 1 << LH0;
+    const int SB = u >> LH0;
+    const int j = u & (STRIDE - 1);
+    double *p = s + lds_pad(seg_base + (SB << (LH0 + R)) + j);
+    double x[E];
+#pragma unroll
+    for (int t = 0; t < E; t++) x[t] = p[unit_off<STRIDE>(t)];
+#pragma unroll
+    for (int st = 0; st < R; st++) {
+        const int half = 1 << st;
+        const double *w = tw + (1 << (LOGN - LH0 - st - 1)) + (SB << (R - 1 - st));
+#pragma unroll
+        for (int t = 0; t < E; t++) {
+            if (t & half) continue;
+            const double a = x[t], b = x[t + half];
+            x[t] = a + b;
+            x[t + half] = mod_mul_lazy(a - b, w[t >> (st + 1)], m);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NRED; t++) x[t] = lazy_reduce(x[t], m);
+#pragma unroll
+    for (int t = 0; t < E; t++) p[unit_off<STRIDE>(t)] = x[t];
+}
+
+// Compile-time form of ntt_schedule: pass i of a degree-2^logn transform in
+// passes of at most maxr stages has sched_r stages starting at sched_l0.
+NTT_HD constexpr int sched_np(int logn, int maxr) { return logn <= 0 ? 0 : (logn + maxr - 1) / maxr; }
+NTT_HD constexpr int sched_r(int logn, int maxr, int i) {
+    return logn / sched_np(logn, maxr) + (i < logn - (logn / sched_np(logn, maxr)) * sched_np(logn, maxr) ? 1 : 0);
+}
+NTT_HD constexpr int sched_l0(int logn, int maxr, int i) { return i <= 0 ? 0 : sched_l0(logn, maxr, i - 1) + sched_r(logn, maxr, i - 1); }
+
 // Pass schedule: ceil(logn / maxr) passes with stage counts as even as
 // possible, larger first (e.g. 11 -> 4,4,3; 10 -> 4,3,3). Returns the count.
 NTT_HD int ntt_schedule(int logn, int maxr, int *rs) {

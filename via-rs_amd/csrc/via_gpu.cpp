@@ -701,6 +701,26 @@ namespace {
 
 }  // namespace
 
+namespace {
+std::mutex g_tuned_mu;
+std::map<std::pair<int, int>, std::vector<StepTune>> g_tuned;  // (device, batch size)
+}  // namespace
+
+void via::tuned_store(int device, int Q, const std::vector<StepTune> &steps) {
+    std::lock_guard<std::mutex> g(g_tuned_mu);
+    g_tuned[{device, Q}] = steps;
+}
+
+bool via::tuned_lookup(int device, int Q, std::vector<StepTune> *steps) {
+    std::lock_guard<std::mutex> g(g_tuned_mu);
+    auto it = g_tuned.upper_bound({device, Q});  // first entry past (device, Q)
+    if (it == g_tuned.begin()) return false;
+    --it;
+    if (it->first.first != device) return false;
+    *steps = it->second;
+    return true;
+}
+
 // Client parameters agree apart from the record degree, which belongs to the
 // database (the reference's N_REC is a server type parameter).
 bool via::same_client_params(const via_params &a, const via_params &b) {
@@ -766,8 +786,9 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     StepTune step_tune[kDecompSteps];
     {
         const std::vector<StepTune> *tg = nullptr;
+        std::vector<StepTune> found;
         if (!c->tune_override.empty()) tg = &c->tune_override;
-        else if (c->tuned.count(Q)) tg = &c->tuned[Q];
+        else if (tuned_lookup(c->device, Q, &found)) tg = &found;
         for (int i = 0; i < kDecompSteps; i++) {
             if (tg && (int)tg->size() == kDecompSteps) {
                 step_tune[i] = (*tg)[i];
@@ -1204,7 +1225,7 @@ via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via
         if (groups_out) groups_out[i] = best[i].groups;
         if (wide_out) wide_out[i] = best[i].wide;
     }
-    c->tuned[Q] = best;
+    tuned_store(c->device, Q, best);
     return VIA_OK;
 }
 
