@@ -72,6 +72,14 @@ via_status via_ctx_step_times(via_ctx *ctx, double *t7);
 /* Seconds of the last FirstDim database-stream kernel alone (HIP events on
  * the context stream, around the one launch that reads the I*J cells). */
 via_status via_ctx_first_dim_time(via_ctx *ctx, double *seconds);
+/* hipGraph mode (no counterpart in the reference), off by default. When on,
+ * via_answer_device / via_answer_many_device run a call eagerly the first
+ * time its database, batch size, key sets, query and answer buffers and query
+ * length occur (and after the workspace grows or the autotuner stores new
+ * shapes), capture the same launch sequence, and replay the capture for every
+ * later identical call (up to eight captures per context). Step times are not
+ * recorded by replays. Turning it off drops the captures. */
+via_status via_ctx_set_graphs(via_ctx *ctx, int on);
 /* Stream timer: which = 0 records the start event, 1 the stop event, on the
  * context stream; via_ctx_marked_time waits for the stop event and returns
  * the seconds between them. No counterpart in the reference (its benches
@@ -132,17 +140,19 @@ via_status via_answer_many_device(via_ctx *ctx, const via_db *db, uint32_t nq, c
  * queries with each candidate launch shape of every query-decompression step
  * (the number of digit groups, i.e. workgroups splitting the 18 digits of one
  * gadget product, and the kernel variant: 256 threads with one digit
- * transform at a time, or 1024 threads with several side by side), reps timed
+ * transform at a time in radix-8 or radix-16 passes, or 1024 threads with
+ * several side by side), reps timed
  * runs per candidate, and keeps the fastest per step for batches of nq
- * queries. groups_out and wide_out (12 entries each: the eleven
- * cascade steps and the RLWE -> RGSW conversion) receive the choice when not
- * null. The choice serves every context of the process on the same device
+ * queries. groups_out and variant_out (12 entries each: the eleven
+ * cascade steps and the RLWE -> RGSW conversion; variant 0 = 256 threads
+ * radix 8, 2 = 256 threads radix 16, 1 = 1024 threads) receive the choice
+ * when not null. The choice serves every context of the process on the same device
  * (the serving loop's included); an untuned batch size takes the shapes of
  * the largest tuned size below it. The answers written meanwhile are those
  * of via_answer_many_device. */
 via_status via_ctx_autotune(via_ctx *ctx, const via_db *db, uint32_t nq, const via_keys *const *keys,
                             const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_masks,
-                            uint64_t *const *d_bodies, int reps, int *groups_out, int *wide_out);
+                            uint64_t *const *d_bodies, int reps, int *groups_out, int *variant_out);
 
 /* Server::answer_batch::<T> (answer.rs:561-602) -> answer_batch
  * (batch.rs:72-163), VIA-B: answer_through_crot for each of the T queries of

@@ -20,6 +20,7 @@ struct DevTab {
 
 struct Workspace {
     size_t Q = 0, B = 0, I = 0, J = 0, bits = 0;
+    uint64_t gen = 0;  // incremented whenever the buffers are reallocated
     double *lwe_f = nullptr, *masks[2] = {nullptr, nullptr}, *body[2] = {nullptr, nullptr};
     double *pm = nullptr, *pb = nullptr, *rgsw = nullptr, *rgsw_eval = nullptr;
     double *dm[2] = {nullptr, nullptr}, *sw = nullptr, *sel = nullptr, *fd = nullptr;
@@ -38,10 +39,12 @@ constexpr int kDecompSteps = 12;
 
 // Launch shape of one decompression step: digit groups (workgroups per item
 // and tile along the digits) and the kernel variant (wide: the digit
-// transforms of a group side by side).
+// transforms of a group side by side; radix16: the 256-thread kernel with
+// radix-16 rather than radix-8 transform passes).
 struct StepTune {
     int groups = 1;
     int wide = 0;
+    int radix16 = 0;
 };
 
 // Buffers of the VIA-B repack, sized for batches of up to T queries.
@@ -71,6 +74,17 @@ struct via_ctx {
     hipEvent_t ev_dec[kDecompSteps + 1] = {};  // around each decompression step (autotuner)
     bool time_decomp = false;
     std::vector<StepTune> tune_override;  // set while the autotuner measures
+    // hipGraph mode (via_ctx_set_graphs): captured batched answers keyed by
+    // every pointer and size they bake in; `capturing` suppresses the step
+    // events while a capture runs
+    bool graphs = false;
+    bool capturing = false;
+    struct GraphEntry {
+        std::vector<uintptr_t> key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<GraphEntry> graph_cache;
     bool have_times = false;
     // pointers to the last answer's stage buffers
     const double *stage_ptr[7] = {};
@@ -78,12 +92,14 @@ struct via_ctx {
 };
 
 struct via_db {
+    uint64_t id = 0;  // unique per process (captured graphs key on it)
     via_ctx *ctx = nullptr;
     via_params pp;
     double *cells = nullptr;  // eval form at q2, [I][J][n1]
 };
 
 struct via_keys {
+    uint64_t id = 0;  // unique per process (captured graphs key on it)
     via_ctx *ctx = nullptr;
     via_params pp;
     std::vector<double *> cascade;  // per step, [k][part][limb][rank_in*n_out]
@@ -104,6 +120,10 @@ namespace via {
 // was not tuned takes the largest tuned size below it.
 void tuned_store(int device, int Q, const std::vector<StepTune> &steps);
 bool tuned_lookup(int device, int Q, std::vector<StepTune> *steps);
+// Incremented by every tuned_store: captured graphs bake in launch shapes.
+uint64_t tuned_generation();
+// Process-wide unique ids for databases and key sets.
+uint64_t next_object_id();
 // Answers Q <= kMaxQ queries (device buffers) against one database on the
 // context's stream, asynchronously.
 via_status answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via_keys *const *keys,

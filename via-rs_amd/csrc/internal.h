@@ -83,6 +83,7 @@ struct MacArgs {
     int reduce_body;
     int items;
     int wide;                // 1: digit transforms of a group side by side (k_gadget_mac_wide)
+    int radix16;             // 256-thread kernel: radix-16 transform passes (else radix 8)
 };
 void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t s);
 

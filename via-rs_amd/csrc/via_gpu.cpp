@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -222,17 +223,20 @@ void free_ws(Workspace &w) {
 }
 
 // Launch shapes the autotuner tries for `items` gadget products of depth L:
-// the 256-thread kernel with 1, 2, 3, 4, 6, 9 or 18 digit groups and the wide
-// kernel with 1, 2, 3 or 5 (group counts up to L), as long as a launch stays
-// within 8192 workgroups (two tiles per item).
+// the 256-thread kernel with radix-8 or radix-16 transform passes and 1, 2,
+// 3, 4, 6, 9 or 18 digit groups, and the wide kernel with 1, 2, 3 or 5
+// (group counts up to L), as long as a launch stays within 8192 workgroups
+// (two tiles per item).
 std::vector<StepTune> tune_candidates(int items, int L) {
     std::vector<StepTune> out;
-    for (int wide = 0; wide < 2; wide++)
-        for (int g : wide ? std::initializer_list<int>{1, 2, 3, 5} : std::initializer_list<int>{1, 2, 3, 4, 6, 9, 18})
+    for (int variant = 0; variant < 3; variant++)
+        for (int g : variant == 2 ? std::initializer_list<int>{1, 2, 3, 5}
+                                  : std::initializer_list<int>{1, 2, 3, 4, 6, 9, 18})
             if (g <= L && (long)items * 2 * g <= 8192) {
                 StepTune t;
                 t.groups = g;
-                t.wide = wide;
+                t.wide = variant == 2;
+                t.radix16 = variant == 1;
                 out.push_back(t);
             }
     if (out.empty()) out.push_back(StepTune());
@@ -256,6 +260,7 @@ via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_
     J = std::max(J, w.J);
     bits = std::max(bits, w.bits);
     free_ws(w);
+    w.gen++;
     const size_t n = pp->n1, nl = 2, S = kSlots, lq = pp->l_query;
     w.Q = Q;
     w.B = B;
@@ -438,6 +443,10 @@ void via_ctx_destroy(via_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    for (auto &e : c->graph_cache) {
+        (void)hipGraphExecDestroy(e.exec);
+        (void)hipGraphDestroy(e.graph);
+    }
     free_ws(c->ws);
     for (void *p : c->rw.all) (void)hipFree(p);
     for (auto &kv : c->tabs) {
@@ -506,6 +515,7 @@ static via_status db_common(via_ctx *c, const via_params *pp, const uint64_t *re
     if (bytes && (pp->p != 16 || nr % 2)) return VIA_E_INVALID;
     HIP_TRY(hipSetDevice(c->device));
     via_db *db = new via_db();
+    db->id = next_object_id();
     db->ctx = c;
     db->pp = *pp;
     if (hipMalloc(&db->cells, sizeof(double) * I * J * n1) != hipSuccess) {
@@ -581,6 +591,7 @@ via_status via_keys_create(via_ctx *c, const via_params *pp, const uint64_t *cas
     if (s) return s;
     HIP_TRY(hipSetDevice(c->device));
     via_keys *k = new via_keys();
+    k->id = next_object_id();
     k->ctx = c;
     k->pp = *pp;
     const uint32_t n1 = pp->n1, nl = 2, L = pp->l_ck;
@@ -704,11 +715,23 @@ namespace {
 namespace {
 std::mutex g_tuned_mu;
 std::map<std::pair<int, int>, std::vector<StepTune>> g_tuned;  // (device, batch size)
+uint64_t g_tuned_gen = 0;
 }  // namespace
 
 void via::tuned_store(int device, int Q, const std::vector<StepTune> &steps) {
     std::lock_guard<std::mutex> g(g_tuned_mu);
     g_tuned[{device, Q}] = steps;
+    g_tuned_gen++;
+}
+
+uint64_t via::next_object_id() {
+    static std::atomic<uint64_t> next{1};
+    return next++;
+}
+
+uint64_t via::tuned_generation() {
+    std::lock_guard<std::mutex> g(g_tuned_mu);
+    return g_tuned_gen;
 }
 
 bool via::tuned_lookup(int device, int Q, std::vector<StepTune> *steps) {
@@ -731,6 +754,13 @@ bool via::same_client_params(const via_params &a, const via_params &b) {
 }
 
 namespace {
+
+// Step events are not recorded while a graph capture runs (the graph replays
+// kernels and copies only).
+#define REC_EVENT(ev, st)                                 \
+    do {                                                  \
+        if (!c->capturing) HIP_TRY(hipEventRecord(ev, st)); \
+    } while (0)
 
 // answer_through_crot (answer.rs:83-228) for Q queries against one database,
 // every step batched across the queries; query q uses the keys of its own
@@ -775,7 +805,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     uint64_t q2 = pp->q2;
     if ((s = make_ring(c, (uint32_t)n, &q2, 1, &r2))) return s;
 
-    HIP_TRY(hipEventRecord(c->ev[0], st));
+    REC_EVENT(c->ev[0], st);
     // ---- step 1: query_decomp (query_decomp.rs:39-95) ----
     for (int q = 0; q < Q; q++) launch_u64_to_f64(d_lwes[q], w.lwe_f + (size_t)q * B * lw, B * lw, st);
     const Gadget gck = make_gadget(pp->ck_base, pp->l_ck);
@@ -798,7 +828,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
             }
         }
     }
-    if (c->time_decomp) HIP_TRY(hipEventRecord(c->ev_dec[0], st));
+    if (c->time_decomp) REC_EVENT(c->ev_dec[0], st);
     int cur = 0;
     for (uint32_t nout = 2, step = 0; nout <= n; nout <<= 1, step++) {
         RingDev r;
@@ -816,6 +846,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         a.tiles = 2;
         a.groups = step_tune[step].groups;
         a.wide = step_tune[step].wide;
+        a.radix16 = step_tune[step].radix16;
         a.out_mask = w.pm;
         a.out_body = w.pb;
         a.reduce_body = 1;
@@ -834,7 +865,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         cb.body_next = w.body[cur ^ 1];
         cb.dig_next = w.dig;
         launch_cascade_combine(r, gck, dpk, r1n, cb, st);
-        if (c->time_decomp) HIP_TRY(hipEventRecord(c->ev_dec[step + 1], st));
+        if (c->time_decomp) REC_EVENT(c->ev_dec[step + 1], st);
         cur ^= 1;
     }
     // rlwe_to_rgsw_eval (convert.rs:162-180) with each client's conversion
@@ -851,6 +882,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         a.tiles = 1;
         a.groups = step_tune[kDecompSteps - 1].groups;
         a.wide = step_tune[kDecompSteps - 1].wide;
+        a.radix16 = step_tune[kDecompSteps - 1].radix16;
         a.out_mask = w.pm;
         a.out_body = w.pb;
         a.reduce_body = 0;
@@ -858,12 +890,12 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         launch_gadget_mac(r1n, gck, a, st);
         launch_r2g_combine(r1n, w.pm, w.pb, a.groups, w.masks[cur], w.body[cur], w.rgsw, (int)(Q * bits), (int)lq,
                            st);
-        if (c->time_decomp) HIP_TRY(hipEventRecord(c->ev_dec[kDecompSteps], st));
+        if (c->time_decomp) REC_EVENT(c->ev_dec[kDecompSteps], st);
     }
     const size_t rgw = 4 * lq * nl * n;  // words per RGSW at q1
     c->stage_ptr[0] = w.rgsw;
     c->stage_words[0] = bits * rgw;
-    HIP_TRY(hipEventRecord(c->ev[1], st));
+    REC_EVENT(c->ev[1], st);
 
     // ---- step 2: dmux_tree from trivial(Delta) (answer.rs:181-191, mux.rs:222-251) ----
     const Gadget g1 = make_gadget(pp->b1, pp->l_query);
@@ -920,25 +952,25 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     }
     c->stage_ptr[1] = w.dm[dc];
     c->stage_words[1] = I * ctw1;
-    HIP_TRY(hipEventRecord(c->ev[2], st));
+    REC_EVENT(c->ev[2], st);
 
     // ---- step 3: mod_switch_sym q1 -> q2 per row ----
     launch_modswitch_q1_q2(r1n, pp->q2, w.dm[dc], w.sw, (size_t)Q * I * 2, st);
     c->stage_ptr[2] = w.sw;
     c->stage_words[2] = I * 2 * n;
-    HIP_TRY(hipEventRecord(c->ev[3], st));
+    REC_EVENT(c->ev[3], st);
 
     // ---- step 4: first_dim (first_dim.rs:94-184), one database pass for all queries ----
     HIP_TRY(hipMemcpyAsync(w.sel, w.sw, sizeof(double) * Q * I * 2 * n, hipMemcpyDeviceToDevice, st));
     launch_ntt(r2, w.sel, (size_t)Q * I * 2, 1, false, st);
     const int chunks = first_dim_chunks((int)I, (int)J, (int)n, Q);
-    HIP_TRY(hipEventRecord(c->ev_fd[0], st));
+    REC_EVENT(c->ev_fd[0], st);
     launch_first_dim(db->cells, w.sel, w.fd, (int)I, (int)J, (int)n, Q, chunks, fmod_make(pp->q2), st);
-    HIP_TRY(hipEventRecord(c->ev_fd[1], st));
+    REC_EVENT(c->ev_fd[1], st);
     launch_ntt(r2, w.fd, (size_t)Q * J * 2, 1, true, st);
     c->stage_ptr[3] = w.fd;
     c->stage_words[3] = J * 2 * n;
-    HIP_TRY(hipEventRecord(c->ev[4], st));
+    REC_EVENT(c->ev[4], st);
 
     // ---- step 5: cmux_tree over mod-switched RGSWs (answer.rs:194-203) ----
     const Gadget g2 = make_gadget(pp->b2, pp->l_query);
@@ -973,7 +1005,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     size_t res_qstride = J * ctw2;
     c->stage_ptr[4] = cm_in;
     c->stage_words[4] = ctw2;
-    HIP_TRY(hipEventRecord(c->ev[5], st));
+    REC_EVENT(c->ev[5], st);
 
     // ---- step 6: crot SlotExtract (rotate.rs:128-149) ----
     int rc = 0;
@@ -1002,7 +1034,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     }
     c->stage_ptr[5] = res;
     c->stage_words[5] = ctw2;
-    HIP_TRY(hipEventRecord(c->ev[6], st));
+    REC_EVENT(c->ev[6], st);
 
     *res_out = res;
     return VIA_OK;
@@ -1052,7 +1084,7 @@ via_status via::answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via
     via_status s = prefix_impl(c, db, Q, keys, d_lwes, n_lwe, err, &res);
     if (s) return s;
     if ((s = resp_comp_impl(c, &keys[0]->pp, Q, keys, res, d_mask, d_body))) return s;
-    HIP_TRY(hipEventRecord(c->ev[7], c->stream));
+    REC_EVENT(c->ev[7], c->stream);
     HIP_TRY(hipGetLastError());
     c->have_times = true;
     return VIA_OK;
@@ -1154,11 +1186,82 @@ via_status via::answer_vb_impl(via_ctx *c, const via_db *db, const via_keys *key
 
 extern "C" {
 
+// answer_batch_impl, or in graph mode the replay of its captured launches:
+// a call whose database, batch size, keys, query and answer buffers, query
+// length, launch shapes and workspace match a cached capture replays it; any
+// other call runs eagerly (which also grows the workspace and loads the
+// tables a capture cannot allocate) and then captures the same sequence for
+// the next identical call. Databases of one row (the DMux root is built on
+// the host) always run eagerly.
+static via_status answer_graph_or_eager(via_ctx *c, const via_db *db, int Q, const via_keys *const *keys,
+                                        const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_mask,
+                                        uint64_t *const *d_body, via_err_detail *err) {
+    if (!c->graphs || db->pp.num_rows < 2) return answer_batch_impl(c, db, Q, keys, d_lwes, n_lwe, d_mask, d_body, err);
+    std::vector<uintptr_t> key = {(uintptr_t)db->id, (uintptr_t)Q, (uintptr_t)n_lwe, (uintptr_t)tuned_generation(),
+                                  (uintptr_t)c->ws.gen};
+    for (int q = 0; q < Q; q++) {
+        key.push_back((uintptr_t)keys[q]->id);
+        key.push_back((uintptr_t)d_lwes[q]);
+        key.push_back((uintptr_t)d_mask[q]);
+        key.push_back((uintptr_t)d_body[q]);
+    }
+    for (auto &e : c->graph_cache)
+        if (e.key == key) {
+            HIP_TRY(hipGraphLaunch(e.exec, c->stream));
+            c->have_times = false;
+            return VIA_OK;
+        }
+    via_status s = answer_batch_impl(c, db, Q, keys, d_lwes, n_lwe, d_mask, d_body, err);
+    if (s) return s;
+    // the eager run may have grown the workspace: key the capture on the current one
+    key[4] = (uintptr_t)c->ws.gen;
+    via_ctx::GraphEntry e;
+    e.key = key;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    s = answer_batch_impl(c, db, Q, keys, d_lwes, n_lwe, d_mask, d_body, err);
+    c->capturing = false;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &e.graph);
+    if (s || ec != hipSuccess) {
+        if (e.graph) (void)hipGraphDestroy(e.graph);
+        return s ? s : VIA_E_DEVICE;
+    }
+    if (hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(e.graph);
+        return VIA_E_DEVICE;
+    }
+    if (c->graph_cache.size() >= 8) {  // keep the eight most recent captures
+        (void)hipGraphExecDestroy(c->graph_cache.front().exec);
+        (void)hipGraphDestroy(c->graph_cache.front().graph);
+        c->graph_cache.erase(c->graph_cache.begin());
+    }
+    c->graph_cache.push_back(e);
+    c->have_times = true;
+    return VIA_OK;
+}
+
+static void drop_graphs(via_ctx *c) {
+    for (auto &e : c->graph_cache) {
+        (void)hipGraphExecDestroy(e.exec);
+        (void)hipGraphDestroy(e.graph);
+    }
+    c->graph_cache.clear();
+}
+
+via_status via_ctx_set_graphs(via_ctx *c, int on) {
+    if (!c) return VIA_E_INVALID;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!on) drop_graphs(c);
+    c->graphs = on != 0;
+    return VIA_OK;
+}
+
 via_status via_answer_device(via_ctx *c, const via_db *db, const via_keys *keys, const uint64_t *d_lwes,
                              size_t n_lwe, uint64_t *d_mask, uint64_t *d_body, via_err_detail *err) {
     if (!c || !db || !keys || !d_mask || !d_body || (n_lwe && !d_lwes)) return VIA_E_INVALID;
     HIP_TRY(hipSetDevice(c->device));
-    return answer_batch_impl(c, db, 1, &keys, &d_lwes, n_lwe, &d_mask, &d_body, err);
+    return answer_graph_or_eager(c, db, 1, &keys, &d_lwes, n_lwe, &d_mask, &d_body, err);
 }
 
 via_status via_answer_many_device(via_ctx *c, const via_db *db, uint32_t nq, const via_keys *const *keys,
@@ -1168,7 +1271,7 @@ via_status via_answer_many_device(via_ctx *c, const via_db *db, uint32_t nq, con
     HIP_TRY(hipSetDevice(c->device));
     for (uint32_t q0 = 0; q0 < nq; q0 += kMaxQ) {
         const int Q = (int)std::min<uint32_t>(kMaxQ, nq - q0);
-        via_status s = answer_batch_impl(c, db, Q, keys + q0, d_lwes + q0, n_lwe, d_masks + q0, d_bodies + q0, err);
+        via_status s = answer_graph_or_eager(c, db, Q, keys + q0, d_lwes + q0, n_lwe, d_masks + q0, d_bodies + q0, err);
         if (s) return s;
     }
     return VIA_OK;
@@ -1180,7 +1283,7 @@ via_status via_answer_many_device(via_ctx *c, const via_db *db, uint32_t nq, con
 // stream, and keeps the fastest shape per step for this batch size.
 via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via_keys *const *keys,
                             const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_mask,
-                            uint64_t *const *d_body, int reps, int *groups_out, int *wide_out) {
+                            uint64_t *const *d_body, int reps, int *groups_out, int *variant_out) {
     if (!c || !db || !keys || !d_lwes || !d_mask || !d_body || nq < 1 || nq > (uint32_t)kMaxQ || reps < 1)
         return VIA_E_INVALID;
     for (uint32_t q = 0; q < nq; q++)
@@ -1223,7 +1326,7 @@ via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via
             if (t[ci][i] < t[bi][i]) bi = ci;
         best[i] = cands[bi];
         if (groups_out) groups_out[i] = best[i].groups;
-        if (wide_out) wide_out[i] = best[i].wide;
+        if (variant_out) variant_out[i] = best[i].wide ? 1 : (best[i].radix16 ? 2 : 0);
     }
     tuned_store(c->device, Q, best);
     return VIA_OK;

@@ -110,6 +110,7 @@ def lib():
             "via_ctx_synchronize": (C.c_int, [vp]),
             "via_ctx_first_dim_time": (C.c_int, [vp, C.POINTER(C.c_double)]),
             "via_ctx_mark": (C.c_int, [vp, C.c_int]),
+            "via_ctx_set_graphs": (C.c_int, [vp, C.c_int]),
             "via_ctx_autotune": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.c_size_t,
                                            C.POINTER(vp), C.POINTER(vp), C.c_int, C.POINTER(C.c_int),
                                            C.POINTER(C.c_int)]),
@@ -216,7 +217,9 @@ class Context:
 
     def autotune(self, db, keys, d_lwes, n_lwe, d_masks, d_bodies, reps=2):
         """via_ctx_autotune: per-step launch shapes for batches of len(keys)
-        queries (device buffers); returns the twelve (groups, wide) choices."""
+        queries (device buffers); returns the twelve (groups, variant)
+        choices (variant 0: 256 threads radix 8, 1: 1024 threads, 2: 256
+        threads radix 16)."""
         nq = len(keys)
         vp = C.c_void_p
         K = (vp * nq)(*[k.h.value for k in keys])
@@ -227,6 +230,10 @@ class Context:
         w = (C.c_int * 12)()
         _check(lib().via_ctx_autotune(self.h, db.h, nq, K, Qp, n_lwe, M, B, reps, g, w))
         return list(zip(list(g), list(w)))
+
+    def set_graphs(self, on):
+        """hipGraph mode: repeated identical answers replay a captured graph."""
+        _check(lib().via_ctx_set_graphs(self.h, int(bool(on))))
 
     def mark(self, which):
         """Record the start (0) or stop (1) event of the stream timer."""
