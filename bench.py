@@ -437,6 +437,7 @@ def main():
     ap.add_argument("--vb", type=int, default=0, help="VIA-B batch size T (answer_batch), 0: VIA-C")
     ap.add_argument("--indices", type=int, default=64, help="VIA-B indices per step")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (heuristic digit groups)")
+    ap.add_argument("--graphs", action="store_true", help="also time the answers replayed from a hipGraph")
     ap.add_argument("--ntt", action="store_true", help="NTT->pointwise->iNTT microbench")
     ap.add_argument("--polys", type=int, default=8192, help="NTT microbench: RNS polynomial pairs")
     ap.add_argument("--ntt-sample", type=int, default=8192, help="NTT microbench: products checked on the CPU")
@@ -604,6 +605,24 @@ def main():
             "algorithmic_bytes": fd_bytes,
         },
     }
+    if args.graphs:
+        # the same answers replayed from a captured hipGraph
+        ctx.set_graphs(True)
+        answer()  # eager run and capture
+        ctx.synchronize()
+        glat = []
+        t_g = time.perf_counter()
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            answer()
+            ctx.synchronize()
+            glat.append(time.perf_counter() - t0)
+        g_el = time.perf_counter() - t_g
+        ctx.set_graphs(False)
+        glat_ms = np.array(glat) * 1e3
+        out["graph"] = {"answers_per_s": round(args.steps * Q / g_el, 3),
+                        "p50_ms": round(float(np.percentile(glat_ms, 50)), 3),
+                        "p99_ms": round(float(np.percentile(glat_ms, 99)), 3)}
     if args.batched > 1 and Q == 1:
         out["batched"] = batched_run(via_gpu, ctx, pp, db, args.batched, args)
         out["batched"]["vs_one_query"] = round(out["batched"]["answers_per_s"] / out["value"], 2)

@@ -3,6 +3,8 @@
 Run on an MI355X: pytest -m gpu. Every comparison is exact equality of u64
 residues; the paper-stack cases also check client recovery of the record.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -176,3 +178,62 @@ def test_autotuned_answers_match(ctx):
     for d in d_q + d_m + d_b:
         tctx.free(d)
     tctx.close()
+
+
+def test_graph_replays_match_eager():
+    # hipGraph mode: the first call of a shape runs eagerly and is captured;
+    # later identical calls replay the capture. Every replay, with eager work
+    # of another shape and host round trips between the replays, must equal
+    # the eager answers (and the oracle's).
+    import via_gpu
+
+    cases = [paper_case(4, 4, seed=b"via-gpu/graph/client%d" % q, index=(7 * q + 2) % 64, records="random",
+                        rec_seed=b"via-gpu/graph-db") for q in range(3)]
+    pp = cases[0].pp_gpu
+    g = via_gpu.Context(0)
+    db = via_gpu.PreparedDb(g, pp, records=cases[0].records)
+    keys = [via_gpu.PreparedKeys(g, pp, c.cascade_key, c.conv_key, c.rsk) for c in cases]
+    d_q = [g.alloc(c.lwes.nbytes) for c in cases]
+    for d, c in zip(d_q, cases):
+        g.to_device(d, c.lwes)
+    d_m = [g.alloc(8 * pp.n2) for _ in cases]
+    d_b = [g.alloc(8 * pp.n2) for _ in cases]
+    L = via_gpu.lib()
+    err = via_gpu.ErrDetail()
+    vp = C.c_void_p
+
+    def many(idx):
+        n = len(idx)
+        K = (vp * n)(*[keys[i].h.value for i in idx])
+        Qa = (vp * n)(*[d_q[i].value for i in idx])
+        M = (vp * n)(*[d_m[i].value for i in idx])
+        B = (vp * n)(*[d_b[i].value for i in idx])
+        assert L.via_answer_many_device(g.h, db.h, n, K, Qa, pp.query_lwes, M, B, C.byref(err)) == 0
+        g.synchronize()
+
+    def read(i):
+        return (g.to_host(np.zeros(pp.n2, np.uint64), d_m[i]), g.to_host(np.zeros(pp.n2, np.uint64), d_b[i]))
+
+    def clear(i):
+        g.to_device(d_m[i], np.zeros(pp.n2, np.uint64))
+        g.to_device(d_b[i], np.zeros(pp.n2, np.uint64))
+
+    many([0, 1])
+    eager = [read(0), read(1)]
+    many([2])
+    eager.append(read(2))
+    for c, (m, b) in zip(cases, eager):
+        assert np.array_equal(m, c.ans_mask) and np.array_equal(b, c.ans_body)
+    g.set_graphs(True)
+    for rep in range(4):
+        for i in range(3):
+            clear(i)
+        many([0, 1])  # rep 0: eager and captured; later: replayed
+        many([2])     # a second shape between the replays
+        for i in range(3):
+            m, b = read(i)
+            assert np.array_equal(m, eager[i][0]) and np.array_equal(b, eager[i][1]), (rep, i)
+    g.set_graphs(False)
+    for d in d_q + d_m + d_b:
+        g.free(d)
+    g.close()
