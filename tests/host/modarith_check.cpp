@@ -152,6 +152,69 @@ int main() {
         CHECK(rns_reconstruct(a0, a1, b) == x, "garner\n");
         count++;
     }
+    // fp64 two-limb gadget scale (the DMux gadget B = 55879, L = 2 at Q1, and
+    // B = 81, L = 2 for the bound) against the exact centred scale: random
+    // values, values within a few units of every rounding boundary
+    // (c ~ (k + 1/2) Q / B^L) and of the centring boundary Q/2, on both sides.
+    {
+        long decided = 0, total = 0;
+        const uint64_t bases[] = {55879, 81, 1024};
+        for (uint64_t B : bases) {
+            const via_u128 bl = (via_u128)B * B;
+            const ScaleF64 k = scale_f64_make((double)(uint64_t)bl, b.q1, u128_to_f64(Q1));
+            auto exact = [&](via_u128 c, SMag &v) {
+                const int neg = c > (Q1 >> 1);
+                const via_u128 mag = neg ? Q1 - c : c;
+                v.mag = round_mul_div(mag, bl, wq1);
+                v.neg = neg && v.mag != 0;
+            };
+            auto check = [&](via_u128 c) {
+                c %= Q1;
+                const double a0 = (double)(uint64_t)(c % b.q0), a1 = (double)(uint64_t)(c % b.q1);
+                SMag want, got;
+                exact(c, want);
+                total++;
+                if (gadget_scale2_f64(a0, a1, b, k, got)) {
+                    decided++;
+                    CHECK(got.mag == want.mag && got.neg == want.neg, "scale2 f64 B=%llu\n", (unsigned long long)B);
+                }
+                count++;
+            };
+            const long d0 = decided, t0 = total;
+            for (int t = 0; t < 300000; t++) check((((via_u128)rng() << 64) | rng()) % Q1);
+            // random values are decided by the fast path nearly always
+            CHECK(decided - d0 > (total - t0) * 99 / 100, "scale2 f64 B=%llu decided only %ld of %ld\n",
+                  (unsigned long long)B, decided - d0, total - t0);
+            for (int t = 0; t < 100000; t++) {
+                // (k + 1/2) Q / bl for a random k < bl, then a few units either side
+                const via_u128 kk = (((via_u128)rng() << 64) | rng()) % bl;
+                const via_u128 c0 = ((2 * kk + 1) * (Q1 / bl) + ((2 * kk + 1) * (Q1 % bl)) / bl) / 2;
+                for (int d = -3; d <= 3; d++) {
+                    check(c0 + (via_u128)(d + 3) - 3);
+                    check(Q1 - (c0 + (via_u128)(d + 3) - 3));
+                }
+            }
+            for (int d = -50; d <= 50; d++) check((Q1 >> 1) + (via_u128)(d + 50) - 50);
+        }
+    }
+    // fp64 single-modulus scale against 64-bit integer division: the CMux
+    // gadget at q2 (81^2) and the ring-switch gadget at q3 (8^8).
+    {
+        const uint64_t qs[] = {17175674881ULL, 8380417ULL};
+        const uint64_t bls[] = {6561, 16777216};
+        for (int i = 0; i < 2; i++) {
+            const uint64_t q = qs[i], bl = bls[i];
+            const FMod m = fmod_make(q);
+            for (int t = 0; t < 300000; t++) {
+                uint64_t mag = rng() % (q / 2 + 1);
+                if (t < 4) mag = (t & 1) ? q / 2 : (uint64_t)t;
+                const uint64_t want = (uint64_t)(((via_u128)mag * bl + (q >> 1)) / q);
+                CHECK(scale_div_f64(mag, (double)bl, m.q, m.qinv, q >> 1) == want, "scale1 f64 q=%llu\n",
+                      (unsigned long long)q);
+                count++;
+            }
+        }
+    }
     if (fails) {
         printf("FAIL %d\n", fails);
         return 1;

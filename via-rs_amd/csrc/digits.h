@@ -14,6 +14,7 @@ template <int NL>
 __device__ __forceinline__ SMag gadget_scale(double a0, double a1, const RingDev &r, const Gadget &g) {
     SMag v;
     if (NL == 2) {
+        if (g.f64 == 2 && gadget_scale2_f64(a0, a1, r.rns, g.sf, v)) return v;
         via_u128 x = rns_reconstruct(a0, a1, r.rns);
         via_u128 Q = r.rns.big_q;
         int neg = x > (Q >> 1);
@@ -28,6 +29,8 @@ __device__ __forceinline__ SMag gadget_scale(double a0, double a1, const RingDev
         if (g.wide) {
             // B^L beyond 64-bit products (the base-18 depth-18 gadget at q2)
             v.mag = round_mul_div((via_u128)mag, g.bl, r.qdiv);
+        } else if (g.f64 == 1) {
+            v.mag = (via_u128)scale_div_f64(mag, g.sf.bl, r.t[0].m.q, r.t[0].m.qinv, q >> 1);
         } else {
             // (mag * B^L + floor(q/2)) / q fits in 64 bits (checked by the host).
             uint64_t bl = (uint64_t)g.bl;

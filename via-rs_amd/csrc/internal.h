@@ -45,6 +45,11 @@ struct Gadget {
     int L;
     via_u128 bl;
     int wide;
+    // fp64 fast path of the scale (set by the gate launcher): 0 none,
+    // 1 single modulus with an exact fp64 numerator, 2 two-limb estimate
+    // with the exact path as fallback (modarith.h)
+    int f64;
+    ScaleF64 sf;
 };
 
 // ---- launchers (ntt.hip / kernels.hip) ----------------------------------------

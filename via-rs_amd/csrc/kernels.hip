@@ -595,9 +595,19 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
         }
 }
 
-void launch_gate(const RingDev &r, const Gadget &g, const GateArgs &a_in, hipStream_t st) {
+void launch_gate(const RingDev &r, const Gadget &g_in, const GateArgs &a_in, hipStream_t st) {
     GateArgs a = a_in;
-    a.dp = digit_plan(g.base, g.L);
+    a.dp = digit_plan(g_in.base, g_in.L);
+    // the fp64 scale where it is exact (one limb) or decided (two limbs)
+    Gadget g = g_in;
+    g.f64 = 0;
+    if (r.nl == 2 && g.bl < ((via_u128)1 << 40)) {
+        g.f64 = 2;
+        g.sf = scale_f64_make((double)(uint64_t)g.bl, r.rns.q1, u128_to_f64(r.rns.big_q));
+    } else if (r.nl == 1 && !g.wide && ((via_u128)(r.q_single / 2 + 1) * g.bl + r.q_single) < ((via_u128)1 << 53)) {
+        g.f64 = 1;
+        g.sf.bl = (double)(uint64_t)g.bl;
+    }
     const int blocks = a.items_per_query * a.queries;
     if (g.L <= 2 && r.n == kSlots) {
         if (r.nl == 2)

@@ -207,6 +207,67 @@ struct SMag {
     int neg;
 };
 
+// ---- fp64 fast paths of the gadget scale --------------------------------------
+//
+// The gadget scale round-half-up(|c_centred| * B^L / Q) (gadget.rs:115-151,
+// wide.rs:145-153) of the exact wide path above, computed in fp64 where that
+// is exact or provably decided.
+
+// Single modulus with (q/2 + 1) * B^L + q below 2^53: the numerator
+// mag * B^L + floor(q/2) is an exact fp64 integer, the quotient estimate is
+// within one, and the exact remainder corrects it.
+VIA_HD uint64_t scale_div_f64(uint64_t mag, double bl, double q, double qinv, uint64_t half_q) {
+    const double num = (double)mag * bl + (double)half_q;  // exact below 2^53
+    double y = floor(num * qinv);
+    const double r = fma(-y, q, num);  // exact: |r| < 2q
+    if (r < 0.0) y -= 1.0;
+    else if (r >= q) y += 1.0;
+    return (uint64_t)y;
+}
+
+// Two-limb RNS value (a0 mod q0, a1 mod q1), Q = q0 * q1, B^L = bl < 2^40:
+// with Garner's t (c = a0 + q0 * t), f = c * bl / Q = t * (bl / q1) +
+// a0 * (bl / Q) in fp64 is within 2^-52 * bl of the exact value (two
+// roundings of terms below bl), and f + 1/2 within 1.5 * 2^-52 * bl; the
+// centring test (c > Q/2 iff f > bl/2, up to bl / 2Q) and the rounding of f
+// or of bl - f (exact by Sterbenz) are decided when they lie further than
+// `margin` = 2^-47 * bl (20x the error) from their boundaries; otherwise the
+// caller takes the exact path.
+struct ScaleF64 {
+    double bl;      // B^L (exact)
+    double bl_q1;   // bl / q1
+    double bl_Q;    // bl / Q
+    double margin;  // undecided band half-width
+};
+
+VIA_HD ScaleF64 scale_f64_make(double bl, uint64_t q1, double Q) {
+    ScaleF64 k;
+    k.bl = bl;
+    k.bl_q1 = bl / (double)q1;
+    k.bl_Q = bl / Q;
+    k.margin = bl * 7.105427357601002e-15;  // 2^-47 * bl
+    return k;
+}
+
+VIA_HD bool gadget_scale2_f64(double a0, double a1, const Rns2 &b, const ScaleF64 &k, SMag &v) {
+    const double a0m = mod_reduce_small(a0, b.m1);
+    const double diff = mod_sub(a1, a0m, b.m1);
+    const double t = mod_mul(diff, (double)b.q0_inv_mod_q1, b.m1);
+    const double f = fma(t, k.bl_q1, a0 * k.bl_Q);
+    const double half = 0.5 * k.bl;
+    int neg;
+    if (f - half > k.margin) neg = 1;
+    else if (half - f > k.margin) neg = 0;
+    else return false;
+    const double g = (neg ? k.bl - f : f) + 0.5;
+    const double fl = floor(g);
+    const double fr = g - fl;
+    if (fr < k.margin || fr > 1.0 - k.margin) return false;
+    v.mag = (via_u128)(uint64_t)fl;
+    v.neg = neg && fl != 0.0;
+    return true;
+}
+
 // One LSB digit of base B (B < 2^20), returning the digit and advancing val
 // to (val - digit) / B, with the Euclidean remainder and the "d > B/2 => d -= B"
 // rebalance of the reference.

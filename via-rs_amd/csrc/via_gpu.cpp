@@ -164,6 +164,8 @@ Gadget make_gadget(uint64_t base, uint32_t L) {
     g.bl = 1;
     for (uint32_t k = 0; k < L; k++) g.bl *= base;
     g.wide = 0;
+    g.f64 = 0;
+    g.sf = ScaleF64();
     return g;
 }
 
