@@ -191,6 +191,14 @@ via_status via_poly_mul_device(via_ctx *ctx, uint32_t n, const uint64_t *moduli,
 via_status via_ntt(via_ctx *ctx, uint32_t n, const uint64_t *moduli, uint32_t nmod, size_t group, uint64_t *polys,
                    size_t count, int inverse);
 
+/* FirstDim (via-server/src/first_dim.rs:94-184, `first_dim`) for nq <= 32
+ * queries in one database pass, the batched GPU form the reference's
+ * first_dim.rs:81-88 describes: switched [q][I][part][n1] holds each query's
+ * I mod-switched RLWE selection ciphertexts, out [q][J][part][n1] receives
+ * the J output ciphertexts, both coefficient form at q2, host buffers. The
+ * database must belong to ctx. */
+via_status via_first_dim(via_ctx *ctx, const via_db *db, uint32_t nq, const uint64_t *switched, uint64_t *out);
+
 /* ---- serving loop (no counterpart in the reference, which has no server
  * process; the answers it produces are those of Server::answer) ----------- */
 typedef struct via_server via_server;

@@ -42,6 +42,12 @@ def test_lazy_ntt_passes(tmp_path):
     build_and_run("ntt_lazy_check.cpp", tmp_path)
 
 
+def test_first_dim_limbs(tmp_path):
+    # 7-bit limb split and the Horner fold of limb-product sums used by the
+    # int8 matrix-core FirstDim: exact against 128-bit row sums
+    build_and_run("fd_limbs_check.cpp", tmp_path)
+
+
 def declared_symbols():
     src = open(os.path.join(ROOT, "include", "via_gpu.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)

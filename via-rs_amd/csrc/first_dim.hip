@@ -1,0 +1,342 @@
+// first_dim.hip — FirstDim (via-rs via-server/src/first_dim.rs:94-184) on the
+// int8 matrix cores, and the prepared-database layout it streams.
+//
+// For every slot k the pass is a matrix product over the database rows i,
+//   out[q][j][part][k] = sum_i db[i][j][k] * sel[q][i][part][k]  (mod q2),
+// of slot k's J x I database block with the I x 2Q selection block of the
+// pass's Q queries. Residues are split into five 7-bit limbs (fd_limbs.h);
+// the 25 limb products of every (database, selection) pair accumulate exactly
+// in int32 on v_mfma_i32_16x16x64_i8 (16x16x32 for 32-row databases), into
+// nine sums by limb shift, which limb_fold brings back to a residue.
+//
+// Prepared database, 5 bytes per residue: for slot k, column block mb (16
+// columns j), row chunk kc (KC rows i) and limb l, one MFMA operand fragment
+// of 16 x KC bytes in which lane 16 g + r holds column j = 16 mb + r, rows
+// i = KC kc + (KC/4) g + [0, KC/4). Fragments run [k][mb][kc][l], so slot k's
+// block is contiguous and its workgroup streams it once.
+// Selection, packed per pass: [k][kc][l][g][qp][KC/4 bytes] with qp = 2q +
+// part and the same row order, staged through LDS by each workgroup.
+// The MFMA B operand's lane 16 g + c supplies column c for the same rows, so
+// a fragment pair multiplies exactly the rows it should whatever order the
+// hardware gives the bytes inside a lane group.
+// Output of a workgroup (slot k, segment of kFdSegRows rows):
+// out_k[seg][k][q][part][j], coalesced; k_fd_gather adds the segments and
+// transposes to [q][j][part][k] for the inverse transform.
+#include "internal.h"
+#include "fd_limbs.h"
+
+namespace via {
+
+typedef int fd_v4i __attribute__((ext_vector_type(4)));
+
+template <int KC>
+struct FdOp;
+template <>
+struct FdOp<64> {
+    typedef fd_v4i T;  // 16 bytes per lane
+    static __device__ __forceinline__ T zero() { return fd_v4i{0, 0, 0, 0}; }
+    static __device__ __forceinline__ fd_v4i mac(T a, T b, fd_v4i c) {
+        return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct FdOp<32> {
+    typedef long T;  // 8 bytes per lane
+    static __device__ __forceinline__ T zero() { return 0; }
+    static __device__ __forceinline__ fd_v4i mac(T a, T b, fd_v4i c) {
+        return __builtin_amdgcn_mfma_i32_16x16x32_i8(a, b, c, 0, 0, 0);
+    }
+};
+
+FdPlan fd_plan(int I, int J, int n, int Q) {
+    FdPlan p;
+    p.n = n;
+    p.I = I;
+    p.J = J;
+    p.Q = Q;
+    p.Ipad = I <= 32 ? 32 : (I + 63) / 64 * 64;
+    p.KC = p.Ipad % 64 == 0 ? 64 : 32;
+    p.Jpad = (J + 15) / 16 * 16;
+    p.MB = p.Jpad / 16;
+    p.KCN = p.Ipad / p.KC;
+    p.QP = 2 * Q;
+    p.NB = (p.QP + 15) / 16;
+    const int ks = kFdSegRows / p.KC;
+    p.nseg = (p.KCN + ks - 1) / ks;
+    return p;
+}
+
+size_t fd_db_bytes(const FdPlan &p) { return (size_t)p.n * p.Ipad * p.Jpad * kLimbs; }
+size_t fd_sel_bytes(const FdPlan &p) { return (size_t)p.n * p.Ipad * kLimbs * p.QP; }
+size_t fd_out_words(const FdPlan &p) { return (size_t)p.nseg * p.n * p.Q * 2 * p.J; }
+
+// KB limb bytes of each of the five limbs of values v[0..KB), packed little
+// end first into words.
+template <int KB>
+__device__ __forceinline__ void pack_limbs(const double *v, uint32_t (&wd)[kLimbs][KB / 4]) {
+#pragma unroll
+    for (int a = 0; a < kLimbs; a++)
+#pragma unroll
+        for (int w = 0; w < KB / 4; w++) wd[a][w] = 0;
+#pragma unroll
+    for (int b = 0; b < KB; b++) {
+        uint32_t l[kLimbs];
+        limbs7(v[b], l);
+#pragma unroll
+        for (int a = 0; a < kLimbs; a++) wd[a][b >> 2] |= l[a] << (8 * (b & 3));
+    }
+}
+
+template <int KB>
+__device__ __forceinline__ void store_words(uint8_t *dst, const uint32_t (&w)[KB / 4]) {
+    if constexpr (KB == 16) *(uint4 *)dst = make_uint4(w[0], w[1], w[2], w[3]);
+    else *(uint2 *)dst = make_uint2(w[0], w[1]);
+}
+
+// ---- prepared database ------------------------------------------------------------
+
+// cells [I][J][n] (canonical residues) -> fragments; rows i >= I and columns
+// j >= J are zero. Eight consecutive slots per lane group read 64-byte runs.
+template <int KC>
+__global__ __launch_bounds__(256) void k_pack_db(const double *cells, uint8_t *dbl, FdPlan p) {
+    constexpr int KB = KC / 4, FRAG = 64 * KB;
+    const size_t total = (size_t)p.n * p.MB * p.KCN * 64;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int kl = (int)(t & 7), lane = (int)((t >> 3) & 63);
+        size_t rest = t >> 9;
+        const int kc = (int)(rest % p.KCN);
+        rest /= p.KCN;
+        const int mb = (int)(rest % p.MB);
+        const int k = (int)(rest / p.MB) * 8 + kl;
+        const int r = lane & 15, g = lane >> 4, j = mb * 16 + r;
+        double v[KB];
+#pragma unroll
+        for (int b = 0; b < KB; b++) {
+            const int i = kc * KC + g * KB + b;
+            v[b] = (i < p.I && j < p.J) ? cells[((size_t)i * p.J + j) * p.n + k] : 0.0;
+        }
+        uint32_t wd[kLimbs][KB / 4];
+        pack_limbs<KB>(v, wd);
+        uint8_t *dst = dbl + ((((size_t)k * p.MB + mb) * p.KCN + kc) * kLimbs) * FRAG + (size_t)lane * KB;
+#pragma unroll
+        for (int a = 0; a < kLimbs; a++) store_words<KB>(dst + (size_t)a * FRAG, wd[a]);
+    }
+}
+
+// fragments -> u64 cells [I][J][n] (the prepared database as the reference
+// holds it, for inspection).
+template <int KC>
+__global__ __launch_bounds__(256) void k_unpack_db(const uint8_t *dbl, uint64_t *out, FdPlan p) {
+    constexpr int KB = KC / 4, FRAG = 64 * KB;
+    const size_t total = (size_t)p.I * p.J * p.n;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(t % p.n);
+        const size_t rest = t / p.n;
+        const int j = (int)(rest % p.J), i = (int)(rest / p.J);
+        const int mb = j >> 4, r = j & 15, kc = i / KC, g = (i % KC) / KB, b = i % KB;
+        const uint8_t *src = dbl + ((((size_t)k * p.MB + mb) * p.KCN + kc) * kLimbs) * FRAG + (size_t)(g * 16 + r) * KB + b;
+        uint64_t v = 0;
+        for (int a = kLimbs - 1; a >= 0; a--) v = v * 128 + src[(size_t)a * FRAG];
+        out[t] = v;
+    }
+}
+
+void launch_pack_db(const double *cells, uint8_t *dbl, const FdPlan &p, hipStream_t st) {
+    const size_t total = (size_t)p.n * p.MB * p.KCN * 64;
+    const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 65536);
+    if (p.KC == 64) hipLaunchKernelGGL(k_pack_db<64>, dim3(blocks), dim3(256), 0, st, cells, dbl, p);
+    else hipLaunchKernelGGL(k_pack_db<32>, dim3(blocks), dim3(256), 0, st, cells, dbl, p);
+}
+
+void launch_unpack_db(const uint8_t *dbl, uint64_t *out, const FdPlan &p, hipStream_t st) {
+    const size_t total = (size_t)p.I * p.J * p.n;
+    const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 65536);
+    if (p.KC == 64) hipLaunchKernelGGL(k_unpack_db<64>, dim3(blocks), dim3(256), 0, st, dbl, out, p);
+    else hipLaunchKernelGGL(k_unpack_db<32>, dim3(blocks), dim3(256), 0, st, dbl, out, p);
+}
+
+// ---- selection pack -------------------------------------------------------------------
+
+// sel [q][i][part][n] (canonical residues at q2) -> [k][kc][l][g][qp][KB].
+// A wavefront covers eight slots x eight columns qp: 64-byte read runs and
+// full 128-byte write lines.
+template <int KC>
+__global__ __launch_bounds__(256) void k_sel_pack(const double *sel, uint8_t *sell, FdPlan p) {
+    constexpr int KB = KC / 4;
+    const int QP8 = (p.QP + 7) / 8;
+    // 64 threads per (slot group of eight, chunk, lane group, column group)
+    const size_t total = (size_t)(p.n / 8) * p.KCN * 4 * QP8 * 64;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int qpl = (int)(t & 7), kl = (int)((t >> 3) & 7);
+        size_t rest = t >> 6;
+        const int qph = (int)(rest % QP8);
+        rest /= QP8;
+        const int g = (int)(rest & 3);
+        rest >>= 2;
+        const int kc = (int)(rest % p.KCN);
+        const int k = (int)(rest / p.KCN) * 8 + kl;
+        const int qp = qph * 8 + qpl;
+        if (qp >= p.QP) continue;
+        const int q = qp >> 1, part = qp & 1;
+        double v[KB];
+#pragma unroll
+        for (int b = 0; b < KB; b++) {
+            const int i = kc * KC + g * KB + b;
+            v[b] = i < p.I ? sel[(((size_t)q * p.I + i) * 2 + part) * p.n + k] : 0.0;
+        }
+        uint32_t wd[kLimbs][KB / 4];
+        pack_limbs<KB>(v, wd);
+        uint8_t *dst = sell + (size_t)k * p.KCN * kLimbs * p.QP * KC +
+                       ((((size_t)kc * kLimbs) * 4 + g) * p.QP + qp) * KB;
+#pragma unroll
+        for (int a = 0; a < kLimbs; a++) store_words<KB>(dst + (size_t)a * 4 * p.QP * KB, wd[a]);
+    }
+}
+
+void launch_sel_pack(const double *sel, uint8_t *sell, const FdPlan &p, hipStream_t st) {
+    const size_t total = (size_t)(p.n / 8) * p.KCN * 4 * ((p.QP + 7) / 8) * 64;
+    const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 65536);
+    if (p.KC == 64) hipLaunchKernelGGL(k_sel_pack<64>, dim3(blocks), dim3(256), 0, st, sel, sell, p);
+    else hipLaunchKernelGGL(k_sel_pack<32>, dim3(blocks), dim3(256), 0, st, sel, sell, p);
+}
+
+// ---- the matrix-core pass ------------------------------------------------------------
+
+// Workgroup (slot k, row segment): the segment's packed selection goes to LDS
+// once; wavefront w takes column blocks mb = w, w + 4, ...; for each row
+// chunk it holds the five database limb fragments (the next chunk's loads in
+// flight) and runs 25 MFMAs per selection block of 16 columns into the nine
+// shift sums, then folds and stores its 16 x 2Q outputs.
+template <int KC, int NB>
+__global__ __launch_bounds__(256) void k_fd_mfma(const uint8_t *__restrict__ dbl, const uint8_t *__restrict__ sell,
+                                                 double *__restrict__ out, FdPlan p, FMod m) {
+    typedef FdOp<KC> Op;
+    typedef typename Op::T T;
+    constexpr int KB = KC / 4, FRAG = 64 * KB;
+    constexpr int KS = kFdSegRows / KC;
+    __shared__ __attribute__((aligned(16))) uint8_t sb[KS * kLimbs * 16 * NB * KC];
+    const int k = blockIdx.x, seg = blockIdx.y;
+    const int kc0 = seg * KS, nkc = min(KS, p.KCN - kc0);
+    const size_t chunk = (size_t)kLimbs * p.QP * KC;  // packed selection bytes per row chunk
+    {
+        const uint8_t *src = sell + ((size_t)k * p.KCN + kc0) * chunk;
+        const int bytes = nkc * (int)chunk;  // a multiple of 16
+        for (int o = threadIdx.x * 16; o < bytes; o += 256 * 16) *(uint4 *)(sb + o) = *(const uint4 *)(src + o);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const size_t mbstride = (size_t)p.KCN * kLimbs * FRAG;
+    const uint8_t *dk = dbl + (size_t)k * p.MB * mbstride + (size_t)kc0 * kLimbs * FRAG + (size_t)lane * KB;
+    const size_t P = (size_t)p.Q * 2 * p.J;
+    double *ok = out + ((size_t)seg * p.n + k) * P;
+    for (int mb = w; mb < p.MB; mb += 4) {
+        fd_v4i acc[NB][kLimbShifts];
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+            for (int s = 0; s < kLimbShifts; s++) acc[nb][s] = fd_v4i{0, 0, 0, 0};
+        const uint8_t *da = dk + (size_t)mb * mbstride;
+        T a[kLimbs], an[kLimbs];
+#pragma unroll
+        for (int l = 0; l < kLimbs; l++) a[l] = __builtin_nontemporal_load((const T *)(da + (size_t)l * FRAG));
+        for (int t = 0; t < nkc; t++) {
+            if (t + 1 < nkc) {
+#pragma unroll
+                for (int l = 0; l < kLimbs; l++)
+                    an[l] = __builtin_nontemporal_load((const T *)(da + ((size_t)(t + 1) * kLimbs + l) * FRAG));
+            }
+            const uint8_t *bt = sb + (size_t)t * chunk;
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++) {
+                const int qp = nb * 16 + c;
+#pragma unroll
+                for (int bl = 0; bl < kLimbs; bl++) {
+                    const T b = qp < p.QP ? *(const T *)(bt + ((size_t)(bl * 4 + g) * p.QP + qp) * KB) : Op::zero();
+#pragma unroll
+                    for (int al = 0; al < kLimbs; al++) acc[nb][al + bl] = Op::mac(a[al], b, acc[nb][al + bl]);
+                }
+            }
+            if (t + 1 < nkc) {
+#pragma unroll
+                for (int l = 0; l < kLimbs; l++) a[l] = an[l];
+            }
+        }
+        // lane (g, c) of block nb holds rows j = 16 mb + 4 g + [0, 4) of column qp
+        const int j0 = mb * 16 + 4 * g;
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) {
+            const int qp = nb * 16 + c;
+            if (qp >= p.QP) continue;
+            double v[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int32_t C[kLimbShifts];
+#pragma unroll
+                for (int s = 0; s < kLimbShifts; s++) C[s] = acc[nb][s][r];
+                v[r] = limb_fold(C, m);
+            }
+            double *o = ok + (size_t)qp * p.J + j0;  // [q][part][j] with qp = 2q + part
+            if (j0 + 4 <= p.J) {
+                typedef double dbl2 __attribute__((ext_vector_type(2)));
+                *(dbl2 *)o = dbl2{v[0], v[1]};
+                *(dbl2 *)(o + 2) = dbl2{v[2], v[3]};
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (j0 + r < p.J) o[r] = v[r];
+            }
+        }
+    }
+}
+
+template <int KC>
+static void fd_mfma_kc(const uint8_t *dbl, const uint8_t *sell, double *out_k, const FdPlan &p, FMod m,
+                       hipStream_t st) {
+    const dim3 grid(p.n, p.nseg);
+    switch (p.NB) {
+        case 1: hipLaunchKernelGGL((k_fd_mfma<KC, 1>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m); break;
+        case 2: hipLaunchKernelGGL((k_fd_mfma<KC, 2>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m); break;
+        case 3: hipLaunchKernelGGL((k_fd_mfma<KC, 3>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m); break;
+        default: hipLaunchKernelGGL((k_fd_mfma<KC, 4>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m); break;
+    }
+}
+
+void launch_fd_mfma(const uint8_t *dbl, const uint8_t *sell, double *out_k, const FdPlan &p, FMod m, hipStream_t st) {
+    if (p.KC == 64) fd_mfma_kc<64>(dbl, sell, out_k, p, m, st);
+    else fd_mfma_kc<32>(dbl, sell, out_k, p, m, st);
+}
+
+// out_k[seg][k][q][part][j] -> out[q][j][part][k], the segments added mod q;
+// 64 x 64 tiles through LDS so both sides move whole lines.
+__global__ __launch_bounds__(256) void k_fd_gather(const double *ok, double *out, FdPlan p, FMod m) {
+    __shared__ double t[64][65];
+    const size_t P = (size_t)p.Q * 2 * p.J;
+    const int k0 = blockIdx.x * 64;
+    const size_t e0 = (size_t)blockIdx.y * 64;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int kk = e >> 6, ee = e & 63;
+        const size_t idx = e0 + ee;
+        double v = 0.0;
+        if (idx < P && k0 + kk < p.n)
+            for (int sg = 0; sg < p.nseg; sg++) v += ok[((size_t)sg * p.n + k0 + kk) * P + idx];
+        t[kk][ee] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int ee = e >> 6, kk = e & 63;
+        const size_t idx = e0 + ee;
+        if (idx >= P || k0 + kk >= p.n) continue;
+        const size_t qp = idx / p.J, j = idx % p.J;
+        const size_t q = qp >> 1, part = qp & 1;
+        out[((q * p.J + j) * 2 + part) * p.n + k0 + kk] = mod_reduce_small(t[kk][ee], m);
+    }
+}
+
+void launch_fd_gather(const double *out_k, double *out_eval, const FdPlan &p, FMod m, hipStream_t st) {
+    const size_t P = (size_t)p.Q * 2 * p.J;
+    const dim3 grid((p.n + 63) / 64, (unsigned)((P + 63) / 64));
+    hipLaunchKernelGGL(k_fd_gather, grid, dim3(256), 0, st, out_k, out_eval, p, m);
+}
+
+}  // namespace via

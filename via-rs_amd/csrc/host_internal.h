@@ -23,13 +23,13 @@ struct Workspace {
     uint64_t gen = 0;  // incremented whenever the buffers are reallocated
     double *lwe_f = nullptr, *masks[2] = {nullptr, nullptr}, *body[2] = {nullptr, nullptr};
     double *pm = nullptr, *pb = nullptr, *rgsw = nullptr, *rgsw_eval = nullptr;
-    double *dm[2] = {nullptr, nullptr}, *sw = nullptr, *sel = nullptr, *fd = nullptr;
+    double *dm[2] = {nullptr, nullptr}, *sw = nullptr, *sel = nullptr, *fd = nullptr, *fdk = nullptr;
     double *cm[2] = {nullptr, nullptr}, *rgsw2 = nullptr, *res[2] = {nullptr, nullptr};
     uint64_t *lwe_u = nullptr, *ans_u = nullptr;
     int16_t *dig = nullptr;
+    uint8_t *sell = nullptr;  // packed selection limbs of a FirstDim pass
     int groups = 1;      // heuristic digit groups per gadget product
     int groups_max = 1;  // the partial-sum planes pm / pb hold this many groups
-    int fd_chunks = 1;
     std::vector<void *> all;
 };
 
@@ -95,7 +95,9 @@ struct via_db {
     uint64_t id = 0;  // unique per process (captured graphs key on it)
     via_ctx *ctx = nullptr;
     via_params pp;
-    double *cells = nullptr;  // eval form at q2, [I][J][n1]
+    // eval form at q2 as 7-bit limb fragments of the matrix-core FirstDim
+    // (first_dim.hip), 5 bytes per residue
+    uint8_t *limbs = nullptr;
 };
 
 struct via_keys {

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "modarith.h"
 #include "xcd_map.h"
 
@@ -58,6 +60,8 @@ struct Gadget {
 // poly p uses limb (p / limb_group) % r.nl.
 void launch_ntt(const RingDev &r, double *data, size_t npolys, size_t limb_group, bool inverse,
                 hipStream_t s);
+// The same forward transform from src into dst (src may equal dst).
+void launch_ntt_to(const RingDev &r, const double *src, double *dst, size_t npolys, size_t limb_group, hipStream_t s);
 void launch_u64_to_f64(const uint64_t *src, double *dst, size_t n, hipStream_t s);
 void launch_f64_to_u64(const double *src, uint64_t *dst, size_t n, hipStream_t s);
 // Negacyclic products c = iNTT(NTT(a) * NTT(b)) of `count` u64 polys per limb
@@ -142,12 +146,28 @@ void launch_dmux_trivial(const RingDev &r, const double *rgsw_coeff, size_t rgsw
 void launch_modswitch_q1_q2(const RingDev &r1, uint64_t q2, const double *in, double *out, size_t polys,
                             hipStream_t s);
 
-// FirstDim multiply-accumulate for Q queries in one database pass:
-// out[q][j][part][k] = sum_i db[i][j][k] * sel[q][i][part][k]. out_eval holds
-// `chunks` partial planes of Q*J*2*n words; the result lands in the first.
-int first_dim_chunks(int I, int J, int n, int Q);
-void launch_first_dim(const double *db, const double *sel_eval, double *out_eval, int I, int J, int n, int Q,
-                      int chunks, FMod m, hipStream_t s);
+// FirstDim on the int8 matrix cores (first_dim.hip). The prepared database
+// is held as 7-bit limb fragments (5 bytes per residue); a pass packs the
+// selection vectors the same way, runs one workgroup per (slot, segment of
+// kFdSegRows rows) and gathers the segment outputs to [q][j][part][k].
+constexpr int kFdSegRows = 256;
+struct FdPlan {
+    int n, I, J, Q;
+    int Ipad, Jpad;  // rows padded to KC (at least 32), columns to 16
+    int KC;          // rows per MFMA fragment: 64, or 32 when Ipad is 32
+    int MB, KCN;     // column blocks of 16, row chunks of KC
+    int QP, NB;      // selection columns 2Q, blocks of 16
+    int nseg;        // row segments
+};
+FdPlan fd_plan(int I, int J, int n, int Q);
+size_t fd_db_bytes(const FdPlan &p);   // prepared database
+size_t fd_sel_bytes(const FdPlan &p);  // packed selection of a pass
+size_t fd_out_words(const FdPlan &p);  // segment outputs (doubles)
+void launch_pack_db(const double *cells, uint8_t *dbl, const FdPlan &p, hipStream_t s);
+void launch_unpack_db(const uint8_t *dbl, uint64_t *out, const FdPlan &p, hipStream_t s);
+void launch_sel_pack(const double *sel_eval, uint8_t *sell, const FdPlan &p, hipStream_t s);
+void launch_fd_mfma(const uint8_t *dbl, const uint8_t *sell, double *out_k, const FdPlan &p, FMod m, hipStream_t s);
+void launch_fd_gather(const double *out_k, double *out_eval, const FdPlan &p, FMod m, hipStream_t s);
 
 // RespComp for Q ciphertexts at (n1, q2): mod switch to q3, ring switch to n2
 // with each query's eval RSK ([j][k][part][n2]), then mask at q3 / body at q4.

@@ -690,237 +690,6 @@ void launch_modswitch_q1_q2(const RingDev &r1, uint64_t q2, const double *in, do
     hipLaunchKernelGGL(k_modswitch_q1_q2, dim3(blocks), dim3(256), 0, st, r1, q2, r1.qdiv, in, out, polys);
 }
 
-// ---- FirstDim (first_dim.rs:94-184) -------------------------------------------------
-
-// out[chunk][q][j][part][k] = sum over the chunk's rows i of db[i][j][k] *
-// sel[q][i][part][k] for QB queries sharing each database load. Each lane
-// owns two adjacent slots k (16-byte loads) of TJ columns and keeps 4*QB*TJ
-// accumulators; a workgroup covers 512 slots. Row i+1 is loaded while row i
-// multiplies (two rows in registers), with each row's selection values
-// requested before its database values so that waiting for them never waits
-// for a later row. Products are summed lazily (exact below 2^53) and reduced
-// once.
-template <int QB, int TJ>
-__global__ __launch_bounds__(256) void k_first_dim(const double *db, const double *sel, double *out, int I, int J,
-                                                   int n, int Q, int q0, int rows_per_chunk, FMod m) {
-    typedef double dbl2 __attribute__((ext_vector_type(2)));
-    const int k = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
-    if (k >= n) return;
-    const int j0 = blockIdx.y * TJ;
-    const int i0 = blockIdx.z * rows_per_chunk;
-    const int i1 = min(I, i0 + rows_per_chunk);
-    double am[QB][TJ][2], ab[QB][TJ][2];
-#pragma unroll
-    for (int qb = 0; qb < QB; qb++)
-#pragma unroll
-        for (int t = 0; t < TJ; t++) am[qb][t][0] = am[qb][t][1] = ab[qb][t][0] = ab[qb][t][1] = 0.0;
-    const size_t qs = (size_t)I * 2 * n;
-    const size_t rstride = (size_t)J * n;
-    const double *col = db + (size_t)j0 * n + k;
-    const double *selk = sel + (size_t)q0 * qs + k;
-    auto load_row = [&](int i, dbl2 (&d)[TJ], dbl2 (&sm)[QB], dbl2 (&sb)[QB]) {
-#pragma unroll
-        for (int qb = 0; qb < QB; qb++) {
-            sm[qb] = *(const dbl2 *)(selk + (size_t)qb * qs + (size_t)(2 * i) * n);
-            sb[qb] = *(const dbl2 *)(selk + (size_t)qb * qs + (size_t)(2 * i + 1) * n);
-        }
-        // the database is read once per pass: stream it past the caches
-#pragma unroll
-        for (int t = 0; t < TJ; t++)
-            d[t] = __builtin_nontemporal_load((const dbl2 *)(col + (size_t)i * rstride + (size_t)t * n));
-    };
-    auto mac_row = [&](const dbl2 (&d)[TJ], const dbl2 (&sm)[QB], const dbl2 (&sb)[QB]) {
-#pragma unroll
-        for (int qb = 0; qb < QB; qb++)
-#pragma unroll
-            for (int t = 0; t < TJ; t++) {
-                am[qb][t][0] += mod_mul_lazy(d[t].x, sm[qb].x, m);
-                am[qb][t][1] += mod_mul_lazy(d[t].y, sm[qb].y, m);
-                ab[qb][t][0] += mod_mul_lazy(d[t].x, sb[qb].x, m);
-                ab[qb][t][1] += mod_mul_lazy(d[t].y, sb[qb].y, m);
-            }
-    };
-    dbl2 dA[TJ], smA[QB], sbA[QB], dB[TJ], smB[QB], sbB[QB];
-    int i = i0;
-    if (i < i1) load_row(i, dA, smA, sbA);
-    for (; i + 1 < i1; i += 2) {
-        load_row(i + 1, dB, smB, sbB);
-        mac_row(dA, smA, sbA);
-        load_row(min(i + 2, i1 - 1), dA, smA, sbA);  // clamped: the last reload is unused
-        mac_row(dB, smB, sbB);
-    }
-    if (i < i1) mac_row(dA, smA, sbA);
-    double *o = out + (size_t)blockIdx.z * Q * J * 2 * n;
-#pragma unroll
-    for (int qb = 0; qb < QB; qb++)
-#pragma unroll
-        for (int t = 0; t < TJ; t++) {
-            dbl2 vm, vb;
-            vm.x = mod_reduce_small(am[qb][t][0], m);
-            vm.y = mod_reduce_small(am[qb][t][1], m);
-            vb.x = mod_reduce_small(ab[qb][t][0], m);
-            vb.y = mod_reduce_small(ab[qb][t][1], m);
-            double *oq = o + (size_t)(q0 + qb) * J * 2 * n + ((size_t)(j0 + t) * 2) * n + k;
-            *(dbl2 *)oq = vm;
-            *(dbl2 *)(oq + n) = vb;
-        }
-}
-
-// Batched FirstDim (eight queries per database load), where the
-// multiply-accumulate rather than the stream bounds the pass: each row's
-// selection tile (QB x 2 parts x 64 slots) is staged in LDS once per
-// workgroup and shared by four wavefronts of TJ columns each, instead of
-// every wavefront fetching it through the caches; one slot per lane; the
-// next row's database values and selection tile load while the current row
-// multiplies (double-buffered tile, one barrier per row). Measured 753 us
-// against 865 us for k_first_dim<8, 2> at 256 x 256 cells.
-template <int QB, int TJ>
-__global__ __launch_bounds__(256) void k_first_dim_lds(const double *db, const double *sel, double *out, int I, int J,
-                                                       int n, int Q, int q0, int rows_per_chunk, FMod m) {
-    __shared__ double st[2][QB * 2 * 64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int k0 = blockIdx.x * 64;
-    const int k = k0 + lane;
-    const int j0 = (blockIdx.y * 4 + wv) * TJ;
-    const bool active = j0 < J;  // narrow databases: waves past J only help stage the tiles
-    const int i0 = blockIdx.z * rows_per_chunk;
-    const int i1 = min(I, i0 + rows_per_chunk);
-    double am[QB][TJ], ab[QB][TJ];
-#pragma unroll
-    for (int qb = 0; qb < QB; qb++)
-#pragma unroll
-        for (int t = 0; t < TJ; t++) am[qb][t] = ab[qb][t] = 0.0;
-    const size_t qs = (size_t)I * 2 * n;
-    const size_t rstride = (size_t)J * n;
-    const double *col = db + (size_t)(active ? j0 : 0) * n + k;
-    const double *selq = sel + (size_t)q0 * qs + k0;
-    // tile entry e = (qb * 2 + part) * 64 + slot
-    auto stage = [&](int i, double *dst) {
-        for (int e = threadIdx.x; e < QB * 2 * 64; e += 256) {
-            const int qp = e >> 6, sl = e & 63;
-            dst[e] = selq[(size_t)(qp >> 1) * qs + (size_t)(2 * i + (qp & 1)) * n + sl];
-        }
-    };
-    double d[TJ], dn[TJ];
-#pragma unroll
-    for (int t = 0; t < TJ; t++) d[t] = dn[t] = 0.0;
-    if (i0 < i1) {
-        stage(i0, st[0]);
-        if (active)
-#pragma unroll
-            for (int t = 0; t < TJ; t++) d[t] = __builtin_nontemporal_load(col + (size_t)i0 * rstride + (size_t)t * n);
-    }
-    __syncthreads();
-    for (int i = i0; i < i1; i++) {
-        const int cb = (i - i0) & 1;
-        const int inext = min(i + 1, i1 - 1);  // clamped: the last reload is unused
-        if (active)
-#pragma unroll
-            for (int t = 0; t < TJ; t++)
-                dn[t] = __builtin_nontemporal_load(col + (size_t)inext * rstride + (size_t)t * n);
-        if (i + 1 < i1) stage(i + 1, st[cb ^ 1]);
-        const double *sv = st[cb];
-#pragma unroll
-        for (int qb = 0; qb < QB; qb++) {
-            const double sm = sv[(qb * 2) * 64 + lane], sb = sv[(qb * 2 + 1) * 64 + lane];
-#pragma unroll
-            for (int t = 0; t < TJ; t++) {
-                am[qb][t] += mod_mul_lazy(d[t], sm, m);
-                ab[qb][t] += mod_mul_lazy(d[t], sb, m);
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < TJ; t++) d[t] = dn[t];
-        __syncthreads();
-    }
-    if (!active) return;
-    double *o = out + (size_t)blockIdx.z * Q * J * 2 * n;
-#pragma unroll
-    for (int qb = 0; qb < QB; qb++)
-#pragma unroll
-        for (int t = 0; t < TJ; t++) {
-            double *oq = o + (size_t)(q0 + qb) * J * 2 * n + ((size_t)(j0 + t) * 2) * n + k;
-            oq[0] = mod_reduce_small(am[qb][t], m);
-            oq[n] = mod_reduce_small(ab[qb][t], m);
-        }
-}
-
-// Sum of the row-chunk partials into chunk 0.
-__global__ void k_sum_chunks(double *part, size_t words, int chunks, FMod m) {
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < words; t += (size_t)gridDim.x * blockDim.x) {
-        double acc = part[t];
-        for (int c = 1; c < chunks; c++) acc += part[(size_t)c * words + t];
-        part[t] = mod_reduce_small(acc, m);
-    }
-}
-
-static int fd_qb(int Q) { return Q >= 8 ? 8 : (Q >= 4 ? 4 : (Q >= 2 ? 2 : 1)); }
-// columns per workgroup: 4 while J allows (J is a power of two), 2 from four
-// queries on (their 4*QB*TJ accumulators stay within the register file)
-static int fd_tj(int J, int QB) {
-    const int t = J >= 4 ? 4 : J;
-    return QB >= 4 && t > 2 ? 2 : t;
-}
-
-// One chunk of rows: the whole column per workgroup measured fastest for one
-// and for eight queries per pass at 256 x 256 cells (the partial planes and
-// their sum cost more than the extra workgroups gain).
-int first_dim_chunks(int I, int J, int n, int Q) {
-    (void)I;
-    (void)J;
-    (void)n;
-    (void)Q;
-    return 1;
-}
-
-template <int QB, int TJ>
-static void fd_pass(const double *db, const double *sel, double *out, int I, int J, int n, int Q, int q0, int chunks,
-                    FMod m, hipStream_t st) {
-    const int rows = (I + chunks - 1) / chunks;
-    dim3 grid((n / 2 + 255) / 256, J / TJ, chunks);
-    hipLaunchKernelGGL((k_first_dim<QB, TJ>), grid, dim3(256), 0, st, db, sel, out, I, J, n, Q, q0, rows, m);
-}
-
-template <int QB>
-static void fd_pass_q(const double *db, const double *sel, double *out, int I, int J, int n, int Q, int q0, int chunks,
-                      FMod m, hipStream_t st) {
-    if (QB == 8 && n % 64 == 0) {
-        // four wavefronts of four columns per workgroup; J < 16 leaves waves idle
-        const int tj = J >= 4 ? 4 : J;
-        const int rows = (I + chunks - 1) / chunks;
-        dim3 grid(n / 64, (J + 4 * tj - 1) / (4 * tj), chunks);
-        if (tj == 4)
-            hipLaunchKernelGGL((k_first_dim_lds<8, 4>), grid, dim3(256), 0, st, db, sel, out, I, J, n, Q, q0, rows, m);
-        else if (tj == 2)
-            hipLaunchKernelGGL((k_first_dim_lds<8, 2>), grid, dim3(256), 0, st, db, sel, out, I, J, n, Q, q0, rows, m);
-        else
-            hipLaunchKernelGGL((k_first_dim_lds<8, 1>), grid, dim3(256), 0, st, db, sel, out, I, J, n, Q, q0, rows, m);
-        return;
-    }
-    const int tj = fd_tj(J, QB);
-    if (QB < 4 && tj == 4) fd_pass<QB, (QB < 4 ? 4 : 2)>(db, sel, out, I, J, n, Q, q0, chunks, m, st);
-    else if (tj == 2) fd_pass<QB, 2>(db, sel, out, I, J, n, Q, q0, chunks, m, st);
-    else fd_pass<QB, 1>(db, sel, out, I, J, n, Q, q0, chunks, m, st);
-}
-
-void launch_first_dim(const double *db, const double *sel_eval, double *out_eval, int I, int J, int n, int Q,
-                      int chunks, FMod m, hipStream_t st) {
-    for (int q0 = 0; q0 < Q;) {
-        const int qb = fd_qb(Q - q0);
-        if (qb == 8) fd_pass_q<8>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
-        else if (qb == 4) fd_pass_q<4>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
-        else if (qb == 2) fd_pass_q<2>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
-        else fd_pass_q<1>(db, sel_eval, out_eval, I, J, n, Q, q0, chunks, m, st);
-        q0 += qb;
-    }
-    if (chunks > 1) {
-        const size_t words = (size_t)Q * J * 2 * n;
-        unsigned blocks = (unsigned)((words + 255) / 256);
-        if (blocks > 8192) blocks = 8192;
-        hipLaunchKernelGGL(k_sum_chunks, dim3(blocks), dim3(256), 0, st, out_eval, words, chunks, m);
-    }
-}
-
 // ---- RespComp (resp_comp.rs:46-76) -----------------------------------------------------
 
 // One workgroup of 1024 threads per query. mod_switch_sym q2 -> q3 of the

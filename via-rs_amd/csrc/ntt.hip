@@ -8,7 +8,8 @@ namespace via {
 // n == kSlots). The polys of a workgroup share a limb (checked by the host).
 // Radix-16 passes: these kernels hold little else in registers.
 template <bool INV>
-__global__ __launch_bounds__(kThreads) void k_ntt_batch(RingDev r, double *data, size_t npolys, size_t limb_group) {
+__global__ __launch_bounds__(kThreads) void k_ntt_batch(RingDev r, const double *src, double *data, size_t npolys,
+                                                        size_t limb_group) {
     __shared__ double s[lds_words(kSlots)];
     const int n = r.n;
     const int P = kSlots / n;
@@ -18,8 +19,9 @@ __global__ __launch_bounds__(kThreads) void k_ntt_batch(RingDev r, double *data,
     RingDev r1 = r;
     if (l == 1) r1.t[0] = r.t[1];
     double *g = data + p0 * n;
+    const double *gs = src + p0 * n;
     const int S = cnt * n;
-    for (int i = threadIdx.x; i < S; i += blockDim.x) s[lds_pad(i)] = g[i];
+    for (int i = threadIdx.x; i < S; i += blockDim.x) s[lds_pad(i)] = gs[i];
     __syncthreads();
     if (!INV) {
         lds_ntt_fwd<1, 4>(s, S, r1);
@@ -36,9 +38,17 @@ void launch_ntt(const RingDev &r, double *data, size_t npolys, size_t limb_group
     const size_t P = kSlots / r.n;
     const size_t blocks = (npolys + P - 1) / P;
     if (inverse)
-        hipLaunchKernelGGL(k_ntt_batch<true>, dim3(blocks), dim3(kThreads), 0, st, r, data, npolys, limb_group);
+        hipLaunchKernelGGL(k_ntt_batch<true>, dim3(blocks), dim3(kThreads), 0, st, r, data, data, npolys, limb_group);
     else
-        hipLaunchKernelGGL(k_ntt_batch<false>, dim3(blocks), dim3(kThreads), 0, st, r, data, npolys, limb_group);
+        hipLaunchKernelGGL(k_ntt_batch<false>, dim3(blocks), dim3(kThreads), 0, st, r, data, data, npolys, limb_group);
+}
+
+void launch_ntt_to(const RingDev &r, const double *src, double *dst, size_t npolys, size_t limb_group,
+                   hipStream_t st) {
+    if (!npolys) return;
+    const size_t P = kSlots / r.n;
+    const size_t blocks = (npolys + P - 1) / P;
+    hipLaunchKernelGGL(k_ntt_batch<false>, dim3(blocks), dim3(kThreads), 0, st, r, src, dst, npolys, limb_group);
 }
 
 __global__ void k_u64_to_f64(const uint64_t *src, double *dst, size_t n) {

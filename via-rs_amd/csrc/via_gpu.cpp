@@ -192,6 +192,7 @@ via_status check_params(const via_params *pp) {
     const uint64_t qs[4] = {pp->q1[0], pp->q1[1], pp->q2, pp->q3};
     for (uint64_t q : qs)
         if (q < 3 || q >= kMaxModulus || (q - 1) % (2ULL * pp->n1)) return VIA_E_INVALID;
+    if (pp->q2 >= (1ULL << 35)) return VIA_E_INVALID;  // FirstDim limbs: five 7-bit limbs
     if (!is_pow2(pp->q4) || !is_pow2(pp->p)) return VIA_E_INVALID;
     u128 Q1 = (u128)pp->q1[0] * pp->q1[1];
     if (Q1 >> 96) return VIA_E_INVALID;
@@ -261,8 +262,9 @@ via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_
     I = std::max(I, w.I);
     J = std::max(J, w.J);
     bits = std::max(bits, w.bits);
+    const uint64_t gen = w.gen;
     free_ws(w);
-    w.gen++;
+    w.gen = gen + 1;  // captured graphs of the old buffers never match again
     const size_t n = pp->n1, nl = 2, S = kSlots, lq = pp->l_query;
     w.Q = Q;
     w.B = B;
@@ -279,7 +281,7 @@ via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_
     w.groups = groups;
     // room for the autotuner's candidates (tune_candidates)
     w.groups_max = std::max(groups, max_tune_groups((int)items, (int)pp->l_ck));
-    w.fd_chunks = first_dim_chunks((int)I, (int)J, (int)n, (int)Q);
+    const FdPlan fp = fd_plan((int)I, (int)J, (int)n, (int)Q);
     via_status s;
 #define A(p, cnt) \
     if ((s = dalloc(w, &(p), (cnt)))) return s
@@ -298,7 +300,9 @@ via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_
     A(w.dm[1], Q * I * 2 * nl * n);
     A(w.sw, Q * I * 2 * n);
     A(w.sel, Q * I * 2 * n);
-    A(w.fd, (size_t)w.fd_chunks * Q * J * 2 * n);
+    A(w.fd, Q * J * 2 * n);
+    A(w.fdk, fd_out_words(fp));
+    A(w.sell, fd_sel_bytes(fp));
     A(w.cm[0], Q * J * 2 * n);
     A(w.cm[1], Q * J * 2 * n);
     A(w.rgsw2, Q * bits * 4 * lq * n);
@@ -520,7 +524,16 @@ static via_status db_common(via_ctx *c, const via_params *pp, const uint64_t *re
     db->id = next_object_id();
     db->ctx = c;
     db->pp = *pp;
-    if (hipMalloc(&db->cells, sizeof(double) * I * J * n1) != hipSuccess) {
+    const FdPlan fp = fd_plan((int)I, (int)J, (int)n1, 1);
+    if (hipMalloc(&db->limbs, fd_db_bytes(fp)) != hipSuccess) {
+        delete db;
+        return VIA_E_NOMEM;
+    }
+    // setup_db and the transform run over fp64 cells, then the cells are
+    // packed into the limb fragments and released
+    double *cells = nullptr;
+    if (hipMalloc(&cells, sizeof(double) * I * J * n1) != hipSuccess) {
+        (void)hipFree(db->limbs);
         delete db;
         return VIA_E_NOMEM;
     }
@@ -529,23 +542,33 @@ static via_status db_common(via_ctx *c, const via_params *pp, const uint64_t *re
     size_t chunk = n_records ? n_records : 1;
     void *stage = nullptr;
     if (hipMalloc(&stage, rec_words * chunk) != hipSuccess) {
-        (void)hipFree(db->cells);
+        (void)hipFree(cells);
+        (void)hipFree(db->limbs);
         delete db;
         return VIA_E_NOMEM;
     }
     if (n_records) HIP_TRY(hipMemcpyAsync(stage, bytes ? (const void *)bytes : (const void *)records, rec_words * n_records,
                                           hipMemcpyHostToDevice, c->stream));
     if (bytes)
-        launch_setup_db_bytes((const uint8_t *)stage, n_records, db->cells, (int)I, (int)J, (int)n1, (int)nr, c->stream);
+        launch_setup_db_bytes((const uint8_t *)stage, n_records, cells, (int)I, (int)J, (int)n1, (int)nr, c->stream);
     else
-        launch_setup_db((const uint64_t *)stage, n_records, db->cells, (int)I, (int)J, (int)n1, (int)nr, pp->p,
+        launch_setup_db((const uint64_t *)stage, n_records, cells, (int)I, (int)J, (int)n1, (int)nr, pp->p,
                         c->stream);
     RingDev r2;
     uint64_t q2 = pp->q2;
-    if ((s = make_ring(c, (uint32_t)n1, &q2, 1, &r2))) return s;
-    launch_ntt(r2, db->cells, I * J, 1, false, c->stream);
+    if ((s = make_ring(c, (uint32_t)n1, &q2, 1, &r2))) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(stage);
+        (void)hipFree(cells);
+        (void)hipFree(db->limbs);
+        delete db;
+        return s;
+    }
+    launch_ntt(r2, cells, I * J, 1, false, c->stream);
+    launch_pack_db(cells, db->limbs, fp, c->stream);
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipFree(stage));
+    HIP_TRY(hipFree(cells));
     HIP_TRY(hipGetLastError());
     *out = db;
     return VIA_OK;
@@ -566,7 +589,7 @@ void via_db_destroy(via_db *db) {
     if (!db) return;
     (void)hipSetDevice(db->ctx->device);
     (void)hipStreamSynchronize(db->ctx->stream);
-    (void)hipFree(db->cells);
+    (void)hipFree(db->limbs);
     delete db;
 }
 
@@ -577,7 +600,7 @@ via_status via_db_read_cells(const via_db *db, uint64_t *out, size_t words) {
     via_ctx *c = db->ctx;
     uint64_t *tmp = nullptr;
     HIP_TRY(hipMalloc(&tmp, sizeof(uint64_t) * total));
-    launch_f64_to_u64(db->cells, tmp, total, c->stream);
+    launch_unpack_db(db->limbs, tmp, fd_plan((int)db->pp.num_rows, (int)db->pp.num_cols, (int)db->pp.n1, 1), c->stream);
     HIP_TRY(hipMemcpyAsync(out, tmp, sizeof(uint64_t) * total, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipFree(tmp));
@@ -963,12 +986,14 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     REC_EVENT(c->ev[3], st);
 
     // ---- step 4: first_dim (first_dim.rs:94-184), one database pass for all queries ----
-    HIP_TRY(hipMemcpyAsync(w.sel, w.sw, sizeof(double) * Q * I * 2 * n, hipMemcpyDeviceToDevice, st));
-    launch_ntt(r2, w.sel, (size_t)Q * I * 2, 1, false, st);
-    const int chunks = first_dim_chunks((int)I, (int)J, (int)n, Q);
+    launch_ntt_to(r2, w.sw, w.sel, (size_t)Q * I * 2, 1, st);
+    const FdPlan fp = fd_plan((int)I, (int)J, (int)n, Q);
+    const FMod m2 = fmod_make(pp->q2);
+    launch_sel_pack(w.sel, w.sell, fp, st);
     REC_EVENT(c->ev_fd[0], st);
-    launch_first_dim(db->cells, w.sel, w.fd, (int)I, (int)J, (int)n, Q, chunks, fmod_make(pp->q2), st);
+    launch_fd_mfma(db->limbs, w.sell, w.fdk, fp, m2, st);
     REC_EVENT(c->ev_fd[1], st);
+    launch_fd_gather(w.fdk, w.fd, fp, m2, st);
     launch_ntt(r2, w.fd, (size_t)Q * J * 2, 1, true, st);
     c->stage_ptr[3] = w.fd;
     c->stage_words[3] = J * 2 * n;
@@ -1495,6 +1520,43 @@ via_status via_ntt(via_ctx *c, uint32_t n, const uint64_t *moduli, uint32_t nmod
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipFree(u));
     HIP_TRY(hipFree(d));
+    return VIA_OK;
+}
+
+// FirstDim alone over host buffers (first_dim.rs:94-184 batched over
+// queries): switched [q][I][part][n1] -> out [q][J][part][n1], both
+// coefficient form at q2; the same kernels as the answer path.
+via_status via_first_dim(via_ctx *c, const via_db *db, uint32_t nq, const uint64_t *switched, uint64_t *out) {
+    if (!c || !db || !switched || !out || nq < 1 || nq > (uint32_t)kMaxQ) return VIA_E_INVALID;
+    if (db->ctx != c) return VIA_E_INVALID;
+    HIP_TRY(hipSetDevice(c->device));
+    const via_params *pp = &db->pp;
+    via_status s = ensure_ws(c, pp, nq, 1, 1);
+    if (s) return s;
+    Workspace &w = c->ws;
+    const size_t I = pp->num_rows, J = pp->num_cols, n = pp->n1;
+    const size_t in_words = (size_t)nq * I * 2 * n, out_words = (size_t)nq * J * 2 * n;
+    RingDev r2;
+    uint64_t q2 = pp->q2;
+    if ((s = make_ring(c, (uint32_t)n, &q2, 1, &r2))) return s;
+    uint64_t *u = nullptr;
+    HIP_TRY(hipMalloc(&u, sizeof(uint64_t) * std::max(in_words, out_words)));
+    hipStream_t st = c->stream;
+    HIP_TRY(hipMemcpyAsync(u, switched, sizeof(uint64_t) * in_words, hipMemcpyHostToDevice, st));
+    launch_u64_to_f64(u, w.sw, in_words, st);
+    launch_ntt_to(r2, w.sw, w.sel, (size_t)nq * I * 2, 1, st);
+    const FdPlan fp = fd_plan((int)I, (int)J, (int)n, (int)nq);
+    const FMod m2 = fmod_make(pp->q2);
+    launch_sel_pack(w.sel, w.sell, fp, st);
+    launch_fd_mfma(db->limbs, w.sell, w.fdk, fp, m2, st);
+    launch_fd_gather(w.fdk, w.fd, fp, m2, st);
+    launch_ntt(r2, w.fd, (size_t)nq * J * 2, 1, true, st);
+    launch_f64_to_u64(w.fd, u, out_words, st);
+    HIP_TRY(hipMemcpyAsync(out, u, sizeof(uint64_t) * out_words, hipMemcpyDeviceToHost, st));
+    const hipError_t e = hipStreamSynchronize(st);
+    (void)hipFree(u);
+    if (e != hipSuccess) return VIA_E_DEVICE;
+    HIP_TRY(hipGetLastError());
     return VIA_OK;
 }
 

@@ -119,6 +119,7 @@ def lib():
             "via_db_create_bytes": (C.c_int, [vp, P, C.c_char_p, C.c_size_t, C.POINTER(vp)]),
             "via_db_destroy": (None, [vp]),
             "via_db_read_cells": (C.c_int, [vp, u64p, C.c_size_t]),
+            "via_first_dim": (C.c_int, [vp, vp, C.c_uint32, u64p, u64p]),
             "via_keys_create": (C.c_int, [vp, P, u64p, u64p, u64p, C.POINTER(vp)]),
             "via_keys_destroy": (None, [vp]),
             "via_answer": (C.c_int, [vp, vp, vp, u64p, C.c_size_t, u64p, u64p, C.POINTER(ErrDetail)]),
@@ -294,6 +295,17 @@ class PreparedDb:
             _check(lib().via_db_create(ctx.h, C.byref(pp), _p(r), r.size // pp.n_rec, C.byref(h)))
         self.h, self.ctx, self.pp = h, ctx, pp
         ctx.adopt(self)
+
+    def first_dim(self, switched, nq=1):
+        """first_dim (first_dim.rs:94-184) of nq queries: switched
+        [q][I][part][n1] -> [q][J][part][n1], coefficient form at q2."""
+        pp = self.pp
+        sw = _u64(switched)
+        if sw.size != nq * pp.num_rows * 2 * pp.n1:
+            raise ValueError("switched holds %d words, expected %d" % (sw.size, nq * pp.num_rows * 2 * pp.n1))
+        out = np.zeros(nq * pp.num_cols * 2 * pp.n1, dtype=np.uint64)
+        _check(lib().via_first_dim(self.ctx.h, self.h, nq, _p(sw), _p(out)))
+        return out
 
     def cells(self):
         words = self.pp.num_rows * self.pp.num_cols * self.pp.n1
