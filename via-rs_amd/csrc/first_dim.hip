@@ -204,16 +204,17 @@ void launch_sel_pack(const double *sel, uint8_t *sell, const FdPlan &p, hipStrea
 
 // Workgroup (slot k, row segment): the segment's packed selection goes to LDS
 // once; wavefront w takes column blocks mb = w, w + 4, ...; for each row
-// chunk it holds the five database limb fragments (the next chunk's loads in
+// chunk it holds the five database limb fragments (two more chunks' loads in
 // flight) and runs 25 MFMAs per selection block of 16 columns into the nine
 // shift sums, then folds and stores its 16 x 2Q outputs.
-template <int KC, int NB>
-__global__ __launch_bounds__(256) void k_fd_mfma(const uint8_t *__restrict__ dbl, const uint8_t *__restrict__ sell,
+template <int KC, int NBW, int SPLIT>
+__global__ __launch_bounds__(256, 2) void k_fd_mfma(const uint8_t *__restrict__ dbl, const uint8_t *__restrict__ sell,
                                                  double *__restrict__ out, FdPlan p, FMod m) {
     typedef FdOp<KC> Op;
     typedef typename Op::T T;
     constexpr int KB = KC / 4, FRAG = 64 * KB;
     constexpr int KS = kFdSegRows / KC;
+    constexpr int NB = NBW * SPLIT;  // selection blocks the LDS holds
     __shared__ __attribute__((aligned(16))) uint8_t sb[KS * kLimbs * 16 * NB * KC];
     const int k = blockIdx.x, seg = blockIdx.y;
     const int kc0 = seg * KS, nkc = min(KS, p.KCN - kc0);
@@ -224,56 +225,55 @@ __global__ __launch_bounds__(256) void k_fd_mfma(const uint8_t *__restrict__ dbl
         for (int o = threadIdx.x * 16; o < bytes; o += 256 * 16) *(uint4 *)(sb + o) = *(const uint4 *)(src + o);
     }
     __syncthreads();
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // SPLIT wavefronts share a column block, each with NBW selection blocks
+    const int w = wv / SPLIT, nb0 = (wv % SPLIT) * NBW;
+    constexpr int MBW = 4 / SPLIT;  // column blocks in flight per workgroup
     const int c = lane & 15, g = lane >> 4;
     const size_t mbstride = (size_t)p.KCN * kLimbs * FRAG;
     const uint8_t *dk = dbl + (size_t)k * p.MB * mbstride + (size_t)kc0 * kLimbs * FRAG + (size_t)lane * KB;
     const size_t P = (size_t)p.Q * 2 * p.J;
     double *ok = out + ((size_t)seg * p.n + k) * P;
-    for (int mb = w; mb < p.MB; mb += 4) {
-        fd_v4i acc[NB][kLimbShifts];
+    // the wave's steps: (column block mb = w + 4 u, row chunk t) for u, t in order
+    const int nmb = (p.MB - w + MBW - 1) / MBW;
+    const int steps = nmb > 0 ? nmb * nkc : 0;
+    auto load = [&](T (&dst)[kLimbs], int s) {
+        const int u = s / nkc, t = s - u * nkc;
+        const uint8_t *da = dk + (size_t)(w + MBW * u) * mbstride + (size_t)t * kLimbs * FRAG;
 #pragma unroll
-        for (int nb = 0; nb < NB; nb++)
+        for (int l = 0; l < kLimbs; l++) dst[l] = __builtin_nontemporal_load((const T *)(da + (size_t)l * FRAG));
+    };
+    fd_v4i acc[NBW][kLimbShifts];
 #pragma unroll
-            for (int s = 0; s < kLimbShifts; s++) acc[nb][s] = fd_v4i{0, 0, 0, 0};
-        const uint8_t *da = dk + (size_t)mb * mbstride;
-        T a[kLimbs], an[kLimbs];
+    for (int nb = 0; nb < NBW; nb++)
 #pragma unroll
-        for (int l = 0; l < kLimbs; l++) a[l] = __builtin_nontemporal_load((const T *)(da + (size_t)l * FRAG));
-        for (int t = 0; t < nkc; t++) {
-            if (t + 1 < nkc) {
+        for (int sh = 0; sh < kLimbShifts; sh++) acc[nb][sh] = fd_v4i{0, 0, 0, 0};
+    auto step = [&](const T (&a)[kLimbs], int s) {
+        const int u = s / nkc, t = s - u * nkc;
+        const uint8_t *bt = sb + (size_t)t * chunk;
 #pragma unroll
-                for (int l = 0; l < kLimbs; l++)
-                    an[l] = __builtin_nontemporal_load((const T *)(da + ((size_t)(t + 1) * kLimbs + l) * FRAG));
-            }
-            const uint8_t *bt = sb + (size_t)t * chunk;
+        for (int nb = 0; nb < NBW; nb++) {
+            const int qp = (nb0 + nb) * 16 + c;
 #pragma unroll
-            for (int nb = 0; nb < NB; nb++) {
-                const int qp = nb * 16 + c;
+            for (int bl = 0; bl < kLimbs; bl++) {
+                const T b = qp < p.QP ? *(const T *)(bt + ((size_t)(bl * 4 + g) * p.QP + qp) * KB) : Op::zero();
 #pragma unroll
-                for (int bl = 0; bl < kLimbs; bl++) {
-                    const T b = qp < p.QP ? *(const T *)(bt + ((size_t)(bl * 4 + g) * p.QP + qp) * KB) : Op::zero();
-#pragma unroll
-                    for (int al = 0; al < kLimbs; al++) acc[nb][al + bl] = Op::mac(a[al], b, acc[nb][al + bl]);
-                }
-            }
-            if (t + 1 < nkc) {
-#pragma unroll
-                for (int l = 0; l < kLimbs; l++) a[l] = an[l];
+                for (int al = 0; al < kLimbs; al++) acc[nb][al + bl] = Op::mac(a[al], b, acc[nb][al + bl]);
             }
         }
+        if (t != nkc - 1) return;
         // lane (g, c) of block nb holds rows j = 16 mb + 4 g + [0, 4) of column qp
-        const int j0 = mb * 16 + 4 * g;
+        const int j0 = (w + MBW * u) * 16 + 4 * g;
 #pragma unroll
-        for (int nb = 0; nb < NB; nb++) {
-            const int qp = nb * 16 + c;
+        for (int nb = 0; nb < NBW; nb++) {
+            const int qp = (nb0 + nb) * 16 + c;
             if (qp >= p.QP) continue;
             double v[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 int32_t C[kLimbShifts];
 #pragma unroll
-                for (int s = 0; s < kLimbShifts; s++) C[s] = acc[nb][s][r];
+                for (int sh = 0; sh < kLimbShifts; sh++) C[sh] = acc[nb][sh][r];
                 v[r] = limb_fold(C, m);
             }
             double *o = ok + (size_t)qp * p.J + j0;  // [q][part][j] with qp = 2q + part
@@ -287,6 +287,25 @@ __global__ __launch_bounds__(256) void k_fd_mfma(const uint8_t *__restrict__ dbl
                     if (j0 + r < p.J) o[r] = v[r];
             }
         }
+#pragma unroll
+        for (int nb = 0; nb < NBW; nb++)
+#pragma unroll
+            for (int sh = 0; sh < kLimbShifts; sh++) acc[nb][sh] = fd_v4i{0, 0, 0, 0};
+    };
+    // three fragment sets in rotation: the loads of steps s + 1 and s + 2 are
+    // in flight while step s multiplies (across column blocks too)
+    T f0[kLimbs], f1[kLimbs], f2[kLimbs];
+    if (steps > 0) load(f0, 0);
+    if (steps > 1) load(f1, 1);
+    for (int s = 0; s < steps; s += 3) {
+        if (s + 2 < steps) load(f2, s + 2);
+        step(f0, s);
+        if (s + 1 >= steps) break;
+        if (s + 3 < steps) load(f0, s + 3);
+        step(f1, s + 1);
+        if (s + 2 >= steps) break;
+        if (s + 4 < steps) load(f1, s + 4);
+        step(f2, s + 2);
     }
 }
 
@@ -295,10 +314,11 @@ static void fd_mfma_kc(const uint8_t *dbl, const uint8_t *sell, double *out_k, c
                        hipStream_t st) {
     const dim3 grid(p.n, p.nseg);
     switch (p.NB) {
-        case 1: hipLaunchKernelGGL((k_fd_mfma<KC, 1>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m); break;
-        case 2: hipLaunchKernelGGL((k_fd_mfma<KC, 2>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m); break;
-        case 3: hipLaunchKernelGGL((k_fd_mfma<KC, 3>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m); break;
-        default: hipLaunchKernelGGL((k_fd_mfma<KC, 4>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m); break;
+        case 1: hipLaunchKernelGGL((k_fd_mfma<KC, 1, 1>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m); break;
+        case 2: hipLaunchKernelGGL((k_fd_mfma<KC, 2, 1>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m); break;
+        // three or four blocks: two wavefronts per column block (the block past
+        // 2Q columns, if any, multiplies zeros)
+        default: hipLaunchKernelGGL((k_fd_mfma<KC, 2, 2>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m); break;
     }
 }
 

@@ -62,6 +62,16 @@ void launch_ntt(const RingDev &r, double *data, size_t npolys, size_t limb_group
                 hipStream_t s);
 // The same forward transform from src into dst (src may equal dst).
 void launch_ntt_to(const RingDev &r, const double *src, double *dst, size_t npolys, size_t limb_group, hipStream_t s);
+// Forward transforms of `groups` groups of gpolys polys: group g's poly i at
+// src + g*src_gstride + i*n into dst + g*dst_gstride + i*n; the limb of poly
+// i is (i / limb_group) % r.nl.
+void launch_ntt_groups(const RingDev &r, const double *src, double *dst, size_t gpolys, size_t groups,
+                       size_t src_gstride, size_t dst_gstride, size_t limb_group, hipStream_t s);
+// u64 -> f64 of `count` arrays of `words` words, array q to dst + q*words.
+struct U64Srcs {
+    const uint64_t *p[kMaxQ];
+};
+void launch_u64_to_f64_multi(const U64Srcs &srcs, int count, double *dst, size_t words, hipStream_t s);
 void launch_u64_to_f64(const uint64_t *src, double *dst, size_t n, hipStream_t s);
 void launch_f64_to_u64(const double *src, uint64_t *dst, size_t n, hipStream_t s);
 // Negacyclic products c = iNTT(NTT(a) * NTT(b)) of `count` u64 polys per limb
@@ -145,6 +155,10 @@ void launch_dmux_trivial(const RingDev &r, const double *rgsw_coeff, size_t rgsw
 // 2 polys [row][part][limb][n] -> [row][part][n].
 void launch_modswitch_q1_q2(const RingDev &r1, uint64_t q2, const double *in, double *out, size_t polys,
                             hipStream_t s);
+// The same for `groups` groups of polys: group g reads in + g*in_gstride and
+// writes out + g*out_gstride.
+void launch_modswitch_q1_q2_groups(const RingDev &r1, uint64_t q2, const double *in, size_t in_gstride, double *out,
+                                   size_t out_gstride, size_t polys, int groups, hipStream_t s);
 
 // FirstDim on the int8 matrix cores (first_dim.hip). The prepared database
 // is held as 7-bit limb fragments (5 bytes per residue); a pass packs the

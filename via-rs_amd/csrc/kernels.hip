@@ -667,10 +667,13 @@ void launch_dmux_trivial(const RingDev &r, const double *rgsw_coeff, size_t rgsw
 
 // ---- mod switch q1 -> q2 (mod_switch.rs:57-86) ----------------------------------
 
-__global__ void k_modswitch_q1_q2(RingDev r1, uint64_t q2, WideDiv qd, const double *in, double *out,
-                                  size_t polys) {
+// Group blockIdx.y reads in + y*in_gstride, writes out + y*out_gstride.
+__global__ void k_modswitch_q1_q2(RingDev r1, uint64_t q2, WideDiv qd, const double *in, size_t in_gstride,
+                                  double *out, size_t out_gstride, size_t polys) {
     const int n = r1.n;
     const size_t total = polys * n;
+    in += (size_t)blockIdx.y * in_gstride;
+    out += (size_t)blockIdx.y * out_gstride;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const size_t p = t / n, c = t - p * n;
         const double a0 = in[p * 2 * n + c], a1 = in[p * 2 * n + n + c];
@@ -681,13 +684,20 @@ __global__ void k_modswitch_q1_q2(RingDev r1, uint64_t q2, WideDiv qd, const dou
     }
 }
 
+void launch_modswitch_q1_q2_groups(const RingDev &r1, uint64_t q2, const double *in, size_t in_gstride, double *out,
+                                   size_t out_gstride, size_t polys, int groups, hipStream_t st) {
+    const size_t total = polys * r1.n;
+    if (!total || groups < 1) return;
+    size_t blocks = (total + 255) / 256;
+    const size_t cap = std::max<size_t>(1, 16384 / (size_t)groups);
+    if (blocks > cap) blocks = cap;
+    hipLaunchKernelGGL(k_modswitch_q1_q2, dim3((unsigned)blocks, groups), dim3(256), 0, st, r1, q2, r1.qdiv, in,
+                       in_gstride, out, out_gstride, polys);
+}
+
 void launch_modswitch_q1_q2(const RingDev &r1, uint64_t q2, const double *in, double *out, size_t polys,
                             hipStream_t st) {
-    const size_t total = polys * r1.n;
-    if (!total) return;
-    unsigned blocks = (unsigned)((total + 255) / 256);
-    if (blocks > 16384) blocks = 16384;
-    hipLaunchKernelGGL(k_modswitch_q1_q2, dim3(blocks), dim3(256), 0, st, r1, q2, r1.qdiv, in, out, polys);
+    launch_modswitch_q1_q2_groups(r1, q2, in, 0, out, 0, polys, 1, st);
 }
 
 // ---- RespComp (resp_comp.rs:46-76) -----------------------------------------------------

@@ -6,20 +6,24 @@ namespace via {
 
 // One workgroup transforms kSlots/n polys of one limb (or one poly when
 // n == kSlots). The polys of a workgroup share a limb (checked by the host).
-// Radix-16 passes: these kernels hold little else in registers.
+// Radix-16 passes: these kernels hold little else in registers. Polys come in
+// groups of gpolys (group g's poly r at src + g*src_gstride + r*n, written to
+// data + g*dst_gstride + r*n): several queries' arrays in one launch.
 template <bool INV>
 __global__ __launch_bounds__(kThreads) void k_ntt_batch(RingDev r, const double *src, double *data, size_t npolys,
-                                                        size_t limb_group) {
+                                                        size_t limb_group, size_t gpolys, size_t src_gstride,
+                                                        size_t dst_gstride) {
     __shared__ double s[lds_words(kSlots)];
     const int n = r.n;
     const int P = kSlots / n;
     const size_t p0 = (size_t)blockIdx.x * P;
     const int cnt = (int)((npolys - p0) < (size_t)P ? (npolys - p0) : (size_t)P);
-    const int l = (int)((p0 / limb_group) % (size_t)r.nl);
+    const size_t gq = p0 / gpolys, pr = p0 - gq * gpolys;
+    const int l = (int)((pr / limb_group) % (size_t)r.nl);
     RingDev r1 = r;
     if (l == 1) r1.t[0] = r.t[1];
-    double *g = data + p0 * n;
-    const double *gs = src + p0 * n;
+    double *g = data + gq * dst_gstride + pr * n;
+    const double *gs = src + gq * src_gstride + pr * n;
     const int S = cnt * n;
     for (int i = threadIdx.x; i < S; i += blockDim.x) s[lds_pad(i)] = gs[i];
     __syncthreads();
@@ -38,9 +42,11 @@ void launch_ntt(const RingDev &r, double *data, size_t npolys, size_t limb_group
     const size_t P = kSlots / r.n;
     const size_t blocks = (npolys + P - 1) / P;
     if (inverse)
-        hipLaunchKernelGGL(k_ntt_batch<true>, dim3(blocks), dim3(kThreads), 0, st, r, data, data, npolys, limb_group);
+        hipLaunchKernelGGL(k_ntt_batch<true>, dim3(blocks), dim3(kThreads), 0, st, r, data, data, npolys, limb_group,
+                           npolys, (size_t)0, (size_t)0);
     else
-        hipLaunchKernelGGL(k_ntt_batch<false>, dim3(blocks), dim3(kThreads), 0, st, r, data, data, npolys, limb_group);
+        hipLaunchKernelGGL(k_ntt_batch<false>, dim3(blocks), dim3(kThreads), 0, st, r, data, data, npolys, limb_group,
+                           npolys, (size_t)0, (size_t)0);
 }
 
 void launch_ntt_to(const RingDev &r, const double *src, double *dst, size_t npolys, size_t limb_group,
@@ -48,7 +54,24 @@ void launch_ntt_to(const RingDev &r, const double *src, double *dst, size_t npol
     if (!npolys) return;
     const size_t P = kSlots / r.n;
     const size_t blocks = (npolys + P - 1) / P;
-    hipLaunchKernelGGL(k_ntt_batch<false>, dim3(blocks), dim3(kThreads), 0, st, r, src, dst, npolys, limb_group);
+    hipLaunchKernelGGL(k_ntt_batch<false>, dim3(blocks), dim3(kThreads), 0, st, r, src, dst, npolys, limb_group,
+                       npolys, (size_t)0, (size_t)0);
+}
+
+void launch_ntt_groups(const RingDev &r, const double *src, double *dst, size_t gpolys, size_t groups,
+                       size_t src_gstride, size_t dst_gstride, size_t limb_group, hipStream_t st) {
+    const size_t npolys = gpolys * groups;
+    if (!npolys) return;
+    const size_t P = kSlots / r.n;
+    if (gpolys % P) {  // a workgroup's polys must not straddle two groups: one launch per group
+        for (size_t g = 0; g < groups; g++)
+            hipLaunchKernelGGL(k_ntt_batch<false>, dim3((gpolys + P - 1) / P), dim3(kThreads), 0, st, r,
+                               src + g * src_gstride, dst + g * dst_gstride, gpolys, limb_group, gpolys, (size_t)0,
+                               (size_t)0);
+        return;
+    }
+    hipLaunchKernelGGL(k_ntt_batch<false>, dim3(npolys / P), dim3(kThreads), 0, st, r, src, dst, npolys, limb_group,
+                       gpolys, src_gstride, dst_gstride);
 }
 
 __global__ void k_u64_to_f64(const uint64_t *src, double *dst, size_t n) {
@@ -64,6 +87,19 @@ __global__ void k_f64_to_u64(const double *src, uint64_t *dst, size_t n) {
 static unsigned grid_for(size_t n) {
     size_t b = (n + 255) / 256;
     return (unsigned)(b > 8192 ? 8192 : (b ? b : 1));
+}
+
+__global__ void k_u64_to_f64_multi(U64Srcs srcs, double *dst, size_t words) {
+    const uint64_t *src = srcs.p[blockIdx.y];
+    double *d = dst + (size_t)blockIdx.y * words;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+        d[i] = (double)src[i];
+}
+
+void launch_u64_to_f64_multi(const U64Srcs &srcs, int count, double *dst, size_t words, hipStream_t st) {
+    if (!words || count < 1) return;
+    const unsigned bx = std::max(1u, grid_for(words) / (unsigned)count);
+    hipLaunchKernelGGL(k_u64_to_f64_multi, dim3(bx, count), dim3(256), 0, st, srcs, dst, words);
 }
 
 void launch_u64_to_f64(const uint64_t *src, double *dst, size_t n, hipStream_t st) {

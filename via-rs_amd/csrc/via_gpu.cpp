@@ -832,7 +832,12 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
 
     REC_EVENT(c->ev[0], st);
     // ---- step 1: query_decomp (query_decomp.rs:39-95) ----
-    for (int q = 0; q < Q; q++) launch_u64_to_f64(d_lwes[q], w.lwe_f + (size_t)q * B * lw, B * lw, st);
+    {
+        U64Srcs srcs;
+        memset(&srcs, 0, sizeof srcs);
+        for (int q = 0; q < Q; q++) srcs.p[q] = d_lwes[q];
+        launch_u64_to_f64_multi(srcs, Q, w.lwe_f, B * lw, st);
+    }
     const Gadget gck = make_gadget(pp->ck_base, pp->l_ck);
     const DigitPlan dpk = digit_plan((uint32_t)pp->ck_base, (int)pp->l_ck);
     launch_lwe_digits(r1n, gck, dpk, w.lwe_f, (int)items, w.dig, st);
@@ -926,12 +931,9 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     const Gadget g1 = make_gadget(pp->b1, pp->l_query);
     const size_t ctw1 = 2 * nl * n;  // words per ciphertext at q1
     if (nd > 1) {
-        // eval form of the DMux bits 1..nd-1 of every query
-        for (int q = 0; q < Q; q++)
-            HIP_TRY(hipMemcpyAsync(w.rgsw_eval + ((size_t)q * bits + 1) * rgw, w.rgsw + ((size_t)q * bits + 1) * rgw,
-                                   sizeof(double) * (nd - 1) * rgw, hipMemcpyDeviceToDevice, st));
-        for (int q = 0; q < Q; q++)
-            launch_ntt(r1n, w.rgsw_eval + ((size_t)q * bits + 1) * rgw, (size_t)(nd - 1) * 4 * lq * nl, 1, false, st);
+        // eval form of the DMux bits 1..nd-1 of every query, one launch
+        launch_ntt_groups(r1n, w.rgsw + rgw, w.rgsw_eval + rgw, (size_t)(nd - 1) * 4 * lq * nl, (size_t)Q, bits * rgw,
+                          bits * rgw, 1, st);
     }
     int dc = 0;
     const u128 Q1 = (u128)pp->q1[0] * pp->q1[1];
@@ -1003,11 +1005,12 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     const Gadget g2 = make_gadget(pp->b2, pp->l_query);
     const size_t rgw2 = 4 * lq * n;
     const size_t ctw2 = 2 * n;
+    // the CMux and CRot bits of query q, mod-switched: w.rgsw2 + q * rq2
+    const size_t rq2 = (size_t)(nc + nr) * rgw2;
     if (nc + nr > 0) {
-        for (int q = 0; q < Q; q++)
-            launch_modswitch_q1_q2(r1n, pp->q2, w.rgsw + ((size_t)q * bits + nd) * rgw,
-                                   w.rgsw2 + (size_t)q * bits * rgw2, (size_t)(nc + nr) * 4 * lq, st);
-        launch_ntt(r2, w.rgsw2, (size_t)Q * bits * 4 * lq, 1, false, st);
+        launch_modswitch_q1_q2_groups(r1n, pp->q2, w.rgsw + (size_t)nd * rgw, bits * rgw, w.rgsw2, rq2,
+                                      (size_t)(nc + nr) * 4 * lq, Q, st);
+        launch_ntt(r2, w.rgsw2, (size_t)Q * (nc + nr) * 4 * lq, 1, false, st);
     }
     const double *cm_in = w.fd;
     int cc = 0;
@@ -1015,7 +1018,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         GateArgs ga;
         ga.mode = GATE_CMUX;
         ga.rgsw = w.rgsw2 + (size_t)b * rgw2;
-        ga.rgsw_qstride = bits * rgw2;
+        ga.rgsw_qstride = rq2;
         ga.in = cm_in;
         ga.in_qstride = J * ctw2;
         ga.out = w.cm[cc];
@@ -1040,7 +1043,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         GateArgs ga;
         ga.mode = GATE_CROT;
         ga.rgsw = w.rgsw2 + (size_t)(nc + b) * rgw2;
-        ga.rgsw_qstride = bits * rgw2;
+        ga.rgsw_qstride = rq2;
         ga.in = res;
         ga.in_qstride = res_qstride;
         ga.out = w.res[rc];
