@@ -95,9 +95,9 @@ int main() {
     }
     // int64 split digits against the u128 reference digits.
     {
-        const uint32_t bs[] = {18, 55879, 81, 8, 64};
-        const int Ls[] = {18, 2, 2, 8, 7};
-        for (int bi = 0; bi < 5; bi++) {
+        const uint32_t bs[] = {18, 55879, 81, 8, 64, 3, 255, 32767};
+        const int Ls[] = {18, 2, 2, 8, 7, 40, 9, 5};
+        for (int bi = 0; bi < 8; bi++) {
             uint32_t B = bs[bi];
             int L = Ls[bi];
             DigitPlan pl = digit_plan(B, L);
@@ -112,16 +112,22 @@ int main() {
                 SMag ref = v;
                 int64_t lo, hi;
                 digit_split(v, pl, lo, hi);
+                double flo, fhi;  // the fp64 digit path of the kernels
+                digit_split_f64(v, pl, flo, fhi);
                 for (int k = 0; k < L; k++) {
                     int32_t want = extract_digit(ref, B);
-                    int32_t got;
+                    int32_t got, gotf;
                     if (k < pl.h) {
                         got = digit_i64(lo, B, pl.inv_base);
                         if (k == pl.h - 1) hi += lo;
+                        gotf = digit_f64(flo, pl);
+                        if (k == pl.h - 1) fhi += flo;
                     } else {
                         got = digit_i64(hi, B, pl.inv_base);
+                        gotf = digit_f64(fhi, pl);
                     }
                     CHECK(got == want, "split digit B=%u t=%d k=%d\n", B, t, k);
+                    CHECK(gotf == want, "fp64 split digit B=%u t=%d k=%d\n", B, t, k);
                 }
                 count++;
             }

@@ -43,17 +43,17 @@ __device__ __forceinline__ SMag gadget_scale(double a0, double a1, const RingDev
 
 __device__ __forceinline__ double digit_residue(int32_t d, const FMod &m) { return d >= 0 ? (double)d : (double)d + m.q; }
 
-// Digit k of a value split by digit_split into (lo, hi) (LSB first,
+// Digit k of a value split by digit_split_f64 into (lo, hi) (LSB first,
 // gadget.rs:168-190): the first dp.h digits come from lo, whose final carry
-// joins hi, the rest from hi. Call for k = 0, 1, ... in order. The fp64
-// reciprocal of the base replaces the 128-bit divisions of extract_digit.
-__device__ __forceinline__ int32_t next_digit(int64_t &lo, int64_t &hi, int k, const DigitPlan &dp) {
+// joins hi, the rest from hi. Call for k = 0, 1, ... in order. Exact fp64
+// arithmetic on both halves replaces the 128-bit divisions of extract_digit.
+__device__ __forceinline__ int32_t next_digit(double &lo, double &hi, int k, const DigitPlan &dp) {
     int32_t d;
     if (k < dp.h) {
-        d = digit_i64(lo, dp.base, dp.inv_base);
+        d = digit_f64(lo, dp);
         if (k == dp.h - 1) hi += lo;
     } else {
-        d = digit_i64(hi, dp.base, dp.inv_base);
+        d = digit_f64(hi, dp);
     }
     return d;
 }
@@ -64,8 +64,8 @@ template <int NL>
 __device__ __forceinline__ void write_digits(double a0, double a1, const RingDev &r, const Gadget &g,
                                              const DigitPlan &dp, int16_t *dig, size_t stride) {
     const SMag v = gadget_scale<NL>(a0, a1, r, g);
-    int64_t lo, hi;
-    digit_split(v, dp, lo, hi);
+    double lo, hi;
+    digit_split_f64(v, dp, lo, hi);
     for (int k = 0; k < g.L; k++) dig[(size_t)k * stride] = (int16_t)next_digit(lo, hi, k, dp);
 }
 

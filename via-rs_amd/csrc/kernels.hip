@@ -418,14 +418,14 @@ __global__ __launch_bounds__(TPB) void k_gate(RingDev r, Gadget g, GateArgs ga) 
 #pragma unroll
         for (int e = 0; e < PER; e++) am[l][e] = ab[l][e] = 0.0;
     for (int half = 0; half < 2; half++) {  // 0: ct.mask with -s*m, 1: ct.body with m
-        int64_t vlo[PER], vhi[PER];
+        double vlo[PER], vhi[PER];
 #pragma unroll
         for (int e = 0; e < PER; e++) {
             const int sg = threadIdx.x + e * TPB;
             double b0, c0, b1 = 0.0, c1 = 0.0;
             gate_input<NL>(mode, in, j, half, 0, sg, n, shift, m0, b0, c0);
             if (NL == 2) gate_input<NL>(mode, in, j, half, 1, sg, n, shift, m1, b1, c1);
-            digit_split(gadget_scale<NL>(c0, c1, r, g), ga.dp, vlo[e], vhi[e]);
+            digit_split_f64(gadget_scale<NL>(c0, c1, r, g), ga.dp, vlo[e], vhi[e]);
         }
         const double *key = rgsw + (size_t)half * g.L * 2 * pw;
         for (int k = 0; k < g.L; k++) {
@@ -521,8 +521,8 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
             double b0, c0, b1 = 0.0, c1 = 0.0;
             gate_input<NL>(mode, in, j, half, 0, c, n, shift, m0, b0, c0);
             if (NL == 2) gate_input<NL>(mode, in, j, half, 1, c, n, shift, m1, b1, c1);
-            int64_t lo, hi;
-            digit_split(gadget_scale<NL>(c0, c1, r, g), ga.dp, lo, hi);
+            double lo, hi;
+            digit_split_f64(gadget_scale<NL>(c0, c1, r, g), ga.dp, lo, hi);
             for (int k = 0; k < L; k++) {
                 const double d = (double)next_digit(lo, hi, k, ga.dp);
                 const int b0 = (half * L + k) * NL * S;
@@ -753,8 +753,8 @@ __global__ __launch_bounds__(1024) void k_resp_comp(RespArgs a) {
             SMag v;
             v.mag = (via_u128)((mag * bl + (q3 >> 1)) / q3);
             v.neg = neg && v.mag != 0;
-            int64_t lo, hi;
-            digit_split(v, a.dp, lo, hi);
+            double lo, hi;
+            digit_split_f64(v, a.dp, lo, hi);
             for (int k = 0; k < L; k++) {
                 const int dg = next_digit(lo, hi, k, a.dp);
                 const int pidx = j * L + k;

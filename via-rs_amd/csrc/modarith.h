@@ -309,7 +309,7 @@ VIA_HD int32_t extract_digit(SMag &v, uint32_t B) {
     return d;
 }
 
-// ---- digits in int64 lanes -------------------------------------------------------
+// ---- digits in int64 / fp64 lanes --------------------------------------------------
 //
 // The scaled value |val| <= B^L/2 + 1 is split as val = hi*B^h + lo with
 // B^h < 2^50, so both halves fit exact fp64/int64 arithmetic. Extracting the
@@ -324,6 +324,9 @@ struct DigitPlan {
     uint64_t bh;      // B^h
     double inv_base;  // 1/B
     double inv_bh;    // 1/B^h
+    double base_f;    // B
+    double half_f;    // floor(B/2): the largest digit
+    double neg_f;     // B - 1 - floor(B/2): minus the smallest digit
 };
 
 VIA_HD DigitPlan digit_plan(uint32_t base, int L) {
@@ -339,6 +342,9 @@ VIA_HD DigitPlan digit_plan(uint32_t base, int L) {
     p.bh = bh;
     p.inv_base = 1.0 / (double)base;
     p.inv_bh = 1.0 / (double)bh;
+    p.base_f = (double)base;
+    p.half_f = (double)(base >> 1);
+    p.neg_f = (double)(base - 1 - (base >> 1));
     return p;
 }
 
@@ -361,6 +367,19 @@ VIA_HD int32_t digit_i64(int64_t &x, uint32_t base, double inv_base) {
     }
     x = q;
     return (int32_t)r;
+}
+
+// The same digit of an exact integer |x| < 2^52 held in fp64: the balanced
+// digit r in [-(B-1-floor(B/2)), floor(B/2)] with x = B*q + r is
+// x - B*floor((x + B-1-floor(B/2)) / B); the quotient estimate is within one,
+// r = x - B*q is exact (fma), and one correction each way fixes it. Equal to
+// rem_euclid then the rebalance above B/2 (gadget.rs:168-190).
+VIA_HD int32_t digit_f64(double &x, const DigitPlan &p) {
+    const double q = floor((x + p.neg_f) * p.inv_base);
+    const double r = fma(-q, p.base_f, x);
+    const double corr = r > p.half_f ? 1.0 : (r < -p.neg_f ? -1.0 : 0.0);
+    x = q + corr;
+    return (int32_t)fma(-corr, p.base_f, r);
 }
 
 // Splits the scaled magnitude into signed low/high int64 parts.
@@ -386,4 +405,13 @@ VIA_HD void digit_split(const SMag &v, const DigitPlan &p, int64_t &lo, int64_t 
     const int64_t l = (int64_t)((uint64_t)m - qh * p.bh);
     lo = v.neg ? -l : l;
     hi = v.neg ? -(int64_t)qh : (int64_t)qh;
+}
+
+// digit_split into exact fp64 halves (|lo| < 2^50; |hi| < 2^52 for every
+// gadget the host admits), for digit_f64.
+VIA_HD void digit_split_f64(const SMag &v, const DigitPlan &p, double &lo, double &hi) {
+    int64_t l, h;
+    digit_split(v, p, l, h);
+    lo = (double)l;
+    hi = (double)h;
 }

@@ -175,6 +175,18 @@ bool fits_single(uint64_t q, uint64_t base, uint32_t L) {
     return (u128)(q / 2 + 1) * bl + q < ((u128)1 << 63);
 }
 
+// The high half of the digit split, |hi| <= B^(L-h), stays an exact fp64
+// integer below 2^52 (digit_f64).
+static bool digits_fit_f64(uint64_t base, uint32_t L) {
+    const DigitPlan dp = digit_plan((uint32_t)base, (int)L);
+    u128 b = 1;
+    for (int k = dp.h; k < (int)L; k++) {
+        b *= base;
+        if (b > ((u128)1 << 52)) return false;
+    }
+    return true;
+}
+
 // Gadget at a single modulus q, taking the 192-bit scaling when B^L is wide.
 Gadget make_gadget_q(uint64_t base, uint32_t L, uint64_t q) {
     Gadget g = make_gadget(base, L);
@@ -208,6 +220,9 @@ via_status check_params(const via_params *pp) {
         return VIA_E_INVALID;
     if (pp->l_query < 1 || pp->l_ck < 1 || pp->l_rsk < 1) return VIA_E_INVALID;
     if (pp->ck_base >= (1u << 15)) return VIA_E_INVALID;  // int16 digit planes
+    if (!digits_fit_f64(pp->ck_base, pp->l_ck) || !digits_fit_f64(pp->b1, pp->l_query) ||
+        !digits_fit_f64(pp->b2, pp->l_query) || !digits_fit_f64(pp->b_rsk, pp->l_rsk))
+        return VIA_E_INVALID;
     if ((u128)pp->q2 * pp->q3 >> 63) return VIA_E_INVALID;  // q2 -> q3 switch in 64 bits
     if ((u128)pp->q3 * pp->q4 >> 63) return VIA_E_INVALID;
     return VIA_OK;
