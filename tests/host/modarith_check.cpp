@@ -221,6 +221,85 @@ int main() {
             }
         }
     }
+    // fp64 wide split (gadget_split_w) against the exact scale and digit
+    // split: the cascade gadget 18^18 at Q1 (two limbs) and at q2 (one limb,
+    // the VIA-B repack), 16^19 and 7^26 at Q1; random values, values near
+    // every rounding boundary ((k + 1/2) Q / B^L) and near Q/2, both sides.
+    {
+        struct Case {
+            uint64_t base;
+            int L;
+            int nl;
+        };
+        const Case cases[] = {{18, 18, 2}, {16, 19, 2}, {7, 26, 2}, {18, 18, 1}};
+        const uint64_t q2 = 17175674881ULL;
+        for (const Case &cs : cases) {
+            via_u128 bl = 1;
+            for (int k = 0; k < cs.L; k++) bl *= cs.base;
+            const DigitPlan dp = digit_plan((uint32_t)cs.base, cs.L);
+            const via_u128 Qm = cs.nl == 2 ? Q1 : (via_u128)q2;
+            const via_u128 q1 = cs.nl == 2 ? (via_u128)b.q1 : (via_u128)q2;
+            const WideDiv wq = widediv_make(Qm);
+            ScaleW sw;  // as scale_w_make builds it
+            sw.K = (double)(uint64_t)(bl / q1);
+            sw.r_q1 = (double)(uint64_t)(bl % q1) / (double)(uint64_t)q1;
+            sw.bl_Q = cs.nl == 2 ? u128_to_f64(bl) / u128_to_f64(Q1) : 0.0;
+            sw.bh = (double)dp.bh;
+            sw.inv_bh = dp.inv_bh;
+            long decided = 0, total = 0;
+            auto check = [&](via_u128 c) {
+                c %= Qm;
+                const int neg = c > (Qm >> 1);
+                SMag v;
+                v.mag = round_mul_div(neg ? Qm - c : c, bl, wq);
+                v.neg = neg && v.mag != 0;
+                int64_t lo, hi;
+                digit_split(v, dp, lo, hi);
+                double flo, fhi;
+                const double a0 = (double)(uint64_t)(c % (cs.nl == 2 ? (via_u128)b.q0 : Qm));
+                const double a1 = cs.nl == 2 ? (double)(uint64_t)(c % b.q1) : 0.0;
+                const bool ok = cs.nl == 2 ? gadget_split_w<2>(a0, a1, b, 0.0, sw, flo, fhi)
+                                           : gadget_split_w<1>(a0, 0.0, b, (double)q2, sw, flo, fhi);
+                total++;
+                if (ok) {
+                    decided++;
+                    CHECK(flo == (double)lo && fhi == (double)hi, "wide split B=%llu L=%d nl=%d\n",
+                          (unsigned long long)cs.base, cs.L, cs.nl);
+                }
+                count++;
+            };
+            for (int t = 0; t < 300000; t++) check((((via_u128)rng() << 64) | rng()) % Qm);
+            CHECK(decided > total * 99 / 100, "wide split B=%llu decided only %ld of %ld\n",
+                  (unsigned long long)cs.base, decided, total);
+            const WideDiv w2bl = widediv_make(2 * bl);
+            const long near0 = decided;
+            for (int t = 0; t < 100000; t++) {
+                // c near (k + 1/2) Q / bl: mag * bl / Q has fraction near 1/2
+                const via_u128 kk = (((via_u128)rng() << 64) | rng()) % (bl / 2);
+                const via_u128 num = (2 * kk + 1);
+                // floor(num * Qm / (2 bl)) exactly, through the 192-bit product
+                const via_u128 c0 = div_u192(mul_u128_u128_192(num, Qm), w2bl);
+                for (int d = -3; d <= 3; d++) {
+                    check(c0 + (via_u128)(d + 3) - 3);
+                    check(Qm - (c0 + (via_u128)(d + 3) - 3));
+                }
+            }
+            // B^L/Q is not near an integer here, so integers c land anywhere in
+            // the rounding interval: the band is reached by some of them
+            CHECK(decided - near0 < 1400000, "wide split B=%llu: no near-boundary value declined\n",
+                  (unsigned long long)cs.base);
+            for (int d = -50; d <= 50; d++) check((Qm >> 1) + (via_u128)(d + 50) - 50);
+            for (int d = 0; d < 4; d++) check((via_u128)d);
+            if (cs.nl == 2) {
+                // multiples of q0 (a0 = 0) on both sides of Q/2 and near Q
+                for (int d = 0; d < 6; d++) {
+                    check((via_u128)b.q0 * (b.q1 / 2 - 2 + d));
+                    check((via_u128)b.q0 * (b.q1 - 1 - d));
+                }
+                for (int t = 0; t < 20000; t++) check((via_u128)b.q0 * (rng() % b.q1));
+            }
+        }
+    }
     if (fails) {
         printf("FAIL %d\n", fails);
         return 1;

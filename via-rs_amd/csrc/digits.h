@@ -63,9 +63,9 @@ __device__ __forceinline__ int32_t next_digit(double &lo, double &hi, int k, con
 template <int NL>
 __device__ __forceinline__ void write_digits(double a0, double a1, const RingDev &r, const Gadget &g,
                                              const DigitPlan &dp, int16_t *dig, size_t stride) {
-    const SMag v = gadget_scale<NL>(a0, a1, r, g);
     double lo, hi;
-    digit_split_f64(v, dp, lo, hi);
+    if (!(g.f64 == 3 && gadget_split_w<NL>(a0, a1, r.rns, (double)r.q_single, g.sw, lo, hi)))
+        digit_split_f64(gadget_scale<NL>(a0, a1, r, g), dp, lo, hi);
     for (int k = 0; k < g.L; k++) dig[(size_t)k * stride] = (int16_t)next_digit(lo, hi, k, dp);
 }
 

@@ -52,7 +52,13 @@ struct Gadget {
     // with the exact path as fallback (modarith.h)
     int f64;
     ScaleF64 sf;
+    ScaleW sw;  // f64 == 3: the wide split of gadget_split_w
 };
+
+// The wide fp64 split of a gadget over the ring r (two limbs, or one): true
+// and *sw filled when its bounds hold (K, q1 < 2^39 for two limbs; q < 2^39
+// and K < 2^53 for one; B^(L-h) < 2^40).
+bool scale_w_make(const RingDev &r, const Gadget &g, const DigitPlan &dp, ScaleW *sw);
 
 // ---- launchers (ntt.hip / kernels.hip) ----------------------------------------
 

@@ -26,8 +26,30 @@ __global__ __launch_bounds__(256) void k_poly_digits(RingDev r1, Gadget g, Digit
     write_digits<NL>(a0, a1, r1, g, dp, dig + (size_t)item * g.L * n + c, (size_t)n);
 }
 
-void launch_poly_digits(const RingDev &r1, const Gadget &g, const DigitPlan &dp, const double *src, size_t item_stride,
-                        size_t limb_stride, int n, int items, int16_t *dig, hipStream_t st) {
+bool scale_w_make(const RingDev &r, const Gadget &g, const DigitPlan &dp, ScaleW *sw) {
+    const via_u128 two39 = (via_u128)1 << 39;
+    const via_u128 q1 = r.nl == 2 ? (via_u128)r.rns.q1 : (via_u128)r.q_single;
+    const via_u128 K = g.bl / q1, R = g.bl % q1;
+    if (q1 >= two39) return false;
+    if (r.nl == 2 ? K >= two39 : K >= ((via_u128)1 << 53)) return false;
+    via_u128 bhi = 1;  // B^(L-h) bounds the high half
+    for (int k = dp.h; k < g.L; k++) {
+        bhi *= g.base;
+        if (bhi >= ((via_u128)1 << 40)) return false;
+    }
+    sw->K = (double)(uint64_t)K;
+    sw->r_q1 = (double)(uint64_t)R / (double)(uint64_t)q1;
+    sw->bl_Q = r.nl == 2 ? u128_to_f64(g.bl) / u128_to_f64(r.rns.big_q) : 0.0;
+    sw->bh = (double)dp.bh;
+    sw->inv_bh = dp.inv_bh;
+    return true;
+}
+
+void launch_poly_digits(const RingDev &r1, const Gadget &g_in, const DigitPlan &dp, const double *src,
+                        size_t item_stride, size_t limb_stride, int n, int items, int16_t *dig, hipStream_t st) {
+    // wide gadgets (the base-18 depth-18 cascade gadget) take the fp64 split
+    Gadget g = g_in;
+    if (g.f64 == 0 && g.bl >= ((via_u128)1 << 40) && scale_w_make(r1, g, dp, &g.sw)) g.f64 = 3;
     dim3 grid((n + 255) / 256, items);
     if (r1.nl == 2)
         hipLaunchKernelGGL(k_poly_digits<2>, grid, dim3(256), 0, st, r1, g, dp, src, item_stride, limb_stride, n, dig);

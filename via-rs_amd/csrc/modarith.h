@@ -268,6 +268,70 @@ VIA_HD bool gadget_scale2_f64(double a0, double a1, const Rns2 &b, const ScaleF6
     return true;
 }
 
+// Wide B^L (beyond 2^40, e.g. 18^18 ~ 2^75), split for the digits directly.
+// With B^L = K*q1 + R (0 <= R < q1) and the centred magnitude
+// mag = a0' + q0*t' (two limbs, a0' < q0; one limb: mag = t', a0' = 0):
+//   mag * B^L / Q = t'*K + F,  F = t' * (R/q1) + a0' * (B^L/Q) < 2^41,
+// so the scale floor((mag * B^L + floor(Q/2)) / Q) is t'*K + floor(F + 1/2 -
+// 1/(2Q)). With K, q1 < 2^39 (F < 2^40) F is computed in fp64 within 2^-11
+// and its rounding taken when it lies more than 2^-9 from a boundary (else
+// the exact path). t'*K < 2^91 is the exact pair ph + pl; the split
+// val = hi*B^h + lo takes one quotient estimate of hi (within one while
+// hi < 2^40) and the fma-exact remainder. The host enables it only within
+// these bounds (scale_w_make).
+struct ScaleW {
+    double K;       // floor(B^L / q1)
+    double r_q1;    // R / q1
+    double bl_Q;    // B^L / Q (two limbs)
+    double bh;      // B^h of the digit split
+    double inv_bh;  // 1 / B^h
+};
+
+// Two limbs (a0 mod q0, a1 mod q1, basis b), or one limb a0 mod q (NL = 1,
+// q1 = q). Returns the signed halves of the digit split of the scale.
+template <int NL>
+VIA_HD bool gadget_split_w(double a0, double a1, const Rns2 &b, double q_single, const ScaleW &k, double &lo,
+                           double &hi) {
+    double tp, F;
+    int neg;
+    if (NL == 2) {
+        const double q0 = b.m0.q, q1 = b.m1.q;
+        const double a0m = mod_reduce_small(a0, b.m1);
+        const double diff = mod_sub(a1, a0m, b.m1);
+        const double t = mod_mul(diff, (double)b.q0_inv_mod_q1, b.m1);  // c = a0 + q0*t
+        const double c1 = 0.5 * (q1 - 1.0), c0 = 0.5 * (q0 - 1.0);
+        neg = t > c1 || (t == c1 && a0 > c0);  // c > floor(Q/2)
+        // Q - c = (q0 - a0) + q0*(q1 - 1 - t), or q0*(q1 - t) when a0 = 0
+        const double a0p = neg ? (a0 == 0.0 ? 0.0 : q0 - a0) : a0;
+        tp = neg ? (a0 == 0.0 ? q1 - t : q1 - 1.0 - t) : t;
+        F = fma(tp, k.r_q1, a0p * k.bl_Q);
+    } else {
+        const double q = q_single;
+        neg = a0 > 0.5 * (q - 1.0);
+        tp = neg ? q - a0 : a0;
+        F = tp * k.r_q1;
+    }
+    const double G = F + 0.5;
+    const double fl = floor(G);
+    const double fr = G - fl;
+    if (fr < 0.001953125 || fr > 1.0 - 0.001953125) return false;  // 2^-9
+    const double ph = tp * k.K;
+    const double pl = fma(tp, k.K, -ph);
+    const double s = pl + fl;  // exact: |pl| < 2^38, fl < 2^41
+    double h = floor((ph + s) * k.inv_bh);
+    double l = fma(-h, k.bh, ph) + s;  // exact: |ph - h*B^h| < 2^52
+    if (l < 0.0) {
+        l += k.bh;
+        h -= 1.0;
+    } else if (l >= k.bh) {
+        l -= k.bh;
+        h += 1.0;
+    }
+    lo = neg ? -l : l;
+    hi = neg ? -h : h;
+    return true;
+}
+
 // One LSB digit of base B (B < 2^20), returning the digit and advancing val
 // to (val - digit) / B, with the Euclidean remainder and the "d > B/2 => d -= B"
 // rebalance of the reference.

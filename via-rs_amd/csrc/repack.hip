@@ -64,8 +64,16 @@ __global__ __launch_bounds__(256) void k_repack_extract(RingDev r, Gadget g, Dig
     }
 }
 
-void launch_repack_extract(const RingDev &r, const Gadget &g, const DigitPlan &dp, const double *rot, int T, int G,
+// Wide gadgets take the fp64 split of gadget_split_w where its bounds hold.
+static Gadget with_wide_split(const RingDev &r, const Gadget &g_in, const DigitPlan &dp) {
+    Gadget g = g_in;
+    if (g.f64 == 0 && g.bl >= ((via_u128)1 << 40) && scale_w_make(r, g, dp, &g.sw)) g.f64 = 3;
+    return g;
+}
+
+void launch_repack_extract(const RingDev &r, const Gadget &g_in, const DigitPlan &dp, const double *rot, int T, int G,
                            int pairs, double *body0, size_t body_stride, int16_t *dig, hipStream_t st) {
+    const Gadget g = with_wide_split(r, g_in, dp);
     const int lt = __builtin_ctz((unsigned)T);
     dim3 grid(pairs, (2 * r.n + 255) / 256);
     hipLaunchKernelGGL(k_repack_extract, grid, dim3(256), 0, st, r, g, dp, rot, T, lt, G, body0, body_stride, dig);
@@ -139,8 +147,9 @@ __global__ __launch_bounds__(kThreads) void k_repack_combine(RingDev r, Gadget g
     }
 }
 
-void launch_repack_combine(const RingDev &r, const Gadget &g, const DigitPlan &dp, const RepackCombineArgs &a,
+void launch_repack_combine(const RingDev &r, const Gadget &g_in, const DigitPlan &dp, const RepackCombineArgs &a,
                            hipStream_t st) {
+    const Gadget g = with_wide_split(r, g_in, dp);
     int chunks = kSlots / r.n;
     if (chunks > 8) chunks = 8;
     dim3 grid(a.items, chunks);

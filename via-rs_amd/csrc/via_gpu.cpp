@@ -166,6 +166,7 @@ Gadget make_gadget(uint64_t base, uint32_t L) {
     g.wide = 0;
     g.f64 = 0;
     g.sf = ScaleF64();
+    g.sw = ScaleW();
     return g;
 }
 
