@@ -519,13 +519,20 @@ __global__ __launch_bounds__(TPB) void k_gate(RingDev r, Gadget g, GateArgs ga) 
 // inverse transforms (mask and body parts) concurrently; phase 5 combines.
 // The critical path is one forward and one inverse transform instead of
 // 2L + 2 in sequence, which is what the narrow tree levels wait on.
-template <int NL>
+// SPLIT (two limbs): the limbs of a gate are independent once the digits are
+// known, so workgroup 2g + l computes gate g's digits from both limbs and
+// transforms, multiplies and writes limb l only: half the work and latency
+// per workgroup, twice the workgroups on the narrow levels.
+template <int NL, int SPLIT>
 __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateArgs ga) {
     constexpr int S = kSlots;
     constexpr int TPB = 1024;
-    __shared__ double s[lds_words(4 * NL * S)];
-    const int q = blockIdx.x / ga.items_per_query;
-    const int j = blockIdx.x - q * ga.items_per_query;
+    constexpr int NLC = SPLIT ? 1 : NL;  // limbs this workgroup computes
+    __shared__ double s[lds_words(4 * NLC * S)];
+    const int gate = SPLIT ? (int)(blockIdx.x >> 1) : (int)blockIdx.x;
+    const int lsel = SPLIT ? (int)(blockIdx.x & 1) : 0;
+    const int q = gate / ga.items_per_query;
+    const int j = gate - q * ga.items_per_query;
     const int mode = ga.mode, shift = ga.shift;
     const double *rgsw = ga.rgsw + (size_t)q * ga.rgsw_qstride;
     const double *in = ga.in + (size_t)q * ga.in_qstride;
@@ -533,6 +540,8 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
     const int n = r.n;
     const size_t pw = (size_t)NL * n;
     const FMod m0 = r.t[0].m, m1 = r.t[1].m;
+    RingDev rl = r;  // the transforms of the computed limb(s)
+    if (SPLIT && lsel) rl.t[0] = r.t[1];
     const int L = g.L;
     const int tid = threadIdx.x;
     // phase 1: digits of ct's two halves (0: mask, with -s*m; 1: body, with m)
@@ -547,9 +556,9 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
             digit_split_f64(gadget_scale<NL>(c0, c1, r, g), ga.dp, lo, hi);
             for (int k = 0; k < L; k++) {
                 const double d = (double)next_digit(lo, hi, k, ga.dp);
-                const int b0 = (half * L + k) * NL * S;
+                const int b0 = (half * L + k) * NLC * S;
                 s[lds_pad(b0 + c)] = d;
-                if (NL == 2) s[lds_pad(b0 + S + c)] = d;
+                if (NLC == 2) s[lds_pad(b0 + S + c)] = d;
             }
         }
     }
@@ -559,21 +568,21 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
         const int np = 2 * L;
         const int per = TPB / np;
         const int gi = tid / per;
-        ntt_fwd<NL>(s + lds_words(gi * NL * S), S, n, r.logn, limbs_fwd(r), tid - gi * per, per);
+        ntt_fwd<NLC>(s + lds_words(gi * NLC * S), S, n, r.logn, limbs_fwd(rl), tid - gi * per, per);
     }
-    // phase 3: external product sums (both parts) for NL * S slots per part
-    constexpr int PER = NL * S / TPB;
+    // phase 3: external product sums (both parts) for NLC * S slots per part
+    constexpr int PER = NLC * S / TPB;
     double am[PER], ab[PER];
 #pragma unroll
     for (int e = 0; e < PER; e++) {
         const int t = tid + e * TPB;  // limb-major slot
-        const int l = NL == 2 && t >= S ? 1 : 0;
-        const int c = t - l * S;
+        const int l = SPLIT ? lsel : (NL == 2 && t >= S ? 1 : 0);
+        const int c = SPLIT ? t : t - l * S;
         const FMod md = l ? m1 : m0;
         double sm = 0.0, sb = 0.0;
         for (int half = 0; half < 2; half++)
             for (int k = 0; k < L; k++) {
-                const double x = s[lds_pad((half * L + k) * NL * S + t)];
+                const double x = s[lds_pad((half * L + k) * NLC * S + t)];
                 const double *key = rgsw + ((size_t)half * L + (L - 1 - k)) * 2 * pw + (size_t)l * n + c;
                 sm += mod_mul_lazy(x, key[0], md);
                 sb += mod_mul_lazy(x, key[pw], md);
@@ -586,13 +595,13 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
     for (int e = 0; e < PER; e++) {
         const int t = tid + e * TPB;
         s[lds_pad(t)] = am[e];
-        s[lds_pad(NL * S + t)] = ab[e];
+        s[lds_pad(NLC * S + t)] = ab[e];
     }
     __syncthreads();
     // phase 4: inverse transforms of the two parts side by side
     {
         const int part = tid >> 9;
-        ntt_inv_noscale<NL>(s + lds_words(part * NL * S), S, n, r.logn, limbs_inv(r), tid & 511, 512);
+        ntt_inv_noscale<NLC>(s + lds_words(part * NLC * S), S, n, r.logn, limbs_inv(rl), tid & 511, 512);
     }
     // phase 5: gate combine
 #pragma unroll
@@ -600,11 +609,11 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
 #pragma unroll
         for (int e = 0; e < PER; e++) {
             const int t = tid + e * TPB;
-            const int l = NL == 2 && t >= S ? 1 : 0;
-            const int c = t - l * S;
+            const int l = SPLIT ? lsel : (NL == 2 && t >= S ? 1 : 0);
+            const int c = SPLIT ? t : t - l * S;
             const FMod md = l ? m1 : m0;
             const double ni = l ? r.t[1].n_inv : r.t[0].n_inv;
-            const double p = mod_mul(s[lds_pad(part * NL * S + t)], ni, md);
+            const double p = mod_mul(s[lds_pad(part * NLC * S + t)], ni, md);
             double bv, cv;
             gate_input<NL>(mode, in, j, part, l, c, n, shift, md, bv, cv);
             const size_t o = (size_t)part * pw + (size_t)l * n + c;
@@ -632,10 +641,15 @@ void launch_gate(const RingDev &r, const Gadget &g_in, const GateArgs &a_in, hip
     }
     const int blocks = a.items_per_query * a.queries;
     if (g.L <= 2 && r.n == kSlots) {
-        if (r.nl == 2)
-            hipLaunchKernelGGL(k_gate_fused<2>, dim3(blocks), dim3(1024), 0, st, r, g, a);
+        // two limbs: one workgroup per limb of each gate on levels narrower
+        // than the CU count (latency-bound); wider levels keep both limbs in
+        // one workgroup (the digits are extracted once)
+        if (r.nl == 2 && blocks < 256)
+            hipLaunchKernelGGL((k_gate_fused<2, 1>), dim3(2 * blocks), dim3(1024), 0, st, r, g, a);
+        else if (r.nl == 2)
+            hipLaunchKernelGGL((k_gate_fused<2, 0>), dim3(blocks), dim3(1024), 0, st, r, g, a);
         else
-            hipLaunchKernelGGL(k_gate_fused<1>, dim3(blocks), dim3(1024), 0, st, r, g, a);
+            hipLaunchKernelGGL((k_gate_fused<1, 0>), dim3(blocks), dim3(1024), 0, st, r, g, a);
         return;
     }
     const bool wide = blocks < 64;  // narrow levels: shorter per-workgroup latency
