@@ -287,7 +287,9 @@ def vb_run(via_gpu, args):
         "data": "synthetic: uniform random records, uniform key and query residues",
         "config": {"workload": f"VIA-B, {nb * T} indices per step as {nb} batch(es) of T={T}, record degree "
                                f"{pp.n_rec} ({pp.n_rec // 2} B records), I={I} J={J}, prepared DB "
-                               f"{I * J * n * 8 / 2**30:.2f} GiB", "num_rows": I, "num_cols": J, "T": T},
+                               f"{I * J * n * 5 / 2**30:.2f} GiB as 7-bit limb fragments "
+                               f"({I * J * n * 8 / 2**30:.2f} GiB as u64 residues)",
+                   "num_rows": I, "num_cols": J, "T": T},
         "setup_s": round(t_setup, 3),
     }
     print(json.dumps(out))
