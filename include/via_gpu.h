@@ -140,12 +140,13 @@ via_status via_answer_many_device(via_ctx *ctx, const via_db *db, uint32_t nq, c
  * queries with each candidate launch shape of every query-decompression step
  * (the number of digit groups, i.e. workgroups splitting the 18 digits of one
  * gadget product, and the kernel variant: 256 threads with one digit
- * transform at a time in radix-8 or radix-16 passes, or 1024 threads with
- * several side by side), reps timed
+ * transform at a time in radix-8 or radix-16 passes, 512 threads in radix-8
+ * passes, or 1024 threads with several side by side), reps timed
  * runs per candidate, and keeps the fastest per step for batches of nq
  * queries. groups_out and variant_out (12 entries each: the eleven
  * cascade steps and the RLWE -> RGSW conversion; variant 0 = 256 threads
- * radix 8, 2 = 256 threads radix 16, 1 = 1024 threads) receive the choice
+ * radix 8, 2 = 256 threads radix 16, 3 = 512 threads radix 8, 1 = 1024
+ * threads) receive the choice
  * when not null. The choice serves every context of the process on the same device
  * (the serving loop's included); an untuned batch size takes the shapes of
  * the largest tuned size below it. The answers written meanwhile are those
@@ -153,6 +154,11 @@ via_status via_answer_many_device(via_ctx *ctx, const via_db *db, uint32_t nq, c
 via_status via_ctx_autotune(via_ctx *ctx, const via_db *db, uint32_t nq, const via_keys *const *keys,
                             const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_masks,
                             uint64_t *const *d_bodies, int reps, int *groups_out, int *variant_out);
+/* No counterpart in the reference: stores launch shapes for batches of nq
+ * queries as via_ctx_autotune would (12 entries each, the same encoding as
+ * its outputs), e.g. a choice saved from an earlier tuning run; groups null
+ * erases the entry. VIA_E_INVALID for a shape the tuner never offers. */
+via_status via_ctx_set_tuning(via_ctx *ctx, uint32_t nq, const int *groups, const int *variants);
 
 /* Server::answer_batch::<T> (answer.rs:561-602) -> answer_batch
  * (batch.rs:72-163), VIA-B: answer_through_crot for each of the T queries of

@@ -45,6 +45,7 @@ struct StepTune {
     int groups = 1;
     int wide = 0;
     int radix16 = 0;
+    int t512 = 0;  // the 512-thread kernel (radix-8 passes)
 };
 
 // Buffers of the VIA-B repack, sized for batches of up to T queries.
@@ -120,6 +121,7 @@ namespace via {
 // process on the same device (the serving loop's slots included): per
 // device and batch size, the twelve decompression steps. A batch size that
 // was not tuned takes the largest tuned size below it.
+void tuned_erase(int device, int Q);
 void tuned_store(int device, int Q, const std::vector<StepTune> &steps);
 bool tuned_lookup(int device, int Q, std::vector<StepTune> *steps);
 // Incremented by every tuned_store: captured graphs bake in launch shapes.

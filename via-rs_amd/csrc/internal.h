@@ -109,6 +109,7 @@ struct MacArgs {
     int items;
     int wide;                // 1: digit transforms of a group side by side (k_gadget_mac_wide)
     int radix16;             // 256-thread kernel: radix-16 transform passes (else radix 8)
+    int t512;                // 512-thread kernel with radix-8 passes
 };
 void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t s);
 

@@ -66,8 +66,9 @@ void launch_lwe_digits(const RingDev &r1, const Gadget &g, const DigitPlan &dp, 
 // gadget_product, rlev.rs:259-283, without the inverse): workgroup (item,
 // group, tile) adds NTT(digit_k) (.) key[L-1-k] for k in its group. Inputs
 // that are embedded (x -> x^2) transform n/2 coefficients.
-template <int NL, int MAXR>
-__global__ __launch_bounds__(kThreads) void k_gadget_mac(RingDev r, Gadget g, MacArgs a) {
+template <int NL, int MAXR, int TPB>
+__global__ __launch_bounds__(TPB) void k_gadget_mac(RingDev r, Gadget g, MacArgs a) {
+    constexpr int PER = kSlots / TPB;  // slots per thread
     __shared__ double s[lds_words(NL * kSlots)];
     const MacTask task = mac_task(blockIdx.x, a.items, a.items_per_key, a.groups, a.tiles);
     const int item = task.item, grp = task.grp, tile = task.tile;
@@ -78,11 +79,11 @@ __global__ __launch_bounds__(kThreads) void k_gadget_mac(RingDev r, Gadget g, Ma
     const NttLimbs T = limbs_fwd(r);
     const FMod m0 = r.t[0].m, m1 = r.t[1].m;
     const int k0 = grp * g.L / a.groups, k1 = (grp + 1) * g.L / a.groups;
-    double am[NL][kPer], ab[NL][kPer];
+    double am[NL][PER], ab[NL][PER];
 #pragma unroll
     for (int l = 0; l < NL; l++)
 #pragma unroll
-        for (int e = 0; e < kPer; e++) am[l][e] = ab[l][e] = 0.0;
+        for (int e = 0; e < PER; e++) am[l][e] = ab[l][e] = 0.0;
     const int16_t *dig = a.dig + (size_t)item * g.L * a.dig_coeffs + (size_t)tile * cin;
     const double *key = a.key[item / a.items_per_key] + (size_t)tile * S;
     for (int k = k0; k < k1; k++) {
@@ -102,8 +103,8 @@ __global__ __launch_bounds__(kThreads) void k_gadget_mac(RingDev r, Gadget g, Ma
             const double *km = key + ((size_t)(ks * 2 + 0) * NL + l) * a.key_slots;
             const double *kb = key + ((size_t)(ks * 2 + 1) * NL + l) * a.key_slots;
 #pragma unroll
-            for (int e = 0; e < kPer; e++) {
-                const int sg = threadIdx.x + e * kThreads;
+            for (int e = 0; e < PER; e++) {
+                const int sg = threadIdx.x + e * TPB;
                 const double x = s[lds_pad(l * cin + (a.embed ? (sg >> 1) : sg))];
                 am[l][e] += mod_mul_lazy(x, km[sg], md);
                 ab[l][e] += mod_mul_lazy(x, kb[sg], md);
@@ -115,8 +116,8 @@ __global__ __launch_bounds__(kThreads) void k_gadget_mac(RingDev r, Gadget g, Ma
 #pragma unroll
     for (int l = 0; l < NL; l++)
 #pragma unroll
-        for (int e = 0; e < kPer; e++) {
-            const int sg = threadIdx.x + e * kThreads;
+        for (int e = 0; e < PER; e++) {
+            const int sg = threadIdx.x + e * TPB;
             const FMod md = l ? m1 : m0;
             am[l][e] = mod_reduce_small(am[l][e], md);
             ab[l][e] = mod_reduce_small(ab[l][e], md);
@@ -240,17 +241,23 @@ void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipS
             hipLaunchKernelGGL((k_gadget_mac_wide<1, 8>), grid, dim3(1024), 0, st, r, g, a);
         return;
     }
-    // radix-16 passes (a.radix16) trade registers for fewer LDS round trips
+    // radix-16 passes (a.radix16) trade registers for fewer LDS round trips;
+    // 512 threads (a.t512) hold half the accumulators per thread, so more
+    // wavefronts fit each SIMD
     if (r.nl == 2) {
-        if (a.radix16)
-            hipLaunchKernelGGL((k_gadget_mac<2, 4>), grid, dim3(kThreads), 0, st, r, g, a);
+        if (a.t512)
+            hipLaunchKernelGGL((k_gadget_mac<2, 3, 512>), grid, dim3(512), 0, st, r, g, a);
+        else if (a.radix16)
+            hipLaunchKernelGGL((k_gadget_mac<2, 4, kThreads>), grid, dim3(kThreads), 0, st, r, g, a);
         else
-            hipLaunchKernelGGL((k_gadget_mac<2, 3>), grid, dim3(kThreads), 0, st, r, g, a);
+            hipLaunchKernelGGL((k_gadget_mac<2, 3, kThreads>), grid, dim3(kThreads), 0, st, r, g, a);
     } else {
-        if (a.radix16)
-            hipLaunchKernelGGL((k_gadget_mac<1, 4>), grid, dim3(kThreads), 0, st, r, g, a);
+        if (a.t512)
+            hipLaunchKernelGGL((k_gadget_mac<1, 3, 512>), grid, dim3(512), 0, st, r, g, a);
+        else if (a.radix16)
+            hipLaunchKernelGGL((k_gadget_mac<1, 4, kThreads>), grid, dim3(kThreads), 0, st, r, g, a);
         else
-            hipLaunchKernelGGL((k_gadget_mac<1, 3>), grid, dim3(kThreads), 0, st, r, g, a);
+            hipLaunchKernelGGL((k_gadget_mac<1, 3, kThreads>), grid, dim3(kThreads), 0, st, r, g, a);
     }
 }
 

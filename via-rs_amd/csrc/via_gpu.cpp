@@ -242,13 +242,13 @@ void free_ws(Workspace &w) {
 }
 
 // Launch shapes the autotuner tries for `items` gadget products of depth L:
-// the 256-thread kernel with radix-8 or radix-16 transform passes and 1, 2,
-// 3, 4, 6, 9 or 18 digit groups, and the wide kernel with 1, 2, 3 or 5
-// (group counts up to L), as long as a launch stays within 8192 workgroups
-// (two tiles per item).
+// the 256-thread kernel with radix-8 or radix-16 transform passes and the
+// 512-thread kernel with radix-8 passes, each with 1, 2, 3, 4, 6, 9 or 18
+// digit groups, and the wide kernel with 1, 2, 3 or 5 (group counts up to
+// L), as long as a launch stays within 8192 workgroups (two tiles per item).
 std::vector<StepTune> tune_candidates(int items, int L) {
     std::vector<StepTune> out;
-    for (int variant = 0; variant < 3; variant++)
+    for (int variant = 0; variant < 4; variant++)
         for (int g : variant == 2 ? std::initializer_list<int>{1, 2, 3, 5}
                                   : std::initializer_list<int>{1, 2, 3, 4, 6, 9, 18})
             if (g <= L && (long)items * 2 * g <= 8192) {
@@ -256,6 +256,7 @@ std::vector<StepTune> tune_candidates(int items, int L) {
                 t.groups = g;
                 t.wide = variant == 2;
                 t.radix16 = variant == 1;
+                t.t512 = variant == 3;
                 out.push_back(t);
             }
     if (out.empty()) out.push_back(StepTune());
@@ -765,6 +766,12 @@ void via::tuned_store(int device, int Q, const std::vector<StepTune> &steps) {
     g_tuned_gen++;
 }
 
+void via::tuned_erase(int device, int Q) {
+    std::lock_guard<std::mutex> g(g_tuned_mu);
+    g_tuned.erase({device, Q});
+    g_tuned_gen++;
+}
+
 uint64_t via::next_object_id() {
     static std::atomic<uint64_t> next{1};
     return next++;
@@ -893,6 +900,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         a.groups = step_tune[step].groups;
         a.wide = step_tune[step].wide;
         a.radix16 = step_tune[step].radix16;
+        a.t512 = step_tune[step].t512;
         a.out_mask = w.pm;
         a.out_body = w.pb;
         a.reduce_body = 1;
@@ -929,6 +937,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         a.groups = step_tune[kDecompSteps - 1].groups;
         a.wide = step_tune[kDecompSteps - 1].wide;
         a.radix16 = step_tune[kDecompSteps - 1].radix16;
+        a.t512 = step_tune[kDecompSteps - 1].t512;
         a.out_mask = w.pm;
         a.out_body = w.pb;
         a.reduce_body = 0;
@@ -1327,6 +1336,30 @@ via_status via_answer_many_device(via_ctx *c, const via_db *db, uint32_t nq, con
 // (tune_candidates) for every decompression step, `reps` timed runs each
 // after one untimed run, timing each step with HIP events on the context
 // stream, and keeps the fastest shape per step for this batch size.
+// Stores (or, with groups null, erases) the launch shapes of batches of nq
+// queries, as via_ctx_autotune would: a shape the tuner could choose.
+via_status via_ctx_set_tuning(via_ctx *c, uint32_t nq, const int *groups, const int *variants) {
+    if (!c || nq < 1 || nq > (uint32_t)kMaxQ) return VIA_E_INVALID;
+    if (!groups) {
+        tuned_erase(c->device, (int)nq);
+        return VIA_OK;
+    }
+    if (!variants) return VIA_E_INVALID;
+    std::vector<StepTune> steps(kDecompSteps);
+    for (int i = 0; i < kDecompSteps; i++) {
+        const int v = variants[i], gr = groups[i];
+        if (v < 0 || v > 3 || gr < 1 || gr > 18) return VIA_E_INVALID;
+        if (v == 1 && gr != 1 && gr != 2 && gr != 3 && gr != 5) return VIA_E_INVALID;
+        if (v != 1 && gr != 1 && gr != 2 && gr != 3 && gr != 4 && gr != 6 && gr != 9 && gr != 18) return VIA_E_INVALID;
+        steps[i].groups = gr;
+        steps[i].wide = v == 1;
+        steps[i].radix16 = v == 2;
+        steps[i].t512 = v == 3;
+    }
+    tuned_store(c->device, (int)nq, steps);
+    return VIA_OK;
+}
+
 via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via_keys *const *keys,
                             const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_mask,
                             uint64_t *const *d_body, int reps, int *groups_out, int *variant_out) {
@@ -1372,7 +1405,7 @@ via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via
             if (t[ci][i] < t[bi][i]) bi = ci;
         best[i] = cands[bi];
         if (groups_out) groups_out[i] = best[i].groups;
-        if (variant_out) variant_out[i] = best[i].wide ? 1 : (best[i].radix16 ? 2 : 0);
+        if (variant_out) variant_out[i] = best[i].wide ? 1 : (best[i].radix16 ? 2 : (best[i].t512 ? 3 : 0));
     }
     tuned_store(c->device, Q, best);
     return VIA_OK;

@@ -111,6 +111,7 @@ def lib():
             "via_ctx_first_dim_time": (C.c_int, [vp, C.POINTER(C.c_double)]),
             "via_ctx_mark": (C.c_int, [vp, C.c_int]),
             "via_ctx_set_graphs": (C.c_int, [vp, C.c_int]),
+            "via_ctx_set_tuning": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "via_ctx_autotune": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.c_size_t,
                                            C.POINTER(vp), C.POINTER(vp), C.c_int, C.POINTER(C.c_int),
                                            C.POINTER(C.c_int)]),
@@ -216,11 +217,21 @@ class Context:
         _check(lib().via_ctx_first_dim_time(self.h, C.byref(t)))
         return t.value
 
+    def set_tuning(self, nq, shapes):
+        """via_ctx_set_tuning: pin the twelve (groups, variant) launch shapes
+        of batches of nq queries; shapes None erases them."""
+        if shapes is None:
+            _check(lib().via_ctx_set_tuning(self.h, nq, None, None))
+            return
+        g = (C.c_int * 12)(*[x[0] for x in shapes])
+        v = (C.c_int * 12)(*[x[1] for x in shapes])
+        _check(lib().via_ctx_set_tuning(self.h, nq, g, v))
+
     def autotune(self, db, keys, d_lwes, n_lwe, d_masks, d_bodies, reps=2):
         """via_ctx_autotune: per-step launch shapes for batches of len(keys)
         queries (device buffers); returns the twelve (groups, variant)
         choices (variant 0: 256 threads radix 8, 1: 1024 threads, 2: 256
-        threads radix 16)."""
+        threads radix 16, 3: 512 threads radix 8)."""
         nq = len(keys)
         vp = C.c_void_p
         K = (vp * nq)(*[k.h.value for k in keys])
