@@ -39,7 +39,10 @@ typedef enum {
     VIA_E_NOMEM = 4,       /* HBM admission refused / allocation failed */
     VIA_E_INVALID = 5,     /* parameters outside what the device path supports */
     VIA_E_DEVICE = 6,      /* HIP runtime error */
-    VIA_E_BATCH_LEN = 7    /* answer_batch: wrong number of queries (batch.rs:124-128) */
+    VIA_E_BATCH_LEN = 7,   /* answer_batch: wrong number of queries (batch.rs:124-128) */
+    VIA_E_BUFFER_SHORT = 8, /* ViaError::BufferTooShort { needed, got } (err: expected, got) */
+    VIA_E_FORMAT_TAG = 9,  /* ViaError::UnknownFormatTag(tag) (err: got = tag) */
+    VIA_E_PRG_SEED = 10    /* ViaError::InvalidPrgSeed: a seed longer than 64 bytes */
 } via_status;
 
 /* PIRParams (via-protocol/src/params.rs) restricted to the server path, plus
@@ -204,6 +207,39 @@ via_status via_ntt(via_ctx *ctx, uint32_t n, const uint64_t *moduli, uint32_t nm
  * the J output ciphertexts, both coefficient form at q2, host buffers. The
  * database must belong to ctx. */
 via_status via_first_dim(via_ctx *ctx, const via_db *db, uint32_t nq, const uint64_t *switched, uint64_t *out);
+
+/* ---- wire formats (via-protocol wire/format.rs) ----------------------------- */
+
+/* Uncompressed (format.rs:93-132): [0x00][count u64 LE][count coefficients
+ * u64 LE]. via_wire_uncompressed_size gives the packet length; encode
+ * writes it into out (VIA_E_INVALID when out_len is smaller). decode
+ * (WireFormat::deserialize) returns VIA_E_BUFFER_SHORT {needed, got} and
+ * VIA_E_FORMAT_TAG {tag} as the reference's ViaError does; with out null it
+ * only reports *count. An answer travels as the Uncompressed mask followed by
+ * the Uncompressed body. */
+size_t via_wire_uncompressed_size(size_t count);
+via_status via_wire_encode_uncompressed(const uint64_t *coeffs, size_t count, uint8_t *out, size_t out_len);
+via_status via_wire_decode_uncompressed(const uint8_t *buf, size_t len, uint64_t *out, size_t out_cap,
+                                        size_t *count, via_err_detail *err);
+/* PrgCompressed<32>::deserialize (format.rs:163-190): [0x01][seed, 32
+ * bytes][count u64 LE][count body coefficients]; the seed goes to seed_out
+ * (32 bytes) when not null, the body coefficients as in the Uncompressed
+ * decode. */
+via_status via_wire_decode_prg(const uint8_t *buf, size_t len, uint8_t *seed_out, uint64_t *out, size_t out_cap,
+                               size_t *count, via_err_detail *err);
+/* PrgCompressed::regenerate_masks (format.rs:143-160) on the GPU: count draws
+ * Shake256Prg(seed).uniform_below(q) (prg.rs) into out (host buffer).
+ * VIA_E_PRG_SEED for a seed longer than 64 bytes (the reference asserts). */
+via_status via_prg_regenerate(via_ctx *ctx, const uint8_t *seed, size_t seed_len, uint64_t q, size_t count,
+                              uint64_t *out);
+/* A query whose LWE masks travel as a PRG seed (the regenerate_masks
+ * contract: masks row-major, one draw per q1 limb, limb 0 first): writes the
+ * device LWE array d_lwes [n_lwe][limb][n1 + 1] with the masks regenerated on
+ * the GPU and the bodies [n_lwe][limb] (host) verbatim, ready for
+ * via_answer_device. No counterpart beyond regenerate_masks: it saves the
+ * 2 x n1 x 8 bytes per LWE of mask transfer. */
+via_status via_query_from_prg(via_ctx *ctx, const via_params *pp, const uint8_t *seed, size_t seed_len,
+                              const uint64_t *bodies, size_t n_lwe, uint64_t *d_lwes);
 
 /* ---- serving loop (no counterpart in the reference, which has no server
  * process; the answers it produces are those of Server::answer) ----------- */

@@ -48,6 +48,24 @@ def test_first_dim_limbs(tmp_path):
     build_and_run("fd_limbs_check.cpp", tmp_path)
 
 
+def test_prg_blocks_match_shake256(tmp_path):
+    # the device Keccak / Shake256Prg block against hashlib's SHAKE-256 of
+    # seed || u64_le(k) (prg.rs:1-17), empty to 64-byte seeds, counters up
+    # to 2^64 - 1
+    import hashlib
+
+    exe = tmp_path / "keccak"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", str(exe), os.path.join(ROOT, "tests", "host", "keccak_check.cpp")],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60, check=True).stdout.splitlines()
+    assert len(out) == 35
+    for line in out:
+        parts = line.split(" ")
+        seed_hex, k, block = (parts[0], parts[1], parts[2]) if len(parts) == 3 else ("", parts[1], parts[2])
+        want = hashlib.shake_256(bytes.fromhex(seed_hex) + int(k).to_bytes(8, "little")).digest(136).hex()
+        assert block == want, (seed_hex, k)
+
+
 def declared_symbols():
     src = open(os.path.join(ROOT, "include", "via_gpu.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)

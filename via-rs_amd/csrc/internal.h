@@ -206,6 +206,28 @@ struct RespArgs {
 };
 void launch_resp_comp(const RespArgs &a, hipStream_t s);
 
+// ---- Shake256Prg draws (prg.hip) ----
+// Seeds of up to kMaxQ streams (at most 64 bytes each, prg.rs SEED_CAP).
+struct PrgSeeds {
+    uint8_t bytes[kMaxQ][64];
+    int len[kMaxQ];
+};
+// count uniform_below draws per stream, draw d at modulus q[d % nmod]
+// (nbytes bytes per attempt for every modulus); out[s*out_sstride + d], or
+// with lwe set, draw d of row d / (n*nmod), limb d % nmod, coefficient
+// (d / nmod) % n at row*row_stride + limb*limb_stride + coefficient.
+struct PrgPlan {
+    uint64_t q[2], mask[2];
+    int nmod, nbytes;
+    size_t count, attempts, blocks;
+    int lwe, n;
+    size_t row_stride, limb_stride, out_sstride;
+};
+// stream: streams * blocks * 17 words; short_flag set when a stream had
+// fewer accepted attempts than draws (the caller sizes attempts with slack).
+void launch_prg_draws(const PrgSeeds &seeds, int streams, const PrgPlan &pl, uint64_t *stream, uint64_t *out,
+                      int *short_flag, hipStream_t s);
+
 // ---- VIA-B repack (repack.hip) ----
 // Level-0 digits [pair][k][2*n1] and interleaved bodies [pair][body_stride]
 // from the T post-CRot ciphertexts rot [T][part][n1] (ring r: q2 at n1).

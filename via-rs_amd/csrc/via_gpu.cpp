@@ -423,6 +423,9 @@ const char *via_status_str(via_status s) {
         case VIA_E_INVALID: return "parameters not supported by the device path";
         case VIA_E_DEVICE: return "HIP runtime error";
         case VIA_E_BATCH_LEN: return "batch length mismatch";
+        case VIA_E_BUFFER_SHORT: return "buffer too short";
+        case VIA_E_FORMAT_TAG: return "unknown wire format tag";
+        case VIA_E_PRG_SEED: return "invalid PRG seed";
     }
     return "unknown";
 }

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libvia_gpu.so")
 u64p = C.POINTER(C.c_uint64)
 
 VIA_OK, VIA_E_DIM, VIA_E_QUERY_LEN, VIA_E_INDEX, VIA_E_NOMEM, VIA_E_INVALID, VIA_E_DEVICE, VIA_E_BATCH_LEN = range(8)
+VIA_E_BUFFER_SHORT, VIA_E_FORMAT_TAG, VIA_E_PRG_SEED = 8, 9, 10
 
 
 class Params(C.Structure):
@@ -120,6 +121,14 @@ def lib():
             "via_db_create_bytes": (C.c_int, [vp, P, C.c_char_p, C.c_size_t, C.POINTER(vp)]),
             "via_db_destroy": (None, [vp]),
             "via_db_read_cells": (C.c_int, [vp, u64p, C.c_size_t]),
+            "via_wire_uncompressed_size": (C.c_size_t, [C.c_size_t]),
+            "via_wire_encode_uncompressed": (C.c_int, [u64p, C.c_size_t, C.c_char_p, C.c_size_t]),
+            "via_wire_decode_uncompressed": (C.c_int, [C.c_char_p, C.c_size_t, u64p, C.c_size_t,
+                                                       C.POINTER(C.c_size_t), C.POINTER(ErrDetail)]),
+            "via_wire_decode_prg": (C.c_int, [C.c_char_p, C.c_size_t, C.c_char_p, u64p, C.c_size_t,
+                                              C.POINTER(C.c_size_t), C.POINTER(ErrDetail)]),
+            "via_prg_regenerate": (C.c_int, [vp, C.c_char_p, C.c_size_t, C.c_uint64, C.c_size_t, u64p]),
+            "via_query_from_prg": (C.c_int, [vp, P, C.c_char_p, C.c_size_t, u64p, C.c_size_t, vp]),
             "via_first_dim": (C.c_int, [vp, vp, C.c_uint32, u64p, u64p]),
             "via_keys_create": (C.c_int, [vp, P, u64p, u64p, u64p, C.POINTER(vp)]),
             "via_keys_destroy": (None, [vp]),
@@ -165,9 +174,48 @@ def lib():
 def _check(status, err=None):
     if status != VIA_OK:
         msg = lib().via_status_str(status).decode()
-        if err is not None and status == VIA_E_QUERY_LEN:
+        if err is not None and status in (VIA_E_QUERY_LEN, VIA_E_BUFFER_SHORT, VIA_E_FORMAT_TAG):
             raise ViaError(status, err.expected, err.got, msg)
         raise ViaError(status, msg=msg)
+
+
+# ---- wire formats (via-protocol wire/format.rs) ----------------------------------
+
+def wire_encode(coeffs):
+    """Uncompressed::serialize (format.rs:107-113): [0x00][count][coeffs]."""
+    c = _u64(coeffs)
+    n = lib().via_wire_uncompressed_size(c.size)
+    buf = C.create_string_buffer(n)
+    _check(lib().via_wire_encode_uncompressed(_p(c), c.size, buf, n))
+    return buf.raw
+
+
+def wire_decode(buf):
+    """Uncompressed::deserialize (format.rs:115-131); ViaError with the
+    reference's BufferTooShort {needed, got} / UnknownFormatTag(tag)."""
+    b = bytes(buf)
+    err, cnt = ErrDetail(), C.c_size_t()
+    _check(lib().via_wire_decode_uncompressed(b, len(b), None, 0, C.byref(cnt), C.byref(err)), err)
+    out = np.zeros(cnt.value, dtype=np.uint64)
+    _check(lib().via_wire_decode_uncompressed(b, len(b), _p(out), out.size, C.byref(cnt), C.byref(err)), err)
+    return out
+
+
+def wire_decode_prg(buf):
+    """PrgCompressed<32>::deserialize (format.rs:163-190): the seed and the
+    body coefficients."""
+    b = bytes(buf)
+    err, cnt = ErrDetail(), C.c_size_t()
+    seed = C.create_string_buffer(32)
+    _check(lib().via_wire_decode_prg(b, len(b), seed, None, 0, C.byref(cnt), C.byref(err)), err)
+    out = np.zeros(cnt.value, dtype=np.uint64)
+    _check(lib().via_wire_decode_prg(b, len(b), seed, _p(out), out.size, C.byref(cnt), C.byref(err)), err)
+    return seed.raw, out
+
+
+def answer_bytes(mask, body):
+    """An answer on the wire: the Uncompressed mask, then the Uncompressed body."""
+    return wire_encode(mask) + wire_encode(body)
 
 
 def _u64(a):
@@ -216,6 +264,22 @@ class Context:
         t = C.c_double()
         _check(lib().via_ctx_first_dim_time(self.h, C.byref(t)))
         return t.value
+
+    def prg_regenerate(self, seed, q, count):
+        """PrgCompressed::regenerate_masks (format.rs:143-160) on the GPU."""
+        out = np.zeros(count, dtype=np.uint64)
+        _check(lib().via_prg_regenerate(self.h, bytes(seed), len(seed), q, count, _p(out)))
+        return out
+
+    def query_from_prg(self, pp, seed, bodies):
+        """Device LWE array [n_lwe][limb][n1 + 1] of a query whose masks are
+        regenerated from seed (limb-0-first RNS draws) and whose bodies
+        [n_lwe][limb] are given; the caller frees it with free()."""
+        b = _u64(bodies)
+        n_lwe = b.size // 2
+        d = self.alloc(8 * n_lwe * 2 * (pp.n1 + 1))
+        _check(lib().via_query_from_prg(self.h, C.byref(pp), bytes(seed), len(seed), _p(b), n_lwe, d))
+        return d
 
     def set_tuning(self, nq, shapes):
         """via_ctx_set_tuning: pin the twelve (groups, variant) launch shapes
