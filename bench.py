@@ -576,6 +576,7 @@ def main():
 
     I, J, n = pp.num_rows, pp.num_cols, pp.n1
     fd_bytes = fd_plan_bytes(I, J, n, Q)
+    fd_u64_bytes = I * J * n * 8 + Q * (2 * I * n * 8 + 2 * J * n * 8)
     fd_s = float(np.mean(fd_ms)) * 1e-3
     fd_gbs = fd_bytes / fd_s / 1e9
     lat_ms = np.array(lat) * 1e3
@@ -617,6 +618,10 @@ def main():
             "frac": round(fd_gbs / HBM_PEAK_GBS, 4),
             "traffic": FD_TRAFFIC.get((I, J, Q)),
             "algorithmic_bytes": fd_bytes,
+            # the same launch priced at the reference's u64 database storage
+            # (8 bytes per residue), which this layout does not move
+            "u64_equivalent_bytes": fd_u64_bytes,
+            "u64_equivalent_GBs": round(fd_u64_bytes / fd_s / 1e9, 1),
         },
     }
     if args.graphs:
