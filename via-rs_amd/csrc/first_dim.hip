@@ -202,21 +202,26 @@ void launch_sel_pack(const double *sel, uint8_t *sell, const FdPlan &p, hipStrea
 
 // ---- the matrix-core pass ------------------------------------------------------------
 
-// Workgroup (slot k, row segment): the segment's packed selection goes to LDS
-// once; wavefront w takes column blocks mb = w, w + 4, ...; for each row
+// Workgroup (slot k, column-block group, row segment): the segment's packed
+// selection goes to LDS once; wavefront w takes the group's column blocks
+// w, w + 4, ... (G groups of ceil(MB / G) blocks per slot: with one
+// selection block, G = 4 makes four times as many, shorter workgroups, so
+// the last round of the grid leaves fewer CUs idle); for each row
 // chunk it holds the five database limb fragments (two more chunks' loads in
 // flight) and runs 25 MFMAs per selection block of 16 columns into the nine
 // shift sums, then folds and stores its 16 x 2Q outputs.
 template <int KC, int NBW, int SPLIT>
 __global__ __launch_bounds__(256, 2) void k_fd_mfma(const uint8_t *__restrict__ dbl, const uint8_t *__restrict__ sell,
-                                                 double *__restrict__ out, FdPlan p, FMod m) {
+                                                 double *__restrict__ out, FdPlan p, FMod m, int G) {
     typedef FdOp<KC> Op;
     typedef typename Op::T T;
     constexpr int KB = KC / 4, FRAG = 64 * KB;
     constexpr int KS = kFdSegRows / KC;
     constexpr int NB = NBW * SPLIT;  // selection blocks the LDS holds
     __shared__ __attribute__((aligned(16))) uint8_t sb[KS * kLimbs * 16 * NB * KC];
-    const int k = blockIdx.x, seg = blockIdx.y;
+    const int k = blockIdx.x / G, gi = blockIdx.x - k * G, seg = blockIdx.y;
+    const int mbper = (p.MB + G - 1) / G;
+    const int mb0 = min(p.MB, gi * mbper), mb1 = min(p.MB, mb0 + mbper);
     const int kc0 = seg * KS, nkc = min(KS, p.KCN - kc0);
     const size_t chunk = (size_t)kLimbs * p.QP * KC;  // packed selection bytes per row chunk
     {
@@ -235,11 +240,11 @@ __global__ __launch_bounds__(256, 2) void k_fd_mfma(const uint8_t *__restrict__ 
     const size_t P = (size_t)p.Q * 2 * p.J;
     double *ok = out + ((size_t)seg * p.n + k) * P;
     // the wave's steps: (column block mb = w + 4 u, row chunk t) for u, t in order
-    const int nmb = (p.MB - w + MBW - 1) / MBW;
+    const int nmb = mb1 - mb0 - w > 0 ? (mb1 - mb0 - w + MBW - 1) / MBW : 0;
     const int steps = nmb > 0 ? nmb * nkc : 0;
     auto load = [&](T (&dst)[kLimbs], int s) {
         const int u = s / nkc, t = s - u * nkc;
-        const uint8_t *da = dk + (size_t)(w + MBW * u) * mbstride + (size_t)t * kLimbs * FRAG;
+        const uint8_t *da = dk + (size_t)(mb0 + w + MBW * u) * mbstride + (size_t)t * kLimbs * FRAG;
 #pragma unroll
         for (int l = 0; l < kLimbs; l++) dst[l] = __builtin_nontemporal_load((const T *)(da + (size_t)l * FRAG));
     };
@@ -263,7 +268,7 @@ __global__ __launch_bounds__(256, 2) void k_fd_mfma(const uint8_t *__restrict__ 
         }
         if (t != nkc - 1) return;
         // lane (g, c) of block nb holds rows j = 16 mb + 4 g + [0, 4) of column qp
-        const int j0 = (w + MBW * u) * 16 + 4 * g;
+        const int j0 = (mb0 + w + MBW * u) * 16 + 4 * g;
 #pragma unroll
         for (int nb = 0; nb < NBW; nb++) {
             const int qp = (nb0 + nb) * 16 + c;
@@ -312,13 +317,16 @@ __global__ __launch_bounds__(256, 2) void k_fd_mfma(const uint8_t *__restrict__ 
 template <int KC>
 static void fd_mfma_kc(const uint8_t *dbl, const uint8_t *sell, double *out_k, const FdPlan &p, FMod m,
                        hipStream_t st) {
-    const dim3 grid(p.n, p.nseg);
+    // column-block groups per slot: four with one selection block (its
+    // staging is small), else one (the staging is shared by every block)
+    const int G = p.NB == 1 && p.MB >= 16 ? 4 : 1;
+    const dim3 grid(p.n * G, p.nseg);
     switch (p.NB) {
-        case 1: hipLaunchKernelGGL((k_fd_mfma<KC, 1, 1>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m); break;
-        case 2: hipLaunchKernelGGL((k_fd_mfma<KC, 2, 1>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m); break;
+        case 1: hipLaunchKernelGGL((k_fd_mfma<KC, 1, 1>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m, G); break;
+        case 2: hipLaunchKernelGGL((k_fd_mfma<KC, 2, 1>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m, G); break;
         // three or four blocks: two wavefronts per column block (the block past
         // 2Q columns, if any, multiplies zeros)
-        default: hipLaunchKernelGGL((k_fd_mfma<KC, 2, 2>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m); break;
+        default: hipLaunchKernelGGL((k_fd_mfma<KC, 2, 2>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m, G); break;
     }
 }
 
