@@ -101,6 +101,15 @@ via_status via_db_create(via_ctx *ctx, const via_params *pp, const uint64_t *rec
 via_status via_db_create_bytes(via_ctx *ctx, const via_params *pp, const uint8_t *bytes, size_t n_records,
                                via_db **out);
 void via_db_destroy(via_db *db);
+/* Incremental record updates (no counterpart in the reference, whose
+ * PreparedDb is rebuilt whole): replaces records indices[u] (record m is slot
+ * m / (I*J) of cell m mod I*J, setup_db.rs:58-72) with records[u]
+ * ([n][n_rec] coefficients, reduced mod p) in the HBM-resident database;
+ * every touched cell is rebuilt on the device (inverse transform, slot
+ * rewrite, transform, repack). A repeated index takes its last record;
+ * VIA_E_INDEX for an index beyond I*J*(n1/n_rec). Answers afterwards are
+ * those of a database set up with the updated records. */
+via_status via_db_update(via_db *db, const uint64_t *indices, size_t n, const uint64_t *records);
 /* The prepared cells, eval form at q2, [I][J][n1] (PreparedDb::cell). */
 via_status via_db_read_cells(const via_db *db, uint64_t *out, size_t words);
 

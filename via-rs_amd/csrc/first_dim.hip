@@ -24,6 +24,7 @@
 // transposes to [q][j][part][k] for the inverse transform.
 #include "internal.h"
 #include "fd_limbs.h"
+#include "ntt_device.h"
 
 namespace via {
 
@@ -153,6 +154,65 @@ void launch_unpack_db(const uint8_t *dbl, uint64_t *out, const FdPlan &p, hipStr
     const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 65536);
     if (p.KC == 64) hipLaunchKernelGGL(k_unpack_db<64>, dim3(blocks), dim3(256), 0, st, dbl, out, p);
     else hipLaunchKernelGGL(k_unpack_db<32>, dim3(blocks), dim3(256), 0, st, dbl, out, p);
+}
+
+// Incremental update of whole cells (one workgroup of 256 threads per
+// touched cell): gather the cell's 2048 eval residues from the fragments,
+// inverse-transform at q2 to the coefficient form setup_db builds
+// (setup_db.rs:58-72: record slot s holds coefficient c at d*c + s), rewrite
+// the slots listed in slot_src (index of the new record in recs, or -1 to
+// keep), transform back and scatter the limb bytes. Distinct cells own
+// distinct bytes of the fragments.
+template <int KC>
+__global__ __launch_bounds__(256) void k_db_update(uint8_t *dbl, FdPlan p, RingDev r2, const int64_t *cells,
+                                                   const int32_t *slot_src, int d, const uint64_t *recs, int n_rec,
+                                                   uint64_t pmod) {
+    constexpr int KB = KC / 4, FRAG = 64 * KB;
+    __shared__ double s[lds_words(kSlots)];
+    const int64_t cell = cells[blockIdx.x];
+    const int i = (int)(cell / p.J), j = (int)(cell - (int64_t)i * p.J);
+    const int mb = j >> 4, rr = j & 15, kc = i / KC, g = (i % KC) / KB, b = i % KB;
+    const size_t lane_off = (size_t)(g * 16 + rr) * KB + b;
+    const size_t kstride = (size_t)p.MB * p.KCN * kLimbs * FRAG;
+    const size_t frag0 = (((size_t)mb * p.KCN + kc) * kLimbs) * FRAG + lane_off;
+    const int n = p.n;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const uint8_t *src = dbl + (size_t)k * kstride + frag0;
+        double v = 0.0;
+        for (int a = kLimbs - 1; a >= 0; a--) v = v * 128.0 + (double)src[(size_t)a * FRAG];
+        s[lds_pad(k)] = v;
+    }
+    __syncthreads();
+    lds_ntt_inv_noscale<1, 4>(s, n, r2);
+    __syncthreads();
+    const FMod m = r2.t[0].m;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) s[lds_pad(c)] = mod_mul(s[lds_pad(c)], r2.t[0].n_inv, m);
+    __syncthreads();
+    for (int t = threadIdx.x; t < d * n_rec; t += blockDim.x) {
+        const int slot = t / n_rec, c = t - slot * n_rec;
+        const int src = slot_src[(size_t)blockIdx.x * d + slot];
+        if (src >= 0) s[lds_pad(d * c + slot)] = (double)(recs[(size_t)src * n_rec + c] % pmod);
+    }
+    __syncthreads();
+    lds_ntt_fwd<1, 4>(s, n, r2);
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        uint32_t l[kLimbs];
+        limbs7(mod_reduce_small(s[lds_pad(k)], m), l);
+        uint8_t *dst = dbl + (size_t)k * kstride + frag0;
+        for (int a = 0; a < kLimbs; a++) dst[(size_t)a * FRAG] = (uint8_t)l[a];
+    }
+}
+
+void launch_db_update(uint8_t *dbl, const FdPlan &p, const RingDev &r2, const int64_t *cells, int ncells,
+                      const int32_t *slot_src, int d, const uint64_t *recs, int n_rec, uint64_t pmod, hipStream_t st) {
+    if (ncells < 1) return;
+    if (p.KC == 64)
+        hipLaunchKernelGGL(k_db_update<64>, dim3(ncells), dim3(256), 0, st, dbl, p, r2, cells, slot_src, d, recs,
+                           n_rec, pmod);
+    else
+        hipLaunchKernelGGL(k_db_update<32>, dim3(ncells), dim3(256), 0, st, dbl, p, r2, cells, slot_src, d, recs,
+                           n_rec, pmod);
 }
 
 // ---- selection pack -------------------------------------------------------------------

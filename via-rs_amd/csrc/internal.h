@@ -189,6 +189,11 @@ void launch_unpack_db(const uint8_t *dbl, uint64_t *out, const FdPlan &p, hipStr
 void launch_sel_pack(const double *sel_eval, uint8_t *sell, const FdPlan &p, hipStream_t s);
 void launch_fd_mfma(const uint8_t *dbl, const uint8_t *sell, double *out_k, const FdPlan &p, FMod m, hipStream_t s);
 void launch_fd_gather(const double *out_k, double *out_eval, const FdPlan &p, FMod m, hipStream_t s);
+// Rewrites record slots of whole cells in place (r2: q2 at n1): cell c =
+// cells[x] takes, for slot s < d, record slot_src[x*d + s] of recs (n_rec
+// coefficients each, reduced mod pmod) or keeps its own (-1).
+void launch_db_update(uint8_t *dbl, const FdPlan &p, const RingDev &r2, const int64_t *cells, int ncells,
+                      const int32_t *slot_src, int d, const uint64_t *recs, int n_rec, uint64_t pmod, hipStream_t s);
 
 // RespComp for Q ciphertexts at (n1, q2): mod switch to q3, ring switch to n2
 // with each query's eval RSK ([j][k][part][n2]), then mask at q3 / body at q4.

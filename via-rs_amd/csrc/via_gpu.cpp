@@ -613,6 +613,68 @@ void via_db_destroy(via_db *db) {
     delete db;
 }
 
+// Incremental record updates: record m = s*I*J + i*J + j is slot s of cell
+// (i, j) (setup_db.rs:58-72); each touched cell is rebuilt in place on the
+// device (launch_db_update). A repeated index takes its last record.
+via_status via_db_update(via_db *db, const uint64_t *indices, size_t n, const uint64_t *records) {
+    if (!db || (n && (!indices || !records))) return VIA_E_INVALID;
+    if (!n) return VIA_OK;
+    via_ctx *c = db->ctx;
+    const via_params *pp = &db->pp;
+    const size_t I = pp->num_rows, J = pp->num_cols, n1 = pp->n1, nr = pp->n_rec, d = n1 / nr;
+    const size_t cap = I * J * d;
+    for (size_t u = 0; u < n; u++)
+        if (indices[u] >= cap) return VIA_E_INDEX;
+    // cell -> its d slots' sources (index into records, or -1)
+    std::map<int64_t, std::vector<int32_t>> touched;
+    for (size_t u = 0; u < n; u++) {
+        const uint64_t m = indices[u];
+        const int64_t cell = (int64_t)(m % (I * J));
+        auto &v = touched[cell];
+        if (v.empty()) v.assign(d, -1);
+        v[m / (I * J)] = (int32_t)u;
+    }
+    std::vector<int64_t> cells;
+    std::vector<int32_t> slots;
+    for (auto &kv : touched) {
+        cells.push_back(kv.first);
+        slots.insert(slots.end(), kv.second.begin(), kv.second.end());
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    RingDev r2;
+    uint64_t q2 = pp->q2;
+    via_status s = make_ring(c, (uint32_t)n1, &q2, 1, &r2);
+    if (s) return s;
+    const FdPlan fp = fd_plan((int)I, (int)J, (int)n1, 1);
+    int64_t *d_cells = nullptr;
+    int32_t *d_slots = nullptr;
+    uint64_t *d_recs = nullptr;
+    if (hipMalloc(&d_cells, sizeof(int64_t) * cells.size()) != hipSuccess) return VIA_E_NOMEM;
+    if (hipMalloc(&d_slots, sizeof(int32_t) * slots.size()) != hipSuccess) {
+        (void)hipFree(d_cells);
+        return VIA_E_NOMEM;
+    }
+    if (hipMalloc(&d_recs, sizeof(uint64_t) * n * nr) != hipSuccess) {
+        (void)hipFree(d_cells);
+        (void)hipFree(d_slots);
+        return VIA_E_NOMEM;
+    }
+    hipStream_t st = c->stream;
+    bool ok = hipMemcpyAsync(d_cells, cells.data(), sizeof(int64_t) * cells.size(), hipMemcpyHostToDevice, st) ==
+                  hipSuccess &&
+              hipMemcpyAsync(d_slots, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice, st) ==
+                  hipSuccess &&
+              hipMemcpyAsync(d_recs, records, sizeof(uint64_t) * n * nr, hipMemcpyHostToDevice, st) == hipSuccess;
+    if (ok) {
+        launch_db_update(db->limbs, fp, r2, d_cells, (int)cells.size(), d_slots, (int)d, d_recs, (int)nr, pp->p, st);
+        ok = hipStreamSynchronize(st) == hipSuccess && hipGetLastError() == hipSuccess;
+    }
+    (void)hipFree(d_cells);
+    (void)hipFree(d_slots);
+    (void)hipFree(d_recs);
+    return ok ? VIA_OK : VIA_E_DEVICE;
+}
+
 via_status via_db_read_cells(const via_db *db, uint64_t *out, size_t words) {
     if (!db || !out) return VIA_E_INVALID;
     const size_t total = (size_t)db->pp.num_rows * db->pp.num_cols * db->pp.n1;

@@ -121,6 +121,7 @@ def lib():
             "via_db_create_bytes": (C.c_int, [vp, P, C.c_char_p, C.c_size_t, C.POINTER(vp)]),
             "via_db_destroy": (None, [vp]),
             "via_db_read_cells": (C.c_int, [vp, u64p, C.c_size_t]),
+            "via_db_update": (C.c_int, [vp, u64p, C.c_size_t, u64p]),
             "via_wire_uncompressed_size": (C.c_size_t, [C.c_size_t]),
             "via_wire_encode_uncompressed": (C.c_int, [u64p, C.c_size_t, C.c_char_p, C.c_size_t]),
             "via_wire_decode_uncompressed": (C.c_int, [C.c_char_p, C.c_size_t, u64p, C.c_size_t,
@@ -381,6 +382,12 @@ class PreparedDb:
         out = np.zeros(nq * pp.num_cols * 2 * pp.n1, dtype=np.uint64)
         _check(lib().via_first_dim(self.ctx.h, self.h, nq, _p(sw), _p(out)))
         return out
+
+    def update(self, indices, records):
+        """via_db_update: replace records in place ([n][n_rec] coefficients)."""
+        idx = _u64(indices)
+        recs = _u64(records)
+        _check(lib().via_db_update(self.h, _p(idx), idx.size, _p(recs)))
 
     def cells(self):
         words = self.pp.num_rows * self.pp.num_cols * self.pp.n1
