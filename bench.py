@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # dispatch, scaled by what the same counters read for a 256 MiB device copy
 # (fetched bytes = 2.000 x FETCH_SIZE KiB, written bytes = 1.000 x WRITE_SIZE
 # KiB on this part).
-FD_TRAFFIC = {(256, 256, 1): (2.0 * 330307.8 + 8192.0) * 1024}
+FD_TRAFFIC = {(256, 256, 1): (2.0 * 337981.8 + 8192.0) * 1024}
 
 
 def fd_plan_bytes(I, J, n, Q):
