@@ -250,6 +250,19 @@ via_status via_prg_regenerate(via_ctx *ctx, const uint8_t *seed, size_t seed_len
 via_status via_query_from_prg(via_ctx *ctx, const via_params *pp, const uint8_t *seed, size_t seed_len,
                               const uint64_t *bodies, size_t n_lwe, uint64_t *d_lwes);
 
+/* ---- client recovery (via-client), host code, no device ---------------------- */
+
+/* Client::recover (client.rs:222-239 -> decrypt_asymmetric, rlwe.rs:187-224,
+ * and decode, rlwe.rs:296-330): the n2 plaintext coefficients (mod p) of an
+ * answer (mask [n2] < q3, body [n2] < q4) under the client's S2 [n2] < q3.
+ * VIA_E_INVALID for non-canonical inputs. */
+via_status via_client_recover(const via_params *pp, const uint64_t *sk2, const uint64_t *mask, const uint64_t *body,
+                              uint64_t *plain);
+/* deinterleave_batch (via-client batch.rs:31-52): record t < T of a VIA-B
+ * plaintext is coefficient d*c + t, c < n_rec, d = n2 / n_rec; records
+ * [T][n_rec]. VIA_E_INVALID when T * n_rec > n2. */
+via_status via_client_deinterleave(const via_params *pp, uint32_t T, const uint64_t *plain, uint64_t *records);
+
 /* ---- serving loop (no counterpart in the reference, which has no server
  * process; the answers it produces are those of Server::answer) ----------- */
 typedef struct via_server via_server;

@@ -122,6 +122,8 @@ def lib():
             "via_db_destroy": (None, [vp]),
             "via_db_read_cells": (C.c_int, [vp, u64p, C.c_size_t]),
             "via_db_update": (C.c_int, [vp, u64p, C.c_size_t, u64p]),
+            "via_client_recover": (C.c_int, [P, u64p, u64p, u64p, u64p]),
+            "via_client_deinterleave": (C.c_int, [P, C.c_uint32, u64p, u64p]),
             "via_wire_uncompressed_size": (C.c_size_t, [C.c_size_t]),
             "via_wire_encode_uncompressed": (C.c_int, [u64p, C.c_size_t, C.c_char_p, C.c_size_t]),
             "via_wire_decode_uncompressed": (C.c_int, [C.c_char_p, C.c_size_t, u64p, C.c_size_t,
@@ -212,6 +214,20 @@ def wire_decode_prg(buf):
     out = np.zeros(cnt.value, dtype=np.uint64)
     _check(lib().via_wire_decode_prg(b, len(b), seed, _p(out), out.size, C.byref(cnt), C.byref(err)), err)
     return seed.raw, out
+
+
+def client_recover(pp, sk2, mask, body):
+    """Client::recover (client.rs:222-239): the n2 plaintext coefficients."""
+    out = np.zeros(pp.n2, dtype=np.uint64)
+    _check(lib().via_client_recover(C.byref(pp), _p(_u64(sk2)), _p(_u64(mask)), _p(_u64(body)), _p(out)))
+    return out
+
+
+def client_deinterleave(pp, T, plain):
+    """deinterleave_batch (batch.rs:31-52): T records of n_rec coefficients."""
+    out = np.zeros((T, pp.n_rec), dtype=np.uint64)
+    _check(lib().via_client_deinterleave(C.byref(pp), T, _p(_u64(plain)), _p(out)))
+    return out
 
 
 def answer_bytes(mask, body):
