@@ -250,6 +250,49 @@ via_status via_prg_regenerate(via_ctx *ctx, const uint8_t *seed, size_t seed_len
 via_status via_query_from_prg(via_ctx *ctx, const via_params *pp, const uint8_t *seed, size_t seed_len,
                               const uint64_t *bodies, size_t n_lwe, uint64_t *d_lwes);
 
+/* ---- client key generation and queries (via-client) -------------------------- */
+
+/* Distribution (sampling/distribution.rs:38-60): kind 0 Ternary, kind 1
+ * BoundedUniform { bound } ([-bound, bound]); the Gaussian arm is not
+ * implemented (VIA_E_INVALID). */
+typedef struct {
+    int32_t kind;
+    uint32_t bound;
+} via_dist;
+
+/* Shake256Prg (sampling/prg.rs): the stream of a seed of at most 64 bytes
+ * (VIA_E_PRG_SEED beyond), consumed by via_client_setup and
+ * via_client_query in the reference's draw order; position = bytes used. */
+typedef struct via_prg via_prg;
+via_status via_prg_create(const uint8_t *seed, size_t seed_len, via_prg **out);
+void via_prg_destroy(via_prg *prg);
+uint64_t via_prg_position(const via_prg *prg);
+
+/* Words of the client's keys: cascade key (log2(n1) steps of 2*n1 RLev
+ * samples' worth, [k][part][limb][n_out] per key), conversion key
+ * RLev_S1(S1^2) [k][part][limb][n1], ring-switch key [j][k][part][n2]. */
+via_status via_client_key_words(const via_params *pp, size_t *cascade, size_t *conv, size_t *rsk);
+
+/* Client::setup (client.rs:89-173): S1 (key1, at q1, [limb][n1]), S2 (key2,
+ * at q3, [n2]), the cascade key, RLev_S1(S1^2) and the ring-switch key, all
+ * drawn from prg exactly as the reference draws them. The stream's blocks
+ * are computed on the GPU and parsed on the host; the ring products run on
+ * the GPU. VIA_E_DIM for a non-power-of-two num_rows or num_cols
+ * (client.rs:125-131); VIA_E_INVALID for parameters outside the device path
+ * (n1 = 2048 over two NTT primes, q3 NTT-friendly at n2). The PRG advances
+ * only on success. Host buffers. */
+via_status via_client_setup(via_ctx *ctx, const via_params *pp, via_dist key1, via_dist key2, via_dist err,
+                            via_prg *prg, uint64_t *sk1, uint64_t *sk2, uint64_t *cascade_key, uint64_t *conv_key,
+                            uint64_t *rsk);
+
+/* Client::query (client.rs:183-207 -> build_compressed_query, query.rs:
+ * 75-130): the query's raw LWEs ([lwe][limb][n1 + 1], bits in DMux, CMux,
+ * CRot order, L_QUERY per bit) under S1, drawn from prg. VIA-B batch queries
+ * (client.rs:256-300) are T calls in batch order with n_rec the record
+ * degree. VIA_E_INDEX for index >= (n1/n_rec) * num_rows * num_cols. */
+via_status via_client_query(via_ctx *ctx, const via_params *pp, const uint64_t *sk1, size_t index, via_dist err,
+                            via_prg *prg, uint64_t *lwes);
+
 /* ---- client recovery (via-client), host code, no device ---------------------- */
 
 /* Client::recover (client.rs:222-239 -> decrypt_asymmetric, rlwe.rs:187-224,

@@ -190,3 +190,20 @@ def test_recover_toy_via_b_batch(oracle):
     plain = via_gpu.client_recover(pp, sk2, am, ab)
     got = via_gpu.client_deinterleave(pp, 8, plain)
     assert got.tolist() == [recs[i] for i in idxs]
+
+
+def test_prg_handles():
+    # Shake256Prg seeds of 0 to 64 bytes; longer ones are InvalidPrgSeed
+    for n in (0, 32, 64):
+        g = via_gpu.Prg(bytes(range(n)))
+        assert g.position == 0
+        g.close()
+    with pytest.raises(via_gpu.ViaError) as e:
+        via_gpu.Prg(bytes(65))
+    assert e.value.status == via_gpu.VIA_E_PRG_SEED
+
+
+def test_client_key_words_match_oracle(oracle):
+    for opp in (paper_params(2, 2), paper_b_params(4, 4, T=8)):
+        assert via_gpu.client_key_words(lib_params(opp)) == (
+            oracle.words_cascade_key(opp), oracle.words_conv_key(opp), oracle.words_rsk(opp))

@@ -122,6 +122,9 @@ namespace via {
 // device and batch size, the twelve decompression steps. A batch size that
 // was not tuned takes the largest tuned size below it.
 void tuned_erase(int device, int Q);
+// Device ring of degree n over nl NTT-friendly moduli (twiddle tables cached
+// on the context).
+via_status ctx_ring(via_ctx *c, uint32_t n, const uint64_t *q, int nl, RingDev *r);
 void tuned_store(int device, int Q, const std::vector<StepTune> &steps);
 bool tuned_lookup(int device, int Q, std::vector<StepTune> *steps);
 // Incremented by every tuned_store: captured graphs bake in launch shapes.

@@ -232,6 +232,9 @@ struct PrgPlan {
 // fewer accepted attempts than draws (the caller sizes attempts with slack).
 void launch_prg_draws(const PrgSeeds &seeds, int streams, const PrgPlan &pl, uint64_t *stream, uint64_t *out,
                       int *short_flag, hipStream_t s);
+// Blocks first .. first + count - 1 of stream 0 of `seeds`, 17 little-endian
+// lanes each, into out (count * 17 words).
+void launch_prg_window(const PrgSeeds &seeds, uint64_t first, size_t count, uint64_t *out, hipStream_t s);
 
 // ---- VIA-B repack (repack.hip) ----
 // Level-0 digits [pair][k][2*n1] and interleaved bodies [pair][body_stride]

@@ -112,6 +112,25 @@ __global__ __launch_bounds__(1024) void k_prg_draws(const uint64_t *stream, PrgP
     }
 }
 
+// Blocks first .. first + count - 1 of one stream (seed index 0) into
+// out[(b - first) * 17 ...]: a window of the byte stream for host-side
+// parsing (the client key generation).
+__global__ __launch_bounds__(256) void k_prg_window(PrgSeeds seeds, uint64_t first, size_t count, uint64_t *out) {
+    const size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= count) return;
+    uint64_t lanes[17];
+    prg_block(seeds.bytes[0], seeds.len[0], first + b, lanes);
+    uint64_t *o = out + b * 17;
+#pragma unroll
+    for (int i = 0; i < 17; i++) o[i] = lanes[i];
+}
+
+void launch_prg_window(const PrgSeeds &seeds, uint64_t first, size_t count, uint64_t *out, hipStream_t st) {
+    if (count)
+        hipLaunchKernelGGL(k_prg_window, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, seeds, first, count,
+                           out);
+}
+
 void launch_prg_draws(const PrgSeeds &seeds, int streams, const PrgPlan &pl, uint64_t *stream, uint64_t *out,
                       int *short_flag, hipStream_t st) {
     const dim3 g1((unsigned)((pl.blocks + 255) / 256), streams);

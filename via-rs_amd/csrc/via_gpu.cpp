@@ -831,6 +831,10 @@ void via::tuned_store(int device, int Q, const std::vector<StepTune> &steps) {
     g_tuned_gen++;
 }
 
+via_status via::ctx_ring(via_ctx *c, uint32_t n, const uint64_t *q, int nl, RingDev *r) {
+    return make_ring(c, n, q, nl, r);
+}
+
 void via::tuned_erase(int device, int Q) {
     std::lock_guard<std::mutex> g(g_tuned_mu);
     g_tuned.erase({device, Q});

@@ -71,6 +71,16 @@ def paper_b_params(num_rows, num_cols, T=8):
     return pp
 
 
+class Dist(C.Structure):
+    """via_dist: Distribution (sampling/distribution.rs:38-60), kind 0
+    Ternary, kind 1 BoundedUniform { bound }."""
+
+    _fields_ = [("kind", C.c_int32), ("bound", C.c_uint32)]
+
+
+TERNARY = Dist(0, 0)
+
+
 class ErrDetail(C.Structure):
     _fields_ = [("expected", C.c_uint64), ("got", C.c_uint64), ("msg", C.c_char * 128)]
 
@@ -123,6 +133,13 @@ def lib():
             "via_db_read_cells": (C.c_int, [vp, u64p, C.c_size_t]),
             "via_db_update": (C.c_int, [vp, u64p, C.c_size_t, u64p]),
             "via_client_recover": (C.c_int, [P, u64p, u64p, u64p, u64p]),
+            "via_prg_create": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(vp)]),
+            "via_prg_destroy": (None, [vp]),
+            "via_prg_position": (C.c_uint64, [vp]),
+            "via_client_key_words": (C.c_int, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
+                                               C.POINTER(C.c_size_t)]),
+            "via_client_setup": (C.c_int, [vp, P, Dist, Dist, Dist, vp, u64p, u64p, u64p, u64p, u64p]),
+            "via_client_query": (C.c_int, [vp, P, u64p, C.c_size_t, Dist, vp, u64p]),
             "via_client_deinterleave": (C.c_int, [P, C.c_uint32, u64p, u64p]),
             "via_wire_uncompressed_size": (C.c_size_t, [C.c_size_t]),
             "via_wire_encode_uncompressed": (C.c_int, [u64p, C.c_size_t, C.c_char_p, C.c_size_t]),
@@ -214,6 +231,56 @@ def wire_decode_prg(buf):
     out = np.zeros(cnt.value, dtype=np.uint64)
     _check(lib().via_wire_decode_prg(b, len(b), seed, _p(out), out.size, C.byref(cnt), C.byref(err)), err)
     return seed.raw, out
+
+
+class Prg:
+    """Shake256Prg (sampling/prg.rs) over a seed of at most 64 bytes."""
+
+    def __init__(self, seed):
+        h = C.c_void_p()
+        _check(lib().via_prg_create(seed, len(seed), C.byref(h)))
+        self.h = h
+
+    @property
+    def position(self):
+        return lib().via_prg_position(self.h)
+
+    def close(self):
+        if self.h:
+            lib().via_prg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def client_key_words(pp):
+    """(cascade, conv, rsk) key sizes in words."""
+    a, b, c = C.c_size_t(), C.c_size_t(), C.c_size_t()
+    _check(lib().via_client_key_words(C.byref(pp), C.byref(a), C.byref(b), C.byref(c)))
+    return a.value, b.value, c.value
+
+
+def client_setup(ctx, pp, prg, key1=TERNARY, key2=TERNARY, err=TERNARY):
+    """Client::setup (client.rs:89-173): dict of sk1, sk2, cascade_key,
+    conv_key, rsk drawn from prg."""
+    wc, wv, wr = client_key_words(pp)
+    out = {"sk1": np.zeros(2 * pp.n1, np.uint64), "sk2": np.zeros(pp.n2, np.uint64),
+           "cascade_key": np.zeros(wc, np.uint64), "conv_key": np.zeros(wv, np.uint64),
+           "rsk": np.zeros(wr, np.uint64)}
+    _check(lib().via_client_setup(ctx.h, C.byref(pp), key1, key2, err, prg.h, _p(out["sk1"]), _p(out["sk2"]),
+                                  _p(out["cascade_key"]), _p(out["conv_key"]), _p(out["rsk"])))
+    return out
+
+
+def client_query(ctx, pp, sk1, index, prg, err=TERNARY):
+    """Client::query (client.rs:183-207): the query's raw LWEs."""
+    out = np.zeros(pp.query_lwes * pp.lwe_words, np.uint64)
+    _check(lib().via_client_query(ctx.h, C.byref(pp), _p(_u64(sk1)), index, err, prg.h, _p(out)))
+    return out
 
 
 def client_recover(pp, sk2, mask, body):
