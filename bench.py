@@ -380,6 +380,65 @@ def ntt_run(via_gpu, args):
     ctx.close()
 
 
+def keygen_run(via_gpu, args):
+    """Client::setup and Client::query of the library (PRG stream on the GPU,
+    host parse, GPU products) at the paper stack; every key word checked
+    against the C restatement on the first setup."""
+    pp = via_gpu.paper_params(args.rows, args.cols)
+    ctx = via_gpu.Context(0)
+    g = via_gpu.Prg(b"bench/client-keys")
+    for _ in range(args.warmup):
+        keys = via_gpu.client_setup(ctx, pp, g)
+    t_setup = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        keys = via_gpu.client_setup(ctx, pp, g)
+        t_setup.append(time.perf_counter() - t)
+    elapsed = time.perf_counter() - t0
+    t_query = []
+    for i in range(4 * args.steps):
+        t = time.perf_counter()
+        via_gpu.client_query(ctx, pp, keys["sk1"], (7919 * i) % (4 * args.rows * args.cols), g)
+        t_query.append(time.perf_counter() - t)
+    wc, wv, wr = via_gpu.client_key_words(pp)
+    out = {
+        "metric": "client setups/sec (Client::setup: S1, S2, cascade key, conversion key, ring-switch key), "
+                  "bit-exact vs CPU",
+        "value": round(args.steps / elapsed, 2), "unit": "setups/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "p50_ms": round(float(np.median(t_setup)) * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int (exact modular arithmetic, u64 residues)",
+        "data": "synthetic: keys drawn from Shake256Prg(b'bench/client-keys')",
+        "config": {"workload": f"paper stack n1=2048 n2=512, L_CK=18 (base 18), L_RSK=8; keys of "
+                               f"{8 * (wc + wv + wr) / 2**20:.2f} MiB per client; query of "
+                               f"{pp.query_lwes} LWEs at I={args.rows} J={args.cols}",
+                   "num_rows": args.rows, "num_cols": args.cols},
+        "query_ms_p50": round(float(np.median(t_query)) * 1e3, 3),
+    }
+    if not args.no_cpu:
+        from oracle.vo import TERNARY, Oracle, paper_params, u64p
+
+        o = Oracle()
+        opp = paper_params(args.rows, args.cols)
+        og = o.prg(b"bench/client-keys/cpu")
+        bufs = [np.zeros(2 * pp.n1, np.uint64), np.zeros(pp.n2, np.uint64), np.zeros(wc, np.uint64),
+                np.zeros(wv, np.uint64), np.zeros(wr, np.uint64)]
+        t0 = time.perf_counter()
+        o.client_setup(opp, TERNARY, TERNARY, TERNARY, og.h, *[b.ctypes.data_as(u64p) for b in bufs])
+        t_cpu = time.perf_counter() - t0
+        g2 = via_gpu.Prg(b"bench/client-keys/cpu")
+        mine = via_gpu.client_setup(ctx, pp, g2)
+        same = all(np.array_equal(mine[k], b) for k, b in zip(("sk1", "sk2", "cascade_key", "conv_key", "rsk"), bufs))
+        out["cpu_baseline"] = {"value": round(1.0 / t_cpu, 3), "unit": "setups/s", "cores": 1, "kind": "port",
+                               "sample": f"one Client::setup by the C restatement, one thread ({t_cpu:.2f} s)"}
+        out["bit_exact_vs_cpu"] = bool(same)
+        g2.close()
+    print(json.dumps(out))
+    g.close()
+    ctx.close()
+
+
 def batched_run(via_gpu, ctx, pp, db, Q, args, seed=31):
     """Q queries from Q distinct clients answered per pass (via_answer_many):
     each client has its own device copy of its keys (key material cycles
@@ -454,6 +513,7 @@ def main():
     ap.add_argument("--ntt", action="store_true", help="NTT->pointwise->iNTT microbench")
     ap.add_argument("--polys", type=int, default=8192, help="NTT microbench: RNS polynomial pairs")
     ap.add_argument("--ntt-sample", type=int, default=8192, help="NTT microbench: products checked on the CPU")
+    ap.add_argument("--keygen", action="store_true", help="client key generation and query encryption")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one device copy of known size (256 MiB; one NTT operand "
                          "in --ntt mode): a plain copy kernel for counter calibration")
@@ -471,6 +531,9 @@ def main():
         return
     if args.ntt:
         ntt_run(via_gpu, args)
+        return
+    if args.keygen:
+        keygen_run(via_gpu, args)
         return
     pp = via_gpu.paper_params(args.rows, args.cols)
     cascade, conv, rsk, lwes, record_bytes, n_rec = synthetic_inputs(pp)

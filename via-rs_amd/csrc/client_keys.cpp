@@ -13,6 +13,7 @@
 // adds errors and gadget-scaled messages.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -29,6 +30,10 @@ struct via_prg {
     uint8_t seed[kPrgSeedCap];
     int seed_len = 0;
     uint64_t pos = 0;  // stream bytes consumed
+    // stream bytes [cache_base, cache_base + cache.size()) computed by an
+    // earlier call and not yet consumed (whole blocks)
+    std::vector<uint8_t> cache;
+    uint64_t cache_base = 0;
 };
 
 via_status via_prg_create(const uint8_t *seed, size_t seed_len, via_prg **out) {
@@ -46,6 +51,7 @@ via_status via_prg_create(const uint8_t *seed, size_t seed_len, via_prg **out) {
 void via_prg_destroy(via_prg *g) {
     if (!g) return;
     memset(g->seed, 0, sizeof g->seed);
+    if (!g->cache.empty()) memset(g->cache.data(), 0, g->cache.size());
     delete g;
 }
 
@@ -53,28 +59,45 @@ uint64_t via_prg_position(const via_prg *g) { return g ? g->pos : 0; }
 
 namespace {
 
-constexpr size_t kWindowBlocks = 1 << 15;  // 4.25 MiB of stream per GPU window
+constexpr size_t kWindowBlocks = 1 << 15;  // at most 4.25 MiB of stream per GPU window
 
 // The stream from the PRG's position: windows of whole blocks computed on
-// the device, parsed here. The PRG advances only when the caller commits.
+// the device, parsed here; the bytes left over stay with the PRG for its
+// next call. The PRG advances only when the caller commits.
 struct Reader {
     via_ctx *c;
     via_prg *g;
     PrgSeeds seeds;
     std::vector<uint8_t> buf;  // stream bytes [base, base + buf.size())
     uint64_t base, pos;
+    size_t next_blocks;        // size of the next window
     uint64_t *d = nullptr;
     via_status err = VIA_OK;
 
-    Reader(via_ctx *ctx, via_prg *prg) : c(ctx), g(prg) {
+    // expect: about how many stream bytes the call will read (sizes the
+    // first window)
+    Reader(via_ctx *ctx, via_prg *prg, size_t expect) : c(ctx), g(prg) {
         memset(&seeds, 0, sizeof seeds);
         memcpy(seeds.bytes[0], g->seed, (size_t)g->seed_len);
         seeds.len[0] = g->seed_len;
         pos = g->pos;
-        base = pos / kPrgBlock * kPrgBlock;
+        buf.swap(g->cache);
+        base = g->cache_base;
+        if (pos < base || pos > base + buf.size()) {
+            buf.clear();
+            base = pos / kPrgBlock * kPrgBlock;
+        }
+        next_blocks = std::min(kWindowBlocks, expect / kPrgBlock + 64);
     }
     ~Reader() {
         if (d) (void)hipFree(d);
+        const uint64_t keep = pos / kPrgBlock * kPrgBlock;  // hand the unread whole blocks back
+        if (keep > base && keep - base <= buf.size()) {
+            buf.erase(buf.begin(), buf.begin() + (ptrdiff_t)(keep - base));
+            base = keep;
+        }
+        g->cache.swap(buf);
+        g->cache_base = base;
     }
     void commit() { g->pos = pos; }
 
@@ -92,14 +115,16 @@ struct Reader {
                 err = VIA_E_NOMEM;
                 return false;
             }
+            const size_t nb = next_blocks;
+            next_blocks = kWindowBlocks;
             const uint64_t first = (base + buf.size()) / kPrgBlock;
             const size_t old = buf.size();
-            buf.resize(old + kWindowBlocks * kPrgBlock);
-            launch_prg_window(seeds, first, kWindowBlocks, d, c->stream);
+            buf.resize(old + nb * kPrgBlock);
+            launch_prg_window(seeds, first, nb, d, c->stream);
             // 17 little-endian lanes are the block's 136 bytes in stream order
-            if (hipMemcpyAsync(buf.data() + old, d, kWindowBlocks * kPrgBlock, hipMemcpyDeviceToHost, c->stream) !=
-                    hipSuccess ||
+            if (hipMemcpyAsync(buf.data() + old, d, nb * kPrgBlock, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
                 hipStreamSynchronize(c->stream) != hipSuccess) {
+                buf.resize(old);
                 err = VIA_E_DEVICE;
                 return false;
             }
@@ -312,7 +337,11 @@ via_status via_client_setup(via_ctx *c, const via_params *pp, via_dist key1, via
     if (hipSetDevice(c->device) != hipSuccess) return VIA_E_DEVICE;
     const size_t n1 = pp->n1, n2 = pp->n2, nl = 2, L = pp->l_ck;
     const uint64_t q3 = pp->q3;
-    Reader rd(c, prg);
+    // stream use: every mask coefficient takes a 5-byte attempt per limb
+    // (about 2 for q0 just above 2^37) and every error a byte or so
+    size_t words = 0;
+    via_client_key_words(pp, &words, nullptr, nullptr);
+    Reader rd(c, prg, words / 2 * 8);
     // 1-2. secret keys (SecretKey::keygen: centred samples lifted to each limb)
     std::vector<int64_t> c1(n1), c2(n2);
     rd.sample(key1, c1.data(), n1);
@@ -419,7 +448,8 @@ via_status via_client_query(via_ctx *c, const via_params *pp, const uint64_t *sk
     const int nd = __builtin_ctzll(I), nc = __builtin_ctzll(J), nr = __builtin_ctzll(d);
     const size_t n = pp->n1, nl = 2, lw = nl * (n + 1);
     const u128 Q = (u128)pp->q1[0] * pp->q1[1];
-    Reader rd(c, prg);
+    const size_t nlwe = (size_t)(nd + nc + nr) * pp->l_query;
+    Reader rd(c, prg, nlwe * n * 16);
     std::vector<uint64_t> masks(nl * n);
     size_t cnt = 0;
     for (int grp = 0; grp < 3; grp++) {
