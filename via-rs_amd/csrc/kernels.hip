@@ -272,7 +272,8 @@ void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipS
 // at the large output degrees (one or two polynomials per item, where the
 // workgroups rather than the work per workgroup are scarce), 256 otherwise.
 template <int NL, int TPB>
-__global__ __launch_bounds__(TPB) void k_cascade_combine(RingDev r, Gadget g, DigitPlan dp, CombineArgs a) {
+__global__ __launch_bounds__(TPB) void k_cascade_combine(RingDev r, Gadget g, DigitPlan dp, RingDev rq,
+                                                         CombineArgs a) {
     __shared__ double s[lds_words(NL * kSlots)];
     const int item = blockIdx.x, chunk = blockIdx.y, chunks = gridDim.y - 1;
     const int S = kSlots, n = r.n;
@@ -299,6 +300,9 @@ __global__ __launch_bounds__(TPB) void k_cascade_combine(RingDev r, Gadget g, Di
             double *mo = a.masks_next + (size_t)item * NL * S + c0 + c;
             mo[0] = x0;
             if (NL == 2) mo[S] = x1;
+            // the next step's gadget digits of this coefficient (both limbs
+            // are in hand; rq carries the RNS constants of the q1 ring)
+            if (a.dig_next) write_digits<NL>(x0, x1, rq, g, dp, a.dig_next + (size_t)item * g.L * S + c0 + c, (size_t)S);
         }
         return;
     }
@@ -321,27 +325,27 @@ __global__ __launch_bounds__(TPB) void k_cascade_combine(RingDev r, Gadget g, Di
     }
 }
 
-void launch_cascade_combine(const RingDev &r, const Gadget &g, const DigitPlan &dp, const RingDev &rq,
+void launch_cascade_combine(const RingDev &r, const Gadget &g_in, const DigitPlan &dp, const RingDev &rq,
                             const CombineArgs &a, hipStream_t st) {
     int chunks = kSlots / r.n;
     if (chunks > 8) chunks = 8;
     dim3 grid(a.items, chunks + 1);
     const bool wide = chunks <= 2;
+    // digits of every output coefficient with the scale of launch_poly_digits
+    // (rq: the q1 ring at n1, whose RNS and gadget constants the scale uses)
+    Gadget g = g_in;
+    if (a.dig_next && g.f64 == 0 && g.bl >= ((via_u128)1 << 40) && scale_w_make(rq, g, dp, &g.sw)) g.f64 = 3;
     if (r.nl == 2) {
         if (wide)
-            hipLaunchKernelGGL((k_cascade_combine<2, 1024>), grid, dim3(1024), 0, st, r, g, dp, a);
+            hipLaunchKernelGGL((k_cascade_combine<2, 1024>), grid, dim3(1024), 0, st, r, g, dp, rq, a);
         else
-            hipLaunchKernelGGL((k_cascade_combine<2, 256>), grid, dim3(256), 0, st, r, g, dp, a);
+            hipLaunchKernelGGL((k_cascade_combine<2, 256>), grid, dim3(256), 0, st, r, g, dp, rq, a);
     } else {
         if (wide)
-            hipLaunchKernelGGL((k_cascade_combine<1, 1024>), grid, dim3(1024), 0, st, r, g, dp, a);
+            hipLaunchKernelGGL((k_cascade_combine<1, 1024>), grid, dim3(1024), 0, st, r, g, dp, rq, a);
         else
-            hipLaunchKernelGGL((k_cascade_combine<1, 256>), grid, dim3(256), 0, st, r, g, dp, a);
+            hipLaunchKernelGGL((k_cascade_combine<1, 256>), grid, dim3(256), 0, st, r, g, dp, rq, a);
     }
-    // digits of every output coefficient (rq: the q1 ring at n1, whose RNS
-    // and gadget constants the scale uses)
-    if (a.dig_next)
-        launch_poly_digits(rq, g, dp, a.masks_next, (size_t)r.nl * kSlots, kSlots, kSlots, a.items, a.dig_next, st);
 }
 
 // ---- rlwe_to_rgsw combine (convert.rs:162-180) -----------------------------------
@@ -569,6 +573,28 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
             }
         }
     }
+    // One computed limb (2 slots per thread): the RGSW rows of phase 3 and
+    // the inputs of phase 5 are loaded ahead, so that their latency hides
+    // behind the transforms instead of following them.
+    constexpr int PER = NLC * S / TPB;
+    constexpr bool PF = NLC == 1;
+    double kv[PF ? PER : 1][2][2][2];  // [slot][half][digit][part]
+    if constexpr (PF) {
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            const int t = tid + e * TPB;
+#pragma unroll
+            for (int half = 0; half < 2; half++)
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    if (k >= L) continue;
+                    const double *key =
+                        rgsw + ((size_t)half * L + (L - 1 - k)) * 2 * pw + (size_t)(SPLIT ? lsel : 0) * n + t;
+                    kv[e][half][k][0] = key[0];
+                    kv[e][half][k][1] = key[pw];
+                }
+        }
+    }
     __syncthreads();
     // phase 2: the 2L forward transforms side by side
     {
@@ -578,7 +604,6 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
         ntt_fwd<NLC>(s + lds_words(gi * NLC * S), S, n, r.logn, limbs_fwd(rl), tid - gi * per, per);
     }
     // phase 3: external product sums (both parts) for NLC * S slots per part
-    constexpr int PER = NLC * S / TPB;
     double am[PER], ab[PER];
 #pragma unroll
     for (int e = 0; e < PER; e++) {
@@ -587,13 +612,25 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
         const int c = SPLIT ? t : t - l * S;
         const FMod md = l ? m1 : m0;
         double sm = 0.0, sb = 0.0;
-        for (int half = 0; half < 2; half++)
-            for (int k = 0; k < L; k++) {
-                const double x = s[lds_pad((half * L + k) * NLC * S + t)];
-                const double *key = rgsw + ((size_t)half * L + (L - 1 - k)) * 2 * pw + (size_t)l * n + c;
-                sm += mod_mul_lazy(x, key[0], md);
-                sb += mod_mul_lazy(x, key[pw], md);
-            }
+        if constexpr (PF) {
+#pragma unroll
+            for (int half = 0; half < 2; half++)
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    if (k >= L) continue;
+                    const double x = s[lds_pad((half * L + k) * NLC * S + t)];
+                    sm += mod_mul_lazy(x, kv[e][half][k][0], md);
+                    sb += mod_mul_lazy(x, kv[e][half][k][1], md);
+                }
+        } else {
+            for (int half = 0; half < 2; half++)
+                for (int k = 0; k < L; k++) {
+                    const double x = s[lds_pad((half * L + k) * NLC * S + t)];
+                    const double *key = rgsw + ((size_t)half * L + (L - 1 - k)) * 2 * pw + (size_t)l * n + c;
+                    sm += mod_mul_lazy(x, key[0], md);
+                    sb += mod_mul_lazy(x, key[pw], md);
+                }
+        }
         am[e] = mod_reduce_small(sm, md);
         ab[e] = mod_reduce_small(sb, md);
     }
@@ -603,6 +640,17 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
         const int t = tid + e * TPB;
         s[lds_pad(t)] = am[e];
         s[lds_pad(NLC * S + t)] = ab[e];
+    }
+    double bvp[2][PF ? PER : 1], cvp[2][PF ? PER : 1];
+    if constexpr (PF) {
+#pragma unroll
+        for (int part = 0; part < 2; part++)
+#pragma unroll
+            for (int e = 0; e < PER; e++) {
+                const int t = tid + e * TPB;
+                const int l = SPLIT ? lsel : 0;
+                gate_input<NL>(mode, in, j, part, l, t, n, shift, l ? m1 : m0, bvp[part][e], cvp[part][e]);
+            }
     }
     __syncthreads();
     // phase 4: inverse transforms of the two parts side by side
@@ -622,7 +670,12 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
             const double ni = l ? r.t[1].n_inv : r.t[0].n_inv;
             const double p = mod_mul(s[lds_pad(part * NLC * S + t)], ni, md);
             double bv, cv;
-            gate_input<NL>(mode, in, j, part, l, c, n, shift, md, bv, cv);
+            if constexpr (PF) {
+                bv = bvp[part][e];
+                cv = cvp[part][e];
+            } else {
+                gate_input<NL>(mode, in, j, part, l, c, n, shift, md, bv, cv);
+            }
             const size_t o = (size_t)part * pw + (size_t)l * n + c;
             if (mode == GATE_DMUX) {
                 out[(size_t)(2 * j) * 2 * pw + o] = mod_sub(cv, p, md);
