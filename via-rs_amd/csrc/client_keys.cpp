@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <unordered_map>
 #include <vector>
 
 #include "host_internal.h"
@@ -25,6 +26,8 @@
 using namespace via;
 
 typedef unsigned __int128 u128;
+
+static_assert(__BYTE_ORDER__ == __ORDER_LITTLE_ENDIAN__, "stream bytes are read as little-endian words");
 
 struct via_prg {
     uint8_t seed[kPrgSeedCap];
@@ -141,10 +144,17 @@ struct Reader {
         const int nbytes = (bits + 7) / 8;
         const uint64_t mask = bits == 64 ? ~0ULL : ((1ULL << bits) - 1);
         for (;;) {
-            if (pos + nbytes > base + buf.size() && !more((size_t)nbytes)) return 0;
-            const uint8_t *p = buf.data() + (pos - base);
-            uint64_t v = 0;
-            for (int b = 0; b < nbytes; b++) v |= (uint64_t)p[b] << (8 * b);
+            uint64_t v;
+            if (pos + 8 <= base + buf.size()) {
+                // eight bytes at once (little-endian host): the bit mask
+                // clears every byte past the attempt's nbytes
+                memcpy(&v, buf.data() + (pos - base), 8);
+            } else {
+                if (pos + nbytes > base + buf.size() && !more((size_t)nbytes)) return 0;
+                const uint8_t *p = buf.data() + (pos - base);
+                v = 0;
+                for (int b = 0; b < nbytes; b++) v |= (uint64_t)p[b] << (8 * b);
+            }
             pos += (uint64_t)nbytes;
             v &= mask;
             if (v < bound) return v;
@@ -191,19 +201,28 @@ u128 gadget_g(u128 Q, uint64_t base, uint32_t k) {
 }
 
 // One ring of the key material: degree n, nl limbs, the NTT products of
-// every mask with its secret gathered for one batched launch.
+// every mask with its secret gathered for one batched launch; each secret is
+// stored once and products name it by index.
 struct Group {
     uint32_t n = 0;
     int nl = 0;
     uint64_t q[2] = {0, 0};
-    std::vector<uint64_t> a, b;  // [product][limb][n]
+    std::vector<uint64_t> a;       // masks [product][limb][n]
+    std::vector<uint64_t> keys;    // secrets [key][limb][n]
+    std::vector<uint32_t> kidx;    // product -> secret
+    std::unordered_map<const uint64_t *, uint32_t> key_id;
     size_t count = 0;
-    std::vector<uint64_t> c;     // products, the same layout
+    std::vector<uint64_t> c;       // products [product][limb][n]
 
     size_t add(const uint64_t *mask, const uint64_t *key) {
         const size_t w = (size_t)nl * n;
         a.insert(a.end(), mask, mask + w);
-        b.insert(b.end(), key, key + w);
+        auto it = key_id.find(key);
+        if (it == key_id.end()) {
+            it = key_id.emplace(key, (uint32_t)key_id.size()).first;
+            keys.insert(keys.end(), key, key + w);
+        }
+        kidx.push_back(it->second);
         return count++;
     }
     const uint64_t *prod(size_t i) const { return c.data() + i * (size_t)nl * n; }
@@ -215,16 +234,19 @@ via_status run_group(via_ctx *c, Group &gr) {
     RingDev r;
     via_status s = ctx_ring(c, gr.n, gr.q, gr.nl, &r);
     if (s) return s;
-    const size_t words = gr.a.size();
+    const size_t words = gr.a.size(), kw = gr.keys.size();
     uint64_t *d = nullptr;
-    if (hipMalloc(&d, sizeof(uint64_t) * 3 * words) != hipSuccess) return VIA_E_NOMEM;
+    if (hipMalloc(&d, sizeof(uint64_t) * (2 * words + kw) + sizeof(uint32_t) * gr.count) != hipSuccess)
+        return VIA_E_NOMEM;
+    uint32_t *dk = (uint32_t *)(d + 2 * words + kw);
     s = VIA_OK;
     if (hipMemcpyAsync(d, gr.a.data(), 8 * words, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        hipMemcpyAsync(d + words, gr.b.data(), 8 * words, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        hipMemcpyAsync(d + 2 * words, gr.keys.data(), 8 * kw, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(dk, gr.kidx.data(), 4 * gr.count, hipMemcpyHostToDevice, c->stream) != hipSuccess)
         s = VIA_E_DEVICE;
-    if (!s) launch_polymul_u64(r, d, d + words, d + 2 * words, gr.count * (size_t)gr.nl, 1, c->stream);
+    if (!s) launch_polymul_u64(r, d, d + 2 * words, d + words, gr.count * (size_t)gr.nl, 1, c->stream, dk);
     if (!s && (hipGetLastError() != hipSuccess ||
-               hipMemcpyAsync(gr.c.data(), d + 2 * words, 8 * words, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+               hipMemcpyAsync(gr.c.data(), d + words, 8 * words, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
                hipStreamSynchronize(c->stream) != hipSuccess))
         s = VIA_E_DEVICE;
     (void)hipFree(d);
@@ -265,6 +287,20 @@ void rlev_sample(Reader &rd, Group &gr, const uint64_t *key, const uint64_t *msg
     todo.push_back(std::move(rl));
 }
 
+// x * g mod q for a fixed g < q < 2^63 and x < q (Shoup: gs = floor(g 2^64 / q))
+inline uint64_t mulmod_shoup(uint64_t x, uint64_t g, uint64_t gs, uint64_t q) {
+    const uint64_t t = (uint64_t)(((u128)x * gs) >> 64);
+    const uint64_t r = x * g - t * q;  // exact result in [0, 2q), computed mod 2^64
+    return r >= q ? r - q : r;
+}
+
+// e mod q for the small errors (|e| < q), the general reduction otherwise
+inline uint64_t err_mod(int64_t e, uint64_t q) {
+    if (e >= 0 && (uint64_t)e < q) return (uint64_t)e;
+    if (e < 0 && (uint64_t)(-e) < q) return q - (uint64_t)(-e);
+    return mod_i64(e, q);
+}
+
 void rlev_bodies(const Rlev &rl) {
     const Group &gr = *rl.gr;
     const size_t n = gr.n, w = (size_t)gr.nl * n;
@@ -275,10 +311,13 @@ void rlev_bodies(const Rlev &rl) {
         uint64_t *body = rl.out + (size_t)k * 2 * w + w;
         for (int l = 0; l < gr.nl; l++) {
             const uint64_t q = gr.q[l], gl = (uint64_t)(gk % q);
+            const uint64_t gs = (uint64_t)(((u128)gl << 64) / q);
+            const uint64_t *msg = rl.msg + (size_t)l * n;
+            const int64_t *e = rl.e.data() + (size_t)k * n;
             for (size_t i = 0; i < n; i++) {
-                const uint64_t m = (uint64_t)((u128)rl.msg[(size_t)l * n + i] * gl % q);
-                const uint64_t e = mod_i64(rl.e[(size_t)k * n + i], q);
-                body[(size_t)l * n + i] = addm(addm(p[(size_t)l * n + i], e, q), m, q);
+                uint64_t v = addm(p[(size_t)l * n + i], err_mod(e[i], q), q);
+                if (msg[i]) v = addm(v, mulmod_shoup(msg[i], gl, gs, q), q);
+                body[(size_t)l * n + i] = v;
             }
         }
     }
@@ -368,16 +407,18 @@ via_status via_client_setup(via_ctx *c, const via_params *pp, via_dist key1, via
         gr.q[0] = pp->q1[0];
         gr.q[1] = pp->q1[1];
         gr.a.reserve(rank_in * L * nl * nout);
-        gr.b.reserve(rank_in * L * nl * nout);
+        const size_t k0 = keys.size();
+        for (size_t j = 0; j < m; j++) {  // S projected at slot j (degree nout)
+            keys.emplace_back(nl * nout);
+            project_at((int)nl, n1, nout, sk1, j, keys.back().data());
+        }
         std::vector<uint64_t> src(nl * nin);
         for (size_t idx = 0; idx < rank_in; idx++) {
             const size_t grp = idx / m, j = idx % m;
             project_at((int)nl, n1, nin, sk1, m * grp + j, src.data());
             msgs.emplace_back(nl * nout);
             embed_at((int)nl, nin, nout, src.data(), 0, msgs.back().data());
-            keys.emplace_back(nl * nout);
-            project_at((int)nl, n1, nout, sk1, j, keys.back().data());
-            rlev_sample(rd, gr, keys.back().data(), msgs.back().data(), pp->ck_base, (uint32_t)L, err,
+            rlev_sample(rd, gr, keys[k0 + j].data(), msgs.back().data(), pp->ck_base, (uint32_t)L, err,
                         cascade_key + off, todo);
             off += L * 2 * nl * nout;
         }

@@ -82,8 +82,10 @@ void launch_u64_to_f64(const uint64_t *src, double *dst, size_t n, hipStream_t s
 void launch_f64_to_u64(const double *src, uint64_t *dst, size_t n, hipStream_t s);
 // Negacyclic products c = iNTT(NTT(a) * NTT(b)) of `count` u64 polys per limb
 // group; poly p uses limb (p / limb_group) % r.nl.
+// (bidx: product i's b operand is entry bidx[i] of b, nl polys each, so a
+// shared operand is stored once; null: b holds one entry per product)
 void launch_polymul_u64(const RingDev &r, const uint64_t *a, const uint64_t *b, uint64_t *c, size_t count,
-                        size_t limb_group, hipStream_t s);
+                        size_t limb_group, hipStream_t s, const uint32_t *bidx = nullptr);
 
 // Digits of the LWE masks of `items` LWEs (int16 planes [item][k][n]).
 void launch_lwe_digits(const RingDev &r1, const Gadget &g, const DigitPlan &dp, const double *lwe, int items,

@@ -114,7 +114,7 @@ void launch_f64_to_u64(const double *src, uint64_t *dst, size_t n, hipStream_t s
 // operand in LDS: both operands share the buffer as two "limbs" of one
 // modulus).
 __global__ __launch_bounds__(kThreads) void k_polymul(RingDev r, const uint64_t *a, const uint64_t *b, uint64_t *c,
-                                                      size_t limb_group) {
+                                                      size_t limb_group, const uint32_t *bidx) {
     __shared__ double s[lds_words(2 * kSlots)];
     const int n = r.n;
     const size_t p = blockIdx.x;
@@ -122,7 +122,9 @@ __global__ __launch_bounds__(kThreads) void k_polymul(RingDev r, const uint64_t 
     RingDev r2 = r;
     r2.t[0] = l ? r.t[1] : r.t[0];
     r2.t[1] = r2.t[0];
-    const uint64_t *ga = a + p * n, *gb = b + p * n;
+    // bidx: operand b of product p / nl is b's entry bidx[p / nl] (limb p % nl)
+    const size_t pb = bidx ? (size_t)bidx[p / r.nl] * r.nl + p % r.nl : p;
+    const uint64_t *ga = a + p * n, *gb = b + pb * n;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         s[lds_pad(i)] = (double)ga[i];
         s[lds_pad(n + i)] = (double)gb[i];
@@ -140,8 +142,8 @@ __global__ __launch_bounds__(kThreads) void k_polymul(RingDev r, const uint64_t 
 }
 
 void launch_polymul_u64(const RingDev &r, const uint64_t *a, const uint64_t *b, uint64_t *c, size_t count,
-                        size_t limb_group, hipStream_t st) {
-    if (count) hipLaunchKernelGGL(k_polymul, dim3(count), dim3(kThreads), 0, st, r, a, b, c, limb_group);
+                        size_t limb_group, hipStream_t st, const uint32_t *bidx) {
+    if (count) hipLaunchKernelGGL(k_polymul, dim3(count), dim3(kThreads), 0, st, r, a, b, c, limb_group, bidx);
 }
 
 }  // namespace via
