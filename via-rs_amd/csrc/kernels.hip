@@ -12,9 +12,10 @@ namespace via {
 // Gadget digits of `items` coefficient vectors of n coefficients: item i,
 // limb l, coefficient c at src[i*item_stride + l*limb_stride + c], digits to
 // int16 planes dig[i][k][n]. One thread per coefficient over the whole grid
-// (the wide gadget scale dominates; the combine kernels that produce the
-// inputs run one workgroup per polynomial). Used for the LWE masks (the
-// rank-n degree-1 input of the first conv step) and every cascade output.
+// (the wide gadget scale dominates). Used for the LWE masks (the rank-n
+// degree-1 input of the first conv step); the digits of every cascade
+// output come from k_cascade_combine, which computes the same values with
+// the same scale in the workgroup that produces the coefficient.
 template <int NL>
 __global__ __launch_bounds__(256) void k_poly_digits(RingDev r1, Gadget g, DigitPlan dp, const double *src,
                                                      size_t item_stride, size_t limb_stride, int n, int16_t *dig) {
@@ -262,8 +263,9 @@ void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipS
 }
 
 // Cascade combine: masks_next = -iNTT(sum of mask partials), body_next =
-// embed(body_prev) - iNTT(sum of body partials); the digits of masks_next for
-// the next gadget product follow in k_poly_digits. Workgroup (item, chunk)
+// embed(body_prev) - iNTT(sum of body partials); each thread that writes a
+// mask coefficient (both limbs) also writes its digits for the next gadget
+// product into dig_next (when set). Workgroup (item, chunk)
 // handles the chunk's mask polynomials (chunks of at least one degree-n
 // polynomial). The partial sums of all digit groups and both tiles are added lazily
 // (canonical terms, at most 2 * 18 of them: below 2^44) and brought into
