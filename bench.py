@@ -104,6 +104,33 @@ def synthetic_inputs(pp, seed=2048):
     return cascade, conv, rsk, lwes, record_bytes, n_rec
 
 
+def seeded_inputs(via_gpu, ctx, pp, tag):
+    """The workload's inputs from Shake256Prg streams, all drawn by the
+    library: the records are fill_bytes of b"via-gpu/<tag>/db" (record
+    coefficient 2k = byte k & 0xF, 2k+1 = byte k >> 4), the keys Client::setup
+    over b"via-gpu/<tag>/setup" (ternary keys and errors), the index
+    uniform_below(n_records) of b"via-gpu/<tag>/query" and the query
+    Client::query continuing the setup's stream."""
+    n_rec = (pp.n1 // pp.n_rec) * pp.num_rows * pp.num_cols
+    g_db = via_gpu.Prg(b"via-gpu/%s/db" % tag.encode())
+    record_bytes = g_db.fill(ctx, n_rec * pp.n_rec // 2)
+    g_db.close()
+    g_setup = via_gpu.Prg(b"via-gpu/%s/setup" % tag.encode())
+    keys = via_gpu.client_setup(ctx, pp, g_setup)
+    g_q = via_gpu.Prg(b"via-gpu/%s/query" % tag.encode())
+    index = g_q.uniform_below(ctx, n_rec)
+    lwes = via_gpu.client_query(ctx, pp, keys["sk1"], index, g_setup)
+    return keys, lwes, record_bytes, n_rec, index, (g_setup, g_q)
+
+
+def record_of(record_bytes, pp, index):
+    b = record_bytes[index * pp.n_rec // 2:(index + 1) * pp.n_rec // 2].astype(np.uint64)
+    rec = np.empty(pp.n_rec, np.uint64)
+    rec[0::2] = b & 0xF
+    rec[1::2] = b >> 4
+    return rec
+
+
 def cpu_baseline(pp, cascade, conv, rsk, lwes, record_bytes, n_rec):
     """The C restatement (oracle/) on one answer of the same workload, with
     FirstDim fanned out over the host's cores as the reference does."""
@@ -441,13 +468,20 @@ def keygen_run(via_gpu, args):
 
 def batched_run(via_gpu, ctx, pp, db, Q, args, seed=31):
     """Q queries from Q distinct clients answered per pass (via_answer_many):
-    each client has its own device copy of its keys (key material cycles
-    through eight synthetic sets; the work does not depend on the residues).
+    client q's keys are Client::setup over b"via-gpu/<tag>/batch/setup/<q>",
+    its query Client::query at an index drawn from b"via-gpu/<tag>/batch/query".
     Autotuned for Q, warmed, then `steps` timed passes."""
-    rng = np.random.default_rng(seed)
-    sets = [synthetic_keys(pp, rng) for _ in range(min(8, Q))]
-    keys = [via_gpu.PreparedKeys(ctx, pp, *sets[q % len(sets)]) for q in range(Q)]
-    queries = [synthetic_query(pp, rng) for _ in range(Q)]
+    tag = f"paper-{pp.num_rows}x{pp.num_cols}".encode()
+    n_rec = (pp.n1 // pp.n_rec) * pp.num_rows * pp.num_cols
+    g_q = via_gpu.Prg(b"via-gpu/%s/batch/query" % tag)
+    keys, queries = [], []
+    for q in range(Q):
+        g = via_gpu.Prg(b"via-gpu/%s/batch/setup/%d" % (tag, q))
+        k = via_gpu.client_setup(ctx, pp, g)
+        keys.append(via_gpu.PreparedKeys(ctx, pp, k["cascade_key"], k["conv_key"], k["rsk"]))
+        queries.append(via_gpu.client_query(ctx, pp, k["sk1"], g_q.uniform_below(ctx, n_rec), g))
+        g.close()
+    g_q.close()
     d_q = [ctx.alloc(q.nbytes) for q in queries]
     for d, q in zip(d_q, queries):
         ctx.to_device(d, q)
@@ -536,9 +570,10 @@ def main():
         keygen_run(via_gpu, args)
         return
     pp = via_gpu.paper_params(args.rows, args.cols)
-    cascade, conv, rsk, lwes, record_bytes, n_rec = synthetic_inputs(pp)
-
     ctx = via_gpu.Context(0)
+    tag = f"paper-{args.rows}x{args.cols}"
+    ckeys, lwes, record_bytes, n_rec, index, prgs = seeded_inputs(via_gpu, ctx, pp, tag)
+    cascade, conv, rsk = ckeys["cascade_key"], ckeys["conv_key"], ckeys["rsk"]
     t0 = time.perf_counter()
     db = via_gpu.PreparedDb(ctx, pp, record_bytes=record_bytes.tobytes(), n_records=n_rec)
     keys = via_gpu.PreparedKeys(ctx, pp, cascade, conv, rsk)
@@ -587,9 +622,14 @@ def main():
 
     L = via_gpu.lib()
     Q = args.batch
-    rng = np.random.default_rng(99)
-    client_keys = [keys] + [via_gpu.PreparedKeys(ctx, pp, *synthetic_keys(pp, rng)) for _ in range(Q - 1)]
-    queries = [lwes] + [synthetic_query(pp, rng) for _ in range(Q - 1)]
+    # clients 1 .. Q-1: their own Client::setup and a query at a seeded index
+    client_keys, queries = [keys], [lwes]
+    for q in range(1, Q):
+        g = via_gpu.Prg(b"via-gpu/%s/setup/%d" % (tag.encode(), q))
+        k = via_gpu.client_setup(ctx, pp, g)
+        client_keys.append(via_gpu.PreparedKeys(ctx, pp, k["cascade_key"], k["conv_key"], k["rsk"]))
+        queries.append(via_gpu.client_query(ctx, pp, k["sk1"], prgs[1].uniform_below(ctx, n_rec), g))
+        g.close()
     d_q = [ctx.alloc(q.nbytes) for q in queries]
     for d, q in zip(d_q, queries):
         ctx.to_device(d, q)
@@ -660,7 +700,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int (exact modular, residues < 2^38 in fp64 lanes)",
-        "data": "synthetic: uniform random 256-byte records, uniform key and query residues",
+        "data": f"seeded: records fill_bytes of Shake256Prg(b'via-gpu/{tag}/db'), keys Client::setup and "
+                f"queries Client::query of b'via-gpu/{tag}/setup' (ternary), index {index} = uniform_below of "
+                f"b'via-gpu/{tag}/query'",
+        "recovered": bool(np.array_equal(via_gpu.client_recover(pp, ckeys["sk2"], gpu_mask, gpu_body),
+                                         record_of(record_bytes, pp, index))),
         "config": {
             "workload": f"VIA-C paper stack, {Q} quer{'y' if Q == 1 else 'ies from distinct clients'} per pass, "
                         f"I={I} J={J}, {n_rec} records x 256 B, "

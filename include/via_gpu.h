@@ -267,6 +267,11 @@ typedef struct via_prg via_prg;
 via_status via_prg_create(const uint8_t *seed, size_t seed_len, via_prg **out);
 void via_prg_destroy(via_prg *prg);
 uint64_t via_prg_position(const via_prg *prg);
+/* Shake256Prg::fill_bytes (prg.rs:143-158): the next len bytes of the
+ * stream (computed on the GPU). */
+via_status via_prg_fill(via_ctx *ctx, via_prg *prg, uint8_t *out, size_t len);
+/* Shake256Prg::uniform_below (prg.rs:184-207): one draw below bound (>= 1). */
+via_status via_prg_uniform_below(via_ctx *ctx, via_prg *prg, uint64_t bound, uint64_t *out);
 
 /* Words of the client's keys: cascade key (log2(n1) steps of 2*n1 RLev
  * samples' worth, [k][part][limb][n_out] per key), conversion key

@@ -118,6 +118,22 @@ def test_via_b_batch_queries_match(ctx, oracle):
                               oracle_query(oracle, opp, want["sk1"], idx, ODist(0, 0), og)), idx
 
 
+def test_prg_fill_and_uniform_below_match(ctx, oracle):
+    # Shake256Prg::fill_bytes and uniform_below (prg.rs:143-207) interleaved,
+    # across GPU stream windows (5 MB > one 4.25 MiB window)
+    seed = b"via-gpu/prg/fill"
+    og = oracle.prg(seed)
+    g = via_gpu.Prg(seed)
+    for step, (kind, arg) in enumerate([("fill", 100), ("draw", (1 << 37) + 12345), ("fill", 5_000_000),
+                                        ("draw", 3), ("draw", 1), ("fill", 7), ("draw", 8380417)]):
+        if kind == "fill":
+            want = ctypes.create_string_buffer(arg)
+            oracle.L.vo_prg_fill(og.h, want, arg)
+            assert g.fill(ctx, arg).tobytes() == want.raw, step
+        else:
+            assert g.uniform_below(ctx, arg) == oracle.L.vo_prg_uniform_below(og.h, arg), step
+
+
 def test_library_client_end_to_end(ctx):
     # keys and queries from the library, answered on the GPU, recovered by
     # the library: every queried record comes back

@@ -353,6 +353,34 @@ bool dist_ok(const via_dist &d) { return d.kind == 0 || d.kind == 1; }
 
 }  // namespace
 
+via_status via_prg_fill(via_ctx *c, via_prg *prg, uint8_t *out, size_t len) {
+    if (!c || !prg || (len && !out)) return VIA_E_INVALID;
+    if (!len) return VIA_OK;
+    if (hipSetDevice(c->device) != hipSuccess) return VIA_E_DEVICE;
+    Reader rd(c, prg, std::min(len, kWindowBlocks * kPrgBlock));
+    size_t done = 0;
+    while (done < len) {
+        const size_t take = std::min(len - done, kWindowBlocks * kPrgBlock / 2);
+        if (!rd.more(take)) return rd.err ? rd.err : VIA_E_DEVICE;
+        memcpy(out + done, rd.buf.data() + (rd.pos - rd.base), take);
+        rd.pos += take;
+        done += take;
+    }
+    rd.commit();
+    return VIA_OK;
+}
+
+via_status via_prg_uniform_below(via_ctx *c, via_prg *prg, uint64_t bound, uint64_t *out) {
+    if (!c || !prg || !out || !bound) return VIA_E_INVALID;
+    if (hipSetDevice(c->device) != hipSuccess) return VIA_E_DEVICE;
+    Reader rd(c, prg, 64);
+    const uint64_t v = rd.uniform_below(bound);
+    if (rd.err) return rd.err;
+    rd.commit();
+    *out = v;
+    return VIA_OK;
+}
+
 via_status via_client_key_words(const via_params *pp, size_t *cascade, size_t *conv, size_t *rsk) {
     if (!pp || !pp->n1 || !pp->n2 || !pp->q1_nl) return VIA_E_INVALID;
     const size_t n = pp->n1, nl = pp->q1_nl, logn = (size_t)__builtin_ctz(pp->n1);

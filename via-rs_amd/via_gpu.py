@@ -136,6 +136,8 @@ def lib():
             "via_prg_create": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(vp)]),
             "via_prg_destroy": (None, [vp]),
             "via_prg_position": (C.c_uint64, [vp]),
+            "via_prg_fill": (C.c_int, [vp, vp, C.c_void_p, C.c_size_t]),
+            "via_prg_uniform_below": (C.c_int, [vp, vp, C.c_uint64, C.POINTER(C.c_uint64)]),
             "via_client_key_words": (C.c_int, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
                                                C.POINTER(C.c_size_t)]),
             "via_client_setup": (C.c_int, [vp, P, Dist, Dist, Dist, vp, u64p, u64p, u64p, u64p, u64p]),
@@ -244,6 +246,17 @@ class Prg:
     @property
     def position(self):
         return lib().via_prg_position(self.h)
+
+    def fill(self, ctx, n):
+        """fill_bytes: the next n stream bytes as uint8."""
+        out = np.zeros(n, np.uint8)
+        _check(lib().via_prg_fill(ctx.h, self.h, out.ctypes.data_as(C.c_void_p), n))
+        return out
+
+    def uniform_below(self, ctx, bound):
+        v = C.c_uint64()
+        _check(lib().via_prg_uniform_below(ctx.h, self.h, bound, C.byref(v)))
+        return v.value
 
     def close(self):
         if self.h:
