@@ -262,7 +262,9 @@ typedef struct {
 
 /* Shake256Prg (sampling/prg.rs): the stream of a seed of at most 64 bytes
  * (VIA_E_PRG_SEED beyond), consumed by via_client_setup and
- * via_client_query in the reference's draw order; position = bytes used. */
+ * via_client_query in the reference's draw order; position = bytes used.
+ * One call at a time per via_prg (it is a sequential stream); a failed call
+ * leaves its position unchanged. */
 typedef struct via_prg via_prg;
 via_status via_prg_create(const uint8_t *seed, size_t seed_len, via_prg **out);
 void via_prg_destroy(via_prg *prg);
@@ -285,7 +287,8 @@ via_status via_client_key_words(const via_params *pp, size_t *cascade, size_t *c
  * the GPU. VIA_E_DIM for a non-power-of-two num_rows or num_cols
  * (client.rs:125-131); VIA_E_INVALID for parameters outside the device path
  * (n1 = 2048 over two NTT primes, q3 NTT-friendly at n2). The PRG advances
- * only on success. Host buffers. */
+ * only on success; on an error the output buffers hold partial values.
+ * Host buffers. */
 via_status via_client_setup(via_ctx *ctx, const via_params *pp, via_dist key1, via_dist key2, via_dist err,
                             via_prg *prg, uint64_t *sk1, uint64_t *sk2, uint64_t *cascade_key, uint64_t *conv_key,
                             uint64_t *rsk);
@@ -294,7 +297,8 @@ via_status via_client_setup(via_ctx *ctx, const via_params *pp, via_dist key1, v
  * 75-130): the query's raw LWEs ([lwe][limb][n1 + 1], bits in DMux, CMux,
  * CRot order, L_QUERY per bit) under S1, drawn from prg. VIA-B batch queries
  * (client.rs:256-300) are T calls in batch order with n_rec the record
- * degree. VIA_E_INDEX for index >= (n1/n_rec) * num_rows * num_cols. */
+ * degree. VIA_E_INDEX for index >= (n1/n_rec) * num_rows * num_cols,
+ * VIA_E_DIM when num_rows, num_cols or n1/n_rec is not a power of two. */
 via_status via_client_query(via_ctx *ctx, const via_params *pp, const uint64_t *sk1, size_t index, via_dist err,
                             via_prg *prg, uint64_t *lwes);
 

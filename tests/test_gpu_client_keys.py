@@ -162,6 +162,9 @@ def test_client_errors(ctx):
     with pytest.raises(via_gpu.ViaError) as e:  # client.rs:125-131
         via_gpu.client_setup(ctx, bad, g)
     assert e.value.status == via_gpu.VIA_E_DIM
+    with pytest.raises(via_gpu.ViaError) as e:  # no bit decomposition of 3 rows
+        via_gpu.client_query(ctx, bad, sk1, 0, g)
+    assert e.value.status == via_gpu.VIA_E_DIM
     with pytest.raises(via_gpu.ViaError) as e:  # the Gaussian arm
         via_gpu.client_setup(ctx, pp, g, err=via_gpu.Dist(2, 0))
     assert e.value.status == via_gpu.VIA_E_INVALID

@@ -505,8 +505,10 @@ via_status via_client_query(via_ctx *c, const via_params *pp, const uint64_t *sk
     if (!c || !pp || !sk1 || !prg || !lwes) return VIA_E_INVALID;
     via_status s = check_client(pp);
     if (s) return s;
-    if (!dist_ok(err) || !pp->l_query || !pp->num_rows || !pp->num_cols) return VIA_E_INVALID;
+    if (!dist_ok(err) || !pp->l_query) return VIA_E_INVALID;
     const size_t I = pp->num_rows, J = pp->num_cols, d = pp->n1 / pp->n_rec;
+    // the bit decomposition needs powers of two (Client::setup refuses others)
+    if (!I || !J || (I & (I - 1)) || (J & (J - 1)) || (d & (d - 1))) return VIA_E_DIM;
     if (index >= d * I * J) return VIA_E_INDEX;  // client.rs:190-195
     if (hipSetDevice(c->device) != hipSuccess) return VIA_E_DEVICE;
     // decompose_index (decompose.rs:25-54): gamma, alpha, beta; DMux bits MSB
