@@ -161,25 +161,86 @@ struct Reader {
         }
     }
 
+    // Runs of attempts of nbytes each until `total` are accepted: take(v)
+    // masks and tests attempt v (the eight stream bytes at its position;
+    // the bit mask drops the bytes past nbytes), stores it in its slot
+    // whether or not it is accepted (a rejected one is overwritten by the
+    // next attempt) and returns 1 on acceptance. No branch depends on the
+    // stream: rejections are frequent (half the attempts for q0 just above
+    // 2^37) and would mispredict.
+    template <class F>
+    void run(int nbytes, size_t total, F take) {
+        size_t done = 0;
+        while (done < total) {
+            if (pos + 8 > base + buf.size() && !more(8)) return;
+            const uint64_t end = base + buf.size();
+            const size_t avail = (size_t)((end - pos - 8) / (uint64_t)nbytes) + 1;
+            const uint8_t *p = buf.data() + (pos - base);
+            size_t a = 0;
+            for (; a < avail && done < total; a++) {
+                uint64_t v;
+                memcpy(&v, p + a * (size_t)nbytes, 8);
+                done += take(v, done);
+            }
+            pos += (uint64_t)a * (uint64_t)nbytes;
+        }
+    }
+
     // Distribution::sample_into (sampling/distribution.rs:88-115): ternary
     // {0, 1, 2} -> {0, 1, -1}; bounded uniform uniform_below(2b + 1) - b.
     void sample(const via_dist &dist, int64_t *out, size_t n) {
         if (dist.kind == 0) {
-            for (size_t i = 0; i < n; i++) {
-                const uint64_t u = uniform_below(3);
-                out[i] = u == 0 ? 0 : (u == 1 ? 1 : -1);
-            }
-        } else {
-            const uint64_t range = 2ULL * dist.bound + 1;
-            for (size_t i = 0; i < n; i++) out[i] = (int64_t)uniform_below(range) - (int64_t)dist.bound;
+            static const int64_t tern[4] = {0, 1, -1, 0};
+            run(1, n, [&](uint64_t v, size_t d) -> size_t {
+                v &= 3;
+                out[d] = tern[v];
+                return v < 3;
+            });
+            return;
         }
+        const uint64_t range = 2ULL * dist.bound + 1;
+        if (range <= 1) {  // uniform_below(1) draws nothing
+            for (size_t i = 0; i < n; i++) out[i] = 0;
+            return;
+        }
+        const int bits = 64 - __builtin_clzll(range);
+        const uint64_t mask = bits == 64 ? ~0ULL : ((1ULL << bits) - 1);
+        const int64_t shift = (int64_t)dist.bound;
+        run((bits + 7) / 8, n, [&](uint64_t v, size_t d) -> size_t {
+            v &= mask;
+            out[d] = (int64_t)v - shift;
+            return v < range;
+        });
     }
 
     // RingPoly::random_uniform (element.rs:235, rns_element.rs:201-212): per
-    // coefficient, one draw per limb, limb 0 first; out [limb][n].
+    // coefficient, one draw per limb, limb 0 first; out [limb][n]. Draw d is
+    // coefficient d / nl of limb d % nl.
     void poly_random(const uint64_t *q, int nl, size_t n, uint64_t *out) {
-        for (size_t i = 0; i < n; i++)
-            for (int l = 0; l < nl; l++) out[(size_t)l * n + i] = uniform_below(q[l]);
+        const int bits0 = 64 - __builtin_clzll(q[0]);
+        const int bits1 = nl == 2 ? 64 - __builtin_clzll(q[1]) : bits0;
+        if (nl > 2 || q[0] < 2 || (nl == 2 && q[1] < 2) || (bits0 + 7) / 8 != (bits1 + 7) / 8) {
+            for (size_t i = 0; i < n; i++)
+                for (int l = 0; l < nl; l++) out[(size_t)l * n + i] = uniform_below(q[l]);
+            return;
+        }
+        const uint64_t m0 = bits0 == 64 ? ~0ULL : ((1ULL << bits0) - 1);
+        const uint64_t m1 = bits1 == 64 ? ~0ULL : ((1ULL << bits1) - 1);
+        if (nl == 1) {
+            run((bits0 + 7) / 8, n, [&](uint64_t v, size_t d) -> size_t {
+                v &= m0;
+                out[d] = v;
+                return v < q[0];
+            });
+            return;
+        }
+        const uint64_t qa = q[0], qb = q[1];
+        run((bits0 + 7) / 8, 2 * n, [&](uint64_t v, size_t d) -> size_t {
+            const size_t l = d & 1;
+            v &= l ? m1 : m0;
+            out[l * n + (d >> 1)] = v;
+            return v < (l ? qb : qa);
+        });
     }
 };
 
