@@ -30,6 +30,26 @@ HBM_PEAK_GBS = 8000.0
 # KiB on this part).
 FD_TRAFFIC = {(256, 256, 1): (2.0 * 337981.8 + 8192.0) * 1024}
 
+# Modular multiplications (NTT butterflies, pointwise and scaling products
+# counted alike) of one answer's steps at the paper stack, counted from the
+# reference's loops (cascade + RLWE -> RGSW, the I - 1 DMux external
+# products at q1, the CMux and CRot trees at q2, RespComp), per (I, J).
+STEP_MODMUL = {
+    (128, 128): {"query_decomp": 296e6, "dmux": 49.9e6, "cmux_crot": 25.4e6, "resp_comp": 0.12e6},
+    (256, 256): {"query_decomp": 333e6, "dmux": 100.3e6, "cmux_crot": 50.5e6, "resp_comp": 0.12e6},
+}
+
+
+def step_work(counts, step_s, Q):
+    """Per-step modmul-equivalents of the pass (Q queries) and the rate the
+    step's measured time gives them (G/s), where the counts are known."""
+    if not counts:
+        return None
+    t = dict(zip(["query_decomp", "dmux", "mod_switch", "first_dim", "cmux", "crot", "resp_comp"], step_s))
+    t["cmux_crot"] = t["cmux"] + t["crot"]
+    return {k: {"modmul_eq": c * Q, "ms": round(t[k] * 1e3, 3), "G_per_s": round(c * Q / t[k] / 1e9, 1)}
+            for k, c in counts.items() if t[k] > 0}
+
 
 def fd_plan_bytes(I, J, n, Q):
     """Algorithmic bytes of one k_fd_mfma launch: the prepared database as
@@ -716,6 +736,7 @@ def main():
             ["query_decomp", "dmux", "mod_switch", "first_dim", "cmux", "crot", "resp_comp"], steps_acc)},
         "setup_s": round(t_setup, 3),
         "tuned_digit_groups": tuned,
+        "step_work": step_work(STEP_MODMUL.get((I, J)), steps_acc / args.steps, Q),
         "roofline": {
             "kernel": "k_fd_mfma (database limb stream, int8 MFMA multiply-accumulate)",
             "bound": "hbm",
