@@ -198,24 +198,30 @@ def tune_batches(via_gpu, ctx, pp, db, keys, sizes, seed=11):
     return out
 
 
-def serve_setup(via_gpu, pp, db, n_clients, max_batch, slots, seed=7, db_b=None, key_sets=8):
+def serve_setup(via_gpu, ctx, pp, db, n_clients, max_batch, slots, db_b=None):
     """A serving loop with n_clients admitted clients (their eval keys
     resident in HBM under the loop's key budget) and one VIA-C query (and,
     with db_b, one VIA-B batch of T = n2 / n_rec(db_b) queries) per client.
-    Key material cycles through `key_sets` distinct synthetic sets: every
-    client still gets its own device copy, and the work of an answer does not
-    depend on the residues."""
-    rng = np.random.default_rng(seed)
+    Client c's keys are Client::setup over Shake256Prg(b"via-gpu/serve/setup/<c>")
+    and its queries Client::query continuing that stream, at indices drawn
+    from b"via-gpu/serve/query"."""
     kpp = db_b.pp if db_b is not None else pp
     T = pp.n2 // kpp.n_rec if db_b is not None else 0
     loop = via_gpu.ServeLoop(pp, db, key_budget_bytes=64 << 30, max_batch=max_batch, slots=slots, db_b=db_b)
-    sets = [synthetic_keys(kpp, rng) for _ in range(min(key_sets, n_clients))]
+    n_c = (pp.n1 // pp.n_rec) * pp.num_rows * pp.num_cols
+    n_b = (kpp.n1 // kpp.n_rec) * kpp.num_rows * kpp.num_cols
+    g_q = via_gpu.Prg(b"via-gpu/serve/query")
     queries, bqueries = [], []
     for c in range(n_clients):
-        loop.admit(c, *sets[c % len(sets)])
-        queries.append(synthetic_query(pp, rng))
+        g = via_gpu.Prg(b"via-gpu/serve/setup/%d" % c)
+        k = via_gpu.client_setup(ctx, kpp, g)  # the keys do not depend on the record degree
+        loop.admit(c, k["cascade_key"], k["conv_key"], k["rsk"])
+        queries.append(via_gpu.client_query(ctx, pp, k["sk1"], g_q.uniform_below(ctx, n_c), g))
         if db_b is not None:
-            bqueries.append([synthetic_query(kpp, rng) for _ in range(T)])
+            bqueries.append([via_gpu.client_query(ctx, kpp, k["sk1"], g_q.uniform_below(ctx, n_b), g)
+                             for _ in range(T)])
+        g.close()
+    g_q.close()
     # warm every slot once
     warm = [loop.submit(c % n_clients, queries[c % n_clients]) for c in range(2 * max_batch * slots)]
     if db_b is not None:
@@ -608,13 +614,15 @@ def main():
         if args.mix:
             pb = via_gpu.paper_b_params(args.rows, args.cols, T=8)
             nb_rec = (pb.n1 // pb.n_rec) * pb.num_rows * pb.num_cols
-            rb = np.random.default_rng(4097).integers(0, 256, size=nb_rec * pb.n_rec // 2, dtype=np.uint8)
+            g_b = via_gpu.Prg(b"via-gpu/serve/db-b")
+            rb = g_b.fill(ctx, nb_rec * pb.n_rec // 2)
+            g_b.close()
             db_b = via_gpu.PreparedDb(ctx, pb, record_bytes=rb.tobytes(), n_records=nb_rec)
             del rb
         tuned = None
         if not args.no_tune:
             tuned = tune_batches(via_gpu, ctx, pp, db, keys, [q for q in (1, 2, 4, 8, 16, 32) if q <= args.batch])
-        loop, queries, bqueries, T = serve_setup(via_gpu, pp, db, args.clients, max_batch=args.batch, slots=2,
+        loop, queries, bqueries, T = serve_setup(via_gpu, ctx, pp, db, args.clients, max_batch=args.batch, slots=2,
                                                  db_b=db_b)
         runs = []
         for r in rates:
@@ -628,7 +636,9 @@ def main():
             "warmup": 0, "ms_per_step": round(1e3 / best["answers_per_s"], 3),
             "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "int (exact modular, residues < 2^38 in fp64 lanes)",
-            "data": "synthetic: uniform random records, uniform key and query residues",
+            "data": f"seeded: records fill_bytes of Shake256Prg(b'via-gpu/{tag}/db') (and b'via-gpu/serve/db-b'), "
+                    f"client c's keys Client::setup of b'via-gpu/serve/setup/<c>' and its queries Client::query, "
+                    f"indices uniform_below of b'via-gpu/serve/query'",
             "config": {"workload": (f"serving loop, Poisson arrivals from {args.clients} clients, "
                                     + ("70% VIA-C queries / 30% VIA-B requests of 4..64 records in T=8 batches, "
                                        if args.mix else "VIA-C queries, ")
