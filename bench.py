@@ -1,11 +1,14 @@
 """VIA-C server answer benchmark on one MI355X.
 
 A step is one Server::answer of the paper VIA-C stack (n1=2048, two-prime
-q1, q2~2^34, q3~2^23) over a synthetic database of uniform random 256-byte
-records, with the query and the answer buffers resident in HBM. Prints one
-JSON line: answers/s, p50/p99 answer latency, the FirstDim roofline (HBM
-bytes per launch from live HIP events) and the CPU restatement timed on one
-answer of the same workload, whose answer must equal the GPU's bit for bit.
+q1, q2~2^34, q3~2^23) over a database of 256-byte records drawn from a
+Shake256Prg stream, with keys from Client::setup and the query from
+Client::query (the library's own client path), query and answer buffers
+resident in HBM. Prints one JSON line: answers/s, p50/p99 answer latency,
+per-step times and rates, the FirstDim roofline (HBM bytes per launch from
+live HIP events), whether the answer recovers the record, and the CPU
+restatement timed on one answer of the same workload, whose answer must
+equal the GPU's bit for bit.
 
     python bench.py [--steps K] [--warmup W] [--rows I] [--cols J] [--no-cpu]
 """
@@ -62,29 +65,6 @@ def fd_plan_bytes(I, J, n, Q):
     return ipad * jpad * n * 5 + n * ipad * 5 * 2 * Q + nseg * n * Q * J * 2 * 8
 
 
-def synthetic_keys(pp, rng):
-    """One client's public keys (cascade, conversion, ring switch) as uniform
-    residues in the layouts of include/via_gpu.h."""
-    q0, q1 = pp.q1[0], pp.q1[1]
-    n1, L = pp.n1, pp.l_ck
-
-    def limbs(words, period):
-        idx = (np.arange(words) // period) % 2
-        out = rng.integers(0, q0, size=words, dtype=np.uint64)
-        out[idx == 1] = rng.integers(0, q1, size=int((idx == 1).sum()), dtype=np.uint64)
-        return out
-
-    steps = []
-    nout = 2
-    while nout <= n1:
-        steps.append(limbs((2 * n1 // nout) * L * 2 * 2 * nout, nout))
-        nout *= 2
-    cascade = np.concatenate(steps)
-    conv = limbs(L * 2 * 2 * n1, n1)
-    rsk = rng.integers(0, pp.q3, size=(n1 // pp.n2) * pp.l_rsk * 2 * pp.n2, dtype=np.uint64)
-    return cascade, conv, rsk
-
-
 def synthetic_query(pp, rng):
     q0, q1 = pp.q1[0], pp.q1[1]
     words = pp.query_lwes * 2 * (pp.n1 + 1)
@@ -92,36 +72,6 @@ def synthetic_query(pp, rng):
     out = rng.integers(0, q0, size=words, dtype=np.uint64)
     out[idx == 1] = rng.integers(0, q1, size=int((idx == 1).sum()), dtype=np.uint64)
     return out
-
-
-def synthetic_inputs(pp, seed=2048):
-    """Uniform residues in the layouts of include/via_gpu.h. The answer path
-    is exact integer arithmetic whose work does not depend on the values, so
-    uniform keys and LWEs exercise it exactly as real ones do."""
-    rng = np.random.default_rng(seed)
-    q0, q1 = pp.q1[0], pp.q1[1]
-    n1, L = pp.n1, pp.l_ck
-
-    def limbs(words, period):
-        out = np.empty(words, dtype=np.uint64)
-        idx = (np.arange(words) // period) % 2
-        out[idx == 0] = rng.integers(0, q0, size=int((idx == 0).sum()), dtype=np.uint64)
-        out[idx == 1] = rng.integers(0, q1, size=int((idx == 1).sum()), dtype=np.uint64)
-        return out
-
-    steps = []
-    nout = 2
-    while nout <= n1:
-        steps.append(limbs((2 * n1 // nout) * L * 2 * 2 * nout, nout))
-        nout *= 2
-    cascade = np.concatenate(steps)
-    conv = limbs(L * 2 * 2 * n1, n1)
-    d = n1 // pp.n2
-    rsk = rng.integers(0, pp.q3, size=d * pp.l_rsk * 2 * pp.n2, dtype=np.uint64)
-    lwes = limbs(pp.query_lwes * 2 * (n1 + 1), n1 + 1)
-    n_rec = 4 * pp.num_rows * pp.num_cols
-    record_bytes = rng.integers(0, 256, size=n_rec * pp.n_rec // 2, dtype=np.uint8)
-    return cascade, conv, rsk, lwes, record_bytes, n_rec
 
 
 def seeded_inputs(via_gpu, ctx, pp, tag):
@@ -288,25 +238,36 @@ def vb_run(via_gpu, args):
     T = args.vb
     pp = via_gpu.paper_b_params(args.rows, args.cols, T=T)
     I, J, n = pp.num_rows, pp.num_cols, pp.n1
-    rng = np.random.default_rng(4096)
     ctx = via_gpu.Context(0)
     n_rec = (n // pp.n_rec) * I * J
-    record_bytes = rng.integers(0, 256, size=n_rec * pp.n_rec // 2, dtype=np.uint8)
+    tag = f"vb{T}-{I}x{J}".encode()
+    # records, keys, distinct indices and batch queries from seeded streams
+    g_db = via_gpu.Prg(b"via-gpu/%s/db" % tag)
+    record_bytes = g_db.fill(ctx, n_rec * pp.n_rec // 2)
+    g_db.close()
+    g = via_gpu.Prg(b"via-gpu/%s/setup" % tag)
+    ck = via_gpu.client_setup(ctx, pp, g)
     t0 = time.perf_counter()
     db = via_gpu.PreparedDb(ctx, pp, record_bytes=record_bytes.tobytes(), n_records=n_rec)
-    del record_bytes
-    keys = via_gpu.PreparedKeys(ctx, pp, *synthetic_keys(pp, rng))
+    keys = via_gpu.PreparedKeys(ctx, pp, ck["cascade_key"], ck["conv_key"], ck["rsk"])
     t_setup = time.perf_counter() - t0
     nb = max(1, args.indices // T)
-    lw = pp.query_lwes * 2 * (n + 1)
+    g_q = via_gpu.Prg(b"via-gpu/%s/query" % tag)
+    idx = []
+    while len(idx) < nb * T:
+        i = g_q.uniform_below(ctx, n_rec)
+        if i not in idx:
+            idx.append(i)
+    g_q.close()
     d_q, d_m, d_b = [], [], []
-    for _ in range(nb):
-        qs = np.concatenate([synthetic_query(pp, rng) for _ in range(T)])
+    for b in range(nb):
+        qs = np.concatenate([via_gpu.client_query(ctx, pp, ck["sk1"], i, g) for i in idx[b * T:(b + 1) * T]])
         d = ctx.alloc(qs.nbytes)
         ctx.to_device(d, qs)
         d_q.append(d)
         d_m.append(ctx.alloc(8 * pp.n2))
         d_b.append(ctx.alloc(8 * pp.n2))
+    g.close()
     L = via_gpu.lib()
     err = via_gpu.ErrDetail()
 
@@ -329,6 +290,14 @@ def vb_run(via_gpu, args):
     elapsed = time.perf_counter() - t_start
     lat_ms = np.array(lat) * 1e3
     recs = args.steps * nb * T
+    # every batch's answer recovers its T records (recover + deinterleave_batch)
+    ok = True
+    for b in range(nb):
+        m = ctx.to_host(np.zeros(pp.n2, np.uint64), d_m[b])
+        bd = ctx.to_host(np.zeros(pp.n2, np.uint64), d_b[b])
+        got = via_gpu.client_deinterleave(pp, T, via_gpu.client_recover(pp, ck["sk2"], m, bd))
+        ok = ok and all(np.array_equal(got[t], record_of(record_bytes, pp, i))
+                        for t, i in enumerate(idx[b * T:(b + 1) * T]))
     out = {
         "metric": "VIA-B records/sec (answer_batch of T seeded indices) + p50/p99 batch answer latency",
         "value": round(recs / elapsed, 2), "unit": "records/s", "n_gpus": 1, "steps": args.steps,
@@ -337,7 +306,10 @@ def vb_run(via_gpu, args):
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3), "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int (exact modular, residues < 2^38 in fp64 lanes)",
-        "data": "synthetic: uniform random records, uniform key and query residues",
+        "data": f"seeded: records fill_bytes of Shake256Prg(b'via-gpu/{tag.decode()}/db'), keys Client::setup and "
+                f"queries Client::query of b'via-gpu/{tag.decode()}/setup', distinct indices uniform_below of "
+                f"b'via-gpu/{tag.decode()}/query'",
+        "recovered": bool(ok),
         "config": {"workload": f"VIA-B, {nb * T} indices per step as {nb} batch(es) of T={T}, record degree "
                                f"{pp.n_rec} ({pp.n_rec // 2} B records), I={I} J={J}, prepared DB "
                                f"{I * J * n * 5 / 2**30:.2f} GiB as 7-bit limb fragments "
@@ -462,7 +434,7 @@ def keygen_run(via_gpu, args):
         "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
         "p50_ms": round(float(np.median(t_setup)) * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "int (exact modular arithmetic, u64 residues)",
-        "data": "synthetic: keys drawn from Shake256Prg(b'bench/client-keys')",
+        "data": "seeded: keys drawn from Shake256Prg(b'bench/client-keys')",
         "config": {"workload": f"paper stack n1=2048 n2=512, L_CK=18 (base 18), L_RSK=8; keys of "
                                f"{8 * (wc + wv + wr) / 2**20:.2f} MiB per client; query of "
                                f"{pp.query_lwes} LWEs at I={args.rows} J={args.cols}",
