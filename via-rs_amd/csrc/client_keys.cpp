@@ -400,13 +400,26 @@ void embed_at(int nl, size_t ns, size_t nd, const uint64_t *src, size_t slot, ui
 
 bool ntt_ring(uint64_t q, size_t n) { return q > 2 && q < kMaxModulus && (q - 1) % (2 * n) == 0; }
 
+// B^L stays within 128 bits (the gadget vector round(Q / B^(k+1)), k < L)
+bool gadget_ok(uint64_t base, uint32_t L) {
+    if (base < 2 || !L) return false;
+    u128 v = 1;
+    for (uint32_t k = 0; k < L; k++) {
+        if (v > (~(u128)0) / base) return false;
+        v *= base;
+    }
+    return true;
+}
+
 via_status check_client(const via_params *pp) {
     if (!pp) return VIA_E_INVALID;
     if (pp->n1 != (uint32_t)kSlots || pp->q1_nl != 2 || !pp->n2 || pp->n1 % pp->n2 || !pp->n_rec ||
         pp->n1 % pp->n_rec)
         return VIA_E_INVALID;
     if (!ntt_ring(pp->q1[0], pp->n1) || !ntt_ring(pp->q1[1], pp->n1) || !ntt_ring(pp->q3, pp->n2)) return VIA_E_INVALID;
-    if (pp->ck_base < 2 || pp->b_rsk < 2 || !pp->l_ck || !pp->l_rsk) return VIA_E_INVALID;
+    if (!gadget_ok(pp->ck_base, pp->l_ck) || !gadget_ok(pp->b_rsk, pp->l_rsk) || !gadget_ok(pp->b1, pp->l_query) ||
+        !gadget_ok(pp->b2, pp->l_query))
+        return VIA_E_INVALID;
     return VIA_OK;
 }
 
