@@ -125,7 +125,8 @@ struct Reader {
             buf.resize(old + nb * kPrgBlock);
             launch_prg_window(seeds, first, nb, d, c->stream);
             // 17 little-endian lanes are the block's 136 bytes in stream order
-            if (hipMemcpyAsync(buf.data() + old, d, nb * kPrgBlock, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(buf.data() + old, d, nb * kPrgBlock, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
                 hipStreamSynchronize(c->stream) != hipSuccess) {
                 buf.resize(old);
                 err = VIA_E_DEVICE;
