@@ -158,6 +158,55 @@ int main() {
         CHECK(rns_reconstruct(a0, a1, b) == x, "garner\n");
         count++;
     }
+    // fp64 mod switch q1 -> q2 (modswitch_f64) against floor((x q2 +
+    // floor(Q/2)) / Q) mod q2: random x, x within a few units of every
+    // rounding boundary (x q2 = (m + 1/2) Q), the ends of [0, Q) and
+    // multiples of q0; a decided value must be exact, and values right at a
+    // boundary must be declined.
+    {
+        const uint64_t q2 = 17175674881ULL;
+        SwitchF sf;
+        sf.q2 = (double)q2;
+        sf.inv_q1 = 1.0 / (double)b.q1;
+        sf.q2_Q = (double)q2 / u128_to_f64(Q1);
+        long decided = 0, declined = 0, declined_near = 0;
+        auto check = [&](via_u128 x, bool near) {
+            const double a0 = (double)(uint64_t)(x % b.q0), a1 = (double)(uint64_t)(x % b.q1);
+            const uint64_t want = (uint64_t)(exact_rmd(x, q2, Q1) % q2);
+            double y;
+            if (modswitch_f64(a0, a1, b, sf, y)) {
+                decided++;
+                CHECK((uint64_t)y == want, "modswitch x=%llu:%llu got %llu want %llu\n",
+                      (unsigned long long)(x >> 64), (unsigned long long)x, (unsigned long long)y,
+                      (unsigned long long)want);
+            } else {
+                declined++;
+                if (near) declined_near++;
+            }
+            count++;
+        };
+        for (int t = 0; t < 400000; t++) check((((via_u128)rng() << 64) | rng()) % Q1, false);
+        const long random_declined = declined;
+        for (int t = 0; t < 20000; t++) {
+            // x q2 within a few units of (m + 1/2) Q: x0 = floor(((2m + 1) Q) / (2 q2))
+            const via_u128 m = (((via_u128)rng() << 64) | rng()) % q2;
+            const via_u128 x0 = exact_rmd(2 * m + 1, Q1, 2 * (via_u128)q2) - 1;
+            for (int d = -2; d <= 2; d++) {
+                const via_u128 x = x0 + (via_u128)(d + 2) - 2;
+                if (x < Q1) check(x, true);
+            }
+        }
+        for (int d = 0; d < 4; d++) {
+            check((via_u128)d, false);
+            check(Q1 - 1 - d, false);
+            check((via_u128)b.q0 * (d + 1), false);
+            check((via_u128)b.q0 * (b.q1 - 1 - d), false);
+        }
+        CHECK(random_declined < 10, "modswitch: %ld random values declined\n", random_declined);
+        CHECK(decided > 400000, "modswitch: too few decided (%ld)\n", decided);
+        // x0 + 1 sits where x q2 / Q + 1/2 is within q2 / Q of an integer
+        CHECK(declined_near > 0, "modswitch: no boundary value declined\n");
+    }
     // fp64 two-limb gadget scale (the DMux gadget B = 55879, L = 2 at Q1, and
     // B = 81, L = 2 for the bound) against the exact centred scale: random
     // values, values within a few units of every rounding boundary

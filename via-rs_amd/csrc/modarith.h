@@ -191,6 +191,34 @@ struct Rns2 {
     via_u128 big_q;
 };
 
+// Mod switch of a two-limb residue to q2 in fp64, the value the 128-bit path
+// gives, floor((x q2 + floor(Q/2)) / Q) mod q2 with x = a0 + q0 t (Garner).
+// With t q2 = K q1 + R (R = t q2 mod q1 exactly; K from one fp64 quotient,
+// within 2^-14 of the integer): x q2 / Q = K + F, F = R / q1 + a0 q2 / Q,
+// so y = K + floor(F + 1/2 - 1/(2Q)). F is taken to within 2^-50; when
+// F + 1/2 lies within 2^-30 of an integer the call declines (false) and the
+// caller takes the 128-bit path. Needs q2 < q1 < 2^38 and q0 < 2^38.
+struct SwitchF {
+    double q2;    // the target modulus
+    double inv_q1;  // 1 / q1
+    double q2_Q;    // q2 / Q
+};
+
+VIA_HD bool modswitch_f64(double a0, double a1, const Rns2 &b, const SwitchF &f, double &y) {
+    const double a0m = mod_reduce_small(a0, b.m1);
+    const double t = mod_mul(mod_sub(a1, a0m, b.m1), (double)b.q0_inv_mod_q1, b.m1);
+    const double R = mod_mul(t, f.q2, b.m1);
+    const double K = rint((t * f.q2 - R) * f.inv_q1);
+    const double G = R * f.inv_q1 + a0 * f.q2_Q + 0.5;
+    const double fl = floor(G);
+    const double fr = G - fl;
+    if (fr < 0x1p-30 || fr > 1.0 - 0x1p-30) return false;
+    double v = K + fl;
+    if (v >= f.q2) v -= f.q2;
+    y = v;
+    return true;
+}
+
 // Garner reconstruction (via-rs rns/basis.rs:101-109), limbs given as doubles.
 VIA_HD via_u128 rns_reconstruct(double a0, double a1, const Rns2 &b) {
     double a0m = mod_reduce_small(a0, b.m1);  // a0 < q0 < 2^38 < 2^52
