@@ -66,7 +66,25 @@ typedef struct via_ctx via_ctx;
 typedef struct via_db via_db;
 typedef struct via_keys via_keys;
 
-/* Device context: one HIP stream and the NTT tables of every ring in use. */
+/* Device context: one HIP stream and the NTT tables of every ring in use.
+ *
+ * Threading (Server::answer(&self) is re-entrant, answer.rs:534-554): every
+ * entry point may be called from several threads at once on one context and
+ * shared databases and key sets. Host-buffer entry points (via_answer,
+ * via_answer_many, via_answer_batch, via_first_dim, via_ntt) run on the
+ * context when it is free and otherwise on one of up to seven lanes the
+ * context creates on demand (own stream and workspace), so concurrent calls
+ * overlap on the device. Device-buffer entry points run in call order on the
+ * context's own stream, whose work the caller synchronises with
+ * via_ctx_synchronize. A database or key set must outlive the calls that use
+ * it and must not be destroyed while a call uses it.
+ *
+ * Inputs are canonical residues (the reference's Zq types hold nothing
+ * else): a query, key or record word at or above its modulus is refused with
+ * VIA_E_INVALID, by the host-buffer calls before they return and, for the
+ * device-buffer answer calls, by the next via_ctx_synchronize (the word is
+ * taken as 0 meanwhile, so no out-of-range value enters the arithmetic). On
+ * any error the output buffers hold unspecified values. */
 via_status via_ctx_create(int device, via_ctx **out);
 void via_ctx_destroy(via_ctx *ctx);
 /* Seconds spent in each of the seven answer steps of the last via_answer on
@@ -129,14 +147,17 @@ void via_keys_destroy(via_keys *keys);
 via_status via_answer(via_ctx *ctx, const via_db *db, const via_keys *keys, const uint64_t *lwes, size_t n_lwe,
                       uint64_t *ans_mask, uint64_t *ans_body, via_err_detail *err);
 /* Same with device-resident query and answer buffers; enqueued on the
- * context stream, returns without waiting. */
+ * context stream, returns without waiting (a non-canonical query word is
+ * reported by the next via_ctx_synchronize as VIA_E_INVALID). */
 via_status via_answer_device(via_ctx *ctx, const via_db *db, const via_keys *keys, const uint64_t *d_lwes,
                              size_t n_lwe, uint64_t *d_ans_mask, uint64_t *d_ans_body, via_err_detail *err);
+/* Waits for the context stream; VIA_E_INVALID (once) when a device query
+ * since the last call held a non-canonical word. */
 via_status via_ctx_synchronize(via_ctx *ctx);
 
 /* Server::answer for nq queries against one database, each with the keys of
  * its own client (keys[q]); every step runs batched across the queries, the
- * database is streamed once per group of up to 8 queries. Each answer is
+ * database is streamed once per group of up to 32 queries. Each answer is
  * bit-identical to answering the query alone. Host buffers: lwes[q] holds
  * n_lwe LWEs, masks[q] / bodies[q] receive n2 words. No counterpart in the
  * reference, which answers one query per call (answer.rs:534-554). */
@@ -198,6 +219,14 @@ via_status via_answer_batch_device(via_ctx *ctx, const via_db *db, const via_key
  * 7 the repacked RLWE [part][n1] at q2 of the last VIA-B answer. Stages 1-6
  * of a VIA-B answer are those of its last group of queries. */
 via_status via_debug_stage(via_ctx *ctx, int stage, uint64_t *out, size_t words);
+
+/* Test switch with no counterpart in the reference: on != 0 makes every
+ * later launch of the process take the exact 128/192-bit paths of the gadget
+ * scale and of the q1 -> q2 mod switch instead of their fp64 fast paths (the
+ * paths the fast paths hand the values they decline to). Answers are the same
+ * either way, only slower; graphs captured before the switch keep the paths
+ * they were captured with. */
+via_status via_debug_set_exact_paths(int on);
 
 /* RingPolyEval batch (ntt_abstraction.rs:61-78): c = a * b in
  * Z_q[X]/(X^n+1) for count polys; poly p uses modulus moduli[(p / group) % nmod].

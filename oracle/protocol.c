@@ -413,6 +413,53 @@ static void resp_comp(const vo_server *s, const uint64_t *cr, uint64_t *ans_mask
 }
 
 /* answer_through_crot (answer.rs:83-228) + answer_one_query (answer.rs:289-354). */
+/* Threads for the query decomposition's per-LWE work. 1 (the default) is the
+ * reference's serial loop, which the CPU baseline times; tests raise it to
+ * check GPU answers sooner (the values do not depend on it). */
+static int g_decomp_threads = 1;
+void vo_set_decomp_threads(int n) { g_decomp_threads = n > 0 ? n : 1; }
+
+/* Level i of bit b: the cascade of its LWE (cascade.rs:388-405) into the m
+ * half of the RGSW, then rlwe_to_rgsw_eval (convert.rs:162-180) of that
+ * level into the neg_s_m half; gp holds one ciphertext of scratch. */
+static void decomp_level(const vo_server *s, const vo_ring *r1, const uint64_t *lwes, size_t lw, uint64_t *rgsw,
+                         size_t gw1, uint32_t LQ, uint32_t b, uint32_t i, uint64_t *gp) {
+    const vo_params *pp = &s->pp;
+    size_t w1 = vo_pw(r1), cw1 = 2 * w1;
+    uint64_t *g = rgsw + b * gw1;
+    uint64_t *levels = g + 2 * (size_t)LQ * w1; /* the m half holds the RLWE levels */
+    cascade_eval(r1, lwes + ((size_t)b * LQ + i) * lw, s->cascade_eval, pp->ck_base, pp->l_ck, levels + i * cw1);
+    vo_gadget_product_eval(r1, s->conv_eval, pp->l_ck, pp->ck_base, levels + i * cw1, gp, gp + w1, 0);
+    vo_from_eval(r1, gp);
+    vo_from_eval(r1, gp + w1);
+    vo_poly_add(r1, g + i * cw1, levels + i * cw1 + w1, gp);
+    memcpy(g + i * cw1 + w1, gp + w1, 8 * w1);
+}
+
+typedef struct {
+    const vo_server *s;
+    const vo_ring *r1;
+    const uint64_t *lwes;
+    size_t lw;
+    uint64_t *rgsw;
+    size_t gw1;
+    uint32_t LQ;
+    size_t total, next;
+} decomp_job;
+
+static void *decomp_worker(void *arg) {
+    decomp_job *j = (decomp_job *)arg;
+    uint64_t *gp = vo_alloc(2 * vo_pw(j->r1));
+    for (;;) {
+        size_t k = __atomic_fetch_add(&j->next, 1, __ATOMIC_RELAXED);
+        if (k >= j->total) break;
+        decomp_level(j->s, j->r1, j->lwes, j->lw, j->rgsw, j->gw1, j->LQ, (uint32_t)(k / j->LQ),
+                     (uint32_t)(k % j->LQ), gp);
+    }
+    free(gp);
+    return NULL;
+}
+
 static int answer_stages(const vo_server *s, const uint64_t *lwes, size_t n_lwe, uint64_t *rgsw_bits,
                          uint64_t *dmux_out, uint64_t *switched, uint64_t *fd_out, uint64_t *cmux_out,
                          uint64_t *crot_out, uint64_t *ans_mask, uint64_t *ans_body, double *times) {
@@ -428,23 +475,20 @@ static int answer_stages(const vo_server *s, const uint64_t *lwes, size_t n_lwe,
     /* step 1: query_decomp (query_decomp.rs:39-95). */
     uint64_t *rgsw = rgsw_bits ? rgsw_bits : vo_alloc(bits * gw1);
     size_t lw = vo_words_lwe(pp);
-    uint64_t *gp = vo_alloc(cw1);
-    for (uint32_t b = 0; b < bits; b++) {
-        uint64_t *g = rgsw + b * gw1;
-        uint64_t *levels = g + 2 * (size_t)LQ * w1; /* the m half holds the RLWE levels */
-        for (uint32_t i = 0; i < LQ; i++)
-            cascade_eval(&r1, lwes + ((size_t)b * LQ + i) * lw, s->cascade_eval, pp->ck_base, pp->l_ck,
-                         levels + i * cw1);
-        /* rlwe_to_rgsw_eval (convert.rs:162-180) */
-        for (uint32_t i = 0; i < LQ; i++) {
-            vo_gadget_product_eval(&r1, s->conv_eval, pp->l_ck, pp->ck_base, levels + i * cw1, gp, gp + w1, 0);
-            vo_from_eval(&r1, gp);
-            vo_from_eval(&r1, gp + w1);
-            vo_poly_add(&r1, g + i * cw1, levels + i * cw1 + w1, gp);
-            memcpy(g + i * cw1 + w1, gp + w1, 8 * w1);
-        }
+    if (g_decomp_threads > 1) {
+        /* test-only fan-out of the independent per-LWE work (same values) */
+        decomp_job job = {s, &r1, lwes, lw, rgsw, gw1, LQ, (size_t)bits * LQ, 0};
+        size_t nth = (size_t)g_decomp_threads < job.total ? (size_t)g_decomp_threads : job.total;
+        pthread_t *th = (pthread_t *)calloc(nth, sizeof *th);
+        for (size_t k = 0; k < nth; k++) pthread_create(&th[k], NULL, decomp_worker, &job);
+        for (size_t k = 0; k < nth; k++) pthread_join(th[k], NULL);
+        free(th);
+    } else {
+        uint64_t *gp = vo_alloc(cw1);
+        for (uint32_t b = 0; b < bits; b++)
+            for (uint32_t i = 0; i < LQ; i++) decomp_level(s, &r1, lwes, lw, rgsw, gw1, LQ, b, i, gp);
+        free(gp);
     }
-    free(gp);
     double t1 = now_s();
 
     /* step 2: dmux_tree from trivial(Delta) (answer.rs:181-191). */

@@ -199,6 +199,10 @@ void vo_project_at(uint32_t nl, uint32_t ns, uint32_t nd, const uint64_t *src, s
 
 /* Threads the parallel parts use (the CPUs of this process's affinity mask). */
 int vo_host_threads(void);
+/* Threads for the query decomposition's per-LWE cascades and conversions
+ * (default 1: the reference's serial loop, as the CPU baseline times it;
+ * tests raise it, the values are the same). */
+void vo_set_decomp_threads(int n);
 
 /* Wide helpers exposed for tests of the 256-bit path. */
 void vo_round_mul_div(uint64_t a_lo, uint64_t a_hi, uint64_t s_lo, uint64_t s_hi,

@@ -190,6 +190,7 @@ class Oracle:
             "vo_repack_key_from_cascade": (None, [R, u64p, C.c_uint64, C.c_uint32, C.c_uint32, u64p]),
             "vo_repack": (C.c_int, [R, C.c_uint32, C.c_uint32, u64p, u64p, C.c_uint64, C.c_uint32, u64p]),
             "vo_host_threads": (C.c_int, []),
+            "vo_set_decomp_threads": (None, [C.c_int]),
             "vo_embed_at": (None, [C.c_uint32, C.c_uint32, C.c_uint32, u64p, C.c_size_t, u64p]),
             "vo_project_at": (None, [C.c_uint32, C.c_uint32, C.c_uint32, u64p, C.c_size_t, u64p]),
         }

@@ -19,6 +19,6 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def oracle():
-    from oracle.vo import Oracle
+    from tests.paper_setup import oracle as shared_oracle
 
-    return Oracle()
+    return shared_oracle()

@@ -26,6 +26,8 @@ def oracle():
     global _ORACLE
     if _ORACLE is None:
         _ORACLE = Oracle()
+        # the checker's query decomposition over every host CPU (same values)
+        _ORACLE.set_decomp_threads(_ORACLE.host_threads())
     return _ORACLE
 
 

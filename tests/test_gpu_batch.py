@@ -86,17 +86,41 @@ def paper_b(oracle):
     server.ctx.close()
 
 
-def test_paper_t8_batch_matches_oracle(oracle, paper_b):
+@pytest.fixture(scope="module")
+def paper_b_t8(oracle, paper_b):
     pp, client, records, server = paper_b
     idxs = [3, 17, 42, 5, 88, 100, 7, 120]
     qs = client.queries(idxs)
+    return idxs, qs, oracle_batch(oracle, pp, client, records, 8, qs)
+
+
+def test_paper_t8_batch_matches_oracle(paper_b, paper_b_t8):
+    pp, client, records, server = paper_b
+    idxs, qs, (wm, wb, wpacked) = paper_b_t8
     print("case: paper VIA-B T=8 I=J=2", flush=True)
     am, ab = server.answer_batch(8, qs)
     packed = server.ctx.stage(7, 2 * pp.n1)
-    wm, wb, wpacked = oracle_batch(oracle, pp, client, records, 8, qs)
     assert np.array_equal(packed, wpacked)
     assert np.array_equal(am, wm) and np.array_equal(ab, wb)
     assert client.recover(8, am, ab) == [paper_b_record(i) for i in idxs]
+
+
+def test_paper_t8_exact_paths_match_oracle(paper_b, paper_b_t8):
+    # every coefficient through the exact paths the fp64 fast paths fall back
+    # to, in the VIA-C prefix of the eight queries and in the repack at q2
+    import via_gpu
+
+    pp, client, records, server = paper_b
+    idxs, qs, (wm, wb, wpacked) = paper_b_t8
+    L = via_gpu.lib()
+    assert L.via_debug_set_exact_paths(1) == 0
+    try:
+        print("case: paper VIA-B T=8 I=J=2, exact paths", flush=True)
+        am, ab = server.answer_batch(8, qs)
+        assert np.array_equal(server.ctx.stage(7, 2 * pp.n1), wpacked)
+        assert np.array_equal(am, wm) and np.array_equal(ab, wb)
+    finally:
+        L.via_debug_set_exact_paths(0)
 
 
 def test_batch_errors(paper_b):

@@ -86,10 +86,12 @@ def run_case(ctx, case):
     return server, mask, body
 
 
-def test_answer_paper_2x2_index15(ctx):
-    # client_server_e2e_paper.rs:316-349, answered on the GPU.
-    case = paper_case(2, 2, seed=b"via-c-paper-scale-e2e", index=15, stages=True)
-    server, mask, body = run_case(ctx, case)
+@pytest.fixture(scope="module")
+def case_2x2_15():
+    return paper_case(2, 2, seed=b"via-c-paper-scale-e2e", index=15, stages=True)
+
+
+def check_stages(ctx, case, mask, body):
     n1 = 2048
     for k, words in [(1, 4 * 2 * 2 * 2 * 2 * n1), (2, 2 * 2 * 2 * n1), (3, 2 * 2 * n1), (4, 2 * 2 * n1),
                      (5, 2 * n1), (6, 2 * n1)]:
@@ -97,6 +99,29 @@ def test_answer_paper_2x2_index15(ctx):
         assert np.array_equal(got, case.st[k]), f"stage {k} differs"
     assert np.array_equal(mask, case.ans_mask) and np.array_equal(body, case.ans_body)
     assert np.array_equal(case.recover(mask, body), case.record)
+
+
+def test_answer_paper_2x2_index15(ctx, case_2x2_15):
+    # client_server_e2e_paper.rs:316-349, answered on the GPU.
+    server, mask, body = run_case(ctx, case_2x2_15)
+    check_stages(ctx, case_2x2_15, mask, body)
+
+
+def test_exact_paths_match_oracle(ctx, case_2x2_15):
+    # the fp64 fast paths of the gadget scale (one limb, two limbs, the wide
+    # cascade split) and of the q1 -> q2 mod switch decline values next to a
+    # rounding boundary and hand them to the exact 128/192-bit paths; random
+    # values almost never reach those, so the switch sends every coefficient
+    # through them: every stage stays word-exact with the oracle
+    import via_gpu
+
+    L = via_gpu.lib()
+    assert L.via_debug_set_exact_paths(1) == 0
+    try:
+        server, mask, body = run_case(ctx, case_2x2_15)
+        check_stages(ctx, case_2x2_15, mask, body)
+    finally:
+        L.via_debug_set_exact_paths(0)
 
 
 @pytest.mark.parametrize("rows,cols,index", [(4, 8, 77), (8, 4, 3), (16, 16, 1000)])

@@ -6,6 +6,7 @@
 
 #include <map>
 #include <mutex>
+#include <condition_variable>
 #include <utility>
 #include <vector>
 
@@ -19,7 +20,10 @@ struct DevTab {
 };
 
 struct Workspace {
-    size_t Q = 0, B = 0, I = 0, J = 0, bits = 0;
+    // sizes the buffers were allocated for; l_ck sizes the digit planes and
+    // partial-sum planes, so a parameter set with a deeper cascade gadget
+    // regrows them
+    size_t Q = 0, B = 0, I = 0, J = 0, bits = 0, l_ck = 0;
     uint64_t gen = 0;  // incremented whenever the buffers are reallocated
     double *lwe_f = nullptr, *masks[2] = {nullptr, nullptr}, *body[2] = {nullptr, nullptr};
     double *pm = nullptr, *pb = nullptr, *rgsw = nullptr, *rgsw_eval = nullptr;
@@ -52,6 +56,7 @@ struct StepTune {
 struct RepackWs {
     int T = 0;
     size_t n_lwe = 0;
+    size_t l_ck = 0, n2 = 0;  // gadget depth and answer degree the buffers hold
     double *rot = nullptr;      // [T][part][n1] post-CRot ciphertexts at q2
     double *packed = nullptr;   // [part][n1] repacked RLWE
     double *body[2] = {nullptr, nullptr};  // [pair][n1] interleaved bodies
@@ -67,6 +72,26 @@ struct via_ctx {
     hipStream_t stream = nullptr;
     std::map<std::pair<uint64_t, uint32_t>, DevTab> tabs;
     std::mutex mu;
+    // One call at a time uses this context's stream and workspace. Server::
+    // answer(&self) is re-entrant in the reference (answer.rs:534-554): the
+    // host-buffer entry points borrow a lane (a child context with its own
+    // stream and workspace, sharing the parent's twiddle tables) when the
+    // context is busy, so concurrent callers overlap on the device; the
+    // device-buffer entry points wait for the context itself, whose stream
+    // their caller synchronises on.
+    std::recursive_mutex call_mu;
+    via_ctx *parent = nullptr;
+    std::mutex lane_mu;
+    std::vector<via_ctx *> lanes;
+    // Mapped host flag set by the query conversion when a word is not a
+    // canonical residue (the word is taken as 0); read after the call's
+    // synchronisation, or by via_ctx_synchronize for device entry points.
+    // Slot 0 serves the device-buffer entry points, slot 1 the host-buffer
+    // ones (which clear and read it within the call); flag_slot names the
+    // slot of the call in progress.
+    int *bad_h = nullptr;
+    int *bad_d = nullptr;
+    int flag_slot = 0;
     Workspace ws;
     RepackWs rw;
     hipEvent_t ev[8] = {};
@@ -153,6 +178,9 @@ bool same_client_params(const via_params &a, const via_params &b);
 size_t keys_footprint(const via_params *pp);
 // Largest VIA-B batch the serving loop stages.
 constexpr uint32_t kServeMaxT = 64;
+// True when every word of n_lwe LWEs ([lwe][limb][n1 + 1]) is a canonical
+// residue of its q1 limb.
+bool lwes_canonical(const via_params *pp, const uint64_t *lwes, size_t n_lwe);
 // Builds the twiddle tables of every ring the answer path uses on this context.
 via_status prepare_tables(via_ctx *c, const via_params *pp);
 }  // namespace via

@@ -60,6 +60,13 @@ struct Gadget {
 // and K < 2^53 for one; B^(L-h) < 2^40).
 bool scale_w_make(const RingDev &r, const Gadget &g, const DigitPlan &dp, ScaleW *sw);
 
+// Test switch (via_debug_set_exact_paths): when set, every launcher takes
+// the exact 128/192-bit paths of the gadget scale and the q1 -> q2 mod switch
+// in place of their fp64 fast paths, so the paths those fast paths fall back
+// to run on every coefficient. The values are the same either way.
+bool exact_paths_only();
+void set_exact_paths_only(bool on);
+
 // ---- launchers (ntt.hip / kernels.hip) ----------------------------------------
 
 // In-place batched NTT over `npolys` polys of degree r.n stored contiguously;
@@ -77,7 +84,10 @@ void launch_ntt_groups(const RingDev &r, const double *src, double *dst, size_t 
 struct U64Srcs {
     const uint64_t *p[kMaxQ];
 };
-void launch_u64_to_f64_multi(const U64Srcs &srcs, int count, double *dst, size_t words, hipStream_t s);
+// The LWE arrays of a batch (two limbs of n1 + 1 words per LWE, limb l below
+// q[l]); a non-canonical word is taken as 0 and sets *bad (device pointer).
+void launch_u64_to_f64_multi(const U64Srcs &srcs, int count, double *dst, size_t words, const uint64_t q[2], int n1,
+                             int *bad, hipStream_t s);
 void launch_u64_to_f64(const uint64_t *src, double *dst, size_t n, hipStream_t s);
 void launch_f64_to_u64(const double *src, uint64_t *dst, size_t n, hipStream_t s);
 // Negacyclic products c = iNTT(NTT(a) * NTT(b)) of `count` u64 polys per limb
@@ -263,7 +273,7 @@ void launch_key_relayout_q2(const double *src, double *dst, int rank_in, int L, 
 void launch_key_relayout(const uint64_t *src, double *dst, int rank_in, int L, int nl, int n, hipStream_t s);
 // DB packing: records [rec][n_rec] (u64 < p) -> cells [i][j][n1] doubles (setup_db.rs:39-75).
 void launch_setup_db(const uint64_t *records, size_t n_records, double *cells, int I, int J, int n1, int n_rec,
-                     uint64_t p, hipStream_t s);
+                     uint64_t p, int *bad, hipStream_t s);
 void launch_setup_db_bytes(const uint8_t *bytes, size_t n_records, double *cells, int I, int J, int n1,
                            int n_rec, hipStream_t s);
 

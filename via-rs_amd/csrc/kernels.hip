@@ -2,6 +2,8 @@
 //
 // Every kernel computes exactly the integers the reference computes; the
 // comments name the via-rs function each one restates.
+#include <atomic>
+
 #include "digits.h"
 #include "ntt_device.h"
 
@@ -27,7 +29,12 @@ __global__ __launch_bounds__(256) void k_poly_digits(RingDev r1, Gadget g, Digit
     write_digits<NL>(a0, a1, r1, g, dp, dig + (size_t)item * g.L * n + c, (size_t)n);
 }
 
+static std::atomic<int> g_exact_only{0};
+bool exact_paths_only() { return g_exact_only.load(std::memory_order_relaxed) != 0; }
+void set_exact_paths_only(bool on) { g_exact_only.store(on ? 1 : 0, std::memory_order_relaxed); }
+
 bool scale_w_make(const RingDev &r, const Gadget &g, const DigitPlan &dp, ScaleW *sw) {
+    if (exact_paths_only()) return false;
     const via_u128 two39 = (via_u128)1 << 39;
     const via_u128 q1 = r.nl == 2 ? (via_u128)r.rns.q1 : (via_u128)r.q_single;
     const via_u128 K = g.bl / q1, R = g.bl % q1;
@@ -694,7 +701,9 @@ void launch_gate(const RingDev &r, const Gadget &g_in, const GateArgs &a_in, hip
     // the fp64 scale where it is exact (one limb) or decided (two limbs)
     Gadget g = g_in;
     g.f64 = 0;
-    if (r.nl == 2 && g.bl < ((via_u128)1 << 40)) {
+    if (exact_paths_only()) {
+        // the 128-bit scale for every coefficient
+    } else if (r.nl == 2 && g.bl < ((via_u128)1 << 40)) {
         g.f64 = 2;
         g.sf = scale_f64_make((double)(uint64_t)g.bl, r.rns.q1, u128_to_f64(r.rns.big_q));
     } else if (r.nl == 1 && !g.wide && ((via_u128)(r.q_single / 2 + 1) * g.bl + r.q_single) < ((via_u128)1 << 53)) {
@@ -799,7 +808,8 @@ void launch_modswitch_q1_q2_groups(const RingDev &r1, uint64_t q2, const double 
     sf.q2 = (double)q2;
     sf.inv_q1 = 1.0 / (double)r1.rns.q1;
     sf.q2_Q = (double)q2 / u128_to_f64(r1.rns.big_q);
-    const int fast = r1.nl == 2 && q2 < r1.rns.q1 && r1.rns.q1 < kMaxModulus && r1.rns.q0 < kMaxModulus;
+    const int fast = !exact_paths_only() && r1.nl == 2 && q2 < r1.rns.q1 && r1.rns.q1 < kMaxModulus &&
+                     r1.rns.q0 < kMaxModulus;
     hipLaunchKernelGGL(k_modswitch_q1_q2, dim3((unsigned)blocks, groups), dim3(256), 0, st, r1, q2, r1.qdiv, sf, fast,
                        in, in_gstride, out, out_gstride, polys);
 }
@@ -938,8 +948,10 @@ void launch_key_relayout(const uint64_t *src, double *dst, int rank_in, int L, i
     hipLaunchKernelGGL(k_key_relayout, dim3(blocks), dim3(256), 0, st, src, dst, rank_in, L, nl, n);
 }
 
+// records [rec][n_rec] must hold values below p (Poly<N_REC, p>); a word at
+// or above p is taken as 0 and sets *bad.
 __global__ void k_setup_db(const uint64_t *records, const uint8_t *bytes, size_t n_records, double *cells, int I,
-                           int J, int n1, int n_rec, uint64_t p) {
+                           int J, int n1, int n_rec, uint64_t p, int *bad) {
     const int d = n1 / n_rec;
     const size_t total = (size_t)I * J * n1;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
@@ -951,7 +963,9 @@ __global__ void k_setup_db(const uint64_t *records, const uint8_t *bytes, size_t
         double v = 0.0;
         if (flat < n_records) {
             if (records) {
-                v = (double)(records[flat * n_rec + c] % p);
+                const uint64_t x = records[flat * n_rec + c];
+                if (x >= p) *bad = 1;
+                else v = (double)x;
             } else {
                 const uint8_t b = bytes[flat * (n_rec / 2) + (c >> 1)];
                 v = (double)((c & 1) ? (b >> 4) : (b & 0xF));
@@ -962,12 +976,12 @@ __global__ void k_setup_db(const uint64_t *records, const uint8_t *bytes, size_t
 }
 
 void launch_setup_db(const uint64_t *records, size_t n_records, double *cells, int I, int J, int n1, int n_rec,
-                     uint64_t p, hipStream_t st) {
+                     uint64_t p, int *bad, hipStream_t st) {
     const size_t total = (size_t)I * J * n1;
     unsigned blocks = (unsigned)((total + 255) / 256);
     if (blocks > 65535) blocks = 65535;
     hipLaunchKernelGGL(k_setup_db, dim3(blocks), dim3(256), 0, st, records, (const uint8_t *)nullptr, n_records,
-                       cells, I, J, n1, n_rec, p);
+                       cells, I, J, n1, n_rec, p, bad);
 }
 
 void launch_setup_db_bytes(const uint8_t *bytes, size_t n_records, double *cells, int I, int J, int n1, int n_rec,
@@ -976,7 +990,7 @@ void launch_setup_db_bytes(const uint8_t *bytes, size_t n_records, double *cells
     unsigned blocks = (unsigned)((total + 255) / 256);
     if (blocks > 65535) blocks = 65535;
     hipLaunchKernelGGL(k_setup_db, dim3(blocks), dim3(256), 0, st, (const uint64_t *)nullptr, bytes, n_records,
-                       cells, I, J, n1, n_rec, (uint64_t)16);
+                       cells, I, J, n1, n_rec, (uint64_t)16, (int *)nullptr);
 }
 
 }  // namespace via

@@ -89,17 +89,34 @@ static unsigned grid_for(size_t n) {
     return (unsigned)(b > 8192 ? 8192 : (b ? b : 1));
 }
 
-__global__ void k_u64_to_f64_multi(U64Srcs srcs, double *dst, size_t words) {
+// LWE arrays of `count` queries to fp64. Word i of an LWE (n + 1 words per
+// limb) is a residue of limb (i mod lw) / (n + 1); a word at or above its
+// modulus (the reference's Zq types hold canonical residues only) is taken
+// as 0 and raises *bad, so no value outside the exactness bounds of the fp64
+// arithmetic enters the pipeline.
+__global__ void k_u64_to_f64_multi(U64Srcs srcs, double *dst, size_t words, uint64_t q0, uint64_t q1, int n1p,
+                                   int *bad) {
     const uint64_t *src = srcs.p[blockIdx.y];
     double *d = dst + (size_t)blockIdx.y * words;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
-        d[i] = (double)src[i];
+    const int lw = 2 * n1p;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t x = src[i];
+        const uint64_t q = (int)(i % lw) < n1p ? q0 : q1;
+        if (x >= q) {
+            *bad = 1;
+            d[i] = 0.0;
+        } else {
+            d[i] = (double)x;
+        }
+    }
 }
 
-void launch_u64_to_f64_multi(const U64Srcs &srcs, int count, double *dst, size_t words, hipStream_t st) {
+void launch_u64_to_f64_multi(const U64Srcs &srcs, int count, double *dst, size_t words, const uint64_t q[2], int n1,
+                             int *bad, hipStream_t st) {
     if (!words || count < 1) return;
     const unsigned bx = std::max(1u, grid_for(words) / (unsigned)count);
-    hipLaunchKernelGGL(k_u64_to_f64_multi, dim3(bx, count), dim3(256), 0, st, srcs, dst, words);
+    hipLaunchKernelGGL(k_u64_to_f64_multi, dim3(bx, count), dim3(256), 0, st, srcs, dst, words, q[0], q[1], n1 + 1,
+                       bad);
 }
 
 void launch_u64_to_f64(const uint64_t *src, double *dst, size_t n, hipStream_t st) {

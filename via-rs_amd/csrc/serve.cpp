@@ -178,9 +178,14 @@ void finish_batch(via_server *s, Slot &sl, via_status st) {
 void worker_loop(via_server *s) {
     (void)hipSetDevice(s->device);
     for (;;) {
-        // retire finished batches
-        for (Slot &sl : s->slots)
-            if (sl.busy && hipEventQuery(sl.done) == hipSuccess) finish_batch(s, sl, VIA_OK);
+        // retire finished batches; any status but "not ready" is a device
+        // failure that the batch's waiters receive as VIA_E_DEVICE
+        for (Slot &sl : s->slots) {
+            if (!sl.busy) continue;
+            const hipError_t q = hipEventQuery(sl.done);
+            if (q == hipSuccess) finish_batch(s, sl, VIA_OK);
+            else if (q != hipErrorNotReady) finish_batch(s, sl, VIA_E_DEVICE);
+        }
         // launch pending work on free slots
         bool launched = false;
         for (Slot &sl : s->slots) {
@@ -323,14 +328,22 @@ via_status via_server_admit(via_server *s, uint64_t client_id, const uint64_t *c
     }
     via_keys *k = nullptr;
     via_status st = via_keys_create(s->slots[0].ctx, &s->kp, cascade_key, conv_key, rsk, &k);
-    std::lock_guard<std::mutex> g(s->mu);
-    if (st) {
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (st) {
+            s->key_bytes -= need;
+            return st;
+        }
+        if (!s->clients.count(client_id)) {
+            s->key_bytes += keys_device_bytes(k) - need;
+            s->clients[client_id] = k;
+            s->client_inflight[client_id] = 0;
+            return VIA_OK;
+        }
+        // a concurrent admit of the same client won: keep its keys
         s->key_bytes -= need;
-        return st;
     }
-    s->key_bytes += keys_device_bytes(k) - need;
-    s->clients[client_id] = k;
-    s->client_inflight[client_id] = 0;
+    via_keys_destroy(k);
     return VIA_OK;
 }
 
@@ -375,6 +388,7 @@ via_status via_server_submit(via_server *s, uint64_t client_id, const uint64_t *
                              uint64_t *ticket) {
     if (!s || !lwes || !ticket) return VIA_E_INVALID;
     if (n_lwe != s->n_lwe) return VIA_E_QUERY_LEN;
+    if (!lwes_canonical(&s->pp, lwes, n_lwe)) return VIA_E_INVALID;
     Request r;
     r.client = client_id;
     r.lwes.assign(lwes, lwes + s->lwe_words);
@@ -394,6 +408,7 @@ via_status via_server_submit_batch(via_server *s, uint64_t client_id, uint32_t T
         if (st) return st;
     }
     if (n_lwe != s->n_lwe_b) return VIA_E_QUERY_LEN;
+    if (!lwes_canonical(&s->pp, lwes, (size_t)T * n_lwe)) return VIA_E_INVALID;
     Request r;
     r.client = client_id;
     r.T = T;

@@ -95,6 +95,7 @@ bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
 namespace {
 
 via_status get_tab(via_ctx *c, uint64_t q, uint32_t n, DevTab *out) {
+    if (c->parent) c = c->parent;  // lanes share their context's tables
     std::lock_guard<std::mutex> g(c->mu);
     auto key = std::make_pair(q, n);
     auto it = c->tabs.find(key);
@@ -229,6 +230,65 @@ via_status check_params(const via_params *pp) {
     return VIA_OK;
 }
 
+// The context a call runs on, held for the duration of the call. Device-
+// buffer entry points take the context itself (waiting for a call in
+// progress, since their caller synchronises on its stream); host-buffer
+// entry points take it when it is free and otherwise a free lane, created on
+// demand up to kMaxLanes and then waited for.
+constexpr size_t kMaxLanes = 7;
+struct CallCtx {
+    via_ctx *c = nullptr;
+    CallCtx() = default;
+    CallCtx(const CallCtx &) = delete;
+    CallCtx &operator=(const CallCtx &) = delete;
+    ~CallCtx() {
+        if (c) c->call_mu.unlock();
+    }
+};
+
+void hold_ctx(via_ctx *root, CallCtx *out) {
+    root->call_mu.lock();
+    out->c = root;
+}
+
+via_status hold_lane(via_ctx *root, CallCtx *out) {
+    if (root->call_mu.try_lock()) {
+        out->c = root;
+        return VIA_OK;
+    }
+    {
+        std::lock_guard<std::mutex> g(root->lane_mu);
+        for (via_ctx *l : root->lanes)
+            if (l->call_mu.try_lock()) {
+                out->c = l;
+                return VIA_OK;
+            }
+        if (root->lanes.size() < kMaxLanes) {
+            via_ctx *l = nullptr;
+            if (via_ctx_create(root->device, &l) != VIA_OK) return VIA_E_DEVICE;
+            l->parent = root;
+            l->call_mu.lock();
+            root->lanes.push_back(l);
+            out->c = l;
+            return VIA_OK;
+        }
+    }
+    hold_ctx(root, out);
+    return VIA_OK;
+}
+
+// Host-buffer calls report non-canonical query words through flag slot 1,
+// cleared here and read by host_flag_status once the call has synchronised.
+struct HostFlag {
+    via_ctx *c;
+    explicit HostFlag(via_ctx *ctx) : c(ctx) {
+        c->flag_slot = 1;
+        __atomic_store_n(&c->bad_h[1], 0, __ATOMIC_RELEASE);
+    }
+    ~HostFlag() { c->flag_slot = 0; }
+    via_status status() const { return __atomic_load_n(&c->bad_h[1], __ATOMIC_ACQUIRE) ? VIA_E_INVALID : VIA_OK; }
+};
+
 template <typename T>
 via_status dalloc(Workspace &w, T **p, size_t count) {
     HIP_TRY(hipMalloc((void **)p, sizeof(T) * (count ? count : 1)));
@@ -271,23 +331,25 @@ int max_tune_groups(int items, int L) {
 
 via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_t bits) {
     Workspace &w = c->ws;
-    size_t I = pp->num_rows, J = pp->num_cols;
-    if (w.Q >= Q && w.B >= B && w.I >= I && w.J >= J && w.bits >= bits && w.lwe_f) return VIA_OK;
+    size_t I = pp->num_rows, J = pp->num_cols, l_ck = pp->l_ck;
+    if (w.Q >= Q && w.B >= B && w.I >= I && w.J >= J && w.bits >= bits && w.l_ck >= l_ck && w.lwe_f) return VIA_OK;
     // grow every dimension, so that alternating shapes settle on one workspace
     Q = std::max(Q, w.Q);
     B = std::max(B, w.B);
     I = std::max(I, w.I);
     J = std::max(J, w.J);
     bits = std::max(bits, w.bits);
+    l_ck = std::max(l_ck, w.l_ck);
     const uint64_t gen = w.gen;
     free_ws(w);
     w.gen = gen + 1;  // captured graphs of the old buffers never match again
-    const size_t n = pp->n1, nl = 2, S = kSlots, lq = pp->l_query;
+    const size_t n = pp->n1, nl = 2, S = kSlots;
     w.Q = Q;
     w.B = B;
     w.I = I;
     w.J = J;
     w.bits = bits;
+    w.l_ck = l_ck;
     const size_t items = Q * B;
     // digit groups per gadget product: about eight workgroups per item,
     // between 512 and 4096 in all (measured best at 36, 288 and 1152 items;
@@ -296,8 +358,8 @@ via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_
     int groups = 1;
     while (groups < (int)pp->l_ck && (long)items * 2 * groups < wg_target) groups++;
     w.groups = groups;
-    // room for the autotuner's candidates (tune_candidates)
-    w.groups_max = std::max(groups, max_tune_groups((int)items, (int)pp->l_ck));
+    // room for the autotuner's candidates (tune_candidates) at the deepest gadget held
+    w.groups_max = std::max(groups, max_tune_groups((int)items, (int)l_ck));
     const FdPlan fp = fd_plan((int)I, (int)J, (int)n, (int)Q);
     via_status s;
 #define A(p, cnt) \
@@ -308,11 +370,12 @@ via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_
     A(w.masks[1], items * nl * S);
     A(w.body[0], items * nl * S);
     A(w.body[1], items * nl * S);
-    A(w.dig, items * pp->l_ck * S);
+    A(w.dig, items * l_ck * S);
     A(w.pm, (size_t)w.groups_max * items * 2 * nl * S);
     A(w.pb, (size_t)w.groups_max * items * 2 * nl * S);
-    A(w.rgsw, Q * bits * 4 * lq * nl * n);
-    A(w.rgsw_eval, Q * bits * 4 * lq * nl * n);
+    // RGSW bits: bits * l_query = B LWEs, 4 polys of nl limbs per LWE
+    A(w.rgsw, Q * B * 4 * nl * n);
+    A(w.rgsw_eval, Q * B * 4 * nl * n);
     A(w.dm[0], Q * I * 2 * nl * n);
     A(w.dm[1], Q * I * 2 * nl * n);
     A(w.sw, Q * I * 2 * n);
@@ -322,7 +385,7 @@ via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_
     A(w.sell, fd_sel_bytes(fp));
     A(w.cm[0], Q * J * 2 * n);
     A(w.cm[1], Q * J * 2 * n);
-    A(w.rgsw2, Q * bits * 4 * lq * n);
+    A(w.rgsw2, Q * B * 4 * n);
     A(w.res[0], Q * 2 * n);
     A(w.res[1], Q * 2 * n);
     A(w.ans_u, Q * 2 * 1024);
@@ -333,12 +396,13 @@ via_status ensure_ws(via_ctx *c, const via_params *pp, size_t Q, size_t B, size_
 // Workspace of the VIA-B repack for up to T queries of n_lwe LWEs.
 via_status ensure_rw(via_ctx *c, const via_params *pp, int T, size_t n_lwe) {
     RepackWs &w = c->rw;
-    if (w.T >= T && w.n_lwe >= n_lwe && w.rot) return VIA_OK;
+    if (w.T >= T && w.n_lwe >= n_lwe && w.l_ck >= pp->l_ck && w.n2 >= pp->n2 && w.rot) return VIA_OK;
     T = std::max(T, w.T);
     n_lwe = std::max(n_lwe, w.n_lwe);
+    const size_t L = std::max<size_t>(pp->l_ck, w.l_ck), n2 = std::max<size_t>(pp->n2, w.n2);
     for (void *p : w.all) (void)hipFree(p);
     w = RepackWs();
-    const size_t n = pp->n1, S = kSlots, L = pp->l_ck;
+    const size_t n = pp->n1, S = kSlots;
     const size_t pairs = (size_t)std::max(1, T / 2);
     // largest items * groups over the levels
     size_t ig = 0;
@@ -362,9 +426,11 @@ via_status ensure_rw(via_ctx *c, const via_params *pp, int T, size_t n_lwe) {
     if ((s = al((void **)&w.pb, sizeof(double) * ig * 2 * S))) return s;
     if ((s = al((void **)&w.dig, sizeof(int16_t) * pairs * L * 2 * S))) return s;
     if ((s = al((void **)&w.lwe_u, sizeof(uint64_t) * T * n_lwe * 2 * (n + 1)))) return s;
-    if ((s = al((void **)&w.ans_u, sizeof(uint64_t) * 2 * pp->n2))) return s;
+    if ((s = al((void **)&w.ans_u, sizeof(uint64_t) * 2 * n2))) return s;
     w.T = T;
     w.n_lwe = n_lwe;
+    w.l_ck = L;
+    w.n2 = n2;
     return VIA_OK;
 }
 
@@ -403,6 +469,16 @@ via_status via::prepare_tables(via_ctx *c, const via_params *pp) {
         if ((s = make_ring(c, nout, &q2, 1, &r))) return s;
     DevTab t;
     return get_tab(c, pp->q3, pp->n2, &t);
+}
+
+bool via::lwes_canonical(const via_params *pp, const uint64_t *lwes, size_t n_lwe) {
+    const size_t n1p = (size_t)pp->n1 + 1;
+    for (size_t i = 0; i < n_lwe * 2; i++) {
+        const uint64_t q = pp->q1[i & 1], *x = lwes + i * n1p;
+        for (size_t c = 0; c < n1p; c++)
+            if (x[c] >= q) return false;
+    }
+    return true;
 }
 
 size_t via::keys_device_bytes(const via_keys *k) {
@@ -461,12 +537,21 @@ via_status via_ctx_create(int device, via_ctx **out) {
             delete c;
             return VIA_E_DEVICE;
         }
+    if (hipHostMalloc((void **)&c->bad_h, 2 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) !=
+            hipSuccess ||
+        hipHostGetDevicePointer((void **)&c->bad_d, c->bad_h, 0) != hipSuccess) {
+        delete c;
+        return VIA_E_DEVICE;
+    }
+    c->bad_h[0] = c->bad_h[1] = 0;
     *out = c;
     return VIA_OK;
 }
 
 void via_ctx_destroy(via_ctx *c) {
     if (!c) return;
+    for (via_ctx *l : c->lanes) via_ctx_destroy(l);
+    c->lanes.clear();
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (auto &e : c->graph_cache) {
@@ -483,6 +568,7 @@ void via_ctx_destroy(via_ctx *c) {
     for (auto &e : c->ev_fd) (void)hipEventDestroy(e);
     for (auto &e : c->ev_mark) (void)hipEventDestroy(e);
     for (auto &e : c->ev_dec) (void)hipEventDestroy(e);
+    if (c->bad_h) (void)hipHostFree(c->bad_h);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -490,11 +576,16 @@ void via_ctx_destroy(via_ctx *c) {
 via_status via_ctx_synchronize(via_ctx *c) {
     if (!c) return VIA_E_INVALID;
     HIP_TRY(hipStreamSynchronize(c->stream));
+    // a device query since the last synchronisation held a non-canonical word
+    if (__atomic_exchange_n(&c->bad_h[0], 0, __ATOMIC_ACQ_REL)) return VIA_E_INVALID;
     return VIA_OK;
 }
 
 via_status via_ctx_step_times(via_ctx *c, double *t7) {
-    if (!c || !t7 || !c->have_times) return VIA_E_INVALID;
+    if (!c || !t7) return VIA_E_INVALID;
+    CallCtx cc;
+    hold_ctx(c, &cc);
+    if (!c->have_times) return VIA_E_INVALID;
     HIP_TRY(hipStreamSynchronize(c->stream));
     for (int i = 0; i < 7; i++) {
         float ms = 0;
@@ -505,7 +596,10 @@ via_status via_ctx_step_times(via_ctx *c, double *t7) {
 }
 
 via_status via_ctx_first_dim_time(via_ctx *c, double *seconds) {
-    if (!c || !seconds || !c->have_times) return VIA_E_INVALID;
+    if (!c || !seconds) return VIA_E_INVALID;
+    CallCtx cc;
+    hold_ctx(c, &cc);
+    if (!c->have_times) return VIA_E_INVALID;
     HIP_TRY(hipStreamSynchronize(c->stream));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev_fd[0], c->ev_fd[1]));
@@ -515,12 +609,16 @@ via_status via_ctx_first_dim_time(via_ctx *c, double *seconds) {
 
 via_status via_ctx_mark(via_ctx *c, int which) {
     if (!c || which < 0 || which > 1) return VIA_E_INVALID;
+    CallCtx cc;
+    hold_ctx(c, &cc);
     HIP_TRY(hipEventRecord(c->ev_mark[which], c->stream));
     return VIA_OK;
 }
 
 via_status via_ctx_marked_time(via_ctx *c, double *seconds) {
     if (!c || !seconds) return VIA_E_INVALID;
+    CallCtx cc;
+    hold_ctx(c, &cc);
     HIP_TRY(hipEventSynchronize(c->ev_mark[1]));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev_mark[0], c->ev_mark[1]));
@@ -536,6 +634,9 @@ static via_status db_common(via_ctx *c, const via_params *pp, const uint64_t *re
     via_status s = check_params(pp);
     if (s) return s;
     if (!is_pow2(pp->num_rows) || !is_pow2(pp->num_cols)) return VIA_E_DIM;
+    CallCtx cc;
+    hold_ctx(c, &cc);
+    HostFlag flag(c);
     const size_t I = pp->num_rows, J = pp->num_cols, n1 = pp->n1, nr = pp->n_rec;
     if (n_records > I * J * (n1 / nr)) return VIA_E_DIM;  // answer.rs:503-508
     if (bytes && (pp->p != 16 || nr % 2)) return VIA_E_INVALID;
@@ -573,7 +674,7 @@ static via_status db_common(via_ctx *c, const via_params *pp, const uint64_t *re
         launch_setup_db_bytes((const uint8_t *)stage, n_records, cells, (int)I, (int)J, (int)n1, (int)nr, c->stream);
     else
         launch_setup_db((const uint64_t *)stage, n_records, cells, (int)I, (int)J, (int)n1, (int)nr, pp->p,
-                        c->stream);
+                        c->bad_d + 1, c->stream);
     RingDev r2;
     uint64_t q2 = pp->q2;
     if ((s = make_ring(c, (uint32_t)n1, &q2, 1, &r2))) {
@@ -586,10 +687,15 @@ static via_status db_common(via_ctx *c, const via_params *pp, const uint64_t *re
     }
     launch_ntt(r2, cells, I * J, 1, false, c->stream);
     launch_pack_db(cells, db->limbs, fp, c->stream);
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipFree(stage));
-    HIP_TRY(hipFree(cells));
-    HIP_TRY(hipGetLastError());
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    (void)hipFree(stage);
+    (void)hipFree(cells);
+    if (e != hipSuccess || hipGetLastError() != hipSuccess || (s = flag.status())) {
+        // a record word at or above p (Poly<N_REC, p> holds canonical values only)
+        (void)hipFree(db->limbs);
+        delete db;
+        return e != hipSuccess ? VIA_E_DEVICE : (s ? s : VIA_E_DEVICE);
+    }
     *out = db;
     return VIA_OK;
 }
@@ -607,6 +713,8 @@ via_status via_db_create_bytes(via_ctx *c, const via_params *pp, const uint8_t *
 
 void via_db_destroy(via_db *db) {
     if (!db) return;
+    CallCtx cc;
+    hold_ctx(db->ctx, &cc);
     (void)hipSetDevice(db->ctx->device);
     (void)hipStreamSynchronize(db->ctx->stream);
     (void)hipFree(db->limbs);
@@ -625,6 +733,10 @@ via_status via_db_update(via_db *db, const uint64_t *indices, size_t n, const ui
     const size_t cap = I * J * d;
     for (size_t u = 0; u < n; u++)
         if (indices[u] >= cap) return VIA_E_INDEX;
+    for (size_t w = 0; w < n * nr; w++)
+        if (records[w] >= pp->p) return VIA_E_INVALID;  // records hold canonical values mod p
+    CallCtx cc;
+    hold_ctx(c, &cc);
     // cell -> its d slots' sources (index into records, or -1)
     std::map<int64_t, std::vector<int32_t>> touched;
     for (size_t u = 0; u < n; u++) {
@@ -680,6 +792,8 @@ via_status via_db_read_cells(const via_db *db, uint64_t *out, size_t words) {
     const size_t total = (size_t)db->pp.num_rows * db->pp.num_cols * db->pp.n1;
     if (words < total) return VIA_E_INVALID;
     via_ctx *c = db->ctx;
+    CallCtx cc;
+    hold_ctx(c, &cc);
     uint64_t *tmp = nullptr;
     HIP_TRY(hipMalloc(&tmp, sizeof(uint64_t) * total));
     launch_unpack_db(db->limbs, tmp, fd_plan((int)db->pp.num_rows, (int)db->pp.num_cols, (int)db->pp.n1, 1), c->stream);
@@ -691,102 +805,179 @@ via_status via_db_read_cells(const via_db *db, uint64_t *out, size_t words) {
 
 // ---- keys --------------------------------------------------------------------------------
 
+// Key words are canonical residues (the reference's Zq types hold nothing
+// else): the cascade and conversion keys limb by limb below q1[l], the
+// ring-switch key below q3. A word outside would break the exactness bounds
+// of the fp64 arithmetic, so it is refused (VIA_E_INVALID).
+static bool keys_canonical(const via_params *pp, const uint64_t *cascade_key, const uint64_t *conv_key,
+                           const uint64_t *rsk) {
+    const size_t n1 = pp->n1, L = pp->l_ck;
+    size_t off = 0;
+    for (size_t nout = 2; nout <= n1; nout <<= 1) {
+        // step words [idx][k][part][limb][nout]: limb = (w / nout) % 2
+        const size_t words = (2 * n1 / nout) * L * 2 * 2 * nout;
+        const uint64_t *x = cascade_key + off;
+        for (size_t w = 0; w < words; w += nout) {
+            const uint64_t q = pp->q1[(w / nout) & 1];
+            for (size_t c = 0; c < nout; c++)
+                if (x[w + c] >= q) return false;
+        }
+        off += words;
+    }
+    for (size_t w = 0; w < L * 2 * 2 * n1; w++)
+        if (conv_key[w] >= pp->q1[(w / n1) & 1]) return false;
+    const size_t rw = (size_t)(n1 / pp->n2) * pp->l_rsk * 2 * pp->n2;
+    for (size_t w = 0; w < rw; w++)
+        if (rsk[w] >= pp->q3) return false;
+    return true;
+}
+
+// Bytes of each device buffer of a key set: cascade step s (n_out = 2^(s+1)),
+// the conversion key, the ring-switch key, repack step r (n_out =
+// repack_nout0 * 2^r).
+static size_t key_cascade_bytes(const via_params *pp, size_t s) {
+    const size_t nout = (size_t)2 << s;
+    return sizeof(double) * (2 * pp->n1 / nout) * pp->l_ck * 2 * 2 * nout;
+}
+static size_t key_conv_bytes(const via_params *pp) { return sizeof(double) * (size_t)pp->l_ck * 2 * 2 * pp->n1; }
+static size_t key_rsk_bytes(const via_params *pp) {
+    return sizeof(double) * (size_t)(pp->n1 / pp->n2) * pp->l_rsk * 2 * pp->n2;
+}
+static size_t key_repack_bytes(const via_params *pp, uint32_t nout0, size_t r) {
+    const size_t nout = (size_t)nout0 << r;
+    return sizeof(double) * (2 * pp->n1 / nout) * pp->l_ck * 2 * nout;
+}
+
+// Releases every device buffer of a key set built so far (zeroizing first).
+static void keys_free(via_keys *k) {
+    via_ctx *c = k->ctx;
+    const via_params *pp = &k->pp;
+    (void)hipStreamSynchronize(c->stream);
+    for (size_t i = 0; i < k->cascade.size(); i++)
+        (void)hipMemsetAsync(k->cascade[i], 0, key_cascade_bytes(pp, i), c->stream);
+    if (k->conv) (void)hipMemsetAsync(k->conv, 0, key_conv_bytes(pp), c->stream);
+    if (k->rsk) (void)hipMemsetAsync(k->rsk, 0, key_rsk_bytes(pp), c->stream);
+    for (size_t i = 0; i < k->repack.size(); i++)
+        (void)hipMemsetAsync(k->repack[i], 0, key_repack_bytes(pp, k->repack_nout0, i), c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    for (double *p : k->repack) (void)hipFree(p);
+    for (double *p : k->cascade) (void)hipFree(p);
+    (void)hipFree(k->conv);
+    (void)hipFree(k->rsk);
+    delete k;
+}
+
 via_status via_keys_create(via_ctx *c, const via_params *pp, const uint64_t *cascade_key, const uint64_t *conv_key,
                            const uint64_t *rsk, via_keys **out) {
     if (!c || !out || !cascade_key || !conv_key || !rsk) return VIA_E_INVALID;
     via_status s = check_params(pp);
     if (s) return s;
-    HIP_TRY(hipSetDevice(c->device));
+    if (!keys_canonical(pp, cascade_key, conv_key, rsk)) return VIA_E_INVALID;
+    CallCtx cc;
+    hold_ctx(c, &cc);
+    if (hipSetDevice(c->device) != hipSuccess) return VIA_E_DEVICE;
     via_keys *k = new via_keys();
     k->id = next_object_id();
     k->ctx = c;
     k->pp = *pp;
     const uint32_t n1 = pp->n1, nl = 2, L = pp->l_ck;
-    uint64_t *stage = nullptr;
     const size_t max_step = (size_t)2 * n1 * L * 2 * nl;
-    HIP_TRY(hipMalloc(&stage, sizeof(uint64_t) * max_step));
     // VIA-B: the steps of output degree >= 2 n_rec also serve the repack at
     // q2 (a batch of T has extraction degree n2/T >= n_rec), mod-switched
     // once here (repack.rs:684-805 does it per answer; the values are equal).
     const bool vb = pp->n_rec < pp->n2;
     k->repack_nout0 = vb ? 2 * pp->n_rec : 0;
+    // staging: the u64 words of the largest key piece, and for VIA-B their
+    // fp64 copy and its q2 switch; every return below frees them and the
+    // key buffers made so far
+    uint64_t *stage = nullptr;
     double *tmp_f = nullptr, *tmp_q2 = nullptr;
-    if (vb) {
-        HIP_TRY(hipMalloc(&tmp_f, sizeof(double) * max_step));
-        HIP_TRY(hipMalloc(&tmp_q2, sizeof(double) * max_step / 2));
-    }
+    auto fail = [&](via_status st) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(stage);
+        (void)hipFree(tmp_f);
+        (void)hipFree(tmp_q2);
+        keys_free(k);
+        return st;
+    };
+    auto key_buf = [&](double **d, size_t words) {
+        return hipMalloc((void **)d, sizeof(double) * words) == hipSuccess;
+    };
+    if (hipMalloc(&stage, sizeof(uint64_t) * max_step) != hipSuccess) return fail(VIA_E_NOMEM);
+    if (vb && (hipMalloc(&tmp_f, sizeof(double) * max_step) != hipSuccess ||
+               hipMalloc(&tmp_q2, sizeof(double) * max_step / 2) != hipSuccess))
+        return fail(VIA_E_NOMEM);
     size_t off = 0;
     for (uint32_t nout = 2; nout <= n1; nout <<= 1) {
         const uint32_t rank_in = 2 * n1 / nout;
         const size_t words = (size_t)rank_in * L * 2 * nl * nout;
         double *d = nullptr;
-        HIP_TRY(hipMalloc(&d, sizeof(double) * words));
+        if (!key_buf(&d, words)) return fail(VIA_E_NOMEM);
         k->cascade.push_back(d);
-        k->bytes.push_back(sizeof(double) * words);
-        HIP_TRY(hipMemcpyAsync(stage, cascade_key + off, sizeof(uint64_t) * words, hipMemcpyHostToDevice, c->stream));
+        if (hipMemcpyAsync(stage, cascade_key + off, sizeof(uint64_t) * words, hipMemcpyHostToDevice, c->stream) !=
+            hipSuccess)
+            return fail(VIA_E_DEVICE);
         launch_key_relayout(stage, d, (int)rank_in, (int)L, (int)nl, (int)nout, c->stream);
         RingDev r;
-        if ((s = make_ring(c, nout, pp->q1, 2, &r))) return s;
+        if ((s = make_ring(c, nout, pp->q1, 2, &r))) return fail(s);
         launch_ntt(r, d, (size_t)L * 2 * nl * rank_in, rank_in, false, c->stream);
         if (vb && nout >= k->repack_nout0) {
             // mod_switch_rlev (repack.rs:43-56) of every sample, q1 -> q2
             const size_t w2 = words / nl;
             double *rk = nullptr;
-            HIP_TRY(hipMalloc(&rk, sizeof(double) * w2));
+            if (hipMalloc((void **)&rk, sizeof(double) * w2) != hipSuccess) return fail(VIA_E_NOMEM);
             k->repack.push_back(rk);
             launch_u64_to_f64(stage, tmp_f, words, c->stream);
             launch_modswitch_q1_q2(r, pp->q2, tmp_f, tmp_q2, (size_t)rank_in * L * 2, c->stream);
             launch_key_relayout_q2(tmp_q2, rk, (int)rank_in, (int)L, (int)nout, c->stream);
             RingDev r2;
             uint64_t q2 = pp->q2;
-            if ((s = make_ring(c, nout, &q2, 1, &r2))) return s;
+            if ((s = make_ring(c, nout, &q2, 1, &r2))) return fail(s);
             launch_ntt(r2, rk, (size_t)L * 2 * rank_in, rank_in, false, c->stream);
         }
-        HIP_TRY(hipStreamSynchronize(c->stream));
+        // the stage is reused by the next step
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(VIA_E_DEVICE);
         off += words;
-    }
-    HIP_TRY(hipFree(stage));
-    if (vb) {
-        HIP_TRY(hipFree(tmp_f));
-        HIP_TRY(hipFree(tmp_q2));
     }
     // conversion key RLev(L_CK) at (n1, q1): layout already [k][part][limb][n]
     {
         const size_t words = (size_t)L * 2 * nl * n1;
-        uint64_t *u = nullptr;
-        HIP_TRY(hipMalloc(&u, sizeof(uint64_t) * words));
-        HIP_TRY(hipMalloc(&k->conv, sizeof(double) * words));
-        HIP_TRY(hipMemcpyAsync(u, conv_key, sizeof(uint64_t) * words, hipMemcpyHostToDevice, c->stream));
-        launch_u64_to_f64(u, k->conv, words, c->stream);
+        if (!key_buf(&k->conv, words)) return fail(VIA_E_NOMEM);
+        if (hipMemcpyAsync(stage, conv_key, sizeof(uint64_t) * words, hipMemcpyHostToDevice, c->stream) !=
+            hipSuccess)
+            return fail(VIA_E_DEVICE);
+        launch_u64_to_f64(stage, k->conv, words, c->stream);
         RingDev r;
-        if ((s = make_ring(c, n1, pp->q1, 2, &r))) return s;
+        if ((s = make_ring(c, n1, pp->q1, 2, &r))) return fail(s);
         launch_ntt(r, k->conv, (size_t)L * 2 * nl, 1, false, c->stream);
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        HIP_TRY(hipFree(u));
-        k->bytes.push_back(sizeof(double) * words);
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(VIA_E_DEVICE);
     }
     // ring-switch key: d RLev(L_RSK) at (n2, q3)
     {
         const uint32_t d = n1 / pp->n2;
         const size_t words = (size_t)d * pp->l_rsk * 2 * pp->n2;
-        uint64_t *u = nullptr;
-        HIP_TRY(hipMalloc(&u, sizeof(uint64_t) * words));
-        HIP_TRY(hipMalloc(&k->rsk, sizeof(double) * words));
-        HIP_TRY(hipMemcpyAsync(u, rsk, sizeof(uint64_t) * words, hipMemcpyHostToDevice, c->stream));
-        launch_u64_to_f64(u, k->rsk, words, c->stream);
+        if (!key_buf(&k->rsk, words)) return fail(VIA_E_NOMEM);
+        if (hipMemcpyAsync(stage, rsk, sizeof(uint64_t) * words, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            return fail(VIA_E_DEVICE);
+        launch_u64_to_f64(stage, k->rsk, words, c->stream);
         RingDev r;
         uint64_t q3 = pp->q3;
-        if ((s = make_ring(c, pp->n2, &q3, 1, &r))) return s;
+        if ((s = make_ring(c, pp->n2, &q3, 1, &r))) return fail(s);
         launch_ntt(r, k->rsk, (size_t)d * pp->l_rsk * 2, 1, false, c->stream);
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        HIP_TRY(hipFree(u));
-        k->bytes.push_back(sizeof(double) * words);
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(VIA_E_DEVICE);
     }
-    for (size_t i = 0; i < k->repack.size(); i++) {
-        const uint32_t nout = k->repack_nout0 << i;
-        k->bytes.push_back(sizeof(double) * (size_t)(2 * n1 / nout) * L * 2 * nout);
+    for (size_t i = 0; i < k->cascade.size(); i++) k->bytes.push_back(key_cascade_bytes(pp, i));
+    k->bytes.push_back(key_conv_bytes(pp));
+    k->bytes.push_back(key_rsk_bytes(pp));
+    for (size_t i = 0; i < k->repack.size(); i++) k->bytes.push_back(key_repack_bytes(pp, k->repack_nout0, i));
+    (void)hipFree(stage);
+    (void)hipFree(tmp_f);
+    (void)hipFree(tmp_q2);
+    if (hipGetLastError() != hipSuccess) {
+        stage = nullptr;
+        tmp_f = tmp_q2 = nullptr;
+        return fail(VIA_E_DEVICE);
     }
-    HIP_TRY(hipGetLastError());
     *out = k;
     return VIA_OK;
 }
@@ -794,21 +985,10 @@ via_status via_keys_create(via_ctx *c, const via_params *pp, const uint64_t *cas
 void via_keys_destroy(via_keys *k) {
     if (!k) return;
     via_ctx *c = k->ctx;
+    CallCtx cc;
+    hold_ctx(c, &cc);
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    size_t i = 0;
-    for (double *p : k->cascade) {
-        (void)hipMemsetAsync(p, 0, k->bytes[i++], c->stream);
-    }
-    (void)hipMemsetAsync(k->conv, 0, k->bytes[i++], c->stream);
-    (void)hipMemsetAsync(k->rsk, 0, k->bytes[i++], c->stream);
-    for (double *p : k->repack) (void)hipMemsetAsync(p, 0, k->bytes[i++], c->stream);
-    (void)hipStreamSynchronize(c->stream);
-    for (double *p : k->repack) (void)hipFree(p);
-    for (double *p : k->cascade) (void)hipFree(p);
-    (void)hipFree(k->conv);
-    (void)hipFree(k->rsk);
-    delete k;
+    keys_free(k);
 }
 
 // ---- answer -------------------------------------------------------------------------------
@@ -928,7 +1108,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         U64Srcs srcs;
         memset(&srcs, 0, sizeof srcs);
         for (int q = 0; q < Q; q++) srcs.p[q] = d_lwes[q];
-        launch_u64_to_f64_multi(srcs, Q, w.lwe_f, B * lw, st);
+        launch_u64_to_f64_multi(srcs, Q, w.lwe_f, B * lw, pp->q1, (int)n, c->bad_d + c->flag_slot, st);
     }
     const Gadget gck = make_gadget(pp->ck_base, pp->l_ck);
     const DigitPlan dpk = digit_plan((uint32_t)pp->ck_base, (int)pp->l_ck);
@@ -946,7 +1126,8 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
                 step_tune[i] = (*tg)[i];
                 step_tune[i].groups = std::min(step_tune[i].groups, w.groups_max);
             } else {
-                step_tune[i].groups = w.groups;
+                // the heuristic of the workspace's sizing, within this gadget's depth
+                step_tune[i].groups = std::min(w.groups, (int)pp->l_ck);
             }
         }
     }
@@ -1374,6 +1555,8 @@ static void drop_graphs(via_ctx *c) {
 
 via_status via_ctx_set_graphs(via_ctx *c, int on) {
     if (!c) return VIA_E_INVALID;
+    CallCtx cc;
+    hold_ctx(c, &cc);
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (!on) drop_graphs(c);
@@ -1384,6 +1567,8 @@ via_status via_ctx_set_graphs(via_ctx *c, int on) {
 via_status via_answer_device(via_ctx *c, const via_db *db, const via_keys *keys, const uint64_t *d_lwes,
                              size_t n_lwe, uint64_t *d_mask, uint64_t *d_body, via_err_detail *err) {
     if (!c || !db || !keys || !d_mask || !d_body || (n_lwe && !d_lwes)) return VIA_E_INVALID;
+    CallCtx cc;
+    hold_ctx(c, &cc);
     HIP_TRY(hipSetDevice(c->device));
     return answer_graph_or_eager(c, db, 1, &keys, &d_lwes, n_lwe, &d_mask, &d_body, err);
 }
@@ -1392,6 +1577,8 @@ via_status via_answer_many_device(via_ctx *c, const via_db *db, uint32_t nq, con
                                    const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_masks,
                                    uint64_t *const *d_bodies, via_err_detail *err) {
     if (!c || !db || !keys || !d_lwes || !d_masks || !d_bodies || nq < 1) return VIA_E_INVALID;
+    CallCtx cc;
+    hold_ctx(c, &cc);
     HIP_TRY(hipSetDevice(c->device));
     for (uint32_t q0 = 0; q0 < nq; q0 += kMaxQ) {
         const int Q = (int)std::min<uint32_t>(kMaxQ, nq - q0);
@@ -1436,6 +1623,8 @@ via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via
         return VIA_E_INVALID;
     for (uint32_t q = 0; q < nq; q++)
         if (!keys[q] || !d_lwes[q] || !d_mask[q] || !d_body[q]) return VIA_E_INVALID;
+    CallCtx cc;
+    hold_ctx(c, &cc);
     HIP_TRY(hipSetDevice(c->device));
     struct Guard {
         via_ctx *c;
@@ -1480,10 +1669,17 @@ via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via
     return VIA_OK;
 }
 
-via_status via_answer_many(via_ctx *c, const via_db *db, uint32_t nq, const via_keys *const *keys,
+via_status via_answer_many(via_ctx *root, const via_db *db, uint32_t nq, const via_keys *const *keys,
                             const uint64_t *const *lwes, size_t n_lwe, uint64_t *const *masks, uint64_t *const *bodies,
                             via_err_detail *err) {
-    if (!c || !db || !keys || !lwes || !masks || !bodies || nq < 1) return VIA_E_INVALID;
+    if (!root || !db || !keys || !lwes || !masks || !bodies || nq < 1) return VIA_E_INVALID;
+    for (uint32_t q = 0; q < nq; q++)
+        if (!keys[q] || !lwes[q] || !masks[q] || !bodies[q]) return VIA_E_INVALID;
+    CallCtx cc;
+    via_status hs = hold_lane(root, &cc);
+    if (hs) return hs;
+    via_ctx *c = cc.c;
+    HostFlag flag(c);
     HIP_TRY(hipSetDevice(c->device));
     // the query length follows the database's record degree
     via_params ep = keys[0]->pp;
@@ -1523,6 +1719,10 @@ via_status via_answer_many(via_ctx *c, const via_db *db, uint32_t nq, const via_
             HIP_TRY(hipMemcpyAsync(bodies[q0 + q], dbp[q], sizeof(uint64_t) * pp->n2, hipMemcpyDeviceToHost, c->stream));
         }
         HIP_TRY(hipStreamSynchronize(c->stream));
+        if ((s = flag.status())) {
+            if (err) snprintf(err->msg, sizeof err->msg, "query word not a canonical residue mod q1");
+            return s;
+        }
     }
     return VIA_OK;
 }
@@ -1537,14 +1737,18 @@ via_status via_answer_batch_device(via_ctx *c, const via_db *db, const via_keys 
                                    const uint64_t *d_lwes, size_t n_lwe, uint64_t *d_ans_mask, uint64_t *d_ans_body,
                                    via_err_detail *err) {
     if (!c || !db || !keys || !d_lwes || !d_ans_mask || !d_ans_body) return VIA_E_INVALID;
+    CallCtx cc;
+    hold_ctx(c, &cc);
     HIP_TRY(hipSetDevice(c->device));
     return answer_vb_impl(c, db, keys, (int)T, d_lwes, n_lwe, d_ans_mask, d_ans_body, err);
 }
 
-via_status via_answer_batch(via_ctx *c, const via_db *db, const via_keys *keys, uint32_t T,
+via_status via_answer_batch(via_ctx *root, const via_db *db, const via_keys *keys, uint32_t T,
                             const uint64_t *const *lwes, const size_t *n_lwe, size_t n_queries, uint64_t *ans_mask,
                             uint64_t *ans_body, via_err_detail *err) {
-    if (!c || !db || !keys || !ans_mask || !ans_body || (n_queries && (!lwes || !n_lwe))) return VIA_E_INVALID;
+    if (!root || !db || !keys || !ans_mask || !ans_body || (n_queries && (!lwes || !n_lwe))) return VIA_E_INVALID;
+    for (size_t t = 0; t < n_queries; t++)
+        if (!lwes[t]) return VIA_E_INVALID;
     via_status s = check_vb(db, keys, T);
     if (s) return s;
     if (n_queries != T) {  // batch.rs:124-128
@@ -1569,6 +1773,10 @@ via_status via_answer_batch(via_ctx *c, const via_db *db, const via_keys *keys, 
             }
             return VIA_E_QUERY_LEN;
         }
+    CallCtx cc;
+    if ((s = hold_lane(root, &cc))) return s;
+    via_ctx *c = cc.c;
+    HostFlag flag(c);
     HIP_TRY(hipSetDevice(c->device));
     if ((s = ensure_rw(c, &ep, (int)T, expected))) return s;
     RepackWs &w = c->rw;
@@ -1580,11 +1788,17 @@ via_status via_answer_batch(via_ctx *c, const via_db *db, const via_keys *keys, 
     HIP_TRY(hipMemcpyAsync(ans_mask, w.ans_u, sizeof(uint64_t) * ep.n2, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(ans_body, w.ans_u + ep.n2, sizeof(uint64_t) * ep.n2, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if ((s = flag.status())) {
+        if (err) snprintf(err->msg, sizeof err->msg, "query word not a canonical residue mod q1");
+        return s;
+    }
     return VIA_OK;
 }
 
 via_status via_debug_stage(via_ctx *c, int stage, uint64_t *out, size_t words) {
     if (!c || stage < 1 || stage > 7 || !out) return VIA_E_INVALID;
+    CallCtx cc;
+    hold_ctx(c, &cc);
     const double *p = c->stage_ptr[stage - 1];
     const size_t total = c->stage_words[stage - 1];
     if (!p || words < total) return VIA_E_INVALID;
@@ -1594,6 +1808,11 @@ via_status via_debug_stage(via_ctx *c, int stage, uint64_t *out, size_t words) {
     HIP_TRY(hipMemcpyAsync(out, tmp, sizeof(uint64_t) * total, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipFree(tmp));
+    return VIA_OK;
+}
+
+via_status via_debug_set_exact_paths(int on) {
+    set_exact_paths_only(on != 0);
     return VIA_OK;
 }
 
@@ -1609,6 +1828,8 @@ static via_status ring_for(via_ctx *c, uint32_t n, const uint64_t *moduli, uint3
 via_status via_poly_mul_device(via_ctx *c, uint32_t n, const uint64_t *moduli, uint32_t nmod, size_t group,
                                const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_c, size_t count) {
     if (!c || !moduli || !group) return VIA_E_INVALID;
+    CallCtx cc;
+    hold_ctx(c, &cc);
     HIP_TRY(hipSetDevice(c->device));
     RingDev r;
     via_status s = ring_for(c, n, moduli, nmod, &r);
@@ -1619,9 +1840,12 @@ via_status via_poly_mul_device(via_ctx *c, uint32_t n, const uint64_t *moduli, u
     return VIA_OK;
 }
 
-via_status via_ntt(via_ctx *c, uint32_t n, const uint64_t *moduli, uint32_t nmod, size_t group, uint64_t *polys,
+via_status via_ntt(via_ctx *root, uint32_t n, const uint64_t *moduli, uint32_t nmod, size_t group, uint64_t *polys,
                    size_t count, int inverse) {
-    if (!c || !moduli || !polys || !group) return VIA_E_INVALID;
+    if (!root || !moduli || !polys || !group) return VIA_E_INVALID;
+    CallCtx cc;
+    if (hold_lane(root, &cc)) return VIA_E_DEVICE;
+    via_ctx *c = cc.c;
     HIP_TRY(hipSetDevice(c->device));
     RingDev r;
     via_status s = ring_for(c, n, moduli, nmod, &r);
@@ -1647,9 +1871,12 @@ via_status via_ntt(via_ctx *c, uint32_t n, const uint64_t *moduli, uint32_t nmod
 // FirstDim alone over host buffers (first_dim.rs:94-184 batched over
 // queries): switched [q][I][part][n1] -> out [q][J][part][n1], both
 // coefficient form at q2; the same kernels as the answer path.
-via_status via_first_dim(via_ctx *c, const via_db *db, uint32_t nq, const uint64_t *switched, uint64_t *out) {
-    if (!c || !db || !switched || !out || nq < 1 || nq > (uint32_t)kMaxQ) return VIA_E_INVALID;
-    if (db->ctx != c) return VIA_E_INVALID;
+via_status via_first_dim(via_ctx *root, const via_db *db, uint32_t nq, const uint64_t *switched, uint64_t *out) {
+    if (!root || !db || !switched || !out || nq < 1 || nq > (uint32_t)kMaxQ) return VIA_E_INVALID;
+    if (db->ctx != root) return VIA_E_INVALID;
+    CallCtx cc;
+    if (hold_lane(root, &cc)) return VIA_E_DEVICE;
+    via_ctx *c = cc.c;
     HIP_TRY(hipSetDevice(c->device));
     const via_params *pp = &db->pp;
     via_status s = ensure_ws(c, pp, nq, 1, 1);
