@@ -155,6 +155,22 @@ via_status via_answer_device(via_ctx *ctx, const via_db *db, const via_keys *key
  * since the last call held a non-canonical word. */
 via_status via_ctx_synchronize(via_ctx *ctx);
 
+/* No counterpart in the reference: via_answer_device for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The whole workspace lives in a 96 MiB
+ * array the library holds on each GPU it runs on (from the first context on
+ * that device until the process ends), enough for one query against a grid
+ * of up to I = J = 256 at the paper parameters; a larger workspace is
+ * VIA_E_NOMEM (err: expected = bytes needed, got = bytes available), a
+ * one-row grid VIA_E_INVALID, and parameters whose twiddle tables this
+ * context has not built (via_db_create / via_keys_create on it build them)
+ * VIA_E_INVALID. Nothing is allocated. Every call on one GPU shares the
+ * array: calls whose device work overlaps (other contexts or streams on the
+ * same GPU) race, so serialise them. Same query, answer and error contract
+ * as via_answer_device. */
+via_status via_answer_device_noalloc(via_ctx *ctx, const via_db *db, const via_keys *keys, const uint64_t *d_lwes,
+                                     size_t n_lwe, uint64_t *d_mask, uint64_t *d_body, via_err_detail *err);
+
 /* Server::answer for nq queries against one database, each with the keys of
  * its own client (keys[q]); every step runs batched across the queries, the
  * database is streamed once per group of up to 32 queries. Each answer is

@@ -92,6 +92,11 @@ struct via_ctx {
     int *bad_h = nullptr;
     int *bad_d = nullptr;
     int flag_slot = 0;
+    // the device address of this GPU's no-allocation scratch array, and
+    // whether the call in progress must not allocate (twiddle tables are
+    // then looked up, never built)
+    void *scratch = nullptr;
+    bool no_alloc = false;
     Workspace ws;
     RepackWs rw;
     hipEvent_t ev[8] = {};
@@ -158,9 +163,11 @@ uint64_t tuned_generation();
 uint64_t next_object_id();
 // Answers Q <= kMaxQ queries (device buffers) against one database on the
 // context's stream, asynchronously.
+// carved: a workspace laid out by the caller (the no-allocation entry point);
+// null uses the context's own, grown on demand.
 via_status answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via_keys *const *keys,
                              const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_mask,
-                             uint64_t *const *d_body, via_err_detail *err);
+                             uint64_t *const *d_body, via_err_detail *err, Workspace *carved = nullptr);
 // VIA-B: T queries of one client (d_lwes holds them back to back), repacked
 // into one answer, on the context's stream, asynchronously.
 via_status answer_vb_impl(via_ctx *c, const via_db *db, const via_keys *keys, int T, const uint64_t *d_lwes,
@@ -176,6 +183,11 @@ size_t keys_device_bytes(const via_keys *k);
 bool same_client_params(const via_params &a, const via_params &b);
 // Device bytes via_keys_create allocates for keys of these parameters.
 size_t keys_footprint(const via_params *pp);
+// Bytes of the per-device scratch array of via_answer_device_noalloc.
+constexpr size_t kScratchBytes = (size_t)96 << 20;
+// The device address of this GPU's scratch array (noalloc.hip), looked up
+// once per device; `device` must be current.
+via_status scratch_setup(int device, void **ptr);
 // Largest VIA-B batch the serving loop stages.
 constexpr uint32_t kServeMaxT = 64;
 // True when every word of n_lwe LWEs ([lwe][limb][n1 + 1]) is a canonical

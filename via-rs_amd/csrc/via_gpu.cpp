@@ -95,6 +95,7 @@ bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
 namespace {
 
 via_status get_tab(via_ctx *c, uint64_t q, uint32_t n, DevTab *out) {
+    const bool lookup_only = c->no_alloc;
     if (c->parent) c = c->parent;  // lanes share their context's tables
     std::lock_guard<std::mutex> g(c->mu);
     auto key = std::make_pair(q, n);
@@ -103,6 +104,7 @@ via_status get_tab(via_ctx *c, uint64_t q, uint32_t n, DevTab *out) {
         *out = it->second;
         return VIA_OK;
     }
+    if (lookup_only) return VIA_E_INVALID;  // tables are built by calls that may allocate
     uint64_t psi = find_psi(q, n);
     if (!psi) return VIA_E_INVALID;
     uint64_t psi_inv = inv_mod(psi, q);
@@ -544,6 +546,10 @@ via_status via_ctx_create(int device, via_ctx **out) {
         return VIA_E_DEVICE;
     }
     c->bad_h[0] = c->bad_h[1] = 0;
+    if (scratch_setup(device, &c->scratch) != VIA_OK) {
+        via_ctx_destroy(c);
+        return VIA_E_DEVICE;
+    }
     *out = c;
     return VIA_OK;
 }
@@ -1063,7 +1069,8 @@ namespace {
 // every step batched across the queries; query q uses the keys of its own
 // client. *res_out receives the CRot outputs [q][part][n1] at q2.
 via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *const *keys,
-                       const uint64_t *const *d_lwes, size_t n_lwe, via_err_detail *err, const double **res_out) {
+                       const uint64_t *const *d_lwes, size_t n_lwe, via_err_detail *err, const double **res_out,
+                       Workspace *carved = nullptr) {
     if (Q < 1 || Q > kMaxQ) return VIA_E_INVALID;
     via_params ep = keys[0]->pp;
     ep.n_rec = db->pp.n_rec;
@@ -1089,9 +1096,9 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         }
         return VIA_E_QUERY_LEN;
     }
-    via_status s = ensure_ws(c, pp, (size_t)Q, n_lwe, bits);
-    if (s) return s;
-    Workspace &w = c->ws;
+    via_status s = VIA_OK;
+    if (!carved && (s = ensure_ws(c, pp, (size_t)Q, n_lwe, bits))) return s;
+    Workspace &w = carved ? *carved : c->ws;
     hipStream_t st = c->stream;
     const size_t B = n_lwe, items = (size_t)Q * B;
     const size_t lw = nl * (n + 1);
@@ -1384,9 +1391,9 @@ via_status resp_comp_impl(via_ctx *c, const via_params *pp, int Q, const via_key
 // answer.rs:289-354), every step batched across the queries.
 via_status via::answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via_keys *const *keys,
                                   const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_mask,
-                                  uint64_t *const *d_body, via_err_detail *err) {
+                                  uint64_t *const *d_body, via_err_detail *err, Workspace *carved) {
     const double *res = nullptr;
-    via_status s = prefix_impl(c, db, Q, keys, d_lwes, n_lwe, err, &res);
+    via_status s = prefix_impl(c, db, Q, keys, d_lwes, n_lwe, err, &res, carved);
     if (s) return s;
     if ((s = resp_comp_impl(c, &keys[0]->pp, Q, keys, res, d_mask, d_body))) return s;
     REC_EVENT(c->ev[7], c->stream);
@@ -1571,6 +1578,144 @@ via_status via_answer_device(via_ctx *c, const via_db *db, const via_keys *keys,
     hold_ctx(c, &cc);
     HIP_TRY(hipSetDevice(c->device));
     return answer_graph_or_eager(c, db, 1, &keys, &d_lwes, n_lwe, &d_mask, &d_body, err);
+}
+
+// Workspace of one query (B LWEs of `bits` bits) against an I x J database,
+// carved from `base`. Lifetimes let buffers share space: the query-
+// decomposition buffers (region A) are dead once the RGSW bits exist, so the
+// DMux ping-pong (steps 2-3) and then the CMux ping-pong (step 5) start at
+// region A's base; region C holds the FirstDim pass (steps 3-4); region B the
+// RGSW bits and the CRot outputs, live to the end. Every kernel of one answer
+// runs in stream order, so no two overlapping buffers are live at once.
+// Returns the bytes the layout needs; writes w only when base is not null.
+static size_t carve_ws(const via_params *pp, size_t B, size_t bits, int groups, uint8_t *base, Workspace *w) {
+    const size_t n = pp->n1, S = kSlots, I = pp->num_rows, J = pp->num_cols, L = pp->l_ck;
+    const FdPlan fp = fd_plan((int)I, (int)J, (int)n, 1);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    // region A: query decomposition
+    const size_t a0 = off;
+    const size_t o_lwe = take(8 * B * 2 * (n + 1)), o_dig = take(2 * B * L * S);
+    const size_t o_pm = take(8 * (size_t)groups * B * 2 * 2 * S), o_pb = take(8 * (size_t)groups * B * 2 * 2 * S);
+    size_t o_mb[4];
+    for (size_t &o : o_mb) o = take(8 * B * 2 * S);
+    const size_t end_dec = off;
+    off = a0;  // DMux ping-pong over the dead decomposition buffers
+    const size_t o_dm0 = take(8 * I * 2 * 2 * n), o_dm1 = take(8 * I * 2 * 2 * n);
+    const size_t end_dm = off;
+    off = a0;  // CMux ping-pong, after the DMux outputs were switched to q2
+    const size_t o_cm0 = take(8 * J * 2 * n), o_cm1 = take(8 * J * 2 * n);
+    off = std::max(std::max(end_dec, end_dm), off);
+    // region C: mod switch and FirstDim
+    const size_t o_sw = take(8 * I * 2 * n), o_sel = take(8 * I * 2 * n), o_sell = take(fd_sel_bytes(fp));
+    const size_t o_fdk = take(8 * fd_out_words(fp)), o_fd = take(8 * J * 2 * n);
+    // region B: RGSW bits (q1, eval form, q2) and the CRot outputs
+    const size_t o_rg = take(8 * B * 4 * 2 * n), o_rge = take(8 * B * 4 * 2 * n), o_rg2 = take(8 * B * 4 * n);
+    const size_t o_res0 = take(8 * 2 * n), o_res1 = take(8 * 2 * n);
+    if (base) {
+        *w = Workspace();
+        auto D = [&](size_t o) { return (double *)(base + o); };
+        w->Q = 1;
+        w->B = B;
+        w->I = I;
+        w->J = J;
+        w->bits = bits;
+        w->l_ck = L;
+        w->lwe_f = D(o_lwe);
+        w->dig = (int16_t *)(base + o_dig);
+        w->pm = D(o_pm);
+        w->pb = D(o_pb);
+        w->masks[0] = D(o_mb[0]);
+        w->masks[1] = D(o_mb[1]);
+        w->body[0] = D(o_mb[2]);
+        w->body[1] = D(o_mb[3]);
+        w->dm[0] = D(o_dm0);
+        w->dm[1] = D(o_dm1);
+        w->cm[0] = D(o_cm0);
+        w->cm[1] = D(o_cm1);
+        w->sw = D(o_sw);
+        w->sel = D(o_sel);
+        w->sell = base + o_sell;
+        w->fdk = D(o_fdk);
+        w->fd = D(o_fd);
+        w->rgsw = D(o_rg);
+        w->rgsw_eval = D(o_rge);
+        w->rgsw2 = D(o_rg2);
+        w->res[0] = D(o_res0);
+        w->res[1] = D(o_res1);
+        w->groups = w->groups_max = groups;
+    }
+    return off;
+}
+
+// via_answer_device for one query with no device allocation: the workspace
+// is carved from this GPU's scratch array (kScratchBytes), the twiddle
+// tables are looked up (never built), and the launch sequence is the one of
+// via_answer_device. Calls on one GPU share the array: they must not
+// overlap on the device.
+via_status via_answer_device_noalloc(via_ctx *c, const via_db *db, const via_keys *keys, const uint64_t *d_lwes,
+                                     size_t n_lwe, uint64_t *d_mask, uint64_t *d_body, via_err_detail *err) {
+    if (!c || !db || !keys || !d_lwes || !d_mask || !d_body) return VIA_E_INVALID;
+    if (!c->scratch) return VIA_E_DEVICE;
+    CallCtx cc;
+    hold_ctx(c, &cc);
+    HIP_TRY(hipSetDevice(c->device));
+    via_params ep = keys->pp;
+    ep.n_rec = db->pp.n_rec;
+    if (!is_pow2(ep.num_rows) || !is_pow2(ep.num_cols)) return VIA_E_DIM;
+    const size_t bits = (size_t)(ilog2(ep.num_rows) + ilog2(ep.num_cols) + ilog2(ep.n1 / ep.n_rec));
+    const size_t expected = bits * ep.l_query;
+    if (n_lwe != expected) {  // answer.rs:132-157
+        if (err) {
+            err->expected = expected;
+            err->got = n_lwe;
+            snprintf(err->msg, sizeof err->msg, "query length mismatch: expected %zu LWEs, got %zu", expected, n_lwe);
+        }
+        return VIA_E_QUERY_LEN;
+    }
+    if (ep.num_rows < 2) {  // the trivial DMux root of a one-row grid is built from host memory
+        if (err) snprintf(err->msg, sizeof err->msg, "no-allocation answers need at least two rows");
+        return VIA_E_INVALID;
+    }
+    // the digit groups of the heuristic, fewer while the layout does not fit
+    int groups = 1;
+    while (groups < (int)ep.l_ck && (long)n_lwe * 2 * groups < 512) groups++;
+    while (groups > 1 && carve_ws(&ep, n_lwe, bits, groups, nullptr, nullptr) > kScratchBytes) groups--;
+    const size_t need = carve_ws(&ep, n_lwe, bits, groups, nullptr, nullptr);
+    if (need > kScratchBytes) {
+        if (err) {
+            err->expected = need;
+            err->got = kScratchBytes;
+            snprintf(err->msg, sizeof err->msg, "workspace of %zu bytes exceeds the %zu-byte scratch array", need,
+                     kScratchBytes);
+        }
+        return VIA_E_NOMEM;
+    }
+    {
+        // the rings of the answer: q1 at every cascade degree, q2 at n1, q3 at
+        // n2 (built by via_db_create and via_keys_create on this context)
+        std::lock_guard<std::mutex> g(c->mu);
+        bool ready = c->tabs.count({ep.q2, ep.n1}) && c->tabs.count({ep.q3, ep.n2});
+        for (uint32_t nout = 2; nout <= ep.n1; nout <<= 1)
+            ready = ready && c->tabs.count({ep.q1[0], nout}) && c->tabs.count({ep.q1[1], nout});
+        if (!ready) {
+            if (err) snprintf(err->msg, sizeof err->msg, "twiddle tables of these parameters are not built on this "
+                                                         "context (create the database and keys on it first)");
+            return VIA_E_INVALID;
+        }
+    }
+    Workspace w;
+    carve_ws(&ep, n_lwe, bits, groups, (uint8_t *)c->scratch, &w);
+    c->no_alloc = true;
+    via_status s = answer_batch_impl(c, db, 1, &keys, &d_lwes, n_lwe, &d_mask, &d_body, err, &w);
+    c->no_alloc = false;
+    // stage buffers overlap in the scratch array: not offered for debugging
+    for (auto &p : c->stage_ptr) p = nullptr;
+    return s;
 }
 
 via_status via_answer_many_device(via_ctx *c, const via_db *db, uint32_t nq, const via_keys *const *keys,

@@ -123,6 +123,7 @@ def lib():
             "via_ctx_mark": (C.c_int, [vp, C.c_int]),
             "via_ctx_set_graphs": (C.c_int, [vp, C.c_int]),
             "via_debug_set_exact_paths": (C.c_int, [C.c_int]),
+            "via_answer_device_noalloc": (C.c_int, [vp, vp, vp, vp, C.c_size_t, vp, vp, C.POINTER(ErrDetail)]),
             "via_ctx_set_tuning": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "via_ctx_autotune": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.c_size_t,
                                            C.POINTER(vp), C.POINTER(vp), C.c_int, C.POINTER(C.c_int),
