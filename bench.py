@@ -231,14 +231,46 @@ def serve_run(loop, queries, bqueries, T, qps, n_req, seed=7, frac_b=0.0, b_size
     return out
 
 
-def vb_run(via_gpu, args):
+def serving_measure(via_gpu, ctx, pp, db, args, clients=64, slots=2, max_batch=32):
+    """The mixed serving trace of the default line: `clients` clients with
+    their own Client::setup keys resident in HBM, Poisson arrivals at
+    --serve-qps, 70% VIA-C queries against `db` and 30% VIA-B requests of a
+    number of records uniform in [4, 64] as T = 8 batches against a second
+    database of the same grid at record degree 64 (records from
+    b"via-gpu/serve/db-b"). Every answer is the library's answer of that
+    request alone (tests/test_gpu_serving.py)."""
+    pb = via_gpu.paper_b_params(pp.num_rows, pp.num_cols, T=8)
+    nb_rec = (pb.n1 // pb.n_rec) * pb.num_rows * pb.num_cols
+    g_b = via_gpu.Prg(b"via-gpu/serve/db-b")
+    rb = g_b.fill(ctx, nb_rec * pb.n_rec // 2)
+    g_b.close()
+    db_b = via_gpu.PreparedDb(ctx, pb, record_bytes=rb.tobytes(), n_records=nb_rec)
+    del rb
+    t0 = time.perf_counter()
+    loop, queries, bqueries, T = serve_setup(via_gpu, ctx, pp, db, clients, max_batch=max_batch, slots=slots,
+                                             db_b=db_b)
+    t_setup = time.perf_counter() - t0
+    r = serve_run(loop, queries, bqueries, T, args.serve_qps, args.serve_requests, frac_b=0.3)
+    st = loop.stats()
+    loop.close()
+    db_b.close()
+    r.update({"workload": f"serving loop, Poisson arrivals from {clients} clients (keys resident in HBM), "
+                          f"70% VIA-C queries / 30% VIA-B requests of 4..64 records in T=8 batches, "
+                          f"VIA-C batches <= {max_batch}, {slots} slots, I={pp.num_rows} J={pp.num_cols}",
+              "clients": clients, "key_bytes": st["key_bytes"], "setup_s": round(t_setup, 2),
+              "data": "seeded: client c's keys Client::setup of b'via-gpu/serve/setup/<c>' and its queries "
+                      "Client::query, indices uniform_below of b'via-gpu/serve/query', arrivals from numpy seed 7"})
+    return r
+
+
+def vb_measure(via_gpu, ctx, T, rows, cols, indices, steps, warmup):
     """VIA-B: `indices` seeded distinct indices answered as batched queries of
-    T (answer_batch::<T>) against a database at record degree n2/T; one step
-    answers all of them. Reports records/s and per-batch latency."""
-    T = args.vb
-    pp = via_gpu.paper_b_params(args.rows, args.cols, T=T)
+    T (answer_batch::<T>, device buffers) against a database at record degree
+    n2/T; one step answers all of them. Returns records/s, per-batch latency,
+    the FirstDim pass's roofline (HIP events around k_fd_mfma of the last
+    pass) and whether every batch recovers its T records."""
+    pp = via_gpu.paper_b_params(rows, cols, T=T)
     I, J, n = pp.num_rows, pp.num_cols, pp.n1
-    ctx = via_gpu.Context(0)
     n_rec = (n // pp.n_rec) * I * J
     tag = f"vb{T}-{I}x{J}".encode()
     # records, keys, distinct indices and batch queries from seeded streams
@@ -251,7 +283,7 @@ def vb_run(via_gpu, args):
     db = via_gpu.PreparedDb(ctx, pp, record_bytes=record_bytes.tobytes(), n_records=n_rec)
     keys = via_gpu.PreparedKeys(ctx, pp, ck["cascade_key"], ck["conv_key"], ck["rsk"])
     t_setup = time.perf_counter() - t0
-    nb = max(1, args.indices // T)
+    nb = max(1, indices // T)
     g_q = via_gpu.Prg(b"via-gpu/%s/query" % tag)
     idx = []
     while len(idx) < nb * T:
@@ -276,20 +308,21 @@ def vb_run(via_gpu, args):
         if st:
             raise via_gpu.ViaError(st, msg=L.via_status_str(st).decode())
 
-    for b in range(min(nb, max(1, args.warmup))):
+    for b in range(min(nb, max(1, warmup))):
         one(b)
     ctx.synchronize()
-    lat = []
+    lat, fd_ms = [], []
     t_start = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         for b in range(nb):
             t0 = time.perf_counter()
             one(b)
             ctx.synchronize()
             lat.append(time.perf_counter() - t0)
+            fd_ms.append(ctx.first_dim_time() * 1e3)
     elapsed = time.perf_counter() - t_start
     lat_ms = np.array(lat) * 1e3
-    recs = args.steps * nb * T
+    recs = steps * nb * T
     # every batch's answer recovers its T records (recover + deinterleave_batch)
     ok = True
     for b in range(nb):
@@ -298,29 +331,50 @@ def vb_run(via_gpu, args):
         got = via_gpu.client_deinterleave(pp, T, via_gpu.client_recover(pp, ck["sk2"], m, bd))
         ok = ok and all(np.array_equal(got[t], record_of(record_bytes, pp, i))
                         for t, i in enumerate(idx[b * T:(b + 1) * T]))
-    out = {
-        "metric": "VIA-B records/sec (answer_batch of T seeded indices) + p50/p99 batch answer latency",
-        "value": round(recs / elapsed, 2), "unit": "records/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
-        "batch_answers_per_s": round(args.steps * nb / elapsed, 3), "T": T,
+    for d in d_q + d_m + d_b:
+        ctx.free(d)
+    keys.close()
+    db.close()
+    # the last pass of a batch holds min(T, 32) queries
+    Qp = min(T, 32)
+    fd_bytes = fd_plan_bytes(I, J, n, Qp)
+    fd_s = float(np.median(fd_ms)) * 1e-3
+    return {
+        "T": T, "records_per_s": round(recs / elapsed, 2), "batch_answers_per_s": round(steps * nb / elapsed, 3),
+        "batches_per_step": nb, "steps": steps,
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3), "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "int (exact modular, residues < 2^38 in fp64 lanes)",
+        "latency_samples": len(lat), "recovered": bool(ok), "setup_s": round(t_setup, 3),
+        "workload": f"VIA-B, {nb * T} distinct indices per step as {nb} batch(es) of T={T}, record degree "
+                    f"{pp.n_rec} ({pp.n_rec // 2} B records), I={I} J={J}, prepared DB "
+                    f"{I * J * n * 5 / 2**30:.2f} GiB as 7-bit limb fragments "
+                    f"({I * J * n * 8 / 2**30:.2f} GiB as u64 residues)",
+        "roofline": {"kernel": f"k_fd_mfma, one database pass for {Qp} queries", "bound": "hbm",
+                     "achieved": round(fd_bytes / fd_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(fd_bytes / fd_s / 1e9 / HBM_PEAK_GBS, 4), "algorithmic_bytes": fd_bytes,
+                     "ms": round(fd_s * 1e3, 4)},
         "data": f"seeded: records fill_bytes of Shake256Prg(b'via-gpu/{tag.decode()}/db'), keys Client::setup and "
                 f"queries Client::query of b'via-gpu/{tag.decode()}/setup', distinct indices uniform_below of "
                 f"b'via-gpu/{tag.decode()}/query'",
-        "recovered": bool(ok),
-        "config": {"workload": f"VIA-B, {nb * T} indices per step as {nb} batch(es) of T={T}, record degree "
-                               f"{pp.n_rec} ({pp.n_rec // 2} B records), I={I} J={J}, prepared DB "
-                               f"{I * J * n * 5 / 2**30:.2f} GiB as 7-bit limb fragments "
-                               f"({I * J * n * 8 / 2**30:.2f} GiB as u64 residues)",
-                   "num_rows": I, "num_cols": J, "T": T},
-        "setup_s": round(t_setup, 3),
+    }
+
+
+def vb_run(via_gpu, args):
+    """VIA-B mode (--vb T): one JSON line for batch size T."""
+    ctx = via_gpu.Context(0)
+    r = vb_measure(via_gpu, ctx, args.vb, args.rows, args.cols, args.indices, args.steps, args.warmup)
+    ctx.close()
+    out = {
+        "metric": "VIA-B records/sec (answer_batch of T seeded indices) + p50/p99 batch answer latency",
+        "value": r["records_per_s"], "unit": "records/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * r["T"] * r["batches_per_step"] / r["records_per_s"], 3),
+        "batch_answers_per_s": r["batch_answers_per_s"], "T": r["T"], "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int (exact modular, residues < 2^38 in fp64 lanes)", "data": r["data"],
+        "recovered": r["recovered"], "roofline": r["roofline"],
+        "config": {"workload": r["workload"], "num_rows": args.rows, "num_cols": args.cols, "T": r["T"]},
+        "setup_s": r["setup_s"],
     }
     print(json.dumps(out))
-    for d in d_q + d_m + d_b:
-        ctx.free(d)
-    ctx.close()
 
 
 NTT_PRIMES = (268369921, 536608769)
@@ -546,6 +600,17 @@ def main():
     ap.add_argument("--polys", type=int, default=8192, help="NTT microbench: RNS polynomial pairs")
     ap.add_argument("--ntt-sample", type=int, default=8192, help="NTT microbench: products checked on the CPU")
     ap.add_argument("--keygen", action="store_true", help="client key generation and query encryption")
+    ap.add_argument("--lat-samples", type=int, default=400,
+                    help="one query per pass: answers timed one by one for the latency percentiles")
+    ap.add_argument("--host-samples", type=int, default=100,
+                    help="one query per pass: answers with host query and answer buffers (via_answer, PCIe included)")
+    ap.add_argument("--vb-sweep", type=str, default="8,32,64",
+                    help="default line: VIA-B batch sizes measured at --vb-rows x --vb-cols ('' to skip)")
+    ap.add_argument("--vb-rows", type=int, default=32)
+    ap.add_argument("--vb-cols", type=int, default=4096)
+    ap.add_argument("--serve-qps", type=float, default=80.0,
+                    help="default line: offered request rate of the mixed serving trace (0 to skip)")
+    ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one device copy of known size (256 MiB; one NTT operand "
                          "in --ntt mode): a plain copy kernel for counter calibration")
@@ -734,6 +799,37 @@ def main():
             "u64_equivalent_GBs": round(fd_u64_bytes / fd_s / 1e9, 1),
         },
     }
+    if Q == 1 and args.lat_samples > 0:
+        # the latency percentiles over many single answers (device buffers)
+        slat = []
+        for _ in range(args.lat_samples):
+            t0 = time.perf_counter()
+            answer()
+            ctx.synchronize()
+            slat.append(time.perf_counter() - t0)
+        slat_ms = np.array(slat) * 1e3
+        out["latency"] = {"samples": len(slat), "p50_ms": round(float(np.percentile(slat_ms, 50)), 3),
+                          "p99_ms": round(float(np.percentile(slat_ms, 99)), 3),
+                          "max_ms": round(float(slat_ms.max()), 3)}
+        out["p50_ms"], out["p99_ms"] = out["latency"]["p50_ms"], out["latency"]["p99_ms"]
+        out["p_samples"] = len(slat)
+    if Q == 1 and args.host_samples > 0:
+        # Server::answer's own contract: query and answer in host memory, the
+        # ~1.2 MB of LWEs crossing PCIe inside each answer (via_answer)
+        server = via_gpu.Server(ctx, pp, db, keys)
+        server.answer(lwes)
+        hlat = []
+        t_h = time.perf_counter()
+        for _ in range(args.host_samples):
+            t0 = time.perf_counter()
+            hm, hb = server.answer(lwes)
+            hlat.append(time.perf_counter() - t0)
+        h_el = time.perf_counter() - t_h
+        hlat_ms = np.array(hlat) * 1e3
+        out["host_buffers"] = {"answers_per_s": round(args.host_samples / h_el, 3), "samples": args.host_samples,
+                               "p50_ms": round(float(np.percentile(hlat_ms, 50)), 3),
+                               "p99_ms": round(float(np.percentile(hlat_ms, 99)), 3),
+                               "same_answer": bool(np.array_equal(hm, gpu_mask) and np.array_equal(hb, gpu_body))}
     if args.graphs:
         # the same answers replayed from a captured hipGraph
         ctx.set_graphs(True)
@@ -755,6 +851,11 @@ def main():
     if args.batched > 1 and Q == 1:
         out["batched"] = batched_run(via_gpu, ctx, pp, db, args.batched, args)
         out["batched"]["vs_one_query"] = round(out["batched"]["answers_per_s"] / out["value"], 2)
+    if Q == 1 and args.vb_sweep:
+        out["via_b"] = [vb_measure(via_gpu, ctx, int(T), args.vb_rows, args.vb_cols, 64, 2, 1)
+                        for T in args.vb_sweep.split(",")]
+    if Q == 1 and args.serve_qps > 0:
+        out["serving"] = serving_measure(via_gpu, ctx, pp, db, args)
     if not args.no_cpu:
         am, ab, t_cpu, t_cpu_setup, cpu_steps = cpu_baseline(pp, cascade, conv, rsk, lwes, record_bytes, n_rec)
         from oracle.vo import Oracle

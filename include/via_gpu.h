@@ -91,7 +91,9 @@ void via_ctx_destroy(via_ctx *ctx);
  * this context (answer.rs:166-343 span names step1..step7), by HIP events. */
 via_status via_ctx_step_times(via_ctx *ctx, double *t7);
 /* Seconds of the last FirstDim database-stream kernel alone (HIP events on
- * the context stream, around the one launch that reads the I*J cells). */
+ * the context stream, around the one launch that reads the I*J cells for a
+ * pass of up to 32 queries; for a VIA-B batch of more than 32 queries, its
+ * last pass). */
 via_status via_ctx_first_dim_time(via_ctx *ctx, double *seconds);
 /* hipGraph mode (no counterpart in the reference), off by default. When on,
  * via_answer_device / via_answer_many_device run a call eagerly the first
@@ -190,12 +192,14 @@ via_status via_answer_many_device(via_ctx *ctx, const via_db *db, uint32_t nq, c
  * (the number of digit groups, i.e. workgroups splitting the 18 digits of one
  * gadget product, and the kernel variant: 256 threads with one digit
  * transform at a time in radix-8 or radix-16 passes, 512 threads in radix-8
- * passes, or 1024 threads with several side by side), reps timed
+ * passes, 1024 threads with several side by side, or the 256- and 512-thread
+ * radix-8 kernels with one workgroup per RNS limb), reps timed
  * runs per candidate, and keeps the fastest per step for batches of nq
  * queries. groups_out and variant_out (12 entries each: the eleven
  * cascade steps and the RLWE -> RGSW conversion; variant 0 = 256 threads
  * radix 8, 2 = 256 threads radix 16, 3 = 512 threads radix 8, 1 = 1024
- * threads) receive the choice
+ * threads, 4 = 256 threads per limb, 5 = 512 threads per limb) receive the
+ * choice
  * when not null. The choice serves every context of the process on the same device
  * (the serving loop's included); an untuned batch size takes the shapes of
  * the largest tuned size below it. The answers written meanwhile are those

@@ -196,7 +196,7 @@ def test_autotuned_answers_match(ctx):
     d_b = [tctx.alloc(8 * pp.n2) for _ in cases]
     shapes = tctx.autotune(db, keys, d_q, n_lwe, d_m, d_b, reps=1)
     assert len(shapes) == 12
-    assert all((v in (0, 2, 3) and g in (1, 2, 3, 4, 6, 9, 18)) or (v == 1 and g in (1, 2, 3, 5))
+    assert all((v in (0, 2, 3, 4, 5) and g in (1, 2, 3, 4, 6, 9, 18)) or (v == 1 and g in (1, 2, 3, 5))
                for g, v in shapes)
     masks, bodies = via_gpu.answer_many(tctx, db, keys, [c.lwes for c in cases], pp)
     for c, m, b in zip(cases, masks, bodies):
@@ -208,8 +208,9 @@ def test_autotuned_answers_match(ctx):
 
 def test_every_mac_variant_matches(ctx):
     # each gadget-MAC kernel (256 threads radix 8 / radix 16, 512 threads,
-    # 1024 threads) pinned for every decompression step gives the oracle's
-    # answers; shapes the tuner never offers are refused
+    # 1024 threads, 256 and 512 threads with one workgroup per limb) pinned
+    # for every decompression step gives the oracle's answers; shapes the
+    # tuner never offers are refused
     import via_gpu
 
     cases = [paper_case(4, 4, seed=b"via-gpu/variant/client%d" % q, index=(5 * q + 9) % 64, records="random",
@@ -219,12 +220,12 @@ def test_every_mac_variant_matches(ctx):
     db = via_gpu.PreparedDb(vctx, pp, records=cases[0].records)
     keys = [via_gpu.PreparedKeys(vctx, pp, c.cascade_key, c.conv_key, c.rsk) for c in cases]
     try:
-        for variant, groups in [(0, 2), (1, 2), (2, 3), (3, 1), (3, 6)]:
+        for variant, groups in [(0, 2), (1, 2), (2, 3), (3, 1), (3, 6), (4, 1), (4, 3), (5, 2), (5, 18)]:
             vctx.set_tuning(2, [(groups, variant)] * 12)
             masks, bodies = via_gpu.answer_many(vctx, db, keys, [c.lwes for c in cases], pp)
             for c, m, b in zip(cases, masks, bodies):
                 assert np.array_equal(m, c.ans_mask) and np.array_equal(b, c.ans_body), (variant, groups)
-        for bad in [(7, 0), (4, 1), (2, 4)]:
+        for bad in [(7, 0), (4, 1), (2, 6)]:
             with pytest.raises(via_gpu.ViaError):
                 vctx.set_tuning(2, [bad] * 12)
     finally:

@@ -50,6 +50,7 @@ struct StepTune {
     int wide = 0;
     int radix16 = 0;
     int t512 = 0;  // the 512-thread kernel (radix-8 passes)
+    int lsplit = 0;  // one workgroup per RNS limb (with the 256- or 512-thread kernel)
 };
 
 // Buffers of the VIA-B repack, sized for batches of up to T queries.
@@ -116,7 +117,8 @@ struct via_ctx {
         hipGraphExec_t exec = nullptr;
     };
     std::vector<GraphEntry> graph_cache;
-    bool have_times = false;
+    bool have_times = false;     // step events of the last answer are valid
+    bool have_fd_time = false;   // the FirstDim events of the last database pass are valid
     // pointers to the last answer's stage buffers
     const double *stage_ptr[7] = {};
     size_t stage_words[7] = {};

@@ -122,6 +122,7 @@ struct MacArgs {
     int wide;                // 1: digit transforms of a group side by side (k_gadget_mac_wide)
     int radix16;             // 256-thread kernel: radix-16 transform passes (else radix 8)
     int t512;                // 512-thread kernel with radix-8 passes
+    int lsplit;              // two limbs: one workgroup per limb (not with `wide`)
 };
 void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t s);
 

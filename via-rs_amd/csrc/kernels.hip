@@ -73,23 +73,32 @@ void launch_lwe_digits(const RingDev &r1, const Gadget &g, const DigitPlan &dp, 
 // Gadget-product multiply-accumulate over a digit range (RLevEval::
 // gadget_product, rlev.rs:259-283, without the inverse): workgroup (item,
 // group, tile) adds NTT(digit_k) (.) key[L-1-k] for k in its group. Inputs
-// that are embedded (x -> x^2) transform n/2 coefficients.
-template <int NL, int MAXR, int TPB>
+// that are embedded (x -> x^2) transform n/2 coefficients. LSPLIT (NL = 2):
+// workgroup row blockIdx.y handles limb blockIdx.y alone (the two limbs'
+// transforms and products are independent), holding half the accumulators
+// and LDS of the two-limb form for the same total work.
+template <int NL, int MAXR, int TPB, int LSPLIT = 0>
 __global__ __launch_bounds__(TPB) void k_gadget_mac(RingDev r, Gadget g, MacArgs a) {
     constexpr int PER = kSlots / TPB;  // slots per thread
-    __shared__ double s[lds_words(NL * kSlots)];
+    constexpr int NC = LSPLIT ? 1 : NL;  // limbs this workgroup computes
+    __shared__ double s[lds_words(NC * kSlots)];
     const MacTask task = mac_task(blockIdx.x, a.items, a.items_per_key, a.groups, a.tiles);
     const int item = task.item, grp = task.grp, tile = task.tile;
+    const int lb = LSPLIT ? (int)blockIdx.y : 0;  // first limb of the workgroup
     const int S = kSlots;
     const int cin = a.embed ? S / 2 : S;
     const int nt = a.embed ? r.n / 2 : r.n;
     const int lognt = a.embed ? r.logn - 1 : r.logn;
-    const NttLimbs T = limbs_fwd(r);
-    const FMod m0 = r.t[0].m, m1 = r.t[1].m;
+    NttLimbs T = limbs_fwd(r);
+    if (LSPLIT) {
+        T.tw[0] = T.tw[1] = r.t[lb].fwd;
+        T.m[0] = T.m[1] = r.t[lb].m;
+    }
+    const FMod m0 = r.t[lb].m, m1 = r.t[NC == 2 ? 1 : lb].m;
     const int k0 = grp * g.L / a.groups, k1 = (grp + 1) * g.L / a.groups;
-    double am[NL][PER], ab[NL][PER];
+    double am[NC][PER], ab[NC][PER];
 #pragma unroll
-    for (int l = 0; l < NL; l++)
+    for (int l = 0; l < NC; l++)
 #pragma unroll
         for (int e = 0; e < PER; e++) am[l][e] = ab[l][e] = 0.0;
     const int16_t *dig = a.dig + (size_t)item * g.L * a.dig_coeffs + (size_t)tile * cin;
@@ -100,16 +109,16 @@ __global__ __launch_bounds__(TPB) void k_gadget_mac(RingDev r, Gadget g, MacArgs
             // signed digits are valid NTT inputs (the lazy transform takes (-2q, 2q))
             const double d = (double)dk[c];
             s[lds_pad(c)] = d;
-            if (NL == 2) s[lds_pad(cin + c)] = d;
+            if (NC == 2) s[lds_pad(cin + c)] = d;
         }
         __syncthreads();
-        ntt_fwd<NL, MAXR>(s, cin, nt, lognt, T);
+        ntt_fwd<NC, MAXR>(s, cin, nt, lognt, T);
         const int ks = g.L - 1 - k;
 #pragma unroll
-        for (int l = 0; l < NL; l++) {
+        for (int l = 0; l < NC; l++) {
             const FMod md = l ? m1 : m0;
-            const double *km = key + ((size_t)(ks * 2 + 0) * NL + l) * a.key_slots;
-            const double *kb = key + ((size_t)(ks * 2 + 1) * NL + l) * a.key_slots;
+            const double *km = key + ((size_t)(ks * 2 + 0) * NL + lb + l) * a.key_slots;
+            const double *kb = key + ((size_t)(ks * 2 + 1) * NL + lb + l) * a.key_slots;
 #pragma unroll
             for (int e = 0; e < PER; e++) {
                 const int sg = threadIdx.x + e * TPB;
@@ -122,15 +131,15 @@ __global__ __launch_bounds__(TPB) void k_gadget_mac(RingDev r, Gadget g, MacArgs
     }
     const size_t ot = ((size_t)grp * a.items + item) * a.tiles + tile;
 #pragma unroll
-    for (int l = 0; l < NL; l++)
+    for (int l = 0; l < NC; l++)
 #pragma unroll
         for (int e = 0; e < PER; e++) {
             const int sg = threadIdx.x + e * TPB;
             const FMod md = l ? m1 : m0;
             am[l][e] = mod_reduce_small(am[l][e], md);
             ab[l][e] = mod_reduce_small(ab[l][e], md);
-            a.out_mask[(ot * NL + l) * S + sg] = am[l][e];
-            if (!a.reduce_body) a.out_body[(ot * NL + l) * S + sg] = ab[l][e];
+            a.out_mask[(ot * NL + lb + l) * S + sg] = am[l][e];
+            if (!a.reduce_body) a.out_body[(ot * NL + lb + l) * S + sg] = ab[l][e];
             else s[lds_pad(l * S + sg)] = ab[l][e];
         }
     if (!a.reduce_body) return;
@@ -138,7 +147,7 @@ __global__ __launch_bounds__(TPB) void k_gadget_mac(RingDev r, Gadget g, MacArgs
     // Sum the S/n segment bodies of the tile (exact modular additions).
     const int n = r.n;
     for (int half = (S / n) >> 1; half >= 1; half >>= 1) {
-        for (int t = threadIdx.x; t < NL * half * n; t += blockDim.x) {
+        for (int t = threadIdx.x; t < NC * half * n; t += blockDim.x) {
             const int l = t / (half * n);
             const int rem = t - l * half * n;
             const FMod md = l ? m1 : m0;
@@ -146,9 +155,9 @@ __global__ __launch_bounds__(TPB) void k_gadget_mac(RingDev r, Gadget g, MacArgs
         }
         __syncthreads();
     }
-    for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
+    for (int t = threadIdx.x; t < NC * n; t += blockDim.x) {
         const int l = t / n, c = t - l * n;
-        a.out_body[(ot * NL + l) * n + c] = s[lds_pad(l * S + c)];
+        a.out_body[(ot * NL + lb + l) * n + c] = s[lds_pad(l * S + c)];
     }
 }
 
@@ -252,7 +261,16 @@ void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipS
     // radix-16 passes (a.radix16) trade registers for fewer LDS round trips;
     // 512 threads (a.t512) hold half the accumulators per thread, so more
     // wavefronts fit each SIMD
-    if (r.nl == 2) {
+    if (r.nl == 2 && a.lsplit) {
+        // one workgroup per limb (grid row = limb)
+        const dim3 g2(grid.x, 2);
+        if (a.t512)
+            hipLaunchKernelGGL((k_gadget_mac<2, 3, 512, 1>), g2, dim3(512), 0, st, r, g, a);
+        else if (a.radix16)
+            hipLaunchKernelGGL((k_gadget_mac<2, 4, kThreads, 1>), g2, dim3(kThreads), 0, st, r, g, a);
+        else
+            hipLaunchKernelGGL((k_gadget_mac<2, 3, kThreads, 1>), g2, dim3(kThreads), 0, st, r, g, a);
+    } else if (r.nl == 2) {
         if (a.t512)
             hipLaunchKernelGGL((k_gadget_mac<2, 3, 512>), grid, dim3(512), 0, st, r, g, a);
         else if (a.radix16)

@@ -310,7 +310,9 @@ void free_ws(Workspace &w) {
 // L), as long as a launch stays within 8192 workgroups (two tiles per item).
 std::vector<StepTune> tune_candidates(int items, int L) {
     std::vector<StepTune> out;
-    for (int variant = 0; variant < 4; variant++)
+    // variant: 0 256 threads radix 8, 1 radix 16, 2 wide (1024 threads), 3 512
+    // threads, 4 and 5 the 256- and 512-thread kernels with one workgroup per limb
+    for (int variant = 0; variant < 6; variant++)
         for (int g : variant == 2 ? std::initializer_list<int>{1, 2, 3, 5}
                                   : std::initializer_list<int>{1, 2, 3, 4, 6, 9, 18})
             if (g <= L && (long)items * 2 * g <= 8192) {
@@ -318,7 +320,8 @@ std::vector<StepTune> tune_candidates(int items, int L) {
                 t.groups = g;
                 t.wide = variant == 2;
                 t.radix16 = variant == 1;
-                t.t512 = variant == 3;
+                t.t512 = variant == 3 || variant == 5;
+                t.lsplit = variant >= 4;
                 out.push_back(t);
             }
     if (out.empty()) out.push_back(StepTune());
@@ -605,7 +608,7 @@ via_status via_ctx_first_dim_time(via_ctx *c, double *seconds) {
     if (!c || !seconds) return VIA_E_INVALID;
     CallCtx cc;
     hold_ctx(c, &cc);
-    if (!c->have_times) return VIA_E_INVALID;
+    if (!c->have_fd_time) return VIA_E_INVALID;
     HIP_TRY(hipStreamSynchronize(c->stream));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev_fd[0], c->ev_fd[1]));
@@ -1158,6 +1161,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         a.wide = step_tune[step].wide;
         a.radix16 = step_tune[step].radix16;
         a.t512 = step_tune[step].t512;
+        a.lsplit = step_tune[step].lsplit;
         a.out_mask = w.pm;
         a.out_body = w.pb;
         a.reduce_body = 1;
@@ -1195,6 +1199,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         a.wide = step_tune[kDecompSteps - 1].wide;
         a.radix16 = step_tune[kDecompSteps - 1].radix16;
         a.t512 = step_tune[kDecompSteps - 1].t512;
+        a.lsplit = step_tune[kDecompSteps - 1].lsplit;
         a.out_mask = w.pm;
         a.out_body = w.pb;
         a.reduce_body = 0;
@@ -1277,6 +1282,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     REC_EVENT(c->ev_fd[0], st);
     launch_fd_mfma(db->limbs, w.sell, w.fdk, fp, m2, st);
     REC_EVENT(c->ev_fd[1], st);
+    if (!c->capturing) c->have_fd_time = true;
     launch_fd_gather(w.fdk, w.fd, fp, m2, st);
     launch_ntt(r2, w.fd, (size_t)Q * J * 2, 1, true, st);
     c->stage_ptr[3] = w.fd;
@@ -1749,13 +1755,14 @@ via_status via_ctx_set_tuning(via_ctx *c, uint32_t nq, const int *groups, const 
     std::vector<StepTune> steps(kDecompSteps);
     for (int i = 0; i < kDecompSteps; i++) {
         const int v = variants[i], gr = groups[i];
-        if (v < 0 || v > 3 || gr < 1 || gr > 18) return VIA_E_INVALID;
+        if (v < 0 || v > 5 || gr < 1 || gr > 18) return VIA_E_INVALID;
         if (v == 1 && gr != 1 && gr != 2 && gr != 3 && gr != 5) return VIA_E_INVALID;
         if (v != 1 && gr != 1 && gr != 2 && gr != 3 && gr != 4 && gr != 6 && gr != 9 && gr != 18) return VIA_E_INVALID;
         steps[i].groups = gr;
         steps[i].wide = v == 1;
         steps[i].radix16 = v == 2;
-        steps[i].t512 = v == 3;
+        steps[i].t512 = v == 3 || v == 5;
+        steps[i].lsplit = v >= 4;
     }
     tuned_store(c->device, (int)nq, steps);
     return VIA_OK;
@@ -1808,7 +1815,9 @@ via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via
             if (t[ci][i] < t[bi][i]) bi = ci;
         best[i] = cands[bi];
         if (groups_out) groups_out[i] = best[i].groups;
-        if (variant_out) variant_out[i] = best[i].wide ? 1 : (best[i].radix16 ? 2 : (best[i].t512 ? 3 : 0));
+        if (variant_out)
+            variant_out[i] = best[i].lsplit ? (best[i].t512 ? 5 : 4)
+                                            : best[i].wide ? 1 : (best[i].radix16 ? 2 : (best[i].t512 ? 3 : 0));
     }
     tuned_store(c->device, Q, best);
     return VIA_OK;
