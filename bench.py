@@ -518,6 +518,27 @@ def keygen_run(via_gpu, args):
     ctx.close()
 
 
+def tree_modes(ctx, answer, n=30):
+    """The DMux and CMux step times (ms, HIP events) of `n` answers with the
+    trees launched level by level (mode 0) and as one persistent cooperative
+    launch per tree (mode 1); the context's mode is restored to the tuned
+    choice afterwards."""
+    out = {}
+    for mode in (0, 1):
+        ctx.set_tree_mode(mode)
+        answer()
+        ctx.synchronize()
+        acc = np.zeros(7)
+        for _ in range(n):
+            answer()
+            ctx.synchronize()
+            acc += np.array(ctx.step_times())
+        out["persistent" if mode else "per_level"] = {"dmux_ms": round(acc[1] / n * 1e3, 4),
+                                                      "cmux_ms": round(acc[4] / n * 1e3, 4)}
+    ctx.set_tree_mode(-1)
+    return out
+
+
 def batched_run(via_gpu, ctx, pp, db, Q, args, seed=31):
     """Q queries from Q distinct clients answered per pass (via_answer_many):
     client q's keys are Client::setup over b"via-gpu/<tag>/batch/setup/<q>",
@@ -567,12 +588,14 @@ def batched_run(via_gpu, ctx, pp, db, Q, args, seed=31):
         lat.append(time.perf_counter() - t0)
     elapsed = time.perf_counter() - t_start
     lat_ms = np.array(lat) * 1e3
+    trees = tree_modes(ctx, answer, n=5)
     for d in d_q + d_m + d_b:
         ctx.free(d)
     for k in keys:
         k.close()
     return {"queries_per_pass": Q, "answers_per_s": round(steps * Q / elapsed, 2), "passes": steps,
-            "p50_ms": round(float(np.percentile(lat_ms, 50)), 3), "p99_ms": round(float(np.percentile(lat_ms, 99)), 3)}
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 3), "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+            "tree_tuning": ctx.tree_tuning(Q), "tree_modes": trees}
 
 
 def main():
@@ -799,6 +822,8 @@ def main():
             "u64_equivalent_GBs": round(fd_u64_bytes / fd_s / 1e9, 1),
         },
     }
+    out["tree_tuning"] = ctx.tree_tuning(Q)
+    out["tree_modes"] = tree_modes(ctx, answer)
     if Q == 1 and args.lat_samples > 0:
         # the latency percentiles over many single answers (device buffers)
         slat = []

@@ -207,6 +207,22 @@ via_status via_answer_many_device(via_ctx *ctx, const via_db *db, uint32_t nq, c
 via_status via_ctx_autotune(via_ctx *ctx, const via_db *db, uint32_t nq, const via_keys *const *keys,
                             const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_masks,
                             uint64_t *const *d_bodies, int reps, int *groups_out, int *variant_out);
+/* Tree launch mode (no counterpart in the reference, whose trees are loops,
+ * mux.rs:167-251): mode 1 runs every level of the DMux tree, and of the CMux
+ * tree, in one cooperative launch whose workgroups wait for each level at a
+ * bounded grid barrier (k_tree_persistent); mode 0 launches level by level;
+ * mode -1 (the default) takes the choice via_ctx_autotune stored for the
+ * batch size, or level by level when none is stored. A barrier wait that
+ * gives up is reported by the call (host buffers) or the next
+ * via_ctx_synchronize as VIA_E_DEVICE. Answers are the same in every mode. */
+via_status via_ctx_set_tree_mode(via_ctx *ctx, int mode);
+/* The tree modes via_ctx_autotune chose for batches of nq queries (1 =
+ * persistent) for the DMux and the CMux tree; VIA_E_INVALID when none. */
+via_status via_ctx_tree_tuning(via_ctx *ctx, uint32_t nq, int *dmux, int *cmux);
+/* Test switch: polls a persistent tree's barrier wait makes before it gives
+ * up (default 2^24; 0 gives up at the first wait that finds the level
+ * unfinished). */
+via_status via_debug_set_tree_spin(via_ctx *ctx, long spin);
 /* No counterpart in the reference: stores launch shapes for batches of nq
  * queries as via_ctx_autotune would (12 entries each, the same encoding as
  * its outputs), e.g. a choice saved from an earlier tuning run; groups null

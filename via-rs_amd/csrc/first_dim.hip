@@ -427,4 +427,56 @@ void launch_fd_gather(const double *out_k, double *out_eval, const FdPlan &p, FM
     hipLaunchKernelGGL(k_fd_gather, grid, dim3(256), 0, st, out_k, out_eval, p, m);
 }
 
+// The segment sum, the transpose and the inverse transform of the FirstDim
+// outputs in one pass (first_dim.rs:174-181 inverts each output ciphertext):
+// workgroup b takes IPP consecutive output polys idx = qp * J + j (qp = 2q +
+// part), reads their n slots straight from the slot-major segment outputs
+// (IPP * 8 bytes per slot row and segment), adds the segments, transforms
+// the polys back in LDS and writes them coefficient form to [q][j][part][k].
+// The 16 / IPP workgroups that read one 128-byte row of 16 polys get block
+// ids congruent mod 8, i.e. one XCD and its L2, and run side by side (groups
+// of 8 rows), so each row is fetched from HBM once.
+template <int IPP, int TPB>
+__global__ __launch_bounds__(TPB) void k_fd_intt(const double *__restrict__ ok, double *__restrict__ out, FdPlan p,
+                                                 RingDev r2) {
+    __shared__ double s[lds_words(IPP * kSlots)];
+    constexpr int LPW = 16 / IPP;  // workgroups per row of 16 polys
+    const size_t P = (size_t)p.Q * 2 * p.J;
+    const int n = p.n;
+    const size_t b = blockIdx.x, grp = b / (8 * LPW), wi = b % (8 * LPW);
+    const size_t tile = grp * 8 * LPW + (wi & 7) * LPW + (wi >> 3);
+    const size_t idx0 = tile * IPP;
+    if (idx0 >= P) return;  // the whole workgroup, before any barrier
+    const int np = (int)(P - idx0 < (size_t)IPP ? P - idx0 : (size_t)IPP);
+    const FMod m = r2.t[0].m;
+    for (int e = threadIdx.x; e < IPP * n; e += TPB) {
+        const int pp = e % IPP, k = e / IPP;
+        double v = 0.0;
+        if (pp < np)
+            for (int sg = 0; sg < p.nseg; sg++) v += ok[((size_t)sg * n + k) * P + idx0 + pp];
+        s[lds_pad(pp * n + k)] = mod_reduce_small(v, m);
+    }
+    __syncthreads();
+    lds_ntt_inv_noscale<1, 4>(s, IPP * n, r2);
+    const double ni = r2.t[0].n_inv;
+    for (int pp = 0; pp < np; pp++) {
+        const size_t idx = idx0 + pp, qp = idx / p.J, j = idx % p.J;
+        double *o = out + (((qp >> 1) * p.J + j) * 2 + (qp & 1)) * n;
+        for (int k = threadIdx.x; k < n; k += TPB) o[k] = mod_mul(s[lds_pad(pp * n + k)], ni, m);
+    }
+}
+
+void launch_fd_intt(const double *out_k, double *out, const FdPlan &p, const RingDev &r2, hipStream_t st) {
+    const size_t P = (size_t)p.Q * 2 * p.J;
+    // four polys per workgroup when that still gives a thousand workgroups,
+    // else fewer, so small passes fill the 256 CUs
+    auto blocks = [&](int ipp) { return ((P + ipp - 1) / ipp + 127) / 128 * 128; };
+    if (P >= 4 * 1024)
+        hipLaunchKernelGGL((k_fd_intt<4, 512>), dim3((unsigned)blocks(4)), dim3(512), 0, st, out_k, out, p, r2);
+    else if (P >= 2 * 512)
+        hipLaunchKernelGGL((k_fd_intt<2, 256>), dim3((unsigned)blocks(2)), dim3(256), 0, st, out_k, out, p, r2);
+    else
+        hipLaunchKernelGGL((k_fd_intt<1, 256>), dim3((unsigned)blocks(1)), dim3(256), 0, st, out_k, out, p, r2);
+}
+
 }  // namespace via

@@ -98,6 +98,14 @@ struct via_ctx {
     // then looked up, never built)
     void *scratch = nullptr;
     bool no_alloc = false;
+    // persistent trees (k_tree_persistent): the barrier words, whether this
+    // context may use them (not the serving loop's slots, whose streams run
+    // side by side), and a forced mode (-1: the tuned choice, 0: one launch
+    // per level, 1: one launch per tree). bad_h[2] is the barrier-stall flag.
+    unsigned *tree_bar = nullptr;
+    bool allow_persistent = true;
+    int tree_override = -1;
+    long tree_spin = 1L << 24;  // polls before a barrier wait gives up
     Workspace ws;
     RepackWs rw;
     hipEvent_t ev[8] = {};
@@ -159,6 +167,14 @@ void tuned_erase(int device, int Q);
 via_status ctx_ring(via_ctx *c, uint32_t n, const uint64_t *q, int nl, RingDev *r);
 void tuned_store(int device, int Q, const std::vector<StepTune> &steps);
 bool tuned_lookup(int device, int Q, std::vector<StepTune> *steps);
+// Tree launch mode chosen by via_ctx_autotune per device and batch size:
+// persistent (one launch per tree) or level by level, for the DMux and the
+// CMux tree; the same lookup rule as the decompression shapes.
+struct TreeTune {
+    int dmux = 0, cmux = 0;
+};
+void tree_tuned_store(int device, int Q, const TreeTune &t);
+bool tree_tuned_lookup(int device, int Q, TreeTune *t);
 // Incremented by every tuned_store: captured graphs bake in launch shapes.
 uint64_t tuned_generation();
 // Process-wide unique ids for databases and key sets.

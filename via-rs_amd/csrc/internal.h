@@ -163,6 +163,31 @@ struct GateArgs {
 };
 void launch_gate(const RingDev &r, const Gadget &g, const GateArgs &a, hipStream_t s);
 
+// Every level of one DMux or CMux tree in one cooperative launch
+// (k_tree_persistent): level v's gates read lev[v].in and write lev[v].out
+// with the RGSW lev[v].rgsw; the grid waits between levels at a barrier
+// (bar: two device words, zero before the first use and left consistent by
+// every launch that does not fail) whose waits give up after `spin` polls and
+// set *fail (host-visible).
+constexpr int kMaxTreeLevels = 16;
+struct TreeLevel {
+    const double *rgsw;
+    const double *in;
+    double *out;
+    int items_per_query;
+};
+struct TreeArgs {
+    GateArgs gate;  // mode, strides, queries, shift (the per-level fields are in lev)
+    int nlev;
+    TreeLevel lev[kMaxTreeLevels];
+    unsigned *bar;
+    int *fail;
+    long spin;
+};
+// false when the device cannot run the grid co-resident (the caller then
+// launches level by level)
+bool launch_tree_persistent(const RingDev &r, const Gadget &g, const TreeArgs &a, hipStream_t s);
+
 // Gadget digits of Delta (the trivial DMux root) and Delta per limb.
 struct DeltaDigits {
     int32_t d[8];
@@ -202,6 +227,9 @@ void launch_unpack_db(const uint8_t *dbl, uint64_t *out, const FdPlan &p, hipStr
 void launch_sel_pack(const double *sel_eval, uint8_t *sell, const FdPlan &p, hipStream_t s);
 void launch_fd_mfma(const uint8_t *dbl, const uint8_t *sell, double *out_k, const FdPlan &p, FMod m, hipStream_t s);
 void launch_fd_gather(const double *out_k, double *out_eval, const FdPlan &p, FMod m, hipStream_t s);
+// The segment sum, transpose and inverse transform (r2: q2 at n) of the
+// FirstDim outputs in one launch: out [q][j][part][n] in coefficient form.
+void launch_fd_intt(const double *out_k, double *out, const FdPlan &p, const RingDev &r2, hipStream_t s);
 // Rewrites record slots of whole cells in place (r2: q2 at n1): cell c =
 // cells[x] takes, for slot s < d, record slot_src[x*d + s] of recs (n_rec
 // coefficients each, reduced mod pmod) or keeps its own (-1).

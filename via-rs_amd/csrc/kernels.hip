@@ -561,14 +561,18 @@ __global__ __launch_bounds__(TPB) void k_gate(RingDev r, Gadget g, GateArgs ga) 
 // known, so workgroup 2g + l computes gate g's digits from both limbs and
 // transforms, multiplies and writes limb l only: half the work and latency
 // per workgroup, twice the workgroups on the narrow levels.
+// The body of one fused gate: `blk` is the gate's workgroup index (gate blk,
+// or with SPLIT gate blk >> 1 and limb blk & 1), `s` the workgroup's LDS
+// (lds_words(4 * NLC * kSlots) words). Shared by k_gate_fused (one launch per
+// tree level) and k_tree_persistent (every level in one launch).
 template <int NL, int SPLIT>
-__global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateArgs ga) {
+__device__ __forceinline__ void gate_fused_body(double *s, const RingDev &r, const Gadget &g, const GateArgs &ga,
+                                                int blk) {
     constexpr int S = kSlots;
     constexpr int TPB = 1024;
     constexpr int NLC = SPLIT ? 1 : NL;  // limbs this workgroup computes
-    __shared__ double s[lds_words(4 * NLC * S)];
-    const int gate = SPLIT ? (int)(blockIdx.x >> 1) : (int)blockIdx.x;
-    const int lsel = SPLIT ? (int)(blockIdx.x & 1) : 0;
+    const int gate = SPLIT ? (blk >> 1) : blk;
+    const int lsel = SPLIT ? (blk & 1) : 0;
     const int q = gate / ga.items_per_query;
     const int j = gate - q * ga.items_per_query;
     const int mode = ga.mode, shift = ga.shift;
@@ -711,6 +715,101 @@ __global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateAr
                 out[(size_t)j * 2 * pw + o] = mod_add(bv, p, md);
             }
         }
+}
+
+template <int NL, int SPLIT>
+__global__ __launch_bounds__(1024) void k_gate_fused(RingDev r, Gadget g, GateArgs ga) {
+    __shared__ double s[lds_words(4 * (SPLIT ? 1 : NL) * kSlots)];
+    gate_fused_body<NL, SPLIT>(s, r, g, ga, (int)blockIdx.x);
+}
+
+// ---- persistent tree: every level of a DMux or CMux tree in one launch --------------
+
+// Grid-wide barrier over co-resident workgroups (cooperative launch):
+// bar[0] counts arrivals, bar[1] is the generation. Each wait polls at most
+// `spin` times; a longer wait sets *fail and the workgroup goes on (the host
+// discards the results and reports the failure), so every wave always exits.
+__device__ __forceinline__ void tree_sync(unsigned *bar, unsigned &gen, long spin, int *fail) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned g = gen;
+        if (__hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar[1], g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            long i = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                if (++i > spin) {
+                    __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        gen = g + 1;
+    }
+    __syncthreads();
+}
+
+// Levels lev0 .. of one tree for every query: level v runs its gates over the
+// grid (workgroup b takes gates b, b + gridDim.x, ...), then the grid waits
+// for the level before the next one reads its outputs.
+template <int NL, int SPLIT>
+__global__ __launch_bounds__(1024) void k_tree_persistent(RingDev r, Gadget g, TreeArgs ta) {
+    __shared__ double s[lds_words(4 * (SPLIT ? 1 : NL) * kSlots)];
+    // the generation before this launch's first barrier (no workgroup can
+    // complete that barrier before every one has read it)
+    unsigned gen = __hip_atomic_load(&ta.bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int v = 0; v < ta.nlev; v++) {
+        GateArgs ga = ta.gate;
+        ga.rgsw = ta.lev[v].rgsw;
+        ga.in = ta.lev[v].in;
+        ga.out = ta.lev[v].out;
+        ga.items_per_query = ta.lev[v].items_per_query;
+        const int blocks = ga.items_per_query * ga.queries * (SPLIT ? 2 : 1);
+        for (int b = blockIdx.x; b < blocks; b += gridDim.x) {
+            gate_fused_body<NL, SPLIT>(s, r, g, ga, b);
+            __syncthreads();  // the next gate's phase 1 rewrites the LDS
+        }
+        if (v + 1 < ta.nlev) tree_sync(ta.bar, gen, ta.spin, ta.fail);
+    }
+}
+
+// The gate gadget with its fp64 scale where that is exact (one limb) or
+// decided (two limbs).
+static Gadget gate_gadget(const RingDev &r, const Gadget &g_in) {
+    Gadget g = g_in;
+    g.f64 = 0;
+    if (exact_paths_only()) {
+        // the 128-bit scale for every coefficient
+    } else if (r.nl == 2 && g.bl < ((via_u128)1 << 40)) {
+        g.f64 = 2;
+        g.sf = scale_f64_make((double)(uint64_t)g.bl, r.rns.q1, u128_to_f64(r.rns.big_q));
+    } else if (r.nl == 1 && !g.wide && ((via_u128)(r.q_single / 2 + 1) * g.bl + r.q_single) < ((via_u128)1 << 53)) {
+        g.f64 = 1;
+        g.sf.bl = (double)(uint64_t)g.bl;
+    }
+    return g;
+}
+
+bool launch_tree_persistent(const RingDev &r, const Gadget &g_in, const TreeArgs &a_in, hipStream_t st) {
+    if (g_in.L > 2 || r.n != kSlots || a_in.nlev < 1 || a_in.nlev > kMaxTreeLevels) return false;
+    TreeArgs a = a_in;
+    a.gate.dp = digit_plan(g_in.base, g_in.L);
+    const Gadget g = gate_gadget(r, g_in);
+    // two limbs: one workgroup per limb of each gate (every level)
+    const void *fn = r.nl == 2 ? (const void *)k_tree_persistent<2, 1> : (const void *)k_tree_persistent<1, 0>;
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 1024, 0) != hipSuccess || per_cu < 1)
+        return false;
+    int most = 0;  // workgroups of the widest level
+    for (int v = 0; v < a.nlev; v++)
+        most = std::max(most, a.lev[v].items_per_query * a.gate.queries * (r.nl == 2 ? 2 : 1));
+    const int grid = std::max(1, std::min(most, cus * per_cu));
+    void *args[] = {(void *)&r, (void *)&g, (void *)&a};
+    return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(1024), args, 0, st) == hipSuccess;
 }
 
 void launch_gate(const RingDev &r, const Gadget &g_in, const GateArgs &a_in, hipStream_t st) {

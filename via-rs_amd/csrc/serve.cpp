@@ -267,6 +267,9 @@ via_status via_server_create(int device, const via_params *pp, const via_db *db,
     for (Slot &sl : s->slots) {
         via_status st = via_ctx_create(device, &sl.ctx);
         if (st) return st;
+        // slots overlap on the device: a cooperative grid could not count on
+        // having every CU, so the trees launch level by level
+        sl.ctx->allow_persistent = false;
         if (hipHostMalloc((void **)&sl.h_q, sizeof(uint64_t) * s->q_cap) != hipSuccess ||
             hipHostMalloc((void **)&sl.h_a, sizeof(uint64_t) * 2 * pp->n2 * max_batch) != hipSuccess ||
             hipMalloc((void **)&sl.d_q, sizeof(uint64_t) * s->q_cap) != hipSuccess ||

@@ -542,13 +542,18 @@ via_status via_ctx_create(int device, via_ctx **out) {
             delete c;
             return VIA_E_DEVICE;
         }
-    if (hipHostMalloc((void **)&c->bad_h, 2 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) !=
+    if (hipHostMalloc((void **)&c->bad_h, 3 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) !=
             hipSuccess ||
         hipHostGetDevicePointer((void **)&c->bad_d, c->bad_h, 0) != hipSuccess) {
         delete c;
         return VIA_E_DEVICE;
     }
-    c->bad_h[0] = c->bad_h[1] = 0;
+    c->bad_h[0] = c->bad_h[1] = c->bad_h[2] = 0;
+    if (hipMalloc((void **)&c->tree_bar, 2 * sizeof(unsigned)) != hipSuccess ||
+        hipMemset(c->tree_bar, 0, 2 * sizeof(unsigned)) != hipSuccess) {
+        via_ctx_destroy(c);
+        return VIA_E_NOMEM;
+    }
     if (scratch_setup(device, &c->scratch) != VIA_OK) {
         via_ctx_destroy(c);
         return VIA_E_DEVICE;
@@ -578,13 +583,24 @@ void via_ctx_destroy(via_ctx *c) {
     for (auto &e : c->ev_mark) (void)hipEventDestroy(e);
     for (auto &e : c->ev_dec) (void)hipEventDestroy(e);
     if (c->bad_h) (void)hipHostFree(c->bad_h);
+    if (c->tree_bar) (void)hipFree(c->tree_bar);
     (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+// A persistent tree whose barrier wait gave up left wrong values and the
+// barrier words inconsistent: clear both and report the failure.
+static via_status tree_stall_status(via_ctx *c) {
+    if (!__atomic_exchange_n(&c->bad_h[2], 0, __ATOMIC_ACQ_REL)) return VIA_OK;
+    (void)hipMemsetAsync(c->tree_bar, 0, 2 * sizeof(unsigned), c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    return VIA_E_DEVICE;
 }
 
 via_status via_ctx_synchronize(via_ctx *c) {
     if (!c) return VIA_E_INVALID;
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (tree_stall_status(c)) return VIA_E_DEVICE;
     // a device query since the last synchronisation held a non-canonical word
     if (__atomic_exchange_n(&c->bad_h[0], 0, __ATOMIC_ACQ_REL)) return VIA_E_INVALID;
     return VIA_OK;
@@ -1030,6 +1046,26 @@ void via::tuned_erase(int device, int Q) {
     g_tuned_gen++;
 }
 
+namespace {
+std::map<std::pair<int, int>, TreeTune> g_tree_tuned;  // (device, batch size), under g_tuned_mu
+}  // namespace
+
+void via::tree_tuned_store(int device, int Q, const TreeTune &t) {
+    std::lock_guard<std::mutex> g(g_tuned_mu);
+    g_tree_tuned[{device, Q}] = t;
+    g_tuned_gen++;
+}
+
+bool via::tree_tuned_lookup(int device, int Q, TreeTune *t) {
+    std::lock_guard<std::mutex> g(g_tuned_mu);
+    auto it = g_tree_tuned.upper_bound({device, Q});
+    if (it == g_tree_tuned.begin()) return false;
+    --it;
+    if (it->first.first != device) return false;
+    *t = it->second;
+    return true;
+}
+
 uint64_t via::next_object_id() {
     static std::atomic<uint64_t> next{1};
     return next++;
@@ -1249,7 +1285,43 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
                                    hipMemcpyHostToDevice, st));
         HIP_TRY(hipStreamSynchronize(st));  // triv is a host temporary
     }
-    for (int i = 1; i < nd; i++) {
+    // trees in one persistent launch each: forced by the context's mode, else
+    // the autotuner's choice for this batch size (never while capturing a
+    // graph, nor on the serving loop's slots, whose streams run side by side)
+    bool pers_d = false, pers_c = false;
+    if (!c->capturing && c->allow_persistent) {
+        if (c->tree_override >= 0) {
+            pers_d = pers_c = c->tree_override == 1;
+        } else {
+            TreeTune tt;
+            if (tree_tuned_lookup(c->device, Q, &tt)) {
+                pers_d = tt.dmux != 0;
+                pers_c = tt.cmux != 0;
+            }
+        }
+    }
+    int dmux_from = 1;
+    if (pers_d && nd > 2 && nd - 1 <= kMaxTreeLevels) {
+        TreeArgs ta;
+        memset(&ta, 0, sizeof ta);
+        ta.gate.mode = GATE_DMUX;
+        ta.gate.rgsw_qstride = bits * rgw;
+        ta.gate.in_qstride = I * ctw1;
+        ta.gate.out_qstride = I * ctw1;
+        ta.gate.queries = Q;
+        int x = dc;
+        for (int i = 1; i < nd; i++, x ^= 1)
+            ta.lev[i - 1] = TreeLevel{w.rgsw_eval + (size_t)i * rgw, w.dm[x], w.dm[x ^ 1], 1 << i};
+        ta.nlev = nd - 1;
+        ta.bar = c->tree_bar;
+        ta.fail = c->bad_d + 2;
+        ta.spin = c->tree_spin;
+        if (launch_tree_persistent(r1n, g1, ta, st)) {
+            dc = x;
+            dmux_from = nd;
+        }
+    }
+    for (int i = dmux_from; i < nd; i++) {
         GateArgs ga;
         ga.mode = GATE_DMUX;
         ga.rgsw = w.rgsw_eval + (size_t)i * rgw;
@@ -1283,8 +1355,8 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     launch_fd_mfma(db->limbs, w.sell, w.fdk, fp, m2, st);
     REC_EVENT(c->ev_fd[1], st);
     if (!c->capturing) c->have_fd_time = true;
-    launch_fd_gather(w.fdk, w.fd, fp, m2, st);
-    launch_ntt(r2, w.fd, (size_t)Q * J * 2, 1, true, st);
+    // segment sum, transpose to [q][j][part][k] and inverse transform, one pass
+    launch_fd_intt(w.fdk, w.fd, fp, r2, st);
     c->stage_ptr[3] = w.fd;
     c->stage_words[3] = J * 2 * n;
     REC_EVENT(c->ev[4], st);
@@ -1302,7 +1374,32 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     }
     const double *cm_in = w.fd;
     int cc = 0;
-    for (int b = 0; b < nc; b++) {
+    int cmux_from = 0;
+    if (pers_c && nc > 1 && nc <= kMaxTreeLevels) {
+        TreeArgs ta;
+        memset(&ta, 0, sizeof ta);
+        ta.gate.mode = GATE_CMUX;
+        ta.gate.rgsw_qstride = rq2;
+        ta.gate.in_qstride = J * ctw2;
+        ta.gate.out_qstride = J * ctw2;
+        ta.gate.queries = Q;
+        const double *in = cm_in;
+        int x = cc;
+        for (int b = 0; b < nc; b++, x ^= 1) {
+            ta.lev[b] = TreeLevel{w.rgsw2 + (size_t)b * rgw2, in, w.cm[x], (int)(J >> (b + 1))};
+            in = w.cm[x];
+        }
+        ta.nlev = nc;
+        ta.bar = c->tree_bar;
+        ta.fail = c->bad_d + 2;
+        ta.spin = c->tree_spin;
+        if (launch_tree_persistent(r2, g2, ta, st)) {
+            cm_in = in;
+            cc = x;
+            cmux_from = nc;
+        }
+    }
+    for (int b = cmux_from; b < nc; b++) {
         GateArgs ga;
         ga.mode = GATE_CMUX;
         ga.rgsw = w.rgsw2 + (size_t)b * rgw2;
@@ -1783,6 +1880,7 @@ via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via
         ~Guard() {
             c->time_decomp = false;
             c->tune_override.clear();
+            c->tree_override = -1;
         }
     } guard{c};
     const int Q = (int)nq;
@@ -1820,6 +1918,58 @@ via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via
                                             : best[i].wide ? 1 : (best[i].radix16 ? 2 : (best[i].t512 ? 3 : 0));
     }
     tuned_store(c->device, Q, best);
+    // the tree launch modes with the stored shapes: level by level against
+    // one persistent launch per tree, by the DMux and CMux step events
+    TreeTune tt;
+    if (c->allow_persistent) {
+        c->time_decomp = false;
+        c->tune_override.clear();
+        double td[2] = {0, 0}, tc[2] = {0, 0};
+        for (int mode = 0; mode < 2; mode++) {
+            c->tree_override = mode;
+            for (int r = 0; r <= reps; r++) {
+                if ((s = answer_batch_impl(c, db, Q, keys, d_lwes, n_lwe, d_mask, d_body, nullptr))) return s;
+                HIP_TRY(hipStreamSynchronize(c->stream));
+                if (r == 0) continue;
+                float a = 0, b = 0;
+                HIP_TRY(hipEventElapsedTime(&a, c->ev[1], c->ev[2]));
+                HIP_TRY(hipEventElapsedTime(&b, c->ev[4], c->ev[5]));
+                td[mode] += a;
+                tc[mode] += b;
+            }
+        }
+        c->tree_override = -1;
+        if (tree_stall_status(c) == VIA_OK) {
+            tt.dmux = td[1] < td[0];
+            tt.cmux = tc[1] < tc[0];
+        }
+    }
+    tree_tuned_store(c->device, Q, tt);
+    return VIA_OK;
+}
+
+via_status via_ctx_set_tree_mode(via_ctx *c, int mode) {
+    if (!c || mode < -1 || mode > 1) return VIA_E_INVALID;
+    CallCtx cc;
+    hold_ctx(c, &cc);
+    c->tree_override = mode;
+    return VIA_OK;
+}
+
+via_status via_ctx_tree_tuning(via_ctx *c, uint32_t nq, int *dmux, int *cmux) {
+    if (!c || !dmux || !cmux) return VIA_E_INVALID;
+    TreeTune tt;
+    if (!tree_tuned_lookup(c->device, (int)nq, &tt)) return VIA_E_INVALID;
+    *dmux = tt.dmux;
+    *cmux = tt.cmux;
+    return VIA_OK;
+}
+
+via_status via_debug_set_tree_spin(via_ctx *c, long spin) {
+    if (!c || spin < 0) return VIA_E_INVALID;
+    CallCtx cc;
+    hold_ctx(c, &cc);
+    c->tree_spin = spin;
     return VIA_OK;
 }
 
@@ -1875,6 +2025,10 @@ via_status via_answer_many(via_ctx *root, const via_db *db, uint32_t nq, const v
         HIP_TRY(hipStreamSynchronize(c->stream));
         if ((s = flag.status())) {
             if (err) snprintf(err->msg, sizeof err->msg, "query word not a canonical residue mod q1");
+            return s;
+        }
+        if ((s = tree_stall_status(c))) {
+            if (err) snprintf(err->msg, sizeof err->msg, "persistent tree barrier gave up waiting");
             return s;
         }
     }
@@ -1944,6 +2098,10 @@ via_status via_answer_batch(via_ctx *root, const via_db *db, const via_keys *key
     HIP_TRY(hipStreamSynchronize(c->stream));
     if ((s = flag.status())) {
         if (err) snprintf(err->msg, sizeof err->msg, "query word not a canonical residue mod q1");
+        return s;
+    }
+    if ((s = tree_stall_status(c))) {
+        if (err) snprintf(err->msg, sizeof err->msg, "persistent tree barrier gave up waiting");
         return s;
     }
     return VIA_OK;
@@ -2051,8 +2209,7 @@ via_status via_first_dim(via_ctx *root, const via_db *db, uint32_t nq, const uin
     const FMod m2 = fmod_make(pp->q2);
     launch_sel_pack(w.sel, w.sell, fp, st);
     launch_fd_mfma(db->limbs, w.sell, w.fdk, fp, m2, st);
-    launch_fd_gather(w.fdk, w.fd, fp, m2, st);
-    launch_ntt(r2, w.fd, (size_t)nq * J * 2, 1, true, st);
+    launch_fd_intt(w.fdk, w.fd, fp, r2, st);
     launch_f64_to_u64(w.fd, u, out_words, st);
     HIP_TRY(hipMemcpyAsync(out, u, sizeof(uint64_t) * out_words, hipMemcpyDeviceToHost, st));
     const hipError_t e = hipStreamSynchronize(st);

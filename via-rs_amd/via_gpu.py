@@ -123,6 +123,9 @@ def lib():
             "via_ctx_mark": (C.c_int, [vp, C.c_int]),
             "via_ctx_set_graphs": (C.c_int, [vp, C.c_int]),
             "via_debug_set_exact_paths": (C.c_int, [C.c_int]),
+            "via_ctx_set_tree_mode": (C.c_int, [vp, C.c_int]),
+            "via_ctx_tree_tuning": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "via_debug_set_tree_spin": (C.c_int, [vp, C.c_long]),
             "via_answer_device_noalloc": (C.c_int, [vp, vp, vp, vp, C.c_size_t, vp, vp, C.POINTER(ErrDetail)]),
             "via_ctx_set_tuning": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "via_ctx_autotune": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.c_size_t,
@@ -405,6 +408,16 @@ class Context:
         w = (C.c_int * 12)()
         _check(lib().via_ctx_autotune(self.h, db.h, nq, K, Qp, n_lwe, M, B, reps, g, w))
         return list(zip(list(g), list(w)))
+
+    def set_tree_mode(self, mode):
+        """via_ctx_set_tree_mode: -1 tuned, 0 level by level, 1 persistent."""
+        _check(lib().via_ctx_set_tree_mode(self.h, mode))
+
+    def tree_tuning(self, nq):
+        d, c = C.c_int(), C.c_int()
+        if lib().via_ctx_tree_tuning(self.h, nq, C.byref(d), C.byref(c)) != VIA_OK:
+            return None
+        return {"dmux": d.value, "cmux": c.value}
 
     def set_graphs(self, on):
         """hipGraph mode: repeated identical answers replay a captured graph."""
