@@ -459,6 +459,28 @@ def ntt_run(via_gpu, args):
     ctx.close()
 
 
+def setup_many_measure(via_gpu, ctx, pp, clients=64):
+    """Client::setup of `clients` clients through via_client_setup_many (host
+    threads on lanes of the context), each its own stream
+    b"via-gpu/setup-many/<c>"; client 0's keys compared word for word with a
+    one-at-a-time via_client_setup of the same stream."""
+    prgs = [via_gpu.Prg(b"via-gpu/setup-many/%d" % c) for c in range(clients)]
+    via_gpu.client_setup_many(ctx, pp, prgs[:8])  # warm the lanes
+    for g in prgs:
+        g.close()
+    prgs = [via_gpu.Prg(b"via-gpu/setup-many/%d" % c) for c in range(clients)]
+    t0 = time.perf_counter()
+    many = via_gpu.client_setup_many(ctx, pp, prgs)
+    el = time.perf_counter() - t0
+    for g in prgs:
+        g.close()
+    g = via_gpu.Prg(b"via-gpu/setup-many/0")
+    one = via_gpu.client_setup(ctx, pp, g)
+    g.close()
+    same = all(np.array_equal(one[k], many[0][k]) for k in ("sk1", "sk2", "cascade_key", "conv_key", "rsk"))
+    return {"clients": clients, "setups_per_s": round(clients / el, 2), "s": round(el, 3), "same_as_one": bool(same)}
+
+
 def keygen_run(via_gpu, args):
     """Client::setup and Client::query of the library (PRG stream on the GPU,
     host parse, GPU products) at the paper stack; every key word checked
@@ -494,7 +516,12 @@ def keygen_run(via_gpu, args):
                                f"{pp.query_lwes} LWEs at I={args.rows} J={args.cols}",
                    "num_rows": args.rows, "num_cols": args.cols},
         "query_ms_p50": round(float(np.median(t_query)) * 1e3, 3),
+        "one_at_a_time_setups_per_s": round(args.steps / elapsed, 2),
     }
+    # many clients at once: the throughput a server sees admitting clients
+    out["many"] = setup_many_measure(via_gpu, ctx, pp, clients=max(32, args.steps * 4))
+    out["value"] = out["many"]["setups_per_s"]
+    out["ms_per_step"] = round(1e3 / out["value"], 3)
     if not args.no_cpu:
         from oracle.vo import TERNARY, Oracle, paper_params, u64p
 
@@ -634,6 +661,8 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=80.0,
                     help="default line: offered request rate of the mixed serving trace (0 to skip)")
     ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--keygen-clients", type=int, default=32,
+                    help="default line: Client::setup of this many clients at once (0 to skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one device copy of known size (256 MiB; one NTT operand "
                          "in --ntt mode): a plain copy kernel for counter calibration")
@@ -881,6 +910,8 @@ def main():
                         for T in args.vb_sweep.split(",")]
     if Q == 1 and args.serve_qps > 0:
         out["serving"] = serving_measure(via_gpu, ctx, pp, db, args)
+    if Q == 1 and args.keygen_clients > 0:
+        out["client_setup"] = setup_many_measure(via_gpu, ctx, pp, clients=args.keygen_clients)
     if not args.no_cpu:
         am, ab, t_cpu, t_cpu_setup, cpu_steps = cpu_baseline(pp, cascade, conv, rsk, lwes, record_bytes, n_rec)
         from oracle.vo import Oracle

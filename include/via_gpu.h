@@ -199,11 +199,14 @@ via_status via_answer_many_device(via_ctx *ctx, const via_db *db, uint32_t nq, c
  * cascade steps and the RLWE -> RGSW conversion; variant 0 = 256 threads
  * radix 8, 2 = 256 threads radix 16, 3 = 512 threads radix 8, 1 = 1024
  * threads, 4 = 256 threads per limb, 5 = 512 threads per limb) receive the
- * choice
- * when not null. The choice serves every context of the process on the same device
- * (the serving loop's included); an untuned batch size takes the shapes of
- * the largest tuned size below it. The answers written meanwhile are those
- * of via_answer_many_device. */
+ * choice when not null. With those shapes it then measures, in turn, the
+ * DMux and CMux trees as one persistent launch each against level by level
+ * (via_ctx_tree_tuning), FirstDim's column-block groups per slot (1, 2, 4,
+ * 8) and the gates' limb split (via_ctx_pass_tuning), keeping the faster of
+ * each. The choices serve every context of the process on the same device
+ * (the serving loop's included, whose trees always launch level by level);
+ * an untuned batch size takes the choices of the largest tuned size below
+ * it. The answers written meanwhile are those of via_answer_many_device. */
 via_status via_ctx_autotune(via_ctx *ctx, const via_db *db, uint32_t nq, const via_keys *const *keys,
                             const uint64_t *const *d_lwes, size_t n_lwe, uint64_t *const *d_masks,
                             uint64_t *const *d_bodies, int reps, int *groups_out, int *variant_out);
@@ -219,6 +222,11 @@ via_status via_ctx_set_tree_mode(via_ctx *ctx, int mode);
 /* The tree modes via_ctx_autotune chose for batches of nq queries (1 =
  * persistent) for the DMux and the CMux tree; VIA_E_INVALID when none. */
 via_status via_ctx_tree_tuning(via_ctx *ctx, uint32_t nq, int *dmux, int *cmux);
+/* The FirstDim column-block groups per slot (workgroups sharing one slot's
+ * database block) and the gate limb split (-1: split two-limb tree levels
+ * narrower than the CU count, 0: never, 1: always) that via_ctx_autotune
+ * measured and chose for batches of nq queries; VIA_E_INVALID when none. */
+via_status via_ctx_pass_tuning(via_ctx *ctx, uint32_t nq, int *fd_groups, int *gate_split);
 /* Test switch: polls a persistent tree's barrier wait makes before it gives
  * up (default 2^24; 0 gives up at the first wait that finds the level
  * unfinished). */
@@ -357,6 +365,19 @@ via_status via_client_key_words(const via_params *pp, size_t *cascade, size_t *c
 via_status via_client_setup(via_ctx *ctx, const via_params *pp, via_dist key1, via_dist key2, via_dist err,
                             via_prg *prg, uint64_t *sk1, uint64_t *sk2, uint64_t *cascade_key, uint64_t *conv_key,
                             uint64_t *rsk);
+
+/* Client::setup for n clients at once (no counterpart in the reference,
+ * whose Client::setup is one client per call): client i draws from prgs[i]
+ * (distinct streams) into sk1[i], sk2[i], cascade_key[i], conv_key[i] and
+ * rsk[i], each word for word what via_client_setup gives for that stream.
+ * Up to eight host threads take whole clients in turn, each on a lane of the
+ * context (its own stream), so one client's stream parse and body assembly
+ * overlap other clients' device work. status[i] receives client i's status;
+ * the return value is the first failure (VIA_OK when all succeed). */
+via_status via_client_setup_many(via_ctx *ctx, const via_params *pp, uint32_t n, via_dist key1, via_dist key2,
+                                 via_dist err, via_prg *const *prgs, uint64_t *const *sk1, uint64_t *const *sk2,
+                                 uint64_t *const *cascade_key, uint64_t *const *conv_key, uint64_t *const *rsk,
+                                 via_status *status);
 
 /* Client::query (client.rs:183-207 -> build_compressed_query, query.rs:
  * 75-130): the query's raw LWEs ([lwe][limb][n1 + 1], bits in DMux, CMux,

@@ -169,3 +169,39 @@ def test_client_errors(ctx):
         via_gpu.client_setup(ctx, pp, g, err=via_gpu.Dist(2, 0))
     assert e.value.status == via_gpu.VIA_E_INVALID
     assert g.position == 0  # failed calls leave the stream where it was
+
+
+def test_setup_many_matches_one_at_a_time():
+    # via_client_setup_many over six clients (host threads on lanes of one
+    # context): every word of every client's keys equals via_client_setup on
+    # the same stream, the streams end at the same positions, and the
+    # queries that continue them are the same
+    pp = via_gpu.paper_params(8, 8)
+    ctx = via_gpu.Context(0)
+    try:
+        seeds = [b"via-gpu/setup-many/%d" % i for i in range(6)]
+        one = []
+        pos = []
+        for sd in seeds:
+            g = via_gpu.Prg(sd)
+            one.append(via_gpu.client_setup(ctx, pp, g))
+            pos.append(g.position)
+            one[-1]["q"] = via_gpu.client_query(ctx, pp, one[-1]["sk1"], 17, g)
+            g.close()
+        prgs = [via_gpu.Prg(sd) for sd in seeds]
+        many = via_gpu.client_setup_many(ctx, pp, prgs)
+        for i, (a, b, g) in enumerate(zip(one, many, prgs)):
+            for k in ("sk1", "sk2", "cascade_key", "conv_key", "rsk"):
+                assert np.array_equal(a[k], b[k]), (i, k)
+            assert g.position == pos[i]
+            assert np.array_equal(via_gpu.client_query(ctx, pp, b["sk1"], 17, g), a["q"]), i
+        for g in prgs:
+            g.close()
+        # the same stream twice is refused (one sequential stream per client)
+        g = via_gpu.Prg(b"via-gpu/setup-many/dup")
+        with pytest.raises(via_gpu.ViaError) as e:
+            via_gpu.client_setup_many(ctx, pp, [g, g])
+        assert e.value.status == via_gpu.VIA_E_INVALID
+        g.close()
+    finally:
+        ctx.close()

@@ -14,7 +14,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
+#include <thread>
 #include <cstring>
 #include <new>
 #include <unordered_map>
@@ -465,17 +467,12 @@ via_status via_client_key_words(const via_params *pp, size_t *cascade, size_t *c
     return VIA_OK;
 }
 
-via_status via_client_setup(via_ctx *c, const via_params *pp, via_dist key1, via_dist key2, via_dist err,
-                            via_prg *prg, uint64_t *sk1, uint64_t *sk2, uint64_t *cascade_key, uint64_t *conv_key,
-                            uint64_t *rsk) {
-    if (!c || !pp || !prg || !sk1 || !sk2 || !cascade_key || !conv_key || !rsk) return VIA_E_INVALID;
-    // client.rs:125-131: the tree dimensions must be powers of two
-    if (!pp->num_rows || !pp->num_cols || (pp->num_rows & (pp->num_rows - 1)) ||
-        (pp->num_cols & (pp->num_cols - 1)))
-        return VIA_E_DIM;
-    via_status s = check_client(pp);
-    if (s) return s;
-    if (!dist_ok(key1) || !dist_ok(key2) || !dist_ok(err)) return VIA_E_INVALID;
+// Client::setup of one client on context c (its stream), arguments checked
+// by the caller.
+static via_status client_setup_one(via_ctx *c, const via_params *pp, via_dist key1, via_dist key2, via_dist err,
+                                   via_prg *prg, uint64_t *sk1, uint64_t *sk2, uint64_t *cascade_key,
+                                   uint64_t *conv_key, uint64_t *rsk) {
+    via_status s = VIA_OK;
     if (hipSetDevice(c->device) != hipSuccess) return VIA_E_DEVICE;
     const size_t n1 = pp->n1, n2 = pp->n2, nl = 2, L = pp->l_ck;
     const uint64_t q3 = pp->q3;
@@ -573,6 +570,65 @@ via_status via_client_setup(via_ctx *c, const via_params *pp, via_dist key1, via
     for (const Rlev &rl : todo) rlev_bodies(rl);
     rd.commit();
     return VIA_OK;
+}
+
+static via_status check_setup(const via_params *pp, via_dist key1, via_dist key2, via_dist err) {
+    if (!pp) return VIA_E_INVALID;
+    // client.rs:125-131: the tree dimensions must be powers of two
+    if (!pp->num_rows || !pp->num_cols || (pp->num_rows & (pp->num_rows - 1)) ||
+        (pp->num_cols & (pp->num_cols - 1)))
+        return VIA_E_DIM;
+    via_status s = check_client(pp);
+    if (s) return s;
+    if (!dist_ok(key1) || !dist_ok(key2) || !dist_ok(err)) return VIA_E_INVALID;
+    return VIA_OK;
+}
+
+via_status via_client_setup(via_ctx *c, const via_params *pp, via_dist key1, via_dist key2, via_dist err,
+                            via_prg *prg, uint64_t *sk1, uint64_t *sk2, uint64_t *cascade_key, uint64_t *conv_key,
+                            uint64_t *rsk) {
+    if (!c || !pp || !prg || !sk1 || !sk2 || !cascade_key || !conv_key || !rsk) return VIA_E_INVALID;
+    via_status s = check_setup(pp, key1, key2, err);
+    if (s) return s;
+    return client_setup_one(c, pp, key1, key2, err, prg, sk1, sk2, cascade_key, conv_key, rsk);
+}
+
+// Client::setup of n clients at once: the clients are independent (each its
+// own PRG stream), so up to eight host threads each take whole clients in
+// turn on a lane of the context (a stream of its own): one client's stream
+// parse and body assembly on the host overlap other clients' PRG windows
+// and ring products on the GPU.
+via_status via_client_setup_many(via_ctx *c, const via_params *pp, uint32_t n, via_dist key1, via_dist key2,
+                                 via_dist err, via_prg *const *prgs, uint64_t *const *sk1, uint64_t *const *sk2,
+                                 uint64_t *const *cascade_key, uint64_t *const *conv_key, uint64_t *const *rsk,
+                                 via_status *status) {
+    if (!c || !pp || !prgs || !sk1 || !sk2 || !cascade_key || !conv_key || !rsk || !status) return VIA_E_INVALID;
+    for (uint32_t i = 0; i < n; i++)
+        if (!prgs[i] || !sk1[i] || !sk2[i] || !cascade_key[i] || !conv_key[i] || !rsk[i]) return VIA_E_INVALID;
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t k = 0; k < i; k++)
+            if (prgs[i] == prgs[k]) return VIA_E_INVALID;  // one sequential stream per client
+    via_status s = check_setup(pp, key1, key2, err);
+    if (s) return s;
+    const uint32_t nth = std::min<uint32_t>(n, 8);
+    for (uint32_t i = 0; i < n; i++) status[i] = VIA_E_DEVICE;  // until a worker sets up client i
+    std::atomic<uint32_t> next{0};
+    auto worker = [&]() {
+        via_ctx *lane = nullptr;
+        if (lane_acquire(c, &lane) != VIA_OK) return;  // the other workers take its clients
+        for (uint32_t i; (i = next.fetch_add(1)) < n;)
+            status[i] = client_setup_one(lane, pp, key1, key2, err, prgs[i], sk1[i], sk2[i], cascade_key[i],
+                                         conv_key[i], rsk[i]);
+        lane_release(lane);
+    };
+    std::vector<std::thread> th;
+    for (uint32_t t = 1; t < nth; t++) th.emplace_back(worker);
+    if (nth) worker();
+    for (auto &x : th) x.join();
+    s = VIA_OK;
+    for (uint32_t i = 0; i < n; i++)
+        if (status[i] && !s) s = status[i];
+    return s;
 }
 
 via_status via_client_query(via_ctx *c, const via_params *pp, const uint64_t *sk1, size_t index, via_dist err,

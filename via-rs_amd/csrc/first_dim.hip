@@ -379,7 +379,7 @@ static void fd_mfma_kc(const uint8_t *dbl, const uint8_t *sell, double *out_k, c
                        hipStream_t st) {
     // column-block groups per slot: four with one selection block (its
     // staging is small), else one (the staging is shared by every block)
-    const int G = p.NB == 1 && p.MB >= 16 ? 4 : 1;
+    const int G = p.G > 0 ? std::min(p.G, p.MB) : (p.NB == 1 && p.MB >= 16 ? 4 : 1);
     const dim3 grid(p.n * G, p.nseg);
     switch (p.NB) {
         case 1: hipLaunchKernelGGL((k_fd_mfma<KC, 1, 1>), grid, dim3(256), 0, st, dbl, sell, out_k, p, m, G); break;
@@ -449,12 +449,29 @@ __global__ __launch_bounds__(TPB) void k_fd_intt(const double *__restrict__ ok, 
     if (idx0 >= P) return;  // the whole workgroup, before any barrier
     const int np = (int)(P - idx0 < (size_t)IPP ? P - idx0 : (size_t)IPP);
     const FMod m = r2.t[0].m;
-    for (int e = threadIdx.x; e < IPP * n; e += TPB) {
-        const int pp = e % IPP, k = e / IPP;
-        double v = 0.0;
-        if (pp < np)
-            for (int sg = 0; sg < p.nseg; sg++) v += ok[((size_t)sg * n + k) * P + idx0 + pp];
-        s[lds_pad(pp * n + k)] = mod_reduce_small(v, m);
+    if (p.nseg == 1 && np == IPP && n == kSlots) {
+        // one segment, a whole tile: every load of the thread issued before
+        // the first store (independent loads in flight)
+        constexpr int PT = IPP * kSlots / TPB;
+        double v[PT];
+#pragma unroll
+        for (int x = 0; x < PT; x++) {
+            const int e = threadIdx.x + x * TPB;
+            v[x] = ok[(size_t)(e / IPP) * P + idx0 + e % IPP];
+        }
+#pragma unroll
+        for (int x = 0; x < PT; x++) {
+            const int e = threadIdx.x + x * TPB;
+            s[lds_pad((e % IPP) * n + e / IPP)] = mod_reduce_small(v[x], m);
+        }
+    } else {
+        for (int e = threadIdx.x; e < IPP * n; e += TPB) {
+            const int pp = e % IPP, k = e / IPP;
+            double v = 0.0;
+            if (pp < np)
+                for (int sg = 0; sg < p.nseg; sg++) v += ok[((size_t)sg * n + k) * P + idx0 + pp];
+            s[lds_pad(pp * n + k)] = mod_reduce_small(v, m);
+        }
     }
     __syncthreads();
     lds_ntt_inv_noscale<1, 4>(s, IPP * n, r2);

@@ -105,6 +105,8 @@ struct via_ctx {
     unsigned *tree_bar = nullptr;
     bool allow_persistent = true;
     int tree_override = -1;
+    int fd_groups_override = 0;  // autotuner measurement: FirstDim column-block groups (0: none)
+    int split_override = -2;     // autotuner measurement: gate limb split (-2: none)
     long tree_spin = 1L << 24;  // polls before a barrier wait gives up
     Workspace ws;
     RepackWs rw;
@@ -167,14 +169,18 @@ void tuned_erase(int device, int Q);
 via_status ctx_ring(via_ctx *c, uint32_t n, const uint64_t *q, int nl, RingDev *r);
 void tuned_store(int device, int Q, const std::vector<StepTune> &steps);
 bool tuned_lookup(int device, int Q, std::vector<StepTune> *steps);
-// Tree launch mode chosen by via_ctx_autotune per device and batch size:
-// persistent (one launch per tree) or level by level, for the DMux and the
-// CMux tree; the same lookup rule as the decompression shapes.
-struct TreeTune {
+// Launch choices via_ctx_autotune makes per device and batch size beyond
+// the decompression shapes (the same lookup rule): persistent DMux and CMux
+// trees (one launch per tree) or level by level; FirstDim's column-block
+// groups per slot (0: the heuristic); the gates' limb split (-1: split the
+// two-limb levels narrower than the CU count, 0: never, 1: always).
+struct PassTune {
     int dmux = 0, cmux = 0;
+    int fd_groups = 0;
+    int gate_split = -1;
 };
-void tree_tuned_store(int device, int Q, const TreeTune &t);
-bool tree_tuned_lookup(int device, int Q, TreeTune *t);
+void pass_tuned_store(int device, int Q, const PassTune &t);
+bool pass_tuned_lookup(int device, int Q, PassTune *t);
 // Incremented by every tuned_store: captured graphs bake in launch shapes.
 uint64_t tuned_generation();
 // Process-wide unique ids for databases and key sets.
@@ -211,6 +217,11 @@ constexpr uint32_t kServeMaxT = 64;
 // True when every word of n_lwe LWEs ([lwe][limb][n1 + 1]) is a canonical
 // residue of its q1 limb.
 bool lwes_canonical(const via_params *pp, const uint64_t *lwes, size_t n_lwe);
+// Holds the context, or a lane of it when the context is busy (created on
+// demand, up to seven), for a host-buffer call on the calling thread;
+// lane_release (same thread) lets it go.
+via_status lane_acquire(via_ctx *root, via_ctx **lane);
+void lane_release(via_ctx *lane);
 // Builds the twiddle tables of every ring the answer path uses on this context.
 via_status prepare_tables(via_ctx *c, const via_params *pp);
 }  // namespace via

@@ -160,6 +160,7 @@ struct GateArgs {
     int queries;
     int shift;           // CRot rotation N - 2^i
     DigitPlan dp;        // digit plan of the gadget (set by launch_gate)
+    int split = -1;      // two limbs, one workgroup per limb: -1 below 256 gates, 0 never, 1 always
 };
 void launch_gate(const RingDev &r, const Gadget &g, const GateArgs &a, hipStream_t s);
 
@@ -217,6 +218,7 @@ struct FdPlan {
     int MB, KCN;     // column blocks of 16, row chunks of KC
     int QP, NB;      // selection columns 2Q, blocks of 16
     int nseg;        // row segments
+    int G = 0;       // column-block groups per slot (0: the launcher's heuristic)
 };
 FdPlan fd_plan(int I, int J, int n, int Q);
 size_t fd_db_bytes(const FdPlan &p);   // prepared database

@@ -832,7 +832,8 @@ void launch_gate(const RingDev &r, const Gadget &g_in, const GateArgs &a_in, hip
         // two limbs: one workgroup per limb of each gate on levels narrower
         // than the CU count (latency-bound); wider levels keep both limbs in
         // one workgroup (the digits are extracted once)
-        if (r.nl == 2 && blocks < 256)
+        const bool split = a.split < 0 ? blocks < 256 : a.split == 1;
+        if (r.nl == 2 && split)
             hipLaunchKernelGGL((k_gate_fused<2, 1>), dim3(2 * blocks), dim3(1024), 0, st, r, g, a);
         else if (r.nl == 2)
             hipLaunchKernelGGL((k_gate_fused<2, 0>), dim3(blocks), dim3(1024), 0, st, r, g, a);

@@ -476,6 +476,17 @@ via_status via::prepare_tables(via_ctx *c, const via_params *pp) {
     return get_tab(c, pp->q3, pp->n2, &t);
 }
 
+via_status via::lane_acquire(via_ctx *root, via_ctx **lane) {
+    CallCtx cc;
+    via_status s = hold_lane(root, &cc);
+    if (s) return s;
+    *lane = cc.c;
+    cc.c = nullptr;  // the hold passes to the caller
+    return VIA_OK;
+}
+
+void via::lane_release(via_ctx *lane) { lane->call_mu.unlock(); }
+
 bool via::lwes_canonical(const via_params *pp, const uint64_t *lwes, size_t n_lwe) {
     const size_t n1p = (size_t)pp->n1 + 1;
     for (size_t i = 0; i < n_lwe * 2; i++) {
@@ -1047,19 +1058,19 @@ void via::tuned_erase(int device, int Q) {
 }
 
 namespace {
-std::map<std::pair<int, int>, TreeTune> g_tree_tuned;  // (device, batch size), under g_tuned_mu
+std::map<std::pair<int, int>, PassTune> g_pass_tuned;  // (device, batch size), under g_tuned_mu
 }  // namespace
 
-void via::tree_tuned_store(int device, int Q, const TreeTune &t) {
+void via::pass_tuned_store(int device, int Q, const PassTune &t) {
     std::lock_guard<std::mutex> g(g_tuned_mu);
-    g_tree_tuned[{device, Q}] = t;
+    g_pass_tuned[{device, Q}] = t;
     g_tuned_gen++;
 }
 
-bool via::tree_tuned_lookup(int device, int Q, TreeTune *t) {
+bool via::pass_tuned_lookup(int device, int Q, PassTune *t) {
     std::lock_guard<std::mutex> g(g_tuned_mu);
-    auto it = g_tree_tuned.upper_bound({device, Q});
-    if (it == g_tree_tuned.begin()) return false;
+    auto it = g_pass_tuned.upper_bound({device, Q});
+    if (it == g_pass_tuned.begin()) return false;
     --it;
     if (it->first.first != device) return false;
     *t = it->second;
@@ -1288,18 +1299,21 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     // trees in one persistent launch each: forced by the context's mode, else
     // the autotuner's choice for this batch size (never while capturing a
     // graph, nor on the serving loop's slots, whose streams run side by side)
+    PassTune pt;
+    const bool have_pt = pass_tuned_lookup(c->device, Q, &pt);
     bool pers_d = false, pers_c = false;
     if (!c->capturing && c->allow_persistent) {
         if (c->tree_override >= 0) {
             pers_d = pers_c = c->tree_override == 1;
-        } else {
-            TreeTune tt;
-            if (tree_tuned_lookup(c->device, Q, &tt)) {
-                pers_d = tt.dmux != 0;
-                pers_c = tt.cmux != 0;
-            }
+        } else if (have_pt) {
+            pers_d = pt.dmux != 0;
+            pers_c = pt.cmux != 0;
         }
     }
+    // the gates' limb split and FirstDim's column-block groups: the
+    // autotuner's measurement, its stored choice, or the launchers' heuristics
+    const int gate_split = c->split_override > -2 ? c->split_override : (have_pt ? pt.gate_split : -1);
+    const int fd_groups = c->fd_groups_override > 0 ? c->fd_groups_override : (have_pt ? pt.fd_groups : 0);
     int dmux_from = 1;
     if (pers_d && nd > 2 && nd - 1 <= kMaxTreeLevels) {
         TreeArgs ta;
@@ -1323,6 +1337,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     }
     for (int i = dmux_from; i < nd; i++) {
         GateArgs ga;
+        ga.split = gate_split;
         ga.mode = GATE_DMUX;
         ga.rgsw = w.rgsw_eval + (size_t)i * rgw;
         ga.rgsw_qstride = bits * rgw;
@@ -1348,7 +1363,8 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
 
     // ---- step 4: first_dim (first_dim.rs:94-184), one database pass for all queries ----
     launch_ntt_to(r2, w.sw, w.sel, (size_t)Q * I * 2, 1, st);
-    const FdPlan fp = fd_plan((int)I, (int)J, (int)n, Q);
+    FdPlan fp = fd_plan((int)I, (int)J, (int)n, Q);
+    fp.G = fd_groups;
     const FMod m2 = fmod_make(pp->q2);
     launch_sel_pack(w.sel, w.sell, fp, st);
     REC_EVENT(c->ev_fd[0], st);
@@ -1401,6 +1417,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     }
     for (int b = cmux_from; b < nc; b++) {
         GateArgs ga;
+        ga.split = gate_split;
         ga.mode = GATE_CMUX;
         ga.rgsw = w.rgsw2 + (size_t)b * rgw2;
         ga.rgsw_qstride = rq2;
@@ -1426,6 +1443,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     int rc = 0;
     for (int b = 0; b < nr; b++) {
         GateArgs ga;
+        ga.split = gate_split;
         ga.mode = GATE_CROT;
         ga.rgsw = w.rgsw2 + (size_t)(nc + b) * rgw2;
         ga.rgsw_qstride = rq2;
@@ -1881,6 +1899,8 @@ via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via
             c->time_decomp = false;
             c->tune_override.clear();
             c->tree_override = -1;
+            c->fd_groups_override = 0;
+            c->split_override = -2;
         }
     } guard{c};
     const int Q = (int)nq;
@@ -1920,7 +1940,7 @@ via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via
     tuned_store(c->device, Q, best);
     // the tree launch modes with the stored shapes: level by level against
     // one persistent launch per tree, by the DMux and CMux step events
-    TreeTune tt;
+    PassTune tt;
     if (c->allow_persistent) {
         c->time_decomp = false;
         c->tune_override.clear();
@@ -1944,7 +1964,55 @@ via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via
             tt.cmux = tc[1] < tc[0];
         }
     }
-    tree_tuned_store(c->device, Q, tt);
+    pass_tuned_store(c->device, Q, tt);
+    // FirstDim's column-block groups per slot, by the k_fd_mfma events
+    {
+        const FdPlan fp = fd_plan((int)db->pp.num_rows, (int)db->pp.num_cols, (int)db->pp.n1, Q);
+        double best_t = 0;
+        for (int G : {1, 2, 4, 8}) {
+            if (G > fp.MB) continue;
+            c->fd_groups_override = G;
+            double t = 0;
+            for (int r = 0; r <= reps; r++) {
+                if ((s = answer_batch_impl(c, db, Q, keys, d_lwes, n_lwe, d_mask, d_body, nullptr))) return s;
+                HIP_TRY(hipStreamSynchronize(c->stream));
+                if (r == 0) continue;
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, c->ev_fd[0], c->ev_fd[1]));
+                t += ms;
+            }
+            if (!tt.fd_groups || t < best_t) {
+                tt.fd_groups = G;
+                best_t = t;
+            }
+        }
+        c->fd_groups_override = 0;
+    }
+    // the gates' limb split, by the DMux, CMux and CRot step events
+    {
+        double best_t = 0;
+        int best = -1;
+        for (int sp : {-1, 0, 1}) {
+            c->split_override = sp;
+            double t = 0;
+            for (int r = 0; r <= reps; r++) {
+                if ((s = answer_batch_impl(c, db, Q, keys, d_lwes, n_lwe, d_mask, d_body, nullptr))) return s;
+                HIP_TRY(hipStreamSynchronize(c->stream));
+                if (r == 0) continue;
+                float a = 0, b = 0;
+                HIP_TRY(hipEventElapsedTime(&a, c->ev[1], c->ev[2]));
+                HIP_TRY(hipEventElapsedTime(&b, c->ev[4], c->ev[6]));
+                t += a + b;
+            }
+            if (sp == -1 || t < best_t) {
+                best = sp;
+                best_t = t;
+            }
+        }
+        c->split_override = -2;
+        tt.gate_split = best;
+    }
+    pass_tuned_store(c->device, Q, tt);
     return VIA_OK;
 }
 
@@ -1958,10 +2026,19 @@ via_status via_ctx_set_tree_mode(via_ctx *c, int mode) {
 
 via_status via_ctx_tree_tuning(via_ctx *c, uint32_t nq, int *dmux, int *cmux) {
     if (!c || !dmux || !cmux) return VIA_E_INVALID;
-    TreeTune tt;
-    if (!tree_tuned_lookup(c->device, (int)nq, &tt)) return VIA_E_INVALID;
+    PassTune tt;
+    if (!pass_tuned_lookup(c->device, (int)nq, &tt)) return VIA_E_INVALID;
     *dmux = tt.dmux;
     *cmux = tt.cmux;
+    return VIA_OK;
+}
+
+via_status via_ctx_pass_tuning(via_ctx *c, uint32_t nq, int *fd_groups, int *gate_split) {
+    if (!c || !fd_groups || !gate_split) return VIA_E_INVALID;
+    PassTune tt;
+    if (!pass_tuned_lookup(c->device, (int)nq, &tt)) return VIA_E_INVALID;
+    *fd_groups = tt.fd_groups;
+    *gate_split = tt.gate_split;
     return VIA_OK;
 }
 

@@ -126,6 +126,7 @@ def lib():
             "via_ctx_set_tree_mode": (C.c_int, [vp, C.c_int]),
             "via_ctx_tree_tuning": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "via_debug_set_tree_spin": (C.c_int, [vp, C.c_long]),
+            "via_ctx_pass_tuning": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "via_answer_device_noalloc": (C.c_int, [vp, vp, vp, vp, C.c_size_t, vp, vp, C.POINTER(ErrDetail)]),
             "via_ctx_set_tuning": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "via_ctx_autotune": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.c_size_t,
@@ -146,6 +147,9 @@ def lib():
             "via_client_key_words": (C.c_int, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
                                                C.POINTER(C.c_size_t)]),
             "via_client_setup": (C.c_int, [vp, P, Dist, Dist, Dist, vp, u64p, u64p, u64p, u64p, u64p]),
+            "via_client_setup_many": (C.c_int, [vp, P, C.c_uint32, Dist, Dist, Dist, C.POINTER(vp), C.POINTER(vp),
+                                                C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
+                                                C.POINTER(C.c_int)]),
             "via_client_query": (C.c_int, [vp, P, u64p, C.c_size_t, Dist, vp, u64p]),
             "via_client_deinterleave": (C.c_int, [P, C.c_uint32, u64p, u64p]),
             "via_wire_uncompressed_size": (C.c_size_t, [C.c_size_t]),
@@ -294,6 +298,23 @@ def client_setup(ctx, pp, prg, key1=TERNARY, key2=TERNARY, err=TERNARY):
     return out
 
 
+def client_setup_many(ctx, pp, prgs, key1=TERNARY, key2=TERNARY, err=TERNARY):
+    """Client::setup for several clients at once (via_client_setup_many):
+    one dict per PRG, each as client_setup gives for that stream."""
+    wc, wv, wr = client_key_words(pp)
+    n = len(prgs)
+    outs = [{"sk1": np.zeros(2 * pp.n1, np.uint64), "sk2": np.zeros(pp.n2, np.uint64),
+             "cascade_key": np.zeros(wc, np.uint64), "conv_key": np.zeros(wv, np.uint64),
+             "rsk": np.zeros(wr, np.uint64)} for _ in range(n)]
+    vp = C.c_void_p
+    arr = lambda k: (vp * n)(*[o[k].ctypes.data for o in outs])
+    G = (vp * n)(*[g.h.value if isinstance(g.h, C.c_void_p) else g.h for g in prgs])
+    st = (C.c_int * n)()
+    _check(lib().via_client_setup_many(ctx.h, C.byref(pp), n, key1, key2, err, G, arr("sk1"), arr("sk2"),
+                                       arr("cascade_key"), arr("conv_key"), arr("rsk"), st))
+    return outs
+
+
 def client_query(ctx, pp, sk1, index, prg, err=TERNARY):
     """Client::query (client.rs:183-207): the query's raw LWEs."""
     out = np.zeros(pp.query_lwes * pp.lwe_words, np.uint64)
@@ -417,7 +438,9 @@ class Context:
         d, c = C.c_int(), C.c_int()
         if lib().via_ctx_tree_tuning(self.h, nq, C.byref(d), C.byref(c)) != VIA_OK:
             return None
-        return {"dmux": d.value, "cmux": c.value}
+        g, sp = C.c_int(), C.c_int()
+        _check(lib().via_ctx_pass_tuning(self.h, nq, C.byref(g), C.byref(sp)))
+        return {"dmux": d.value, "cmux": c.value, "fd_groups": g.value, "gate_split": sp.value}
 
     def set_graphs(self, on):
         """hipGraph mode: repeated identical answers replay a captured graph."""
