@@ -27,11 +27,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 # HBM traffic of one FirstDim launch (k_fd_mfma) per (I, J, queries per
-# pass), measured with rocprofv3 --pmc: FETCH_SIZE and WRITE_SIZE per
-# dispatch, scaled by what the same counters read for a 256 MiB device copy
-# (fetched bytes = 2.000 x FETCH_SIZE KiB, written bytes = 1.000 x WRITE_SIZE
-# KiB on this part).
-FD_TRAFFIC = {(256, 256, 1): (2.0 * 337981.8 + 8192.0) * 1024}
+# pass, column-block groups per slot), measured with rocprofv3 --pmc:
+# FETCH_SIZE and WRITE_SIZE per dispatch, scaled by what the same counters
+# read for a 256 MiB device copy (fetched bytes = 2.000 x FETCH_SIZE KiB,
+# written bytes = 1.000 x WRITE_SIZE KiB on this part).
+FD_TRAFFIC = {(256, 256, 1, 1): (2.0 * 330305.4 + 8192.0) * 1024,
+              (256, 256, 1, 2): (2.0 * 332864.0 + 8192.0) * 1024,
+              (256, 256, 1, 4): (2.0 * 337982.2 + 8192.0) * 1024,
+              (256, 256, 1, 8): (2.0 * 348222.0 + 8192.0) * 1024}
 
 # Modular multiplications (NTT butterflies, pointwise and scaling products
 # counted alike) of one answer's steps at the paper stack, counted from the
@@ -843,7 +846,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(fd_gbs / HBM_PEAK_GBS, 4),
-            "traffic": FD_TRAFFIC.get((I, J, Q)),
+            "traffic": FD_TRAFFIC.get((I, J, Q, (ctx.tree_tuning(Q) or {}).get("fd_groups", 0))),
+            "fd_groups": (ctx.tree_tuning(Q) or {}).get("fd_groups"),
             "algorithmic_bytes": fd_bytes,
             # the same launch priced at the reference's u64 database storage
             # (8 bytes per residue), which this layout does not move

@@ -76,9 +76,12 @@ void launch_lwe_digits(const RingDev &r1, const Gadget &g, const DigitPlan &dp, 
 // that are embedded (x -> x^2) transform n/2 coefficients. LSPLIT (NL = 2):
 // workgroup row blockIdx.y handles limb blockIdx.y alone (the two limbs'
 // transforms and products are independent), holding half the accumulators
-// and LDS of the two-limb form for the same total work.
+// and LDS of the two-limb form for the same total work; it also issues each
+// digit's key-row loads before that digit's transform (the transform's
+// barriers wait on LDS only), so their latency hides behind it.
 template <int NL, int MAXR, int TPB, int LSPLIT = 0>
 __global__ __launch_bounds__(TPB) void k_gadget_mac(RingDev r, Gadget g, MacArgs a) {
+    constexpr bool PF = LSPLIT != 0;
     constexpr int PER = kSlots / TPB;  // slots per thread
     constexpr int NC = LSPLIT ? 1 : NL;  // limbs this workgroup computes
     __shared__ double s[lds_words(NC * kSlots)];
@@ -104,6 +107,20 @@ __global__ __launch_bounds__(TPB) void k_gadget_mac(RingDev r, Gadget g, MacArgs
     const int16_t *dig = a.dig + (size_t)item * g.L * a.dig_coeffs + (size_t)tile * cin;
     const double *key = a.key[item / a.items_per_key] + (size_t)tile * S;
     for (int k = k0; k < k1; k++) {
+        const int ks = g.L - 1 - k;
+        double kmr[PF ? NC : 1][PF ? PER : 1], kbr[PF ? NC : 1][PF ? PER : 1];
+        if constexpr (PF) {
+#pragma unroll
+            for (int l = 0; l < NC; l++) {
+                const double *km = key + ((size_t)(ks * 2 + 0) * NL + lb + l) * a.key_slots;
+                const double *kb = key + ((size_t)(ks * 2 + 1) * NL + lb + l) * a.key_slots;
+#pragma unroll
+                for (int e = 0; e < PER; e++) {
+                    kmr[l][e] = km[threadIdx.x + e * TPB];
+                    kbr[l][e] = kb[threadIdx.x + e * TPB];
+                }
+            }
+        }
         const int16_t *dk = dig + (size_t)k * a.dig_coeffs;
         for (int c = threadIdx.x; c < cin; c += blockDim.x) {
             // signed digits are valid NTT inputs (the lazy transform takes (-2q, 2q))
@@ -113,7 +130,6 @@ __global__ __launch_bounds__(TPB) void k_gadget_mac(RingDev r, Gadget g, MacArgs
         }
         __syncthreads();
         ntt_fwd<NC, MAXR>(s, cin, nt, lognt, T);
-        const int ks = g.L - 1 - k;
 #pragma unroll
         for (int l = 0; l < NC; l++) {
             const FMod md = l ? m1 : m0;
@@ -123,8 +139,13 @@ __global__ __launch_bounds__(TPB) void k_gadget_mac(RingDev r, Gadget g, MacArgs
             for (int e = 0; e < PER; e++) {
                 const int sg = threadIdx.x + e * TPB;
                 const double x = s[lds_pad(l * cin + (a.embed ? (sg >> 1) : sg))];
-                am[l][e] += mod_mul_lazy(x, km[sg], md);
-                ab[l][e] += mod_mul_lazy(x, kb[sg], md);
+                if constexpr (PF) {
+                    am[l][e] += mod_mul_lazy(x, kmr[l][e], md);
+                    ab[l][e] += mod_mul_lazy(x, kbr[l][e], md);
+                } else {
+                    am[l][e] += mod_mul_lazy(x, km[sg], md);
+                    ab[l][e] += mod_mul_lazy(x, kb[sg], md);
+                }
             }
         }
         __syncthreads();

@@ -1973,7 +1973,9 @@ via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via
             if (G > fp.MB) continue;
             c->fd_groups_override = G;
             double t = 0;
-            for (int r = 0; r <= reps; r++) {
+            // one launch per answer, a few percent apart: more runs than the
+            // decompression shapes get, so noise does not decide
+            for (int r = 0; r <= 4 * reps; r++) {
                 if ((s = answer_batch_impl(c, db, Q, keys, d_lwes, n_lwe, d_mask, d_body, nullptr))) return s;
                 HIP_TRY(hipStreamSynchronize(c->stream));
                 if (r == 0) continue;
