@@ -57,11 +57,25 @@ constexpr int kNttMaxR = 3;  // default stages per register pass (radix 8)
 
 // Passes P, P+1, ... of a degree-2^LOGN transform over every segment of the
 // NL x S buffer, with the degree and schedule fixed at compile time.
+// With two limbs and a thread count that splits into wave-aligned halves
+// (nth a multiple of 128), the first half of the threads takes the units of
+// limb 0 and the second half those of limb 1: the limb, its twiddle table
+// and its modulus are then the same across a wavefront and live in scalar
+// registers instead of being selected per lane for every unit. Otherwise
+// the units of both limbs are spread over all threads.
 template <int NL, int LOGN, int MAXR, int P>
 __device__ __forceinline__ void fwd_passes(double *s, int S, const NttLimbs &T, int tid, int nth) {
     if constexpr (P < sched_np(LOGN, MAXR)) {
         constexpr int R = sched_r(LOGN, MAXR, P), L0 = sched_l0(LOGN, MAXR, P), USEG = LOGN - R;
         const int upl = S >> R;  // units per limb
+        if (NL == 2 && (nth & 127) == 0) {
+            const int half = nth >> 1;
+            const int l = __builtin_amdgcn_readfirstlane(tid >= half ? 1 : 0);
+            const double *tw = l ? T.tw[1] : T.tw[0];
+            const FMod m = l ? T.m[1] : T.m[0];
+            for (int ua = tid - l * half; ua < upl; ua += half)
+                ntt_fwd_unit_p<R, LOGN, L0>(s, l * S + ((ua >> USEG) << LOGN), ua & ((1 << USEG) - 1), tw, m);
+        } else
         for (int t = tid; t < NL * upl; t += nth) {
             const int l = (NL == 1) ? 0 : (t >= upl ? 1 : 0);
             const int ua = t - l * upl;
@@ -79,6 +93,14 @@ __device__ __forceinline__ void inv_passes(double *s, int S, const NttLimbs &T, 
     if constexpr (P < sched_np(LOGN, MAXR)) {
         constexpr int R = sched_r(LOGN, MAXR, P), LH0 = sched_l0(LOGN, MAXR, P), USEG = LOGN - R;
         const int upl = S >> R;
+        if (NL == 2 && (nth & 127) == 0) {
+            const int half = nth >> 1;
+            const int l = __builtin_amdgcn_readfirstlane(tid >= half ? 1 : 0);
+            const double *tw = l ? T.tw[1] : T.tw[0];
+            const FMod m = l ? T.m[1] : T.m[0];
+            for (int ua = tid - l * half; ua < upl; ua += half)
+                ntt_inv_unit_p<R, LOGN, LH0>(s, l * S + ((ua >> USEG) << LOGN), ua & ((1 << USEG) - 1), tw, m);
+        } else
         for (int t = tid; t < NL * upl; t += nth) {
             const int l = (NL == 1) ? 0 : (t >= upl ? 1 : 0);
             const int ua = t - l * upl;
