@@ -13,6 +13,7 @@
 namespace via {
 
 constexpr int kSlots = 2048;    // coefficients handled by one workgroup
+constexpr int kSlotsLog = 11;
 constexpr int kThreads = 256;   // workgroup size of the polynomial kernels
 constexpr int kPer = kSlots / kThreads;
 constexpr int kMaxQ = 32;       // queries answered together in one batched launch

@@ -81,6 +81,22 @@ VIA_HD double mod_reduce_small(double v, const FMod &m) {
     return r;
 }
 
+// Exact conversions of canonical residues (integers in [0, 2^52)) through the
+// 2^52 bias: the double with exponent 2^52 and mantissa x is 2^52 + x.
+VIA_HD double small_u64_to_f64(uint64_t x) {
+    const uint64_t b = x | 0x4330000000000000ULL;
+    double d;
+    __builtin_memcpy(&d, &b, sizeof d);
+    return d - 4503599627370496.0;
+}
+
+VIA_HD uint64_t small_f64_to_u64(double v) {
+    const double d = v + 4503599627370496.0;
+    uint64_t b;
+    __builtin_memcpy(&b, &d, sizeof b);
+    return b & 0x000FFFFFFFFFFFFFULL;
+}
+
 // ---- 128-bit helpers ---------------------------------------------------------
 
 VIA_HD double u128_to_f64(via_u128 x) {

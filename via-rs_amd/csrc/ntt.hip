@@ -142,19 +142,52 @@ __global__ __launch_bounds__(kThreads) void k_polymul(RingDev r, const uint64_t 
     // bidx: operand b of product p / nl is b's entry bidx[p / nl] (limb p % nl)
     const size_t pb = bidx ? (size_t)bidx[p / r.nl] * r.nl + p % r.nl : p;
     const uint64_t *ga = a + p * n, *gb = b + pb * n;
+    const FMod md = r2.t[0].m;
+    const double ni = r2.t[0].n_inv;
+    uint64_t *gc = c + p * n;
+    if (n == kSlots) {
+        // the common degree: every element phase unrolled, all loads of a
+        // phase in flight together, canonical residues converted exactly
+        // through the 2^52 bias (two instructions each way)
+        constexpr int PT = kSlots / kThreads;
+        uint64_t va[PT], vb[PT];
+#pragma unroll
+        for (int x = 0; x < PT; x++) {
+            va[x] = ga[threadIdx.x + x * kThreads];
+            vb[x] = gb[threadIdx.x + x * kThreads];
+        }
+#pragma unroll
+        for (int x = 0; x < PT; x++) {
+            const int i = threadIdx.x + x * kThreads;
+            s[lds_pad(i)] = small_u64_to_f64(va[x]);
+            s[lds_pad(kSlots + i)] = small_u64_to_f64(vb[x]);
+        }
+        __syncthreads();
+        ntt_fwd<2, 4>(s, kSlots, kSlots, kSlotsLog, limbs_fwd(r2));
+#pragma unroll
+        for (int x = 0; x < PT; x++) {
+            const int i = threadIdx.x + x * kThreads;
+            s[lds_pad(i)] = mod_mul(mod_reduce_small(s[lds_pad(i)], md), s[lds_pad(kSlots + i)], md);
+        }
+        __syncthreads();
+        ntt_inv_noscale<1, 4>(s, kSlots, kSlots, kSlotsLog, limbs_inv(r2));
+#pragma unroll
+        for (int x = 0; x < PT; x++) {
+            const int i = threadIdx.x + x * kThreads;
+            gc[i] = small_f64_to_u64(mod_mul(s[lds_pad(i)], ni, md));
+        }
+        return;
+    }
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         s[lds_pad(i)] = (double)ga[i];
         s[lds_pad(n + i)] = (double)gb[i];
     }
     __syncthreads();
     lds_ntt_fwd<2, 4>(s, n, r2);
-    const FMod md = r2.t[0].m;
     for (int i = threadIdx.x; i < n; i += blockDim.x)
         s[lds_pad(i)] = mod_mul(mod_reduce_small(s[lds_pad(i)], md), s[lds_pad(n + i)], md);
     __syncthreads();
     lds_ntt_inv_noscale<1, 4>(s, n, r2);
-    const double ni = r2.t[0].n_inv;
-    uint64_t *gc = c + p * n;
     for (int i = threadIdx.x; i < n; i += blockDim.x) gc[i] = (uint64_t)mod_mul(s[lds_pad(i)], ni, md);
 }
 
