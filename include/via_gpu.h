@@ -198,8 +198,10 @@ via_status via_answer_many_device(via_ctx *ctx, const via_db *db, uint32_t nq, c
  * queries. groups_out and variant_out (12 entries each: the eleven
  * cascade steps and the RLWE -> RGSW conversion; variant 0 = 256 threads
  * radix 8, 2 = 256 threads radix 16, 3 = 512 threads radix 8, 1 = 1024
- * threads, 4 = 256 threads per limb, 5 = 512 threads per limb) receive the
- * choice when not null. With those shapes it then measures, in turn, the
+ * threads, 4 = 256 threads per limb, 5 = 512 threads per limb; for the
+ * cascade steps, which take the fixed-shape kernel when its exactness bound
+ * holds for the parameters, the variant mod 3 selects 1, 2 or 4 LWEs per
+ * workgroup instead) receive the choice when not null. With those shapes it then measures, in turn, the
  * DMux and CMux trees as one persistent launch each against level by level
  * (via_ctx_tree_tuning), FirstDim's column-block groups per slot (1, 2, 4,
  * 8) and the gates' limb split (via_ctx_pass_tuning), keeping the faster of

@@ -1,5 +1,5 @@
-// Host check of via-rs_amd/csrc/xcd_map.h: the gadget-MAC task map is a
-// bijection from workgroup ids onto (item, group, tile) tasks, and when the
+// Host check of via-rs_amd/csrc/xcd_map.h: the gadget-MAC task map (both
+// task orders) is a bijection from workgroup ids onto (item, group, tile) tasks, and when the
 // key count divides or is a multiple of eight, workgroup lin's key depends
 // only on lin mod 8 in the stated way. Built and run by
 // tests/test_host_arith.py; prints "OK <count>".
@@ -15,11 +15,12 @@ int main() {
     for (int ipk : ipks)
         for (int nkeys = 1; nkeys <= 33; nkeys++)
             for (int groups = 1; groups <= 18; groups += 3)
-                for (int tiles = 1; tiles <= 2; tiles++) {
+                for (int tiles = 1; tiles <= 2; tiles++)
+                for (int order = 0; order <= 1; order++) {
                     const int items = ipk * nkeys, total = items * groups * tiles;
                     std::vector<char> seen(total, 0);
                     for (int lin = 0; lin < total; lin++) {
-                        const MacTask t = mac_task(lin, items, ipk, groups, tiles);
+                        const MacTask t = mac_task(lin, items, ipk, groups, tiles, order);
                         if (t.item < 0 || t.item >= items || t.grp < 0 || t.grp >= groups || t.tile < 0 ||
                             t.tile >= tiles) {
                             fails++;

@@ -1,0 +1,429 @@
+// cascade_mac.hip — the gadget-product multiply-accumulate of the LWE -> RLWE
+// cascade steps (RLevEval::gadget_product, rlev.rs:259-283, inside
+// conv_step_eval, conv.rs:122-150) in a fixed shape for n1 = 2048 and two
+// q1 limbs.
+//
+// A workgroup (item, digit group, tile) adds NTT(digit_k) (.) key[L-1-k] over
+// its digits k, for the 2048 slots of its tile, both RNS limbs: threads
+// 0..127 take limb 0 and 128..255 limb 1, so the limb's modulus, twiddle
+// table and key rows are wave-uniform. The inputs are embedded (degree
+// nt = n_out / 2, 1024 per limb and tile), so thread i ends every transform
+// holding outputs 8i .. 8i+7 in registers: slots 16i .. 16i+15, whose key
+// words the key layout of this kernel stores so that every load of a
+// wavefront is contiguous (cascade_key_pos). Per digit:
+//  * the first pass reads the thread's digits straight from the int16 plane
+//    and evaluates its stages in direct form (the digits are at most B/2 in
+//    magnitude, so every product with a twiddle product is exact and the sums
+//    stay below 2^45 without a reduction);
+//  * the middle passes run over LDS (two buffers, used by alternate digits,
+//    so the last pass of one digit and the first of the next need no barrier);
+//  * the last pass (three stages, contiguous outputs) stays in registers and
+//    feeds the multiply-accumulate directly.
+// The butterflies, twiddles and transform order are those of ntt_lazy.h (the
+// reference's DIT forward transform, ring/ntt.rs:305-343); only the grouping of
+// exact integer operations differs, so every canonical result is identical.
+#include "ntt_device.h"
+
+namespace via {
+
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+constexpr int kCmIn = 1024;    // transform inputs per limb and tile
+constexpr int kCmThr = 128;    // threads per limb
+constexpr int kCmBuf = lds_words(kCmIn);
+
+// Passes of a 2^LOGNT-point transform whose last pass has RL stages
+// (outputs in registers): none for LOGNT = 0, one for LOGNT <= RL, else
+// passes of at most three stages ending with the pass of RL, the stages
+// before it split as evenly as possible, larger first.
+__host__ __device__ constexpr int cm_np(int logn, int rl) {
+    return logn == 0 ? 0 : logn <= rl ? 1 : 1 + (logn - rl + 2) / 3;
+}
+__host__ __device__ constexpr int cm_r(int logn, int rl, int p) {
+    return logn <= rl ? logn
+           : p == cm_np(logn, rl) - 1
+               ? rl
+               : (logn - rl) / (cm_np(logn, rl) - 1) + (p < (logn - rl) % (cm_np(logn, rl) - 1) ? 1 : 0);
+}
+__host__ __device__ constexpr int cm_l0(int logn, int rl, int p) {
+    return p == 0 ? 0 : cm_l0(logn, rl, p - 1) + cm_r(logn, rl, p - 1);
+}
+
+// Direct form of stages 0..R-1 of block 0 on x[0 .. 2^R) (element t at
+// offset t * stride of a unit that starts its segment's block 0), with the
+// ring's direct constants c (direct_consts on the host). Exact: every term is
+// a constant below q times an input of magnitude at most B/2.
+template <int R>
+__device__ __forceinline__ void direct_pass(double *x, const double *c) {
+    if constexpr (R == 1) {
+        const double y = c[0] * x[1];
+        const double a = x[0];
+        x[0] = a + y;
+        x[1] = a - y;
+    } else if constexpr (R == 2) {
+        const double p = fma(c[0], x[2], x[0]), m = fma(-c[0], x[2], x[0]);
+        const double t1 = fma(c[2], x[3], c[1] * x[1]);
+        const double t2 = fma(-c[4], x[3], c[3] * x[1]);
+        x[0] = p + t1;
+        x[1] = p - t1;
+        x[2] = m + t2;
+        x[3] = m - t2;
+    } else if constexpr (R == 3) {
+        const double p = fma(c[0], x[4], x[0]), m = fma(-c[0], x[4], x[0]);
+        const double t1 = fma(c[2], x[6], c[1] * x[2]);
+        const double t2 = fma(-c[4], x[6], c[3] * x[2]);
+        double b[4] = {p + t1, p - t1, m + t2, m - t2};
+        double o[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double *d = c + 5 + 4 * k;
+            o[k] = fma(d[3], x[7], fma(d[2], x[3], fma(d[1], x[5], d[0] * x[1])));
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            x[2 * k] = b[k] + o[k];
+            x[2 * k + 1] = b[k] - o[k];
+        }
+    }
+}
+
+// R lazy stages of a unit held in registers: stage s of the unit's block SB
+// takes twiddle w[(1 << s) - 1 + (t >> (R - s))] (the 2^R - 1 twiddles the
+// unit needs, prefetched by load_unit_tw); the butterflies of ntt_fwd_unit_p.
+template <int R>
+__device__ __forceinline__ void unit_stages(double *x, const double *w, const FMod &m) {
+    constexpr int E = 1 << R;
+#pragma unroll
+    for (int st = 0; st < R; st++) {
+        const int half = E >> (st + 1);
+#pragma unroll
+        for (int t = 0; t < E; t++) {
+            if (t & half) continue;
+            const double y = mod_mul_lazy(x[t + half], w[(1 << st) - 1 + (t >> (R - st))], m);
+            const double a = x[t];
+            x[t] = a + y;
+            x[t + half] = a - y;
+        }
+    }
+}
+
+// The 2^R - 1 twiddles of block SB over stages L0 .. L0+R-1.
+template <int R, int L0>
+__device__ __forceinline__ void load_unit_tw(double *w, const double *tw, int SB) {
+#pragma unroll
+    for (int st = 0; st < R; st++)
+#pragma unroll
+        for (int e = 0; e < (1 << st); e++) w[(1 << st) - 1 + e] = tw[(1 << (L0 + st)) + (SB << st) + e];
+}
+
+// Unit geometry of pass (L0, R) of a 2^LOGNT-point transform over the 1024
+// inputs of a limb: unit u's elements are base + t * 2^JB, its block SB.
+template <int LOGNT, int L0, int R>
+struct UnitGeo {
+    static constexpr int JB = LOGNT - L0 - R;
+    static constexpr int USEG = LOGNT - R;  // log2 units per segment
+    __device__ static __forceinline__ void at(int u, int &base, int &SB) {
+        const int seg = u >> USEG;
+        const int uu = u & ((1 << USEG) - 1);
+        SB = uu >> JB;
+        base = (seg << LOGNT) + (SB << (LOGNT - L0)) + (uu & ((1 << JB) - 1));
+    }
+};
+
+// Middle passes P, P+1, .. NP-2 over the LDS buffers of the IPW items (item
+// `it` at buf + it * kCmBuf): the thread's units w = j + TL v of the
+// items' 1024 >> R units each, twiddles read per unit, a barrier after each.
+template <int LOGNT, int RL, int IPW, int P>
+__device__ __forceinline__ void cm_middle(double *buf, int j, const double *tw, const FMod &m) {
+    constexpr int NP = cm_np(LOGNT, RL);
+    if constexpr (P < NP - 1) {
+        constexpr int R = cm_r(LOGNT, RL, P), L0 = cm_l0(LOGNT, RL, P), E = 1 << R, TL = 128 * IPW;
+        using G = UnitGeo<LOGNT, L0, R>;
+#pragma unroll
+        for (int v = 0; v < (8 >> R); v++) {
+            const int w = j + TL * v;
+            const int it = w >> (10 - R), u = w & ((1024 >> R) - 1);
+            int base, SB;
+            G::at(u, base, SB);
+            double wt[E - 1];
+            load_unit_tw<R, L0>(wt, tw, SB);
+            double *p = buf + it * kCmBuf + lds_pad(base);
+            double x[E];
+#pragma unroll
+            for (int t = 0; t < E; t++) x[t] = p[unit_off<(1 << G::JB)>(t)];
+            unit_stages<R>(x, wt, m);
+#pragma unroll
+            for (int t = 0; t < E; t++) p[unit_off<(1 << G::JB)>(t)] = x[t];
+        }
+        __syncthreads();
+        cm_middle<LOGNT, RL, IPW, P + 1>(buf, j, tw, m);
+    }
+}
+
+}  // namespace
+
+// Position of slot sg (0..2047) of a tile in the key layout of
+// k_cascade_mac: slot 8j + m sits at (m >> 1) * 512 + 2j + (m & 1). Thread j
+// of the two-item kernel owns slots 8j .. 8j+7: its slot pair t is the 16
+// bytes at 4096 t + 16 j. Thread i of the one-item kernel owns slots
+// 16i .. 16i+15: its slot pairs t and t + 4 are the 32 bytes at
+// 4096 t + 32 i. Every load of a wavefront is contiguous either way.
+__host__ __device__ int cascade_key_pos(int sg) {
+    const int j = sg >> 3, m = sg & 7;
+    return ((m >> 1) << 9) | (j << 1) | (m & 1);
+}
+
+// One RNS limb (blockIdx.y) of IPW items (LWEs sharing one key) per
+// workgroup, TL = 128 IPW threads: thread j ends each transform holding
+// outputs OPT j .. OPT j + OPT - 1 (OPT = 8 / IPW) of every item, i.e. slots
+// 2 OPT j .. 2 OPT j + 2 OPT - 1, and each key word it loads serves all IPW
+// items (the key stream per item falls with IPW; the last pass shrinks to
+// log2(OPT) stages). Passes before the last give every thread 8 elements:
+// units w = j + TL v of the items' units.
+template <int LOGNT, int IPW>
+__global__ __launch_bounds__(128 * IPW, IPW == 4 ? 4 : 3) void k_cascade_mac(RingDev r, Gadget g, MacArgs a) {
+    constexpr int TL = 128 * IPW, OPT = 8 / IPW, SPT = 2 * OPT;  // threads, outputs and slots per item
+    constexpr int RL = IPW == 1 ? 3 : IPW == 2 ? 2 : 1;          // stages of the last pass
+    constexpr int NP = cm_np(LOGNT, RL);
+    constexpr int S = kSlots;
+    // key words loaded ahead of the transform where that fits the register
+    // budget of the occupancy target without spills
+    constexpr bool KPF = NP <= 1 || (IPW == 2 && LOGNT < 10) || (IPW == 4 && LOGNT < 5);
+    // [buffer][item] transform buffers of the limb; at the end the body sums
+    // (2048 padded words per item)
+    __shared__ double s[2 * IPW * kCmBuf];
+    const int j = threadIdx.x;
+    const int l = blockIdx.y;
+    const MacTask task = mac_task(blockIdx.x, a.items / IPW, a.items_per_key / IPW, a.groups, a.tiles);
+    const int item0 = task.item * IPW, grp = task.grp, tile = task.tile;
+    const FMod m = r.t[l].m;
+    const double *tw = r.t[l].fwd;
+    const double *dc = r.t[l].dfw;
+    const int k0 = grp * g.L / a.groups, k1 = (grp + 1) * g.L / a.groups;
+    const size_t dplane = (size_t)g.L * a.dig_coeffs;  // digit words per item
+    const int16_t *dig = a.dig + (size_t)item0 * dplane + (size_t)tile * kCmIn;
+    const double2 *key = reinterpret_cast<const double2 *>(a.key[item0 / a.items_per_key] + (size_t)tile * S +
+                                                           (size_t)l * a.key_slots);
+    const size_t krow = a.key_slots;  // double2 stride between part rows (two limbs of key_slots words each)
+    double am[IPW][SPT], ab[IPW][SPT];
+#pragma unroll
+    for (int it = 0; it < IPW; it++)
+#pragma unroll
+        for (int e = 0; e < SPT; e++) am[it][e] = ab[it][e] = 0.0;
+    // the digits of the thread's first-pass elements, one digit ahead
+    constexpr int R1 = cm_r(LOGNT, RL, 0);
+    int dcur[8], dnxt[8];
+    auto load_digits = [&](int k, int *d) {
+        const int16_t *dk = dig + (size_t)k * a.dig_coeffs;
+        if constexpr (NP <= 1) {
+            // OPT contiguous digits of each item
+#pragma unroll
+            for (int it = 0; it < IPW; it++) {
+                const int16_t *p = dk + it * dplane + OPT * j;
+#pragma unroll
+                for (int t = 0; t < OPT; t++) d[it * OPT + t] = p[t];
+            }
+        } else {
+            using Gm = UnitGeo<LOGNT, 0, R1>;
+#pragma unroll
+            for (int v = 0; v < (8 >> R1); v++) {
+                const int w = j + TL * v;
+                const int it = w >> (10 - R1), u = w & ((1024 >> R1) - 1);
+                int base, SB;
+                Gm::at(u, base, SB);
+#pragma unroll
+                for (int t = 0; t < (1 << R1); t++) d[(v << R1) + t] = dk[it * dplane + base + (t << Gm::JB)];
+            }
+        }
+    };
+    if (k0 < k1) load_digits(k0, dcur);
+    for (int k = k0; k < k1; k++) {
+        const int ks = g.L - 1 - k;
+        // this digit's key words (the thread's slot pairs): loaded here, in
+        // flight during the transform, where the registers allow (KPF), else
+        // at the multiply-accumulate
+        const double2 *km = key + (size_t)ks * 2 * krow;
+        const double2 *kb = km + krow;
+        double2 vm[OPT], vb[OPT];
+        auto load_keys = [&]() {
+#pragma unroll
+            for (int t = 0; t < OPT; t++) {
+                const int pos = cascade_key_pos(SPT * j + 2 * t) >> 1;
+                vm[t] = km[pos];
+                vb[t] = kb[pos];
+            }
+        };
+        if constexpr (KPF) load_keys();
+        if (k + 1 < k1) load_digits(k + 1, dnxt);
+        double x[IPW][OPT];
+        if constexpr (NP <= 1) {
+#pragma unroll
+            for (int it = 0; it < IPW; it++) {
+#pragma unroll
+                for (int t = 0; t < OPT; t++) x[it][t] = (double)dcur[it * OPT + t];
+                if constexpr (NP == 1) {
+#pragma unroll
+                    for (int sgm = 0; sgm < (OPT >> LOGNT); sgm++) direct_pass<LOGNT>(x[it] + (sgm << LOGNT), dc);
+                }
+            }
+        } else {
+            double *buf = s + (k & 1) * IPW * kCmBuf;
+            {
+                // first pass: stages 0 .. R1-1 in direct form
+                using Gm = UnitGeo<LOGNT, 0, R1>;
+#pragma unroll
+                for (int v = 0; v < (8 >> R1); v++) {
+                    const int w = j + TL * v;
+                    const int it = w >> (10 - R1), u = w & ((1024 >> R1) - 1);
+                    int base, SB;
+                    Gm::at(u, base, SB);
+                    double y[1 << R1];
+#pragma unroll
+                    for (int t = 0; t < (1 << R1); t++) y[t] = (double)dcur[(v << R1) + t];
+                    direct_pass<R1>(y, dc);
+                    double *p = buf + it * kCmBuf + lds_pad(base);
+#pragma unroll
+                    for (int t = 0; t < (1 << R1); t++) p[unit_off<(1 << Gm::JB)>(t)] = y[t];
+                }
+            }
+            __syncthreads();
+            cm_middle<LOGNT, RL, IPW, 1>(buf, j, tw, m);
+            // last pass: the thread's OPT contiguous outputs of every item, in registers
+            {
+                constexpr int L0 = LOGNT - RL;
+                int base, SB;
+                UnitGeo<LOGNT, L0, RL>::at(j, base, SB);
+                double w[(1 << RL) - 1];
+                load_unit_tw<RL, L0>(w, tw, SB);
+#pragma unroll
+                for (int it = 0; it < IPW; it++) {
+                    const double *p = buf + it * kCmBuf + lds_pad(OPT * j);
+#pragma unroll
+                    for (int t = 0; t < OPT; t++) x[it][t] = p[t];
+                    unit_stages<RL>(x[it], w, m);
+                }
+            }
+        }
+        // multiply-accumulate: slot pair t of the thread is transform output t
+        if constexpr (!KPF) load_keys();
+#pragma unroll
+        for (int t = 0; t < OPT; t++)
+#pragma unroll
+            for (int it = 0; it < IPW; it++) {
+                am[it][2 * t] += mod_mul_lazy(x[it][t], vm[t].x, m);
+                am[it][2 * t + 1] += mod_mul_lazy(x[it][t], vm[t].y, m);
+                ab[it][2 * t] += mod_mul_lazy(x[it][t], vb[t].x, m);
+                ab[it][2 * t + 1] += mod_mul_lazy(x[it][t], vb[t].y, m);
+            }
+#pragma unroll
+        for (int e = 0; e < 8; e++) dcur[e] = dnxt[e];
+    }
+    // Body: the sum over the tile's 2048 / n segments of degree n = 2 nt
+    // (exact fp64 sums of canonical residues, below 2^49), reduced once.
+    constexpr int N = 2 << LOGNT;
+    constexpr int G = N >= SPT ? S / N : TL;  // terms per output
+    constexpr int BW = lds_words(S);         // body words per item
+    static_assert(IPW * BW <= 2 * IPW * kCmBuf, "body sums fit the transform buffers");
+    __syncthreads();  // the transform buffers become the body buffers
+#pragma unroll
+    for (int it = 0; it < IPW; it++) {
+        const size_t ot = ((size_t)grp * a.items + item0 + it) * a.tiles + tile;
+        double2 *om = reinterpret_cast<double2 *>(a.out_mask + (ot * 2 + l) * S + SPT * j);
+#pragma unroll
+        for (int t = 0; t < OPT; t++)
+            om[t] = make_double2(mod_reduce_small(am[it][2 * t], m), mod_reduce_small(am[it][2 * t + 1], m));
+        double *sb = s + it * BW;
+        if constexpr (N >= SPT) {
+#pragma unroll
+            for (int e = 0; e < SPT; e++) sb[lds_pad(SPT * j + e)] = mod_reduce_small(ab[it][e], m);
+        } else {
+            // the thread's slots are SPT / N whole segments: sum them first
+#pragma unroll
+            for (int c = 0; c < N; c++) {
+                double acc = mod_reduce_small(ab[it][c], m);
+#pragma unroll
+                for (int sgm = 1; sgm < SPT / N; sgm++) acc += mod_reduce_small(ab[it][sgm * N + c], m);
+                sb[lds_pad(j * N + c)] = acc;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int half = G >> 1; half >= 1; half >>= 1) {
+#pragma unroll
+        for (int it = 0; it < IPW; it++) {
+            double *sb = s + it * BW;
+            for (int t = j; t < half * N; t += TL) sb[lds_pad(t)] += sb[lds_pad(t + half * N)];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int it = 0; it < IPW; it++) {
+        const size_t ot = ((size_t)grp * a.items + item0 + it) * a.tiles + tile;
+        const double *sb = s + it * BW;
+        for (int c = j; c < N; c += TL) a.out_body[(ot * 2 + l) * N + c] = mod_reduce_small(sb[lds_pad(c)], m);
+    }
+}
+
+bool cascade_mac_fits(const RingDev &r, const Gadget &g, const MacArgs &a) {
+    // n1 = 2048 with two limbs, embedded inputs, two tiles of 2048 slots,
+    // degrees up to 2048, and digits small enough for the exact direct pass:
+    // (B/2) (1 + 7 (q - 1)) + 14 q below 2^45 for the lazy products after it
+    if (r.nl != 2 || !a.embed || a.tiles != 2 || a.dig_coeffs != (size_t)kSlots || !a.reduce_body) return false;
+    if (r.logn < 1 || r.logn > kSlotsLog || a.key_slots != (size_t)2 * kSlots) return false;
+    const double qmax = r.t[0].m.q > r.t[1].m.q ? r.t[0].m.q : r.t[1].m.q;
+    const double half_b = (double)(g.base / 2 + 1);
+    return half_b * 8.0 * qmax + 14.0 * qmax < 35184372088832.0;  // 2^45
+}
+
+template <int IPW>
+static void launch_cm(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t st) {
+    const dim3 grid(a.items / IPW * a.groups * a.tiles, 2), blk(128 * IPW);
+    switch (r.logn - 1) {
+        case 0: hipLaunchKernelGGL((k_cascade_mac<0, IPW>), grid, blk, 0, st, r, g, a); break;
+        case 1: hipLaunchKernelGGL((k_cascade_mac<1, IPW>), grid, blk, 0, st, r, g, a); break;
+        case 2: hipLaunchKernelGGL((k_cascade_mac<2, IPW>), grid, blk, 0, st, r, g, a); break;
+        case 3: hipLaunchKernelGGL((k_cascade_mac<3, IPW>), grid, blk, 0, st, r, g, a); break;
+        case 4: hipLaunchKernelGGL((k_cascade_mac<4, IPW>), grid, blk, 0, st, r, g, a); break;
+        case 5: hipLaunchKernelGGL((k_cascade_mac<5, IPW>), grid, blk, 0, st, r, g, a); break;
+        case 6: hipLaunchKernelGGL((k_cascade_mac<6, IPW>), grid, blk, 0, st, r, g, a); break;
+        case 7: hipLaunchKernelGGL((k_cascade_mac<7, IPW>), grid, blk, 0, st, r, g, a); break;
+        case 8: hipLaunchKernelGGL((k_cascade_mac<8, IPW>), grid, blk, 0, st, r, g, a); break;
+        case 9: hipLaunchKernelGGL((k_cascade_mac<9, IPW>), grid, blk, 0, st, r, g, a); break;
+        case 10: hipLaunchKernelGGL((k_cascade_mac<10, IPW>), grid, blk, 0, st, r, g, a); break;
+        default: break;
+    }
+}
+
+void launch_cascade_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t st) {
+    // a.ipw items per workgroup where that many consecutive items share a key
+    if (a.ipw >= 4 && a.items_per_key % 4 == 0)
+        launch_cm<4>(r, g, a, st);
+    else if (a.ipw >= 2 && a.items_per_key % 2 == 0)
+        launch_cm<2>(r, g, a, st);
+    else
+        launch_cm<1>(r, g, a, st);
+}
+
+// Cascade key rows in the layout of k_cascade_mac: every (k, part, limb)
+// row of 2 * 2048 slots, tile by tile, slot sg of a tile moved to
+// cascade_key_pos(sg).
+__global__ void k_cascade_key_permute(const double *src, double *dst, size_t rows) {
+    const size_t total = rows * 2 * kSlots;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t tile_base = t & ~(size_t)(kSlots - 1);
+        dst[tile_base + cascade_key_pos((int)(t & (kSlots - 1)))] = src[t];
+    }
+}
+
+void launch_cascade_key_permute(const double *src, double *dst, size_t rows, hipStream_t st) {
+    if (!rows) return;
+    const size_t total = rows * 2 * kSlots;
+    const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_cascade_key_permute, dim3(blocks), dim3(256), 0, st, src, dst, rows);
+}
+
+}  // namespace via

@@ -51,7 +51,12 @@ struct StepTune {
     int radix16 = 0;
     int t512 = 0;  // the 512-thread kernel (radix-8 passes)
     int lsplit = 0;  // one workgroup per RNS limb (with the 256- or 512-thread kernel)
+    int ipw = 2;     // fixed-shape cascade MAC: items (LWEs of one key) per workgroup, 1, 2 or 4
 };
+
+// The fixed-shape cascade MAC's items per workgroup that a launch-shape
+// variant (0..5) selects: 1, 2 or 4 for variant mod 3 = 0, 1, 2.
+inline int variant_ipw(int variant) { return variant % 3 == 0 ? 1 : variant % 3 == 1 ? 2 : 4; }
 
 // Buffers of the VIA-B repack, sized for batches of up to T queries.
 struct RepackWs {
@@ -148,6 +153,10 @@ struct via_keys {
     via_ctx *ctx = nullptr;
     via_params pp;
     std::vector<double *> cascade;  // per step, [k][part][limb][rank_in*n_out]
+    // cascade rows in the slot order of k_cascade_mac (cascade_key_pos within
+    // each 2048-slot tile), set when its exactness bound holds for these
+    // parameters (cascade_fixed_ok); otherwise slot order
+    bool cascade_perm = false;
     double *conv = nullptr;         // [k][part][limb][n1]
     double *rsk = nullptr;          // [j][k][part][n2]
     // VIA-B repack keys: the cascade steps of output degree repack_nout0..n1

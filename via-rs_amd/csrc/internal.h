@@ -21,10 +21,13 @@ constexpr int kMaxQ = 32;       // queries answered together in one batched laun
 // exact in fp64 lanes (ntt_lazy.h).
 constexpr uint64_t kMaxModulus = 1ULL << 38;
 
+constexpr int kDirectWords = 24;  // direct first-pass constants stored after each forward table
+
 // Twiddles of one (q, n): bit-reversed psi^k and psi^-k as exact doubles,
 // the layout of via-rs ring/ntt.rs:206-221.
 struct NttTab {
     const double *fwd;
+    const double *dfw;  // kDirectWords constants of the direct first forward pass
     const double *inv;
     double n_inv;
     FMod m;
@@ -124,8 +127,18 @@ struct MacArgs {
     int radix16;             // 256-thread kernel: radix-16 transform passes (else radix 8)
     int t512;                // 512-thread kernel with radix-8 passes
     int lsplit;              // two limbs: one workgroup per limb (not with `wide`)
+    int ipw;                 // fixed-shape cascade MAC: items per workgroup (1, 2 or 4)
 };
 void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t s);
+// The cascade-step MAC in its fixed shape (cascade_mac.hip): two q1 limbs,
+// embedded inputs, two tiles of 2048 slots, keys in the cascade_key_pos
+// order. cascade_mac_fits says whether a launch meets those conditions and
+// the exactness bound of its direct first pass.
+bool cascade_mac_fits(const RingDev &r, const Gadget &g, const MacArgs &a);
+void launch_cascade_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t s);
+__host__ __device__ int cascade_key_pos(int sg);
+// rows of 2 x 2048 slots from slot order into the cascade_key_pos order
+void launch_cascade_key_permute(const double *src, double *dst, size_t rows, hipStream_t s);
 
 // One cascade combine: masks_next = -iNTT(sum pm), body_next =
 // embed(body_prev) - iNTT(sum pb), then the digits of masks_next into

@@ -87,6 +87,33 @@ uint32_t bitrev(uint32_t x, int logn) {
     return r;
 }
 
+// Constants of the direct form of the first forward pass (stages 0..R-1 of
+// block 0, twiddles W_j = tw[j] of the bit-reversed table, products mod q,
+// signs folded in as negative values): [W1, W2, W2W1, W3, W3W1] for R = 2
+// (and W1 alone for R = 1), then for R = 3 the coefficients of x1, x5, x3,
+// x7 in W_{4+k} * b_{2k+1}, k = 0..3, where b_1, b_3, b_5, b_7 are the odd
+// outputs of the first two stages:
+//   b1 = x1 + W1 x5 + W2 x3 + W2W1 x7    b3 = x1 + W1 x5 - W2 x3 - W2W1 x7
+//   b5 = x1 - W1 x5 + W3 x3 - W3W1 x7    b7 = x1 - W1 x5 - W3 x3 + W3W1 x7
+// Entries whose twiddles the table of degree n lacks stay zero.
+void direct_consts(const double *tw, uint32_t n, uint64_t q, double *out) {
+    auto w = [&](uint32_t j) -> uint64_t { return j < n ? (uint64_t)tw[j] : 0; };
+    auto mul = [&](uint64_t a, uint64_t b) -> uint64_t { return (uint64_t)((u128)a * b % q); };
+    const uint64_t w1 = w(1), w2 = w(2), w3 = w(3), w21 = mul(w2, w1), w31 = mul(w3, w1);
+    out[0] = (double)w1;
+    out[1] = (double)w2;
+    out[2] = (double)w21;
+    out[3] = (double)w3;
+    out[4] = (double)w31;
+    // coefficient signs of (x1, x5, x3, x7) in b1, b3, b5, b7
+    const int sg[4][4] = {{1, 1, 1, 1}, {1, 1, -1, -1}, {1, -1, 1, -1}, {1, -1, -1, 1}};
+    for (int k = 0; k < 4; k++) {
+        const uint64_t wk = w(4 + k), wx = k < 2 ? w2 : w3, wx1 = k < 2 ? w21 : w31;
+        const uint64_t v[4] = {wk, mul(wk, w1), mul(wk, wx), mul(wk, wx1)};
+        for (int e = 0; e < 4; e++) out[5 + 4 * k + e] = sg[k][e] * (double)v[e];
+    }
+}
+
 int ilog2(uint64_t x) { return x <= 1 ? 0 : 64 - __builtin_clzll(x - 1); }
 bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
 
@@ -118,12 +145,17 @@ via_status get_tab(via_ctx *c, uint64_t q, uint32_t n, DevTab *out) {
         cf = (uint64_t)((u128)cf * psi % q);
         ci = (uint64_t)((u128)ci * psi_inv % q);
     }
+    // after the forward table: the constants of the direct first pass
+    // (kDirectWords doubles, see direct_consts)
+    std::vector<double> dcon(kDirectWords, 0.0);
+    direct_consts(f.data(), n, q, dcon.data());
+    f.insert(f.end(), dcon.begin(), dcon.end());
     DevTab t;
     t.q = q;
     t.n_inv = (double)inv_mod(n % q, q);
-    HIP_TRY(hipMalloc(&t.fwd, sizeof(double) * n));
+    HIP_TRY(hipMalloc(&t.fwd, sizeof(double) * (n + kDirectWords)));
     HIP_TRY(hipMalloc(&t.inv, sizeof(double) * n));
-    HIP_TRY(hipMemcpy(t.fwd, f.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.fwd, f.data(), sizeof(double) * (n + kDirectWords), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t.inv, iv.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     c->tabs[key] = t;
     *out = t;
@@ -140,6 +172,7 @@ via_status make_ring(via_ctx *c, uint32_t n, const uint64_t *q, int nl, RingDev 
         via_status s = get_tab(c, q[l], n, &t);
         if (s) return s;
         r->t[l].fwd = t.fwd;
+        r->t[l].dfw = t.fwd + n;
         r->t[l].inv = t.inv;
         r->t[l].n_inv = t.n_inv;
         r->t[l].m = fmod_make(q[l]);
@@ -232,6 +265,16 @@ via_status check_params(const via_params *pp) {
     return VIA_OK;
 }
 
+// Whether the cascade steps take the fixed-shape MAC (k_cascade_mac) for
+// these parameters: its direct first pass sums eight products of a digit
+// (at most B/2 in magnitude) with twiddle products below q, and the lazy
+// stages after it add below 2q each, all within 2^45 (the input bound of the
+// lazy products). The cascade keys are then stored in its slot order.
+bool cascade_fixed_ok(const via_params *pp) {
+    const u128 qmax = std::max(pp->q1[0], pp->q1[1]);
+    return ((u128)(pp->ck_base / 2 + 1) * 8 + 14) * qmax < ((u128)1 << 45);
+}
+
 // The context a call runs on, held for the duration of the call. Device-
 // buffer entry points take the context itself (waiting for a call in
 // progress, since their caller synchronises on its stream); host-buffer
@@ -322,6 +365,9 @@ std::vector<StepTune> tune_candidates(int items, int L) {
                 t.radix16 = variant == 1;
                 t.t512 = variant == 3 || variant == 5;
                 t.lsplit = variant >= 4;
+                // the items per workgroup of the fixed-shape cascade MAC follow the
+                // variant's number in via_ctx_set_tuning (1 wide, 2 radix 16)
+                t.ipw = variant_ipw(variant == 1 ? 2 : variant == 2 ? 1 : variant);
                 out.push_back(t);
             }
     if (out.empty()) out.push_back(StepTune());
@@ -927,12 +973,13 @@ via_status via_keys_create(via_ctx *c, const via_params *pp, const uint64_t *cas
     // fp64 copy and its q2 switch; every return below frees them and the
     // key buffers made so far
     uint64_t *stage = nullptr;
-    double *tmp_f = nullptr, *tmp_q2 = nullptr;
+    double *tmp_f = nullptr, *tmp_q2 = nullptr, *tmp_perm = nullptr;
     auto fail = [&](via_status st) {
         (void)hipStreamSynchronize(c->stream);
         (void)hipFree(stage);
         (void)hipFree(tmp_f);
         (void)hipFree(tmp_q2);
+        (void)hipFree(tmp_perm);
         keys_free(k);
         return st;
     };
@@ -942,6 +989,9 @@ via_status via_keys_create(via_ctx *c, const via_params *pp, const uint64_t *cas
     if (hipMalloc(&stage, sizeof(uint64_t) * max_step) != hipSuccess) return fail(VIA_E_NOMEM);
     if (vb && (hipMalloc(&tmp_f, sizeof(double) * max_step) != hipSuccess ||
                hipMalloc(&tmp_q2, sizeof(double) * max_step / 2) != hipSuccess))
+        return fail(VIA_E_NOMEM);
+    k->cascade_perm = cascade_fixed_ok(pp);
+    if (k->cascade_perm && hipMalloc((void **)&tmp_perm, sizeof(double) * max_step) != hipSuccess)
         return fail(VIA_E_NOMEM);
     size_t off = 0;
     for (uint32_t nout = 2; nout <= n1; nout <<= 1) {
@@ -957,6 +1007,13 @@ via_status via_keys_create(via_ctx *c, const via_params *pp, const uint64_t *cas
         RingDev r;
         if ((s = make_ring(c, nout, pp->q1, 2, &r))) return fail(s);
         launch_ntt(r, d, (size_t)L * 2 * nl * rank_in, rank_in, false, c->stream);
+        if (k->cascade_perm) {
+            // rows of rank_in * nout = 2 n1 slots into the order of the fixed-shape MAC
+            launch_cascade_key_permute(d, tmp_perm, (size_t)L * 2 * nl, c->stream);
+            if (hipMemcpyAsync(d, tmp_perm, sizeof(double) * words, hipMemcpyDeviceToDevice, c->stream) !=
+                hipSuccess)
+                return fail(VIA_E_DEVICE);
+        }
         if (vb && nout >= k->repack_nout0) {
             // mod_switch_rlev (repack.rs:43-56) of every sample, q1 -> q2
             const size_t w2 = words / nl;
@@ -1007,6 +1064,7 @@ via_status via_keys_create(via_ctx *c, const via_params *pp, const uint64_t *cas
     k->bytes.push_back(key_rsk_bytes(pp));
     for (size_t i = 0; i < k->repack.size(); i++) k->bytes.push_back(key_repack_bytes(pp, k->repack_nout0, i));
     (void)hipFree(stage);
+    (void)hipFree(tmp_perm);
     (void)hipFree(tmp_f);
     (void)hipFree(tmp_q2);
     if (hipGetLastError() != hipSuccess) {
@@ -1209,11 +1267,17 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         a.radix16 = step_tune[step].radix16;
         a.t512 = step_tune[step].t512;
         a.lsplit = step_tune[step].lsplit;
+        a.ipw = step_tune[step].ipw;
         a.out_mask = w.pm;
         a.out_body = w.pb;
         a.reduce_body = 1;
         a.items = (int)items;
-        launch_gadget_mac(r, gck, a, st);
+        if (keys[0]->cascade_perm) {
+            if (!cascade_mac_fits(r, gck, a)) return VIA_E_INVALID;
+            launch_cascade_mac(r, gck, a, st);
+        } else {
+            launch_gadget_mac(r, gck, a, st);
+        }
         CombineArgs cb;
         cb.pm = w.pm;
         cb.pb = w.pb;
@@ -1878,6 +1942,7 @@ via_status via_ctx_set_tuning(via_ctx *c, uint32_t nq, const int *groups, const 
         steps[i].radix16 = v == 2;
         steps[i].t512 = v == 3 || v == 5;
         steps[i].lsplit = v >= 4;
+        steps[i].ipw = variant_ipw(v);
     }
     tuned_store(c->device, (int)nq, steps);
     return VIA_OK;

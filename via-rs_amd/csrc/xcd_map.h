@@ -15,11 +15,14 @@
 // or a divisor of eight, every task of one key lands on the XCDs congruent to
 // it, so each key streams into one L2 (or 8/keys of them) instead of all
 // eight; otherwise tasks follow the linear order. A bijection in every case.
+// Within a key, order 0 runs the tasks item by item (tiles fastest); order 1
+// tile by tile, then digit group by group, items fastest, so the workgroups
+// in flight at a time read one tile's (and group's) slice of the key.
 struct MacTask {
     int item, grp, tile;
 };
 
-XCD_HD MacTask mac_task(int lin, int items, int items_per_key, int groups, int tiles) {
+XCD_HD MacTask mac_task(int lin, int items, int items_per_key, int groups, int tiles, int order = 0) {
     const int per = groups * tiles;
     const int nkeys = items / items_per_key;
     const int tk = items_per_key * per;
@@ -38,9 +41,17 @@ XCD_HD MacTask mac_task(int lin, int items, int items_per_key, int groups, int t
         tc = lin % tk;
     }
     MacTask t;
-    t.item = key * items_per_key + tc / per;
-    const int rem = tc % per;
-    t.grp = rem / tiles;
-    t.tile = rem % tiles;
+    if (order == 0) {
+        t.item = key * items_per_key + tc / per;
+        const int rem = tc % per;
+        t.grp = rem / tiles;
+        t.tile = rem % tiles;
+    } else {
+        const int ig = items_per_key * groups;
+        t.tile = tc / ig;
+        const int rem = tc % ig;
+        t.grp = rem / items_per_key;
+        t.item = key * items_per_key + rem % items_per_key;
+    }
     return t;
 }
