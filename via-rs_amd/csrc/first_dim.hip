@@ -433,14 +433,15 @@ void launch_fd_gather(const double *out_k, double *out_eval, const FdPlan &p, FM
 // part), reads their n slots straight from the slot-major segment outputs
 // (IPP * 8 bytes per slot row and segment), adds the segments, transforms
 // the polys back in LDS and writes them coefficient form to [q][j][part][k].
-// The 16 / IPP workgroups that read one 128-byte row of 16 polys get block
-// ids congruent mod 8, i.e. one XCD and its L2, and run side by side (groups
-// of 8 rows), so each row is fetched from HBM once.
-template <int IPP, int TPB>
+// The ROWP / IPP workgroups that read one row of ROWP polys (8 ROWP bytes of
+// each slot) get block ids congruent mod 8, i.e. one XCD and its L2, and run
+// side by side (groups of 8 rows), so each row is fetched from HBM once, in
+// runs of 8 ROWP bytes.
+template <int IPP, int TPB, int ROWP>
 __global__ __launch_bounds__(TPB) void k_fd_intt(const double *__restrict__ ok, double *__restrict__ out, FdPlan p,
                                                  RingDev r2) {
     __shared__ double s[lds_words(IPP * kSlots)];
-    constexpr int LPW = 16 / IPP;  // workgroups per row of 16 polys
+    constexpr int LPW = ROWP / IPP;  // workgroups per row group of ROWP polys
     const size_t P = (size_t)p.Q * 2 * p.J;
     const int n = p.n;
     const size_t b = blockIdx.x, grp = b / (8 * LPW), wi = b % (8 * LPW);
@@ -487,13 +488,20 @@ void launch_fd_intt(const double *out_k, double *out, const FdPlan &p, const Rin
     const size_t P = (size_t)p.Q * 2 * p.J;
     // four polys per workgroup when that still gives a thousand workgroups,
     // else fewer, so small passes fill the 256 CUs
-    auto blocks = [&](int ipp) { return ((P + ipp - 1) / ipp + 127) / 128 * 128; };
+    // (grids in whole XCD groups of 8 * ROWP / ipp workgroups)
+    auto blocks = [&](int ipp, int rowp) {
+        const size_t g = (size_t)8 * rowp / ipp;
+        return ((P + ipp - 1) / ipp + g - 1) / g * g;
+    };
     if (P >= 4 * 1024)
-        hipLaunchKernelGGL((k_fd_intt<4, 512>), dim3((unsigned)blocks(4)), dim3(512), 0, st, out_k, out, p, r2);
+        hipLaunchKernelGGL((k_fd_intt<4, 512, 64>), dim3((unsigned)blocks(4, 64)), dim3(512), 0, st, out_k, out, p,
+                           r2);
     else if (P >= 2 * 512)
-        hipLaunchKernelGGL((k_fd_intt<2, 256>), dim3((unsigned)blocks(2)), dim3(256), 0, st, out_k, out, p, r2);
+        hipLaunchKernelGGL((k_fd_intt<2, 256, 16>), dim3((unsigned)blocks(2, 16)), dim3(256), 0, st, out_k, out, p,
+                           r2);
     else
-        hipLaunchKernelGGL((k_fd_intt<1, 256>), dim3((unsigned)blocks(1)), dim3(256), 0, st, out_k, out, p, r2);
+        hipLaunchKernelGGL((k_fd_intt<1, 256, 16>), dim3((unsigned)blocks(1, 16)), dim3(256), 0, st, out_k, out, p,
+                           r2);
 }
 
 }  // namespace via
