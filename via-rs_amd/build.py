@@ -9,7 +9,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["csrc/ntt.hip", "csrc/kernels.hip", "csrc/first_dim.hip", "csrc/repack.hip", "csrc/prg.hip", "csrc/noalloc.hip", "csrc/cascade_mac.hip", "csrc/via_gpu.cpp",
+SOURCES = ["csrc/ntt.hip", "csrc/kernels.hip", "csrc/first_dim.hip", "csrc/repack.hip", "csrc/prg.hip", "csrc/noalloc.hip", "csrc/cascade_mac.hip", "csrc/gate_fx.hip", "csrc/via_gpu.cpp",
            "csrc/serve.cpp", "csrc/wire.cpp", "csrc/client.cpp",
            "csrc/client_keys.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -69,4 +69,34 @@ __device__ __forceinline__ void write_digits(double a0, double a1, const RingDev
     for (int k = 0; k < g.L; k++) dig[(size_t)k * stride] = (int16_t)next_digit(lo, hi, k, dp);
 }
 
+// ---- gate inputs -----------------------------------------------------------------
+
+// Coefficient sg of the gate input ct for item j (see launch_gate).
+template <int NL>
+__device__ __forceinline__ void gate_input(int mode, const double *in, int j, int part, int l, int sg, int n,
+                                           int shift, const FMod &md, double &base_v, double &ct_v) {
+    const size_t pw = (size_t)NL * n;
+    if (mode == GATE_DMUX) {
+        const double x = in[(size_t)j * 2 * pw + part * pw + (size_t)l * n + sg];
+        base_v = x;
+        ct_v = x;
+    } else if (mode == GATE_CMUX) {
+        const double x0 = in[(size_t)(2 * j) * 2 * pw + part * pw + (size_t)l * n + sg];
+        const double x1 = in[(size_t)(2 * j + 1) * 2 * pw + part * pw + (size_t)l * n + sg];
+        base_v = x0;
+        ct_v = mod_sub(x1, x0, md);
+    } else {
+        // CRot SlotExtract (rotate.rs:128-149): rot = -(res * X^shift),
+        // ct = rot - res, out = res + RGSW (x) ct.
+        const double *p = in + part * pw + (size_t)l * n;
+        const double x = p[sg];
+        double rv;
+        if (sg >= shift) rv = p[sg - shift];
+        else rv = mod_neg(p[sg - shift + n], md);
+        const double rneg = mod_neg(rv, md);
+        base_v = x;
+        ct_v = mod_sub(rneg, x, md);
+    }
+}
+
 }  // namespace via

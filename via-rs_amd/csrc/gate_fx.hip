@@ -1,0 +1,279 @@
+// gate_fx.hip — the tree gates (RGSWCiphertext::external_product,
+// rgsw.rs:108-120, then dmux mux.rs:129-137, cmux mux.rs:75-84, or the
+// SlotExtract step of crot, rotate.rs:128-149) in a fixed shape for the wide
+// tree levels: degree 2048, gadget depth 2, one RNS limb of one gate per
+// workgroup of 256 threads.
+//
+// Thread j computes the gadget digits of the input coefficients
+// c = j + 256 t (t < 8) of one ciphertext half at a time, which are exactly
+// the elements of its units in the first forward pass, so digits never pass
+// through LDS; it ends the forward transforms holding eval slots
+// 8j .. 8j+7 of both digit polys, multiplies them with the RGSW rows in
+// registers, and after both halves starts the inverse transforms of the two
+// product parts on those same slots (the first inverse stages pair adjacent
+// slots); the last inverse pass again holds coefficients j + 256 t, where
+// the gate combine reads its inputs. Per half: three barriers; inverse:
+// three. With one-limb digits at most B/2 in magnitude (the CMux and CRot
+// gadget at q2), the first forward pass runs in direct form (ntt_fixed.h).
+#include "digits.h"
+#include "ntt_fixed.h"
+
+namespace via {
+
+namespace {
+
+constexpr int kGxBuf = lds_words(kSlots);
+
+// The exact 128-bit gadget scale of a two-limb coefficient (gadget.rs:115-151)
+// for the rare values the fp64 estimate leaves undecided, as a signed exact
+// double (the scale is below B^L < 2^50). Out of line: it keeps its wide
+// registers out of the kernel's unrolled digit loop.
+__device__ __noinline__ double gx_scaled_exact2(double a0, double a1, const RingDev &r, const Gadget &g) {
+    const via_u128 x = rns_reconstruct(a0, a1, r.rns);
+    const via_u128 Q = r.rns.big_q;
+    const int neg = x > (Q >> 1);
+    const via_u128 mag = neg ? Q - x : x;
+    const double v = (double)(uint64_t)round_mul_div(mag, g.bl, r.qdiv);
+    return neg ? -v : v;
+}
+
+// The signed gadget scale round-half-away(c_centred * B^L / Q) of one
+// coefficient as an exact double: the fp64 paths of gadget_scale (g.f64 = 1
+// with one limb, 2 with two, which the launcher requires).
+template <int NL>
+__device__ __forceinline__ double gx_scaled(double a0, double a1, const RingDev &r, const Gadget &g) {
+    if constexpr (NL == 2) {
+        SMag v;
+        if (gadget_scale2_f64(a0, a1, r.rns, g.sf, v)) {
+            const double mg = (double)(uint64_t)v.mag;
+            return v.neg ? -mg : mg;
+        }
+        return gx_scaled_exact2(a0, a1, r, g);
+    } else {
+        const uint64_t q = r.q_single, x = (uint64_t)a0;
+        const bool neg = x > (q >> 1);
+        const double v = (double)scale_div_f64(neg ? q - x : x, g.sf.bl, r.t[0].m.q, r.t[0].m.qinv, q >> 1);
+        return neg ? -v : v;
+    }
+}
+
+// Forward middle pass (L0, R) over the two digit polys in LDS (poly p at
+// s + p * kGxBuf): units w = j + 256 v of the polys' 2048 >> R units each.
+template <int L0, int R>
+__device__ __forceinline__ void gx_fwd_mid(double *s, int j, const double *tw, const FMod &m) {
+    constexpr int E = 1 << R, UPP = kSlots >> R;
+    using G = UnitGeo<kSlotsLog, L0, R>;
+#pragma unroll
+    for (int v = 0; v < (16 >> R); v++) {
+        const int w = j + 256 * v;
+        const int poly = w / UPP, u = w % UPP;
+        int base, SB;
+        G::at(u, base, SB);
+        double wt[E - 1];
+        load_unit_tw<R, L0>(wt, tw, SB);
+        double *p = s + poly * kGxBuf + lds_pad(base);
+        double x[E];
+#pragma unroll
+        for (int t = 0; t < E; t++) x[t] = p[unit_off<(1 << G::JB)>(t)];
+        unit_stages<R>(x, wt, m);
+#pragma unroll
+        for (int t = 0; t < E; t++) p[unit_off<(1 << G::JB)>(t)] = x[t];
+    }
+}
+
+// Inverse middle pass (LH0, R) over the two product parts in LDS.
+template <int LH0, int R>
+__device__ __forceinline__ void gx_inv_mid(double *s, int j, const double *tw, const FMod &m) {
+    constexpr int E = 1 << R, UPP = kSlots >> R;
+    using G = InvGeo<kSlotsLog, LH0, R>;
+#pragma unroll
+    for (int v = 0; v < (16 >> R); v++) {
+        const int w = j + 256 * v;
+        const int poly = w / UPP, u = w % UPP;
+        int base, SB;
+        G::at(u, base, SB);
+        double wt[E - 1];
+        load_inv_unit_tw<kSlotsLog, LH0, R>(wt, tw, SB);
+        double *p = s + poly * kGxBuf + lds_pad(base);
+        double x[E];
+#pragma unroll
+        for (int t = 0; t < E; t++) x[t] = p[unit_off<(1 << LH0)>(t)];
+        inv_unit_stages<R>(x, wt, m);
+#pragma unroll
+        for (int t = 0; t < E; t++) p[unit_off<(1 << LH0)>(t)] = x[t];
+    }
+}
+
+}  // namespace
+
+template <int NL, bool DIRECT, int MODE>
+__global__ __launch_bounds__(256, 3) void k_gate_fx(RingDev r, Gadget g, GateArgs ga) {
+    constexpr int S = kSlots, L = 2;
+    __shared__ double s[2 * kGxBuf];  // two polys: the digit polys of a half, then the product parts
+    const int j = threadIdx.x;
+    const int gate = blockIdx.x, l = NL == 2 ? (int)blockIdx.y : 0;
+    const int q = gate / ga.items_per_query;
+    const int jg = gate - q * ga.items_per_query;
+    constexpr int mode = MODE;
+    const int shift = ga.shift;
+    const double *rgsw = ga.rgsw + (size_t)q * ga.rgsw_qstride;
+    const double *in = ga.in + (size_t)q * ga.in_qstride;
+    double *out = ga.out + (size_t)q * ga.out_qstride;
+    const size_t pw = (size_t)NL * S;
+    const FMod m = r.t[l].m, m0 = r.t[0].m, m1 = r.t[1].m;
+    const double *twf = r.t[l].fwd, *twi = r.t[l].inv;
+    double am[8], ab[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) am[e] = ab[e] = 0.0;
+    for (int half = 0; half < 2; half++) {  // 0: ct.mask with the -s*m rows, 1: ct.body with the m rows
+        // digits of coefficients j + 256 t (both limbs enter the scale), then
+        // stages 0..2 of both digit polys on them
+        {
+            double y[L][8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const int c = j + 256 * t;
+                double b0, c0, b1 = 0.0, c1 = 0.0;
+                gate_input<NL>(mode, in, jg, half, 0, c, S, shift, m0, b0, c0);
+                if (NL == 2) gate_input<NL>(mode, in, jg, half, 1, c, S, shift, m1, b1, c1);
+                // both digits from the low part (B^2 < 2^50), LSB first (gadget.rs:168-190)
+                double v = gx_scaled<NL>(c0, c1, r, g);
+#pragma unroll
+                for (int k = 0; k < L; k++) y[k][t] = (double)digit_f64(v, ga.dp);
+            }
+#pragma unroll
+            for (int k = 0; k < L; k++) {
+                if constexpr (DIRECT) {
+                    direct_pass<3>(y[k], r.t[l].dfw);
+                } else {
+                    double w0[7];
+                    load_unit_tw<3, 0>(w0, twf, 0);
+                    unit_stages<3>(y[k], w0, m);
+                }
+                double *p = s + k * kGxBuf + lds_pad(j);
+#pragma unroll
+                for (int t = 0; t < 8; t++) p[unit_off<256>(t)] = y[k][t];
+            }
+        }
+        __syncthreads();
+        gx_fwd_mid<3, 3>(s, j, twf, m);
+        __syncthreads();
+        gx_fwd_mid<6, 2>(s, j, twf, m);
+        __syncthreads();
+        // last pass: slots 8j .. 8j+7 of both digit polys, then the RGSW
+        // products (rows [half][i][part][limb][n], digit k with row L-1-k)
+        {
+            int base, SB;
+            UnitGeo<kSlotsLog, 8, 3>::at(j, base, SB);
+            double w[7];
+            load_unit_tw<3, 8>(w, twf, SB);
+#pragma unroll
+            for (int k = 0; k < L; k++) {
+                double x[8];
+                const double *p = s + k * kGxBuf + lds_pad(8 * j);
+#pragma unroll
+                for (int t = 0; t < 8; t++) x[t] = p[t];
+                unit_stages<3>(x, w, m);
+                const double2 *km = reinterpret_cast<const double2 *>(
+                    rgsw + ((size_t)half * L + (L - 1 - k)) * 2 * pw + (size_t)l * S + 8 * j);
+                const double2 *kb = reinterpret_cast<const double2 *>(reinterpret_cast<const double *>(km) + pw);
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const double2 vm = km[t], vb = kb[t];
+                    am[2 * t] += mod_mul_lazy(x[2 * t], vm.x, m);
+                    am[2 * t + 1] += mod_mul_lazy(x[2 * t + 1], vm.y, m);
+                    ab[2 * t] += mod_mul_lazy(x[2 * t], vb.x, m);
+                    ab[2 * t + 1] += mod_mul_lazy(x[2 * t + 1], vb.y, m);
+                }
+            }
+        }
+        __syncthreads();  // the LDS is rewritten next (the other half, or the inverse)
+    }
+    // inverse transforms of the two product parts: stages 0..2 on slots
+    // 8j .. 8j+7 in registers (inputs brought into (-q, 2q))
+    {
+        double w[7];
+        load_inv_unit_tw<kSlotsLog, 0, 3>(w, twi, j);
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            am[e] = lazy_reduce(am[e], m);
+            ab[e] = lazy_reduce(ab[e], m);
+        }
+        inv_unit_stages<3>(am, w, m);
+        inv_unit_stages<3>(ab, w, m);
+        double *p0 = s + lds_pad(8 * j), *p1 = s + kGxBuf + lds_pad(8 * j);
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            p0[t] = am[t];
+            p1[t] = ab[t];
+        }
+    }
+    __syncthreads();
+    gx_inv_mid<3, 3>(s, j, twi, m);
+    __syncthreads();
+    gx_inv_mid<6, 2>(s, j, twi, m);
+    __syncthreads();
+    // last inverse pass: coefficients j + 256 t of both parts, the N^-1
+    // scaling, the gate combine
+    {
+        double w[7];
+        load_inv_unit_tw<kSlotsLog, 8, 3>(w, twi, 0);
+        const double ni = r.t[l].n_inv;
+#pragma unroll
+        for (int part = 0; part < 2; part++) {
+            double x[8];
+            const double *p = s + part * kGxBuf + lds_pad(j);
+#pragma unroll
+            for (int t = 0; t < 8; t++) x[t] = p[unit_off<256>(t)];
+            inv_unit_stages<3>(x, w, m);
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const int c = j + 256 * t;
+                const double pv = mod_mul(x[t], ni, m);
+                double bv, cv;
+                gate_input<NL>(mode, in, jg, part, l, c, S, shift, m, bv, cv);
+                const size_t o = (size_t)part * pw + (size_t)l * S + c;
+                if (mode == GATE_DMUX) {
+                    out[(size_t)(2 * jg) * 2 * pw + o] = mod_sub(cv, pv, m);
+                    out[(size_t)(2 * jg + 1) * 2 * pw + o] = pv;
+                } else {
+                    out[(size_t)jg * 2 * pw + o] = mod_add(bv, pv, m);
+                }
+            }
+        }
+    }
+}
+
+// Whether the fixed-shape gate takes a level: degree 2048, gadget depth 2
+// with B^2 below 2^50 (both digits from the low part), the fp64 scale (one
+// limb: exact, g.f64 = 1; two: decided or exact fallback, g.f64 = 2);
+// `direct` when one limb's digits (at most B/2) allow the direct first pass
+// with the lazy stages after it below 2^45.
+bool gate_fx_fits(const RingDev &r, const Gadget &g, bool *direct) {
+    if (r.n != kSlots || g.L != 2 || (double)g.base * g.base >= 1125899906842624.0) return false;
+    if (g.f64 != (r.nl == 2 ? 2 : 1)) return false;
+    const double qmax = r.t[0].m.q > r.t[1].m.q ? r.t[0].m.q : r.t[1].m.q;
+    *direct = r.nl == 1 && ((double)(g.base / 2 + 1) * 8.0 + 14.0) * qmax < 35184372088832.0;  // 2^45
+    return true;
+}
+
+template <int NL, bool DIRECT>
+static void launch_gx(const dim3 &grid, const RingDev &r, const Gadget &g, const GateArgs &a, hipStream_t st) {
+    switch (a.mode) {
+        case GATE_DMUX: hipLaunchKernelGGL((k_gate_fx<NL, DIRECT, GATE_DMUX>), grid, dim3(256), 0, st, r, g, a); break;
+        case GATE_CMUX: hipLaunchKernelGGL((k_gate_fx<NL, DIRECT, GATE_CMUX>), grid, dim3(256), 0, st, r, g, a); break;
+        default: hipLaunchKernelGGL((k_gate_fx<NL, DIRECT, GATE_CROT>), grid, dim3(256), 0, st, r, g, a); break;
+    }
+}
+
+void launch_gate_fx(const RingDev &r, const Gadget &g, const GateArgs &a, bool direct, hipStream_t st) {
+    const int blocks = a.items_per_query * a.queries;
+    if (r.nl == 2)
+        launch_gx<2, false>(dim3(blocks, 2), r, g, a, st);
+    else if (direct)
+        launch_gx<1, true>(dim3(blocks), r, g, a, st);
+    else
+        launch_gx<1, false>(dim3(blocks), r, g, a, st);
+}
+
+}  // namespace via

@@ -177,6 +177,12 @@ struct GateArgs {
     int split = -1;      // two limbs, one workgroup per limb: -1 below 256 gates, 0 never, 1 always
 };
 void launch_gate(const RingDev &r, const Gadget &g, const GateArgs &a, hipStream_t s);
+// The fixed-shape gate (gate_fx.hip) for one-limb levels of at least
+// kGateFxMin gates: degree 2048, gadget depth 2; *direct when its first
+// forward pass may run in direct form.
+constexpr int kGateFxMin = 256;
+bool gate_fx_fits(const RingDev &r, const Gadget &g, bool *direct);
+void launch_gate_fx(const RingDev &r, const Gadget &g, const GateArgs &a, bool direct, hipStream_t s);
 
 // Every level of one DMux or CMux tree in one cooperative launch
 // (k_tree_persistent): level v's gates read lev[v].in and write lev[v].out

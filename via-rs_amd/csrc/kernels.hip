@@ -448,34 +448,6 @@ void launch_r2g_combine(const RingDev &r, const double *pm, const double *pb, in
 
 // ---- gates: external product with DMux / CMux / CRot combine -------------------
 
-// Coefficient sg of the gate input ct for item j (see launch_gate).
-template <int NL>
-__device__ __forceinline__ void gate_input(int mode, const double *in, int j, int part, int l, int sg, int n,
-                                           int shift, const FMod &md, double &base_v, double &ct_v) {
-    const size_t pw = (size_t)NL * n;
-    if (mode == GATE_DMUX) {
-        const double x = in[(size_t)j * 2 * pw + part * pw + (size_t)l * n + sg];
-        base_v = x;
-        ct_v = x;
-    } else if (mode == GATE_CMUX) {
-        const double x0 = in[(size_t)(2 * j) * 2 * pw + part * pw + (size_t)l * n + sg];
-        const double x1 = in[(size_t)(2 * j + 1) * 2 * pw + part * pw + (size_t)l * n + sg];
-        base_v = x0;
-        ct_v = mod_sub(x1, x0, md);
-    } else {
-        // CRot SlotExtract (rotate.rs:128-149): rot = -(res * X^shift),
-        // ct = rot - res, out = res + RGSW (x) ct.
-        const double *p = in + part * pw + (size_t)l * n;
-        const double x = p[sg];
-        double rv;
-        if (sg >= shift) rv = p[sg - shift];
-        else rv = mod_neg(p[sg - shift + n], md);
-        const double rneg = mod_neg(rv, md);
-        base_v = x;
-        ct_v = mod_sub(rneg, x, md);
-    }
-}
-
 // RGSWCiphertext::external_product (rgsw.rs:108-120) over an eval-form RGSW
 // [half][i][part][limb][n], then dmux (mux.rs:129-137), cmux (mux.rs:75-84)
 // or the SlotExtract step of crot. One workgroup per output item, n == kSlots;
@@ -849,6 +821,14 @@ void launch_gate(const RingDev &r, const Gadget &g_in, const GateArgs &a_in, hip
         g.sf.bl = (double)(uint64_t)g.bl;
     }
     const int blocks = a.items_per_query * a.queries;
+    bool direct = false;
+    if (r.nl == 1 && blocks >= kGateFxMin && gate_fx_fits(r, g, &direct)) {
+        // wide one-limb levels: the fixed-shape gate (the two-limb levels
+        // keep the fused gate, which extracts each coefficient's digits once
+        // for both limbs)
+        launch_gate_fx(r, g, a, direct, st);
+        return;
+    }
     if (g.L <= 2 && r.n == kSlots) {
         // two limbs: one workgroup per limb of each gate on levels narrower
         // than the CU count (latency-bound); wider levels keep both limbs in

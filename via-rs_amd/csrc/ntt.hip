@@ -1,6 +1,6 @@
 // ntt.hip — batched negacyclic NTT, u64 <-> f64 conversion and the batched
 // negacyclic product (RingPolyEval of via-rs ring/ntt_abstraction.rs:61-78).
-#include "ntt_device.h"
+#include "ntt_fixed.h"
 
 namespace via {
 
@@ -191,9 +191,122 @@ __global__ __launch_bounds__(kThreads) void k_polymul(RingDev r, const uint64_t 
     for (int i = threadIdx.x; i < n; i += blockDim.x) gc[i] = (uint64_t)mod_mul(s[lds_pad(i)], ni, md);
 }
 
+// The same product at n = 2048 with the first and last passes of every
+// transform in registers: thread j takes, in the forward transforms, unit
+// j & 127 of operand j >> 7 (elements u + 128 t, read straight from memory)
+// and then slots 8j .. 8j+7 of both operands (the last forward pass), where
+// it forms the pointwise product and runs the first three inverse stages;
+// the inverse ends on threads j < 128 holding coefficients j + 128 t, which
+// they scale and store. Four barriers per product (radix-16 middle passes).
+__global__ __launch_bounds__(kThreads) void k_polymul_fx(RingDev r, const uint64_t *a, const uint64_t *b,
+                                                         uint64_t *c, size_t limb_group, const uint32_t *bidx) {
+    constexpr int BUF = lds_words(kSlots);
+    __shared__ double s[2 * BUF];
+    const size_t p = blockIdx.x;
+    const int l = (int)((p / limb_group) % (size_t)r.nl);
+    const size_t pb = bidx ? (size_t)bidx[p / r.nl] * r.nl + p % r.nl : p;
+    const FMod m = r.t[l].m;
+    const double *twf = r.t[l].fwd, *twi = r.t[l].inv;
+    const int j = threadIdx.x;
+    const int op = j >> 7, u = j & 127;  // the forward passes' operand and unit
+    {
+        // forward stages 0..3: elements u + 128 t of the operand
+        const uint64_t *src = (op ? b + pb * kSlots : a + p * kSlots) + u;
+        double x[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) x[t] = small_u64_to_f64(src[128 * t]);
+        double w[15];
+        load_unit_tw<4, 0>(w, twf, 0);
+        unit_stages<4>(x, w, m);
+        double *q = s + op * BUF + lds_pad(u);
+#pragma unroll
+        for (int t = 0; t < 16; t++) q[unit_off<128>(t)] = x[t];
+    }
+    __syncthreads();
+    {
+        // forward stages 4..7
+        int base, SB;
+        UnitGeo<kSlotsLog, 4, 4>::at(u, base, SB);
+        double w[15];
+        load_unit_tw<4, 4>(w, twf, SB);
+        double *q = s + op * BUF + lds_pad(base);
+        double x[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) x[t] = q[unit_off<8>(t)];
+        unit_stages<4>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 16; t++) q[unit_off<8>(t)] = x[t];
+    }
+    __syncthreads();
+    double y[8];
+    {
+        // forward stages 8..10 of both operands on slots 8j .. 8j+7, the
+        // pointwise product, inverse stages 0..2 on the same slots
+        double w[7];
+        load_unit_tw<3, 8>(w, twf, j);
+        double xa[8], xb[8];
+        const double *qa = s + lds_pad(8 * j), *qb = s + BUF + lds_pad(8 * j);
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            xa[t] = qa[t];
+            xb[t] = qb[t];
+        }
+        unit_stages<3>(xa, w, m);
+        unit_stages<3>(xb, w, m);
+#pragma unroll
+        // lazy product of the two lazy transforms (|x| < 24q): |xa xb| / q stays
+        // below 2^47, so the fp64 quotient estimate is within one and y lies in
+        // [-q, 2q), the inverse transform's input range
+        for (int t = 0; t < 8; t++) y[t] = mod_mul_lazy(xa[t], xb[t], m);
+        double wi[7];
+        load_inv_unit_tw<kSlotsLog, 0, 3>(wi, twi, j);
+        inv_unit_stages<3>(y, wi, m);
+    }
+    __syncthreads();  // every thread has read both operands before the product overwrites operand a
+    {
+        double *q = s + lds_pad(8 * j);
+#pragma unroll
+        for (int t = 0; t < 8; t++) q[t] = y[t];
+    }
+    __syncthreads();
+    if (j < 128) {
+        // inverse stages 3..6 (one product, 128 units of 16)
+        int base, SB;
+        InvGeo<kSlotsLog, 3, 4>::at(j, base, SB);
+        double w[15];
+        load_inv_unit_tw<kSlotsLog, 3, 4>(w, twi, SB);
+        double *q = s + lds_pad(base);
+        double x[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) x[t] = q[unit_off<8>(t)];
+        inv_unit_stages<4>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 16; t++) q[unit_off<8>(t)] = x[t];
+    }
+    __syncthreads();
+    if (j < 128) {
+        // inverse stages 7..10 on coefficients j + 128 t, the N^-1 scaling, the store
+        double w[15];
+        load_inv_unit_tw<kSlotsLog, 7, 4>(w, twi, 0);
+        const double *q = s + lds_pad(j);
+        double x[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) x[t] = q[unit_off<128>(t)];
+        inv_unit_stages<4>(x, w, m);
+        const double ni = r.t[l].n_inv;
+        uint64_t *dst = c + p * kSlots + j;
+#pragma unroll
+        for (int t = 0; t < 16; t++) dst[128 * t] = small_f64_to_u64(mod_mul(x[t], ni, m));
+    }
+}
+
 void launch_polymul_u64(const RingDev &r, const uint64_t *a, const uint64_t *b, uint64_t *c, size_t count,
                         size_t limb_group, hipStream_t st, const uint32_t *bidx) {
-    if (count) hipLaunchKernelGGL(k_polymul, dim3(count), dim3(kThreads), 0, st, r, a, b, c, limb_group, bidx);
+    if (!count) return;
+    if (r.n == kSlots)
+        hipLaunchKernelGGL(k_polymul_fx, dim3(count), dim3(kThreads), 0, st, r, a, b, c, limb_group, bidx);
+    else
+        hipLaunchKernelGGL(k_polymul, dim3(count), dim3(kThreads), 0, st, r, a, b, c, limb_group, bidx);
 }
 
 }  // namespace via
