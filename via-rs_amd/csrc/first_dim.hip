@@ -19,16 +19,31 @@
 // The MFMA B operand's lane 16 g + c supplies column c for the same rows, so
 // a fragment pair multiplies exactly the rows it should whatever order the
 // hardware gives the bytes inside a lane group.
-// Output of a workgroup (slot k, segment of kFdSegRows rows):
-// out_k[seg][k][q][part][j], coalesced; k_fd_gather adds the segments and
-// transposes to [q][j][part][k] for the inverse transform.
+// Output of a workgroup (slot k, segment of kFdSegRows rows): rows of J
+// contiguous words, out_k[seg][k / 16][q][part][k % 16][j] (fd_out_at): the
+// 16 q-part rows a workgroup writes lie within 16 * 2Q * J words, and the
+// inverse transform reads each output poly in 128 runs of 16 rows at a
+// stride of J words. k_fd_intt adds the segments, transposes to
+// [q][j][part][k] and transforms back.
 #include "internal.h"
 #include "fd_limbs.h"
-#include "ntt_device.h"
+#include "ntt_fixed.h"
 
 namespace via {
 
 typedef int fd_v4i __attribute__((ext_vector_type(4)));
+
+// Word of FirstDim output (segment seg, slot k, column qp = 2q + part, row
+// j) in the layout the MFMA pass writes: blocks of kFdKlo slots, then qp,
+// then the slot within the block, then j (n a multiple of kFdKlo). The
+// contributions of the arguments add: fd_out_at(p, s, k, qp, j) =
+// fd_out_at(p, s, k, 0, 0) + fd_out_at(p, 0, 0, qp, j).
+constexpr int kFdKlo = 16;
+__host__ __device__ __forceinline__ size_t fd_out_at(const FdPlan &p, int seg, int k, int qp, int j) {
+    return (((size_t)seg * (p.n / kFdKlo) + (size_t)(k / kFdKlo)) * p.QP + (size_t)qp) * kFdKlo * p.J +
+           (size_t)(k % kFdKlo) * p.J + (size_t)j;
+}
+
 
 template <int KC>
 struct FdOp;
@@ -298,7 +313,7 @@ __global__ __launch_bounds__(256, 2) void k_fd_mfma(const uint8_t *__restrict__ 
     const size_t mbstride = (size_t)p.KCN * kLimbs * FRAG;
     const uint8_t *dk = dbl + (size_t)k * p.MB * mbstride + (size_t)kc0 * kLimbs * FRAG + (size_t)lane * KB;
     const size_t P = (size_t)p.Q * 2 * p.J;
-    double *ok = out + ((size_t)seg * p.n + k) * P;
+    double *ok = out + fd_out_at(p, seg, k, 0, 0);
     // the wave's steps: (column block mb = w + 4 u, row chunk t) for u, t in order
     const int nmb = mb1 - mb0 - w > 0 ? (mb1 - mb0 - w + MBW - 1) / MBW : 0;
     const int steps = nmb > 0 ? nmb * nkc : 0;
@@ -341,7 +356,7 @@ __global__ __launch_bounds__(256, 2) void k_fd_mfma(const uint8_t *__restrict__ 
                 for (int sh = 0; sh < kLimbShifts; sh++) C[sh] = acc[nb][sh][r];
                 v[r] = limb_fold(C, m);
             }
-            double *o = ok + (size_t)qp * p.J + j0;  // [q][part][j] with qp = 2q + part
+            double *o = ok + (size_t)qp * kFdKlo * p.J + j0;  // qp = 2q + part
             if (j0 + 4 <= p.J) {
                 typedef double dbl2 __attribute__((ext_vector_type(2)));
                 *(dbl2 *)o = dbl2{v[0], v[1]};
@@ -395,38 +410,6 @@ void launch_fd_mfma(const uint8_t *dbl, const uint8_t *sell, double *out_k, cons
     else fd_mfma_kc<32>(dbl, sell, out_k, p, m, st);
 }
 
-// out_k[seg][k][q][part][j] -> out[q][j][part][k], the segments added mod q;
-// 64 x 64 tiles through LDS so both sides move whole lines.
-__global__ __launch_bounds__(256) void k_fd_gather(const double *ok, double *out, FdPlan p, FMod m) {
-    __shared__ double t[64][65];
-    const size_t P = (size_t)p.Q * 2 * p.J;
-    const int k0 = blockIdx.x * 64;
-    const size_t e0 = (size_t)blockIdx.y * 64;
-    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-        const int kk = e >> 6, ee = e & 63;
-        const size_t idx = e0 + ee;
-        double v = 0.0;
-        if (idx < P && k0 + kk < p.n)
-            for (int sg = 0; sg < p.nseg; sg++) v += ok[((size_t)sg * p.n + k0 + kk) * P + idx];
-        t[kk][ee] = v;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-        const int ee = e >> 6, kk = e & 63;
-        const size_t idx = e0 + ee;
-        if (idx >= P || k0 + kk >= p.n) continue;
-        const size_t qp = idx / p.J, j = idx % p.J;
-        const size_t q = qp >> 1, part = qp & 1;
-        out[((q * p.J + j) * 2 + part) * p.n + k0 + kk] = mod_reduce_small(t[kk][ee], m);
-    }
-}
-
-void launch_fd_gather(const double *out_k, double *out_eval, const FdPlan &p, FMod m, hipStream_t st) {
-    const size_t P = (size_t)p.Q * 2 * p.J;
-    const dim3 grid((p.n + 63) / 64, (unsigned)((P + 63) / 64));
-    hipLaunchKernelGGL(k_fd_gather, grid, dim3(256), 0, st, out_k, out_eval, p, m);
-}
-
 // The segment sum, the transpose and the inverse transform of the FirstDim
 // outputs in one pass (first_dim.rs:174-181 inverts each output ciphertext):
 // workgroup b takes IPP consecutive output polys idx = qp * J + j (qp = 2q +
@@ -454,23 +437,28 @@ __global__ __launch_bounds__(TPB) void k_fd_intt(const double *__restrict__ ok, 
         // one segment, a whole tile: every load of the thread issued before
         // the first store (independent loads in flight)
         constexpr int PT = IPP * kSlots / TPB;
+        // element e: slot e / IPP of poly idx0 + e % IPP (the same poly for every e of a thread)
+        const size_t idx = idx0 + threadIdx.x % IPP;
+        const double *src = ok + fd_out_at(p, 0, 0, (int)(idx / p.J), (int)(idx % p.J));
         double v[PT];
 #pragma unroll
         for (int x = 0; x < PT; x++) {
             const int e = threadIdx.x + x * TPB;
-            v[x] = ok[(size_t)(e / IPP) * P + idx0 + e % IPP];
+            v[x] = src[fd_out_at(p, 0, e / IPP, 0, 0)];
         }
 #pragma unroll
         for (int x = 0; x < PT; x++) {
             const int e = threadIdx.x + x * TPB;
-            s[lds_pad((e % IPP) * n + e / IPP)] = mod_reduce_small(v[x], m);
+            s[lds_pad((e % IPP) * n + e / IPP)] = v[x];
         }
     } else {
         for (int e = threadIdx.x; e < IPP * n; e += TPB) {
             const int pp = e % IPP, k = e / IPP;
             double v = 0.0;
-            if (pp < np)
-                for (int sg = 0; sg < p.nseg; sg++) v += ok[((size_t)sg * n + k) * P + idx0 + pp];
+            if (pp < np) {
+                const size_t idx = idx0 + pp;
+                for (int sg = 0; sg < p.nseg; sg++) v += ok[fd_out_at(p, sg, k, (int)(idx / p.J), (int)(idx % p.J))];
+            }
             s[lds_pad(pp * n + k)] = mod_reduce_small(v, m);
         }
     }
@@ -484,6 +472,96 @@ __global__ __launch_bounds__(TPB) void k_fd_intt(const double *__restrict__ ok, 
     }
 }
 
+// The same for one row segment and n = 2048 with fixed-shape passes: four
+// polys per workgroup, 128 threads each. The slot-major outputs are staged
+// into LDS transposed as in k_fd_intt (consecutive lanes on consecutive words
+// of a slot row, every load in flight before the first store); inverse
+// stages 0..3 and 4..7 run over LDS with compile-time geometry; thread u of
+// poly pp then holds coefficients u + 256 t and u + 128 + 256 t (two units of
+// stages 8..10) in registers, scales them by N^-1 and stores them. Row groups
+// of 128 polys per XCD. The polys' LDS buffers are two words more than a
+// buffer apart, so the four lanes that stage one slot of the four polys hit
+// different banks.
+__global__ __launch_bounds__(512) void k_fd_intt_fx(const double *__restrict__ ok, double *__restrict__ out, FdPlan p,
+                                                    RingDev r2) {
+    constexpr int IPP = 4, LPW = 128 / IPP, BUF = lds_words(kSlots) + 2, TPB = 512;
+    __shared__ double s[IPP * BUF];
+    const size_t P = (size_t)p.Q * 2 * p.J;
+    const size_t b = blockIdx.x, grp = b / (8 * LPW), wi = b % (8 * LPW);
+    const size_t tile = grp * 8 * LPW + (wi & 7) * LPW + (wi >> 3);
+    const size_t idx0 = tile * IPP;
+    if (idx0 >= P) return;  // the whole workgroup, before any barrier
+    const FMod m = r2.t[0].m;
+    const double *twi = r2.t[0].inv;
+    {
+        constexpr int PT = IPP * kSlots / TPB;
+        const size_t idx = idx0 + threadIdx.x % IPP;
+        const double *src = ok + fd_out_at(p, 0, 0, (int)(idx / p.J), (int)(idx % p.J));
+        double v[PT];
+#pragma unroll
+        for (int x = 0; x < PT; x++) {
+            const int e = threadIdx.x + x * TPB;
+            v[x] = src[fd_out_at(p, 0, e / IPP, 0, 0)];
+        }
+#pragma unroll
+        for (int x = 0; x < PT; x++) {
+            const int e = threadIdx.x + x * TPB;
+            s[(e % IPP) * BUF + lds_pad(e / IPP)] = v[x];  // canonical already
+        }
+    }
+    __syncthreads();
+    const int pp = threadIdx.x >> 7, u = threadIdx.x & 127;
+    double *sp = s + pp * BUF;
+    {
+        // inverse stages 0..3 on slots 16u .. 16u+15 (block SB = u)
+        double w[15];
+        load_inv_unit_tw<kSlotsLog, 0, 4>(w, twi, u);
+        double *q = sp + lds_pad(16 * u);
+        double x[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) x[t] = q[t];
+        inv_unit_stages<4>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 16; t++) q[t] = x[t];
+    }
+    __syncthreads();
+    {
+        // inverse stages 4..7
+        int base, SB;
+        InvGeo<kSlotsLog, 4, 4>::at(u, base, SB);
+        double w[15];
+        load_inv_unit_tw<kSlotsLog, 4, 4>(w, twi, SB);
+        double *q = sp + lds_pad(base);
+        double x[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) x[t] = q[unit_off<16>(t)];
+        inv_unit_stages<4>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 16; t++) q[unit_off<16>(t)] = x[t];
+    }
+    __syncthreads();
+    {
+        // inverse stages 8..10 on coefficients c + 256 t for c = u and u + 128,
+        // the N^-1 scaling, the store to [q][j][part][k]
+        double w[7];
+        load_inv_unit_tw<kSlotsLog, 8, 3>(w, twi, 0);
+        const double ni = r2.t[0].n_inv;
+        const size_t idx = idx0 + pp, qp = idx / p.J, j = idx % p.J;
+        double *o = out + (((qp >> 1) * p.J + j) * 2 + (qp & 1)) * kSlots;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int c = u + 128 * h;
+            const double *q = sp + lds_pad(c);
+            double x[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) x[t] = q[unit_off<256>(t)];
+            inv_unit_stages<3>(x, w, m);
+#pragma unroll
+            for (int t = 0; t < 8; t++) o[c + 256 * t] = mod_mul(x[t], ni, m);
+        }
+    }
+}
+
 void launch_fd_intt(const double *out_k, double *out, const FdPlan &p, const RingDev &r2, hipStream_t st) {
     const size_t P = (size_t)p.Q * 2 * p.J;
     // four polys per workgroup when that still gives a thousand workgroups,
@@ -493,7 +571,9 @@ void launch_fd_intt(const double *out_k, double *out, const FdPlan &p, const Rin
         const size_t g = (size_t)8 * rowp / ipp;
         return ((P + ipp - 1) / ipp + g - 1) / g * g;
     };
-    if (P >= 4 * 1024)
+    if (P >= 4 * 1024 && p.nseg == 1 && p.n == kSlots && P % 4 == 0)
+        hipLaunchKernelGGL(k_fd_intt_fx, dim3((unsigned)blocks(4, 128)), dim3(512), 0, st, out_k, out, p, r2);
+    else if (P >= 4 * 1024)
         hipLaunchKernelGGL((k_fd_intt<4, 512, 64>), dim3((unsigned)blocks(4, 64)), dim3(512), 0, st, out_k, out, p,
                            r2);
     else if (P >= 2 * 512)

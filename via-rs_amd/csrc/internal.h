@@ -248,7 +248,6 @@ void launch_pack_db(const double *cells, uint8_t *dbl, const FdPlan &p, hipStrea
 void launch_unpack_db(const uint8_t *dbl, uint64_t *out, const FdPlan &p, hipStream_t s);
 void launch_sel_pack(const double *sel_eval, uint8_t *sell, const FdPlan &p, hipStream_t s);
 void launch_fd_mfma(const uint8_t *dbl, const uint8_t *sell, double *out_k, const FdPlan &p, FMod m, hipStream_t s);
-void launch_fd_gather(const double *out_k, double *out_eval, const FdPlan &p, FMod m, hipStream_t s);
 // The segment sum, transpose and inverse transform (r2: q2 at n) of the
 // FirstDim outputs in one launch: out [q][j][part][n] in coefficient form.
 void launch_fd_intt(const double *out_k, double *out, const FdPlan &p, const RingDev &r2, hipStream_t s);
