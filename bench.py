@@ -255,6 +255,13 @@ def serving_measure(via_gpu, ctx, pp, db, args, clients=64, slots=2, max_batch=3
     t_setup = time.perf_counter() - t0
     r = serve_run(loop, queries, bqueries, T, args.serve_qps, args.serve_requests, frac_b=0.3)
     st = loop.stats()
+    rc = None
+    if args.serve_c_qps > 0:
+        # the same loop and clients under VIA-C queries only, at a rate near
+        # the loop's saturation: continuous batching against the one-query path
+        rc = serve_run(loop, queries, None, T, args.serve_c_qps, args.serve_c_requests, frac_b=0.0)
+        rc.update({"workload": f"serving loop, Poisson arrivals of VIA-C queries from {clients} clients, "
+                               f"batches <= {max_batch}, {slots} slots, I={pp.num_rows} J={pp.num_cols}"})
     loop.close()
     db_b.close()
     r.update({"workload": f"serving loop, Poisson arrivals from {clients} clients (keys resident in HBM), "
@@ -263,6 +270,8 @@ def serving_measure(via_gpu, ctx, pp, db, args, clients=64, slots=2, max_batch=3
               "clients": clients, "key_bytes": st["key_bytes"], "setup_s": round(t_setup, 2),
               "data": "seeded: client c's keys Client::setup of b'via-gpu/serve/setup/<c>' and its queries "
                       "Client::query, indices uniform_below of b'via-gpu/serve/query', arrivals from numpy seed 7"})
+    if rc is not None:
+        r["via_c_saturation"] = rc
     return r
 
 
@@ -664,6 +673,9 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=80.0,
                     help="default line: offered request rate of the mixed serving trace (0 to skip)")
     ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-c-qps", type=float, default=5500.0,
+                    help="default line: VIA-C-only arrival rate for the serving loop's saturation point (0: skip)")
+    ap.add_argument("--serve-c-requests", type=int, default=3000)
     ap.add_argument("--keygen-clients", type=int, default=32,
                     help="default line: Client::setup of this many clients at once (0 to skip)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -914,6 +926,9 @@ def main():
                         for T in args.vb_sweep.split(",")]
     if Q == 1 and args.serve_qps > 0:
         out["serving"] = serving_measure(via_gpu, ctx, pp, db, args)
+        sat = out["serving"].get("via_c_saturation")
+        if sat:
+            sat["vs_one_query"] = round(sat["answers_per_s"] / out["value"], 2)
     if Q == 1 and args.keygen_clients > 0:
         out["client_setup"] = setup_many_measure(via_gpu, ctx, pp, clients=args.keygen_clients)
     if not args.no_cpu:

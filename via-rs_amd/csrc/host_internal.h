@@ -226,6 +226,8 @@ constexpr uint32_t kServeMaxT = 64;
 // True when every word of n_lwe LWEs ([lwe][limb][n1 + 1]) is a canonical
 // residue of its q1 limb.
 bool lwes_canonical(const via_params *pp, const uint64_t *lwes, size_t n_lwe);
+// The same check fused with a copy of the words to dst (one pass).
+bool lwes_copy_canonical(const via_params *pp, uint64_t *dst, const uint64_t *lwes, size_t n_lwe);
 // Holds the context, or a lane of it when the context is busy (created on
 // demand, up to seven), for a host-buffer call on the calling thread;
 // lane_release (same thread) lets it go.

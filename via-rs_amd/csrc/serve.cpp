@@ -37,6 +37,7 @@ struct Request {
     uint64_t ticket;
     uint64_t client;
     uint32_t T = 0;  // 0: VIA-C query; else a VIA-B batch of T queries
+    int pool = -1;   // VIA-C query: its entry of the pinned query pool, or -1 (then in lwes)
     std::vector<uint64_t> lwes;
     Clock::time_point t_submit;
 };
@@ -54,7 +55,8 @@ struct Slot {
     uint64_t *h_a = nullptr;   // pinned answers, max_batch x 2 x n2
     uint64_t *d_q = nullptr;   // device queries
     uint64_t *d_a = nullptr;   // device answers
-    hipEvent_t done = nullptr;
+    hipEvent_t done = nullptr;      // the batch's answers are on the host
+    hipEvent_t computed = nullptr;  // its kernels have run (the answers' copy may not have)
     std::vector<Request> batch;
     bool busy = false;
 };
@@ -72,7 +74,17 @@ struct via_server {
     size_t n_lwe = 0, lwe_words = 0;      // one VIA-C query
     size_t n_lwe_b = 0, lwe_words_b = 0;  // one query of a VIA-B batch
     size_t q_cap = 0;                     // staging words per slot
+    // Pinned pool of VIA-C queries: submit copies (and checks) a query into a
+    // free entry in one pass, and a batch's queries go to the device by DMA
+    // from there, without a copy on the worker thread. A submit that finds
+    // the pool empty keeps its query in the request (copied at launch).
+    uint64_t *pool = nullptr;
+    std::vector<int> pool_free;
     std::vector<Slot> slots;
+    // The slot whose batch was launched last: the next batch's kernels wait
+    // for its kernels (its query copy does not), so batches run back to back
+    // on the device instead of contending for it, while copies overlap.
+    Slot *last = nullptr;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request> pending;
@@ -89,6 +101,15 @@ struct via_server {
 
 namespace {
 
+// Orders slot sl's kernels after those of the batch launched before it.
+via_status chain_after_last(via_server *s, Slot &sl) {
+    if (s->last && s->last != &sl && s->last->busy &&
+        hipStreamWaitEvent(sl.ctx->stream, s->last->computed, 0) != hipSuccess)
+        return VIA_E_DEVICE;
+    s->last = &sl;
+    return VIA_OK;
+}
+
 // Launch one batch on a free slot (caller holds no lock; the slot is owned by
 // the worker thread).
 via_status launch_vb(via_server *s, Slot &sl) {
@@ -103,9 +124,11 @@ via_status launch_vb(via_server *s, Slot &sl) {
     hipStream_t stream = sl.ctx->stream;
     if (hipMemcpyAsync(sl.d_q, sl.h_q, sizeof(uint64_t) * words, hipMemcpyHostToDevice, stream) != hipSuccess)
         return VIA_E_DEVICE;
-    via_status st = answer_vb_impl(sl.ctx, s->db_b, k, (int)r.T, sl.d_q, s->n_lwe_b, sl.d_a, sl.d_a + s->pp.n2,
-                                   nullptr);
+    via_status st = chain_after_last(s, sl);
     if (st) return st;
+    st = answer_vb_impl(sl.ctx, s->db_b, k, (int)r.T, sl.d_q, s->n_lwe_b, sl.d_a, sl.d_a + s->pp.n2, nullptr);
+    if (st) return st;
+    if (hipEventRecord(sl.computed, stream) != hipSuccess) return VIA_E_DEVICE;
     if (hipMemcpyAsync(sl.h_a, sl.d_a, sizeof(uint64_t) * 2 * s->pp.n2, hipMemcpyDeviceToHost, stream) != hipSuccess)
         return VIA_E_DEVICE;
     if (hipEventRecord(sl.done, stream) != hipSuccess) return VIA_E_DEVICE;
@@ -120,23 +143,27 @@ via_status launch_batch(via_server *s, Slot &sl) {
     std::vector<const via_keys *> keys(Q);
     std::vector<const uint64_t *> dq(Q);
     std::vector<uint64_t *> dm(Q), dbd(Q);
+    hipStream_t stream = sl.ctx->stream;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        for (int q = 0; q < Q; q++) keys[q] = s->clients.at(sl.batch[q].client);
+    }
+    const size_t qbytes = sizeof(uint64_t) * s->lwe_words;
     for (int q = 0; q < Q; q++) {
         const Request &r = sl.batch[q];
-        {
-            std::lock_guard<std::mutex> g(s->mu);
-            keys[q] = s->clients.at(r.client);
-        }
-        memcpy(sl.h_q + (size_t)q * s->lwe_words, r.lwes.data(), sizeof(uint64_t) * s->lwe_words);
         dq[q] = sl.d_q + (size_t)q * s->lwe_words;
         dm[q] = sl.d_a + (size_t)q * 2 * s->pp.n2;
         dbd[q] = dm[q] + s->pp.n2;
+        const uint64_t *src = r.pool >= 0 ? s->pool + (size_t)r.pool * s->lwe_words : sl.h_q + (size_t)q * s->lwe_words;
+        if (r.pool < 0) memcpy(sl.h_q + (size_t)q * s->lwe_words, r.lwes.data(), qbytes);
+        if (hipMemcpyAsync(sl.d_q + (size_t)q * s->lwe_words, src, qbytes, hipMemcpyHostToDevice, stream) !=
+            hipSuccess)
+            return VIA_E_DEVICE;
     }
-    hipStream_t stream = sl.ctx->stream;
-    if (hipMemcpyAsync(sl.d_q, sl.h_q, sizeof(uint64_t) * s->lwe_words * Q, hipMemcpyHostToDevice, stream) !=
-        hipSuccess)
-        return VIA_E_DEVICE;
+    if ((st = chain_after_last(s, sl))) return st;
     st = answer_batch_impl(sl.ctx, s->db, Q, keys.data(), dq.data(), s->n_lwe, dm.data(), dbd.data(), nullptr);
     if (st) return st;
+    if (hipEventRecord(sl.computed, stream) != hipSuccess) return VIA_E_DEVICE;
     if (hipMemcpyAsync(sl.h_a, sl.d_a, sizeof(uint64_t) * 2 * s->pp.n2 * Q, hipMemcpyDeviceToHost, stream) !=
         hipSuccess)
         return VIA_E_DEVICE;
@@ -166,6 +193,7 @@ void finish_batch(via_server *s, Slot &sl, via_status st) {
         }
         s->records += r.T ? r.T : 1;
         s->client_inflight[r.client]--;
+        if (r.pool >= 0) s->pool_free.push_back(r.pool);  // its DMA preceded the completion event
     }
     s->answered += Q;
     s->batches++;
@@ -188,11 +216,21 @@ void worker_loop(via_server *s) {
         }
         // launch pending work on free slots
         bool launched = false;
+        bool other_busy = false;
+        for (Slot &sl : s->slots) other_busy |= sl.busy;
         for (Slot &sl : s->slots) {
             if (sl.busy) continue;
             {
                 std::unique_lock<std::mutex> lk(s->mu);
                 if (s->pending.empty()) break;
+                // while a batch runs, VIA-C queries gather until a full batch
+                // is pending (it then queues behind the running one); a VIA-B
+                // request, or an idle device, launches at once
+                if (other_busy && !s->pending.front().T) {
+                    size_t c = 0;
+                    for (const Request &r : s->pending) c += r.T ? 0 : 1;
+                    if (c < s->max_batch) break;
+                }
                 if (s->pending.front().T) {
                     // a VIA-B request runs on its own
                     sl.batch.push_back(std::move(s->pending.front()));
@@ -216,6 +254,7 @@ void worker_loop(via_server *s) {
                 finish_batch(s, sl, st);
             }
             launched = true;
+            other_busy = true;
         }
         bool any_busy = false;
         for (Slot &sl : s->slots) any_busy |= sl.busy;
@@ -263,6 +302,12 @@ via_status via_server_create(int device, const via_params *pp, const via_db *db,
         s->lwe_words_b = s->n_lwe_b * pp->q1_nl * (pp->n1 + 1);
         s->q_cap = std::max(s->q_cap, s->lwe_words_b * std::min<uint32_t>(t_b, kServeMaxT));
     }
+    {
+        const int entries = (int)(2 * slots * max_batch + max_batch);
+        if (hipHostMalloc((void **)&s->pool, sizeof(uint64_t) * s->lwe_words * entries) != hipSuccess)
+            return VIA_E_NOMEM;
+        for (int e = entries - 1; e >= 0; e--) s->pool_free.push_back(e);
+    }
     s->slots.resize(slots);
     for (Slot &sl : s->slots) {
         via_status st = via_ctx_create(device, &sl.ctx);
@@ -274,7 +319,8 @@ via_status via_server_create(int device, const via_params *pp, const via_db *db,
             hipHostMalloc((void **)&sl.h_a, sizeof(uint64_t) * 2 * pp->n2 * max_batch) != hipSuccess ||
             hipMalloc((void **)&sl.d_q, sizeof(uint64_t) * s->q_cap) != hipSuccess ||
             hipMalloc((void **)&sl.d_a, sizeof(uint64_t) * 2 * pp->n2 * max_batch) != hipSuccess ||
-            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.computed, hipEventDisableTiming) != hipSuccess)
             return VIA_E_NOMEM;
         st = ensure_workspace(sl.ctx, pp, max_batch, s->n_lwe);
         if (st) return st;
@@ -311,8 +357,10 @@ void via_server_destroy(via_server *s) {
         (void)hipFree(sl.d_q);
         (void)hipFree(sl.d_a);
         if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.computed) (void)hipEventDestroy(sl.computed);
         via_ctx_destroy(sl.ctx);
     }
+    (void)hipHostFree(s->pool);
     delete s;
 }
 
@@ -391,12 +439,37 @@ via_status via_server_submit(via_server *s, uint64_t client_id, const uint64_t *
                              uint64_t *ticket) {
     if (!s || !lwes || !ticket) return VIA_E_INVALID;
     if (n_lwe != s->n_lwe) return VIA_E_QUERY_LEN;
-    if (!lwes_canonical(&s->pp, lwes, n_lwe)) return VIA_E_INVALID;
     Request r;
     r.client = client_id;
-    r.lwes.assign(lwes, lwes + s->lwe_words);
     r.t_submit = Clock::now();
-    return enqueue(s, std::move(r), client_id, ticket);
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (!s->pool_free.empty()) {
+            r.pool = s->pool_free.back();
+            s->pool_free.pop_back();
+        }
+    }
+    bool ok;
+    if (r.pool >= 0) {
+        ok = lwes_copy_canonical(&s->pp, s->pool + (size_t)r.pool * s->lwe_words, lwes, n_lwe);
+    } else {
+        r.lwes.resize(s->lwe_words);
+        ok = lwes_copy_canonical(&s->pp, r.lwes.data(), lwes, n_lwe);
+    }
+    if (!ok) {
+        if (r.pool >= 0) {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->pool_free.push_back(r.pool);
+        }
+        return VIA_E_INVALID;
+    }
+    const int pool = r.pool;
+    via_status st = enqueue(s, std::move(r), client_id, ticket);
+    if (st && pool >= 0) {  // refused (client not admitted): the entry was not queued
+        std::lock_guard<std::mutex> g(s->mu);
+        s->pool_free.push_back(pool);
+    }
+    return st;
 }
 
 via_status via_server_submit_batch(via_server *s, uint64_t client_id, uint32_t T, const uint64_t *lwes,

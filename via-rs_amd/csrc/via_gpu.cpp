@@ -535,12 +535,27 @@ void via::lane_release(via_ctx *lane) { lane->call_mu.unlock(); }
 
 bool via::lwes_canonical(const via_params *pp, const uint64_t *lwes, size_t n_lwe) {
     const size_t n1p = (size_t)pp->n1 + 1;
+    uint64_t bad = 0;
     for (size_t i = 0; i < n_lwe * 2; i++) {
         const uint64_t q = pp->q1[i & 1], *x = lwes + i * n1p;
-        for (size_t c = 0; c < n1p; c++)
-            if (x[c] >= q) return false;
+        for (size_t c = 0; c < n1p; c++) bad |= (uint64_t)(x[c] >= q);  // branch-free: vectorises
     }
-    return true;
+    return bad == 0;
+}
+
+bool via::lwes_copy_canonical(const via_params *pp, uint64_t *dst, const uint64_t *lwes, size_t n_lwe) {
+    const size_t n1p = (size_t)pp->n1 + 1;
+    uint64_t bad = 0;
+    for (size_t i = 0; i < n_lwe * 2; i++) {
+        const uint64_t q = pp->q1[i & 1], *x = lwes + i * n1p;
+        uint64_t *d = dst + i * n1p;
+        for (size_t c = 0; c < n1p; c++) {
+            const uint64_t v = x[c];
+            d[c] = v;
+            bad |= (uint64_t)(v >= q);
+        }
+    }
+    return bad == 0;
 }
 
 size_t via::keys_device_bytes(const via_keys *k) {
