@@ -37,6 +37,81 @@ __global__ __launch_bounds__(kThreads) void k_ntt_batch(RingDev r, const double 
     }
 }
 
+// Forward transforms at n = 2048 with compile-time passes: two polys per
+// workgroup, 128 threads each (the halves may take different limbs). Thread
+// u reads elements u + 128 t of its poly straight into stages 0..3, runs
+// stages 4..7 over LDS, and holds slots 8 v .. 8 v + 7 for v = u, u + 128 in
+// the last pass, which it stores canonical. Polys come in groups as in
+// k_ntt_batch; src may equal dst (every poly is read before it is written).
+__global__ __launch_bounds__(kThreads) void k_ntt_fx(RingDev r, const double *src, double *dst, size_t npolys,
+                                                     size_t limb_group, size_t gpolys, size_t src_gstride,
+                                                     size_t dst_gstride) {
+    constexpr int BUF = lds_words(kSlots);
+    __shared__ double s[2 * BUF];
+    const int h = threadIdx.x >> 7, u = threadIdx.x & 127;
+    const size_t p = (size_t)blockIdx.x * 2 + h;
+    const bool live = p < npolys;  // the same for a whole wavefront
+    const size_t gq = live ? p / gpolys : 0, pr = live ? p - gq * gpolys : 0;
+    const int l = __builtin_amdgcn_readfirstlane((int)((pr / limb_group) % (size_t)r.nl));
+    const FMod m = r.t[l].m;
+    const double *twf = r.t[l].fwd;
+    double *sp = s + h * BUF;
+    if (live) {
+        const double *g = src + gq * src_gstride + pr * kSlots + u;
+        double x[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) x[t] = g[128 * t];
+        double w[15];
+        load_unit_tw<4, 0>(w, twf, 0);
+        unit_stages<4>(x, w, m);
+        double *q = sp + lds_pad(u);
+#pragma unroll
+        for (int t = 0; t < 16; t++) q[unit_off<128>(t)] = x[t];
+    }
+    __syncthreads();
+    if (live) {
+        int base, SB;
+        UnitGeo<kSlotsLog, 4, 4>::at(u, base, SB);
+        double w[15];
+        load_unit_tw<4, 4>(w, twf, SB);
+        double *q = sp + lds_pad(base);
+        double x[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) x[t] = q[unit_off<8>(t)];
+        unit_stages<4>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 16; t++) q[unit_off<8>(t)] = x[t];
+    }
+    __syncthreads();
+    if (live) {
+        double *g = dst + gq * dst_gstride + pr * kSlots;
+#pragma unroll
+        for (int v = 0; v < 2; v++) {
+            const int unit = u + 128 * v;
+            double w[7];
+            load_unit_tw<3, 8>(w, twf, unit);
+            const double *q = sp + lds_pad(8 * unit);
+            double x[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) x[t] = q[t];
+            unit_stages<3>(x, w, m);
+            double2 *o = reinterpret_cast<double2 *>(g + 8 * unit);
+#pragma unroll
+            for (int t = 0; t < 4; t++) o[t] = make_double2(mod_reduce_small(x[2 * t], m), mod_reduce_small(x[2 * t + 1], m));
+        }
+    }
+}
+
+static void launch_ntt_fwd(const RingDev &r, const double *src, double *dst, size_t npolys, size_t limb_group,
+                           size_t gpolys, size_t src_gstride, size_t dst_gstride, hipStream_t st) {
+    if (r.n == kSlots)
+        hipLaunchKernelGGL(k_ntt_fx, dim3((unsigned)((npolys + 1) / 2)), dim3(kThreads), 0, st, r, src, dst, npolys,
+                           limb_group, gpolys, src_gstride, dst_gstride);
+    else
+        hipLaunchKernelGGL(k_ntt_batch<false>, dim3((unsigned)((npolys * r.n + kSlots - 1) / kSlots)), dim3(kThreads), 0,
+                           st, r, src, dst, npolys, limb_group, gpolys, src_gstride, dst_gstride);
+}
+
 void launch_ntt(const RingDev &r, double *data, size_t npolys, size_t limb_group, bool inverse, hipStream_t st) {
     if (!npolys) return;
     const size_t P = kSlots / r.n;
@@ -45,23 +120,23 @@ void launch_ntt(const RingDev &r, double *data, size_t npolys, size_t limb_group
         hipLaunchKernelGGL(k_ntt_batch<true>, dim3(blocks), dim3(kThreads), 0, st, r, data, data, npolys, limb_group,
                            npolys, (size_t)0, (size_t)0);
     else
-        hipLaunchKernelGGL(k_ntt_batch<false>, dim3(blocks), dim3(kThreads), 0, st, r, data, data, npolys, limb_group,
-                           npolys, (size_t)0, (size_t)0);
+        launch_ntt_fwd(r, data, data, npolys, limb_group, npolys, 0, 0, st);
 }
 
 void launch_ntt_to(const RingDev &r, const double *src, double *dst, size_t npolys, size_t limb_group,
                    hipStream_t st) {
     if (!npolys) return;
-    const size_t P = kSlots / r.n;
-    const size_t blocks = (npolys + P - 1) / P;
-    hipLaunchKernelGGL(k_ntt_batch<false>, dim3(blocks), dim3(kThreads), 0, st, r, src, dst, npolys, limb_group,
-                       npolys, (size_t)0, (size_t)0);
+    launch_ntt_fwd(r, src, dst, npolys, limb_group, npolys, 0, 0, st);
 }
 
 void launch_ntt_groups(const RingDev &r, const double *src, double *dst, size_t gpolys, size_t groups,
                        size_t src_gstride, size_t dst_gstride, size_t limb_group, hipStream_t st) {
     const size_t npolys = gpolys * groups;
     if (!npolys) return;
+    if (r.n == kSlots) {  // every poly addressed by its own group
+        launch_ntt_fwd(r, src, dst, npolys, limb_group, gpolys, src_gstride, dst_gstride, st);
+        return;
+    }
     const size_t P = kSlots / r.n;
     if (gpolys % P) {  // a workgroup's polys must not straddle two groups: one launch per group
         for (size_t g = 0; g < groups; g++)
