@@ -31,28 +31,27 @@ namespace {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef double v2d __attribute__((ext_vector_type(2)));
 
-constexpr int kCmIn = 1024;    // transform inputs per limb and tile
-constexpr int kCmThr = 128;    // threads per limb
-constexpr int kCmBuf = lds_words(kCmIn);
 
 // Middle passes P, P+1, .. NP-2 over the LDS buffers of the IPW items (item
-// `it` at buf + it * kCmBuf): the thread's units w = j + TL v of the
-// items' 1024 >> R units each, twiddles read per unit, a barrier after each.
-template <int LOGNT, int RL, int IPW, int P>
+// `it` at buf + it * lds_words(2^LOGNIN)): the thread's units w = j + TL v of
+// the items' 2^LOGNIN >> R units each, twiddles read per unit, a barrier
+// after each.
+template <int LOGNT, int RL, int IPW, int LOGNIN, int P>
 __device__ __forceinline__ void cm_middle(double *buf, int j, const double *tw, const FMod &m) {
     constexpr int NP = cm_np(LOGNT, RL);
     if constexpr (P < NP - 1) {
         constexpr int R = cm_r(LOGNT, RL, P), L0 = cm_l0(LOGNT, RL, P), E = 1 << R, TL = 128 * IPW;
+        constexpr int EPT = (1 << LOGNIN) / 128, IB = lds_words(1 << LOGNIN);
         using G = UnitGeo<LOGNT, L0, R>;
 #pragma unroll
-        for (int v = 0; v < (8 >> R); v++) {
+        for (int v = 0; v < (EPT >> R); v++) {
             const int w = j + TL * v;
-            const int it = w >> (10 - R), u = w & ((1024 >> R) - 1);
+            const int it = w >> (LOGNIN - R), u = w & ((1 << (LOGNIN - R)) - 1);
             int base, SB;
             G::at(u, base, SB);
             double wt[E - 1];
             load_unit_tw<R, L0>(wt, tw, SB);
-            double *p = buf + it * kCmBuf + lds_pad(base);
+            double *p = buf + it * IB + lds_pad(base);
             double x[E];
 #pragma unroll
             for (int t = 0; t < E; t++) x[t] = p[unit_off<(1 << G::JB)>(t)];
@@ -61,7 +60,7 @@ __device__ __forceinline__ void cm_middle(double *buf, int j, const double *tw, 
             for (int t = 0; t < E; t++) p[unit_off<(1 << G::JB)>(t)] = x[t];
         }
         __syncthreads();
-        cm_middle<LOGNT, RL, IPW, P + 1>(buf, j, tw, m);
+        cm_middle<LOGNT, RL, IPW, LOGNIN, P + 1>(buf, j, tw, m);
     }
 }
 
@@ -79,24 +78,30 @@ __host__ __device__ int cascade_key_pos(int sg) {
 }
 
 // One RNS limb (blockIdx.y) of IPW items (LWEs sharing one key) per
-// workgroup, TL = 128 IPW threads: thread j ends each transform holding
-// outputs OPT j .. OPT j + OPT - 1 (OPT = 8 / IPW) of every item, i.e. slots
-// 2 OPT j .. 2 OPT j + 2 OPT - 1, and each key word it loads serves all IPW
-// items (the key stream per item falls with IPW; the last pass shrinks to
-// log2(OPT) stages). Passes before the last give every thread 8 elements:
-// units w = j + TL v of the items' units.
-template <int LOGNT, int IPW>
-__global__ __launch_bounds__(128 * IPW, IPW == 4 ? 4 : 3) void k_cascade_mac(RingDev r, Gadget g, MacArgs a) {
-    constexpr int TL = 128 * IPW, OPT = 8 / IPW, SPT = 2 * OPT;  // threads, outputs and slots per item
-    constexpr int RL = IPW == 1 ? 3 : IPW == 2 ? 2 : 1;          // stages of the last pass
+// workgroup, TL = 128 IPW threads. EMB: the cascade steps (embedded inputs,
+// 1024 per tile, slots 2p and 2p + 1 take transform output p); otherwise the
+// RLWE -> RGSW conversion (2048 inputs, one slot per output, no segment
+// sums). Thread j ends each transform holding outputs OPT j .. OPT j + OPT - 1
+// of every item, i.e. slots SPT j .. SPT j + SPT - 1 (SPT = 16 / IPW either
+// way), and each key word it loads serves all IPW items (the key stream per
+// item falls with IPW; the last pass shrinks to log2(OPT) stages). Passes
+// before the last give every thread EPT elements: units w = j + TL v of the
+// items' units.
+template <int LOGNT, int IPW, bool EMB>
+__global__ __launch_bounds__(128 * IPW, !EMB ? 2 : IPW == 4 ? 4 : 3) void k_cascade_mac(RingDev r, Gadget g, MacArgs a) {
+    constexpr int LOGNIN = EMB ? 10 : 11, NIN = 1 << LOGNIN;         // transform inputs per item and tile
+    constexpr int TL = 128 * IPW, EPT = NIN / 128;                   // threads; elements per thread and pass
+    constexpr int OPT = NIN / TL, SPT = EMB ? 2 * OPT : OPT;         // outputs and slots per item
+    constexpr int RL = OPT == 16 ? 4 : OPT == 8 ? 3 : OPT == 4 ? 2 : 1;  // stages of the last pass
     constexpr int NP = cm_np(LOGNT, RL);
     constexpr int S = kSlots;
+    constexpr int IB = lds_words(NIN);  // LDS words per item and buffer
     // key words loaded ahead of the transform where that fits the register
     // budget of the occupancy target without spills
-    constexpr bool KPF = NP <= 1 || (IPW == 2 && LOGNT < 10) || (IPW == 4 && LOGNT < 5);
+    constexpr bool KPF = EMB && (NP <= 1 || (IPW == 2 && LOGNT < 10) || (IPW == 4 && LOGNT < 5));
     // [buffer][item] transform buffers of the limb; at the end the body sums
     // (2048 padded words per item)
-    __shared__ double s[2 * IPW * kCmBuf];
+    __shared__ double s[2 * IPW * IB];
     const int j = threadIdx.x;
     const int l = blockIdx.y;
     const MacTask task = mac_task(blockIdx.x, a.items / IPW, a.items_per_key / IPW, a.groups, a.tiles);
@@ -106,7 +111,7 @@ __global__ __launch_bounds__(128 * IPW, IPW == 4 ? 4 : 3) void k_cascade_mac(Rin
     const double *dc = r.t[l].dfw;
     const int k0 = grp * g.L / a.groups, k1 = (grp + 1) * g.L / a.groups;
     const size_t dplane = (size_t)g.L * a.dig_coeffs;  // digit words per item
-    const int16_t *dig = a.dig + (size_t)item0 * dplane + (size_t)tile * kCmIn;
+    const int16_t *dig = a.dig + (size_t)item0 * dplane + (size_t)tile * NIN;
     const double2 *key = reinterpret_cast<const double2 *>(a.key[item0 / a.items_per_key] + (size_t)tile * S +
                                                            (size_t)l * a.key_slots);
     const size_t krow = a.key_slots;  // double2 stride between part rows (two limbs of key_slots words each)
@@ -117,7 +122,7 @@ __global__ __launch_bounds__(128 * IPW, IPW == 4 ? 4 : 3) void k_cascade_mac(Rin
         for (int e = 0; e < SPT; e++) am[it][e] = ab[it][e] = 0.0;
     // the digits of the thread's first-pass elements, one digit ahead
     constexpr int R1 = cm_r(LOGNT, RL, 0);
-    int dcur[8], dnxt[8];
+    int dcur[EPT], dnxt[EPT];
     auto load_digits = [&](int k, int *d) {
         const int16_t *dk = dig + (size_t)k * a.dig_coeffs;
         if constexpr (NP <= 1) {
@@ -131,9 +136,9 @@ __global__ __launch_bounds__(128 * IPW, IPW == 4 ? 4 : 3) void k_cascade_mac(Rin
         } else {
             using Gm = UnitGeo<LOGNT, 0, R1>;
 #pragma unroll
-            for (int v = 0; v < (8 >> R1); v++) {
+            for (int v = 0; v < (EPT >> R1); v++) {
                 const int w = j + TL * v;
-                const int it = w >> (10 - R1), u = w & ((1024 >> R1) - 1);
+                const int it = w >> (LOGNIN - R1), u = w & ((NIN >> R1) - 1);
                 int base, SB;
                 Gm::at(u, base, SB);
 #pragma unroll
@@ -149,10 +154,11 @@ __global__ __launch_bounds__(128 * IPW, IPW == 4 ? 4 : 3) void k_cascade_mac(Rin
         // at the multiply-accumulate
         const double2 *km = key + (size_t)ks * 2 * krow;
         const double2 *kb = km + krow;
-        double2 vm[OPT], vb[OPT];
+        constexpr int NPAIR = SPT / 2;
+        double2 vm[NPAIR], vb[NPAIR];
         auto load_keys = [&]() {
 #pragma unroll
-            for (int t = 0; t < OPT; t++) {
+            for (int t = 0; t < NPAIR; t++) {
                 const int pos = cascade_key_pos(SPT * j + 2 * t) >> 1;
                 vm[t] = km[pos];
                 vb[t] = kb[pos];
@@ -172,27 +178,27 @@ __global__ __launch_bounds__(128 * IPW, IPW == 4 ? 4 : 3) void k_cascade_mac(Rin
                 }
             }
         } else {
-            double *buf = s + (k & 1) * IPW * kCmBuf;
+            double *buf = s + (k & 1) * IPW * IB;
             {
                 // first pass: stages 0 .. R1-1 in direct form
                 using Gm = UnitGeo<LOGNT, 0, R1>;
 #pragma unroll
-                for (int v = 0; v < (8 >> R1); v++) {
+                for (int v = 0; v < (EPT >> R1); v++) {
                     const int w = j + TL * v;
-                    const int it = w >> (10 - R1), u = w & ((1024 >> R1) - 1);
+                    const int it = w >> (LOGNIN - R1), u = w & ((NIN >> R1) - 1);
                     int base, SB;
                     Gm::at(u, base, SB);
                     double y[1 << R1];
 #pragma unroll
                     for (int t = 0; t < (1 << R1); t++) y[t] = (double)dcur[(v << R1) + t];
                     direct_pass<R1>(y, dc);
-                    double *p = buf + it * kCmBuf + lds_pad(base);
+                    double *p = buf + it * IB + lds_pad(base);
 #pragma unroll
                     for (int t = 0; t < (1 << R1); t++) p[unit_off<(1 << Gm::JB)>(t)] = y[t];
                 }
             }
             __syncthreads();
-            cm_middle<LOGNT, RL, IPW, 1>(buf, j, tw, m);
+            cm_middle<LOGNT, RL, IPW, LOGNIN, 1>(buf, j, tw, m);
             // last pass: the thread's OPT contiguous outputs of every item, in registers
             {
                 constexpr int L0 = LOGNT - RL;
@@ -202,7 +208,7 @@ __global__ __launch_bounds__(128 * IPW, IPW == 4 ? 4 : 3) void k_cascade_mac(Rin
                 load_unit_tw<RL, L0>(w, tw, SB);
 #pragma unroll
                 for (int it = 0; it < IPW; it++) {
-                    const double *p = buf + it * kCmBuf + lds_pad(OPT * j);
+                    const double *p = buf + it * IB + lds_pad(OPT * j);
 #pragma unroll
                     for (int t = 0; t < OPT; t++) x[it][t] = p[t];
                     unit_stages<RL>(x[it], w, m);
@@ -212,30 +218,47 @@ __global__ __launch_bounds__(128 * IPW, IPW == 4 ? 4 : 3) void k_cascade_mac(Rin
         // multiply-accumulate: slot pair t of the thread is transform output t
         if constexpr (!KPF) load_keys();
 #pragma unroll
-        for (int t = 0; t < OPT; t++)
+        for (int t = 0; t < SPT / 2; t++)
 #pragma unroll
             for (int it = 0; it < IPW; it++) {
-                am[it][2 * t] += mod_mul_lazy(x[it][t], vm[t].x, m);
-                am[it][2 * t + 1] += mod_mul_lazy(x[it][t], vm[t].y, m);
-                ab[it][2 * t] += mod_mul_lazy(x[it][t], vb[t].x, m);
-                ab[it][2 * t + 1] += mod_mul_lazy(x[it][t], vb[t].y, m);
+                // slots 2t, 2t + 1: one transform output (embedded) or two
+                const double x0 = EMB ? x[it][t] : x[it][2 * t], x1 = EMB ? x[it][t] : x[it][2 * t + 1];
+                am[it][2 * t] += mod_mul_lazy(x0, vm[t].x, m);
+                am[it][2 * t + 1] += mod_mul_lazy(x1, vm[t].y, m);
+                ab[it][2 * t] += mod_mul_lazy(x0, vb[t].x, m);
+                ab[it][2 * t + 1] += mod_mul_lazy(x1, vb[t].y, m);
             }
 #pragma unroll
-        for (int e = 0; e < 8; e++) dcur[e] = dnxt[e];
+        for (int e = 0; e < EPT; e++) dcur[e] = dnxt[e];
+    }
+    if constexpr (!EMB) {
+        // the conversion's products, slot by slot
+#pragma unroll
+        for (int it = 0; it < IPW; it++) {
+            const size_t ot = ((size_t)grp * a.items + item0 + it) * a.tiles + tile;
+            double2 *om = reinterpret_cast<double2 *>(a.out_mask + (ot * 2 + l) * S + SPT * j);
+            double2 *ob = reinterpret_cast<double2 *>(a.out_body + (ot * 2 + l) * S + SPT * j);
+#pragma unroll
+            for (int t = 0; t < SPT / 2; t++) {
+                om[t] = make_double2(mod_reduce_small(am[it][2 * t], m), mod_reduce_small(am[it][2 * t + 1], m));
+                ob[t] = make_double2(mod_reduce_small(ab[it][2 * t], m), mod_reduce_small(ab[it][2 * t + 1], m));
+            }
+        }
+        return;
     }
     // Body: the sum over the tile's 2048 / n segments of degree n = 2 nt
     // (exact fp64 sums of canonical residues, below 2^49), reduced once.
     constexpr int N = 2 << LOGNT;
     constexpr int G = N >= SPT ? S / N : TL;  // terms per output
     constexpr int BW = lds_words(S);         // body words per item
-    static_assert(IPW * BW <= 2 * IPW * kCmBuf, "body sums fit the transform buffers");
+    static_assert(!EMB || IPW * BW <= 2 * IPW * IB, "body sums fit the transform buffers");
     __syncthreads();  // the transform buffers become the body buffers
 #pragma unroll
     for (int it = 0; it < IPW; it++) {
         const size_t ot = ((size_t)grp * a.items + item0 + it) * a.tiles + tile;
         double2 *om = reinterpret_cast<double2 *>(a.out_mask + (ot * 2 + l) * S + SPT * j);
 #pragma unroll
-        for (int t = 0; t < OPT; t++)
+        for (int t = 0; t < SPT / 2; t++)
             om[t] = make_double2(mod_reduce_small(am[it][2 * t], m), mod_reduce_small(am[it][2 * t + 1], m));
         double *sb = s + it * BW;
         if constexpr (N >= SPT) {
@@ -271,11 +294,18 @@ __global__ __launch_bounds__(128 * IPW, IPW == 4 ? 4 : 3) void k_cascade_mac(Rin
 }
 
 bool cascade_mac_fits(const RingDev &r, const Gadget &g, const MacArgs &a) {
-    // n1 = 2048 with two limbs, embedded inputs, two tiles of 2048 slots,
-    // degrees up to 2048, and digits small enough for the exact direct pass:
-    // (B/2) (1 + 7 (q - 1)) + 14 q below 2^45 for the lazy products after it
-    if (r.nl != 2 || !a.embed || a.tiles != 2 || a.dig_coeffs != (size_t)kSlots || !a.reduce_body) return false;
-    if (r.logn < 1 || r.logn > kSlotsLog || a.key_slots != (size_t)2 * kSlots) return false;
+    // n1 = 2048 with two limbs: the cascade steps (embedded inputs, two tiles
+    // of 2048 slots, degrees up to 2048, body segment sums) or the RLWE ->
+    // RGSW conversion (degree 2048, one tile, products slot by slot); digits
+    // small enough for the exact direct pass: (B/2) (1 + 7 (q - 1)) + 14 q
+    // below 2^45 for the lazy products after it
+    if (r.nl != 2 || a.dig_coeffs != (size_t)kSlots) return false;
+    if (a.embed) {
+        if (a.tiles != 2 || !a.reduce_body || r.logn < 1 || r.logn > kSlotsLog || a.key_slots != (size_t)2 * kSlots)
+            return false;
+    } else if (a.tiles != 1 || a.reduce_body || r.logn != kSlotsLog || a.key_slots != (size_t)kSlots) {
+        return false;
+    }
     const double qmax = r.t[0].m.q > r.t[1].m.q ? r.t[0].m.q : r.t[1].m.q;
     const double half_b = (double)(g.base / 2 + 1);
     return half_b * 8.0 * qmax + 14.0 * qmax < 35184372088832.0;  // 2^45
@@ -284,18 +314,22 @@ bool cascade_mac_fits(const RingDev &r, const Gadget &g, const MacArgs &a) {
 template <int IPW>
 static void launch_cm(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t st) {
     const dim3 grid(a.items / IPW * a.groups * a.tiles, 2), blk(128 * IPW);
+    if (!a.embed) {
+        hipLaunchKernelGGL((k_cascade_mac<11, IPW, false>), grid, blk, 0, st, r, g, a);
+        return;
+    }
     switch (r.logn - 1) {
-        case 0: hipLaunchKernelGGL((k_cascade_mac<0, IPW>), grid, blk, 0, st, r, g, a); break;
-        case 1: hipLaunchKernelGGL((k_cascade_mac<1, IPW>), grid, blk, 0, st, r, g, a); break;
-        case 2: hipLaunchKernelGGL((k_cascade_mac<2, IPW>), grid, blk, 0, st, r, g, a); break;
-        case 3: hipLaunchKernelGGL((k_cascade_mac<3, IPW>), grid, blk, 0, st, r, g, a); break;
-        case 4: hipLaunchKernelGGL((k_cascade_mac<4, IPW>), grid, blk, 0, st, r, g, a); break;
-        case 5: hipLaunchKernelGGL((k_cascade_mac<5, IPW>), grid, blk, 0, st, r, g, a); break;
-        case 6: hipLaunchKernelGGL((k_cascade_mac<6, IPW>), grid, blk, 0, st, r, g, a); break;
-        case 7: hipLaunchKernelGGL((k_cascade_mac<7, IPW>), grid, blk, 0, st, r, g, a); break;
-        case 8: hipLaunchKernelGGL((k_cascade_mac<8, IPW>), grid, blk, 0, st, r, g, a); break;
-        case 9: hipLaunchKernelGGL((k_cascade_mac<9, IPW>), grid, blk, 0, st, r, g, a); break;
-        case 10: hipLaunchKernelGGL((k_cascade_mac<10, IPW>), grid, blk, 0, st, r, g, a); break;
+        case 0: hipLaunchKernelGGL((k_cascade_mac<0, IPW, true>), grid, blk, 0, st, r, g, a); break;
+        case 1: hipLaunchKernelGGL((k_cascade_mac<1, IPW, true>), grid, blk, 0, st, r, g, a); break;
+        case 2: hipLaunchKernelGGL((k_cascade_mac<2, IPW, true>), grid, blk, 0, st, r, g, a); break;
+        case 3: hipLaunchKernelGGL((k_cascade_mac<3, IPW, true>), grid, blk, 0, st, r, g, a); break;
+        case 4: hipLaunchKernelGGL((k_cascade_mac<4, IPW, true>), grid, blk, 0, st, r, g, a); break;
+        case 5: hipLaunchKernelGGL((k_cascade_mac<5, IPW, true>), grid, blk, 0, st, r, g, a); break;
+        case 6: hipLaunchKernelGGL((k_cascade_mac<6, IPW, true>), grid, blk, 0, st, r, g, a); break;
+        case 7: hipLaunchKernelGGL((k_cascade_mac<7, IPW, true>), grid, blk, 0, st, r, g, a); break;
+        case 8: hipLaunchKernelGGL((k_cascade_mac<8, IPW, true>), grid, blk, 0, st, r, g, a); break;
+        case 9: hipLaunchKernelGGL((k_cascade_mac<9, IPW, true>), grid, blk, 0, st, r, g, a); break;
+        case 10: hipLaunchKernelGGL((k_cascade_mac<10, IPW, true>), grid, blk, 0, st, r, g, a); break;
         default: break;
     }
 }
@@ -310,22 +344,22 @@ void launch_cascade_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hip
         launch_cm<1>(r, g, a, st);
 }
 
-// Cascade key rows in the layout of k_cascade_mac: every (k, part, limb)
-// row of 2 * 2048 slots, tile by tile, slot sg of a tile moved to
+// Key rows in the layout of k_cascade_mac: every (k, part, limb) row of
+// tiles * 2048 slots, tile by tile, slot sg of a tile moved to
 // cascade_key_pos(sg).
-__global__ void k_cascade_key_permute(const double *src, double *dst, size_t rows) {
-    const size_t total = rows * 2 * kSlots;
+__global__ void k_cascade_key_permute(const double *src, double *dst, size_t rows, int tiles) {
+    const size_t total = rows * tiles * kSlots;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const size_t tile_base = t & ~(size_t)(kSlots - 1);
         dst[tile_base + cascade_key_pos((int)(t & (kSlots - 1)))] = src[t];
     }
 }
 
-void launch_cascade_key_permute(const double *src, double *dst, size_t rows, hipStream_t st) {
+void launch_cascade_key_permute(const double *src, double *dst, size_t rows, int tiles, hipStream_t st) {
     if (!rows) return;
-    const size_t total = rows * 2 * kSlots;
+    const size_t total = rows * tiles * kSlots;
     const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_cascade_key_permute, dim3(blocks), dim3(256), 0, st, src, dst, rows);
+    hipLaunchKernelGGL(k_cascade_key_permute, dim3(blocks), dim3(256), 0, st, src, dst, rows, tiles);
 }
 
 }  // namespace via

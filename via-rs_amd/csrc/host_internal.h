@@ -153,9 +153,9 @@ struct via_keys {
     via_ctx *ctx = nullptr;
     via_params pp;
     std::vector<double *> cascade;  // per step, [k][part][limb][rank_in*n_out]
-    // cascade rows in the slot order of k_cascade_mac (cascade_key_pos within
-    // each 2048-slot tile), set when its exactness bound holds for these
-    // parameters (cascade_fixed_ok); otherwise slot order
+    // cascade and conversion key rows in the slot order of k_cascade_mac
+    // (cascade_key_pos within each 2048-slot tile), set when its exactness
+    // bound holds for these parameters (cascade_fixed_ok); otherwise slot order
     bool cascade_perm = false;
     double *conv = nullptr;         // [k][part][limb][n1]
     double *rsk = nullptr;          // [j][k][part][n2]

@@ -131,14 +131,15 @@ struct MacArgs {
 };
 void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t s);
 // The cascade-step MAC in its fixed shape (cascade_mac.hip): two q1 limbs,
-// embedded inputs, two tiles of 2048 slots, keys in the cascade_key_pos
-// order. cascade_mac_fits says whether a launch meets those conditions and
-// the exactness bound of its direct first pass.
+// embedded inputs, two tiles of 2048 slots, or the RLWE -> RGSW conversion
+// (2048 inputs, one tile); keys in the cascade_key_pos order.
+// cascade_mac_fits says whether a launch meets those conditions and the
+// exactness bound of its direct first pass.
 bool cascade_mac_fits(const RingDev &r, const Gadget &g, const MacArgs &a);
 void launch_cascade_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipStream_t s);
 __host__ __device__ int cascade_key_pos(int sg);
-// rows of 2 x 2048 slots from slot order into the cascade_key_pos order
-void launch_cascade_key_permute(const double *src, double *dst, size_t rows, hipStream_t s);
+// rows of tiles x 2048 slots from slot order into the cascade_key_pos order
+void launch_cascade_key_permute(const double *src, double *dst, size_t rows, int tiles, hipStream_t s);
 
 // One cascade combine: masks_next = -iNTT(sum pm), body_next =
 // embed(body_prev) - iNTT(sum pb), then the digits of masks_next into

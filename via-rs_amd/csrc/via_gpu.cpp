@@ -1024,7 +1024,7 @@ via_status via_keys_create(via_ctx *c, const via_params *pp, const uint64_t *cas
         launch_ntt(r, d, (size_t)L * 2 * nl * rank_in, rank_in, false, c->stream);
         if (k->cascade_perm) {
             // rows of rank_in * nout = 2 n1 slots into the order of the fixed-shape MAC
-            launch_cascade_key_permute(d, tmp_perm, (size_t)L * 2 * nl, c->stream);
+            launch_cascade_key_permute(d, tmp_perm, (size_t)L * 2 * nl, 2, c->stream);
             if (hipMemcpyAsync(d, tmp_perm, sizeof(double) * words, hipMemcpyDeviceToDevice, c->stream) !=
                 hipSuccess)
                 return fail(VIA_E_DEVICE);
@@ -1058,6 +1058,13 @@ via_status via_keys_create(via_ctx *c, const via_params *pp, const uint64_t *cas
         RingDev r;
         if ((s = make_ring(c, n1, pp->q1, 2, &r))) return fail(s);
         launch_ntt(r, k->conv, (size_t)L * 2 * nl, 1, false, c->stream);
+        if (k->cascade_perm) {
+            // rows of n1 slots in the order of the fixed-shape conversion MAC
+            launch_cascade_key_permute(k->conv, tmp_perm, (size_t)L * 2 * nl, 1, c->stream);
+            if (hipMemcpyAsync(k->conv, tmp_perm, sizeof(double) * words, hipMemcpyDeviceToDevice, c->stream) !=
+                hipSuccess)
+                return fail(VIA_E_DEVICE);
+        }
         if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(VIA_E_DEVICE);
     }
     // ring-switch key: d RLev(L_RSK) at (n2, q3)
@@ -1326,11 +1333,17 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         a.radix16 = step_tune[kDecompSteps - 1].radix16;
         a.t512 = step_tune[kDecompSteps - 1].t512;
         a.lsplit = step_tune[kDecompSteps - 1].lsplit;
+        a.ipw = step_tune[kDecompSteps - 1].ipw;
         a.out_mask = w.pm;
         a.out_body = w.pb;
         a.reduce_body = 0;
         a.items = (int)items;
-        launch_gadget_mac(r1n, gck, a, st);
+        if (keys[0]->cascade_perm) {
+            if (!cascade_mac_fits(r1n, gck, a)) return VIA_E_INVALID;
+            launch_cascade_mac(r1n, gck, a, st);
+        } else {
+            launch_gadget_mac(r1n, gck, a, st);
+        }
         launch_r2g_combine(r1n, w.pm, w.pb, a.groups, w.masks[cur], w.body[cur], w.rgsw, (int)(Q * bits), (int)lq,
                            st);
         if (c->time_decomp) REC_EVENT(c->ev_dec[kDecompSteps], st);
