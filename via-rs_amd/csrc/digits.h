@@ -80,6 +80,11 @@ __device__ __forceinline__ void gate_input(int mode, const double *in, int j, in
         const double x = in[(size_t)j * 2 * pw + part * pw + (size_t)l * n + sg];
         base_v = x;
         ct_v = x;
+    } else if (mode == GATE_CMUX_EV) {
+        // column 2j holds c1 - c0 (coefficient form); c0 is added in eval form
+        const double x = in[(size_t)(2 * j) * 2 * pw + part * pw + (size_t)l * n + sg];
+        base_v = 0.0;
+        ct_v = x;
     } else if (mode == GATE_CMUX) {
         const double x0 = in[(size_t)(2 * j) * 2 * pw + part * pw + (size_t)l * n + sg];
         const double x1 = in[(size_t)(2 * j + 1) * 2 * pw + part * pw + (size_t)l * n + sg];

@@ -312,7 +312,6 @@ __global__ __launch_bounds__(256, 2) void k_fd_mfma(const uint8_t *__restrict__ 
     const int c = lane & 15, g = lane >> 4;
     const size_t mbstride = (size_t)p.KCN * kLimbs * FRAG;
     const uint8_t *dk = dbl + (size_t)k * p.MB * mbstride + (size_t)kc0 * kLimbs * FRAG + (size_t)lane * KB;
-    const size_t P = (size_t)p.Q * 2 * p.J;
     double *ok = out + fd_out_at(p, seg, k, 0, 0);
     // the wave's steps: (column block mb = w + 4 u, row chunk t) for u, t in order
     const int nmb = mb1 - mb0 - w > 0 ? (mb1 - mb0 - w + MBW - 1) / MBW : 0;
@@ -560,6 +559,134 @@ __global__ __launch_bounds__(512) void k_fd_intt_fx(const double *__restrict__ o
             for (int t = 0; t < 8; t++) o[c + 256 * t] = mod_mul(x[t], ni, m);
         }
     }
+}
+
+// The FirstDim outputs in the form the first CMux level takes from them
+// (cmux(b, c0, c1) = c0 + b (x) (c1 - c0), mux.rs:75-84, on columns
+// 2i, 2i + 1): one inverse transform per pair instead of two. Workgroup b
+// takes the polys of eight consecutive columns j (J a multiple of 8, so the
+// same q and part), i.e. four pairs: thread t reads slot t / 4 + 128 x of
+// pair t % 4 as one 16-byte (c0, c1) load (the workgroup reads 64-byte runs
+// of each slot row; row groups of 128 polys per XCD as in k_fd_intt_fx),
+// stores c0 in eval form to column 2i + 1 of out (16 consecutive slots of
+// each pair per wavefront store: 128-byte runs) and stages the difference
+// c1 - c0 in LDS. The four differences are transformed back as in
+// k_fd_intt_fx (128 threads each: stages 0..3 and 4..7 over LDS, 8..10 in
+// registers, N^-1) into column 2i. The gate adds c0 to its product before
+// its own inverse transform.
+__global__ __launch_bounds__(512) void k_fd_intt_pairs(const double *__restrict__ ok, double *__restrict__ out,
+                                                       FdPlan p, RingDev r2) {
+    constexpr int IPP = 8, NPR = IPP / 2, LPW = 128 / IPP, BUF = lds_words(kSlots) + 2, TPB = 512;
+    constexpr int PT = NPR * kSlots / TPB;  // slots per thread (one pair)
+    __shared__ double s[NPR * BUF];
+    const size_t P = (size_t)p.Q * 2 * p.J;
+    const size_t b = blockIdx.x, grp = b / (8 * LPW), wi = b % (8 * LPW);
+    const size_t tile = grp * 8 * LPW + (wi & 7) * LPW + (wi >> 3);
+    const size_t idx0 = tile * IPP;
+    if (idx0 >= P) return;  // the whole workgroup, before any barrier
+    const FMod m = r2.t[0].m;
+    const double *twi = r2.t[0].inv;
+    const size_t qp = idx0 / p.J, j0 = idx0 % p.J;
+    const size_t q = qp >> 1, part = qp & 1;
+    {
+        const int pr = threadIdx.x % NPR, k0 = threadIdx.x / NPR;
+        const double *src = ok + fd_out_at(p, 0, 0, (int)qp, (int)(j0 + 2 * pr));
+        double *oe = out + ((q * p.J + j0 + 2 * pr + 1) * 2 + part) * kSlots;
+        double2 v[PT];
+#pragma unroll
+        for (int x = 0; x < PT; x++)
+            v[x] = *reinterpret_cast<const double2 *>(src + fd_out_at(p, 0, k0 + 128 * x, 0, 0));
+#pragma unroll
+        for (int x = 0; x < PT; x++) {
+            oe[k0 + 128 * x] = v[x].x;
+            s[pr * BUF + lds_pad(k0 + 128 * x)] = mod_sub(v[x].y, v[x].x, m);
+        }
+    }
+    __syncthreads();
+    const int pp = threadIdx.x >> 7, u = threadIdx.x & 127;
+    double *sp = s + pp * BUF;
+    {
+        // inverse stages 0..3 on slots 16u .. 16u+15 (block SB = u)
+        double w[15];
+        load_inv_unit_tw<kSlotsLog, 0, 4>(w, twi, u);
+        double *qq = sp + lds_pad(16 * u);
+        double x[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) x[t] = qq[t];
+        inv_unit_stages<4>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 16; t++) qq[t] = x[t];
+    }
+    __syncthreads();
+    {
+        // inverse stages 4..7
+        int base, SB;
+        InvGeo<kSlotsLog, 4, 4>::at(u, base, SB);
+        double w[15];
+        load_inv_unit_tw<kSlotsLog, 4, 4>(w, twi, SB);
+        double *qq = sp + lds_pad(base);
+        double x[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) x[t] = qq[unit_off<16>(t)];
+        inv_unit_stages<4>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 16; t++) qq[unit_off<16>(t)] = x[t];
+    }
+    __syncthreads();
+    const size_t jp = j0 + 2 * pp;  // this thread's pair: columns jp, jp + 1
+    {
+        // inverse stages 8..10 on coefficients c + 256 t for c = u and u + 128,
+        // N^-1, the difference to column jp
+        double w[7];
+        load_inv_unit_tw<kSlotsLog, 8, 3>(w, twi, 0);
+        const double ni = r2.t[0].n_inv;
+        double *od = out + ((q * p.J + jp) * 2 + part) * kSlots;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int c = u + 128 * h;
+            const double *qq = sp + lds_pad(c);
+            double x[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) x[t] = qq[unit_off<256>(t)];
+            inv_unit_stages<3>(x, w, m);
+#pragma unroll
+            for (int t = 0; t < 8; t++) od[c + 256 * t] = mod_mul(x[t], ni, m);
+        }
+    }
+}
+
+bool fd_pairs_fits(const FdPlan &p) { return p.nseg == 1 && p.n == kSlots && p.J % 8 == 0; }
+
+void launch_fd_intt_pairs(const double *out_k, double *out, const FdPlan &p, const RingDev &r2, hipStream_t st) {
+    const size_t P = (size_t)p.Q * 2 * p.J;
+    const size_t g = (size_t)8 * 128 / 8;  // whole XCD row groups of 128 polys
+    const size_t blocks = ((P + 7) / 8 + g - 1) / g * g;
+    hipLaunchKernelGGL(k_fd_intt_pairs, dim3((unsigned)blocks), dim3(512), 0, st, out_k, out, p, r2);
+}
+
+// Coefficient form [q][j][part][n] of query 0's FirstDim outputs from the
+// pair form k_fd_intt_pairs writes (c0 = iNTT(column 2i + 1), c1 = c0 +
+// column 2i), for inspection: one workgroup per (pair, part).
+__global__ __launch_bounds__(256) void k_fd_unpair(const double *in, double *out, int J, RingDev r2) {
+    __shared__ double s[lds_words(kSlots)];
+    const int i = blockIdx.x >> 1, part = blockIdx.x & 1, n = kSlots;
+    const double *e0 = in + ((size_t)(2 * i + 1) * 2 + part) * n;
+    const double *d = in + ((size_t)(2 * i) * 2 + part) * n;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) s[lds_pad(k)] = e0[k];
+    __syncthreads();
+    lds_ntt_inv_noscale<1, 4>(s, n, r2);
+    __syncthreads();
+    const FMod m = r2.t[0].m;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const double c0 = mod_mul(s[lds_pad(k)], r2.t[0].n_inv, m);
+        out[((size_t)(2 * i) * 2 + part) * n + k] = c0;
+        out[((size_t)(2 * i + 1) * 2 + part) * n + k] = mod_add(c0, d[k], m);
+    }
+    (void)J;
+}
+
+void launch_fd_unpair(const double *in, double *out, int J, const RingDev &r2, hipStream_t st) {
+    hipLaunchKernelGGL(k_fd_unpair, dim3((unsigned)J), dim3(256), 0, st, in, out, J, r2);
 }
 
 void launch_fd_intt(const double *out_k, double *out, const FdPlan &p, const RingDev &r2, hipStream_t st) {

@@ -194,6 +194,19 @@ __global__ __launch_bounds__(256, 3) void k_gate_fx(RingDev r, Gadget g, GateArg
     {
         double w[7];
         load_inv_unit_tw<kSlotsLog, 0, 3>(w, twi, j);
+        if constexpr (MODE == GATE_CMUX_EV) {
+            // c0 of the pair (column 2jg + 1, eval form), slots 8j .. 8j+7
+            const double2 *e0m = reinterpret_cast<const double2 *>(in + (size_t)(2 * jg + 1) * 2 * pw + 8 * j);
+            const double2 *e0b = reinterpret_cast<const double2 *>(in + (size_t)(2 * jg + 1) * 2 * pw + pw + 8 * j);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const double2 vm = e0m[t], vb = e0b[t];
+                am[2 * t] += vm.x;
+                am[2 * t + 1] += vm.y;
+                ab[2 * t] += vb.x;
+                ab[2 * t + 1] += vb.y;
+            }
+        }
 #pragma unroll
         for (int e = 0; e < 8; e++) {
             am[e] = lazy_reduce(am[e], m);
@@ -230,9 +243,13 @@ __global__ __launch_bounds__(256, 3) void k_gate_fx(RingDev r, Gadget g, GateArg
             for (int t = 0; t < 8; t++) {
                 const int c = j + 256 * t;
                 const double pv = mod_mul(x[t], ni, m);
+                const size_t o = (size_t)part * pw + (size_t)l * S + c;
+                if constexpr (MODE == GATE_CMUX_EV) {
+                    out[(size_t)jg * 2 * pw + o] = pv;  // c0 entered before the transform
+                    continue;
+                }
                 double bv, cv;
                 gate_input<NL>(mode, in, jg, part, l, c, S, shift, m, bv, cv);
-                const size_t o = (size_t)part * pw + (size_t)l * S + c;
                 if (mode == GATE_DMUX) {
                     out[(size_t)(2 * jg) * 2 * pw + o] = mod_sub(cv, pv, m);
                     out[(size_t)(2 * jg + 1) * 2 * pw + o] = pv;
@@ -262,6 +279,9 @@ static void launch_gx(const dim3 &grid, const RingDev &r, const Gadget &g, const
     switch (a.mode) {
         case GATE_DMUX: hipLaunchKernelGGL((k_gate_fx<NL, DIRECT, GATE_DMUX>), grid, dim3(256), 0, st, r, g, a); break;
         case GATE_CMUX: hipLaunchKernelGGL((k_gate_fx<NL, DIRECT, GATE_CMUX>), grid, dim3(256), 0, st, r, g, a); break;
+        case GATE_CMUX_EV:
+            hipLaunchKernelGGL((k_gate_fx<NL, DIRECT, GATE_CMUX_EV>), grid, dim3(256), 0, st, r, g, a);
+            break;
         default: hipLaunchKernelGGL((k_gate_fx<NL, DIRECT, GATE_CROT>), grid, dim3(256), 0, st, r, g, a); break;
     }
 }

@@ -134,9 +134,13 @@ struct via_ctx {
     std::vector<GraphEntry> graph_cache;
     bool have_times = false;     // step events of the last answer are valid
     bool have_fd_time = false;   // the FirstDim events of the last database pass are valid
-    // pointers to the last answer's stage buffers
+    // pointers to the last answer's stage buffers; the FirstDim outputs
+    // (stage 4) may be in the pair form of launch_fd_intt_pairs, over the
+    // ring recorded beside them
     const double *stage_ptr[7] = {};
     size_t stage_words[7] = {};
+    bool stage4_pairs = false;
+    via::RingDev stage4_ring{};
 };
 
 struct via_db {

@@ -162,7 +162,9 @@ void launch_r2g_combine(const RingDev &r, const double *pm, const double *pb, in
 
 // Gate kernels over an eval-form RGSW (layout [half][i][part][limb][n]). Item
 // b of the launch is output j = b % items_per_query of query b / items_per_query.
-enum GateMode { GATE_DMUX = 0, GATE_CMUX = 1, GATE_CROT = 2 };
+// GATE_CMUX_EV: the first CMux level over FirstDim outputs in pair form
+// (launch_fd_intt_pairs), fixed-shape gates only (gate_takes_fx).
+enum GateMode { GATE_DMUX = 0, GATE_CMUX = 1, GATE_CROT = 2, GATE_CMUX_EV = 3 };
 struct GateArgs {
     int mode;
     const double *rgsw;  // query q's RGSW at rgsw + q * rgsw_qstride
@@ -178,6 +180,8 @@ struct GateArgs {
     int split = -1;      // two limbs, one workgroup per limb: -1 below 256 gates, 0 never, 1 always
 };
 void launch_gate(const RingDev &r, const Gadget &g, const GateArgs &a, hipStream_t s);
+// whether launch_gate runs a level of `blocks` gates on the fixed-shape gate
+bool gate_takes_fx(const RingDev &r, const Gadget &g, int blocks);
 // The fixed-shape gate (gate_fx.hip) for one-limb levels of at least
 // kGateFxMin gates: degree 2048, gadget depth 2; *direct when its first
 // forward pass may run in direct form.
@@ -252,6 +256,13 @@ void launch_fd_mfma(const uint8_t *dbl, const uint8_t *sell, double *out_k, cons
 // The segment sum, transpose and inverse transform (r2: q2 at n) of the
 // FirstDim outputs in one launch: out [q][j][part][n] in coefficient form.
 void launch_fd_intt(const double *out_k, double *out, const FdPlan &p, const RingDev &r2, hipStream_t s);
+// The same outputs in pair form for a first CMux level of GATE_CMUX_EV
+// gates: column 2i holds c1 - c0 in coefficient form, column 2i + 1 c0 in
+// eval form (fd_pairs_fits: one row segment, n = 2048, J a multiple of 8).
+bool fd_pairs_fits(const FdPlan &p);
+void launch_fd_intt_pairs(const double *out_k, double *out, const FdPlan &p, const RingDev &r2, hipStream_t s);
+// query 0's pair-form outputs (J columns) back to coefficient form
+void launch_fd_unpair(const double *in, double *out, int J, const RingDev &r2, hipStream_t s);
 // Rewrites record slots of whole cells in place (r2: q2 at n1): cell c =
 // cells[x] takes, for slot s < d, record slot_src[x*d + s] of recs (n_rec
 // coefficients each, reduced mod pmod) or keeps its own (-1).

@@ -805,21 +805,15 @@ bool launch_tree_persistent(const RingDev &r, const Gadget &g_in, const TreeArgs
     return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(1024), args, 0, st) == hipSuccess;
 }
 
+bool gate_takes_fx(const RingDev &r, const Gadget &g_in, int blocks) {
+    bool direct = false;
+    return r.nl == 1 && blocks >= kGateFxMin && gate_fx_fits(r, gate_gadget(r, g_in), &direct);
+}
+
 void launch_gate(const RingDev &r, const Gadget &g_in, const GateArgs &a_in, hipStream_t st) {
     GateArgs a = a_in;
     a.dp = digit_plan(g_in.base, g_in.L);
-    // the fp64 scale where it is exact (one limb) or decided (two limbs)
-    Gadget g = g_in;
-    g.f64 = 0;
-    if (exact_paths_only()) {
-        // the 128-bit scale for every coefficient
-    } else if (r.nl == 2 && g.bl < ((via_u128)1 << 40)) {
-        g.f64 = 2;
-        g.sf = scale_f64_make((double)(uint64_t)g.bl, r.rns.q1, u128_to_f64(r.rns.big_q));
-    } else if (r.nl == 1 && !g.wide && ((via_u128)(r.q_single / 2 + 1) * g.bl + r.q_single) < ((via_u128)1 << 53)) {
-        g.f64 = 1;
-        g.sf.bl = (double)(uint64_t)g.bl;
-    }
+    const Gadget g = gate_gadget(r, g_in);
     const int blocks = a.items_per_query * a.queries;
     bool direct = false;
     if (r.nl == 1 && blocks >= kGateFxMin && gate_fx_fits(r, g, &direct)) {

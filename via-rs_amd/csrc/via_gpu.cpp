@@ -1463,14 +1463,23 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     launch_fd_mfma(db->limbs, w.sell, w.fdk, fp, m2, st);
     REC_EVENT(c->ev_fd[1], st);
     if (!c->capturing) c->have_fd_time = true;
-    // segment sum, transpose to [q][j][part][k] and inverse transform, one pass
-    launch_fd_intt(w.fdk, w.fd, fp, r2, st);
+    // segment sum, transpose to [q][j][part][k] and inverse transform, one
+    // pass; in pair form when the first CMux level runs the fixed-shape gate
+    // level by level (one inverse transform per pair of columns)
+    const Gadget g2 = make_gadget(pp->b2, pp->l_query);
+    const bool pers_tree_c = pers_c && nc > 1 && nc <= kMaxTreeLevels;
+    const bool fd_pairs = nc >= 1 && !pers_tree_c && fd_pairs_fits(fp) && gate_takes_fx(r2, g2, (int)(Q * (J / 2)));
+    if (fd_pairs)
+        launch_fd_intt_pairs(w.fdk, w.fd, fp, r2, st);
+    else
+        launch_fd_intt(w.fdk, w.fd, fp, r2, st);
     c->stage_ptr[3] = w.fd;
     c->stage_words[3] = J * 2 * n;
+    c->stage4_pairs = fd_pairs;
+    c->stage4_ring = r2;
     REC_EVENT(c->ev[4], st);
 
     // ---- step 5: cmux_tree over mod-switched RGSWs (answer.rs:194-203) ----
-    const Gadget g2 = make_gadget(pp->b2, pp->l_query);
     const size_t rgw2 = 4 * lq * n;
     const size_t ctw2 = 2 * n;
     // the CMux and CRot bits of query q, mod-switched: w.rgsw2 + q * rq2
@@ -1483,7 +1492,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     const double *cm_in = w.fd;
     int cc = 0;
     int cmux_from = 0;
-    if (pers_c && nc > 1 && nc <= kMaxTreeLevels) {
+    if (pers_tree_c) {
         TreeArgs ta;
         memset(&ta, 0, sizeof ta);
         ta.gate.mode = GATE_CMUX;
@@ -1510,7 +1519,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     for (int b = cmux_from; b < nc; b++) {
         GateArgs ga;
         ga.split = gate_split;
-        ga.mode = GATE_CMUX;
+        ga.mode = b == 0 && fd_pairs ? GATE_CMUX_EV : GATE_CMUX;
         ga.rgsw = w.rgsw2 + (size_t)b * rgw2;
         ga.rgsw_qstride = rq2;
         ga.in = cm_in;
@@ -2288,10 +2297,22 @@ via_status via_debug_stage(via_ctx *c, int stage, uint64_t *out, size_t words) {
     if (!p || words < total) return VIA_E_INVALID;
     uint64_t *tmp = nullptr;
     HIP_TRY(hipMalloc(&tmp, sizeof(uint64_t) * total));
+    double *coef = nullptr;
+    if (stage == 4 && c->stage4_pairs) {
+        // FirstDim outputs kept in pair form: back to coefficient form
+        const int J = (int)(total / (2 * kSlots));
+        if (hipMalloc(&coef, sizeof(double) * total) != hipSuccess) {
+            (void)hipFree(tmp);
+            return VIA_E_NOMEM;
+        }
+        launch_fd_unpair(p, coef, J, c->stage4_ring, c->stream);
+        p = coef;
+    }
     launch_f64_to_u64(p, tmp, total, c->stream);
     HIP_TRY(hipMemcpyAsync(out, tmp, sizeof(uint64_t) * total, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipFree(tmp));
+    if (coef) HIP_TRY(hipFree(coef));
     return VIA_OK;
 }
 
