@@ -4,7 +4,7 @@ in eval form and added by the gate before its own inverse transform.
 
 The pass takes that form when the first CMux level runs the fixed-shape gate
 level by level (at least 256 gates: four clients x J/2 = 64 here, J a
-multiple of 8). Four clients' queries at I=8, J=128 in one pass:
+multiple of 16). Four clients' queries at I=8, J=128 in one pass:
 
 * every answer equals the oracle's (server_answer) word for word;
 * query 0's FirstDim outputs, brought back to coefficient form from the

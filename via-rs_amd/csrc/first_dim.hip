@@ -564,19 +564,21 @@ __global__ __launch_bounds__(512) void k_fd_intt_fx(const double *__restrict__ o
 // The FirstDim outputs in the form the first CMux level takes from them
 // (cmux(b, c0, c1) = c0 + b (x) (c1 - c0), mux.rs:75-84, on columns
 // 2i, 2i + 1): one inverse transform per pair instead of two. Workgroup b
-// takes the polys of eight consecutive columns j (J a multiple of 8, so the
-// same q and part), i.e. four pairs: thread t reads slot t / 4 + 128 x of
-// pair t % 4 as one 16-byte (c0, c1) load (the workgroup reads 64-byte runs
-// of each slot row; row groups of 128 polys per XCD as in k_fd_intt_fx),
-// stores c0 in eval form to column 2i + 1 of out (16 consecutive slots of
-// each pair per wavefront store: 128-byte runs) and stages the difference
-// c1 - c0 in LDS. The four differences are transformed back as in
+// takes the polys of 2 NPR consecutive columns j (J a multiple of 2 NPR, so
+// the same q and part), i.e. NPR pairs: thread t reads slot t / NPR + 128 x
+// of pair t % NPR as one 16-byte (c0, c1) load (the workgroup reads
+// 16 NPR-byte runs of each slot row; row groups of 128 polys per XCD as in
+// k_fd_intt_fx), stores c0 in eval form to column 2i + 1 of out (runs of
+// consecutive slots of each pair per wavefront store) and stages the
+// difference c1 - c0 in LDS. The differences are transformed back as in
 // k_fd_intt_fx (128 threads each: stages 0..3 and 4..7 over LDS, 8..10 in
 // registers, N^-1) into column 2i. The gate adds c0 to its product before
-// its own inverse transform.
-__global__ __launch_bounds__(512) void k_fd_intt_pairs(const double *__restrict__ ok, double *__restrict__ out,
-                                                       FdPlan p, RingDev r2) {
-    constexpr int IPP = 8, NPR = IPP / 2, LPW = 128 / IPP, BUF = lds_words(kSlots) + 2, TPB = 512;
+// its own inverse transform. NPR = 8 (128-byte runs, 1024 threads, one
+// workgroup per CU) reads faster than NPR = 4 (64-byte runs, two per CU).
+template <int NPR>
+__global__ __launch_bounds__(128 * NPR) void k_fd_intt_pairs(const double *__restrict__ ok, double *__restrict__ out,
+                                                             FdPlan p, RingDev r2) {
+    constexpr int IPP = 2 * NPR, LPW = 128 / IPP, BUF = lds_words(kSlots) + 2, TPB = 128 * NPR;
     constexpr int PT = NPR * kSlots / TPB;  // slots per thread (one pair)
     __shared__ double s[NPR * BUF];
     const size_t P = (size_t)p.Q * 2 * p.J;
@@ -655,13 +657,17 @@ __global__ __launch_bounds__(512) void k_fd_intt_pairs(const double *__restrict_
     }
 }
 
-bool fd_pairs_fits(const FdPlan &p) { return p.nseg == 1 && p.n == kSlots && p.J % 8 == 0; }
+constexpr int kFdPairsNpr = 8;  // column pairs per workgroup of k_fd_intt_pairs
+
+bool fd_pairs_fits(const FdPlan &p) { return p.nseg == 1 && p.n == kSlots && p.J % (2 * kFdPairsNpr) == 0; }
 
 void launch_fd_intt_pairs(const double *out_k, double *out, const FdPlan &p, const RingDev &r2, hipStream_t st) {
+    constexpr int IPP = 2 * kFdPairsNpr;
     const size_t P = (size_t)p.Q * 2 * p.J;
-    const size_t g = (size_t)8 * 128 / 8;  // whole XCD row groups of 128 polys
-    const size_t blocks = ((P + 7) / 8 + g - 1) / g * g;
-    hipLaunchKernelGGL(k_fd_intt_pairs, dim3((unsigned)blocks), dim3(512), 0, st, out_k, out, p, r2);
+    const size_t g = (size_t)8 * 128 / IPP;  // whole XCD row groups of 128 polys
+    const size_t blocks = ((P + IPP - 1) / IPP + g - 1) / g * g;
+    hipLaunchKernelGGL(k_fd_intt_pairs<kFdPairsNpr>, dim3((unsigned)blocks), dim3(128 * kFdPairsNpr), 0, st, out_k,
+                       out, p, r2);
 }
 
 // Coefficient form [q][j][part][n] of query 0's FirstDim outputs from the

@@ -258,7 +258,7 @@ void launch_fd_mfma(const uint8_t *dbl, const uint8_t *sell, double *out_k, cons
 void launch_fd_intt(const double *out_k, double *out, const FdPlan &p, const RingDev &r2, hipStream_t s);
 // The same outputs in pair form for a first CMux level of GATE_CMUX_EV
 // gates: column 2i holds c1 - c0 in coefficient form, column 2i + 1 c0 in
-// eval form (fd_pairs_fits: one row segment, n = 2048, J a multiple of 8).
+// eval form (fd_pairs_fits: one row segment, n = 2048, J a multiple of 16).
 bool fd_pairs_fits(const FdPlan &p);
 void launch_fd_intt_pairs(const double *out_k, double *out, const FdPlan &p, const RingDev &r2, hipStream_t s);
 // query 0's pair-form outputs (J columns) back to coefficient form
