@@ -261,6 +261,207 @@ __global__ __launch_bounds__(256, 3) void k_gate_fx(RingDev r, Gadget g, GateArg
     }
 }
 
+// Narrow levels (fewer than kGateFxMin gates, where one gate's latency is
+// the level's): one gate, or with two limbs one limb of a gate (workgroup
+// 2 gate + l), per 1024-thread workgroup with compile-time passes. Threads
+// 256 h + j (h = 2 half + k) compute digit k of coefficients j + 256 t of
+// the ciphertext half (both limbs enter the scale) and run the first
+// forward pass on them in registers; stages 3..5 and 6..7 run over LDS; the
+// last pass (8..10) leaves slots 8j .. 8j+7 of poly h in LDS. Threads
+// 512 p + u then multiply slots 4u .. 4u+3 of the four digit polys with the
+// RGSW rows of part p (loaded at the start), run inverse stages 0..1 on the
+// sums in registers, 2..4 and 5..7 over LDS (256 threads per part), and
+// 8..10 with N^-1 and the gate combine on coefficients u + 256 t. Eight
+// barriers per gate.
+template <int NL, bool DIRECT, int MODE>
+__global__ __launch_bounds__(1024) void k_gate_nx(RingDev r, Gadget g, GateArgs ga) {
+    constexpr int S = kSlots, L = 2;
+    __shared__ double s[4 * kGxBuf];  // the four digit polys, then the two product parts
+    const int tid = threadIdx.x;
+    const int gate = NL == 2 ? (int)(blockIdx.x >> 1) : (int)blockIdx.x;
+    const int l = NL == 2 ? (int)(blockIdx.x & 1) : 0;
+    const int q = gate / ga.items_per_query;
+    const int jg = gate - q * ga.items_per_query;
+    const int shift = ga.shift;
+    const double *rgsw = ga.rgsw + (size_t)q * ga.rgsw_qstride;
+    const double *in = ga.in + (size_t)q * ga.in_qstride;
+    double *out = ga.out + (size_t)q * ga.out_qstride;
+    const size_t pw = (size_t)NL * S;
+    const FMod m = r.t[l].m, m0 = r.t[0].m, m1 = r.t[1].m;
+    const double *twf = r.t[l].fwd, *twi = r.t[l].inv;
+    // product role: part p, slots 4u .. 4u+3; combine role (u < 256):
+    // coefficients u + 256 t of part p
+    const int p = tid >> 9, u = tid & 511;
+    {
+        // digit k of coefficients j + 256 t of the half, first forward pass
+        const int h = tid >> 8, j = tid & 255, half = h >> 1, k = h & 1;
+        double y[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const int c = j + 256 * t;
+            double b0, c0, b1 = 0.0, c1 = 0.0;
+            gate_input<NL>(MODE, in, jg, half, 0, c, S, shift, m0, b0, c0);
+            if (NL == 2) gate_input<NL>(MODE, in, jg, half, 1, c, S, shift, m1, b1, c1);
+            double v = gx_scaled<NL>(c0, c1, r, g);
+            double d = (double)digit_f64(v, ga.dp);  // LSB first (gadget.rs:168-190)
+            if (k == 1) d = (double)digit_f64(v, ga.dp);
+            y[t] = d;
+        }
+        if constexpr (DIRECT) {
+            direct_pass<3>(y, r.t[l].dfw);
+        } else {
+            double w0[7];
+            load_unit_tw<3, 0>(w0, twf, 0);
+            unit_stages<3>(y, w0, m);
+        }
+        double *pp = s + h * kGxBuf + lds_pad(j);
+#pragma unroll
+        for (int t = 0; t < 8; t++) pp[unit_off<256>(t)] = y[t];
+    }
+    // the RGSW row words of the products, in flight during the forward passes
+    double kr[4][4];  // RGSW row words [poly h][slot]
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const int half = h >> 1, k = h & 1;
+        const double2 *row = reinterpret_cast<const double2 *>(
+            rgsw + ((size_t)half * L + (L - 1 - k)) * 2 * pw + (size_t)p * pw + (size_t)l * S + 4 * u);
+        const double2 a = row[0], b = row[1];
+        kr[h][0] = a.x;
+        kr[h][1] = a.y;
+        kr[h][2] = b.x;
+        kr[h][3] = b.y;
+    }
+    __syncthreads();
+    {
+        const int h = tid >> 8, j = tid & 255;
+        {
+            using G = UnitGeo<kSlotsLog, 3, 3>;
+            int base, SB;
+            G::at(j, base, SB);
+            double w[7];
+            load_unit_tw<3, 3>(w, twf, SB);
+            double *pp = s + h * kGxBuf + lds_pad(base);
+            double x[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) x[t] = pp[unit_off<(1 << G::JB)>(t)];
+            unit_stages<3>(x, w, m);
+#pragma unroll
+            for (int t = 0; t < 8; t++) pp[unit_off<(1 << G::JB)>(t)] = x[t];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 2; v++) {
+            using G = UnitGeo<kSlotsLog, 6, 2>;
+            int base, SB;
+            G::at(j + 256 * v, base, SB);
+            double w[3];
+            load_unit_tw<2, 6>(w, twf, SB);
+            double *pp = s + h * kGxBuf + lds_pad(base);
+            double x[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) x[t] = pp[unit_off<(1 << G::JB)>(t)];
+            unit_stages<2>(x, w, m);
+#pragma unroll
+            for (int t = 0; t < 4; t++) pp[unit_off<(1 << G::JB)>(t)] = x[t];
+        }
+        __syncthreads();
+        {
+            int base, SB;
+            UnitGeo<kSlotsLog, 8, 3>::at(j, base, SB);
+            double w[7];
+            load_unit_tw<3, 8>(w, twf, SB);
+            double *pp = s + h * kGxBuf + lds_pad(8 * j);
+            double x[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) x[t] = pp[t];
+            unit_stages<3>(x, w, m);
+#pragma unroll
+            for (int t = 0; t < 8; t++) pp[t] = x[t];
+        }
+    }
+    // the combine's inputs, in flight during the inverse passes
+    double bvp[8], cvp[8];
+    if (u < 256) {
+#pragma unroll
+        for (int t = 0; t < 8; t++) gate_input<NL>(MODE, in, jg, p, l, u + 256 * t, S, shift, m, bvp[t], cvp[t]);
+    }
+    __syncthreads();
+    // products of part p at slots 4u .. 4u+3 (digit k with row L-1-k), then
+    // inverse stages 0..1 on them
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const double *src = s + h * kGxBuf + lds_pad(4 * u);
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc[t] += mod_mul_lazy(src[t], kr[h][t], m);
+    }
+    {
+        double w[3];
+        load_inv_unit_tw<kSlotsLog, 0, 2>(w, twi, u);
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc[t] = lazy_reduce(acc[t], m);
+        inv_unit_stages<2>(acc, w, m);
+    }
+    __syncthreads();  // every product read the digit polys
+    double *sp = s + p * kGxBuf;
+#pragma unroll
+    for (int t = 0; t < 4; t++) sp[lds_pad(4 * u) + t] = acc[t];
+    __syncthreads();
+    if (u < 256) {
+        using G = InvGeo<kSlotsLog, 2, 3>;
+        int base, SB;
+        G::at(u, base, SB);
+        double w[7];
+        load_inv_unit_tw<kSlotsLog, 2, 3>(w, twi, SB);
+        double *pp = sp + lds_pad(base);
+        double x[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) x[t] = pp[unit_off<4>(t)];
+        inv_unit_stages<3>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 8; t++) pp[unit_off<4>(t)] = x[t];
+    }
+    __syncthreads();
+    if (u < 256) {
+        using G = InvGeo<kSlotsLog, 5, 3>;
+        int base, SB;
+        G::at(u, base, SB);
+        double w[7];
+        load_inv_unit_tw<kSlotsLog, 5, 3>(w, twi, SB);
+        double *pp = sp + lds_pad(base);
+        double x[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) x[t] = pp[unit_off<32>(t)];
+        inv_unit_stages<3>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 8; t++) pp[unit_off<32>(t)] = x[t];
+    }
+    __syncthreads();
+    if (u < 256) {
+        // inverse stages 8..10 on coefficients u + 256 t, N^-1, the combine
+        double w[7];
+        load_inv_unit_tw<kSlotsLog, 8, 3>(w, twi, 0);
+        const double ni = r.t[l].n_inv;
+        const double *pp = sp + lds_pad(u);
+        double x[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) x[t] = pp[unit_off<256>(t)];
+        inv_unit_stages<3>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const int c = u + 256 * t;
+            const double pv = mod_mul(x[t], ni, m);
+            const size_t o = (size_t)p * pw + (size_t)l * S + c;
+            if (MODE == GATE_DMUX) {
+                out[(size_t)(2 * jg) * 2 * pw + o] = mod_sub(cvp[t], pv, m);
+                out[(size_t)(2 * jg + 1) * 2 * pw + o] = pv;
+            } else {
+                out[(size_t)jg * 2 * pw + o] = mod_add(bvp[t], pv, m);
+            }
+        }
+    }
+}
+
 // Whether the fixed-shape gate takes a level: degree 2048, gadget depth 2
 // with B^2 below 2^50 (both digits from the low part), the fp64 scale (one
 // limb: exact, g.f64 = 1; two: decided or exact fallback, g.f64 = 2);
@@ -284,6 +485,25 @@ static void launch_gx(const dim3 &grid, const RingDev &r, const Gadget &g, const
             break;
         default: hipLaunchKernelGGL((k_gate_fx<NL, DIRECT, GATE_CROT>), grid, dim3(256), 0, st, r, g, a); break;
     }
+}
+
+template <int NL, bool DIRECT>
+static void launch_nx(const dim3 &grid, const RingDev &r, const Gadget &g, const GateArgs &a, hipStream_t st) {
+    switch (a.mode) {
+        case GATE_DMUX: hipLaunchKernelGGL((k_gate_nx<NL, DIRECT, GATE_DMUX>), grid, dim3(1024), 0, st, r, g, a); break;
+        case GATE_CMUX: hipLaunchKernelGGL((k_gate_nx<NL, DIRECT, GATE_CMUX>), grid, dim3(1024), 0, st, r, g, a); break;
+        default: hipLaunchKernelGGL((k_gate_nx<NL, DIRECT, GATE_CROT>), grid, dim3(1024), 0, st, r, g, a); break;
+    }
+}
+
+void launch_gate_nx(const RingDev &r, const Gadget &g, const GateArgs &a, bool direct, hipStream_t st) {
+    const int blocks = a.items_per_query * a.queries;
+    if (r.nl == 2)
+        launch_nx<2, false>(dim3(2 * blocks), r, g, a, st);
+    else if (direct)
+        launch_nx<1, true>(dim3(blocks), r, g, a, st);
+    else
+        launch_nx<1, false>(dim3(blocks), r, g, a, st);
 }
 
 void launch_gate_fx(const RingDev &r, const Gadget &g, const GateArgs &a, bool direct, hipStream_t st) {

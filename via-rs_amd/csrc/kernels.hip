@@ -828,6 +828,14 @@ void launch_gate(const RingDev &r, const Gadget &g_in, const GateArgs &a_in, hip
         // than the CU count (latency-bound); wider levels keep both limbs in
         // one workgroup (the digits are extracted once)
         const bool split = a.split < 0 ? blocks < 256 : a.split == 1;
+        // narrow one-limb levels of the fixed shape: the compile-time passes
+        // of the fixed gate with 1024 threads per gate (two limbs keep the
+        // fused gate, which computes each coefficient's two-limb scale once
+        // per workgroup instead of once per digit poly)
+        if (r.nl == 1 && blocks < kGateFxMin && a.mode != GATE_CMUX_EV && gate_fx_fits(r, g, &direct)) {
+            launch_gate_nx(r, g, a, direct, st);
+            return;
+        }
         if (r.nl == 2 && split)
             hipLaunchKernelGGL((k_gate_fused<2, 1>), dim3(2 * blocks), dim3(1024), 0, st, r, g, a);
         else if (r.nl == 2)
