@@ -261,25 +261,26 @@ __global__ __launch_bounds__(256, 3) void k_gate_fx(RingDev r, Gadget g, GateArg
     }
 }
 
-// Narrow levels (fewer than kGateFxMin gates, where one gate's latency is
-// the level's): one gate, or with two limbs one limb of a gate (workgroup
-// 2 gate + l), per 1024-thread workgroup with compile-time passes. Threads
-// 256 h + j (h = 2 half + k) compute digit k of coefficients j + 256 t of
-// the ciphertext half (both limbs enter the scale) and run the first
-// forward pass on them in registers; stages 3..5 and 6..7 run over LDS; the
+// Narrow one-limb levels (fewer than kGateFxMin gates, where one gate's
+// latency is the level's): one gate per 1024-thread workgroup with
+// compile-time passes. Threads 256 h + j (h = 2 half + k) compute digit k of
+// coefficients j + 256 t of the ciphertext half and run the first forward
+// pass on them in registers; stages 3..5 and 6..7 run over LDS; the
 // last pass (8..10) leaves slots 8j .. 8j+7 of poly h in LDS. Threads
 // 512 p + u then multiply slots 4u .. 4u+3 of the four digit polys with the
 // RGSW rows of part p (loaded at the start), run inverse stages 0..1 on the
 // sums in registers, 2..4 and 5..7 over LDS (256 threads per part), and
 // 8..10 with N^-1 and the gate combine on coefficients u + 256 t. Eight
-// barriers per gate.
-template <int NL, bool DIRECT, int MODE>
+// barriers per gate. (A two-limb version, one limb per workgroup, was slower
+// than the fused gate on the narrow DMux levels, even with each
+// coefficient's two-limb scale computed once per workgroup.)
+template <bool DIRECT, int MODE>
 __global__ __launch_bounds__(1024) void k_gate_nx(RingDev r, Gadget g, GateArgs ga) {
-    constexpr int S = kSlots, L = 2;
+    constexpr int S = kSlots, L = 2, NL = 1;
     __shared__ double s[4 * kGxBuf];  // the four digit polys, then the two product parts
     const int tid = threadIdx.x;
-    const int gate = NL == 2 ? (int)(blockIdx.x >> 1) : (int)blockIdx.x;
-    const int l = NL == 2 ? (int)(blockIdx.x & 1) : 0;
+    const int gate = blockIdx.x;
+    constexpr int l = 0;
     const int q = gate / ga.items_per_query;
     const int jg = gate - q * ga.items_per_query;
     const int shift = ga.shift;
@@ -287,7 +288,7 @@ __global__ __launch_bounds__(1024) void k_gate_nx(RingDev r, Gadget g, GateArgs 
     const double *in = ga.in + (size_t)q * ga.in_qstride;
     double *out = ga.out + (size_t)q * ga.out_qstride;
     const size_t pw = (size_t)NL * S;
-    const FMod m = r.t[l].m, m0 = r.t[0].m, m1 = r.t[1].m;
+    const FMod m = r.t[0].m;
     const double *twf = r.t[l].fwd, *twi = r.t[l].inv;
     // product role: part p, slots 4u .. 4u+3; combine role (u < 256):
     // coefficients u + 256 t of part p
@@ -299,10 +300,9 @@ __global__ __launch_bounds__(1024) void k_gate_nx(RingDev r, Gadget g, GateArgs 
 #pragma unroll
         for (int t = 0; t < 8; t++) {
             const int c = j + 256 * t;
-            double b0, c0, b1 = 0.0, c1 = 0.0;
-            gate_input<NL>(MODE, in, jg, half, 0, c, S, shift, m0, b0, c0);
-            if (NL == 2) gate_input<NL>(MODE, in, jg, half, 1, c, S, shift, m1, b1, c1);
-            double v = gx_scaled<NL>(c0, c1, r, g);
+            double b0, c0;
+            gate_input<NL>(MODE, in, jg, half, 0, c, S, shift, m, b0, c0);
+            double v = gx_scaled<NL>(c0, 0.0, r, g);
             double d = (double)digit_f64(v, ga.dp);  // LSB first (gadget.rs:168-190)
             if (k == 1) d = (double)digit_f64(v, ga.dp);
             y[t] = d;
@@ -487,23 +487,21 @@ static void launch_gx(const dim3 &grid, const RingDev &r, const Gadget &g, const
     }
 }
 
-template <int NL, bool DIRECT>
+template <bool DIRECT>
 static void launch_nx(const dim3 &grid, const RingDev &r, const Gadget &g, const GateArgs &a, hipStream_t st) {
     switch (a.mode) {
-        case GATE_DMUX: hipLaunchKernelGGL((k_gate_nx<NL, DIRECT, GATE_DMUX>), grid, dim3(1024), 0, st, r, g, a); break;
-        case GATE_CMUX: hipLaunchKernelGGL((k_gate_nx<NL, DIRECT, GATE_CMUX>), grid, dim3(1024), 0, st, r, g, a); break;
-        default: hipLaunchKernelGGL((k_gate_nx<NL, DIRECT, GATE_CROT>), grid, dim3(1024), 0, st, r, g, a); break;
+        case GATE_DMUX: hipLaunchKernelGGL((k_gate_nx<DIRECT, GATE_DMUX>), grid, dim3(1024), 0, st, r, g, a); break;
+        case GATE_CMUX: hipLaunchKernelGGL((k_gate_nx<DIRECT, GATE_CMUX>), grid, dim3(1024), 0, st, r, g, a); break;
+        default: hipLaunchKernelGGL((k_gate_nx<DIRECT, GATE_CROT>), grid, dim3(1024), 0, st, r, g, a); break;
     }
 }
 
 void launch_gate_nx(const RingDev &r, const Gadget &g, const GateArgs &a, bool direct, hipStream_t st) {
-    const int blocks = a.items_per_query * a.queries;
-    if (r.nl == 2)
-        launch_nx<2, false>(dim3(2 * blocks), r, g, a, st);
-    else if (direct)
-        launch_nx<1, true>(dim3(blocks), r, g, a, st);
+    const dim3 grid(a.items_per_query * a.queries);
+    if (direct)
+        launch_nx<true>(grid, r, g, a, st);
     else
-        launch_nx<1, false>(dim3(blocks), r, g, a, st);
+        launch_nx<false>(grid, r, g, a, st);
 }
 
 void launch_gate_fx(const RingDev &r, const Gadget &g, const GateArgs &a, bool direct, hipStream_t st) {
