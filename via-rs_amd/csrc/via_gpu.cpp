@@ -2071,8 +2071,9 @@ via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via
     {
         const FdPlan fp = fd_plan((int)db->pp.num_rows, (int)db->pp.num_cols, (int)db->pp.n1, Q);
         double best_t = 0;
-        for (int G : {1, 2, 4, 8}) {
-            if (G > fp.MB) continue;
+        for (int G : {1, 2, 4, 8, 16, 32, 64}) {
+            // up to one group per 4 column blocks (one per wavefront)
+            if (G > 1 && 4 * G > fp.MB) continue;
             c->fd_groups_override = G;
             double t = 0;
             // one launch per answer, a few percent apart: more runs than the
