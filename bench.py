@@ -673,7 +673,7 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=80.0,
                     help="default line: offered request rate of the mixed serving trace (0 to skip)")
     ap.add_argument("--serve-requests", type=int, default=400)
-    ap.add_argument("--serve-c-qps", type=float, default=5500.0,
+    ap.add_argument("--serve-c-qps", type=float, default=5800.0,
                     help="default line: VIA-C-only arrival rate for the serving loop's saturation point (0: skip)")
     ap.add_argument("--serve-c-requests", type=int, default=3000)
     ap.add_argument("--keygen-clients", type=int, default=32,
