@@ -282,6 +282,7 @@ struct RespArgs {
     int n1, n2, d, l_rsk;
     uint32_t b_rsk;
     uint64_t q2, q3, q4;
+    uint64_t minv2 = 0, minv3 = 0;  // floor((2^64 - 1) / q2), / q3 (set by launch_resp_comp)
     NttTab t3;        // q3 at n2
     int queries;
     DigitPlan dp;     // digit plan of (b_rsk, l_rsk) (set by launch_resp_comp)

@@ -953,6 +953,28 @@ void launch_modswitch_q1_q2(const RingDev &r1, uint64_t q2, const double *in, do
 // mod_switch_asym (mod_switch.rs:124-154) to mask mod q3, body mod q4.
 constexpr int kRespBufWords = 16384;  // digit polynomials of one round
 
+// floor(N / d) for any 64-bit N, with minv = floor((2^64 - 1) / d): the
+// estimate umulhi(N, minv) is at most two below the quotient (minv is below
+// 2^64 / d by less than one, so N minv / 2^64 is below N / d by less than
+// N / 2^64 < 1). A few instructions instead of a 64-bit division.
+__device__ __forceinline__ uint64_t udiv_pre(uint64_t N, uint64_t d, uint64_t minv) {
+    uint64_t qh = __umul64hi(N, minv);
+    uint64_t r = N - qh * d;
+    if (r >= d) {
+        qh++;
+        r -= d;
+    }
+    if (r >= d) qh++;
+    return qh;
+}
+
+// round(x * qt / qs) mod qt for x < qs, i.e. floor((x qt + qs / 2) / qs),
+// which is at most qt (mod_switch.rs rounding)
+__device__ __forceinline__ uint64_t switch_round(uint64_t x, uint64_t qt, uint64_t qs, uint64_t minv_s) {
+    const uint64_t v = udiv_pre(x * qt + (qs >> 1), qs, minv_s);
+    return v >= qt ? v - qt : v;
+}
+
 __global__ __launch_bounds__(1024) void k_resp_comp(RespArgs a) {
     __shared__ double c3m[kSlots];           // mask at q3, degree n1
     __shared__ double buf[lds_words(kRespBufWords)];  // digit polynomials, then the two parts
@@ -963,8 +985,7 @@ __global__ __launch_bounds__(1024) void k_resp_comp(RespArgs a) {
     const FMod m3 = a.t3.m;
     const double *ct = a.ct + (size_t)qy * 2 * n1;
     for (int c = threadIdx.x; c < n1; c += blockDim.x) {
-        const uint64_t x = (uint64_t)ct[c];
-        c3m[c] = (double)(((x * q3 + (q2 >> 1)) / q2) % q3);
+        c3m[c] = (double)switch_round((uint64_t)ct[c], q3, q2, a.minv2);
     }
     __syncthreads();
     NttLimbs Tf, Ti;
@@ -991,7 +1012,7 @@ __global__ __launch_bounds__(1024) void k_resp_comp(RespArgs a) {
             const int neg = x > (q3 >> 1);
             const uint64_t mag = neg ? q3 - x : x;
             SMag v;
-            v.mag = (via_u128)((mag * bl + (q3 >> 1)) / q3);
+            v.mag = (via_u128)udiv_pre(mag * bl + (q3 >> 1), q3, a.minv3);
             v.neg = neg && v.mag != 0;
             double lo, hi;
             digit_split_f64(v, a.dp, lo, hi);
@@ -1027,18 +1048,19 @@ __global__ __launch_bounds__(1024) void k_resp_comp(RespArgs a) {
     for (int i = threadIdx.x; i < n2; i += blockDim.x) {
         const double rm = mod_mul(buf[lds_pad(i)], a.t3.n_inv, m3);
         const double rb = mod_mul(buf[lds_pad(n2 + i)], a.t3.n_inv, m3);
-        const uint64_t xb = (uint64_t)ct[n1 + d * i];
-        const double body_proj = (double)(((xb * q3 + (q2 >> 1)) / q2) % q3);
+        const double body_proj = (double)switch_round((uint64_t)ct[n1 + d * i], q3, q2, a.minv2);
         const uint64_t mask = (uint64_t)mod_neg(rm, m3);
         const uint64_t body = (uint64_t)mod_sub(body_proj, rb, m3);
-        a.out_mask[qy][i] = ((mask * q3 + (q3 >> 1)) / q3) % q3;
-        a.out_body[qy][i] = ((body * q4 + (q3 >> 1)) / q3) % q4;
+        a.out_mask[qy][i] = switch_round(mask, q3, q3, a.minv3);
+        a.out_body[qy][i] = switch_round(body, q4, q3, a.minv3);
     }
 }
 
 void launch_resp_comp(const RespArgs &a_in, hipStream_t st) {
     RespArgs a = a_in;
     a.dp = digit_plan(a.b_rsk, a.l_rsk);
+    a.minv2 = ~0ULL / a.q2;
+    a.minv3 = ~0ULL / a.q3;
     hipLaunchKernelGGL(k_resp_comp, dim3(a.queries), dim3(1024), 0, st, a);
 }
 
