@@ -322,7 +322,9 @@ void launch_gadget_mac(const RingDev &r, const Gadget &g, const MacArgs &a, hipS
 template <int NL, int TPB>
 __global__ __launch_bounds__(TPB) void k_cascade_combine(RingDev r, Gadget g, DigitPlan dp, RingDev rq,
                                                          CombineArgs a) {
-    __shared__ double s[lds_words(NL * kSlots)];
+    // lds_words(NL * max(chunk, n)) words, sized by the launcher: small at
+    // the low degrees, so more workgroups share a CU
+    extern __shared__ double s[];
     const int item = blockIdx.x, chunk = blockIdx.y, chunks = gridDim.y - 1;
     const int S = kSlots, n = r.n;
     const FMod m0 = r.t[0].m, m1 = r.t[1].m;
@@ -379,20 +381,22 @@ void launch_cascade_combine(const RingDev &r, const Gadget &g_in, const DigitPla
     if (chunks > 8) chunks = 8;
     dim3 grid(a.items, chunks + 1);
     const bool wide = chunks <= 2;
+    const int cs = kSlots / chunks;
+    const size_t lds = sizeof(double) * lds_words(r.nl * (cs > (int)r.n ? cs : (int)r.n));
     // digits of every output coefficient with the scale of launch_poly_digits
     // (rq: the q1 ring at n1, whose RNS and gadget constants the scale uses)
     Gadget g = g_in;
     if (a.dig_next && g.f64 == 0 && g.bl >= ((via_u128)1 << 40) && scale_w_make(rq, g, dp, &g.sw)) g.f64 = 3;
     if (r.nl == 2) {
         if (wide)
-            hipLaunchKernelGGL((k_cascade_combine<2, 1024>), grid, dim3(1024), 0, st, r, g, dp, rq, a);
+            hipLaunchKernelGGL((k_cascade_combine<2, 1024>), grid, dim3(1024), lds, st, r, g, dp, rq, a);
         else
-            hipLaunchKernelGGL((k_cascade_combine<2, 256>), grid, dim3(256), 0, st, r, g, dp, rq, a);
+            hipLaunchKernelGGL((k_cascade_combine<2, 256>), grid, dim3(256), lds, st, r, g, dp, rq, a);
     } else {
         if (wide)
-            hipLaunchKernelGGL((k_cascade_combine<1, 1024>), grid, dim3(1024), 0, st, r, g, dp, rq, a);
+            hipLaunchKernelGGL((k_cascade_combine<1, 1024>), grid, dim3(1024), lds, st, r, g, dp, rq, a);
         else
-            hipLaunchKernelGGL((k_cascade_combine<1, 256>), grid, dim3(256), 0, st, r, g, dp, rq, a);
+            hipLaunchKernelGGL((k_cascade_combine<1, 256>), grid, dim3(256), lds, st, r, g, dp, rq, a);
     }
 }
 
