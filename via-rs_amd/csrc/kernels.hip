@@ -377,8 +377,11 @@ __global__ __launch_bounds__(TPB) void k_cascade_combine(RingDev r, Gadget g, Di
 
 void launch_cascade_combine(const RingDev &r, const Gadget &g_in, const DigitPlan &dp, const RingDev &rq,
                             const CombineArgs &a, hipStream_t st) {
+    // chunks of whole degree-n polys per item: up to 16 when few items
+    // leave CUs idle (one query: 36 LWEs), else up to 8
+    const int cap = a.items < 128 ? 16 : 8;
     int chunks = kSlots / r.n;
-    if (chunks > 8) chunks = 8;
+    if (chunks > cap) chunks = cap;
     dim3 grid(a.items, chunks + 1);
     const bool wide = chunks <= 2;
     const int cs = kSlots / chunks;
