@@ -261,9 +261,9 @@ __global__ __launch_bounds__(256, 3) void k_gate_fx(RingDev r, Gadget g, GateArg
     }
 }
 
-// Narrow one-limb levels (fewer than kGateFxMin gates, where one gate's
-// latency is the level's): one gate per 1024-thread workgroup with
-// compile-time passes. Threads 256 h + j (h = 2 half + k) compute digit k of
+// Narrow levels (fewer than kGateFxMin gates, where one gate's latency is
+// the level's): one gate (one limb of it with two limbs) per 1024-thread
+// workgroup with compile-time passes. Threads 256 h + j (h = 2 half + k) compute digit k of
 // coefficients j + 256 t of the ciphertext half and run the first forward
 // pass on them in registers; stages 3..5 and 6..7 run over LDS; the
 // last pass (8..10) leaves slots 8j .. 8j+7 of poly h in LDS. Threads
@@ -271,16 +271,18 @@ __global__ __launch_bounds__(256, 3) void k_gate_fx(RingDev r, Gadget g, GateArg
 // RGSW rows of part p (loaded at the start), run inverse stages 0..1 on the
 // sums in registers, 2..4 and 5..7 over LDS (256 threads per part), and
 // 8..10 with N^-1 and the gate combine on coefficients u + 256 t. Eight
-// barriers per gate. (A two-limb version, one limb per workgroup, was slower
-// than the fused gate on the narrow DMux levels, even with each
-// coefficient's two-limb scale computed once per workgroup.)
-template <bool DIRECT, int MODE>
+// barriers per gate. Two limbs (the DMux at q1): one limb per workgroup
+// (workgroup 2 gate + l), each coefficient's two-limb scale computed once
+// per workgroup (the threads of digit k take half of the coefficients and
+// hand the other digit over through LDS) with the scale inlined: an
+// out-of-line exact fallback in that loop cost more than the whole gain.
+template <int NL, bool DIRECT, int MODE>
 __global__ __launch_bounds__(1024) void k_gate_nx(RingDev r, Gadget g, GateArgs ga) {
-    constexpr int S = kSlots, L = 2, NL = 1;
+    constexpr int S = kSlots, L = 2;
     __shared__ double s[4 * kGxBuf];  // the four digit polys, then the two product parts
     const int tid = threadIdx.x;
-    const int gate = blockIdx.x;
-    constexpr int l = 0;
+    const int gate = NL == 2 ? (int)(blockIdx.x >> 1) : (int)blockIdx.x;
+    const int l = NL == 2 ? (int)(blockIdx.x & 1) : 0;
     const int q = gate / ga.items_per_query;
     const int jg = gate - q * ga.items_per_query;
     const int shift = ga.shift;
@@ -288,7 +290,7 @@ __global__ __launch_bounds__(1024) void k_gate_nx(RingDev r, Gadget g, GateArgs 
     const double *in = ga.in + (size_t)q * ga.in_qstride;
     double *out = ga.out + (size_t)q * ga.out_qstride;
     const size_t pw = (size_t)NL * S;
-    const FMod m = r.t[0].m;
+    const FMod m = r.t[l].m;
     const double *twf = r.t[l].fwd, *twi = r.t[l].inv;
     // product role: part p, slots 4u .. 4u+3; combine role (u < 256):
     // coefficients u + 256 t of part p
@@ -297,15 +299,42 @@ __global__ __launch_bounds__(1024) void k_gate_nx(RingDev r, Gadget g, GateArgs 
         // digit k of coefficients j + 256 t of the half, first forward pass
         const int h = tid >> 8, j = tid & 255, half = h >> 1, k = h & 1;
         double y[8];
+        if constexpr (NL == 1) {
 #pragma unroll
-        for (int t = 0; t < 8; t++) {
-            const int c = j + 256 * t;
-            double b0, c0;
-            gate_input<NL>(MODE, in, jg, half, 0, c, S, shift, m, b0, c0);
-            double v = gx_scaled<NL>(c0, 0.0, r, g);
-            double d = (double)digit_f64(v, ga.dp);  // LSB first (gadget.rs:168-190)
-            if (k == 1) d = (double)digit_f64(v, ga.dp);
-            y[t] = d;
+            for (int t = 0; t < 8; t++) {
+                const int c = j + 256 * t;
+                double b0, c0;
+                gate_input<NL>(MODE, in, jg, half, 0, c, S, shift, m, b0, c0);
+                double v = gx_scaled<NL>(c0, 0.0, r, g);
+                double d = (double)digit_f64(v, ga.dp);  // LSB first (gadget.rs:168-190)
+                if (k == 1) d = (double)digit_f64(v, ga.dp);
+                y[t] = d;
+            }
+        } else {
+            // the two-limb scale of a coefficient once per workgroup: the
+            // threads of digit k scale t = 4k .. 4k+3, keep digit k and hand
+            // the other digit to the partner thread through its buffer
+            const FMod m0 = r.t[0].m, m1 = r.t[1].m;
+            double *mine = s + h * kGxBuf + lds_pad(j), *other = s + (h ^ 1) * kGxBuf + lds_pad(j);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int t = 4 * k + e, c = j + 256 * t;
+                double b0, c0, b1, c1;
+                gate_input<NL>(MODE, in, jg, half, 0, c, S, shift, m0, b0, c0);
+                gate_input<NL>(MODE, in, jg, half, 1, c, S, shift, m1, b1, c1);
+                double lo, hi;
+                digit_split_f64(gadget_scale<NL>(c0, c1, r, g), ga.dp, lo, hi);
+                const double d0 = (double)next_digit(lo, hi, 0, ga.dp);
+                const double d1 = (double)next_digit(lo, hi, 1, ga.dp);
+                y[t] = k ? d1 : d0;
+                other[unit_off<256>(t)] = k ? d0 : d1;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int t = 4 * (1 - k) + e;
+                y[t] = mine[unit_off<256>(t)];
+            }
         }
         if constexpr (DIRECT) {
             direct_pass<3>(y, r.t[l].dfw);
@@ -487,21 +516,23 @@ static void launch_gx(const dim3 &grid, const RingDev &r, const Gadget &g, const
     }
 }
 
-template <bool DIRECT>
+template <int NL, bool DIRECT>
 static void launch_nx(const dim3 &grid, const RingDev &r, const Gadget &g, const GateArgs &a, hipStream_t st) {
     switch (a.mode) {
-        case GATE_DMUX: hipLaunchKernelGGL((k_gate_nx<DIRECT, GATE_DMUX>), grid, dim3(1024), 0, st, r, g, a); break;
-        case GATE_CMUX: hipLaunchKernelGGL((k_gate_nx<DIRECT, GATE_CMUX>), grid, dim3(1024), 0, st, r, g, a); break;
-        default: hipLaunchKernelGGL((k_gate_nx<DIRECT, GATE_CROT>), grid, dim3(1024), 0, st, r, g, a); break;
+        case GATE_DMUX: hipLaunchKernelGGL((k_gate_nx<NL, DIRECT, GATE_DMUX>), grid, dim3(1024), 0, st, r, g, a); break;
+        case GATE_CMUX: hipLaunchKernelGGL((k_gate_nx<NL, DIRECT, GATE_CMUX>), grid, dim3(1024), 0, st, r, g, a); break;
+        default: hipLaunchKernelGGL((k_gate_nx<NL, DIRECT, GATE_CROT>), grid, dim3(1024), 0, st, r, g, a); break;
     }
 }
 
 void launch_gate_nx(const RingDev &r, const Gadget &g, const GateArgs &a, bool direct, hipStream_t st) {
-    const dim3 grid(a.items_per_query * a.queries);
-    if (direct)
-        launch_nx<true>(grid, r, g, a, st);
+    const int blocks = a.items_per_query * a.queries;
+    if (r.nl == 2)
+        launch_nx<2, false>(dim3(2 * blocks), r, g, a, st);
+    else if (direct)
+        launch_nx<1, true>(dim3(blocks), r, g, a, st);
     else
-        launch_nx<false>(grid, r, g, a, st);
+        launch_nx<1, false>(dim3(blocks), r, g, a, st);
 }
 
 void launch_gate_fx(const RingDev &r, const Gadget &g, const GateArgs &a, bool direct, hipStream_t st) {

@@ -188,8 +188,9 @@ bool gate_takes_fx(const RingDev &r, const Gadget &g, int blocks);
 constexpr int kGateFxMin = 256;
 bool gate_fx_fits(const RingDev &r, const Gadget &g, bool *direct);
 void launch_gate_fx(const RingDev &r, const Gadget &g, const GateArgs &a, bool direct, hipStream_t s);
-// The same shape for narrow one-limb levels (fewer than kGateFxMin gates):
-// one gate per 1024-thread workgroup (not for GATE_CMUX_EV).
+// The same shape for narrow levels (fewer than kGateFxMin gates): one gate,
+// or one limb of a two-limb gate, per 1024-thread workgroup (not for
+// GATE_CMUX_EV).
 void launch_gate_nx(const RingDev &r, const Gadget &g, const GateArgs &a, bool direct, hipStream_t s);
 
 // Every level of one DMux or CMux tree in one cooperative launch

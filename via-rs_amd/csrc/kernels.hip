@@ -835,11 +835,9 @@ void launch_gate(const RingDev &r, const Gadget &g_in, const GateArgs &a_in, hip
         // than the CU count (latency-bound); wider levels keep both limbs in
         // one workgroup (the digits are extracted once)
         const bool split = a.split < 0 ? blocks < 256 : a.split == 1;
-        // narrow one-limb levels of the fixed shape: the compile-time passes
-        // of the fixed gate with 1024 threads per gate (two limbs keep the
-        // fused gate, which computes each coefficient's two-limb scale once
-        // per workgroup instead of once per digit poly)
-        if (r.nl == 1 && blocks < kGateFxMin && a.mode != GATE_CMUX_EV && gate_fx_fits(r, g, &direct)) {
+        // narrow levels of the fixed shape: the compile-time passes of the
+        // fixed gate with 1024 threads per gate (per limb with two limbs)
+        if (blocks < kGateFxMin && a.mode != GATE_CMUX_EV && (r.nl == 1 || split) && gate_fx_fits(r, g, &direct)) {
             launch_gate_nx(r, g, a, direct, st);
             return;
         }
