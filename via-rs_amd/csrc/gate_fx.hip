@@ -137,9 +137,16 @@ __global__ __launch_bounds__(256, 3) void k_gate_fx(RingDev r, Gadget g, GateArg
                 gate_input<NL>(mode, in, jg, half, 0, c, S, shift, m0, b0, c0);
                 if (NL == 2) gate_input<NL>(mode, in, jg, half, 1, c, S, shift, m1, b1, c1);
                 // both digits from the low part (B^2 < 2^50), LSB first (gadget.rs:168-190)
-                double v = gx_scaled<NL>(c0, c1, r, g);
+                if constexpr (NL == 2) {
+                    double lo, hi;
+                    digit_split_f64(gadget_scale<NL>(c0, c1, r, g), ga.dp, lo, hi);
 #pragma unroll
-                for (int k = 0; k < L; k++) y[k][t] = (double)digit_f64(v, ga.dp);
+                    for (int k = 0; k < L; k++) y[k][t] = (double)next_digit(lo, hi, k, ga.dp);
+                } else {
+                    double v = gx_scaled<NL>(c0, c1, r, g);
+#pragma unroll
+                    for (int k = 0; k < L; k++) y[k][t] = (double)digit_f64(v, ga.dp);
+                }
             }
 #pragma unroll
             for (int k = 0; k < L; k++) {
@@ -268,7 +275,7 @@ __global__ __launch_bounds__(256, 3) void k_gate_fx(RingDev r, Gadget g, GateArg
 // pass on them in registers; stages 3..5 and 6..7 run over LDS; the
 // last pass (8..10) leaves slots 8j .. 8j+7 of poly h in LDS. Threads
 // 512 p + u then multiply slots 4u .. 4u+3 of the four digit polys with the
-// RGSW rows of part p (loaded at the start), run inverse stages 0..1 on the
+// RGSW rows of part p (loaded during the forward passes), run inverse stages 0..1 on the
 // sums in registers, 2..4 and 5..7 over LDS (256 threads per part), and
 // 8..10 with N^-1 and the gate combine on coefficients u + 256 t. Eight
 // barriers per gate. Two limbs (the DMux at q1): one limb per workgroup

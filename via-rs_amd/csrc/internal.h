@@ -180,11 +180,12 @@ struct GateArgs {
     int split = -1;      // two limbs, one workgroup per limb: -1 below 256 gates, 0 never, 1 always
 };
 void launch_gate(const RingDev &r, const Gadget &g, const GateArgs &a, hipStream_t s);
-// whether launch_gate runs a level of `blocks` gates on the fixed-shape gate
+// whether launch_gate runs a one-limb level of `blocks` gates on the
+// fixed-shape gate (the condition of the pair-form FirstDim outputs)
 bool gate_takes_fx(const RingDev &r, const Gadget &g, int blocks);
-// The fixed-shape gate (gate_fx.hip) for one-limb levels of at least
-// kGateFxMin gates: degree 2048, gadget depth 2; *direct when its first
-// forward pass may run in direct form.
+// The fixed-shape gate (gate_fx.hip) for levels of at least kGateFxMin
+// gates (one workgroup per limb with two limbs): degree 2048, gadget depth
+// 2; *direct when its first forward pass may run in direct form.
 constexpr int kGateFxMin = 256;
 bool gate_fx_fits(const RingDev &r, const Gadget &g, bool *direct);
 void launch_gate_fx(const RingDev &r, const Gadget &g, const GateArgs &a, bool direct, hipStream_t s);

@@ -823,10 +823,9 @@ void launch_gate(const RingDev &r, const Gadget &g_in, const GateArgs &a_in, hip
     const Gadget g = gate_gadget(r, g_in);
     const int blocks = a.items_per_query * a.queries;
     bool direct = false;
-    if (r.nl == 1 && blocks >= kGateFxMin && gate_fx_fits(r, g, &direct)) {
-        // wide one-limb levels: the fixed-shape gate (the two-limb levels
-        // keep the fused gate, which extracts each coefficient's digits once
-        // for both limbs)
+    if (blocks >= kGateFxMin && gate_fx_fits(r, g, &direct)) {
+        // wide levels: the fixed-shape gate (two limbs: one workgroup per
+        // limb, each computing the digits from both limbs)
         launch_gate_fx(r, g, a, direct, st);
         return;
     }
