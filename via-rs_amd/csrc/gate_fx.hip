@@ -24,37 +24,16 @@ namespace {
 
 constexpr int kGxBuf = lds_words(kSlots);
 
-// The exact 128-bit gadget scale of a two-limb coefficient (gadget.rs:115-151)
-// for the rare values the fp64 estimate leaves undecided, as a signed exact
-// double (the scale is below B^L < 2^50). Out of line: it keeps its wide
-// registers out of the kernel's unrolled digit loop.
-__device__ __noinline__ double gx_scaled_exact2(double a0, double a1, const RingDev &r, const Gadget &g) {
-    const via_u128 x = rns_reconstruct(a0, a1, r.rns);
-    const via_u128 Q = r.rns.big_q;
-    const int neg = x > (Q >> 1);
-    const via_u128 mag = neg ? Q - x : x;
-    const double v = (double)(uint64_t)round_mul_div(mag, g.bl, r.qdiv);
+// The signed gadget scale round-half-away(c_centred * B^L / q) of a
+// one-limb coefficient as an exact double (gadget_scale's fp64 path,
+// g.f64 = 1, which the launcher requires). Two-limb coefficients take
+// gadget_scale inline: an out-of-line exact fallback inside the unrolled
+// digit loops measured slower than the whole inlined path.
+__device__ __forceinline__ double gx_scaled1(double a0, const RingDev &r, const Gadget &g) {
+    const uint64_t q = r.q_single, x = (uint64_t)a0;
+    const bool neg = x > (q >> 1);
+    const double v = (double)scale_div_f64(neg ? q - x : x, g.sf.bl, r.t[0].m.q, r.t[0].m.qinv, q >> 1);
     return neg ? -v : v;
-}
-
-// The signed gadget scale round-half-away(c_centred * B^L / Q) of one
-// coefficient as an exact double: the fp64 paths of gadget_scale (g.f64 = 1
-// with one limb, 2 with two, which the launcher requires).
-template <int NL>
-__device__ __forceinline__ double gx_scaled(double a0, double a1, const RingDev &r, const Gadget &g) {
-    if constexpr (NL == 2) {
-        SMag v;
-        if (gadget_scale2_f64(a0, a1, r.rns, g.sf, v)) {
-            const double mg = (double)(uint64_t)v.mag;
-            return v.neg ? -mg : mg;
-        }
-        return gx_scaled_exact2(a0, a1, r, g);
-    } else {
-        const uint64_t q = r.q_single, x = (uint64_t)a0;
-        const bool neg = x > (q >> 1);
-        const double v = (double)scale_div_f64(neg ? q - x : x, g.sf.bl, r.t[0].m.q, r.t[0].m.qinv, q >> 1);
-        return neg ? -v : v;
-    }
 }
 
 // Forward middle pass (L0, R) over the two digit polys in LDS (poly p at
@@ -143,7 +122,7 @@ __global__ __launch_bounds__(256, 3) void k_gate_fx(RingDev r, Gadget g, GateArg
 #pragma unroll
                     for (int k = 0; k < L; k++) y[k][t] = (double)next_digit(lo, hi, k, ga.dp);
                 } else {
-                    double v = gx_scaled<NL>(c0, c1, r, g);
+                    double v = gx_scaled1(c0, r, g);
 #pragma unroll
                     for (int k = 0; k < L; k++) y[k][t] = (double)digit_f64(v, ga.dp);
                 }
@@ -312,7 +291,7 @@ __global__ __launch_bounds__(1024) void k_gate_nx(RingDev r, Gadget g, GateArgs 
                 const int c = j + 256 * t;
                 double b0, c0;
                 gate_input<NL>(MODE, in, jg, half, 0, c, S, shift, m, b0, c0);
-                double v = gx_scaled<NL>(c0, 0.0, r, g);
+                double v = gx_scaled1(c0, r, g);
                 double d = (double)digit_f64(v, ga.dp);  // LSB first (gadget.rs:168-190)
                 if (k == 1) d = (double)digit_f64(v, ga.dp);
                 y[t] = d;
