@@ -238,18 +238,19 @@ void launch_db_update(uint8_t *dbl, const FdPlan &p, const RingDev &r2, const in
 template <int KC>
 __global__ __launch_bounds__(256) void k_sel_pack(const double *sel, uint8_t *sell, FdPlan p) {
     constexpr int KB = KC / 4;
-    const int QP8 = (p.QP + 7) / 8;
-    // 64 threads per (slot group of eight, chunk, lane group, column group)
-    const size_t total = (size_t)(p.n / 8) * p.KCN * 4 * QP8 * 64;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+    const unsigned QP8 = (p.QP + 7) / 8, KCN = p.KCN;
+    // 64 threads per (slot group of eight, chunk, lane group, column group);
+    // below 2^32 threads in all, so the index splits are 32-bit divisions
+    const unsigned total = (unsigned)(p.n / 8) * KCN * 4 * QP8 * 64;
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
         const int qpl = (int)(t & 7), kl = (int)((t >> 3) & 7);
-        size_t rest = t >> 6;
+        unsigned rest = t >> 6;
         const int qph = (int)(rest % QP8);
         rest /= QP8;
         const int g = (int)(rest & 3);
         rest >>= 2;
-        const int kc = (int)(rest % p.KCN);
-        const int k = (int)(rest / p.KCN) * 8 + kl;
+        const int kc = (int)(rest % KCN);
+        const int k = (int)(rest / KCN) * 8 + kl;
         const int qp = qph * 8 + qpl;
         if (qp >= p.QP) continue;
         const int q = qp >> 1, part = qp & 1;

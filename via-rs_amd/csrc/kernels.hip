@@ -908,7 +908,7 @@ __global__ void k_modswitch_q1_q2(RingDev r1, uint64_t q2, WideDiv qd, SwitchF s
     in += (size_t)blockIdx.y * in_gstride;
     out += (size_t)blockIdx.y * out_gstride;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t p = t / n, c = t - p * n;
+        const size_t p = t >> r1.logn, c = t & (size_t)(n - 1);  // n a power of two
         const double a0 = in[p * 2 * n + c], a1 = in[p * 2 * n + n + c];
         double yf;
         if (fast && modswitch_f64(a0, a1, r1.rns, sf, yf)) {
