@@ -173,10 +173,11 @@ __global__ void k_u64_to_f64_multi(U64Srcs srcs, double *dst, size_t words, uint
                                    int *bad) {
     const uint64_t *src = srcs.p[blockIdx.y];
     double *d = dst + (size_t)blockIdx.y * words;
-    const int lw = 2 * n1p;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x) {
+    const unsigned lw = 2 * n1p;
+    // one query's words (below 2^32): 32-bit index arithmetic
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < (unsigned)words; i += gridDim.x * blockDim.x) {
         const uint64_t x = src[i];
-        const uint64_t q = (int)(i % lw) < n1p ? q0 : q1;
+        const uint64_t q = i % lw < (unsigned)n1p ? q0 : q1;
         if (x >= q) {
             *bad = 1;
             d[i] = 0.0;
