@@ -141,6 +141,11 @@ struct via_ctx {
     size_t stage_words[7] = {};
     bool stage4_pairs = false;
     via::RingDev stage4_ring{};
+    // stage 3 (the q1 -> q2 switch) may be fused into the next transform:
+    // then its pointer is stage 2's array, switched again when asked for
+    bool stage3_switch = false;
+    via::RingDev stage3_ring{};
+    uint64_t stage3_q2 = 0;
 };
 
 struct via_db {

@@ -71,6 +71,18 @@ bool scale_w_make(const RingDev &r, const Gadget &g, const DigitPlan &dp, ScaleW
 bool exact_paths_only();
 void set_exact_paths_only(bool on);
 
+// The q1 -> q2 switch of two-limb coefficients as kernels take it: the fp64
+// estimate (fast, for q2 < q1 and both limbs within the fp64 path) with the
+// exact wide division as the fallback of the coefficients it leaves open.
+struct SwitchIn {
+    Rns2 rns;
+    WideDiv qd;
+    SwitchF sf;
+    uint64_t q2;
+    int fast;
+};
+SwitchIn switch_in(const RingDev &r1, uint64_t q2);
+
 // ---- launchers (ntt.hip / kernels.hip) ----------------------------------------
 
 // In-place batched NTT over `npolys` polys of degree r.n stored contiguously;
@@ -235,6 +247,12 @@ void launch_modswitch_q1_q2(const RingDev &r1, uint64_t q2, const double *in, do
 // writes out + g*out_gstride.
 void launch_modswitch_q1_q2_groups(const RingDev &r1, uint64_t q2, const double *in, size_t in_gstride, double *out,
                                    size_t out_gstride, size_t polys, int groups, hipStream_t s);
+// Mod switch q1 -> q2 and forward transform in one pass: the two-limb polys
+// [poly][limb][n] of src (at the degree kSlots) into the evaluation form mod q2
+// [poly][n] of dst; the array launch_modswitch_q1_q2 writes is not formed.
+bool ntt_switch_fits(const RingDev &r1, const RingDev &r2);
+void launch_ntt_switch(const RingDev &r1, const RingDev &r2, uint64_t q2, const double *src, double *dst,
+                       size_t npolys, hipStream_t s);
 
 // FirstDim on the int8 matrix cores (first_dim.hip). The prepared database
 // is held as 7-bit limb fragments (5 bytes per residue); a pass packs the

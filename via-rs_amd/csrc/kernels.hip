@@ -901,8 +901,8 @@ void launch_dmux_trivial(const RingDev &r, const double *rgsw_coeff, size_t rgsw
 
 // Group blockIdx.y reads in + y*in_gstride, writes out + y*out_gstride.
 // fast: modswitch_f64 first (the 128-bit path for the values it declines)
-__global__ void k_modswitch_q1_q2(RingDev r1, uint64_t q2, WideDiv qd, SwitchF sf, int fast, const double *in,
-                                  size_t in_gstride, double *out, size_t out_gstride, size_t polys) {
+__global__ void k_modswitch_q1_q2(RingDev r1, SwitchIn si, const double *in, size_t in_gstride, double *out,
+                                  size_t out_gstride, size_t polys) {
     const int n = r1.n;
     const size_t total = polys * n;
     in += (size_t)blockIdx.y * in_gstride;
@@ -911,15 +911,28 @@ __global__ void k_modswitch_q1_q2(RingDev r1, uint64_t q2, WideDiv qd, SwitchF s
         const size_t p = t >> r1.logn, c = t & (size_t)(n - 1);  // n a power of two
         const double a0 = in[p * 2 * n + c], a1 = in[p * 2 * n + n + c];
         double yf;
-        if (fast && modswitch_f64(a0, a1, r1.rns, sf, yf)) {
+        if (si.fast && modswitch_f64(a0, a1, r1.rns, si.sf, yf)) {
             out[t] = yf;
             continue;
         }
         const via_u128 x = rns_reconstruct(a0, a1, r1.rns);
-        uint64_t y = (uint64_t)round_mul_div(x, (via_u128)q2, qd);
-        if (y >= q2) y -= q2;
+        uint64_t y = (uint64_t)round_mul_div(x, (via_u128)si.q2, si.qd);
+        if (y >= si.q2) y -= si.q2;
         out[t] = (double)y;
     }
+}
+
+SwitchIn switch_in(const RingDev &r1, uint64_t q2) {
+    SwitchIn si;
+    si.rns = r1.rns;
+    si.qd = r1.qdiv;
+    si.q2 = q2;
+    si.sf.q2 = (double)q2;
+    si.sf.inv_q1 = 1.0 / (double)r1.rns.q1;
+    si.sf.q2_Q = (double)q2 / u128_to_f64(r1.rns.big_q);
+    si.fast = !exact_paths_only() && r1.nl == 2 && q2 < r1.rns.q1 && r1.rns.q1 < kMaxModulus &&
+              r1.rns.q0 < kMaxModulus;
+    return si;
 }
 
 void launch_modswitch_q1_q2_groups(const RingDev &r1, uint64_t q2, const double *in, size_t in_gstride, double *out,
@@ -929,14 +942,8 @@ void launch_modswitch_q1_q2_groups(const RingDev &r1, uint64_t q2, const double 
     size_t blocks = (total + 255) / 256;
     const size_t cap = std::max<size_t>(1, 16384 / (size_t)groups);
     if (blocks > cap) blocks = cap;
-    SwitchF sf;
-    sf.q2 = (double)q2;
-    sf.inv_q1 = 1.0 / (double)r1.rns.q1;
-    sf.q2_Q = (double)q2 / u128_to_f64(r1.rns.big_q);
-    const int fast = !exact_paths_only() && r1.nl == 2 && q2 < r1.rns.q1 && r1.rns.q1 < kMaxModulus &&
-                     r1.rns.q0 < kMaxModulus;
-    hipLaunchKernelGGL(k_modswitch_q1_q2, dim3((unsigned)blocks, groups), dim3(256), 0, st, r1, q2, r1.qdiv, sf, fast,
-                       in, in_gstride, out, out_gstride, polys);
+    hipLaunchKernelGGL(k_modswitch_q1_q2, dim3((unsigned)blocks, groups), dim3(256), 0, st, r1, switch_in(r1, q2), in,
+                       in_gstride, out, out_gstride, polys);
 }
 
 void launch_modswitch_q1_q2(const RingDev &r1, uint64_t q2, const double *in, double *out, size_t polys,

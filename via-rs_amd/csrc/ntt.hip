@@ -37,15 +37,29 @@ __global__ __launch_bounds__(kThreads) void k_ntt_batch(RingDev r, const double 
     }
 }
 
+// The exact q1 -> q2 switch of one coefficient: round(x q2 / q1) mod q2
+// for x the CRT value of its limbs (mod_switch.rs:57-86).
+__device__ double switch_exact(double a0, double a1, const SwitchIn &si) {
+    const via_u128 x = rns_reconstruct(a0, a1, si.rns);
+    uint64_t y = (uint64_t)round_mul_div(x, (via_u128)si.q2, si.qd);
+    if (y >= si.q2) y -= si.q2;
+    return (double)y;
+}
+
 // Forward transforms at n = 2048 with compile-time passes: two polys per
 // workgroup, 128 threads each (the halves may take different limbs). Thread
 // u reads elements u + 128 t of its poly straight into stages 0..3, runs
 // stages 4..7 over LDS, and holds slots 8 v .. 8 v + 7 for v = u, u + 128 in
 // the last pass, which it stores canonical. Polys come in groups as in
 // k_ntt_batch; src may equal dst (every poly is read before it is written).
+// SW: the polys are the two-limb q1 polys of src (poly p's limbs at
+// src + 2 n p and src + 2 n p + n, one group) switched to q2 as they are
+// loaded (mod_switch.rs:57-86), the switch of k_modswitch_q1_q2 without its
+// intermediate array.
+template <bool SW>
 __global__ __launch_bounds__(kThreads) void k_ntt_fx(RingDev r, const double *src, double *dst, size_t npolys,
                                                      size_t limb_group, size_t gpolys, size_t src_gstride,
-                                                     size_t dst_gstride) {
+                                                     size_t dst_gstride, SwitchIn si) {
     constexpr int BUF = lds_words(kSlots);
     __shared__ double s[2 * BUF];
     const int h = threadIdx.x >> 7, u = threadIdx.x & 127;
@@ -57,10 +71,32 @@ __global__ __launch_bounds__(kThreads) void k_ntt_fx(RingDev r, const double *sr
     const double *twf = r.t[l].fwd;
     double *sp = s + h * BUF;
     if (live) {
-        const double *g = src + gq * src_gstride + pr * kSlots + u;
         double x[16];
+        if constexpr (SW) {
+            const double *g = src + p * (2 * kSlots) + u;
+            unsigned bad = si.fast ? 0u : 0xffffu;
+            if (si.fast) {
 #pragma unroll
-        for (int t = 0; t < 16; t++) x[t] = g[128 * t];
+                for (int t = 0; t < 16; t++)
+                    if (!modswitch_f64(g[128 * t], g[kSlots + 128 * t], si.rns, si.sf, x[t])) bad |= 1u << t;
+            }
+            if (bad) {  // rare: the exact switch, one coefficient at a time, through this thread's LDS words
+                double *q = sp + lds_pad(u);
+#pragma unroll
+                for (int t = 0; t < 16; t++) q[unit_off<128>(t)] = x[t];
+                while (bad) {
+                    const int t = __builtin_ctz(bad);
+                    bad &= bad - 1;
+                    q[unit_off<128>(t)] = switch_exact(g[128 * t], g[kSlots + 128 * t], si);
+                }
+#pragma unroll
+                for (int t = 0; t < 16; t++) x[t] = q[unit_off<128>(t)];
+            }
+        } else {
+            const double *g = src + gq * src_gstride + pr * kSlots + u;
+#pragma unroll
+            for (int t = 0; t < 16; t++) x[t] = g[128 * t];
+        }
         double w[15];
         load_unit_tw<4, 0>(w, twf, 0);
         unit_stages<4>(x, w, m);
@@ -105,8 +141,8 @@ __global__ __launch_bounds__(kThreads) void k_ntt_fx(RingDev r, const double *sr
 static void launch_ntt_fwd(const RingDev &r, const double *src, double *dst, size_t npolys, size_t limb_group,
                            size_t gpolys, size_t src_gstride, size_t dst_gstride, hipStream_t st) {
     if (r.n == kSlots)
-        hipLaunchKernelGGL(k_ntt_fx, dim3((unsigned)((npolys + 1) / 2)), dim3(kThreads), 0, st, r, src, dst, npolys,
-                           limb_group, gpolys, src_gstride, dst_gstride);
+        hipLaunchKernelGGL(k_ntt_fx<false>, dim3((unsigned)((npolys + 1) / 2)), dim3(kThreads), 0, st, r, src, dst,
+                           npolys, limb_group, gpolys, src_gstride, dst_gstride, SwitchIn{});
     else
         hipLaunchKernelGGL(k_ntt_batch<false>, dim3((unsigned)((npolys * r.n + kSlots - 1) / kSlots)), dim3(kThreads), 0,
                            st, r, src, dst, npolys, limb_group, gpolys, src_gstride, dst_gstride);
@@ -127,6 +163,15 @@ void launch_ntt_to(const RingDev &r, const double *src, double *dst, size_t npol
                    hipStream_t st) {
     if (!npolys) return;
     launch_ntt_fwd(r, src, dst, npolys, limb_group, npolys, 0, 0, st);
+}
+
+bool ntt_switch_fits(const RingDev &r1, const RingDev &r2) { return r1.nl == 2 && r2.nl == 1 && r2.n == kSlots && r1.n == kSlots; }
+
+void launch_ntt_switch(const RingDev &r1, const RingDev &r2, uint64_t q2, const double *src, double *dst,
+                       size_t npolys, hipStream_t st) {
+    if (!npolys) return;
+    hipLaunchKernelGGL(k_ntt_fx<true>, dim3((unsigned)((npolys + 1) / 2)), dim3(kThreads), 0, st, r2, src, dst,
+                       npolys, (size_t)1, npolys, (size_t)0, (size_t)0, switch_in(r1, q2));
 }
 
 void launch_ntt_groups(const RingDev &r, const double *src, double *dst, size_t gpolys, size_t groups,

@@ -1447,14 +1447,27 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     c->stage_words[1] = I * ctw1;
     REC_EVENT(c->ev[2], st);
 
-    // ---- step 3: mod_switch_sym q1 -> q2 per row ----
-    launch_modswitch_q1_q2(r1n, pp->q2, w.dm[dc], w.sw, (size_t)Q * I * 2, st);
-    c->stage_ptr[2] = w.sw;
+    // ---- step 3: mod_switch_sym q1 -> q2 per row, fused with step 4's forward
+    // transform of the selection vector where the degree allows ----
+    const bool sw_fused = ntt_switch_fits(r1n, r2);
+    if (sw_fused) {
+        // the switched coefficient form is recomputed from stage 2 when asked for
+        c->stage_ptr[2] = w.dm[dc];
+        c->stage3_ring = r1n;
+        c->stage3_q2 = pp->q2;
+    } else {
+        launch_modswitch_q1_q2(r1n, pp->q2, w.dm[dc], w.sw, (size_t)Q * I * 2, st);
+        c->stage_ptr[2] = w.sw;
+    }
+    c->stage3_switch = sw_fused;
     c->stage_words[2] = I * 2 * n;
     REC_EVENT(c->ev[3], st);
 
     // ---- step 4: first_dim (first_dim.rs:94-184), one database pass for all queries ----
-    launch_ntt_to(r2, w.sw, w.sel, (size_t)Q * I * 2, 1, st);
+    if (sw_fused)
+        launch_ntt_switch(r1n, r2, pp->q2, w.dm[dc], w.sel, (size_t)Q * I * 2, st);
+    else
+        launch_ntt_to(r2, w.sw, w.sel, (size_t)Q * I * 2, 1, st);
     FdPlan fp = fd_plan((int)I, (int)J, (int)n, Q);
     fp.G = fd_groups;
     const FMod m2 = fmod_make(pp->q2);
@@ -2299,6 +2312,15 @@ via_status via_debug_stage(via_ctx *c, int stage, uint64_t *out, size_t words) {
     uint64_t *tmp = nullptr;
     HIP_TRY(hipMalloc(&tmp, sizeof(uint64_t) * total));
     double *coef = nullptr;
+    if (stage == 3 && c->stage3_switch) {
+        // the answer switched and transformed in one pass: switch stage 2 again
+        if (hipMalloc(&coef, sizeof(double) * total) != hipSuccess) {
+            (void)hipFree(tmp);
+            return VIA_E_NOMEM;
+        }
+        launch_modswitch_q1_q2(c->stage3_ring, c->stage3_q2, p, coef, total / kSlots, c->stream);
+        p = coef;
+    }
     if (stage == 4 && c->stage4_pairs) {
         // FirstDim outputs kept in pair form: back to coefficient form
         const int J = (int)(total / (2 * kSlots));
