@@ -290,17 +290,31 @@ int main() {
             const via_u128 q1 = cs.nl == 2 ? (via_u128)b.q1 : (via_u128)q2;
             const WideDiv wq = widediv_make(Qm);
             ScaleW sw;  // as scale_w_make builds it
-            sw.K = (double)(uint64_t)(bl / q1);
-            sw.r_q1 = (double)(uint64_t)(bl % q1) / (double)(uint64_t)q1;
-            sw.bl_Q = cs.nl == 2 ? u128_to_f64(bl) / u128_to_f64(Q1) : 0.0;
+            CHECK(scale_w_consts(bl, q1, Qm, cs.nl, sw), "wide split constants B=%llu L=%d\n",
+                  (unsigned long long)cs.base, cs.L);
             sw.bh = (double)dp.bh;
             sw.inv_bh = dp.inv_bh;
+            // (x * y) mod Qm for x, y < Qm < 2^78 by doubling
+            auto mulmod = [&](via_u128 x, via_u128 y) {
+                via_u128 acc = 0;
+                for (int i = 127; i >= 0; i--) {
+                    acc <<= 1;
+                    if (acc >= Qm) acc -= Qm;
+                    if ((y >> i) & 1) {
+                        acc += x;
+                        if (acc >= Qm) acc -= Qm;
+                    }
+                }
+                return acc;
+            };
+            const via_u128 blm = bl % Qm;
             long decided = 0, total = 0;
             auto check = [&](via_u128 c) {
                 c %= Qm;
                 const int neg = c > (Qm >> 1);
+                const via_u128 mg = neg ? Qm - c : c;
                 SMag v;
-                v.mag = round_mul_div(neg ? Qm - c : c, bl, wq);
+                v.mag = round_mul_div(mg, bl, wq);
                 v.neg = neg && v.mag != 0;
                 int64_t lo, hi;
                 digit_split(v, dp, lo, hi);
@@ -314,14 +328,21 @@ int main() {
                     decided++;
                     CHECK(flo == (double)lo && fhi == (double)hi, "wide split B=%llu L=%d nl=%d\n",
                           (unsigned long long)cs.base, cs.L, cs.nl);
+                } else {
+                    // declined only within 2^-38 of a rounding boundary: the
+                    // fraction of mag * B^L / Q + 1/2 is (mg * bl mod Q + Q/2) / Q
+                    via_u128 fr = mulmod(mg, blm) + (Qm >> 1);
+                    if (fr >= Qm) fr -= Qm;
+                    const via_u128 dist = fr < Qm - fr ? fr : Qm - fr;
+                    CHECK(dist <= (Qm >> 38), "wide split B=%llu L=%d nl=%d declined a decided value\n",
+                          (unsigned long long)cs.base, cs.L, cs.nl);
                 }
                 count++;
             };
             for (int t = 0; t < 300000; t++) check((((via_u128)rng() << 64) | rng()) % Qm);
-            CHECK(decided > total * 99 / 100, "wide split B=%llu decided only %ld of %ld\n",
-                  (unsigned long long)cs.base, decided, total);
+            CHECK(decided == total, "wide split B=%llu decided only %ld of %ld\n", (unsigned long long)cs.base,
+                  decided, total);
             const WideDiv w2bl = widediv_make(2 * bl);
-            const long near0 = decided;
             for (int t = 0; t < 100000; t++) {
                 // c near (k + 1/2) Q / bl: mag * bl / Q has fraction near 1/2
                 const via_u128 kk = (((via_u128)rng() << 64) | rng()) % (bl / 2);
@@ -333,10 +354,34 @@ int main() {
                     check(Qm - (c0 + (via_u128)(d + 3) - 3));
                 }
             }
-            // B^L/Q is not near an integer here, so integers c land anywhere in
-            // the rounding interval: the band is reached by some of them
-            CHECK(decided - near0 < 1400000, "wide split B=%llu: no near-boundary value declined\n",
-                  (unsigned long long)cs.base);
+            // magnitudes whose rounding argument lies delta / Q from a boundary:
+            // mag * B^L = delta - floor(Q/2) (mod Q), through the inverse of B^L
+            {
+                via_i128 t0 = 0, t1 = 1, r0 = (via_i128)Qm, r1 = (via_i128)blm;
+                while (r1 != 0) {
+                    const via_i128 qq = r0 / r1, t2 = t0 - qq * t1, r2 = r0 - qq * r1;
+                    t0 = t1;
+                    t1 = t2;
+                    r0 = r1;
+                    r1 = r2;
+                }
+                CHECK(r0 == 1, "B^L not invertible mod Q\n");
+                const via_u128 inv = (via_u128)(t0 < 0 ? t0 + (via_i128)Qm : t0);
+                const long before = decided, tot0 = total;
+                const via_u128 steps[] = {0, 1, 2, 3, 64, Qm >> 44, Qm >> 40, (Qm >> 40) * 3, Qm >> 36, Qm >> 30};
+                for (via_u128 st : steps)
+                    for (int sgn = 0; sgn < 2; sgn++) {
+                        const via_u128 target = (Qm - (Qm >> 1) + (sgn ? Qm - st % Qm : st % Qm)) % Qm;
+                        const via_u128 mg = mulmod(target, inv);
+                        if (mg > (Qm >> 1)) continue;  // not a centred magnitude
+                        check(mg);
+                        if (mg) check(Qm - mg);
+                    }
+                // two limbs: the smallest distances (below 2^-40) are declined
+                if (cs.nl == 2)
+                    CHECK(total - tot0 > decided - before, "wide split B=%llu: no boundary value declined\n",
+                          (unsigned long long)cs.base);
+            }
             for (int d = -50; d <= 50; d++) check((Qm >> 1) + (via_u128)(d + 50) - 50);
             for (int d = 0; d < 4; d++) check((via_u128)d);
             if (cs.nl == 2) {

@@ -253,6 +253,10 @@ void launch_modswitch_q1_q2_groups(const RingDev &r1, uint64_t q2, const double 
 bool ntt_switch_fits(const RingDev &r1, const RingDev &r2);
 void launch_ntt_switch(const RingDev &r1, const RingDev &r2, uint64_t q2, const double *src, double *dst,
                        size_t npolys, hipStream_t s);
+// The same for `groups` groups of gpolys polys: group g's two-limb polys at
+// src + g*src_gstride into dst + g*dst_gstride.
+void launch_ntt_switch_groups(const RingDev &r1, const RingDev &r2, uint64_t q2, const double *src, size_t src_gstride,
+                              double *dst, size_t dst_gstride, size_t gpolys, size_t groups, hipStream_t s);
 
 // FirstDim on the int8 matrix cores (first_dim.hip). The prepared database
 // is held as 7-bit limb fragments (5 bytes per residue); a pass packs the

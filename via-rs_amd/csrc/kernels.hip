@@ -35,19 +35,14 @@ void set_exact_paths_only(bool on) { g_exact_only.store(on ? 1 : 0, std::memory_
 
 bool scale_w_make(const RingDev &r, const Gadget &g, const DigitPlan &dp, ScaleW *sw) {
     if (exact_paths_only()) return false;
-    const via_u128 two39 = (via_u128)1 << 39;
     const via_u128 q1 = r.nl == 2 ? (via_u128)r.rns.q1 : (via_u128)r.q_single;
-    const via_u128 K = g.bl / q1, R = g.bl % q1;
-    if (q1 >= two39) return false;
-    if (r.nl == 2 ? K >= two39 : K >= ((via_u128)1 << 53)) return false;
+    const via_u128 Q = r.nl == 2 ? r.rns.big_q : q1;
     via_u128 bhi = 1;  // B^(L-h) bounds the high half
     for (int k = dp.h; k < g.L; k++) {
         bhi *= g.base;
         if (bhi >= ((via_u128)1 << 40)) return false;
     }
-    sw->K = (double)(uint64_t)K;
-    sw->r_q1 = (double)(uint64_t)R / (double)(uint64_t)q1;
-    sw->bl_Q = r.nl == 2 ? u128_to_f64(g.bl) / u128_to_f64(r.rns.big_q) : 0.0;
+    if (!scale_w_consts(g.bl, q1, Q, r.nl, *sw)) return false;
     sw->bh = (double)dp.bh;
     sw->inv_bh = dp.inv_bh;
     return true;

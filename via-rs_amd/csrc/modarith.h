@@ -317,26 +317,60 @@ VIA_HD bool gadget_scale2_f64(double a0, double a1, const Rns2 &b, const ScaleF6
 // mag = a0' + q0*t' (two limbs, a0' < q0; one limb: mag = t', a0' = 0):
 //   mag * B^L / Q = t'*K + F,  F = t' * (R/q1) + a0' * (B^L/Q) < 2^41,
 // so the scale floor((mag * B^L + floor(Q/2)) / Q) is t'*K + floor(F + 1/2 -
-// 1/(2Q)). With K, q1 < 2^39 (F < 2^40) F is computed in fp64 within 2^-11
-// and its rounding taken when it lies more than 2^-9 from a boundary (else
-// the exact path). t'*K < 2^91 is the exact pair ph + pl; the split
-// val = hi*B^h + lo takes one quotient estimate of hi (within one while
-// hi < 2^40) and the fma-exact remainder. The host enables it only within
-// these bounds (scale_w_make).
+// 1/(2Q)). With K, q1 < 2^39 (F < 2^40) F is computed as an unevaluated sum
+// s + low: R/q1 and B^L/Q are held as double-double constants, the products
+// with t' and a0' split exactly with fma and the sum of the leading parts
+// exactly (two-sum), so s + low is within 2^-52 of F and the rounding is
+// taken when F + 1/2 lies more than 2^-40 from an integer (else the exact
+// path, about one coefficient in 2^39). t'*K < 2^91 is the exact pair
+// ph + pl; the split val = hi*B^h + lo takes one quotient estimate of hi
+// (within one while hi < 2^40) and the fma-exact remainder. The host enables
+// it only within these bounds (scale_w_make, through scale_w_consts).
 struct ScaleW {
-    double K;       // floor(B^L / q1)
-    double r_q1;    // R / q1
-    double bl_Q;    // B^L / Q (two limbs)
-    double bh;      // B^h of the digit split
-    double inv_bh;  // 1 / B^h
+    double K;         // floor(B^L / q1)
+    double r_q1;      // R / q1, leading part
+    double r_q1_lo;   // its remainder: R/q1 - r_q1 to 2^-106
+    double bl_Q;      // B^L / Q (two limbs), leading part
+    double bl_Q_lo;   // its remainder to 2^-100
+    double bh;        // B^h of the digit split
+    double inv_bh;    // 1 / B^h
 };
+
+// The constants of ScaleW that depend on the gadget and the modulus (host):
+// q1 the second limb (two limbs, Q = q0 * q1) or the modulus (one limb,
+// Q = q1). False outside the bounds above (K, q1 < 2^39 for two limbs,
+// K < 2^53 for one, Q < 2^78, B^L / Q < 2^11).
+inline bool scale_w_consts(via_u128 bl, via_u128 q1, via_u128 Q, int nl, ScaleW &k) {
+    const via_u128 two39 = (via_u128)1 << 39;
+    const via_u128 K = bl / q1, R = bl % q1;
+    if (q1 >= two39 || Q >= ((via_u128)1 << 78)) return false;
+    if (nl == 2 ? K >= two39 : K >= ((via_u128)1 << 53)) return false;
+    k.K = (double)(uint64_t)K;
+    const double Rf = (double)(uint64_t)R, qf = (double)(uint64_t)q1;
+    k.r_q1 = Rf / qf;
+    k.r_q1_lo = fma(-k.r_q1, qf, Rf) / qf;  // the residual R - r_q1 q1 is exact
+    k.bl_Q = k.bl_Q_lo = 0.0;
+    if (nl == 2) {
+        // B^L / Q = ip + f1 2^-50 + f2 2^-100 + (below 2^-100), by long division
+        const via_u128 ip = bl / Q, rem = bl % Q;
+        if (ip >= ((via_u128)1 << 11)) return false;
+        const via_u128 f1 = (rem << 50) / Q, rem1 = (rem << 50) % Q;
+        const via_u128 f2 = (rem1 << 50) / Q;
+        const int64_t N = (int64_t)((ip << 50) | f1);  // below 2^61
+        const double Nf = (double)N;                   // N rounded to 53 bits
+        const int64_t diff = N - (int64_t)Nf;          // exact, |diff| <= 2^7
+        k.bl_Q = Nf * 0x1p-50;
+        k.bl_Q_lo = ((double)diff + (double)(uint64_t)f2 * 0x1p-50) * 0x1p-50;
+    }
+    return true;
+}
 
 // Two limbs (a0 mod q0, a1 mod q1, basis b), or one limb a0 mod q (NL = 1,
 // q1 = q). Returns the signed halves of the digit split of the scale.
 template <int NL>
 VIA_HD bool gadget_split_w(double a0, double a1, const Rns2 &b, double q_single, const ScaleW &k, double &lo,
                            double &hi) {
-    double tp, F;
+    double tp, F, low;
     int neg;
     if (NL == 2) {
         const double q0 = b.m0.q, q1 = b.m1.q;
@@ -348,17 +382,30 @@ VIA_HD bool gadget_split_w(double a0, double a1, const Rns2 &b, double q_single,
         // Q - c = (q0 - a0) + q0*(q1 - 1 - t), or q0*(q1 - t) when a0 = 0
         const double a0p = neg ? (a0 == 0.0 ? 0.0 : q0 - a0) : a0;
         tp = neg ? (a0 == 0.0 ? q1 - t : q1 - 1.0 - t) : t;
-        F = fma(tp, k.r_q1, a0p * k.bl_Q);
+        const double p1 = tp * k.r_q1, p2 = a0p * k.bl_Q;
+        const double e1 = fma(tp, k.r_q1, -p1), e2 = fma(a0p, k.bl_Q, -p2);  // exact
+        F = p1 + p2;
+        const double z = F - p1;
+        const double es = (p1 - (F - z)) + (p2 - z);  // F + es = p1 + p2 exactly
+        low = (es + e1) + (e2 + fma(tp, k.r_q1_lo, a0p * k.bl_Q_lo));
     } else {
         const double q = q_single;
         neg = a0 > 0.5 * (q - 1.0);
         tp = neg ? q - a0 : a0;
         F = tp * k.r_q1;
+        low = fma(tp, k.r_q1, -F) + tp * k.r_q1_lo;
     }
-    const double G = F + 0.5;
-    const double fl = floor(G);
-    const double fr = G - fl;
-    if (fr < 0.001953125 || fr > 1.0 - 0.001953125) return false;  // 2^-9
+    const double G = F + 0.5;  // exact: F < 2^41
+    double fl = floor(G);
+    double fr = (G - fl) + low;  // |low| < 2^-10
+    if (fr < 0.0) {
+        fl -= 1.0;
+        fr += 1.0;
+    } else if (fr >= 1.0) {
+        fl += 1.0;
+        fr -= 1.0;
+    }
+    if (fr < 0x1p-40 || fr > 1.0 - 0x1p-40) return false;
     const double ph = tp * k.K;
     const double pl = fma(tp, k.K, -ph);
     const double s = pl + fl;  // exact: |pl| < 2^38, fl < 2^41

@@ -52,10 +52,10 @@ __device__ double switch_exact(double a0, double a1, const SwitchIn &si) {
 // stages 4..7 over LDS, and holds slots 8 v .. 8 v + 7 for v = u, u + 128 in
 // the last pass, which it stores canonical. Polys come in groups as in
 // k_ntt_batch; src may equal dst (every poly is read before it is written).
-// SW: the polys are the two-limb q1 polys of src (poly p's limbs at
-// src + 2 n p and src + 2 n p + n, one group) switched to q2 as they are
-// loaded (mod_switch.rs:57-86), the switch of k_modswitch_q1_q2 without its
-// intermediate array.
+// SW: the polys are two-limb q1 polys (poly r of group g with its limbs at
+// src + g*src_gstride + 2 n r and n words further) switched to q2 as they
+// are loaded (mod_switch.rs:57-86), the switch of k_modswitch_q1_q2 without
+// its intermediate array.
 template <bool SW>
 __global__ __launch_bounds__(kThreads) void k_ntt_fx(RingDev r, const double *src, double *dst, size_t npolys,
                                                      size_t limb_group, size_t gpolys, size_t src_gstride,
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_fx(RingDev r, const double *sr
     if (live) {
         double x[16];
         if constexpr (SW) {
-            const double *g = src + p * (2 * kSlots) + u;
+            const double *g = src + gq * src_gstride + pr * (2 * kSlots) + u;
             unsigned bad = si.fast ? 0u : 0xffffu;
             if (si.fast) {
 #pragma unroll
@@ -167,11 +167,17 @@ void launch_ntt_to(const RingDev &r, const double *src, double *dst, size_t npol
 
 bool ntt_switch_fits(const RingDev &r1, const RingDev &r2) { return r1.nl == 2 && r2.nl == 1 && r2.n == kSlots && r1.n == kSlots; }
 
-void launch_ntt_switch(const RingDev &r1, const RingDev &r2, uint64_t q2, const double *src, double *dst,
-                       size_t npolys, hipStream_t st) {
+void launch_ntt_switch_groups(const RingDev &r1, const RingDev &r2, uint64_t q2, const double *src, size_t src_gstride,
+                              double *dst, size_t dst_gstride, size_t gpolys, size_t groups, hipStream_t st) {
+    const size_t npolys = gpolys * groups;
     if (!npolys) return;
     hipLaunchKernelGGL(k_ntt_fx<true>, dim3((unsigned)((npolys + 1) / 2)), dim3(kThreads), 0, st, r2, src, dst,
-                       npolys, (size_t)1, npolys, (size_t)0, (size_t)0, switch_in(r1, q2));
+                       npolys, (size_t)1, gpolys, src_gstride, dst_gstride, switch_in(r1, q2));
+}
+
+void launch_ntt_switch(const RingDev &r1, const RingDev &r2, uint64_t q2, const double *src, double *dst,
+                       size_t npolys, hipStream_t st) {
+    launch_ntt_switch_groups(r1, r2, q2, src, 0, dst, 0, npolys, 1, st);
 }
 
 void launch_ntt_groups(const RingDev &r, const double *src, double *dst, size_t gpolys, size_t groups,

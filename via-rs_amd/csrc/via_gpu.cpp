@@ -1498,9 +1498,14 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     // the CMux and CRot bits of query q, mod-switched: w.rgsw2 + q * rq2
     const size_t rq2 = (size_t)(nc + nr) * rgw2;
     if (nc + nr > 0) {
-        launch_modswitch_q1_q2_groups(r1n, pp->q2, w.rgsw + (size_t)nd * rgw, bits * rgw, w.rgsw2, rq2,
-                                      (size_t)(nc + nr) * 4 * lq, Q, st);
-        launch_ntt(r2, w.rgsw2, (size_t)Q * (nc + nr) * 4 * lq, 1, false, st);
+        if (sw_fused) {
+            launch_ntt_switch_groups(r1n, r2, pp->q2, w.rgsw + (size_t)nd * rgw, bits * rgw, w.rgsw2, rq2,
+                                     (size_t)(nc + nr) * 4 * lq, Q, st);
+        } else {
+            launch_modswitch_q1_q2_groups(r1n, pp->q2, w.rgsw + (size_t)nd * rgw, bits * rgw, w.rgsw2, rq2,
+                                          (size_t)(nc + nr) * 4 * lq, Q, st);
+            launch_ntt(r2, w.rgsw2, (size_t)Q * (nc + nr) * 4 * lq, 1, false, st);
+        }
     }
     const double *cm_in = w.fd;
     int cc = 0;
