@@ -477,6 +477,262 @@ __global__ __launch_bounds__(1024) void k_gate_nx(RingDev r, Gadget g, GateArgs 
     }
 }
 
+// Narrow levels across twice the compute units: each gate (each limb of it
+// with two limbs) is split into two launches, since one gate per compute
+// unit leaves the level's latency to that unit's VALU throughput. First
+// k_gate_nxa, one 512-thread workgroup per ciphertext half: the digits of
+// the half and their forward transforms as in k_gate_nx (threads 256 k + j
+// hold digit k), then thread u multiplies slots 4u .. 4u+3 of both digit
+// polys with the RGSW rows of both product parts and writes the two lazy
+// sums to the partial buffer P[gate][limb][half][part][slot]. Then
+// k_gate_nxb, one 512-thread workgroup per product part: the sum of the two
+// halves' partials is the gate's product sum (exact integers in fp64, so
+// the same value as k_gate_nx's sum over all four digit polys), then the
+// inverse transform and the combine as in k_gate_nx.
+template <int NL, bool DIRECT, int MODE>
+__global__ __launch_bounds__(512) void k_gate_nxa(RingDev r, Gadget g, GateArgs ga) {
+    constexpr int S = kSlots, L = 2;
+    __shared__ double s[2 * kGxBuf];  // the two digit polys of the half
+    const int tid = threadIdx.x;
+    const int blk = (int)blockIdx.x, half = blk & 1, gl = blk >> 1;
+    const int gate = NL == 2 ? gl >> 1 : gl;
+    const int l = NL == 2 ? (gl & 1) : 0;
+    const int q = gate / ga.items_per_query;
+    const int jg = gate - q * ga.items_per_query;
+    const int shift = ga.shift;
+    const double *rgsw = ga.rgsw + (size_t)q * ga.rgsw_qstride;
+    const double *in = ga.in + (size_t)q * ga.in_qstride;
+    const size_t pw = (size_t)NL * S;
+    const FMod m = r.t[l].m;
+    const double *twf = r.t[l].fwd;
+    const int k = tid >> 8, j = tid & 255;
+    {
+        double y[8];
+        if constexpr (NL == 1) {
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const int c = j + 256 * t;
+                double b0, c0;
+                gate_input<NL>(MODE, in, jg, half, 0, c, S, shift, m, b0, c0);
+                double v = gx_scaled1(c0, r, g);
+                double d = (double)digit_f64(v, ga.dp);  // LSB first (gadget.rs:168-190)
+                if (k == 1) d = (double)digit_f64(v, ga.dp);
+                y[t] = d;
+            }
+        } else {
+            // the threads of digit k scale t = 4k .. 4k+3, keep digit k and
+            // hand the other digit to the partner thread through its buffer
+            const FMod m0 = r.t[0].m, m1 = r.t[1].m;
+            double *mine = s + k * kGxBuf + lds_pad(j), *other = s + (k ^ 1) * kGxBuf + lds_pad(j);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int t = 4 * k + e, c = j + 256 * t;
+                double b0, c0, b1, c1;
+                gate_input<NL>(MODE, in, jg, half, 0, c, S, shift, m0, b0, c0);
+                gate_input<NL>(MODE, in, jg, half, 1, c, S, shift, m1, b1, c1);
+                double lo, hi;
+                digit_split_f64(gadget_scale<NL>(c0, c1, r, g), ga.dp, lo, hi);
+                const double d0 = (double)next_digit(lo, hi, 0, ga.dp);
+                const double d1 = (double)next_digit(lo, hi, 1, ga.dp);
+                y[t] = k ? d1 : d0;
+                other[unit_off<256>(t)] = k ? d0 : d1;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int t = 4 * (1 - k) + e;
+                y[t] = mine[unit_off<256>(t)];
+            }
+        }
+        if constexpr (DIRECT) {
+            direct_pass<3>(y, r.t[l].dfw);
+        } else {
+            double w0[7];
+            load_unit_tw<3, 0>(w0, twf, 0);
+            unit_stages<3>(y, w0, m);
+        }
+        double *pp = s + k * kGxBuf + lds_pad(j);
+#pragma unroll
+        for (int t = 0; t < 8; t++) pp[unit_off<256>(t)] = y[t];
+    }
+    // the RGSW row words of both parts, in flight during the forward passes
+    const int u = tid;
+    double kr[2][2][4];  // [digit k][part][slot]
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            const double2 *row = reinterpret_cast<const double2 *>(
+                rgsw + ((size_t)half * L + (L - 1 - kk)) * 2 * pw + (size_t)p * pw + (size_t)l * S + 4 * u);
+            const double2 a = row[0], b = row[1];
+            kr[kk][p][0] = a.x;
+            kr[kk][p][1] = a.y;
+            kr[kk][p][2] = b.x;
+            kr[kk][p][3] = b.y;
+        }
+    __syncthreads();
+    {
+        using G = UnitGeo<kSlotsLog, 3, 3>;
+        int base, SB;
+        G::at(j, base, SB);
+        double w[7];
+        load_unit_tw<3, 3>(w, twf, SB);
+        double *pp = s + k * kGxBuf + lds_pad(base);
+        double x[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) x[t] = pp[unit_off<(1 << G::JB)>(t)];
+        unit_stages<3>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 8; t++) pp[unit_off<(1 << G::JB)>(t)] = x[t];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+        using G = UnitGeo<kSlotsLog, 6, 2>;
+        int base, SB;
+        G::at(j + 256 * v, base, SB);
+        double w[3];
+        load_unit_tw<2, 6>(w, twf, SB);
+        double *pp = s + k * kGxBuf + lds_pad(base);
+        double x[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) x[t] = pp[unit_off<(1 << G::JB)>(t)];
+        unit_stages<2>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 4; t++) pp[unit_off<(1 << G::JB)>(t)] = x[t];
+    }
+    __syncthreads();
+    {
+        int base, SB;
+        UnitGeo<kSlotsLog, 8, 3>::at(j, base, SB);
+        double w[7];
+        load_unit_tw<3, 8>(w, twf, SB);
+        double *pp = s + k * kGxBuf + lds_pad(8 * j);
+        double x[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) x[t] = pp[t];
+        unit_stages<3>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 8; t++) pp[t] = x[t];
+    }
+    __syncthreads();
+    // the half's products at slots 4u .. 4u+3 (digit k with row L-1-k)
+    double *P = ga.partials + ((size_t)gl * 2 + half) * 2 * S;
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 2; kk++) {
+            const double *src = s + kk * kGxBuf + lds_pad(4 * u);
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[t] += mod_mul_lazy(src[t], kr[kk][p][t], m);
+        }
+        double2 *o = reinterpret_cast<double2 *>(P + (size_t)p * S + 4 * u);
+        o[0] = make_double2(acc[0], acc[1]);
+        o[1] = make_double2(acc[2], acc[3]);
+    }
+}
+
+template <int NL, int MODE>
+__global__ __launch_bounds__(512) void k_gate_nxb(RingDev r, GateArgs ga) {
+    constexpr int S = kSlots;
+    __shared__ double s[kGxBuf];  // the product part
+    const int tid = threadIdx.x;
+    const int blk = (int)blockIdx.x, p = blk & 1, gl = blk >> 1;
+    const int gate = NL == 2 ? gl >> 1 : gl;
+    const int l = NL == 2 ? (gl & 1) : 0;
+    const int q = gate / ga.items_per_query;
+    const int jg = gate - q * ga.items_per_query;
+    const int shift = ga.shift;
+    const double *in = ga.in + (size_t)q * ga.in_qstride;
+    double *out = ga.out + (size_t)q * ga.out_qstride;
+    const size_t pw = (size_t)NL * S;
+    const FMod m = r.t[l].m;
+    const double *twi = r.t[l].inv;
+    const int u = tid;
+    // the sum over both halves at slots 4u .. 4u+3, inverse stages 0..1
+    const double *P = ga.partials + (size_t)gl * 4 * S + (size_t)p * S + 4 * u;
+    double acc[4];
+    {
+        const double2 *h0 = reinterpret_cast<const double2 *>(P);
+        const double2 *h1 = reinterpret_cast<const double2 *>(P + 2 * S);
+        const double2 a0 = h0[0], a1 = h0[1], b0 = h1[0], b1 = h1[1];
+        acc[0] = a0.x + b0.x;
+        acc[1] = a0.y + b0.y;
+        acc[2] = a1.x + b1.x;
+        acc[3] = a1.y + b1.y;
+    }
+    // the combine's inputs, in flight during the inverse passes
+    double bvp[8], cvp[8];
+    if (u < 256) {
+#pragma unroll
+        for (int t = 0; t < 8; t++) gate_input<NL>(MODE, in, jg, p, l, u + 256 * t, S, shift, m, bvp[t], cvp[t]);
+    }
+    {
+        double w[3];
+        load_inv_unit_tw<kSlotsLog, 0, 2>(w, twi, u);
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc[t] = lazy_reduce(acc[t], m);
+        inv_unit_stages<2>(acc, w, m);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) s[lds_pad(4 * u) + t] = acc[t];
+    __syncthreads();
+    if (u < 256) {
+        using G = InvGeo<kSlotsLog, 2, 3>;
+        int base, SB;
+        G::at(u, base, SB);
+        double w[7];
+        load_inv_unit_tw<kSlotsLog, 2, 3>(w, twi, SB);
+        double *pp = s + lds_pad(base);
+        double x[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) x[t] = pp[unit_off<4>(t)];
+        inv_unit_stages<3>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 8; t++) pp[unit_off<4>(t)] = x[t];
+    }
+    __syncthreads();
+    if (u < 256) {
+        using G = InvGeo<kSlotsLog, 5, 3>;
+        int base, SB;
+        G::at(u, base, SB);
+        double w[7];
+        load_inv_unit_tw<kSlotsLog, 5, 3>(w, twi, SB);
+        double *pp = s + lds_pad(base);
+        double x[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) x[t] = pp[unit_off<32>(t)];
+        inv_unit_stages<3>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 8; t++) pp[unit_off<32>(t)] = x[t];
+    }
+    __syncthreads();
+    if (u < 256) {
+        // inverse stages 8..10 on coefficients u + 256 t, N^-1, the combine
+        double w[7];
+        load_inv_unit_tw<kSlotsLog, 8, 3>(w, twi, 0);
+        const double ni = r.t[l].n_inv;
+        const double *pp = s + lds_pad(u);
+        double x[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) x[t] = pp[unit_off<256>(t)];
+        inv_unit_stages<3>(x, w, m);
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const int c = u + 256 * t;
+            const double pv = mod_mul(x[t], ni, m);
+            const size_t o = (size_t)p * pw + (size_t)l * S + c;
+            if (MODE == GATE_DMUX) {
+                out[(size_t)(2 * jg) * 2 * pw + o] = mod_sub(cvp[t], pv, m);
+                out[(size_t)(2 * jg + 1) * 2 * pw + o] = pv;
+            } else {
+                out[(size_t)jg * 2 * pw + o] = mod_add(bvp[t], pv, m);
+            }
+        }
+    }
+}
+
 // Whether the fixed-shape gate takes a level: degree 2048, gadget depth 2
 // with B^2 below 2^50 (both digits from the low part), the fp64 scale (one
 // limb: exact, g.f64 = 1; two: decided or exact fallback, g.f64 = 2);
@@ -511,8 +767,36 @@ static void launch_nx(const dim3 &grid, const RingDev &r, const Gadget &g, const
     }
 }
 
+template <int NL, bool DIRECT>
+static void launch_nx_split(int blocks, const RingDev &r, const Gadget &g, const GateArgs &a, hipStream_t st) {
+    const dim3 grid(2 * NL * blocks);
+    switch (a.mode) {
+        case GATE_DMUX:
+            hipLaunchKernelGGL((k_gate_nxa<NL, DIRECT, GATE_DMUX>), grid, dim3(512), 0, st, r, g, a);
+            hipLaunchKernelGGL((k_gate_nxb<NL, GATE_DMUX>), grid, dim3(512), 0, st, r, a);
+            break;
+        case GATE_CMUX:
+            hipLaunchKernelGGL((k_gate_nxa<NL, DIRECT, GATE_CMUX>), grid, dim3(512), 0, st, r, g, a);
+            hipLaunchKernelGGL((k_gate_nxb<NL, GATE_CMUX>), grid, dim3(512), 0, st, r, a);
+            break;
+        default:
+            hipLaunchKernelGGL((k_gate_nxa<NL, DIRECT, GATE_CROT>), grid, dim3(512), 0, st, r, g, a);
+            hipLaunchKernelGGL((k_gate_nxb<NL, GATE_CROT>), grid, dim3(512), 0, st, r, a);
+            break;
+    }
+}
+
 void launch_gate_nx(const RingDev &r, const Gadget &g, const GateArgs &a, bool direct, hipStream_t st) {
     const int blocks = a.items_per_query * a.queries;
+    if (a.partials && blocks <= kGateNxSplitMax && a.partials_words >= (size_t)blocks * r.nl * 4 * kSlots) {
+        if (r.nl == 2)
+            launch_nx_split<2, false>(blocks, r, g, a, st);
+        else if (direct)
+            launch_nx_split<1, true>(blocks, r, g, a, st);
+        else
+            launch_nx_split<1, false>(blocks, r, g, a, st);
+        return;
+    }
     if (r.nl == 2)
         launch_nx<2, false>(dim3(2 * blocks), r, g, a, st);
     else if (direct)

@@ -1440,6 +1440,9 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         ga.items_per_query = 1 << i;
         ga.queries = Q;
         ga.shift = 0;
+        // the selection array is free until step 3: the split gates' partials
+        ga.partials = w.sw;
+        ga.partials_words = (size_t)Q * I * 2 * n;
         launch_gate(r1n, g1, ga, st);
         dc ^= 1;
     }
@@ -1547,6 +1550,10 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         ga.items_per_query = (int)(J >> (b + 1));
         ga.queries = Q;
         ga.shift = 0;
+        if (sw_fused) {  // the selection array held no stage for debugging
+            ga.partials = w.sw;
+            ga.partials_words = (size_t)Q * I * 2 * n;
+        }
         launch_gate(r2, g2, ga, st);
         cm_in = w.cm[cc];
         cc ^= 1;
@@ -1573,6 +1580,10 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         ga.items_per_query = 1;
         ga.queries = Q;
         ga.shift = (int)(n - (1u << b));
+        if (sw_fused) {
+            ga.partials = w.sw;
+            ga.partials_words = (size_t)Q * I * 2 * n;
+        }
         launch_gate(r2, g2, ga, st);
         res = w.res[rc];
         res_qstride = ctw2;
