@@ -103,6 +103,14 @@ via_status via_ctx_first_dim_time(via_ctx *ctx, double *seconds);
  * later identical call (up to eight captures per context). Step times are not
  * recorded by replays. Turning it off drops the captures. */
 via_status via_ctx_set_graphs(via_ctx *ctx, int on);
+
+/* Step timing (no counterpart in the reference): on (the default), every
+ * answer records the HIP events behind via_ctx_step_times; off, only the
+ * pair around the FirstDim pass (via_ctx_first_dim_time) is recorded, which
+ * saves the few microseconds of idle device time each event record costs
+ * between two kernels (about 40 us of a one-query answer). The autotuner
+ * records them while it measures either way. */
+via_status via_ctx_set_step_timing(via_ctx *ctx, int on);
 /* Stream timer: which = 0 records the start event, 1 the stop event, on the
  * context stream; via_ctx_marked_time waits for the stop event and returns
  * the seconds between them. No counterpart in the reference (its benches

@@ -107,6 +107,25 @@ def test_answer_paper_2x2_index15(ctx, case_2x2_15):
     check_stages(ctx, case_2x2_15, mask, body)
 
 
+def test_step_timing_off(ctx, case_2x2_15):
+    # answers that record only the FirstDim events: the same words and
+    # stages; the step split is refused until the events are on again
+    import via_gpu
+
+    ctx.set_step_timing(False)
+    try:
+        server, mask, body = run_case(ctx, case_2x2_15)
+        check_stages(ctx, case_2x2_15, mask, body)
+        with pytest.raises(via_gpu.ViaError):
+            ctx.step_times()
+        assert ctx.first_dim_time() > 0
+    finally:
+        ctx.set_step_timing(True)
+    server, mask, body = run_case(ctx, case_2x2_15)
+    assert np.array_equal(mask, case_2x2_15.ans_mask) and np.array_equal(body, case_2x2_15.ans_body)
+    assert len(ctx.step_times()) == 7
+
+
 def test_exact_paths_match_oracle(ctx, case_2x2_15):
     # the fp64 fast paths of the gadget scale (one limb, two limbs, the wide
     # cascade split) and of the q1 -> q2 mod switch decline values next to a

@@ -120,6 +120,9 @@ struct via_ctx {
     hipEvent_t ev_mark[2] = {};  // via_ctx_mark start / stop
     hipEvent_t ev_dec[kDecompSteps + 1] = {};  // around each decompression step (autotuner)
     bool time_decomp = false;
+    // the step events ev[] of every answer (via_ctx_set_step_timing); each
+    // record leaves the device idle for a few microseconds between kernels
+    bool step_events = true;
     std::vector<StepTune> tune_override;  // set while the autotuner measures
     // hipGraph mode (via_ctx_set_graphs): captured batched answers keyed by
     // every pointer and size they bake in; `capturing` suppresses the step

@@ -315,6 +315,7 @@ via_status via_server_create(int device, const via_params *pp, const via_db *db,
         // slots overlap on the device: a cooperative grid could not count on
         // having every CU, so the trees launch level by level
         sl.ctx->allow_persistent = false;
+        sl.ctx->step_events = false;  // nothing reads a slot's step times
         if (hipHostMalloc((void **)&sl.h_q, sizeof(uint64_t) * s->q_cap) != hipSuccess ||
             hipHostMalloc((void **)&sl.h_a, sizeof(uint64_t) * 2 * pp->n2 * max_batch) != hipSuccess ||
             hipMalloc((void **)&sl.d_q, sizeof(uint64_t) * s->q_cap) != hipSuccess ||

@@ -1194,6 +1194,11 @@ namespace {
     do {                                                  \
         if (!c->capturing) HIP_TRY(hipEventRecord(ev, st)); \
     } while (0)
+// the step events, when the context records them
+#define REC_STEP(ev, st)                                                    \
+    do {                                                                    \
+        if (!c->capturing && c->step_events) HIP_TRY(hipEventRecord(ev, st)); \
+    } while (0)
 
 // answer_through_crot (answer.rs:83-228) for Q queries against one database,
 // every step batched across the queries; query q uses the keys of its own
@@ -1239,7 +1244,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     uint64_t q2 = pp->q2;
     if ((s = make_ring(c, (uint32_t)n, &q2, 1, &r2))) return s;
 
-    REC_EVENT(c->ev[0], st);
+    REC_STEP(c->ev[0], st);
     // ---- step 1: query_decomp (query_decomp.rs:39-95) ----
     {
         U64Srcs srcs;
@@ -1351,7 +1356,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     const size_t rgw = 4 * lq * nl * n;  // words per RGSW at q1
     c->stage_ptr[0] = w.rgsw;
     c->stage_words[0] = bits * rgw;
-    REC_EVENT(c->ev[1], st);
+    REC_STEP(c->ev[1], st);
 
     // ---- step 2: dmux_tree from trivial(Delta) (answer.rs:181-191, mux.rs:222-251) ----
     const Gadget g1 = make_gadget(pp->b1, pp->l_query);
@@ -1448,7 +1453,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     }
     c->stage_ptr[1] = w.dm[dc];
     c->stage_words[1] = I * ctw1;
-    REC_EVENT(c->ev[2], st);
+    REC_STEP(c->ev[2], st);
 
     // ---- step 3: mod_switch_sym q1 -> q2 per row, fused with step 4's forward
     // transform of the selection vector where the degree allows ----
@@ -1464,7 +1469,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     }
     c->stage3_switch = sw_fused;
     c->stage_words[2] = I * 2 * n;
-    REC_EVENT(c->ev[3], st);
+    REC_STEP(c->ev[3], st);
 
     // ---- step 4: first_dim (first_dim.rs:94-184), one database pass for all queries ----
     if (sw_fused)
@@ -1493,7 +1498,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     c->stage_words[3] = J * 2 * n;
     c->stage4_pairs = fd_pairs;
     c->stage4_ring = r2;
-    REC_EVENT(c->ev[4], st);
+    REC_STEP(c->ev[4], st);
 
     // ---- step 5: cmux_tree over mod-switched RGSWs (answer.rs:194-203) ----
     const size_t rgw2 = 4 * lq * n;
@@ -1563,7 +1568,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     size_t res_qstride = J * ctw2;
     c->stage_ptr[4] = cm_in;
     c->stage_words[4] = ctw2;
-    REC_EVENT(c->ev[5], st);
+    REC_STEP(c->ev[5], st);
 
     // ---- step 6: crot SlotExtract (rotate.rs:128-149) ----
     int rc = 0;
@@ -1597,7 +1602,7 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
     }
     c->stage_ptr[5] = res;
     c->stage_words[5] = ctw2;
-    REC_EVENT(c->ev[6], st);
+    REC_STEP(c->ev[6], st);
 
     *res_out = res;
     return VIA_OK;
@@ -1647,9 +1652,9 @@ via_status via::answer_batch_impl(via_ctx *c, const via_db *db, int Q, const via
     via_status s = prefix_impl(c, db, Q, keys, d_lwes, n_lwe, err, &res, carved);
     if (s) return s;
     if ((s = resp_comp_impl(c, &keys[0]->pp, Q, keys, res, d_mask, d_body))) return s;
-    REC_EVENT(c->ev[7], c->stream);
+    REC_STEP(c->ev[7], c->stream);
     HIP_TRY(hipGetLastError());
-    c->have_times = true;
+    c->have_times = c->step_events;
     return VIA_OK;
 }
 
@@ -1799,7 +1804,7 @@ static via_status answer_graph_or_eager(via_ctx *c, const via_db *db, int Q, con
         c->graph_cache.erase(c->graph_cache.begin());
     }
     c->graph_cache.push_back(e);
-    c->have_times = true;
+    c->have_times = c->step_events;
     return VIA_OK;
 }
 
@@ -1809,6 +1814,15 @@ static void drop_graphs(via_ctx *c) {
         (void)hipGraphDestroy(e.graph);
     }
     c->graph_cache.clear();
+}
+
+via_status via_ctx_set_step_timing(via_ctx *c, int on) {
+    if (!c) return VIA_E_INVALID;
+    CallCtx cc;
+    hold_ctx(c, &cc);
+    c->step_events = on != 0;
+    c->have_times = false;
+    return VIA_OK;
 }
 
 via_status via_ctx_set_graphs(via_ctx *c, int on) {
@@ -2026,14 +2040,17 @@ via_status via_ctx_autotune(via_ctx *c, const via_db *db, uint32_t nq, const via
     HIP_TRY(hipSetDevice(c->device));
     struct Guard {
         via_ctx *c;
+        bool step_events;
         ~Guard() {
             c->time_decomp = false;
             c->tune_override.clear();
             c->tree_override = -1;
             c->fd_groups_override = 0;
             c->split_override = -2;
+            c->step_events = step_events;
         }
-    } guard{c};
+    } guard{c, c->step_events};
+    c->step_events = true;  // the candidates are timed by the step events
     const int Q = (int)nq;
     via_status s = answer_batch_impl(c, db, Q, keys, d_lwes, n_lwe, d_mask, d_body, nullptr);
     if (s) return s;

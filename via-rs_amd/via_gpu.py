@@ -122,6 +122,7 @@ def lib():
             "via_ctx_first_dim_time": (C.c_int, [vp, C.POINTER(C.c_double)]),
             "via_ctx_mark": (C.c_int, [vp, C.c_int]),
             "via_ctx_set_graphs": (C.c_int, [vp, C.c_int]),
+            "via_ctx_set_step_timing": (C.c_int, [vp, C.c_int]),
             "via_debug_set_exact_paths": (C.c_int, [C.c_int]),
             "via_ctx_set_tree_mode": (C.c_int, [vp, C.c_int]),
             "via_ctx_tree_tuning": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -441,6 +442,11 @@ class Context:
         g, sp = C.c_int(), C.c_int()
         _check(lib().via_ctx_pass_tuning(self.h, nq, C.byref(g), C.byref(sp)))
         return {"dmux": d.value, "cmux": c.value, "fd_groups": g.value, "gate_split": sp.value}
+
+    def set_step_timing(self, on):
+        """Record the step events behind step_times (the default) or only the
+        FirstDim pair: each event record idles the device for a few us."""
+        _check(lib().via_ctx_set_step_timing(self.h, int(bool(on))))
 
     def set_graphs(self, on):
         """hipGraph mode: repeated identical answers replay a captured graph."""
