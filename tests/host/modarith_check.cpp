@@ -95,17 +95,31 @@ int main() {
     }
     // int64 split digits against the u128 reference digits.
     {
-        const uint32_t bs[] = {18, 55879, 81, 8, 64, 3, 255, 32767};
-        const int Ls[] = {18, 2, 2, 8, 7, 40, 9, 5};
-        for (int bi = 0; bi < 8; bi++) {
+        // 2^100: the high half reaches 2^51, beyond the correction-free quotient
+        const uint32_t bs[] = {18, 55879, 81, 8, 64, 3, 255, 32767, 2, 7};
+        const int Ls[] = {18, 2, 2, 8, 7, 40, 9, 5, 100, 26};
+        for (int bi = 0; bi < 10; bi++) {
             uint32_t B = bs[bi];
             int L = Ls[bi];
             DigitPlan pl = digit_plan(B, L);
             via_u128 bl = 1;
             for (int k = 0; k < L; k++) bl *= B;
+            CHECK(pl.exact_q == (B != 2), "exact quotient flag B=%u L=%d\n", B, L);
             for (int t = 0; t < 200000; t++) {
                 via_u128 mag = ((((via_u128)rng() << 64) | rng()) % (bl / 2 + 2));
                 if (t < 4) mag = (t < 2) ? (via_u128)t : bl / 2 + (t - 2);
+                // every digit on the rounding edge: mag = sum of (B/2) B^k or
+                // of -(B-1-B/2) B^k (balanced extremes), and B^k - 1
+                if (t >= 4 && t < 4 + 3 * L) {
+                    const int kk = (t - 4) % L, kind = (t - 4) / L;
+                    via_u128 pw = 1, acc = 0;
+                    for (int k = 0; k <= kk; k++) {
+                        if (kind == 0) acc += (via_u128)(B / 2) * pw;
+                        pw *= B;
+                    }
+                    mag = kind == 0 ? acc : kind == 1 ? pw - 1 : pw;
+                    if (mag > bl / 2) mag = bl / 2;
+                }
                 SMag v;
                 v.mag = mag;
                 v.neg = (rng() & 1) && mag != 0;

@@ -482,6 +482,8 @@ struct DigitPlan {
     double base_f;    // B
     double half_f;    // floor(B/2): the largest digit
     double neg_f;     // B - 1 - floor(B/2): minus the smallest digit
+    double off_f;     // neg_f + 1/2
+    int exact_q;      // both halves below 2^49 (B^(L-h) < 2^49): digit_f64's quotient is exact
 };
 
 VIA_HD DigitPlan digit_plan(uint32_t base, int L) {
@@ -500,6 +502,10 @@ VIA_HD DigitPlan digit_plan(uint32_t base, int L) {
     p.base_f = (double)base;
     p.half_f = (double)(base >> 1);
     p.neg_f = (double)(base - 1 - (base >> 1));
+    p.off_f = p.neg_f + 0.5;
+    double bhi = 1.0;  // B^(L-h), exact while below 2^53
+    for (int k = p.h; k < L && bhi < 562949953421312.0; k++) bhi *= base;
+    p.exact_q = bhi < 562949953421312.0;  // 2^49
     return p;
 }
 
@@ -528,8 +534,18 @@ VIA_HD int32_t digit_i64(int64_t &x, uint32_t base, double inv_base) {
 // digit r in [-(B-1-floor(B/2)), floor(B/2)] with x = B*q + r is
 // x - B*floor((x + B-1-floor(B/2)) / B); the quotient estimate is within one,
 // r = x - B*q is exact (fma), and one correction each way fixes it. Equal to
-// rem_euclid then the rebalance above B/2 (gadget.rs:168-190).
+// rem_euclid then the rebalance above B/2 (gadget.rs:168-190). With both
+// halves below 2^49 (p.exact_q) no correction is needed: for the integer
+// N = x + neg_f, floor(N/B) = floor((N + 1/2)/B), whose argument lies at
+// least 1/(2B) from an integer, and (N + 1/2) * fl(1/B), N + 1/2 exact below
+// 2^51, is within |N + 1/2| 2^-52 / B < 1/(2B) of it.
 VIA_HD int32_t digit_f64(double &x, const DigitPlan &p) {
+    if (p.exact_q) {
+        const double q = floor((x + p.off_f) * p.inv_base);
+        const double r = fma(-q, p.base_f, x);
+        x = q;
+        return (int32_t)r;
+    }
     const double q = floor((x + p.neg_f) * p.inv_base);
     const double r = fma(-q, p.base_f, x);
     const double corr = r > p.half_f ? 1.0 : (r < -p.neg_f ? -1.0 : 0.0);
