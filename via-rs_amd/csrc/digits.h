@@ -66,7 +66,13 @@ __device__ __forceinline__ void write_digits(double a0, double a1, const RingDev
     double lo, hi;
     if (!(g.f64 == 3 && gadget_split_w<NL>(a0, a1, r.rns, (double)r.q_single, g.sw, lo, hi)))
         digit_split_f64(gadget_scale<NL>(a0, a1, r, g), dp, lo, hi);
-    for (int k = 0; k < g.L; k++) dig[(size_t)k * stride] = (int16_t)next_digit(lo, hi, k, dp);
+    // next_digit's order with its branches hoisted: the low part's digits,
+    // its carry into the high part, the high part's digits
+    const int hl = dp.h < g.L ? dp.h : g.L;
+    int k = 0;
+    for (; k < hl; k++, dig += stride) *dig = (int16_t)digit_f64(lo, dp);
+    hi += lo;
+    for (; k < g.L; k++, dig += stride) *dig = (int16_t)digit_f64(hi, dp);
 }
 
 // ---- gate inputs -----------------------------------------------------------------
