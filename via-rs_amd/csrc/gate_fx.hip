@@ -788,7 +788,7 @@ static void launch_nx_split(int blocks, const RingDev &r, const Gadget &g, const
 
 void launch_gate_nx(const RingDev &r, const Gadget &g, const GateArgs &a, bool direct, hipStream_t st) {
     const int blocks = a.items_per_query * a.queries;
-    if (a.partials && blocks <= kGateNxSplitMax && a.partials_words >= (size_t)blocks * r.nl * 4 * kSlots) {
+    if (a.partials && blocks * r.nl <= kGateNxSplitMax && a.partials_words >= (size_t)blocks * r.nl * 4 * kSlots) {
         if (r.nl == 2)
             launch_nx_split<2, false>(blocks, r, g, a, st);
         else if (direct)

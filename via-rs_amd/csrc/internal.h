@@ -190,8 +190,8 @@ struct GateArgs {
     int shift;           // CRot rotation N - 2^i
     DigitPlan dp;        // digit plan of the gadget (set by launch_gate)
     int split = -1;      // two limbs, one workgroup per limb: -1 below 256 gates, 0 never, 1 always
-    // narrow levels of at most kGateNxSplitMax gates: device words for the
-    // split gate's partial products (gates * limbs * 4 * n), or null
+    // narrow levels of at most kGateNxSplitMax gate limbs: device words for
+    // the split gate's partial products (gates * limbs * 4 * n), or null
     double *partials = nullptr;
     size_t partials_words = 0;
 };
@@ -207,9 +207,10 @@ bool gate_fx_fits(const RingDev &r, const Gadget &g, bool *direct);
 void launch_gate_fx(const RingDev &r, const Gadget &g, const GateArgs &a, bool direct, hipStream_t s);
 // The same shape for narrow levels (fewer than kGateFxMin gates): one gate,
 // or one limb of a two-limb gate, per 1024-thread workgroup (not for
-// GATE_CMUX_EV); with a.partials, levels of at most kGateNxSplitMax gates
-// take two launches of twice the workgroups (halves, then product parts).
-constexpr int kGateNxSplitMax = 128;
+// GATE_CMUX_EV); with a.partials, levels of at most kGateNxSplitMax gate
+// limbs (gates times limbs) take two launches of twice the workgroups
+// (halves, then product parts); above that the one launch is faster.
+constexpr int kGateNxSplitMax = 64;
 void launch_gate_nx(const RingDev &r, const Gadget &g, const GateArgs &a, bool direct, hipStream_t s);
 
 // Every level of one DMux or CMux tree in one cooperative launch
