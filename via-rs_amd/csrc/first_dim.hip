@@ -238,21 +238,23 @@ void launch_db_update(uint8_t *dbl, const FdPlan &p, const RingDev &r2, const in
 template <int KC>
 __global__ __launch_bounds__(256) void k_sel_pack(const double *sel, uint8_t *sell, FdPlan p) {
     constexpr int KB = KC / 4;
-    const unsigned QP8 = (p.QP + 7) / 8, KCN = p.KCN;
-    // 64 threads per (slot group of eight, chunk, lane group, column group);
-    // below 2^32 threads in all, so the index splits are 32-bit divisions
-    const unsigned total = (unsigned)(p.n / 8) * KCN * 4 * QP8 * 64;
+    const unsigned QP4 = (p.QP + 3) / 4, KCN = p.KCN;
+    // 64 threads per (slot group of sixteen, chunk, lane group, column group
+    // of four): each load of a wavefront reads 16 consecutive slots (a
+    // 128-byte line) of four columns; below 2^32 threads in all, so the
+    // index splits are 32-bit divisions
+    const unsigned total = (unsigned)((p.n + 15) / 16) * KCN * 4 * QP4 * 64;
     for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-        const int qpl = (int)(t & 7), kl = (int)((t >> 3) & 7);
+        const int kl = (int)(t & 15), qpl = (int)((t >> 4) & 3);
         unsigned rest = t >> 6;
-        const int qph = (int)(rest % QP8);
-        rest /= QP8;
+        const int qph = (int)(rest % QP4);
+        rest /= QP4;
         const int g = (int)(rest & 3);
         rest >>= 2;
         const int kc = (int)(rest % KCN);
-        const int k = (int)(rest / KCN) * 8 + kl;
-        const int qp = qph * 8 + qpl;
-        if (qp >= p.QP) continue;
+        const int k = (int)(rest / KCN) * 16 + kl;
+        const int qp = qph * 4 + qpl;
+        if (qp >= p.QP || k >= p.n) continue;
         const int q = qp >> 1, part = qp & 1;
         double v[KB];
 #pragma unroll
@@ -270,7 +272,7 @@ __global__ __launch_bounds__(256) void k_sel_pack(const double *sel, uint8_t *se
 }
 
 void launch_sel_pack(const double *sel, uint8_t *sell, const FdPlan &p, hipStream_t st) {
-    const size_t total = (size_t)(p.n / 8) * p.KCN * 4 * ((p.QP + 7) / 8) * 64;
+    const size_t total = (size_t)((p.n + 15) / 16) * p.KCN * 4 * ((p.QP + 3) / 4) * 64;
     const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 65536);
     if (p.KC == 64) hipLaunchKernelGGL(k_sel_pack<64>, dim3(blocks), dim3(256), 0, st, sel, sell, p);
     else hipLaunchKernelGGL(k_sel_pack<32>, dim3(blocks), dim3(256), 0, st, sel, sell, p);
