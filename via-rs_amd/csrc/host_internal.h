@@ -147,6 +147,10 @@ struct via_ctx {
     // stage 3 (the q1 -> q2 switch) may be fused into the next transform:
     // then its pointer is stage 2's array, switched again when asked for
     bool stage3_switch = false;
+    // the selection-vector array of the last prefix when nothing else holds
+    // it after step 4 (switch fused into the transform): RespComp's partials
+    double *resp_partials = nullptr;
+    size_t resp_partials_words = 0;
     via::RingDev stage3_ring{};
     uint64_t stage3_q2 = 0;
 };

@@ -317,6 +317,10 @@ struct RespArgs {
     NttTab t3;        // q3 at n2
     int queries;
     DigitPlan dp;     // digit plan of (b_rsk, l_rsk) (set by launch_resp_comp)
+    // device words for the ring switch's per-j partial sums
+    // (queries * d * 2 * n2), or null: one workgroup per query then
+    double *partials = nullptr;
+    size_t partials_words = 0;
 };
 void launch_resp_comp(const RespArgs &a, hipStream_t s);
 

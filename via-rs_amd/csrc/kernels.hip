@@ -1062,11 +1062,99 @@ __global__ __launch_bounds__(1024) void k_resp_comp(RespArgs a) {
     }
 }
 
+// RespComp over d + 1 workgroups per query where a partial buffer is given:
+// k_resp_part, one 1024-thread workgroup per (query, j), switches the n2
+// coefficients of pi_0(mask * X^j) it needs straight from the q2 mask, forms
+// their l_rsk digit polynomials, transforms them and writes the reduced sum
+// of their products with rsk_j (both parts) to the partial buffer; then
+// k_resp_fin, one per query, adds the d partials (the canonical residue of
+// the same sum k_resp_comp forms), runs the two inverse transforms and the
+// final switches as k_resp_comp does.
+constexpr int kRespPartWords = 8192;  // digit polynomials of one j
+
+__global__ __launch_bounds__(1024) void k_resp_part(RespArgs a) {
+    __shared__ double buf[lds_words(kRespPartWords)];
+    const int d = a.d, L = a.l_rsk, n1 = a.n1, n2 = a.n2;
+    const int qy = blockIdx.x / d, j = blockIdx.x - qy * d;
+    const int logn2 = 31 - __clz(n2);
+    const uint64_t q2 = a.q2, q3 = a.q3;
+    const FMod m3 = a.t3.m;
+    const double *ct = a.ct + (size_t)qy * 2 * n1;
+    uint64_t bl = 1;
+    for (int k = 0; k < L; k++) bl *= a.b_rsk;
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+        const int idx = d * i - j;
+        const double sv = (double)switch_round((uint64_t)ct[idx >= 0 ? idx : idx + n1], q3, q2, a.minv2);
+        const double pv = idx >= 0 ? sv : mod_neg(sv, m3);
+        const uint64_t x = (uint64_t)pv;
+        const int neg = x > (q3 >> 1);
+        const uint64_t mag = neg ? q3 - x : x;
+        SMag v;
+        v.mag = (via_u128)udiv_pre(mag * bl + (q3 >> 1), q3, a.minv3);
+        v.neg = neg && v.mag != 0;
+        double lo, hi;
+        digit_split_f64(v, a.dp, lo, hi);
+        for (int k = 0; k < L; k++) buf[lds_pad(k * n2 + i)] = (double)next_digit(lo, hi, k, a.dp);
+    }
+    __syncthreads();
+    NttLimbs Tf;
+    Tf.tw[0] = Tf.tw[1] = a.t3.fwd;
+    Tf.m[0] = Tf.m[1] = m3;
+    ntt_fwd<1>(buf, L * n2, n2, logn2, Tf);
+    const size_t kw = (size_t)L * 2 * n2;  // key words per j
+    const double *rsk = a.rsk_eval[qy] + (size_t)j * kw;
+    double *out = a.partials + ((size_t)qy * d + j) * 2 * n2;
+    for (int t = threadIdx.x; t < 2 * n2; t += blockDim.x) {
+        const int part = t >= n2 ? 1 : 0, i = t - part * n2;
+        double sacc = 0.0;
+        for (int k = 0; k < L; k++)
+            sacc += mod_mul_lazy(buf[lds_pad(k * n2 + i)], rsk[(size_t)(L - 1 - k) * 2 * n2 + part * n2 + i], m3);
+        out[t] = mod_reduce_small(sacc, m3);
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_resp_fin(RespArgs a) {
+    __shared__ double buf[lds_words(2 * 2048)];
+    const int qy = blockIdx.x;
+    const int d = a.d, n1 = a.n1, n2 = a.n2;
+    const int logn2 = 31 - __clz(n2);
+    const uint64_t q2 = a.q2, q3 = a.q3, q4 = a.q4;
+    const FMod m3 = a.t3.m;
+    const double *ct = a.ct + (size_t)qy * 2 * n1;
+    const double *pj = a.partials + (size_t)qy * d * 2 * n2;
+    for (int t = threadIdx.x; t < 2 * n2; t += blockDim.x) {
+        double acc = pj[t];
+        for (int j = 1; j < d; j++) acc = mod_add(acc, pj[(size_t)j * 2 * n2 + t], m3);
+        buf[lds_pad(t)] = acc;
+    }
+    __syncthreads();
+    NttLimbs Ti;
+    Ti.tw[0] = Ti.tw[1] = a.t3.inv;
+    Ti.m[0] = Ti.m[1] = m3;
+    ntt_inv_noscale<1>(buf, 2 * n2, n2, logn2, Ti);
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+        const double rm = mod_mul(buf[lds_pad(i)], a.t3.n_inv, m3);
+        const double rb = mod_mul(buf[lds_pad(n2 + i)], a.t3.n_inv, m3);
+        const double body_proj = (double)switch_round((uint64_t)ct[n1 + d * i], q3, q2, a.minv2);
+        const uint64_t mask = (uint64_t)mod_neg(rm, m3);
+        const uint64_t body = (uint64_t)mod_sub(body_proj, rb, m3);
+        a.out_mask[qy][i] = switch_round(mask, q3, q3, a.minv3);
+        a.out_body[qy][i] = switch_round(body, q4, q3, a.minv3);
+    }
+}
+
 void launch_resp_comp(const RespArgs &a_in, hipStream_t st) {
     RespArgs a = a_in;
     a.dp = digit_plan(a.b_rsk, a.l_rsk);
     a.minv2 = ~0ULL / a.q2;
     a.minv3 = ~0ULL / a.q3;
+    const bool split = a.partials && a.d >= 2 && a.n2 <= 2048 && (size_t)a.l_rsk * a.n2 <= (size_t)kRespPartWords &&
+                       a.partials_words >= (size_t)a.queries * a.d * 2 * a.n2;
+    if (split) {
+        hipLaunchKernelGGL(k_resp_part, dim3(a.queries * a.d), dim3(1024), 0, st, a);
+        hipLaunchKernelGGL(k_resp_fin, dim3(a.queries), dim3(1024), 0, st, a);
+        return;
+    }
     hipLaunchKernelGGL(k_resp_comp, dim3(a.queries), dim3(1024), 0, st, a);
 }
 

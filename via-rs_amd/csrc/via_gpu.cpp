@@ -1468,6 +1468,8 @@ via_status prefix_impl(via_ctx *c, const via_db *db, int Q, const via_keys *cons
         c->stage_ptr[2] = w.sw;
     }
     c->stage3_switch = sw_fused;
+    c->resp_partials = sw_fused ? w.sw : nullptr;
+    c->resp_partials_words = sw_fused ? (size_t)Q * I * 2 * n : 0;
     c->stage_words[2] = I * 2 * n;
     REC_STEP(c->ev[3], st);
 
@@ -1636,6 +1638,8 @@ via_status resp_comp_impl(via_ctx *c, const via_params *pp, int Q, const via_key
     ra.t3.n_inv = t3.n_inv;
     ra.t3.m = fmod_make(pp->q3);
     ra.queries = Q;
+    ra.partials = c->resp_partials;
+    ra.partials_words = c->resp_partials_words;
     launch_resp_comp(ra, c->stream);
     return VIA_OK;
 }
