@@ -404,29 +404,34 @@ template <int NL>
 __global__ __launch_bounds__(1024) void k_r2g_combine(RingDev r, const double *pm, const double *pb, int groups,
                                                           int items, const double *rlwe_masks,
                                                           const double *rlwe_bodies, double *rgsw, int lq) {
-    __shared__ double s[lds_words(NL * kSlots)];
+    __shared__ double s[lds_words(kSlots)];
     const int item = blockIdx.x;  // bit * lq + i
     const int part = blockIdx.y;  // 0: mask of neg_s_m[i] (and the copy of m[i]), 1: its body
+    const int l = blockIdx.z;     // the limb: one workgroup per limb
     const int bit = item / lq, i = item - bit * lq;
     const int n = r.n;  // == kSlots
-    const FMod m0 = r.t[0].m, m1 = r.t[1].m;
+    const FMod md = r.t[l].m;
     const size_t pw = (size_t)NL * n;
     double *gp = rgsw + (size_t)bit * 4 * lq * pw;
     double *nsm = gp + (size_t)i * 2 * pw;        // neg_s_m[i]
     double *mm = gp + (size_t)(lq + i) * 2 * pw;  // m[i]
     const double *rm = rlwe_masks + (size_t)item * pw, *rb = rlwe_bodies + (size_t)item * pw;
     const double *src = (part ? pb : pm) + (size_t)item * pw;
-    for (int t = threadIdx.x; t < (int)pw; t += blockDim.x) {
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        const int t = l * n + c;
         double acc = src[t];  // canonical partials, at most 18: a lazy sum below 2^43
         for (int gi = 1; gi < groups; gi++) acc += src[(size_t)gi * items * pw + t];
-        s[lds_pad(t)] = lazy_reduce(acc, t >= n ? m1 : m0);
+        s[lds_pad(c)] = lazy_reduce(acc, md);
     }
     __syncthreads();
-    lds_ntt_inv_noscale<NL>(s, n, r);
-    for (int t = threadIdx.x; t < (int)pw; t += blockDim.x) {
-        const int l = t >= n ? 1 : 0;
-        const FMod md = l ? m1 : m0;
-        const double x = mod_mul(s[lds_pad(t)], l ? r.t[1].n_inv : r.t[0].n_inv, md);
+    NttLimbs T;
+    T.tw[0] = T.tw[1] = r.t[l].inv;
+    T.m[0] = T.m[1] = md;
+    ntt_inv_noscale<1>(s, n, n, r.logn, T);
+    const double ni = r.t[l].n_inv;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        const int t = l * n + c;
+        const double x = mod_mul(s[lds_pad(c)], ni, md);
         if (part == 0) {
             nsm[t] = mod_add(rb[t], x, md);
             mm[t] = rm[t];
@@ -441,10 +446,10 @@ void launch_r2g_combine(const RingDev &r, const double *pm, const double *pb, in
                         const double *rlwe_bodies, double *rgsw, int bits, int lq, hipStream_t st) {
     const int items = bits * lq;
     if (r.nl == 2)
-        hipLaunchKernelGGL(k_r2g_combine<2>, dim3(items, 2), dim3(1024), 0, st, r, pm, pb, groups, items,
+        hipLaunchKernelGGL(k_r2g_combine<2>, dim3(items, 2, 2), dim3(1024), 0, st, r, pm, pb, groups, items,
                            rlwe_masks, rlwe_bodies, rgsw, lq);
     else
-        hipLaunchKernelGGL(k_r2g_combine<1>, dim3(items, 2), dim3(1024), 0, st, r, pm, pb, groups, items,
+        hipLaunchKernelGGL(k_r2g_combine<1>, dim3(items, 2, 1), dim3(1024), 0, st, r, pm, pb, groups, items,
                            rlwe_masks, rlwe_bodies, rgsw, lq);
 }
 
